@@ -1,0 +1,187 @@
+/*
+ * cds.h — C ABI of the MI355X k-clique densest-subgraph (DSD) engine.
+ *
+ * The reference (wajidmanzoor/CDS) has no library boundary: its GPU path is a
+ * process, `main <graph> <k> <pSize> <cpSize> <glBufferSize> <partitionSize>
+ * <earlyStop>` (cudaCode/main.cu:865-887), reading the adjacency-list format
+ * of cudaCode/src/graph.cpp:7-40 and printing the report of
+ * cudaCode/main.cu:1069-1104.  Its correctness oracle is the serial CPU path
+ * `CDS::DSD` (cpuCode/c++/src/helpers.cpp:1422-1589).  The entry points below
+ * are what a foreign-function binding of that path needs: load a graph, solve
+ * it, read back the oracle's result block.  The repo's own `cds_main` binary
+ * is the drop-in for the reference's CLI and is built on these calls.
+ *
+ * Conventions
+ *   - Every call returns a cds_status; nothing calls exit().  The message of
+ *     the last failure on the calling thread is cds_last_error().
+ *   - Host arrays passed in stay owned by the caller; arrays returned inside
+ *     cds_graph / cds_result are owned by the library and released by
+ *     cds_graph_free / cds_result_free.
+ *   - Device memory is owned by the engine.  One engine drives one GPU on one
+ *     HIP stream; calls on one engine are serialised internally.
+ *   - Vertex ids are uint32; clique counts are uint64.
+ */
+#ifndef CDS_H
+#define CDS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CDS_OK = 0,
+  CDS_ERR_INVALID = 1,   /* bad argument / malformed input */
+  CDS_ERR_IO = 2,        /* file could not be opened or parsed */
+  CDS_ERR_HIP = 3,       /* HIP runtime error (message has the HIP string) */
+  CDS_ERR_OOM = 4,       /* device allocation or HBM budget exceeded */
+  CDS_ERR_RANGE = 5,     /* value outside the exact range of the engine */
+  CDS_ERR_NOT_CONVERGED = 6, /* an iteration bound was hit (never a hang) */
+  CDS_ERR_UNSUPPORTED = 7,
+  CDS_ERR_BUSY = 8,      /* serving: no slot / budget right now */
+  CDS_ERR_CANCELLED = 9
+} cds_status;
+
+/* Host CSR graph: undirected, symmetric, no self loops.
+ * Replaces the reference's `Graph` (cudaCode/inc/graph.h:11-29), whose CSR is
+ * `offset[n+1]`, `neighbors[2m]` of `ui` (cudaCode/src/graph.cpp:20-33). */
+typedef struct {
+  uint32_t n;          /* vertices */
+  uint64_t m;          /* undirected edges = offsets[n] / 2 */
+  uint64_t m_header;   /* m as written in the file header (0 if from CSR) */
+  uint32_t *offsets;   /* n+1 entries */
+  uint32_t *neighbors; /* offsets[n] entries, each list sorted ascending */
+  uint32_t max_degree;
+} cds_graph;
+
+/* Solve parameters: the reference CLI's positional arguments
+ * (cudaCode/main.cu:875-885) plus the oracle's bound switches
+ * (cpuCode/c++/main.cpp:11-12, both on in the reference's canonical run). */
+typedef struct {
+  uint32_t k;              /* clique size, 2..CDS_MAX_K */
+  uint32_t p_size;         /* pSize: partial-clique partition (capacity hint) */
+  uint32_t cp_size;        /* cpSize: candidate partition (capacity hint) */
+  uint32_t gl_buffer_size; /* glBufferSize: peel frontier buffer per block */
+  uint32_t partition_size; /* partitionSize: active-node partition (flow) */
+  int32_t early_stop;      /* earlyStop; -1 = off (exact) */
+  uint32_t ub1;            /* apply the factorial upper bound (UB1) */
+  uint32_t ub2;            /* apply the clique-degree upper bound (UB2) */
+  uint32_t want_vertex_arrays; /* fill clique_degree[] and core[] */
+} cds_params;
+
+#define CDS_MAX_K 6
+
+/* Per-stage wall times (ms, measured with HIP events on the engine stream). */
+typedef struct {
+  double upload, dag, listing, core_decompose, densest_core, prune,
+      components, exact, total;
+} cds_times;
+
+/* Result of one solve.  Field names follow the reference's report keys
+ * (cudaCode/main.cu:1069-1104 and cpuCode/c++/src/helpers.cpp:1553-1588). */
+typedef struct {
+  uint32_t k;
+  uint32_t n;
+  uint64_t m;
+  uint32_t max_degree;
+  uint64_t total_cliques;           /* Total Cliques / Total_cliques */
+  uint64_t *clique_degree;          /* n entries (optional) */
+  uint32_t *core;                   /* n entries: clique-core numbers (opt.) */
+  uint64_t kmax;                    /* Denseset_core_kmax (max clique core) */
+  uint64_t k_prime;                 /* ceil(max peel-prefix density) */
+  double peel_max_density;          /* max peel-prefix density (rho-hat) */
+  double core_density;              /* Denseset_core_density */
+  uint32_t core_size;               /* Densest_core_size */
+  uint64_t core_edges;              /* Densest_core_edges_count */
+  uint64_t core_cliques;            /* cliques inside the densest core */
+  uint64_t deleted_cliques;         /* Densest_core_deleted_cliques */
+  uint64_t pruned_edges;            /* Remaining_edges_after_prune */
+  uint32_t num_components;          /* Component_number / Total Components */
+  double paper_lower_bound;         /* Paper_lower_bound */
+  float ub1_val, ub2_val;           /* last UB1 / UB2 evaluated */
+  /* final answer, as the oracle reports it */
+  float density;                    /* Density: (float, oracle semantics) */
+  uint32_t size;                    /* Size: */
+  uint32_t *vertices;               /* densest subgraph, original ids */
+  uint64_t set_cliques;             /* k-cliques inside `vertices` */
+  uint32_t set_size;                /* |vertices| (exact fraction denom.) */
+  /* density binary-search trace, one entry per (component, alpha step) */
+  uint32_t trace_len;
+  float *trace_lb, *trace_ub, *trace_alpha;
+  uint32_t *trace_component;
+  double *trace_margin;             /* kT_c - F(alpha), exact */
+  uint32_t flow_networks_solved;    /* components that ran a search */
+  uint64_t flow_sweeps;             /* push-relabel sweeps, all alphas */
+  uint32_t peel_levels;             /* level-synchronous peel rounds */
+  uint32_t replay_phases;           /* phases replayed in exact order */
+  uint32_t oracle_quirk_flags;      /* bit0: sentinel 0xFFFFFF regime */
+  cds_times times;
+} cds_result;
+
+typedef struct {
+  int device;               /* HIP device ordinal */
+  uint64_t hbm_budget_bytes;/* 0 = no budget */
+  uint32_t profile_kernels; /* record per-kernel HIP-event times */
+} cds_opts;
+
+typedef struct cds_engine cds_engine;
+typedef struct cds_dgraph cds_dgraph; /* device-resident graph */
+
+const char *cds_last_error(void);
+const char *cds_version(void);
+
+/* ---- graph I/O (replaces Graph::Graph, cudaCode/src/graph.cpp:7-40) ---- */
+cds_status cds_graph_load_adjlist(const char *path, cds_graph *out);
+cds_status cds_graph_from_csr(uint32_t n, const uint32_t *offsets,
+                              const uint32_t *neighbors, cds_graph *out);
+cds_status cds_graph_write_adjlist(const cds_graph *g, const char *path);
+void cds_graph_free(cds_graph *g);
+
+/* ---- engine (replaces the global device structs of
+ *      cudaCode/src/common.cpp:4-12 and gpuMemoryAllocation.cu) ---- */
+cds_status cds_engine_create(const cds_opts *opts, cds_engine **out);
+void cds_engine_destroy(cds_engine *e);
+
+/* Upload a host graph to HBM once; solves then start from device memory
+ * (replaces memoryAllocationGraph, cudaCode/src/gpuMemoryAllocation.cu:5-27). */
+cds_status cds_graph_upload(cds_engine *e, const cds_graph *g,
+                            cds_dgraph **out);
+void cds_dgraph_free(cds_engine *e, cds_dgraph *dg);
+
+/* Plain path, synchronous: the whole pipeline of cudaCode/main.cu:888-1060
+ * with the oracle's semantics (cpuCode/c++/src/helpers.cpp:1422-1589). */
+cds_status cds_solve(cds_engine *e, const cds_dgraph *dg,
+                     const cds_params *p, cds_result *out);
+void cds_result_free(cds_result *r);
+
+/* Stage entry points (diagnostics and per-stage parity).
+ * cds_count_cliques: T and per-vertex clique degrees
+ *   (listAllCliques + countCliques, cudaCode/main.cu:59-250;
+ *    oracle listCliques, cpuCode/c++/src/helpers.cpp:373-511).
+ *   clique_degree may be NULL.  kernel_ms (may be NULL) receives the time of
+ *   the listing kernel(s) measured with HIP events. */
+cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *dg, uint32_t k,
+                             uint64_t *total, uint64_t *clique_degree,
+                             double *kernel_ms);
+
+/* Batched plain path: many independent graphs, solved with shared launches.
+ * results[i] is identical to cds_solve on graphs[i] with params[i]. */
+cds_status cds_solve_batch(cds_engine *e, const cds_dgraph *const *graphs,
+                           const cds_params *params, size_t count,
+                           cds_result *results);
+
+/* ---- generators (seeded synthetic inputs; not part of the reference) ---- */
+cds_status cds_gen_chung_lu(uint32_t n, double exponent, double mean_degree,
+                            uint32_t planted_size, double planted_p,
+                            uint64_t seed, cds_graph *out);
+cds_status cds_gen_rmat(uint32_t scale, uint32_t edge_factor, double a,
+                        double b, double c, uint32_t planted_count,
+                        uint32_t planted_min, uint32_t planted_max,
+                        double planted_p, uint64_t seed, cds_graph *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CDS_H */
