@@ -1,0 +1,306 @@
+"""Python host side of the MI355X k-clique densest-subgraph engine.
+
+Thin ctypes bindings over the C ABI in include/cds.h (libcds_amd.so, built
+in-tree by cds_amd/build.py).  Mirrors the reference's entry point, the
+`main <graph> <k> <pSize> <cpSize> <glBufferSize> <partitionSize>
+<earlyStop>` process of cudaCode/main.cu:865-887: `solve()` takes the same
+parameters and returns the same report fields plus the CPU oracle's result
+block (cpuCode/c++/src/helpers.cpp:1553-1588).
+
+There is no CPU fallback: if the library is missing, importing works but
+every call raises `EngineUnavailable`.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcds_amd.so")
+CLI_PATH = os.path.join(HERE, "bin", "cds_main")
+
+STATUS = {
+    0: "OK", 1: "INVALID", 2: "IO", 3: "HIP", 4: "OOM", 5: "RANGE",
+    6: "NOT_CONVERGED", 7: "UNSUPPORTED", 8: "BUSY", 9: "CANCELLED",
+    10: "INTERNAL",
+}
+
+
+class EngineUnavailable(RuntimeError):
+    pass
+
+
+class CdsError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+f32p = ctypes.POINTER(ctypes.c_float)
+f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class CGraph(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint64),
+                ("m_header", ctypes.c_uint64), ("offsets", u32p),
+                ("neighbors", u32p), ("max_degree", ctypes.c_uint32)]
+
+
+class CParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("p_size", ctypes.c_uint32),
+                ("cp_size", ctypes.c_uint32),
+                ("gl_buffer_size", ctypes.c_uint32),
+                ("partition_size", ctypes.c_uint32),
+                ("early_stop", ctypes.c_int32), ("ub1", ctypes.c_uint32),
+                ("ub2", ctypes.c_uint32),
+                ("want_vertex_arrays", ctypes.c_uint32)]
+
+
+class CTimes(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "upload", "dag", "listing", "core_decompose", "densest_core", "prune",
+        "components", "exact", "total")]
+
+
+class CResult(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_uint32), ("n", ctypes.c_uint32), ("m", ctypes.c_uint64),
+        ("max_degree", ctypes.c_uint32), ("total_cliques", ctypes.c_uint64),
+        ("clique_degree", u64p), ("core", u32p), ("kmax", ctypes.c_uint64),
+        ("k_prime", ctypes.c_uint64), ("peel_max_density", ctypes.c_double),
+        ("core_density", ctypes.c_double), ("core_size", ctypes.c_uint32),
+        ("core_edges", ctypes.c_uint64), ("core_cliques", ctypes.c_uint64),
+        ("deleted_cliques", ctypes.c_uint64), ("pruned_edges", ctypes.c_uint64),
+        ("num_components", ctypes.c_uint32),
+        ("paper_lower_bound", ctypes.c_double), ("ub1_val", ctypes.c_float),
+        ("ub2_val", ctypes.c_float), ("density", ctypes.c_float),
+        ("size", ctypes.c_uint32), ("vertices", u32p),
+        ("set_cliques", ctypes.c_uint64), ("set_size", ctypes.c_uint32),
+        ("trace_len", ctypes.c_uint32), ("trace_lb", f32p), ("trace_ub", f32p),
+        ("trace_alpha", f32p), ("trace_component", u32p),
+        ("trace_margin", f64p), ("flow_networks_solved", ctypes.c_uint32),
+        ("flow_sweeps", ctypes.c_uint64), ("peel_levels", ctypes.c_uint32),
+        ("replay_phases", ctypes.c_uint32),
+        ("oracle_quirk_flags", ctypes.c_uint32), ("dag_arcs", ctypes.c_uint64),
+        ("list_work", ctypes.c_uint64), ("max_out_degree", ctypes.c_uint32),
+        ("list_items", ctypes.c_uint64), ("t_list_count_ms", ctypes.c_double),
+        ("t_list_write_ms", ctypes.c_double),
+        ("peak_device_bytes", ctypes.c_uint64), ("times", CTimes)]
+
+
+class COpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("profile_kernels", ctypes.c_uint32)]
+
+
+# Every function include/cds.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "cds_last_error": (ctypes.c_char_p, []),
+    "cds_version": (ctypes.c_char_p, []),
+    "cds_struct_size": (ctypes.c_size_t, [ctypes.c_int]),
+    "cds_graph_load_adjlist": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CGraph)]),
+    "cds_graph_from_csr": (ctypes.c_int, [ctypes.c_uint32, u32p, u32p, ctypes.POINTER(CGraph)]),
+    "cds_graph_write_adjlist": (ctypes.c_int, [ctypes.POINTER(CGraph), ctypes.c_char_p]),
+    "cds_graph_free": (None, [ctypes.POINTER(CGraph)]),
+    "cds_engine_create": (ctypes.c_int, [ctypes.POINTER(COpts), ctypes.POINTER(ctypes.c_void_p)]),
+    "cds_engine_destroy": (None, [ctypes.c_void_p]),
+    "cds_graph_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CGraph), ctypes.POINTER(ctypes.c_void_p)]),
+    "cds_dgraph_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cds_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
+    "cds_result_free": (None, [ctypes.POINTER(CResult)]),
+    "cds_count_cliques": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, f64p]),
+    "cds_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(CParams), ctypes.c_size_t, ctypes.POINTER(CResult)]),
+    "cds_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
+    "cds_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libcds_amd.so (raises EngineUnavailable if not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineUnavailable(
+                f"{LIB_PATH} is missing: run `python -m cds_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        for i, cls in enumerate((CGraph, CParams, CResult, COpts)):
+            if L.cds_struct_size(i) != ctypes.sizeof(cls):
+                raise EngineUnavailable(
+                    f"ABI mismatch for {cls.__name__}: C {L.cds_struct_size(i)}"
+                    f" vs ctypes {ctypes.sizeof(cls)}")
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise CdsError(status, lib().cds_last_error().decode())
+
+
+class Graph:
+    """Host CSR graph owned by the library (cds_graph)."""
+
+    def __init__(self):
+        self._g = CGraph()
+
+    @classmethod
+    def load(cls, path):
+        g = cls()
+        _check(lib().cds_graph_load_adjlist(os.fsencode(path), ctypes.byref(g._g)))
+        return g
+
+    @classmethod
+    def from_csr(cls, offsets, neighbors):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+        neighbors = np.ascontiguousarray(neighbors, dtype=np.uint32)
+        g = cls()
+        _check(lib().cds_graph_from_csr(
+            len(offsets) - 1, offsets.ctypes.data_as(u32p),
+            neighbors.ctypes.data_as(u32p), ctypes.byref(g._g)))
+        return g
+
+    @classmethod
+    def from_edges(cls, n, edges):
+        """Undirected graph from an iterable of (u, v) pairs."""
+        adj = [set() for _ in range(n)]
+        for u, v in edges:
+            if u != v:
+                adj[u].add(v)
+                adj[v].add(u)
+        off = np.zeros(n + 1, dtype=np.uint32)
+        for i in range(n):
+            off[i + 1] = off[i] + len(adj[i])
+        nb = np.array([w for i in range(n) for w in sorted(adj[i])], dtype=np.uint32)
+        return cls.from_csr(off, nb)
+
+    @classmethod
+    def chung_lu(cls, n, exponent, mean_degree, planted_size=0, planted_p=0.0,
+                 seed=1):
+        g = cls()
+        _check(lib().cds_gen_chung_lu(n, exponent, mean_degree, planted_size,
+                                      planted_p, seed, ctypes.byref(g._g)))
+        return g
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, planted_count=0,
+             planted_min=0, planted_max=0, planted_p=0.0, seed=1):
+        g = cls()
+        _check(lib().cds_gen_rmat(scale, edge_factor, a, b, c, planted_count,
+                                  planted_min, planted_max, planted_p, seed,
+                                  ctypes.byref(g._g)))
+        return g
+
+    @property
+    def n(self):
+        return self._g.n
+
+    @property
+    def m(self):
+        return self._g.m
+
+    @property
+    def max_degree(self):
+        return self._g.max_degree
+
+    def csr(self):
+        """(offsets, neighbors) as numpy views (valid while self lives)."""
+        off = np.ctypeslib.as_array(self._g.offsets, (self._g.n + 1,))
+        nb = np.ctypeslib.as_array(self._g.neighbors, (max(1, int(off[-1])),))[: int(off[-1])]
+        return off, nb
+
+    def write(self, path):
+        _check(lib().cds_graph_write_adjlist(ctypes.byref(self._g), os.fsencode(path)))
+
+    def __del__(self):
+        if _lib is not None and self._g.offsets:
+            _lib.cds_graph_free(ctypes.byref(self._g))
+
+
+def _arr(ptr, count, dtype):
+    if not ptr or count == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, (count,)).astype(dtype, copy=True)
+
+
+def result_to_dict(r):
+    d = {f: getattr(r, f) for f, _ in CResult._fields_
+         if f not in ("clique_degree", "core", "vertices", "trace_lb", "trace_ub",
+                      "trace_alpha", "trace_component", "trace_margin", "times")}
+    d["vertices"] = _arr(r.vertices, r.set_size, np.uint32).tolist()
+    d["clique_degree"] = _arr(r.clique_degree, r.n, np.uint64) if r.clique_degree else None
+    d["core"] = _arr(r.core, r.n, np.uint32) if r.core else None
+    n = r.trace_len
+    d["trace_lb"] = _arr(r.trace_lb, n, np.float32).tolist()
+    d["trace_ub"] = _arr(r.trace_ub, n, np.float32).tolist()
+    d["trace_alpha"] = _arr(r.trace_alpha, n, np.float32).tolist()
+    d["trace_component"] = _arr(r.trace_component, n, np.uint32).tolist()
+    d["trace_margin"] = _arr(r.trace_margin, n, np.float64).tolist()
+    d["times"] = {f: getattr(r.times, f) for f, _ in CTimes._fields_}
+    return d
+
+
+class Engine:
+    """One engine per GPU (cds_engine): owns a HIP stream and device memory."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, profile_kernels=False):
+        opts = COpts(device, hbm_budget_bytes, 1 if profile_kernels else 0)
+        h = ctypes.c_void_p()
+        _check(lib().cds_engine_create(ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+
+    def upload(self, graph):
+        return DeviceGraph(self, graph)
+
+    def solve(self, dgraph, k, p_size=8000, cp_size=16000, gl_buffer_size=60000,
+              partition_size=30000, early_stop=-1, ub1=True, ub2=True,
+              vertex_arrays=False):
+        p = CParams(k, p_size, cp_size, gl_buffer_size, partition_size,
+                    early_stop, int(ub1), int(ub2), int(vertex_arrays))
+        r = CResult()
+        _check(lib().cds_solve(self._h, dgraph._h, ctypes.byref(p), ctypes.byref(r)))
+        try:
+            return result_to_dict(r)
+        finally:
+            lib().cds_result_free(ctypes.byref(r))
+
+    def count_cliques(self, dgraph, k, clique_degree=True):
+        T = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        cd = np.zeros(dgraph.n, dtype=np.uint64) if clique_degree else None
+        _check(lib().cds_count_cliques(
+            self._h, dgraph._h, k, ctypes.byref(T),
+            cd.ctypes.data_as(u64p) if cd is not None else None, ctypes.byref(ms)))
+        return T.value, cd, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cds_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceGraph:
+    """A graph resident in HBM (cds_dgraph)."""
+
+    def __init__(self, engine, graph):
+        h = ctypes.c_void_p()
+        _check(lib().cds_graph_upload(engine._h, ctypes.byref(graph._g), ctypes.byref(h)))
+        self._h = h
+        self._engine = engine
+        self.n = graph.n
+        self.m = graph.m
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._engine._h:
+            lib().cds_dgraph_free(self._engine._h, self._h)
+            self._h = None

@@ -1,0 +1,70 @@
+"""Build the engine in-tree: cds_amd/libcds_amd.so and cds_amd/bin/cds_main.
+
+Every HIP source is compiled for gfx950 with hipcc; host C++ (which only
+calls the HIP runtime API) is compiled with g++.  All objects use
+-ffp-contract=off so the engine's float bound/search arithmetic rounds as
+written.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libcds_amd.so")
+BIN = os.path.join(HERE, "bin", "cds_main")
+
+HIP_SOURCES = ["scan.hip", "dag.hip", "list.hip", "peel.hip", "glue.hip", "flow.hip"]
+HOST_SOURCES = ["graph_io.cpp", "engine.cpp"]
+ARCH = os.environ.get("CDS_OFFLOAD_ARCH", "gfx950")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+          "-Wno-unused-result"]
+DEVICE_ONLY = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_ONLY = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+
+
+def _hipcc():
+    for cand in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.isabs(cand) and os.path.exists(cand):
+            return cand
+    return "hipcc"
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    hipcc = _hipcc()
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *DEVICE_ONLY,
+                     "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HOST_SOURCES + ["main.cpp"]:
+        obj = os.path.join(BUILD, src + ".o")
+        jobs.append(["g++", *COMMON, *HOST_ONLY, "-c", os.path.join(CSRC, src),
+                     "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout, r.stderr, file=sys.stderr)
+    objs = [os.path.join(BUILD, s + ".o") for s in HIP_SOURCES + HOST_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-lpthread"])
+    os.replace(tmp, LIB)
+    _run([hipcc, "-o", BIN, os.path.join(BUILD, "main.cpp.o"), f"-L{HERE}",
+          "-lcds_amd", "-Wl,-rpath,$ORIGIN/.."])
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

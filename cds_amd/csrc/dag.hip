@@ -1,0 +1,81 @@
+// Degree-ordered DAG (orientation) on the device.
+//
+// The reference orients by a host-computed core (degeneracy) order
+// (cudaCode/src/graph.cpp:41-113, generateDegreeDAG/generateNeighborDAG at
+// cudaCode/src/helpers.cu:112-173; oracle cpuCode/c++/src/helpers.cpp:349-371).
+// Orientation does not change any clique count, degree, core number or
+// flow; it only bounds the out-degree.  Here u -> v iff
+// (deg u, u) < (deg v, v), which bounds d+(u) by sqrt(2m) and is computed in
+// one pass without a host peel.  One wave per vertex; out-lists keep the
+// ascending id order of the input lists (order-preserving ballot
+// compaction), so every N+(u) is sorted.
+#include "dev.h"
+#include "kernels.h"
+
+namespace cds {
+namespace {
+
+__device__ __forceinline__ bool points_up(uint32_t v, uint32_t dv, uint32_t w,
+                                          uint32_t dw) {
+  return dw > dv || (dw == dv && w > v);
+}
+
+__global__ __launch_bounds__(256) void k_dag_count(
+    uint32_t n, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ adj, uint32_t *__restrict__ dcnt) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       v < n; v += waves) {
+    const uint32_t b = off[v], e = off[v + 1], dv = e - b;
+    uint32_t c = 0;
+    for (uint32_t j = b + lane(); j < e; j += kWave) {
+      const uint32_t w = adj[j];
+      c += points_up(v, dv, w, off[w + 1] - off[w]);
+    }
+    c = wave_sum(c);
+    if (lane() == 0) dcnt[v] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dag_fill(
+    uint32_t n, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ adj, const uint32_t *__restrict__ doff,
+    uint32_t *__restrict__ dadj) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       v < n; v += waves) {
+    const uint32_t b = off[v], e = off[v + 1], dv = e - b;
+    uint32_t out = doff[v];
+    for (uint32_t j = b; j < e; j += kWave) {
+      const uint32_t jj = j + lane();
+      bool up = false;
+      uint32_t w = 0;
+      if (jj < e) {
+        w = adj[jj];
+        up = points_up(v, dv, w, off[w + 1] - off[w]);
+      }
+      unsigned cnt;
+      const unsigned slot = wave_slot(up, &cnt);
+      if (up) dadj[out + slot] = w;
+      out += cnt;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                     uint32_t *dcnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_dag_count, dim3(grid_for((size_t)n * kWave, 256, 65535)),
+                     dim3(256), 0, s, n, off, adj, dcnt);
+  return hipGetLastError();
+}
+
+hipError_t dag_fill(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                    const uint32_t *doff, uint32_t *dadj, hipStream_t s) {
+  hipLaunchKernelGGL(k_dag_fill, dim3(grid_for((size_t)n * kWave, 256, 65535)),
+                     dim3(256), 0, s, n, off, adj, doff, dadj);
+  return hipGetLastError();
+}
+
+}  // namespace cds

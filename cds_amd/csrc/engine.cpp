@@ -1,0 +1,960 @@
+// Engine: host orchestration of the HIP pipeline behind the C ABI (cds.h).
+//
+// Stage order follows main() of the reference GPU path
+// (cudaCode/main.cu:888-1060): DAG, listing, clique-core peel, densest
+// core, prune, components, per-component exact search.  The result is the
+// one the reference's CPU oracle reports (CDS::DSD,
+// cpuCode/c++/src/helpers.cpp:1422-1589): its peel-prefix rule for k', its
+// densest core, its component order, its float bound / alpha / density
+// arithmetic, with every max-flow evaluated exactly on the device.
+//
+// This file is compiled with -ffp-contract=off and without fast-math: the
+// float expressions below must round exactly as written.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+using namespace cds;
+
+struct cds_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DeviceArena *arena = nullptr;
+  uint64_t budget = 0;
+  bool profile = false;
+  std::mutex mu;
+  void *pinned = nullptr;  // small readbacks
+};
+
+struct cds_dgraph {
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t max_degree = 0;
+  uint32_t *off = nullptr;
+  uint32_t *adj = nullptr;
+  std::vector<uint32_t> host_off;  // for host-side statistics
+};
+
+namespace {
+
+#define HIPCHK(expr)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(e_ == hipErrorOutOfMemory ? CDS_ERR_OOM : CDS_ERR_HIP,   \
+                  "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                     \
+  } while (0)
+
+#define CHK(expr)                       \
+  do {                                  \
+    cds_status s_ = (expr);             \
+    if (s_ != CDS_OK) return s_;        \
+  } while (0)
+
+// Device buffer owned by the arena for the duration of a solve.
+template <class T> struct DBuf {
+  DeviceArena *a = nullptr;
+  T *p = nullptr;
+  size_t count = 0;
+  explicit DBuf(DeviceArena *arena) : a(arena) {}
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) a->release(p);
+    p = nullptr;
+    count = 0;
+  }
+  hipError_t alloc(size_t n) {
+    reset();
+    count = n;
+    return a->alloc((void **)&p, (n ? n : 1) * sizeof(T));
+  }
+  hipError_t zero() {
+    return hipMemsetAsync(p, 0, (count ? count : 1) * sizeof(T), a->stream());
+  }
+};
+
+struct Timer {
+  hipEvent_t ev[12];
+  int used = 0;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    for (auto &e : ev) (void)hipEventCreate(&e);
+  }
+  ~Timer() {
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+  int mark() {
+    (void)hipEventRecord(ev[used], s);
+    return used++;
+  }
+  double ms(int a, int b) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+    return t;
+  }
+};
+
+template <class T>
+cds_status readback(cds_engine *e, T *host, const T *dev, size_t count) {
+  HIPCHK(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost,
+                        e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return CDS_OK;
+}
+
+template <class T>
+cds_status upload(cds_engine *e, T *dev, const T *host, size_t count) {
+  HIPCHK(hipMemcpyAsync(dev, host, count * sizeof(T), hipMemcpyHostToDevice,
+                        e->stream));
+  return CDS_OK;
+}
+
+// ---- float arithmetic of the oracle's bounds and search --------------------
+
+// UB1 (cpuCode/c++/src/helpers.cpp:1168-1181) as the reference build
+// evaluates it: powf(T, (k-1)/k) * expf(-(sum_i logf(i)) / k).
+float ub1_value(uint32_t k, uint64_t total) {
+  const float kf = (float)k;
+  float log_fact = 0.0f;
+  for (uint32_t i = 1; i <= k; ++i) log_fact = log_fact + logf((float)i);
+  const float num = powf((float)total, (kf - 1.0f) / kf);
+  return num * expf(-(log_fact / kf));
+}
+
+// UB2 (helpers.cpp:1186-1198) as the reference build evaluates it:
+// T * sum_v fma((cd_v * (1/(k T)))^2).
+float ub2_value(uint32_t k, uint64_t total, const uint32_t *cd, uint32_t V) {
+  const float dem = (float)((uint64_t)k * total);
+  const float inv = 1.0f / dem;
+  float acc = 0.0f;
+  for (uint32_t v = 0; v < V; ++v) {
+    const float pv = (float)cd[v] * inv;
+    acc = fmaf(pv, pv, acc);
+  }
+  return acc * (float)total;
+}
+
+// |F - target| < 1e-2f evaluated exactly; F_scaled = F * 2^sc, target is
+// an integral float, 1e-2f = 5368709 * 2^-29.
+bool flow_equal(long long f_scaled, float target, int sc) {
+  const __int128 t = (__int128)(long long)target << sc;
+  __int128 d = (__int128)f_scaled - t;
+  if (d < 0) d = -d;
+  return d * ((__int128)1 << 29) < (__int128)5368709 << sc;
+}
+
+struct Component {
+  uint32_t V = 0;        // vertices
+  uint64_t T = 0;        // cliques
+  uint32_t vbase = 0;    // first flow-vertex id
+  uint64_t cbase = 0;    // first clique index
+  uint32_t first = 0;    // smallest original vertex id
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) {
+  if (!out) return fail(CDS_ERR_INVALID, "null out");
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev == 0)
+    return fail(CDS_ERR_HIP, "no HIP device available (%s)",
+                hipGetErrorString(he));
+  auto *e = new cds_engine();
+  e->device = opts ? opts->device : 0;
+  e->budget = opts ? opts->hbm_budget_bytes : 0;
+  e->profile = opts ? opts->profile_kernels != 0 : false;
+  if (e->device < 0 || e->device >= ndev) {
+    delete e;
+    return fail(CDS_ERR_INVALID, "device %d out of range (%d devices)",
+                opts ? opts->device : 0, ndev);
+  }
+  hipError_t r = hipSetDevice(e->device);
+  if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (r == hipSuccess) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, e->device) == hipSuccess) {
+      uint64_t keep = ~0ull;  // keep freed blocks cached in the pool
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    r = hipHostMalloc(&e->pinned, 4096, hipHostMallocDefault);
+  }
+  if (r != hipSuccess) {
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return fail(CDS_ERR_HIP, "engine init failed: %s", hipGetErrorString(r));
+  }
+  e->arena = new DeviceArena(e->stream);
+  *out = e;
+  return CDS_OK;
+}
+
+extern "C" void cds_engine_destroy(cds_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  delete e->arena;
+  if (e->pinned) (void)hipHostFree(e->pinned);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" cds_status cds_graph_upload(cds_engine *e, const cds_graph *g,
+                                       cds_dgraph **out) {
+  if (!e || !g || !out) return fail(CDS_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  auto *d = new cds_dgraph();
+  d->n = g->n;
+  d->m = g->m;
+  d->max_degree = g->max_degree;
+  const size_t nn = (size_t)g->n + 1, na = (size_t)g->offsets[g->n];
+  if (e->budget && (nn + na) * 4 > e->budget) {
+    delete d;
+    return fail(CDS_ERR_OOM, "graph exceeds the HBM budget");
+  }
+  hipError_t r = hipMalloc(&d->off, nn * 4);
+  if (r == hipSuccess) r = hipMalloc(&d->adj, (na ? na : 1) * 4);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(d->off, g->offsets, nn * 4, hipMemcpyHostToDevice, e->stream);
+  if (r == hipSuccess && na)
+    r = hipMemcpyAsync(d->adj, g->neighbors, na * 4, hipMemcpyHostToDevice, e->stream);
+  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+  if (r != hipSuccess) {
+    (void)hipFree(d->off);
+    (void)hipFree(d->adj);
+    delete d;
+    return fail(CDS_ERR_HIP, "graph upload failed: %s", hipGetErrorString(r));
+  }
+  d->host_off.assign(g->offsets, g->offsets + nn);
+  *out = d;
+  return CDS_OK;
+}
+
+extern "C" void cds_dgraph_free(cds_engine *e, cds_dgraph *d) {
+  if (!d) return;
+  if (e) (void)hipSetDevice(e->device);
+  (void)hipFree(d->off);
+  (void)hipFree(d->adj);
+  delete d;
+}
+
+extern "C" void cds_result_free(cds_result *r) {
+  if (!r) return;
+  free(r->clique_degree);
+  free(r->core);
+  free(r->vertices);
+  free(r->trace_lb);
+  free(r->trace_ub);
+  free(r->trace_alpha);
+  free(r->trace_component);
+  free(r->trace_margin);
+  memset(r, 0, sizeof(*r));
+}
+
+namespace {
+
+// DAG + listing shared by cds_solve and cds_count_cliques.  On return
+// (materialise = true) `cliques` holds T*k ids AoS.
+struct Listing {
+  DBuf<uint32_t> dcnt, doff, dadj, ipv, item_count, cliques, maxd;
+  DBuf<uint64_t> item_off, item_base, cd;
+  DBuf<uint2> items;
+  uint64_t T = 0, items_n = 0, work = 0;
+  uint32_t max_out = 0;
+  double t_count = 0, t_write = 0;
+  explicit Listing(DeviceArena *a)
+      : dcnt(a), doff(a), dadj(a), ipv(a), item_count(a), cliques(a), maxd(a),
+        item_off(a), item_base(a), cd(a), items(a) {}
+};
+
+cds_status run_dag(cds_engine *e, const cds_dgraph *g, Listing &L) {
+  DeviceArena &A = *e->arena;
+  const uint32_t n = g->n;
+  HIPCHK(L.dcnt.alloc(n));
+  HIPCHK(L.doff.alloc((size_t)n + 1));
+  HIPCHK(dag_count(n, g->off, g->adj, L.dcnt.p, e->stream));
+  HIPCHK(scan_u32_32(L.dcnt.p, L.doff.p, n, A));
+  HIPCHK(L.dadj.alloc(g->m));
+  HIPCHK(dag_fill(n, g->off, g->adj, L.doff.p, L.dadj.p, e->stream));
+  HIPCHK(L.maxd.alloc(1));
+  HIPCHK(reduce_max_u32(L.dcnt.p, n, L.maxd.p, e->stream));
+  CHK(readback(e, &L.max_out, L.maxd.p, 1));
+  // W = sum over DAG arcs (u,v) of d+(v) = sum_v indeg(v) * d+(v).
+  std::vector<uint32_t> dc(n);
+  CHK(readback(e, dc.data(), L.dcnt.p, n));
+  uint64_t w = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint64_t deg = g->host_off[v + 1] - g->host_off[v];
+    w += (deg - dc[v]) * (uint64_t)dc[v];
+  }
+  L.work = w;
+  return CDS_OK;
+}
+
+cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
+                       bool materialise, Listing &L, Timer *tm) {
+  DeviceArena &A = *e->arena;
+  const uint32_t n = g->n;
+  HIPCHK(L.cd.alloc(n));
+  HIPCHK(L.cd.zero());
+  if (k == 2) {
+    L.T = g->m;
+    int t0 = tm->mark();
+    HIPCHK(list_edges(n, L.doff.p, L.dadj.p, L.cd.p, nullptr, e->stream));
+    int t1 = tm->mark();
+    if (materialise) {
+      HIPCHK(L.cliques.alloc(2 * L.T));
+      HIPCHK(list_edges(n, L.doff.p, L.dadj.p, nullptr, L.cliques.p, e->stream));
+    }
+    int t2 = tm->mark();
+    HIPCHK(hipStreamSynchronize(e->stream));
+    L.t_count = tm->ms(t0, t1);
+    L.t_write = tm->ms(t1, t2);
+    return CDS_OK;
+  }
+  if (k == 4 && L.max_out > kList4LdsCap)
+    return fail(CDS_ERR_UNSUPPORTED,
+                "k=4 listing needs every DAG out-degree <= %u (max is %u)",
+                kList4LdsCap, L.max_out);
+  HIPCHK(L.ipv.alloc(n));
+  HIPCHK(L.item_off.alloc((size_t)n + 1));
+  HIPCHK(list_item_counts(n, L.doff.p, L.ipv.p, e->stream));
+  HIPCHK(scan_u32(L.ipv.p, L.item_off.p, n, A));
+  CHK(readback(e, &L.items_n, L.item_off.p + n, 1));
+  HIPCHK(L.items.alloc(L.items_n));
+  HIPCHK(list_item_fill(n, L.doff.p, L.item_off.p, L.items.p, e->stream));
+  HIPCHK(L.item_count.alloc(L.items_n));
+  HIPCHK(L.item_base.alloc(L.items_n + 1));
+  int t0 = tm->mark();
+  HIPCHK(list_cliques(k, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+                      L.item_count.p, L.cd.p, nullptr, nullptr, e->stream));
+  int t1 = tm->mark();
+  HIPCHK(scan_u32(L.item_count.p, L.item_base.p, L.items_n, A));
+  CHK(readback(e, &L.T, L.item_base.p + L.items_n, 1));
+  L.t_count = tm->ms(t0, t1);
+  if (materialise) {
+    if (L.T >= 0xFFFFFFFFull || L.T * k >= (1ull << 40))
+      return fail(CDS_ERR_RANGE, "%llu cliques exceed the 32-bit clique ids",
+                  (unsigned long long)L.T);
+    HIPCHK(L.cliques.alloc(L.T * k));
+    int t2 = tm->mark();
+    HIPCHK(list_cliques(k, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+                        nullptr, nullptr, L.item_base.p, L.cliques.p,
+                        e->stream));
+    int t3 = tm->mark();
+    HIPCHK(hipStreamSynchronize(e->stream));
+    L.t_write = tm->ms(t2, t3);
+  }
+  return CDS_OK;
+}
+
+struct Level {
+  uint32_t level;
+  uint64_t removed, killed;
+};
+
+}  // namespace
+
+extern "C" cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *g,
+                                        uint32_t k, uint64_t *total,
+                                        uint64_t *clique_degree,
+                                        double *kernel_ms) {
+  if (!e || !g || !total) return fail(CDS_ERR_INVALID, "null argument");
+  if (k < 2 || k > 4)
+    return fail(CDS_ERR_UNSUPPORTED, "k=%u: the device lister covers k=2..4", k);
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  Timer tm(e->stream);
+  Listing L(e->arena);
+  CHK(run_dag(e, g, L));
+  CHK(run_listing(e, g, k, false, L, &tm));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  *total = L.T;
+  if (clique_degree) CHK(readback(e, clique_degree, L.cd.p, g->n));
+  if (kernel_ms) *kernel_ms = L.t_count;
+  return CDS_OK;
+}
+
+namespace {
+
+cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
+                      cds_result *R) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  DeviceArena &A = *e->arena;
+  hipStream_t st = e->stream;
+  const uint32_t n = g->n, k = p->k;
+  memset(R, 0, sizeof(*R));
+  R->k = k;
+  R->n = n;
+  R->m = g->m;
+  R->max_degree = g->max_degree;
+  if (n == 0) return fail(CDS_ERR_INVALID, "empty graph");
+  Timer tm(st);
+  const int ev_start = tm.mark();
+
+  // ---- (a) DAG + listing ----
+  Listing L(&A);
+  CHK(run_dag(e, g, L));
+  const int ev_dag = tm.mark();
+  CHK(run_listing(e, g, k, true, L, &tm));
+  const int ev_list = tm.mark();
+  const uint64_t T = L.T;
+  R->total_cliques = T;
+  R->dag_arcs = g->m;
+  R->list_work = L.work;
+  R->max_out_degree = L.max_out;
+  R->list_items = L.items_n;
+  R->t_list_count_ms = L.t_count;
+  R->t_list_write_ms = L.t_write;
+  std::vector<uint64_t> cd(n);
+  CHK(readback(e, cd.data(), L.cd.p, n));
+  uint64_t sum_cd = 0, max_cd = 0;
+  for (uint64_t x : cd) {
+    sum_cd += x;
+    max_cd = std::max(max_cd, x);
+  }
+  if (sum_cd != T * k)
+    return fail(CDS_ERR_INTERNAL, "listing invariant: sum cd %llu != k*T %llu",
+                (unsigned long long)sum_cd, (unsigned long long)(T * k));
+  if (max_cd >= 0xFFFFFFFFull)
+    return fail(CDS_ERR_RANGE, "a clique degree exceeds 32 bits");
+  if (sum_cd >= (1ull << 31)) R->oracle_quirk_flags |= 4;  // int total wraps
+
+  // ---- (b) peel ----
+  DBuf<uint64_t> ioff(&A);
+  DBuf<uint32_t> icur(&A), inc(&A), deg(&A), core(&A), kill(&A), fa(&A),
+      fb(&A), cnt(&A);
+  DBuf<unsigned long long> killed(&A);
+  HIPCHK(ioff.alloc((size_t)n + 1));
+  HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
+  HIPCHK(icur.alloc(n));
+  HIPCHK(icur.zero());
+  HIPCHK(inc.alloc(T * k));
+  HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
+  icur.reset();
+  HIPCHK(deg.alloc(n));
+  HIPCHK(core.alloc(n));
+  HIPCHK(kill.alloc(T));
+  HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
+  HIPCHK(fa.alloc(n));
+  HIPCHK(fb.alloc(n));
+  HIPCHK(cnt.alloc(4));  // [0] min, [1] count A, [2] count B
+  HIPCHK(killed.alloc(1));
+  uint32_t *hp = (uint32_t *)e->pinned;
+  std::vector<Level> levels;
+  uint64_t removed_total = 0;
+  uint32_t peel_rounds = 0;
+  while (removed_total < n) {
+    HIPCHK(peel_min_alive(n, deg.p, core.p, cnt.p, st));
+    HIPCHK(hipMemsetAsync(cnt.p + 1, 0, 8, st));
+    HIPCHK(killed.zero());
+    CHK(readback(e, hp, cnt.p, 1));
+    const uint32_t level = hp[0];
+    if (level == kAliveHost)
+      return fail(CDS_ERR_INTERNAL, "peel: no alive vertex but %llu removed",
+                  (unsigned long long)removed_total);
+    HIPCHK(peel_frontier(n, level, deg.p, core.p, fa.p, cnt.p + 1, st));
+    CHK(readback(e, hp + 1, cnt.p + 1, 1));
+    uint32_t f = hp[1];
+    uint64_t removed = 0;
+    uint32_t *cur = fa.p, *nxt = fb.p, *ccur = cnt.p + 1, *cnxt = cnt.p + 2;
+    while (f > 0) {
+      removed += f;
+      HIPCHK(hipMemsetAsync(cnxt, 0, 4, st));
+      HIPCHK(peel_process(k, level, cur, ccur, f, ioff.p, inc.p, L.cliques.p,
+                          deg.p, core.p, kill.p, nxt, cnxt, killed.p, st));
+      CHK(readback(e, hp + 2, cnxt, 1));
+      f = hp[2];
+      std::swap(cur, nxt);
+      std::swap(ccur, cnxt);
+      ++peel_rounds;
+    }
+    unsigned long long kl = 0;
+    CHK(readback(e, &kl, killed.p, 1));
+    levels.push_back({level, removed, kl});
+    removed_total += removed;
+  }
+  fa.reset();
+  fb.reset();
+  R->peel_levels = peel_rounds;
+
+  // Boundary densities: the state before level levels[i] is the clique core
+  // C_level; the oracle's rows include each of them (helpers.cpp:125-134).
+  std::vector<uint64_t> rm_before(levels.size()), kl_before(levels.size());
+  {
+    uint64_t r = 0, kk = 0;
+    for (size_t i = 0; i < levels.size(); ++i) {
+      rm_before[i] = r;
+      kl_before[i] = kk;
+      r += levels[i].removed;
+      kk += levels[i].killed;
+    }
+    if (kk != T)
+      return fail(CDS_ERR_INTERNAL, "peel killed %llu of %llu cliques",
+                  (unsigned long long)kk, (unsigned long long)T);
+  }
+  double rho = 0;
+  for (size_t i = 0; i < levels.size(); ++i) {
+    const uint64_t N = n - rm_before[i], Tr = T - kl_before[i];
+    const double d = (double)(int64_t)Tr / static_cast<double>(N);
+    if (i == 0 || d > rho) rho = d;
+  }
+  int64_t kp = (int64_t)std::ceil(rho);
+  // Levels above ceil(max boundary density) may hold intermediate states of
+  // higher density; replay those whose bound can exceed the current k'.
+  {
+    DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
+    DBuf<uint8_t> palive(&A);
+    DBuf<ReplayOut> rout(&A);
+    bool allocated = false;
+    for (size_t ii = levels.size(); ii-- > 0;) {
+      const Level &lv = levels[ii];
+      if ((int64_t)lv.level <= kp) break;
+      const uint64_t M = lv.removed;
+      if (M < 2) continue;
+      const uint64_t N_end = n - rm_before[ii] - M;
+      const uint64_t T_end = T - kl_before[ii] - lv.killed;
+      // f <= (T_end + x*l) / (N_end + x) for x = M-1 remaining phase vertices
+      const unsigned __int128 num = (unsigned __int128)T_end + (unsigned __int128)(M - 1) * lv.level;
+      const unsigned __int128 den = N_end + M - 1;
+      const unsigned __int128 ceil_ub = (num + den - 1) / den;
+      if ((int64_t)ceil_ub <= kp) continue;
+      if (!allocated) {
+        HIPCHK(pv.alloc(n));
+        HIPCHK(pdeg.alloc(n));
+        HIPCHK(posmap.alloc(n));
+        HIPCHK(palive.alloc(n));
+        HIPCHK(rdead.alloc(T));
+        HIPCHK(rdead.zero());
+        HIPCHK(pc.alloc(1));
+        HIPCHK(rout.alloc(1));
+        allocated = true;
+      }
+      HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
+                          L.cliques.p, T - kl_before[ii], n - rm_before[ii],
+                          (uint32_t)M, pv.p, pdeg.p, posmap.p, palive.p,
+                          rdead.p, pc.p, rout.p, st));
+      ReplayOut ro;
+      CHK(readback(e, &ro, rout.p, 1));
+      R->replay_phases++;
+      if (ro.steps != M)
+        return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
+                    lv.level, ro.steps, (unsigned long long)M);
+      if (ro.found) {
+        const double d = (double)(int64_t)ro.best_num / static_cast<double>(ro.best_den);
+        if (d > rho) rho = d;
+        kp = std::max(kp, (int64_t)std::ceil(d));
+      }
+    }
+  }
+  R->peel_max_density = rho;
+  R->k_prime = (uint64_t)kp;
+  uint64_t kmax = 0;
+  for (auto &lv : levels)
+    if (lv.removed) kmax = std::max<uint64_t>(kmax, lv.level);
+  R->kmax = kmax;
+  const int ev_peel = tm.mark();
+
+  // ---- densest core ----
+  size_t core_level = levels.size();
+  for (size_t i = 0; i < levels.size(); ++i)
+    if ((int64_t)levels[i].level >= kp) {
+      core_level = i;
+      break;
+    }
+  if (core_level == levels.size())
+    return fail(CDS_ERR_INTERNAL, "empty densest core (k'=%lld)", (long long)kp);
+  const uint32_t core_n = (uint32_t)(n - rm_before[core_level]);
+  const uint64_t core_T = T - kl_before[core_level];
+  R->core_size = core_n;
+  R->core_cliques = core_T;
+  R->deleted_cliques = T - core_T;
+  R->core_density = (double)(int64_t)core_T / static_cast<double>(core_n);
+  const uint32_t kpu = (uint32_t)kp;
+  DBuf<unsigned long long> acc(&A);
+  HIPCHK(acc.alloc(2));
+  HIPCHK(core_edge_count(n, g->off, g->adj, core.p, kpu, acc.p, st));
+  unsigned long long ce = 0;
+  CHK(readback(e, &ce, acc.p, 1));
+  R->core_edges = ce / 2;
+  HIPCHK(count_kill_ge(T, kill.p, kpu, acc.p, st));
+  unsigned long long ct = 0;
+  CHK(readback(e, &ct, acc.p, 1));
+  if (ct != core_T)
+    return fail(CDS_ERR_INTERNAL, "core clique count %llu != %llu", ct,
+                (unsigned long long)core_T);
+  const int ev_core = tm.mark();
+
+  // ---- prune ----
+  {
+    DBuf<uint8_t> mark(&A);
+    const size_t na = g->host_off[n];
+    HIPCHK(mark.alloc(na));
+    HIPCHK(mark.zero());
+    HIPCHK(prune_mark(T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st));
+    HIPCHK(count_nonzero_u8(na, mark.p, acc.p, st));
+    unsigned long long pe = 0;
+    CHK(readback(e, &pe, acc.p, 1));
+    R->pruned_edges = pe / 2;
+  }
+  const int ev_prune = tm.mark();
+
+  // ---- components ----
+  DBuf<uint32_t> parent(&A);
+  HIPCHK(parent.alloc(n));
+  HIPCHK(uf_init(n, parent.p, st));
+  HIPCHK(uf_union_cliques(T, k, L.cliques.p, kill.p, kpu, parent.p, st));
+  HIPCHK(uf_flatten(n, parent.p, st));
+  std::vector<uint32_t> hpar(n), hcore(n);
+  CHK(readback(e, hpar.data(), parent.p, n));
+  CHK(readback(e, hcore.data(), core.p, n));
+  std::vector<uint32_t> comp_of(n, 0), fv_of(n, 0), fv_orig(core_n);
+  std::vector<Component> comps;
+  {
+    std::vector<uint32_t> cidx(n, 0);
+    for (uint32_t v = 0; v < n; ++v) {
+      if (hcore[v] < kpu) continue;
+      if (hpar[v] == v) {
+        cidx[v] = (uint32_t)comps.size();
+        Component c;
+        c.first = v;
+        comps.push_back(c);
+      }
+      const uint32_t ci = cidx[hpar[v]];
+      comp_of[v] = ci;
+      comps[ci].V++;
+    }
+    uint32_t base = 0;
+    for (auto &c : comps) {
+      c.vbase = base;
+      base += c.V;
+    }
+    if (base != core_n)
+      return fail(CDS_ERR_INTERNAL, "component sizes %u != core %u", base, core_n);
+    std::vector<uint32_t> fill(comps.size(), 0);
+    for (uint32_t v = 0; v < n; ++v) {
+      if (hcore[v] < kpu) continue;
+      const uint32_t ci = comp_of[v];
+      fv_of[v] = comps[ci].vbase + fill[ci]++;
+      fv_orig[fv_of[v]] = v;
+    }
+  }
+  const uint32_t ncomp = (uint32_t)comps.size();
+  R->num_components = ncomp;
+  DBuf<uint32_t> d_comp_of(&A), d_fv_of(&A), ccount(&A), fcd(&A), ccur(&A),
+      fmem(&A), ficur(&A), finc(&A);
+  DBuf<uint64_t> coff(&A), fioff(&A);
+  HIPCHK(d_comp_of.alloc(n));
+  HIPCHK(d_fv_of.alloc(n));
+  CHK(upload(e, d_comp_of.p, comp_of.data(), n));
+  CHK(upload(e, d_fv_of.p, fv_of.data(), n));
+  HIPCHK(ccount.alloc(ncomp));
+  HIPCHK(ccount.zero());
+  HIPCHK(fcd.alloc(core_n));
+  HIPCHK(fcd.zero());
+  HIPCHK(comp_clique_count(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
+                           d_fv_of.p, ccount.p, fcd.p, st));
+  std::vector<uint32_t> hcc(ncomp), hfcd(core_n);
+  CHK(readback(e, hcc.data(), ccount.p, ncomp));
+  CHK(readback(e, hfcd.data(), fcd.p, core_n));
+  std::vector<uint64_t> hcoff(ncomp + 1, 0);
+  for (uint32_t c = 0; c < ncomp; ++c) {
+    comps[c].T = hcc[c];
+    comps[c].cbase = hcoff[c];
+    hcoff[c + 1] = hcoff[c] + hcc[c];
+  }
+  if (hcoff[ncomp] != core_T)
+    return fail(CDS_ERR_INTERNAL, "bucketed %llu of %llu core cliques",
+                (unsigned long long)hcoff[ncomp], (unsigned long long)core_T);
+  if (core_T * k >= 0xFFFFFFFFull)
+    return fail(CDS_ERR_RANGE, "core clique slots exceed 32 bits");
+  HIPCHK(coff.alloc(ncomp + 1));
+  CHK(upload(e, coff.p, hcoff.data(), ncomp + 1));
+  HIPCHK(ccur.alloc(ncomp));
+  HIPCHK(ccur.zero());
+  HIPCHK(fmem.alloc(core_T * k));
+  HIPCHK(comp_clique_scatter(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
+                             d_fv_of.p, coff.p, ccur.p, fmem.p, st));
+  HIPCHK(fioff.alloc((size_t)core_n + 1));
+  HIPCHK(scan_u32(fcd.p, fioff.p, core_n, A));
+  HIPCHK(ficur.alloc(core_n));
+  HIPCHK(ficur.zero());
+  HIPCHK(finc.alloc(core_T * k));
+  HIPCHK(flow_incidence(core_T, k, fmem.p, fioff.p, ficur.p, finc.p, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int ev_cc = tm.mark();
+
+  // ---- (c) dynamicExact (helpers.cpp:1120-1254) ----
+  // Initial lower bound: the densest component, in float, then the core
+  // density if its ceiling is larger (helpers.cpp:1128-1143).
+  std::vector<double> cdens(ncomp);
+  for (uint32_t c = 0; c < ncomp; ++c)
+    cdens[c] = (double)(int64_t)comps[c].T / ((double)comps[c].V);
+  float lower = 0.0f;
+  uint32_t current = 0;
+  for (uint32_t c = 0; c < ncomp; ++c)
+    if (lower < cdens[c]) {
+      lower = (float)cdens[c];
+      current = c;
+    }
+  if (std::ceil(lower) < std::ceil(R->core_density)) lower = (float)R->core_density;
+  float upper = (float)(int64_t)kmax;
+  double best_density = lower;
+  uint32_t best_comp = current;
+  std::vector<int> best_res;  // empty = the whole component
+  uint64_t best_cliques = comps[current].T;
+  uint32_t best_size = comps[current].V;
+  double lbp = 0;
+  for (auto d : cdens)
+    if (lbp < d) lbp = d;
+  R->paper_lower_bound = lbp;
+
+  std::vector<float> tr_lb, tr_ub, tr_alpha;
+  std::vector<uint32_t> tr_comp;
+  std::vector<double> tr_margin;
+  uint64_t sweeps = 0;
+  // flow state, sized for the largest component
+  uint32_t maxV = 0;
+  uint64_t maxT = 0;
+  for (auto &c : comps) {
+    maxV = std::max(maxV, c.V);
+    maxT = std::max(maxT, c.T);
+  }
+  DBuf<long long> fx(&A), frvs(&A), fex(&A), fdt(&A), fda(&A);
+  DBuf<int> fh(&A), freach(&A);
+  DBuf<uint32_t> fq0(&A), fq1(&A);
+  DBuf<unsigned long long> fmisc(&A);
+  bool flow_alloc = false;
+  std::vector<int> res, reach_h;
+  for (uint32_t ci = 0; ci < ncomp; ++ci) {
+    const Component &C = comps[ci];
+    if (p->ub1) {
+      R->ub1_val = ub1_value(k, C.T);
+      if (R->ub1_val < upper) upper = R->ub1_val;
+    }
+    if (p->ub2) {
+      R->ub2_val = ub2_value(k, C.T, hfcd.data() + C.vbase, C.V);
+      if (R->ub2_val < upper) upper = R->ub2_val;
+    }
+    // exact(): helpers.cpp:1256-1300
+    float lb = lower, ub = upper;
+    float alpha = (lb + ub) * 0.5f;
+    const int32_t vv = (int32_t)((uint32_t)C.V * (uint32_t)(C.V - 1));
+    float bias = 1.0f / (float)vv;
+    if (bias < 1e-2f) bias = 1e-2f;
+    const float target = (float)(int64_t)(C.T * k);
+    res.assign(C.V, 1);
+    uint64_t res_cliques = C.T;
+    bool searched = false;
+    while ((ub - lb) > bias) {
+      if (!flow_alloc) {
+        HIPCHK(fx.alloc(maxT * k));
+        HIPCHK(frvs.alloc(maxV));
+        HIPCHK(fex.alloc(maxV + maxT));
+        HIPCHK(fdt.alloc(maxV + maxT));
+        HIPCHK(fda.alloc(maxV + maxT));
+        HIPCHK(fh.alloc(maxV + maxT));
+        HIPCHK(freach.alloc(maxV));
+        HIPCHK(fq0.alloc(maxV + maxT + 1));
+        HIPCHK(fq1.alloc(maxV + maxT + 1));
+        HIPCHK(fmisc.alloc(8));
+        flow_alloc = true;
+      }
+      searched = true;
+      const float cap_t = alpha * (float)k;
+      int e2 = 0;
+      std::frexp(cap_t, &e2);
+      const int sc = std::max(0, 24 - e2);
+      if (sc > 40)
+        return fail(CDS_ERR_RANGE, "alpha %g too small for exact scaling", alpha);
+      const long long U = 1ll << sc;
+      const long long capT = (long long)std::ldexp((double)cap_t, sc);
+      uint32_t maxcd = 0;
+      for (uint32_t v = 0; v < C.V; ++v) maxcd = std::max(maxcd, hfcd[C.vbase + v]);
+      if ((__int128)maxcd * U >= ((__int128)1 << 62) ||
+          (__int128)capT * C.V >= ((__int128)1 << 62) ||
+          (__int128)C.T * k * U >= ((__int128)1 << 62))
+        return fail(CDS_ERR_RANGE, "component %u exceeds the int64 flow range", ci);
+      FlowProblem P;
+      P.k = k;
+      P.V = C.V;
+      P.T = (uint32_t)C.T;
+      P.vbase = C.vbase;
+      P.sbase = C.cbase * k;
+      P.mem = fmem.p + C.cbase * k;
+      P.ioff = fioff.p + C.vbase;
+      P.inc = finc.p;
+      P.cd = fcd.p + C.vbase;
+      FlowState S;
+      S.x = fx.p;
+      S.rvs = frvs.p;
+      S.excess = fex.p;
+      S.h = fh.p;
+      S.dtgt = fdt.p;
+      S.damt = fda.p;
+      S.queue0 = fq0.p;
+      S.queue1 = fq1.p;
+      S.misc = fmisc.p;
+      S.reach = freach.p;
+      const uint64_t nodes = (uint64_t)C.V + C.T;
+      const uint32_t max_rounds =
+          (uint32_t)std::min<uint64_t>(4000000000ull, 64 * nodes * nodes + 1000000);
+      HIPCHK(max_flow(P, S, capT, U, max_rounds, st));
+      unsigned long long misc[4];
+      CHK(readback(e, misc, fmisc.p, 4));
+      if (misc[2])
+        return fail(CDS_ERR_NOT_CONVERGED,
+                    "max flow of component %u hit the round limit", ci);
+      sweeps += misc[1];
+      const long long F = (long long)misc[0];
+      const bool eq = flow_equal(F, target, sc);
+      tr_comp.push_back(ci);
+      tr_lb.push_back(lb);
+      tr_ub.push_back(ub);
+      tr_alpha.push_back(alpha);
+      tr_margin.push_back((double)(C.T * k) - std::ldexp((double)F, -sc));
+      if (eq) {
+        ub = alpha;
+      } else {
+        lb = alpha;
+        reach_h.resize(C.V);
+        CHK(readback(e, reach_h.data(), freach.p, C.V));
+        for (uint32_t v = 0; v < C.V; ++v) res[v] = reach_h[v] ? 1 : -1;
+        HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, P.mem, C.vbase,
+                                      freach.p, acc.p, st));
+        unsigned long long cc = 0;
+        CHK(readback(e, &cc, acc.p, 1));
+        res_cliques = cc;
+      }
+      alpha = (lb + ub) * 0.5f;
+    }
+    if (searched) R->flow_networks_solved++;
+    // count motifs and vertices in the result (helpers.cpp:1210-1252)
+    uint32_t vcount = 0;
+    for (uint32_t v = 0; v < C.V; ++v) vcount += res[v] != -1;
+    uint64_t ccount_res = res_cliques;
+    if (vcount == 0) vcount = C.V;
+    const float density = (float)(int64_t)ccount_res / (float)vcount;
+    if (density > best_density) {
+      best_density = density;
+      lower = density;
+      best_comp = ci;
+      best_res = res;
+      best_size = vcount;
+      best_cliques = ccount_res;
+    }
+  }
+  R->flow_sweeps = sweeps;
+  const int ev_exact = tm.mark();
+
+  // ---- result ----
+  R->density = (float)best_density;
+  R->size = best_size;
+  std::vector<uint32_t> verts;
+  {
+    const Component &C = comps[best_comp];
+    for (uint32_t v = 0; v < C.V; ++v)
+      if (best_res.empty() || best_res[v] != -1)
+        verts.push_back(fv_orig[C.vbase + v]);
+  }
+  (void)best_cliques;
+  R->set_size = (uint32_t)verts.size();
+  R->vertices = host_alloc<uint32_t>(verts.size());
+  memcpy(R->vertices, verts.data(), verts.size() * 4);
+  {
+    // k-cliques inside the reported set (exact numerator)
+    DBuf<uint8_t> mask(&A);
+    HIPCHK(mask.alloc(n));
+    HIPCHK(mask.zero());
+    std::vector<uint8_t> hm(n, 0);
+    for (uint32_t v : verts) hm[v] = 1;
+    CHK(upload(e, mask.p, hm.data(), n));
+    HIPCHK(count_cliques_in_mask(T, k, L.cliques.p, mask.p, acc.p, st));
+    unsigned long long sc = 0;
+    CHK(readback(e, &sc, acc.p, 1));
+    R->set_cliques = sc;
+  }
+  const size_t tl = tr_alpha.size();
+  R->trace_len = (uint32_t)tl;
+  R->trace_lb = host_alloc<float>(tl);
+  R->trace_ub = host_alloc<float>(tl);
+  R->trace_alpha = host_alloc<float>(tl);
+  R->trace_component = host_alloc<uint32_t>(tl);
+  R->trace_margin = host_alloc<double>(tl);
+  for (size_t i = 0; i < tl; ++i) {
+    R->trace_lb[i] = tr_lb[i];
+    R->trace_ub[i] = tr_ub[i];
+    R->trace_alpha[i] = tr_alpha[i];
+    R->trace_component[i] = tr_comp[i];
+    R->trace_margin[i] = tr_margin[i];
+  }
+  if (p->want_vertex_arrays) {
+    R->clique_degree = host_alloc<uint64_t>(n);
+    memcpy(R->clique_degree, cd.data(), n * 8);
+    R->core = host_alloc<uint32_t>(n);
+    memcpy(R->core, hcore.data(), n * 4);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  R->times.dag = tm.ms(ev_start, ev_dag);
+  R->times.listing = tm.ms(ev_dag, ev_list);
+  R->times.core_decompose = tm.ms(ev_list, ev_peel);
+  R->times.densest_core = tm.ms(ev_peel, ev_core);
+  R->times.prune = tm.ms(ev_core, ev_prune);
+  R->times.components = tm.ms(ev_prune, ev_cc);
+  R->times.exact = tm.ms(ev_cc, ev_exact);
+  R->times.total = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - wall0)
+                       .count();
+  R->peak_device_bytes = A.peak_bytes();
+  return CDS_OK;
+}
+
+}  // namespace
+
+extern "C" cds_status cds_solve(cds_engine *e, const cds_dgraph *g,
+                                const cds_params *p, cds_result *out) {
+  if (!e || !g || !p || !out) return fail(CDS_ERR_INVALID, "null argument");
+  if (p->k < 2 || p->k > 4)
+    return fail(CDS_ERR_UNSUPPORTED, "k=%u: the device lister covers k=2..4",
+                p->k);
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  cds_status s = solve_impl(e, g, p, out);
+  if (s != CDS_OK) {
+    (void)hipStreamSynchronize(e->stream);
+    cds_result_free(out);
+  }
+  return s;
+}
+
+extern "C" cds_status cds_solve_batch(cds_engine *e,
+                                      const cds_dgraph *const *graphs,
+                                      const cds_params *params, size_t count,
+                                      cds_result *results) {
+  if (!e || (!graphs && count) || (!params && count) || (!results && count))
+    return fail(CDS_ERR_INVALID, "null argument");
+  for (size_t i = 0; i < count; ++i) {
+    cds_status s = cds_solve(e, graphs[i], &params[i], &results[i]);
+    if (s != CDS_OK) {
+      for (size_t j = 0; j < i; ++j) cds_result_free(&results[j]);
+      return s;
+    }
+  }
+  return CDS_OK;
+}

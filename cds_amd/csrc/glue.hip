@@ -1,0 +1,284 @@
+// Densest-core extraction, edge pruning and connected components.
+//
+// Reference GPU path: generateDensestCore / generateNeighborDensestCore
+// (cudaCode/src/helpers.cu:821-896), pruneEdges + pruned CSR (:898-1041),
+// componentDecomposek label propagation (:937-978) and the per-component
+// clique bucketing getConnectedComponentStatus / rearrangeCliqueData
+// (:1043-1096).  Oracle: locateDensestCore, pruneInvalidEdges,
+// connectedComponentDecompose (cpuCode/c++/src/helpers.cpp:710-788,
+// :958-1116).
+//
+// The core's cliques are the listed cliques whose kill level (the minimum
+// core number of their members) is >= k', so no re-listing is needed.
+// Components of the pruned graph are the classes of "shares a core clique";
+// they come from a lock-free union-find that always hooks the larger root
+// under the smaller, so every root is its component's smallest vertex id,
+// which is the oracle's component order.
+#include "dev.h"
+#include "kernels.h"
+
+namespace cds {
+namespace {
+
+__global__ void k_core_flags(uint32_t n, const uint32_t *__restrict__ core,
+                             uint32_t kp, uint32_t *__restrict__ flag) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x)
+    flag[v] = core[v] >= kp;
+}
+
+__global__ __launch_bounds__(256) void k_core_edges(
+    uint32_t n, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ adj, const uint32_t *__restrict__ core,
+    uint32_t kp, unsigned long long *out) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  unsigned long long c = 0;
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       v < n; v += waves) {
+    if (core[v] < kp) continue;
+    for (uint32_t j = off[v] + lane(); j < off[v + 1]; j += kWave)
+      c += core[adj[j]] >= kp;
+  }
+  c = wave_sum64(c);
+  if (lane() == 0 && c) atomicAdd(out, c);
+}
+
+__global__ void k_count_kill_ge(uint64_t T, const uint32_t *__restrict__ kill,
+                                uint32_t kp, unsigned long long *out) {
+  unsigned long long c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c += kill[i] >= kp;
+  c = wave_sum64(c);
+  if (lane() == 0 && c) atomicAdd(out, c);
+}
+
+__global__ void k_uf_init(uint32_t n, uint32_t *__restrict__ p) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x)
+    p[v] = v;
+}
+
+__device__ __forceinline__ uint32_t uf_load(uint32_t *p, uint32_t x) {
+  return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving (benign races: every write stores an
+// ancestor of the slot's current value).
+__device__ uint32_t uf_find(uint32_t *p, uint32_t x) {
+  for (;;) {
+    const uint32_t y = uf_load(p, x);
+    if (y == x) return x;
+    const uint32_t z = uf_load(p, y);
+    if (z != y)
+      __hip_atomic_store(&p[x], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = y;
+  }
+}
+
+__device__ void uf_unite(uint32_t *p, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    // hook the larger root a under the smaller root b
+    if (atomicCAS(&p[a], a, b) == a) return;
+  }
+}
+
+__global__ void k_uf_union(uint64_t T, uint32_t k,
+                           const uint32_t *__restrict__ cl,
+                           const uint32_t *__restrict__ kill, uint32_t kp,
+                           uint32_t *__restrict__ p) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] < kp) continue;
+    const uint32_t a = cl[c * k];
+    for (uint32_t j = 1; j < k; ++j) uf_unite(p, a, cl[c * k + j]);
+  }
+}
+
+__global__ void k_uf_flatten(uint32_t n, uint32_t *__restrict__ p) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x)
+    p[v] = uf_find(p, v);
+}
+
+__global__ void k_prune_mark(uint64_t T, uint32_t k,
+                             const uint32_t *__restrict__ cl,
+                             const uint32_t *__restrict__ kill, uint32_t kp,
+                             const uint32_t *__restrict__ off,
+                             const uint32_t *__restrict__ adj,
+                             uint8_t *__restrict__ mark) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] < kp) continue;
+    for (uint32_t i = 0; i < k; ++i) {
+      const uint32_t a = cl[c * k + i];
+      const uint32_t ab = off[a], ad = off[a + 1] - ab;
+      for (uint32_t j = 0; j < k; ++j) {
+        if (i == j) continue;
+        const int64_t pos = find_sorted64(adj + ab, ad, cl[c * k + j]);
+        if (pos >= 0) mark[ab + pos] = 1;
+      }
+    }
+  }
+}
+
+__global__ void k_count_nonzero_u8(size_t n, const uint8_t *__restrict__ a,
+                                   unsigned long long *out) {
+  unsigned long long c = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    c += a[i] != 0;
+  c = wave_sum64(c);
+  if (lane() == 0 && c) atomicAdd(out, c);
+}
+
+__global__ void k_comp_count(uint64_t T, uint32_t k,
+                             const uint32_t *__restrict__ cl,
+                             const uint32_t *__restrict__ kill, uint32_t kp,
+                             const uint32_t *__restrict__ comp_of,
+                             const uint32_t *__restrict__ fv_of,
+                             uint32_t *__restrict__ ccount,
+                             uint32_t *__restrict__ fcd) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] < kp) continue;
+    atomicAdd(&ccount[comp_of[cl[c * k]]], 1u);
+    for (uint32_t j = 0; j < k; ++j) atomicAdd(&fcd[fv_of[cl[c * k + j]]], 1u);
+  }
+}
+
+__global__ void k_comp_scatter(uint64_t T, uint32_t k,
+                               const uint32_t *__restrict__ cl,
+                               const uint32_t *__restrict__ kill, uint32_t kp,
+                               const uint32_t *__restrict__ comp_of,
+                               const uint32_t *__restrict__ fv_of,
+                               const uint64_t *__restrict__ coff,
+                               uint32_t *__restrict__ ccur,
+                               uint32_t *__restrict__ fmem) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] < kp) continue;
+    const uint32_t comp = comp_of[cl[c * k]];
+    const uint64_t slot = coff[comp] + atomicAdd(&ccur[comp], 1u);
+    for (uint32_t j = 0; j < k; ++j) fmem[slot * k + j] = fv_of[cl[c * k + j]];
+  }
+}
+
+__global__ void k_flow_inc(uint64_t TT, uint32_t k,
+                           const uint32_t *__restrict__ fmem,
+                           const uint64_t *__restrict__ fioff,
+                           uint32_t *__restrict__ ficur,
+                           uint32_t *__restrict__ finc) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       s < TT * k; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = fmem[s];
+    finc[fioff[v] + atomicAdd(&ficur[v], 1u)] = (uint32_t)s;
+  }
+}
+
+}  // namespace
+
+hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
+                      uint32_t *flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_core_flags, dim3(grid_for(n, 256, 65535)), dim3(256), 0,
+                     s, n, core, kp, flag);
+  return hipGetLastError();
+}
+
+hipError_t core_edge_count(uint32_t n, const uint32_t *off,
+                           const uint32_t *adj, const uint32_t *core,
+                           uint32_t kp, unsigned long long *out,
+                           hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_core_edges, dim3(grid_for((size_t)n * kWave, 256, 16384)),
+                     dim3(256), 0, s, n, off, adj, core, kp, out);
+  return hipGetLastError();
+}
+
+hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
+                         unsigned long long *out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_count_kill_ge, dim3(grid_for(T, 256, 8192)), dim3(256),
+                     0, s, T, kill, kp, out);
+  return hipGetLastError();
+}
+
+hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s) {
+  hipLaunchKernelGGL(k_uf_init, dim3(grid_for(n, 256, 65535)), dim3(256), 0, s,
+                     n, parent);
+  return hipGetLastError();
+}
+
+hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *kill, uint32_t kp,
+                            uint32_t *parent, hipStream_t s) {
+  hipLaunchKernelGGL(k_uf_union, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
+                     s, T, k, cliques, kill, kp, parent);
+  return hipGetLastError();
+}
+
+hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s) {
+  hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 65535)), dim3(256), 0,
+                     s, n, parent);
+  return hipGetLastError();
+}
+
+hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
+                      const uint32_t *kill, uint32_t kp, const uint32_t *off,
+                      const uint32_t *adj, uint8_t *mark, hipStream_t s) {
+  hipLaunchKernelGGL(k_prune_mark, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
+                     s, T, k, cliques, kill, kp, off, adj, mark);
+  return hipGetLastError();
+}
+
+hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
+                            unsigned long long *out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_count_nonzero_u8, dim3(grid_for(n, 256, 8192)),
+                     dim3(256), 0, s, n, a, out);
+  return hipGetLastError();
+}
+
+hipError_t comp_clique_count(uint64_t T, uint32_t k, const uint32_t *cliques,
+                             const uint32_t *kill, uint32_t kp,
+                             const uint32_t *comp_of, const uint32_t *fv_of,
+                             uint32_t *comp_cliques, uint32_t *fcd,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_comp_count, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
+                     s, T, k, cliques, kill, kp, comp_of, fv_of, comp_cliques,
+                     fcd);
+  return hipGetLastError();
+}
+
+hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
+                               const uint32_t *cliques, const uint32_t *kill,
+                               uint32_t kp, const uint32_t *comp_of,
+                               const uint32_t *fv_of, const uint64_t *coff,
+                               uint32_t *ccur, uint32_t *fmem,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_comp_scatter, dim3(grid_for(T, 256, 65535)), dim3(256),
+                     0, s, T, k, cliques, kill, kp, comp_of, fv_of, coff, ccur,
+                     fmem);
+  return hipGetLastError();
+}
+
+hipError_t flow_incidence(uint64_t TT, uint32_t k, const uint32_t *fmem,
+                          const uint64_t *fioff, uint32_t *ficur,
+                          uint32_t *finc, hipStream_t s) {
+  hipLaunchKernelGGL(k_flow_inc, dim3(grid_for(TT * k, 256, 65535)), dim3(256),
+                     0, s, TT, k, fmem, fioff, ficur, finc);
+  return hipGetLastError();
+}
+
+}  // namespace cds

@@ -78,6 +78,16 @@ using namespace cds;
 extern "C" const char *cds_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char *cds_version(void) { return "cds-mi355x 0.1 (gfx950)"; }
 
+extern "C" size_t cds_struct_size(int which) {
+  switch (which) {
+    case 0: return sizeof(cds_graph);
+    case 1: return sizeof(cds_params);
+    case 2: return sizeof(cds_result);
+    case 3: return sizeof(cds_opts);
+    default: return 0;
+  }
+}
+
 extern "C" void cds_graph_free(cds_graph *g) {
   if (!g) return;
   free(g->offsets);

@@ -1,0 +1,184 @@
+// Host-callable launchers of the HIP kernels (implemented in *.hip).
+// Every launcher enqueues work on `s` and returns the launch status; none
+// synchronises unless its comment says so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace cds {
+
+constexpr uint32_t kAliveHost = 0xFFFFFFFFu;  // same value as kAlive (dev.h)
+
+// Stream-ordered device allocations (hipMallocAsync on the engine's pool).
+class DeviceArena {
+ public:
+  explicit DeviceArena(hipStream_t s) : stream_(s) {}
+  hipError_t alloc(void **p, size_t bytes);
+  void release(void *p);
+  size_t live_bytes() const { return live_; }
+  size_t peak_bytes() const { return peak_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  hipStream_t stream_;
+  size_t live_ = 0, peak_ = 0;
+  std::vector<std::pair<void *, size_t>> sizes_;
+};
+
+// ---- scan / reduce (scan.hip) ----
+// out[0..n]: exclusive prefix sums of in[0..n), out[n] = total.
+hipError_t scan_u32(const uint32_t *in, uint64_t *out, size_t n,
+                    DeviceArena &a);
+hipError_t scan_u64(const uint64_t *in, uint64_t *out, size_t n,
+                    DeviceArena &a);
+// Same with 32-bit output (caller guarantees total < 2^32).
+hipError_t scan_u32_32(const uint32_t *in, uint32_t *out, size_t n,
+                       DeviceArena &a);
+
+// ---- DAG (dag.hip) ----
+// Degree-ordered orientation: u -> v iff (deg u, u) < (deg v, v).
+hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                     uint32_t *dcnt, hipStream_t s);
+hipError_t dag_fill(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                    const uint32_t *doff, uint32_t *dadj, hipStream_t s);
+// max over dcnt (result written to *out, device)
+hipError_t reduce_max_u32(const uint32_t *in, size_t n, uint32_t *out,
+                          hipStream_t s);
+
+// ---- listing (list.hip) ----
+// Work items: (u, first position in N+(u)); each covers up to
+// kListChunk out-neighbours of u.
+constexpr uint32_t kListChunk = 64;
+// N+(u) is staged in LDS up to this many entries (k = 3: list + counters,
+// 64000 B; larger lists are searched in global memory).  k = 4 needs three
+// LDS arrays and requires every out-degree to fit.
+constexpr uint32_t kListLdsCap = 8000;
+constexpr uint32_t kList4LdsCap = 4000;
+hipError_t list_item_counts(uint32_t n, const uint32_t *doff,
+                            uint32_t *items_per_vertex, hipStream_t s);
+hipError_t list_item_fill(uint32_t n, const uint32_t *doff,
+                          const uint64_t *item_off, uint2 *items,
+                          hipStream_t s);
+// k = 3 or 4.  Count mode (out == nullptr): item_count[i] = cliques found by
+// item i, clique_degree += per-vertex contributions.  Write mode: cliques
+// written AoS (k ids each) at out + k * item_base[i].
+hipError_t list_cliques(uint32_t k, uint64_t num_items, const uint2 *items,
+                        const uint32_t *doff, const uint32_t *dadj,
+                        uint32_t max_out_degree, uint32_t *item_count,
+                        uint64_t *clique_degree, const uint64_t *item_base,
+                        uint32_t *out, hipStream_t s);
+// k = 2: every DAG arc is a clique.
+hipError_t list_edges(uint32_t n, const uint32_t *doff, const uint32_t *dadj,
+                      uint64_t *clique_degree, uint32_t *out, hipStream_t s);
+
+// ---- peel (peel.hip) ----
+hipError_t build_incidence(uint64_t T, uint32_t k, const uint32_t *cliques,
+                           const uint64_t *ioff, uint32_t *icur,
+                           uint32_t *inc, hipStream_t s);
+hipError_t peel_init(uint32_t n, uint64_t T, const uint64_t *cd,
+                     uint32_t *deg, uint32_t *core, uint32_t *kill,
+                     hipStream_t s);
+// *out = min deg over vertices with core == kAlive (kAlive if none).
+hipError_t peel_min_alive(uint32_t n, const uint32_t *deg,
+                          const uint32_t *core, uint32_t *out, hipStream_t s);
+// Appends alive vertices with deg <= level to frontier, sets core = level.
+hipError_t peel_frontier(uint32_t n, uint32_t level, const uint32_t *deg,
+                         uint32_t *core, uint32_t *frontier,
+                         uint32_t *count, hipStream_t s);
+// Processes a frontier: kills alive cliques, decrements members, appends
+// vertices that drop to `level` to next; counts killed cliques.
+hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
+                        const uint32_t *fcount, uint32_t fmax,
+                        const uint64_t *ioff, const uint32_t *inc,
+                        const uint32_t *cliques, uint32_t *deg,
+                        uint32_t *core, uint32_t *kill, uint32_t *next,
+                        uint32_t *ncount, unsigned long long *killed,
+                        hipStream_t s);
+// Exact sequential replay of one peel phase (level `level`); see peel.hip.
+struct ReplayOut {
+  unsigned long long best_num, best_den;  // max intermediate density
+  uint32_t steps, found;
+};
+hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
+                        const uint32_t *core, const uint32_t *kill,
+                        const uint64_t *ioff, const uint32_t *inc,
+                        const uint32_t *cliques, uint64_t T_start,
+                        uint64_t N_start, uint32_t M, uint32_t *pv,
+                        uint32_t *pdeg, uint32_t *posmap, uint8_t *palive,
+                        uint32_t *rdead, uint32_t *pcount, ReplayOut *out,
+                        hipStream_t s);
+
+// ---- core / prune / components (glue.hip) ----
+hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
+                      uint32_t *flag, hipStream_t s);
+hipError_t core_edge_count(uint32_t n, const uint32_t *off,
+                           const uint32_t *adj, const uint32_t *core,
+                           uint32_t kp, unsigned long long *out,
+                           hipStream_t s);
+hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
+                         unsigned long long *out, hipStream_t s);
+hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s);
+hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *kill, uint32_t kp,
+                            uint32_t *parent, hipStream_t s);
+hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s);
+hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
+                      const uint32_t *kill, uint32_t kp, const uint32_t *off,
+                      const uint32_t *adj, uint8_t *mark, hipStream_t s);
+hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
+                            unsigned long long *out, hipStream_t s);
+// Per-component clique bucketing.  comp_of / local_of indexed by original
+// vertex id (valid for core vertices).
+hipError_t comp_clique_count(uint64_t T, uint32_t k, const uint32_t *cliques,
+                             const uint32_t *kill, uint32_t kp,
+                             const uint32_t *comp_of, const uint32_t *fv_of,
+                             uint32_t *comp_cliques, uint32_t *fcd,
+                             hipStream_t s);
+hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
+                               const uint32_t *cliques, const uint32_t *kill,
+                               uint32_t kp, const uint32_t *comp_of,
+                               const uint32_t *fv_of, const uint64_t *coff,
+                               uint32_t *ccur, uint32_t *fmem,
+                               hipStream_t s);
+hipError_t flow_incidence(uint64_t TT, uint32_t k, const uint32_t *fmem,
+                          const uint64_t *fioff, uint32_t *ficur,
+                          uint32_t *finc, hipStream_t s);
+
+// ---- max flow (flow.hip) ----
+struct FlowProblem {
+  uint32_t k, V, T;       // vertices, cliques of this component
+  uint32_t vbase;         // flow-vertex id of local vertex 0
+  uint64_t sbase;         // slot id of local slot 0 (first clique * k)
+  const uint32_t *mem;    // T*k member flow-vertex ids (subtract vbase)
+  const uint64_t *ioff;   // V+1 offsets into inc, for local vertices
+  const uint32_t *inc;    // slot ids (subtract sbase) of each vertex
+  const uint32_t *cd;     // V clique degrees
+};
+struct FlowState {
+  long long *x;       // T*k: flow vertex->clique on slot (reversed network)
+  long long *rvs;     // V: residual vertex->sink
+  long long *excess;  // V+T
+  int *h;             // V+T heights
+  long long *dtgt, *damt;     // V+T per-round decisions
+  uint32_t *queue0, *queue1;  // V+T BFS queues
+  unsigned long long *misc;   // [0]=F, [1]=rounds, [2]=limit hit, [3]=BFS
+  int *reach;                 // V: 1 if on the source side of the min cut
+};
+// Max flow of one component at scaled sink capacity `cap_t` and unit `U`
+// (all capacities multiplied by U).  Writes misc[0] = F (scaled), reach[].
+hipError_t max_flow(const FlowProblem &P, const FlowState &S, long long cap_t,
+                    long long unit, uint32_t max_rounds, hipStream_t s);
+
+// Cliques of a component whose members all have reach != 0.
+hipError_t count_cliques_in_reach(uint32_t T, uint32_t k, const uint32_t *mem,
+                                  uint32_t vbase, const int *reach,
+                                  unsigned long long *out, hipStream_t s);
+// Cliques (AoS, k ids) whose members all have mask[id] != 0.
+hipError_t count_cliques_in_mask(uint64_t T, uint32_t k, const uint32_t *cl,
+                                 const uint8_t *mask, unsigned long long *out,
+                                 hipStream_t s);
+
+}  // namespace cds

@@ -1,0 +1,319 @@
+// (a) k-clique listing over the degree-ordered DAG.
+//
+// Reference: listAllCliques (cudaCode/main.cu:59-250) with
+// listIntialCliques / listMidCliques / flushParitions / countCliques /
+// writeFinalCliques (cudaCode/src/helpers.cu:175-574); oracle
+// listCliques (cpuCode/c++/src/helpers.cpp:373-511).  Every k-clique is
+// found exactly once, from its lowest-ranked vertex u, as u < v < w (< x)
+// along DAG arcs.
+//
+// Work decomposition: an item is (u, j0): up to kListChunk consecutive
+// out-neighbours v of u, so a hub's out-list is spread over many
+// workgroups.  One 256-thread workgroup per item stages the sorted N+(u)
+// in LDS and flattens the item's work sum_v d+(v) across its 256 lanes
+// (coalesced reads of each N+(v)); membership of w in N+(u) is a binary
+// search in LDS that also yields w's position, so per-vertex clique
+// degrees are accumulated in LDS counters indexed by position and flushed
+// with one global atomic per touched vertex.  k = 4 adds an LDS bitset of
+// P = N+(u) & N+(v) over N+(u)'s positions and probes each N+(w), w in P.
+//
+// Two passes: count (per-item totals + clique degrees) and write (cliques
+// stored AoS, k ids each, at the item's scanned base).
+#include "dev.h"
+#include "kernels.h"
+
+namespace cds {
+namespace {
+
+constexpr int kListThreads = 256;
+constexpr int kListWaves = kListThreads / kWave;
+
+__global__ void k_item_counts(uint32_t n, const uint32_t *__restrict__ doff,
+                              uint32_t *__restrict__ ipv) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x) {
+    const uint32_t d = doff[v + 1] - doff[v];
+    ipv[v] = (d + kListChunk - 1) / kListChunk;
+  }
+}
+
+__global__ void k_item_fill(uint32_t n, const uint32_t *__restrict__ doff,
+                            const uint64_t *__restrict__ ioff,
+                            uint2 *__restrict__ items) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x) {
+    const uint32_t d = doff[v + 1] - doff[v];
+    const uint32_t nb = (d + kListChunk - 1) / kListChunk;
+    const uint64_t base = ioff[v];
+    for (uint32_t t = 0; t < nb; ++t)
+      items[base + t] = make_uint2(v, t * kListChunk);
+  }
+}
+
+struct ChunkTable {
+  uint32_t pre[kListChunk + 1];  // prefix sums of d+(v) over the chunk
+  uint32_t vid[kListChunk];
+  uint32_t vbase[kListChunk];
+  uint32_t wave_found[kListWaves];
+  uint32_t cursor;
+};
+
+// First wave: load the chunk's v ids and scan their out-degrees.
+__device__ __forceinline__ void load_chunk(ChunkTable &t, const uint32_t *dadj,
+                                           const uint32_t *doff, uint32_t ub,
+                                           uint32_t j0, uint32_t jn) {
+  if (threadIdx.x < kWave) {
+    uint32_t dv = 0;
+    if (threadIdx.x < jn) {
+      const uint32_t v = dadj[ub + j0 + threadIdx.x];
+      const uint32_t vb = doff[v];
+      dv = doff[v + 1] - vb;
+      t.vid[threadIdx.x] = v;
+      t.vbase[threadIdx.x] = vb;
+    }
+    uint32_t incl = dv;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      uint32_t x = __shfl_up(incl, off, kWave);
+      if ((int)lane() >= off) incl += x;
+    }
+    if (threadIdx.x < jn) t.pre[threadIdx.x + 1] = incl;
+    if (threadIdx.x == 0) {
+      t.pre[0] = 0;
+      t.cursor = 0;
+    }
+  }
+}
+
+// Finishes an item in count mode: item total, cd[u], flush LDS counters.
+__device__ __forceinline__ void finish_count(
+    ChunkTable &t, uint32_t found, uint64_t it, uint32_t u, bool staged,
+    const uint32_t *su, const uint32_t *scnt, uint32_t du,
+    uint32_t *item_count, unsigned long long *cd) {
+  found = wave_sum(found);
+  if (lane() == 0) t.wave_found[threadIdx.x / kWave] = found;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kListWaves; ++w) tot += t.wave_found[w];
+    item_count[it] = tot;
+    if (tot) atomicAdd(&cd[u], (unsigned long long)tot);
+  }
+  if (staged)
+    for (uint32_t i = threadIdx.x; i < du; i += blockDim.x) {
+      const uint32_t c = scnt[i];
+      if (c) atomicAdd(&cd[su[i]], (unsigned long long)c);
+    }
+}
+
+__global__ __launch_bounds__(kListThreads) void k_list3(
+    uint64_t num_items, const uint2 *__restrict__ items,
+    const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dadj,
+    uint32_t lds_cap, uint32_t *__restrict__ item_count,
+    unsigned long long *__restrict__ cd, const uint64_t *__restrict__ item_base,
+    uint32_t *__restrict__ out) {
+  extern __shared__ uint32_t dyn[];
+  uint32_t *su = dyn;             // N+(u), sorted
+  uint32_t *scnt = dyn + lds_cap; // clique-degree counters by position
+  __shared__ ChunkTable t;
+  const bool write = out != nullptr;
+  for (uint64_t it = blockIdx.x; it < num_items; it += gridDim.x) {
+    const uint2 item = items[it];
+    const uint32_t u = item.x, j0 = item.y;
+    const uint32_t ub = doff[u], du = doff[u + 1] - ub;
+    const uint32_t jn = min(du - j0, (uint32_t)kListChunk);
+    const bool staged = du <= lds_cap;
+    if (staged)
+      for (uint32_t i = threadIdx.x; i < du; i += blockDim.x) {
+        su[i] = dadj[ub + i];
+        scnt[i] = 0;
+      }
+    load_chunk(t, dadj, doff, ub, j0, jn);
+    __syncthreads();
+    const uint32_t total = t.pre[jn];
+    const uint32_t *nu = staged ? su : dadj + ub;
+    const uint64_t obase = write ? item_base[it] : 0;
+    uint32_t found = 0;
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+      int lo = 0, hi = (int)jn - 1;  // last vi with pre[vi] <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.pre[mid] <= i)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const uint32_t w = dadj[t.vbase[lo] + (i - t.pre[lo])];
+      const int p = find_sorted(nu, (int)du, w);
+      if (p < 0) continue;
+      ++found;
+      if (write) {
+        const uint32_t slot = atomicAdd(&t.cursor, 1u);
+        uint32_t *o = out + 3ull * (obase + slot);
+        o[0] = u;
+        o[1] = t.vid[lo];
+        o[2] = w;
+      } else if (staged) {
+        atomicAdd(&scnt[p], 1u);
+        atomicAdd(&scnt[j0 + lo], 1u);
+      } else {
+        atomicAdd(&cd[w], 1ull);
+        atomicAdd(&cd[t.vid[lo]], 1ull);
+      }
+    }
+    if (!write) finish_count(t, found, it, u, staged, su, scnt, du, item_count, cd);
+    __syncthreads();
+  }
+}
+
+// k = 4.  Requires du <= lds_cap for every u (checked by the host).
+__global__ __launch_bounds__(kListThreads) void k_list4(
+    uint64_t num_items, const uint2 *__restrict__ items,
+    const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dadj,
+    uint32_t lds_cap, uint32_t *__restrict__ item_count,
+    unsigned long long *__restrict__ cd, const uint64_t *__restrict__ item_base,
+    uint32_t *__restrict__ out) {
+  extern __shared__ uint32_t dyn[];
+  uint32_t *su = dyn;
+  uint32_t *scnt = dyn + lds_cap;
+  uint32_t *sbits = dyn + 2 * lds_cap;               // lds_cap/32 + 1 words
+  uint32_t *plist = sbits + (lds_cap / 32 + 1);      // lds_cap
+  __shared__ ChunkTable t;
+  __shared__ uint32_t s_np;
+  const bool write = out != nullptr;
+  const uint32_t wid = threadIdx.x / kWave;
+  for (uint64_t it = blockIdx.x; it < num_items; it += gridDim.x) {
+    const uint2 item = items[it];
+    const uint32_t u = item.x, j0 = item.y;
+    const uint32_t ub = doff[u], du = doff[u + 1] - ub;
+    const uint32_t jn = min(du - j0, (uint32_t)kListChunk);
+    for (uint32_t i = threadIdx.x; i < du; i += blockDim.x) {
+      su[i] = dadj[ub + i];
+      scnt[i] = 0;
+    }
+    if (threadIdx.x == 0) t.cursor = 0;
+    __syncthreads();
+    const uint64_t obase = write ? item_base[it] : 0;
+    const uint32_t nwords = (du + 31) / 32;
+    uint32_t found = 0;
+    for (uint32_t vi = 0; vi < jn; ++vi) {
+      const uint32_t pv = j0 + vi;
+      const uint32_t v = su[pv];
+      const uint32_t vb = doff[v], dv = doff[v + 1] - vb;
+      for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) sbits[i] = 0;
+      if (threadIdx.x == 0) s_np = 0;
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < dv; i += blockDim.x) {
+        const int p = find_sorted(su, (int)du, dadj[vb + i]);
+        if (p >= 0) {
+          atomicOr(&sbits[p >> 5], 1u << (p & 31));
+          plist[atomicAdd(&s_np, 1u)] = (uint32_t)p;
+        }
+      }
+      __syncthreads();
+      const uint32_t np = s_np;
+      for (uint32_t q = wid; q < np; q += kListWaves) {
+        const uint32_t pw = plist[q];
+        const uint32_t w = su[pw];
+        const uint32_t wb = doff[w], dw = doff[w + 1] - wb;
+        for (uint32_t r = lane(); r < dw; r += kWave) {
+          const uint32_t x = dadj[wb + r];
+          const int px = find_sorted(su, (int)du, x);
+          if (px < 0 || !((sbits[px >> 5] >> (px & 31)) & 1u)) continue;
+          ++found;
+          if (write) {
+            const uint32_t slot = atomicAdd(&t.cursor, 1u);
+            uint32_t *o = out + 4ull * (obase + slot);
+            o[0] = u;
+            o[1] = v;
+            o[2] = w;
+            o[3] = x;
+          } else {
+            atomicAdd(&scnt[px], 1u);
+            atomicAdd(&scnt[pw], 1u);
+            atomicAdd(&scnt[pv], 1u);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!write) finish_count(t, found, it, u, true, su, scnt, du, item_count, cd);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_list2(
+    uint32_t n, const uint32_t *__restrict__ doff,
+    const uint32_t *__restrict__ dadj, unsigned long long *__restrict__ cd,
+    uint32_t *__restrict__ out) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       u < n; u += waves) {
+    const uint32_t b = doff[u], e = doff[u + 1];
+    for (uint32_t j = b + lane(); j < e; j += kWave) {
+      const uint32_t v = dadj[j];
+      if (out) {
+        out[2ull * j] = u;
+        out[2ull * j + 1] = v;
+      } else {
+        atomicAdd(&cd[v], 1ull);
+      }
+    }
+    if (!out && lane() == 0 && e > b) atomicAdd(&cd[u], (unsigned long long)(e - b));
+  }
+}
+
+}  // namespace
+
+hipError_t list_item_counts(uint32_t n, const uint32_t *doff,
+                            uint32_t *items_per_vertex, hipStream_t s) {
+  hipLaunchKernelGGL(k_item_counts, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
+                     s, n, doff, items_per_vertex);
+  return hipGetLastError();
+}
+
+hipError_t list_item_fill(uint32_t n, const uint32_t *doff,
+                          const uint64_t *item_off, uint2 *items,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
+                     s, n, doff, item_off, items);
+  return hipGetLastError();
+}
+
+hipError_t list_cliques(uint32_t k, uint64_t num_items, const uint2 *items,
+                        const uint32_t *doff, const uint32_t *dadj,
+                        uint32_t max_out_degree, uint32_t *item_count,
+                        uint64_t *clique_degree, const uint64_t *item_base,
+                        uint32_t *out, hipStream_t s) {
+  if (num_items == 0) return hipSuccess;
+  const unsigned grid = (unsigned)(num_items < 16384 ? num_items : 16384);
+  auto *cd = reinterpret_cast<unsigned long long *>(clique_degree);
+  if (k == 3) {
+    const uint32_t cap = max_out_degree < kListLdsCap ? max_out_degree : kListLdsCap;
+    const size_t lds = 2ull * (cap ? cap : 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_list3, dim3(grid), dim3(kListThreads), lds, s,
+                       num_items, items, doff, dadj, cap ? cap : 1, item_count,
+                       cd, item_base, out);
+    return hipGetLastError();
+  }
+  if (k == 4) {
+    const uint32_t cap = max_out_degree ? max_out_degree : 1;
+    if (cap > kList4LdsCap) return hipErrorInvalidValue;
+    const size_t lds = (3ull * cap + cap / 32 + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_list4, dim3(grid), dim3(kListThreads), lds, s,
+                       num_items, items, doff, dadj, cap, item_count, cd,
+                       item_base, out);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t list_edges(uint32_t n, const uint32_t *doff, const uint32_t *dadj,
+                      uint64_t *clique_degree, uint32_t *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_list2, dim3(grid_for((size_t)n * kWave, 256, 65535)),
+                     dim3(256), 0, s, n, doff, dadj,
+                     reinterpret_cast<unsigned long long *>(clique_degree), out);
+  return hipGetLastError();
+}
+
+}  // namespace cds

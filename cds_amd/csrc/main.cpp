@@ -1,0 +1,99 @@
+// cds_main: drop-in for the reference GPU binary.
+//
+//   cds_main <graph> <k> <pSize> <cpSize> <glBufferSize> <partitionSize> <earlyStop>
+//
+// Same argv (cudaCode/main.cu:865-887), same input format
+// (cudaCode/src/graph.cpp:7-40) and the same report keys
+// (cudaCode/src/graph.cpp:39, :59; cudaCode/main.cu:861, :994-995,
+// :1069-1104), followed by the result block of the reference's CPU oracle
+// (cpuCode/c++/src/helpers.cpp:1580-1588: K, Density, Size, densest
+// subgraph) and one exact line `Density_exact=<cliques>/<size>`.
+// pSize, cpSize, glBufferSize and partitionSize are capacity hints of the
+// reference's fixed partitions; this engine sizes its buffers from the
+// graph, so they are accepted and validated but change nothing.  earlyStop
+// must be -1 (exact search); other values are rejected.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/cds.h"
+
+static int die(const char *what) {
+  fprintf(stderr, "cds_main: %s: %s\n", what, cds_last_error());
+  return 2;
+}
+
+int main(int argc, const char *argv[]) {
+  if (argc != 8) {
+    printf("Server wrong input parameters!\n");
+    return 1;
+  }
+  cds_params p;
+  memset(&p, 0, sizeof p);
+  p.k = (uint32_t)atoi(argv[2]);
+  p.p_size = (uint32_t)atoi(argv[3]);
+  p.cp_size = (uint32_t)atoi(argv[4]);
+  p.gl_buffer_size = (uint32_t)atoi(argv[5]);
+  p.partition_size = (uint32_t)atoi(argv[6]);
+  p.early_stop = atoi(argv[7]);
+  p.ub1 = 1;
+  p.ub2 = 1;
+  if (p.early_stop != -1) {
+    fprintf(stderr, "cds_main: earlyStop=%d: only -1 (exact) is supported\n",
+            p.early_stop);
+    return 1;
+  }
+  cds_graph g;
+  memset(&g, 0, sizeof g);
+  if (cds_graph_load_adjlist(argv[1], &g) != CDS_OK) return die("load");
+  printf("n=%u, m=%llu\n", g.n,
+         (unsigned long long)(g.m_header ? g.m_header : g.m));
+  printf("Max Degree=%u\n", g.max_degree);
+  cds_engine *e = nullptr;
+  if (cds_engine_create(nullptr, &e) != CDS_OK) return die("engine");
+  cds_dgraph *dg = nullptr;
+  if (cds_graph_upload(e, &g, &dg) != CDS_OK) return die("upload");
+  cds_result r;
+  if (cds_solve(e, dg, &p, &r) != CDS_OK) return die("solve");
+  if (r.total_cliques == 0) {
+    printf("Max Density 0 Size 0\n");
+    printf("Time taken: %g ms\n", r.times.total);
+  } else {
+    printf("Final Max Density %g\n", (double)r.density);
+    printf("Total Cliques=%llu\n", (unsigned long long)r.total_cliques);
+    printf("Densest Core Density=%g\n", r.peel_max_density);
+    printf("Total Components=%u\n", r.num_components);
+    printf("\n");
+    printf("DAG Time =%g ms\n", r.times.dag);
+    printf("Clique Listing Time =%g ms\n", r.times.listing);
+    printf("Core Decompose Time =%g ms\n", r.times.core_decompose);
+    printf("Densest Core Time =%g ms\n", r.times.densest_core);
+    printf("Prune Time =%g ms\n", r.times.prune);
+    printf("Connected Component Time =%g ms\n", r.times.components);
+    printf("Exact Time =%g ms\n", r.times.exact);
+    printf("Time taken=%g ms\n", r.times.total);
+    printf("\n");
+    printf("Mac Clique Core=%llu\n", (unsigned long long)r.kmax);
+    printf("lb=");
+    for (uint32_t i = 0; i < r.trace_len; ++i) printf("%g ", (double)r.trace_lb[i]);
+    printf("\nub=");
+    for (uint32_t i = 0; i < r.trace_len; ++i) printf("%g ", (double)r.trace_ub[i]);
+    printf("\neub=");
+    if (p.ub2) printf("%g ", (double)r.ub2_val);
+    printf("\n");
+  }
+  printf("K: %u\n", r.k);
+  printf("Density: %g\n", (double)r.density);
+  printf("Size: %u\n", r.size);
+  printf("densest subgraph: ");
+  for (uint32_t i = 0; i < r.set_size; ++i) printf("%u ", r.vertices[i]);
+  printf("\n");
+  printf("Density_exact=%llu/%u\n", (unsigned long long)r.set_cliques,
+         r.set_size);
+  cds_result_free(&r);
+  cds_dgraph_free(e, dg);
+  cds_engine_destroy(e);
+  cds_graph_free(&g);
+  return 0;
+}

@@ -41,7 +41,8 @@ typedef enum {
   CDS_ERR_NOT_CONVERGED = 6, /* an iteration bound was hit (never a hang) */
   CDS_ERR_UNSUPPORTED = 7,
   CDS_ERR_BUSY = 8,      /* serving: no slot / budget right now */
-  CDS_ERR_CANCELLED = 9
+  CDS_ERR_CANCELLED = 9,
+  CDS_ERR_INTERNAL = 10  /* an internal invariant failed (a bug) */
 } cds_status;
 
 /* Host CSR graph: undirected, symmetric, no self loops.
@@ -116,7 +117,15 @@ typedef struct {
   uint64_t flow_sweeps;             /* push-relabel sweeps, all alphas */
   uint32_t peel_levels;             /* level-synchronous peel rounds */
   uint32_t replay_phases;           /* phases replayed in exact order */
-  uint32_t oracle_quirk_flags;      /* bit0: sentinel 0xFFFFFF regime */
+  uint32_t oracle_quirk_flags;      /* bit2: oracle's int clique total wraps */
+  /* listing statistics (algorithmic-bytes inputs) */
+  uint64_t dag_arcs;                /* m */
+  uint64_t list_work;               /* W = sum over DAG arcs (u,v) of d+(v) */
+  uint32_t max_out_degree;          /* max d+ */
+  uint64_t list_items;              /* listing work items */
+  double t_list_count_ms;           /* listing count kernel (HIP events) */
+  double t_list_write_ms;           /* listing write kernel (HIP events) */
+  uint64_t peak_device_bytes;       /* engine arena peak during the solve */
   cds_times times;
 } cds_result;
 
@@ -131,6 +140,9 @@ typedef struct cds_dgraph cds_dgraph; /* device-resident graph */
 
 const char *cds_last_error(void);
 const char *cds_version(void);
+/* sizeof of the ABI structs (0 cds_graph, 1 cds_params, 2 cds_result,
+ * 3 cds_opts) so foreign bindings can verify their layouts. */
+size_t cds_struct_size(int which);
 
 /* ---- graph I/O (replaces Graph::Graph, cudaCode/src/graph.cpp:7-40) ---- */
 cds_status cds_graph_load_adjlist(const char *path, cds_graph *out);
