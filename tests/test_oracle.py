@@ -1,0 +1,245 @@
+"""CPU tests that pin the oracle (oracle/cds_oracle.cpp).
+
+What pins what:
+  * the reference's own pipeline: committed fixtures produced by
+    cpuCode/c++/src/*.cpp (tests/golden/reference_outputs.json), and live
+    runs of oracle/_ref/dsd_harness when it is built;
+  * closed forms and the reference's known answers (ref_data_test.txt,
+    the graph logged in testnew.log, the notes.ipynb example);
+  * brute force on tiny graphs: clique enumeration, the sequential
+    min-(degree, id) removal, the densest subgraph over all subsets, and the
+    max flow of the vertex/clique network over all vertex subsets;
+  * the two independent implementations inside the oracle
+    (PEEL_FAITHFUL vs PEEL_FAST, FLOW_FLOAT_EK vs FLOW_EXACT).
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cds_amd
+import oracle_ref as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+with open(os.path.join(GOLDEN, "reference_outputs.json")) as f:
+    FIXTURES = json.load(f)
+
+FIELDS = ["total_cliques", "clique_degree", "core", "k_prime", "kmax",
+          "core_density", "core_size", "core_edges", "core_cliques",
+          "deleted_cliques", "pruned_edges", "num_components", "component_size",
+          "component_cliques", "paper_lower_bound", "ub1", "ub2", "density",
+          "size", "vertices"]
+
+
+def same(a, b):
+    if isinstance(a, list):
+        return len(a) == len(b) and all(same(x, y) for x, y in zip(a, b))
+    if isinstance(a, float) or isinstance(b, float):
+        if math.isnan(a) or math.isnan(b):
+            return math.isnan(a) and math.isnan(b)
+        return float(a) == float(b)
+    return a == b
+
+
+@pytest.mark.parametrize("key", sorted(FIXTURES))
+@pytest.mark.parametrize("peel", [O.PEEL_FAITHFUL, O.PEEL_FAST])
+def test_restatement_matches_reference_fixtures(key, peel):
+    fx = FIXTURES[key]
+    g = cds_amd.Graph.load(os.path.join(GOLDEN, fx["graph"]))
+    got = O.oracle_solve(g, fx["k"], peel=peel, flow=O.FLOW_FLOAT_EK)
+    bad = [f for f in FIELDS if not same(got[f], fx[f])]
+    assert not bad, f"{key}: fields differ from the reference: {bad}"
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", range(8))
+def test_restatement_matches_reference_live(tmp_path, seed):
+    rng = np.random.default_rng(seed + 77)
+    n = int(rng.integers(100, 500))
+    g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                               float(rng.uniform(4, 12)), int(rng.integers(10, 30)),
+                               0.8, seed=seed + 77)
+    path = str(tmp_path / "g.txt")
+    g.write(path)
+    for k in (3, 4):
+        ref = O.reference_solve(path, k)
+        got = O.oracle_solve(g, k, peel=O.PEEL_FAST, flow=O.FLOW_FLOAT_EK)
+        bad = [f for f in FIELDS if not same(got[f], ref[f])]
+        assert not bad, f"seed {seed} k {k}: {bad}"
+
+
+# ---- known answers -------------------------------------------------------
+
+def test_reference_data_graph_known_answers():
+    """cudaCode/data/test.txt holds K4 on {6,7,8,9} attached to a path."""
+    g = cds_amd.Graph.load(os.path.join(GOLDEN, "ref_data_test.txt"))
+    assert g.n == 10 and g.m == 14
+    assert O.oracle_count(g, 3)[0] == 4
+    assert O.oracle_count(g, 4)[0] == 1
+    r3 = O.oracle_solve(g, 3)
+    assert r3["vertices"] == [6, 7, 8, 9] and r3["density"] == 1.0
+    r4 = O.oracle_solve(g, 4)
+    assert r4["vertices"] == [6, 7, 8, 9] and r4["density"] == 0.25
+
+
+def test_reference_log_graph_known_answers():
+    """cpuCode/c++/build/testnew.log:1-12 and :229: 8 triangles with clique
+    degrees 3 3 4 5 2 2 1 2 1 1."""
+    g = cds_amd.Graph.load(os.path.join(GOLDEN, "ref_log_graph.txt"))
+    T, cd = O.oracle_count(g, 3)
+    assert T == 8
+    assert cd.tolist() == [3, 3, 4, 5, 2, 2, 1, 2, 1, 1]
+
+
+def test_notes_example_known_answers():
+    """notes.ipynb cell 15: clique degrees 3,3,4,4,1,0 and (3,Psi)-core
+    numbers 3,3,3,3,1,0 for vertices A..F."""
+    g = cds_amd.Graph.load(os.path.join(GOLDEN, "notes_example.txt"))
+    r = O.oracle_solve(g, 3)
+    assert r["clique_degree"] == [3, 3, 4, 4, 1, 0]
+    assert r["core"] == [3, 3, 3, 3, 1, 0]
+
+
+@pytest.mark.parametrize("n,k", [(6, 2), (7, 3), (8, 4), (9, 3)])
+def test_complete_graph_closed_forms(n, k):
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    g = cds_amd.Graph.from_edges(n, edges)
+    r = O.oracle_solve(g, k)
+    T = math.comb(n, k)
+    assert r["total_cliques"] == T
+    assert r["clique_degree"] == [math.comb(n - 1, k - 1)] * n
+    assert r["core"] == [math.comb(n - 1, k - 1)] * n
+    assert r["vertices"] == list(range(n))
+    assert r["density"] == np.float32(T / n)
+
+
+# ---- brute force -----------------------------------------------------------
+
+def brute_cliques(n, adj, k):
+    return [c for c in itertools.combinations(range(n), k)
+            if all(b in adj[a] for a, b in itertools.combinations(c, 2))]
+
+
+def brute_peel(n, cliques):
+    """Sequential removal of the (current clique degree, id)-minimum vertex;
+    returns the removal sequence of (vertex, value) and densities."""
+    alive = set(range(n))
+    live = [set(c) for c in cliques]
+    rows = []
+    T = len(cliques)
+    for i in range(1, n + 1):
+        deg = {v: sum(1 for c in live if v in c) for v in alive}
+        v = min(alive, key=lambda x: (deg[x], x))
+        rows.append((v, deg[v]))
+        killed = [c for c in live if v in c]
+        live = [c for c in live if v not in c]
+        T -= len(killed)
+        alive.remove(v)
+    return rows
+
+
+def random_graph(rng, n, p):
+    adj = [set() for _ in range(n)]
+    for a in range(n):
+        for b in range(a + 1, n):
+            if rng.random() < p:
+                adj[a].add(b)
+                adj[b].add(a)
+    edges = [(a, b) for a in range(n) for b in adj[a] if a < b]
+    return adj, edges
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_bruteforce_small(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(4, 11))
+    adj, edges = random_graph(rng, n, float(rng.uniform(0.3, 0.9)))
+    g = cds_amd.Graph.from_edges(n, edges)
+    for k in (2, 3, 4):
+        cl = brute_cliques(n, adj, k)
+        r = O.oracle_solve(g, k, rows=True)
+        assert r["total_cliques"] == len(cl)
+        assert r["clique_degree"] == [sum(1 for c in cl if v in c) for v in range(n)]
+        rows = brute_peel(n, cl)
+        assert list(zip(r["removal_vertex"][1:], r["removal_value"][1:])) == rows
+        # the reported set is a real subgraph with the reported density and
+        # never denser than the densest subgraph
+        S = r["vertices"]
+        inside = sum(1 for c in cl if all(v in S for v in c))
+        assert r["set_cliques"] == inside
+        best = max((sum(1 for c in cl if set(c) <= set(sub)) / len(sub)
+                    for m in range(1, n + 1)
+                    for sub in itertools.combinations(range(n), m)), default=0)
+        assert r["density"] <= np.float32(best) + 1e-6
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_exact_flow_bruteforce(seed):
+    """F = k T - max_A (k T(A) - c |A|), reach = the minimal maximiser."""
+    rng = np.random.default_rng(seed)
+    V = int(rng.integers(3, 9))
+    k = int(rng.integers(2, 4))
+    combos = list(itertools.combinations(range(V), k))
+    T = int(rng.integers(1, min(len(combos), 10) + 1))
+    picks = [combos[i] for i in rng.choice(len(combos), size=T, replace=False)]
+    cd = [sum(1 for c in picks if v in c) for v in range(V)]
+    c = int(rng.integers(1, 3 * k * 2))  # integer sink capacity
+    F, reach = O.component_flow(V, picks, k, cd, c, 1)
+    best, best_sets = 0, [()]
+    for m in range(1, V + 1):
+        for A in itertools.combinations(range(V), m):
+            val = k * sum(1 for q in picks if set(q) <= set(A)) - c * m
+            if val > best:
+                best, best_sets = val, [A]
+            elif val == best:
+                best_sets.append(A)
+    assert F == k * T - best
+    minimal = min(best_sets, key=len)
+    assert all(set(minimal) <= set(s) for s in best_sets)
+    assert [v for v in range(V) if reach[v]] == list(minimal)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_faithful_and_fast_peel_agree(seed):
+    rng = np.random.default_rng(seed + 500)
+    n = int(rng.integers(200, 600))
+    g = cds_amd.Graph.chung_lu(n, 2.4, 8.0, 20, 0.8, seed=seed + 500)
+    for k in (3, 4):
+        a = O.oracle_solve(g, k, peel=O.PEEL_FAITHFUL, rows=True)
+        b = O.oracle_solve(g, k, peel=O.PEEL_FAST, rows=True)
+        for f in ("removal_vertex", "removal_value", "row_density", "k_prime",
+                  "vertices", "density"):
+            assert a[f] == b[f], f
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_exact_flow_matches_float_flow_when_not_fragile(seed):
+    """FLOW_EXACT reproduces the reference's float search except where the
+    float Edmonds-Karp sum drifts by more than the 1e-2 decision tolerance;
+    such runs are detected from the two traces and reported, not hidden."""
+    rng = np.random.default_rng(seed + 900)
+    n = int(rng.integers(60, 400))
+    g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                               float(rng.uniform(4, 12)), int(rng.integers(8, 30)),
+                               0.85, seed=seed + 900)
+    for k in (2, 3, 4):
+        fl = O.oracle_solve(g, k, flow=O.FLOW_FLOAT_EK)
+        ex = O.oracle_solve(g, k, flow=O.FLOW_EXACT)
+        n_same = 0
+        for a, b, ea, eb in zip(fl["trace_alpha"], ex["trace_alpha"],
+                                fl["trace_equal"], ex["trace_equal"]):
+            if a != b or ea != eb:
+                break
+            n_same += 1
+        if n_same == len(fl["trace_alpha"]) == len(ex["trace_alpha"]):
+            assert fl["vertices"] == ex["vertices"] and fl["density"] == ex["density"]
+        else:
+            # the first differing decision must be a float-fragile one
+            i = n_same
+            assert fl["trace_alpha"][i] == ex["trace_alpha"][i]
+            assert abs(fl["trace_margin"][i] - ex["trace_margin"][i]) > 1e-3
