@@ -87,7 +87,14 @@ class CResult(ctypes.Structure):
         ("list_work", ctypes.c_uint64), ("max_out_degree", ctypes.c_uint32),
         ("list_items", ctypes.c_uint64), ("t_list_count_ms", ctypes.c_double),
         ("t_list_write_ms", ctypes.c_double),
-        ("peak_device_bytes", ctypes.c_uint64), ("times", CTimes)]
+        ("peak_device_bytes", ctypes.c_uint64), ("peel_grid", ctypes.c_uint32),
+        ("flow_launches", ctypes.c_uint32),
+        ("density_certificates", ctypes.c_uint32),
+        ("peel_distinct_levels", ctypes.c_uint32),
+        ("peel_alive_scanned", ctypes.c_uint64),
+        ("peel_split_ms", ctypes.c_double * 4),
+        ("t_incidence_ms", ctypes.c_double), ("t_peel_kernel_ms", ctypes.c_double),
+        ("t_replay_ms", ctypes.c_double), ("times", CTimes)]
 
 
 class COpts(ctypes.Structure):
@@ -111,6 +118,7 @@ _SIGNATURES = {
     "cds_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
     "cds_result_free": (None, [ctypes.POINTER(CResult)]),
     "cds_count_cliques": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, f64p]),
+    "cds_core_decompose": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
     "cds_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(CParams), ctypes.c_size_t, ctypes.POINTER(CResult)]),
     "cds_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
     "cds_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
@@ -233,7 +241,8 @@ def _arr(ptr, count, dtype):
 def result_to_dict(r):
     d = {f: getattr(r, f) for f, _ in CResult._fields_
          if f not in ("clique_degree", "core", "vertices", "trace_lb", "trace_ub",
-                      "trace_alpha", "trace_component", "trace_margin", "times")}
+                      "trace_alpha", "trace_component", "trace_margin", "times",
+                      "peel_split_ms")}
     d["vertices"] = _arr(r.vertices, r.set_size, np.uint32).tolist()
     d["clique_degree"] = _arr(r.clique_degree, r.n, np.uint64) if r.clique_degree else None
     d["core"] = _arr(r.core, r.n, np.uint32) if r.core else None
@@ -244,6 +253,8 @@ def result_to_dict(r):
     d["trace_component"] = _arr(r.trace_component, n, np.uint32).tolist()
     d["trace_margin"] = _arr(r.trace_margin, n, np.float64).tolist()
     d["times"] = {f: getattr(r.times, f) for f, _ in CTimes._fields_}
+    d["peel_split_ms"] = dict(zip(("barrier", "scan", "process", "flush"),
+                                  list(r.peel_split_ms)))
     return d
 
 
@@ -266,6 +277,17 @@ class Engine:
                     early_stop, int(ub1), int(ub2), int(vertex_arrays))
         r = CResult()
         _check(lib().cds_solve(self._h, dgraph._h, ctypes.byref(p), ctypes.byref(r)))
+        try:
+            return result_to_dict(r)
+        finally:
+            lib().cds_result_free(ctypes.byref(r))
+
+    def core_decompose(self, dgraph, k, vertex_arrays=False):
+        """Listing + clique-core peel + densest core only (no flow)."""
+        p = CParams(k, 8000, 16000, 60000, 30000, -1, 1, 1, int(vertex_arrays))
+        r = CResult()
+        _check(lib().cds_core_decompose(self._h, dgraph._h, ctypes.byref(p),
+                                        ctypes.byref(r)))
         try:
             return result_to_dict(r)
         finally:

@@ -24,6 +24,27 @@ __device__ __forceinline__ unsigned wave_slot(bool pred, unsigned *count) {
   return __popcll(mask & lanes_below());
 }
 
+// Wave-aggregated atomicAdd(&ctr[key], 1) for every lane with `active`:
+// lanes with equal keys share one atomic; returns this lane's 0-based slot
+// within its key's reservation (order among equal keys = lane order).
+__device__ __forceinline__ uint32_t aggregated_reserve(uint32_t *ctr,
+                                                       uint32_t key,
+                                                       bool active) {
+  uint64_t pending = __ballot(active);
+  uint32_t slot = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader, kWave);
+    const uint64_t same = __ballot(active && key == lkey) & pending;
+    uint32_t base = 0;
+    if ((int)lane() == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(same));
+    base = __shfl(base, leader, kWave);
+    if (active && key == lkey) slot = base + __popcll(same & lanes_below());
+    pending &= ~same;
+  }
+  return slot;
+}
+
 // Wave-wide sum of a 32-bit value.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll

@@ -32,6 +32,10 @@ struct cds_engine {
   bool profile = false;
   std::mutex mu;
   void *pinned = nullptr;  // small readbacks
+  // components with more flow nodes than this use the whole-GPU flow
+  // (CDS_GRID_FLOW_NODES overrides it, e.g. to exercise that kernel on
+  // small graphs)
+  uint64_t grid_nodes = 1ull << 17;
 };
 
 struct cds_dgraph {
@@ -85,7 +89,7 @@ template <class T> struct DBuf {
 };
 
 struct Timer {
-  hipEvent_t ev[12];
+  hipEvent_t ev[24];
   int used = 0;
   hipStream_t s;
   explicit Timer(hipStream_t st) : s(st) {
@@ -95,6 +99,7 @@ struct Timer {
     for (auto &e : ev) (void)hipEventDestroy(e);
   }
   int mark() {
+    if (used >= 24) return 23;  // out of events: later marks alias the last
     (void)hipEventRecord(ev[used], s);
     return used++;
   }
@@ -147,12 +152,15 @@ float ub2_value(uint32_t k, uint64_t total, const uint32_t *cd, uint32_t V) {
 
 // |F - target| < 1e-2f evaluated exactly; F_scaled = F * 2^sc, target is
 // an integral float, 1e-2f = 5368709 * 2^-29.
-bool flow_equal(long long f_scaled, float target, int sc) {
+bool flow_equal(__int128 f_scaled, float target, int sc) {
   const __int128 t = (__int128)(long long)target << sc;
-  __int128 d = (__int128)f_scaled - t;
+  __int128 d = f_scaled - t;
   if (d < 0) d = -d;
   return d * ((__int128)1 << 29) < (__int128)5368709 << sc;
 }
+
+// Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
+constexpr uint32_t kGridFlowRounds = 20000;
 
 struct Component {
   uint32_t V = 0;        // vertices
@@ -200,6 +208,7 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
     return fail(CDS_ERR_HIP, "engine init failed: %s", hipGetErrorString(r));
   }
   e->arena = new DeviceArena(e->stream);
+  if (const char *gn = getenv("CDS_GRID_FLOW_NODES")) e->grid_nodes = strtoull(gn, nullptr, 10);
   *out = e;
   return CDS_OK;
 }
@@ -394,7 +403,7 @@ extern "C" cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *g,
 namespace {
 
 cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
-                      cds_result *R) {
+                      cds_result *R, bool core_only) {
   const auto wall0 = std::chrono::steady_clock::now();
   DeviceArena &A = *e->arena;
   hipStream_t st = e->stream;
@@ -445,21 +454,66 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
   HIPCHK(icur.alloc(n));
   HIPCHK(icur.zero());
-  HIPCHK(inc.alloc(T * k));
+  HIPCHK(inc.alloc(T * k * k));  // entries: clique id + other k-1 members
+  const int ev_inc0 = tm.mark();
   HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
+  const int ev_inc1 = tm.mark();
   icur.reset();
   HIPCHK(deg.alloc(n));
   HIPCHK(core.alloc(n));
   HIPCHK(kill.alloc(T));
   HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
-  HIPCHK(fa.alloc(n));
-  HIPCHK(fb.alloc(n));
-  HIPCHK(cnt.alloc(4));  // [0] min, [1] count A, [2] count B
-  HIPCHK(killed.alloc(1));
   uint32_t *hp = (uint32_t *)e->pinned;
   std::vector<Level> levels;
   uint64_t removed_total = 0;
   uint32_t peel_rounds = 0;
+  if (n < (1u << 24)) {
+    // every level in one cooperative launch
+    DBuf<uint32_t> s32(&A), lvl(&A);
+    DBuf<uint64_t> s64(&A);
+    DBuf<unsigned long long> lrem(&A), lkil(&A);
+    DBuf<PeelCtl> ctl(&A);
+    HIPCHK(s32.alloc(6ull * n));
+    HIPCHK(s64.alloc(3ull * n));
+    HIPCHK(lvl.alloc(n));
+    HIPCHK(lrem.alloc(n));
+    HIPCHK(lkil.alloc(n));
+    HIPCHK(ctl.alloc(1));
+    unsigned grid = 0;
+    const int ev_p0 = tm.mark();
+    HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
+                           kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
+                           e->device, &grid, st));
+    const int ev_p1 = tm.mark();
+    PeelCtl hc;
+    CHK(readback(e, &hc, ctl.p, 1));
+    if (hc.removed != n || hc.nlevels == 0 || hc.nlevels > n)
+      return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %u (%u levels)",
+                  hc.removed, n, hc.nlevels);
+    std::vector<uint32_t> hl(hc.nlevels);
+    std::vector<unsigned long long> hr(hc.nlevels), hk(hc.nlevels);
+    CHK(readback(e, hl.data(), lvl.p, hc.nlevels));
+    CHK(readback(e, hr.data(), lrem.p, hc.nlevels));
+    CHK(readback(e, hk.data(), lkil.p, hc.nlevels));
+    for (uint32_t i = 0; i < hc.nlevels; ++i) levels.push_back({hl[i], hr[i], hk[i]});
+    R->t_peel_kernel_ms = tm.ms(ev_p0, ev_p1);
+    removed_total = n;
+    peel_rounds = (uint32_t)hc.rounds;
+    R->peel_grid = grid;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate,
+                              e->device) == hipSuccess &&
+        wall_khz > 0) {
+      R->peel_split_ms[0] = (double)hc.t_bar / wall_khz;
+      R->peel_split_ms[1] = (double)hc.t_scan / wall_khz;
+      R->peel_split_ms[2] = (double)hc.t_proc / wall_khz;
+      R->peel_split_ms[3] = (double)hc.t_flush / wall_khz;
+    }
+  }
+  HIPCHK(fa.alloc(n));
+  HIPCHK(fb.alloc(n));
+  HIPCHK(cnt.alloc(4));  // [0] min, [1] count A, [2] count B
+  HIPCHK(killed.alloc(1));
   while (removed_total < n) {
     HIPCHK(peel_min_alive(n, deg.p, core.p, cnt.p, st));
     HIPCHK(hipMemsetAsync(cnt.p + 1, 0, 8, st));
@@ -492,7 +546,17 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   }
   fa.reset();
   fb.reset();
+  R->t_incidence_ms = tm.ms(ev_inc0, ev_inc1);
   R->peel_levels = peel_rounds;
+  R->peel_distinct_levels = (uint32_t)levels.size();
+  {
+    uint64_t alive = n, scanned = 0;
+    for (auto &lv : levels) {
+      scanned += alive;
+      alive -= lv.removed;
+    }
+    R->peel_alive_scanned = scanned;
+  }
 
   // Boundary densities: the state before level levels[i] is the clique core
   // C_level; the oracle's rows include each of them (helpers.cpp:125-134).
@@ -520,9 +584,11 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // higher density; replay those whose bound can exceed the current k'.
   {
     DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
+    DBuf<uint64_t> pioff(&A);
     DBuf<uint8_t> palive(&A);
     DBuf<ReplayOut> rout(&A);
     bool allocated = false;
+    const int ev_r0 = tm.mark();
     for (size_t ii = levels.size(); ii-- > 0;) {
       const Level &lv = levels[ii];
       if ((int64_t)lv.level <= kp) break;
@@ -543,13 +609,16 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         HIPCHK(rdead.alloc(T));
         HIPCHK(rdead.zero());
         HIPCHK(pc.alloc(1));
+        HIPCHK(pioff.alloc((size_t)n + 1));
         HIPCHK(rout.alloc(1));
         allocated = true;
       }
+      DBuf<uint32_t> pinc(&A);
+      HIPCHK(pinc.alloc(lv.killed * k * k));
       HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
-                          L.cliques.p, T - kl_before[ii], n - rm_before[ii],
-                          (uint32_t)M, pv.p, pdeg.p, posmap.p, palive.p,
-                          rdead.p, pc.p, rout.p, st));
+                          T - kl_before[ii], n - rm_before[ii], (uint32_t)M,
+                          pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
+                          pioff.p, pinc.p, rout.p, A, st));
       ReplayOut ro;
       CHK(readback(e, &ro, rout.p, 1));
       R->replay_phases++;
@@ -562,6 +631,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         kp = std::max(kp, (int64_t)std::ceil(d));
       }
     }
+    const int ev_r1 = tm.mark();
+    HIPCHK(hipStreamSynchronize(st));
+    R->t_replay_ms = tm.ms(ev_r0, ev_r1);
   }
   R->peel_max_density = rho;
   R->k_prime = (uint64_t)kp;
@@ -600,10 +672,28 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     return fail(CDS_ERR_INTERNAL, "core clique count %llu != %llu", ct,
                 (unsigned long long)core_T);
   const int ev_core = tm.mark();
+  if (core_only) {
+    if (p->want_vertex_arrays) {
+      R->clique_degree = host_alloc<uint64_t>(n);
+      memcpy(R->clique_degree, cd.data(), n * 8);
+      R->core = host_alloc<uint32_t>(n);
+      CHK(readback(e, R->core, core.p, n));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    R->times.dag = tm.ms(ev_start, ev_dag);
+    R->times.listing = tm.ms(ev_dag, ev_list);
+    R->times.core_decompose = tm.ms(ev_list, ev_peel);
+    R->times.densest_core = tm.ms(ev_peel, ev_core);
+    R->times.total = std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - wall0)
+                         .count();
+    R->peak_device_bytes = A.peak_bytes();
+    return CDS_OK;
+  }
 
   // ---- prune ----
+  DBuf<uint8_t> mark(&A);
   {
-    DBuf<uint8_t> mark(&A);
     const size_t na = g->host_off[n];
     HIPCHK(mark.alloc(na));
     HIPCHK(mark.zero());
@@ -619,7 +709,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint32_t> parent(&A);
   HIPCHK(parent.alloc(n));
   HIPCHK(uf_init(n, parent.p, st));
-  HIPCHK(uf_union_cliques(T, k, L.cliques.p, kill.p, kpu, parent.p, st));
+  // components of the pruned graph = classes of "shares a core clique"
+  HIPCHK(uf_union_edges(n, g->off, g->adj, mark.p, core.p, kpu, parent.p, st));
   HIPCHK(uf_flatten(n, parent.p, st));
   std::vector<uint32_t> hpar(n), hcore(n);
   CHK(readback(e, hpar.data(), parent.p, n));
@@ -667,12 +758,51 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(ccount.alloc(ncomp));
   HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_n));
-  HIPCHK(fcd.zero());
-  HIPCHK(comp_clique_count(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
-                           d_fv_of.p, ccount.p, fcd.p, st));
-  std::vector<uint32_t> hcc(ncomp), hfcd(core_n);
+  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
+                            ccount.p, st));
+  // Work items over the core vertices' incidence lists, cut into chunks so
+  // hub lists spread over many waves.
+  constexpr uint32_t kItem = 4096;
+  std::vector<uint4> items;
+  std::vector<uint32_t> item_fv;
+  {
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      const uint64_t len = cd[v];
+      if (hcore[v] >= kpu)
+        for (uint64_t s0 = 0; s0 < len; s0 += kItem) {
+          const uint64_t at = pos + s0;
+          items.push_back(make_uint4(v, (uint32_t)(at >> 32), (uint32_t)at,
+                                     (uint32_t)std::min<uint64_t>(kItem, len - s0)));
+          item_fv.push_back(fv_of[v]);
+        }
+      pos += len;
+    }
+  }
+  const uint64_t nitems = items.size();
+  DBuf<uint4> d_items(&A);
+  DBuf<uint32_t> d_icnt(&A);
+  DBuf<uint64_t> d_iout(&A);
+  HIPCHK(d_items.alloc(nitems));
+  HIPCHK(d_icnt.alloc(nitems));
+  HIPCHK(d_iout.alloc(nitems));
+  CHK(upload(e, d_items.p, items.data(), nitems));
+  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, kill.p, kpu, d_icnt.p, st));
+  std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0), hicnt(nitems);
+  std::vector<uint64_t> hiout(nitems), hfioff(core_n + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
-  CHK(readback(e, hfcd.data(), fcd.p, core_n));
+  CHK(readback(e, hicnt.data(), d_icnt.p, nitems));
+  for (uint64_t i = 0; i < nitems; ++i) hfcd[item_fv[i]] += hicnt[i];
+  for (uint32_t f = 0; f < core_n; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
+  {
+    std::vector<uint64_t> cursor(hfioff.begin(), hfioff.end() - 1);
+    for (uint64_t i = 0; i < nitems; ++i) {
+      hiout[i] = cursor[item_fv[i]];
+      cursor[item_fv[i]] += hicnt[i];
+    }
+  }
+  CHK(upload(e, fcd.p, hfcd.data(), core_n));
+  CHK(upload(e, d_iout.p, hiout.data(), nitems));
   std::vector<uint64_t> hcoff(ncomp + 1, 0);
   for (uint32_t c = 0; c < ncomp; ++c) {
     comps[c].T = hcc[c];
@@ -689,15 +819,18 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(ccur.alloc(ncomp));
   HIPCHK(ccur.zero());
   HIPCHK(fmem.alloc(core_T * k));
-  HIPCHK(comp_clique_scatter(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
-                             d_fv_of.p, coff.p, ccur.p, fmem.p, st));
-  HIPCHK(fioff.alloc((size_t)core_n + 1));
-  HIPCHK(scan_u32(fcd.p, fioff.p, core_n, A));
-  HIPCHK(ficur.alloc(core_n));
-  HIPCHK(ficur.zero());
-  HIPCHK(finc.alloc(core_T * k));
-  HIPCHK(flow_incidence(core_T, k, fmem.p, fioff.p, ficur.p, finc.p, st));
-  HIPCHK(hipStreamSynchronize(st));
+  {
+    DBuf<uint32_t> cslot(&A);  // flow index of each core clique
+    HIPCHK(cslot.alloc(T));
+    HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
+                              d_fv_of.p, coff.p, ccur.p, fmem.p, cslot.p, st));
+    HIPCHK(fioff.alloc((size_t)core_n + 1));
+    CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
+    HIPCHK(finc.alloc(core_T * k));
+    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, kill.p, kpu,
+                           L.cliques.p, k, cslot.p, finc.p, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   const int ev_cc = tm.mark();
 
   // ---- (c) dynamicExact (helpers.cpp:1120-1254) ----
@@ -742,6 +875,105 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<unsigned long long> fmisc(&A);
   bool flow_alloc = false;
   std::vector<int> res, reach_h;
+  // Exact max flow of component ci with sink capacity capT (all
+  // capacities in units of 1/U); leaves the min-cut side in freach.
+  auto run_flow = [&](uint32_t ci, long long capT, long long U,
+                      long long *F) -> cds_status {
+    const Component &C = comps[ci];
+    uint32_t maxcd = 0;
+    for (uint32_t v = 0; v < C.V; ++v) maxcd = std::max(maxcd, hfcd[C.vbase + v]);
+    if ((__int128)maxcd * U >= ((__int128)1 << 62) ||
+        (__int128)capT * C.V >= ((__int128)1 << 62) ||
+        (__int128)C.T * k * U >= ((__int128)1 << 62))
+      return fail(CDS_ERR_RANGE, "component %u exceeds the int64 flow range", ci);
+    FlowProblem P;
+    P.k = k;
+    P.V = C.V;
+    P.T = (uint32_t)C.T;
+    P.vbase = C.vbase;
+    P.sbase = C.cbase * k;
+    P.mem = fmem.p + C.cbase * k;
+    P.ioff = fioff.p + C.vbase;
+    P.inc = finc.p;
+    P.cd = fcd.p + C.vbase;
+    if ((uint64_t)C.V + C.T > e->grid_nodes) {
+      // whole-GPU push-relabel for a large component
+      DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
+      DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
+      DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
+      DBuf<uint64_t> glc0(&A), glc1(&A);
+      DBuf<GridFlowCtl> gctl(&A);
+      HIPCHK(gx.alloc(C.T * k));
+      HIPCHK(grvs.alloc(C.V));
+      HIPCHK(gexv.alloc(C.V));
+      HIPCHK(grem.alloc(C.V));
+      HIPCHK(gexc.alloc(C.T));
+      HIPCHK(ghv.alloc(C.V));
+      HIPCHK(ghmin.alloc(C.V));
+      HIPCHK(ghc.alloc(C.T));
+      HIPCHK(gl0.alloc(C.V));
+      HIPCHK(gl1.alloc(C.V));
+      HIPCHK(glc0.alloc(C.V));
+      HIPCHK(glc1.alloc(C.V));
+      HIPCHK(gq0.alloc(C.T));
+      HIPCHK(gq1.alloc(C.T));
+      HIPCHK(gctl.alloc(1));
+      GridFlowState G;
+      G.x = gx.p;
+      G.rvs = grvs.p;
+      G.exv = gexv.p;
+      G.rem = grem.p;
+      G.exc = gexc.p;
+      G.hv = ghv.p;
+      G.hmin = ghmin.p;
+      G.hc = ghc.p;
+      G.list0 = gl0.p;
+      G.list1 = gl1.p;
+      G.lchunk0 = glc0.p;
+      G.lchunk1 = glc1.p;
+      G.cq0 = gq0.p;
+      G.cq1 = gq1.p;
+      G.ctl = gctl.p;
+      G.reach = freach.p;
+      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, e->device, st));
+      GridFlowCtl hc;
+      CHK(readback(e, &hc, gctl.p, 1));
+      if (hc.limit)
+        return fail(CDS_ERR_NOT_CONVERGED,
+                    "grid max flow of component %u hit %u rounds", ci, hc.rounds);
+      sweeps += hc.rounds;
+      R->flow_launches++;
+      *F = (long long)hc.sink;
+      return CDS_OK;
+    }
+    FlowState S;
+    S.x = fx.p;
+    S.rvs = frvs.p;
+    S.excess = fex.p;
+    S.h = fh.p;
+    S.dtgt = fdt.p;
+    S.damt = fda.p;
+    S.queue0 = fq0.p;
+    S.queue1 = fq1.p;
+    S.misc = fmisc.p;
+    S.reach = freach.p;
+    // One workgroup sweeps all nodes per round; the round budget keeps a
+    // launch to roughly 2e10 node visits (tens of seconds) so an oversized
+    // component fails with NOT_CONVERGED instead of stalling.
+    const uint64_t nodes = (uint64_t)C.V + C.T;
+    const uint32_t max_rounds = (uint32_t)std::max<uint64_t>(
+        64, std::min<uint64_t>(2000000, 20000000000ull / nodes));
+    HIPCHK(max_flow(P, S, capT, U, max_rounds, st));
+    unsigned long long misc[4];
+    CHK(readback(e, misc, fmisc.p, 4));
+    if (misc[2])
+      return fail(CDS_ERR_NOT_CONVERGED,
+                  "max flow of component %u hit the round limit", ci);
+    sweeps += misc[1];
+    R->flow_launches++;
+    *F = (long long)misc[0];
+    return CDS_OK;
+  };
   for (uint32_t ci = 0; ci < ncomp; ++ci) {
     const Component &C = comps[ci];
     if (p->ub1) {
@@ -762,6 +994,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     res.assign(C.V, 1);
     uint64_t res_cliques = C.T;
     bool searched = false;
+    int whole = -1;  // 1: the component is its own densest subgraph
     while ((ub - lb) > bias) {
       if (!flow_alloc) {
         HIPCHK(fx.alloc(maxT * k));
@@ -785,44 +1018,28 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         return fail(CDS_ERR_RANGE, "alpha %g too small for exact scaling", alpha);
       const long long U = 1ll << sc;
       const long long capT = (long long)std::ldexp((double)cap_t, sc);
-      uint32_t maxcd = 0;
-      for (uint32_t v = 0; v < C.V; ++v) maxcd = std::max(maxcd, hfcd[C.vbase + v]);
-      if ((__int128)maxcd * U >= ((__int128)1 << 62) ||
-          (__int128)capT * C.V >= ((__int128)1 << 62) ||
-          (__int128)C.T * k * U >= ((__int128)1 << 62))
-        return fail(CDS_ERR_RANGE, "component %u exceeds the int64 flow range", ci);
-      FlowProblem P;
-      P.k = k;
-      P.V = C.V;
-      P.T = (uint32_t)C.T;
-      P.vbase = C.vbase;
-      P.sbase = C.cbase * k;
-      P.mem = fmem.p + C.cbase * k;
-      P.ioff = fioff.p + C.vbase;
-      P.inc = finc.p;
-      P.cd = fcd.p + C.vbase;
-      FlowState S;
-      S.x = fx.p;
-      S.rvs = frvs.p;
-      S.excess = fex.p;
-      S.h = fh.p;
-      S.dtgt = fdt.p;
-      S.damt = fda.p;
-      S.queue0 = fq0.p;
-      S.queue1 = fq1.p;
-      S.misc = fmisc.p;
-      S.reach = freach.p;
-      const uint64_t nodes = (uint64_t)C.V + C.T;
-      const uint32_t max_rounds =
-          (uint32_t)std::min<uint64_t>(4000000000ull, 64 * nodes * nodes + 1000000);
-      HIPCHK(max_flow(P, S, capT, U, max_rounds, st));
-      unsigned long long misc[4];
-      CHK(readback(e, misc, fmisc.p, 4));
-      if (misc[2])
-        return fail(CDS_ERR_NOT_CONVERGED,
-                    "max flow of component %u hit the round limit", ci);
-      sweeps += misc[1];
-      const long long F = (long long)misc[0];
+      if (whole < 0) {
+        // One flow at the component's own density c0 = kT/V (capacities
+        // scaled by V): if it saturates, no subset is denser, g(c) =
+        // max(0, kT - cV) for every c and the minimal maximiser is the whole
+        // component below c0 and empty above it.
+        long long F0 = 0;
+        CHK(run_flow(ci, (long long)(C.T * k), (long long)C.V, &F0));
+        whole = (__int128)F0 == (__int128)(C.T * k) * C.V ? 1 : 0;
+        if (whole) R->density_certificates++;
+      }
+      __int128 F = 0;
+      bool from_flow = false;
+      if (whole == 1) {
+        __int128 g = ((__int128)(C.T * k) << sc) - (__int128)capT * C.V;
+        if (g < 0) g = 0;
+        F = ((__int128)(C.T * k) << sc) - g;
+      } else {
+        long long Ff = 0;
+        CHK(run_flow(ci, capT, U, &Ff));
+        F = Ff;
+        from_flow = true;
+      }
       const bool eq = flow_equal(F, target, sc);
       tr_comp.push_back(ci);
       tr_lb.push_back(lb);
@@ -833,14 +1050,19 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         ub = alpha;
       } else {
         lb = alpha;
-        reach_h.resize(C.V);
-        CHK(readback(e, reach_h.data(), freach.p, C.V));
-        for (uint32_t v = 0; v < C.V; ++v) res[v] = reach_h[v] ? 1 : -1;
-        HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, P.mem, C.vbase,
-                                      freach.p, acc.p, st));
-        unsigned long long cc = 0;
-        CHK(readback(e, &cc, acc.p, 1));
-        res_cliques = cc;
+        if (!from_flow) {
+          for (uint32_t v = 0; v < C.V; ++v) res[v] = 1;
+          res_cliques = C.T;
+        } else {
+          reach_h.resize(C.V);
+          CHK(readback(e, reach_h.data(), freach.p, C.V));
+          for (uint32_t v = 0; v < C.V; ++v) res[v] = reach_h[v] ? 1 : -1;
+          HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
+                                        C.vbase, freach.p, acc.p, st));
+          unsigned long long cc = 0;
+          CHK(readback(e, &cc, acc.p, 1));
+          res_cliques = cc;
+        }
       }
       alpha = (lb + ub) * 0.5f;
     }
@@ -935,7 +1157,23 @@ extern "C" cds_status cds_solve(cds_engine *e, const cds_dgraph *g,
                 p->k);
   std::lock_guard<std::mutex> lk(e->mu);
   HIPCHK(hipSetDevice(e->device));
-  cds_status s = solve_impl(e, g, p, out);
+  cds_status s = solve_impl(e, g, p, out, false);
+  if (s != CDS_OK) {
+    (void)hipStreamSynchronize(e->stream);
+    cds_result_free(out);
+  }
+  return s;
+}
+
+extern "C" cds_status cds_core_decompose(cds_engine *e, const cds_dgraph *g,
+                                         const cds_params *p, cds_result *out) {
+  if (!e || !g || !p || !out) return fail(CDS_ERR_INVALID, "null argument");
+  if (p->k < 2 || p->k > 4)
+    return fail(CDS_ERR_UNSUPPORTED, "k=%u: the device lister covers k=2..4",
+                p->k);
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  cds_status s = solve_impl(e, g, p, out, true);
   if (s != CDS_OK) {
     (void)hipStreamSynchronize(e->stream);
     cds_result_free(out);

@@ -1,0 +1,472 @@
+// (c) Exact max flow of one large component on the whole GPU.
+//
+// Same network, orientation and exactness as flow.hip (reversed network,
+// int64 residuals scaled by U, answer = maximum preflow + "can reach the
+// sink" side), for components too big for one workgroup: a dense core of a
+// scale-20 R-MAT graph has ~1e3 vertices and ~1e8 cliques, so vertex nodes
+// have incidence lists of 1e5..1e7 arcs and clique nodes have k.
+//
+// One cooperative launch.  A round is four phases separated by grid
+// barriers:
+//   1. clique nodes (one thread each) push excess to lower members and
+//      relabel if excess remains;
+//   2. vertex nodes push to the sink, and every still-active vertex appends
+//      its incidence list, cut into chunks, to the round's work list;
+//   3. each wave takes chunks: an admissible arc (lower clique) takes
+//      min(residual, what is left of the vertex's excess) through one
+//      64-bit atomic on the vertex's remainder, so a vertex discharges over
+//      all its admissible arcs in one round; non-admissible residual arcs
+//      feed an atomic min for the relabel;
+//   4. vertices keep the remainder as excess and relabel to min + 1 if any
+//      is left.
+// Pushes only go from a higher to a lower node and relabels only raise
+// heights to one above the lowest residual neighbour, so labels stay valid.
+// A global relabel (BFS from the sink over reversed residual arcs,
+// alternating vertex and clique levels) runs every kGridRelabel rounds and
+// whenever no node is active; the solve ends only when that BFS finds no
+// active node, which certifies the preflow maximum.
+#include "dev.h"
+#include "kernels.h"
+
+namespace cds {
+namespace {
+
+constexpr uint32_t kGridChunk = 512;
+constexpr uint32_t kGridLdsVertices = 4096;  // LDS excess accumulator, 32 KB
+constexpr uint32_t kGridRelabel = 24;
+constexpr uint64_t kMask40 = (1ull << 40) - 1;
+
+__device__ __forceinline__ void gsync(GridFlowCtl *ctl) {
+  // The workgroup barrier makes every wave's stores reach L2; one thread
+  // per workgroup then releases them device-wide (the fence writes back
+  // this XCD's L2) and, after the arrival, acquires (invalidates stale
+  // lines) for the whole workgroup, whose waves share the CU's caches.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned gen =
+        __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(
+        &ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE,
+                               __HIP_MEMORY_SCOPE_AGENT) == gen)
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+template <class T> __device__ __forceinline__ T ald(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct G {
+  uint32_t k, V, T;
+  int INF;
+  uint32_t vbase;
+  uint64_t sbase;
+  long long U;
+  const uint32_t *mem;
+  const uint64_t *ioff;
+  const uint32_t *inc;
+  long long *x, *rvs, *exv, *exc, *rem;
+  int *hv, *hc, *hmin;
+  uint32_t *list[2];       // vertex work lists (active vertices / BFS)
+  uint64_t *lchunk[2];     // their chunk starts
+  uint32_t *cq[2];         // clique BFS frontiers
+  GridFlowCtl *ctl;
+  uint32_t gtid, gthreads, gwave, gwaves;
+  bool leader;
+};
+
+__device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
+  return (g.ioff[v + 1] - g.ioff[v] + kGridChunk - 1) / kGridChunk;
+}
+
+__device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
+  const unsigned long long old =
+      atomicAdd(&g.ctl->lpack[b], (1ull << 40) + (unsigned long long)nchunks(g, v));
+  g.list[b][old >> 40] = v;
+  g.lchunk[b][old >> 40] = old & kMask40;
+}
+
+// Maps work chunk ci of list b (count entries) to (vertex, incidence range).
+__device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
+                                         uint64_t ci, uint32_t *v, uint64_t *lo,
+                                         uint64_t *hi) {
+  uint32_t a = 0, z = count - 1;
+  while (a < z) {
+    const uint32_t mid = (a + z + 1) >> 1;
+    if (g.lchunk[b][mid] <= ci)
+      a = mid;
+    else
+      z = mid - 1;
+  }
+  *v = g.list[b][a];
+  *lo = g.ioff[*v] + (ci - g.lchunk[b][a]) * kGridChunk;
+  const uint64_t e = g.ioff[*v + 1];
+  *hi = *lo + kGridChunk < e ? *lo + kGridChunk : e;
+}
+
+// Exact BFS distances to the sink over reversed residual arcs; returns the
+// number of active nodes (excess > 0 and finite height).  Uses list buffer
+// 0/1 for vertex frontiers and cq[0]/cq[1] for clique frontiers.
+__device__ unsigned long long global_relabel(const G &g) {
+  GridFlowCtl *ctl = g.ctl;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
+  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.hc[c] = g.INF;
+  if (g.leader) {
+    ctl->lpack[0] = 0;
+    ctl->lpack[1] = 0;
+    ctl->cqn[0] = 0;
+    ctl->cqn[1] = 0;
+    ctl->active = 0;
+  }
+  gsync(ctl);
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
+    if (g.rvs[v] > 0) {
+      g.hv[v] = 1;
+      list_push(g, 0, v);
+    }
+  gsync(ctl);
+  int vb = 0, cb = 0, d = 1;
+  for (;;) {
+    // vertices at distance d -> cliques at d+1 (arc clique->v residual U+x)
+    const unsigned long long pack = ald(&ctl->lpack[vb]);
+    const uint32_t nv = (uint32_t)(pack >> 40);
+    if (nv == 0) break;
+    const uint64_t nch = pack & kMask40;
+    if (g.leader) {
+      ctl->cqn[cb] = 0;
+      ctl->lpack[1 - vb] = 0;
+    }
+    gsync(ctl);
+    for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
+      uint32_t v;
+      uint64_t lo, hi;
+      chunk_of(g, vb, nv, ci, &v, &lo, &hi);
+      for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+        const uint64_t s = g.inc[p] - g.sbase;
+        if (g.U + g.x[s] <= 0) continue;
+        const uint32_t c = (uint32_t)(s / g.k);
+        if (ald(&g.hc[c]) != g.INF) continue;
+        if (atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF)
+          g.cq[cb][atomicAdd(&ctl->cqn[cb], 1u)] = c;
+      }
+    }
+    gsync(ctl);
+    // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x)
+    const uint32_t nc = ald(&ctl->cqn[cb]);
+    for (uint32_t i = g.gtid; i < nc; i += g.gthreads) {
+      const uint32_t c = g.cq[cb][i];
+      const uint64_t base = (uint64_t)c * g.k;
+      for (uint32_t j = 0; j < g.k; ++j) {
+        if ((long long)(g.k - 1) * g.U - g.x[base + j] <= 0) continue;
+        const uint32_t v = g.mem[base + j] - g.vbase;
+        if (ald(&g.hv[v]) != g.INF) continue;
+        if (atomicCAS(&g.hv[v], g.INF, d + 2) == g.INF) list_push(g, 1 - vb, v);
+      }
+    }
+    gsync(ctl);
+    vb = 1 - vb;
+    cb = 1 - cb;
+    d += 2;
+  }
+  unsigned long long act = 0;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
+    act += g.exv[v] > 0 && g.hv[v] < g.INF;
+  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads)
+    act += g.exc[c] > 0 && g.hc[c] < g.INF;
+  act = wave_sum64(act);
+  if (lane() == 0 && act) atomicAdd(&ctl->active, act);
+  gsync(ctl);
+  const unsigned long long total = ald(&ctl->active);
+  if (g.leader) {
+    ctl->lpack[0] = 0;
+    ctl->lpack[1] = 0;
+  }
+  gsync(ctl);
+  return total;
+}
+
+__global__ __launch_bounds__(256) void k_grid_flow(FlowProblem P,
+                                                   GridFlowState S,
+                                                   long long cap_t, long long U,
+                                                   uint32_t max_rounds) {
+  G g;
+  g.k = P.k;
+  g.V = P.V;
+  g.T = P.T;
+  g.INF = (int)(P.V + P.T) + 1;
+  g.vbase = P.vbase;
+  g.sbase = P.sbase;
+  g.U = U;
+  g.mem = P.mem;
+  g.ioff = P.ioff;
+  g.inc = P.inc;
+  g.x = S.x;
+  g.rvs = S.rvs;
+  g.exv = S.exv;
+  g.exc = S.exc;
+  g.rem = S.rem;
+  g.hv = S.hv;
+  g.hc = S.hc;
+  g.hmin = S.hmin;
+  g.list[0] = S.list0;
+  g.list[1] = S.list1;
+  g.lchunk[0] = S.lchunk0;
+  g.lchunk[1] = S.lchunk1;
+  g.cq[0] = S.cq0;
+  g.cq[1] = S.cq1;
+  g.ctl = S.ctl;
+  g.gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  g.gthreads = gridDim.x * blockDim.x;
+  g.gwave = g.gtid / kWave;
+  g.gwaves = g.gthreads / kWave;
+  g.leader = g.gtid == 0;
+  GridFlowCtl *ctl = S.ctl;
+  const uint64_t slots = (uint64_t)g.T * g.k;
+  extern __shared__ long long lds_ex[];
+  const bool lds_v = g.V <= kGridLdsVertices;
+
+  for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
+  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+    g.rvs[v] = (long long)P.cd[v] * U;
+    g.exv[v] = cap_t;
+  }
+  if (g.leader) {
+    ctl->sink = 0;
+    ctl->rounds = 0;
+    ctl->limit = 0;
+    ctl->relabels = 0;
+  }
+  gsync(ctl);
+  unsigned long long active = global_relabel(g);
+  uint32_t rounds = 0, relabels = 1, since = 0;
+  unsigned long long sink = 0;
+  bool limit = false;
+  while (active) {
+    if (rounds >= max_rounds) {
+      limit = true;
+      break;
+    }
+    if (g.leader) {
+      ctl->lpack[0] = 0;
+      ctl->active = 0;
+    }
+    // 1. cliques push to lower members, then relabel if excess remains.
+    // With few vertices, the excess they receive is summed per workgroup in
+    // LDS and flushed with one atomic per vertex.
+    if (lds_v) {
+      for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) lds_ex[i] = 0;
+      __syncthreads();
+    }
+    unsigned long long act = 0;
+    for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) {
+      long long e = g.exc[c];
+      const int hcc = g.hc[c];
+      if (e <= 0 || hcc >= g.INF) continue;
+      ++act;
+      const uint64_t base = (uint64_t)c * g.k;
+      int hmin = g.INF;
+      for (uint32_t j = 0; j < g.k && e > 0; ++j) {
+        const long long res = U + g.x[base + j];
+        if (res <= 0) continue;
+        const uint32_t v = g.mem[base + j] - g.vbase;
+        const int hvv = ald(&g.hv[v]);
+        if (hvv < hcc) {
+          const long long d = e < res ? e : res;
+          g.x[base + j] -= d;
+          e -= d;
+          if (lds_v)
+            atomicAdd((unsigned long long *)&lds_ex[v], (unsigned long long)d);
+          else
+            atomicAdd((unsigned long long *)&g.exv[v], (unsigned long long)d);
+          if (d < res) hmin = min(hmin, hvv);
+        } else {
+          hmin = min(hmin, hvv);
+        }
+      }
+      g.exc[c] = e;
+      if (e > 0) {
+        // all lower members are saturated; recompute over residual arcs
+        hmin = g.INF;
+        for (uint32_t j = 0; j < g.k; ++j)
+          if (U + g.x[base + j] > 0)
+            hmin = min(hmin, ald(&g.hv[g.mem[base + j] - g.vbase]));
+        g.hc[c] = hmin >= g.INF - 1 ? g.INF : hmin + 1;
+      }
+    }
+    if (lds_v) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) {
+        const long long a = lds_ex[i];
+        if (a) atomicAdd((unsigned long long *)&g.exv[i], (unsigned long long)a);
+      }
+    }
+    gsync(ctl);
+    // 2. vertices push to the sink; active vertices post their chunks
+    for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+      long long e = g.exv[v];
+      if (e <= 0 || g.hv[v] >= g.INF) {
+        g.rem[v] = 0;
+        continue;
+      }
+      ++act;
+      const long long r = g.rvs[v];
+      if (r > 0) {
+        const long long d = e < r ? e : r;
+        g.rvs[v] = r - d;
+        e -= d;
+        sink += (unsigned long long)d;
+      }
+      g.exv[v] = e;
+      g.rem[v] = e;
+      g.hmin[v] = g.INF;
+      if (e > 0) list_push(g, 0, v);
+    }
+    act = wave_sum64(act);
+    if (lane() == 0 && act) atomicAdd(&ctl->active, act);
+    gsync(ctl);
+    const unsigned long long pack = ald(&ctl->lpack[0]);
+    const uint32_t nv = (uint32_t)(pack >> 40);
+    const uint64_t nch = pack & kMask40;
+    const unsigned long long any = ald(&ctl->active);
+    // 3. chunked discharge of active vertices over their incidence lists:
+    // pass 1 sums the chunk's admissible residual (and the lowest
+    // non-admissible height), one atomic reserves that much of the vertex's
+    // remaining excess, pass 2 hands the reservation to the admissible arcs
+    // in chunk order (wave prefix sums).  A slot belongs to one chunk and
+    // clique heights do not change in this phase, so both passes agree.
+    for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
+      uint32_t v;
+      uint64_t lo, hi;
+      chunk_of(g, 0, nv, ci, &v, &lo, &hi);
+      const int hvv = g.hv[v];
+      int hm = g.INF;
+      long long radm = 0;
+      for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+        const uint64_t s = g.inc[p] - g.sbase;
+        const long long res = (long long)(g.k - 1) * U - g.x[s];
+        if (res <= 0) continue;
+        const int hcc = g.hc[s / g.k];
+        if (hcc < hvv)
+          radm += res;
+        else
+          hm = min(hm, hcc);
+      }
+      radm = (long long)wave_sum64((uint64_t)radm);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) hm = min(hm, __shfl_xor(hm, off, kWave));
+      if (lane() == 0 && hm < g.INF) atomicMin(&g.hmin[v], hm);
+      if (radm == 0) continue;
+      long long avail = 0;
+      if (lane() == 0) {
+        const long long old = (long long)atomicAdd(
+            (unsigned long long *)&g.rem[v], (unsigned long long)(-radm));
+        avail = old < 0 ? 0 : (old < radm ? old : radm);
+      }
+      avail = __shfl(avail, 0, kWave);
+      long long before = 0;
+      for (uint64_t p0 = lo; p0 < hi && before < avail; p0 += kWave) {
+        const uint64_t p = p0 + lane();
+        long long ra = 0;
+        uint64_t s = 0;
+        if (p < hi) {
+          s = g.inc[p] - g.sbase;
+          const long long res = (long long)(g.k - 1) * U - g.x[s];
+          if (res > 0 && g.hc[s / g.k] < hvv) ra = res;
+        }
+        long long incl = ra;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const long long t = __shfl_up(incl, off, kWave);
+          if ((int)lane() >= off) incl += t;
+        }
+        const long long start = before + incl - ra;
+        long long take = avail - start;
+        take = take < 0 ? 0 : (take < ra ? take : ra);
+        if (take > 0) {
+          g.x[s] += take;
+          atomicAdd((unsigned long long *)&g.exc[s / g.k], (unsigned long long)take);
+        }
+        before += __shfl(incl, kWave - 1, kWave);
+      }
+    }
+    gsync(ctl);
+    // 4. remainders become excess; relabel vertices that kept some
+    for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
+      const uint32_t v = g.list[0][i];
+      long long r = g.rem[v];
+      if (r < 0) r = 0;
+      g.exv[v] = r;
+      if (r > 0) {
+        const int hm = g.hmin[v];
+        g.hv[v] = hm >= g.INF - 1 ? g.INF : hm + 1;
+      }
+    }
+    gsync(ctl);
+    ++rounds;
+    ++since;
+    if (any == 0 || since >= kGridRelabel) {
+      active = global_relabel(g);
+      ++relabels;
+      since = 0;
+    }
+  }
+  if (!limit) global_relabel(g);
+  sink = wave_sum64(sink);
+  if (lane() == 0 && sink) atomicAdd(&ctl->sink, sink);
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
+    S.reach[v] = g.hv[v] < g.INF ? 1 : 0;
+  if (g.leader) {
+    ctl->rounds = rounds;
+    ctl->limit = limit ? 1 : 0;
+    ctl->relabels = relabels;
+  }
+}
+
+__global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
+  ctl->bar_count = 0;
+  ctl->bar_gen = 0;
+  ctl->lpack[0] = ctl->lpack[1] = 0;
+  ctl->cqn[0] = ctl->cqn[1] = 0;
+  ctl->active = 0;
+  ctl->sink = 0;
+}
+
+}  // namespace
+
+hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
+                         long long cap_t, long long unit, uint32_t max_rounds,
+                         int device, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_ctl_init, dim3(1), dim3(1), 0, s, S.ctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int per_cu = 0, cus = 0;
+  const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+           &per_cu, reinterpret_cast<const void *>(k_grid_flow), 256, lds)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                 device)) != hipSuccess)
+    return e;
+  if (per_cu < 1) return hipErrorLaunchFailure;
+  const unsigned grid = (unsigned)(cus * (per_cu < 2 ? per_cu : 2));
+  FlowProblem p = P;
+  GridFlowState st = S;
+  void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_grid_flow),
+                                    dim3(grid), dim3(256), args, lds, s);
+}
+
+}  // namespace cds

@@ -103,6 +103,23 @@ __global__ void k_uf_union(uint64_t T, uint32_t k,
   }
 }
 
+// Unions the endpoints of every edge marked as covered by a core clique
+// (the pruned graph): one wave per core vertex, each edge once (a < b).
+__global__ __launch_bounds__(256) void k_uf_union_edges(
+    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+    const uint8_t *__restrict__ mark, const uint32_t *__restrict__ core,
+    uint32_t kp, uint32_t *__restrict__ p) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       a < n; a += waves) {
+    if (core[a] < kp) continue;
+    for (uint32_t j = off[a] + lane(); j < off[a + 1]; j += kWave) {
+      const uint32_t b = adj[j];
+      if (b > a && mark[j]) uf_unite(p, a, b);
+    }
+  }
+}
+
 __global__ void k_uf_flatten(uint32_t n, uint32_t *__restrict__ p) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x)
@@ -138,6 +155,153 @@ __global__ void k_count_nonzero_u8(size_t n, const uint8_t *__restrict__ a,
     c += a[i] != 0;
   c = wave_sum64(c);
   if (lane() == 0 && c) atomicAdd(out, c);
+}
+
+// Cliques of each component (wave-aggregated counters).
+__global__ void k_comp_count2(uint64_t T, const uint32_t *__restrict__ cl,
+                              uint32_t k, const uint32_t *__restrict__ kill,
+                              uint32_t kp, const uint32_t *__restrict__ comp_of,
+                              uint32_t *__restrict__ ccount) {
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = base + threadIdx.x;
+    const bool in = c < T && kill[c] >= kp;
+    const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
+    (void)aggregated_reserve(ccount, comp, in);
+  }
+}
+
+// Places every core clique in its component's range of the flow arrays:
+// fmem (flow-vertex ids of the members) and cslot[c] = its flow index.
+__global__ void k_comp_scatter2(uint64_t T, uint32_t k,
+                                const uint32_t *__restrict__ cl,
+                                const uint32_t *__restrict__ kill, uint32_t kp,
+                                const uint32_t *__restrict__ comp_of,
+                                const uint32_t *__restrict__ fv_of,
+                                const uint64_t *__restrict__ coff,
+                                uint32_t *__restrict__ ccur,
+                                uint32_t *__restrict__ fmem,
+                                uint32_t *__restrict__ cslot) {
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = base + threadIdx.x;
+    const bool in = c < T && kill[c] >= kp;
+    const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
+    const uint32_t slot = aggregated_reserve(ccur, comp, in);
+    if (in) {
+      const uint64_t f = coff[comp] + slot;
+      cslot[c] = (uint32_t)f;
+      for (uint32_t j = 0; j < k; ++j) fmem[f * k + j] = fv_of[cl[c * k + j]];
+    }
+  }
+}
+
+// Chunked variants: an item is (vertex, incidence range) of at most a few
+// thousand entries, so hubs spread over many waves.  Pass 1 counts the
+// core cliques of each item; pass 2 writes each item's flow slots at a
+// host-scanned offset, in incidence order.
+__global__ __launch_bounds__(256) void k_item_core_count(
+    uint64_t nitems, const uint4 *__restrict__ items,
+    const uint32_t *__restrict__ inc, uint32_t ik,
+    const uint32_t *__restrict__ kill, uint32_t kp, uint32_t *__restrict__ cnt) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       it < nitems; it += waves) {
+    const uint4 t = items[it];
+    const uint64_t lo = ((uint64_t)t.y << 32) | t.z, hi = lo + t.w;
+    uint32_t d = 0;
+    for (uint64_t q = lo + lane(); q < hi; q += kWave) d += kill[inc[q * ik]] >= kp;
+    d = wave_sum(d);
+    if (lane() == 0) cnt[it] = d;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_item_core_write(
+    uint64_t nitems, const uint4 *__restrict__ items,
+    const uint64_t *__restrict__ item_out, const uint32_t *__restrict__ inc,
+    uint32_t ik, const uint32_t *__restrict__ kill, uint32_t kp,
+    const uint32_t *__restrict__ cl, uint32_t k,
+    const uint32_t *__restrict__ cslot, uint32_t *__restrict__ finc) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       it < nitems; it += waves) {
+    const uint4 t = items[it];
+    const uint32_t v = t.x;
+    const uint64_t lo = ((uint64_t)t.y << 32) | t.z, hi = lo + t.w;
+    uint64_t out = item_out[it];
+    for (uint64_t q0 = lo; q0 < hi; q0 += kWave) {
+      const uint64_t q = q0 + lane();
+      bool take = false;
+      uint32_t slot = 0;
+      if (q < hi) {
+        const uint32_t c = inc[q * ik];
+        if (kill[c] >= kp) {
+          take = true;
+          uint32_t j = 0;
+          while (j + 1 < k && cl[(uint64_t)c * k + j] != v) ++j;
+          slot = cslot[c] * k + j;
+        }
+      }
+      unsigned n_;
+      const unsigned p = wave_slot(take, &n_);
+      if (take) finc[out + p] = slot;
+      out += n_;
+    }
+  }
+}
+
+// Core clique degree of every core vertex: incidence entries whose clique
+// lies in the core.  One wave per vertex, no atomics.
+__global__ __launch_bounds__(256) void k_core_cd(
+    uint32_t n, const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
+    const uint32_t *__restrict__ kill, const uint32_t *__restrict__ core,
+    uint32_t kp, const uint32_t *__restrict__ fv_of, uint32_t *__restrict__ fcd) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       v < n; v += waves) {
+    if (core[v] < kp) continue;
+    uint32_t d = 0;
+    for (uint64_t q = ioff[v] + lane(); q < ioff[v + 1]; q += kWave)
+      d += kill[inc[q]] >= kp;
+    d = wave_sum(d);
+    if (lane() == 0) fcd[fv_of[v]] = d;
+  }
+}
+
+// Flow incidence of every core vertex (slot ids f*k + j of its core
+// cliques), written in incidence order by ballot compaction; no atomics.
+__global__ __launch_bounds__(256) void k_core_inc(
+    uint32_t n, uint32_t k, const uint64_t *__restrict__ ioff,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ kill,
+    const uint32_t *__restrict__ core, uint32_t kp,
+    const uint32_t *__restrict__ cl, const uint32_t *__restrict__ cslot,
+    const uint32_t *__restrict__ fv_of, const uint64_t *__restrict__ fioff,
+    uint32_t *__restrict__ finc) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       v < n; v += waves) {
+    if (core[v] < kp) continue;
+    uint64_t out = fioff[fv_of[v]];
+    const uint64_t e = ioff[v + 1];
+    for (uint64_t q0 = ioff[v]; q0 < e; q0 += kWave) {
+      const uint64_t q = q0 + lane();
+      bool take = false;
+      uint32_t slot = 0;
+      if (q < e) {
+        const uint32_t c = inc[q];
+        if (kill[c] >= kp) {
+          take = true;
+          uint32_t j = 0;
+          while (j + 1 < k && cl[(uint64_t)c * k + j] != v) ++j;
+          slot = cslot[c] * k + j;
+        }
+      }
+      unsigned cnt;
+      const unsigned pos = wave_slot(take, &cnt);
+      if (take) finc[out + pos] = slot;
+      out += cnt;
+    }
+  }
 }
 
 __global__ void k_comp_count(uint64_t T, uint32_t k,
@@ -227,6 +391,15 @@ hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
   return hipGetLastError();
 }
 
+hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                          const uint8_t *mark, const uint32_t *core,
+                          uint32_t kp, uint32_t *parent, hipStream_t s) {
+  hipLaunchKernelGGL(k_uf_union_edges,
+                     dim3(grid_for((size_t)n * kWave, 256, 16384)), dim3(256),
+                     0, s, n, off, adj, mark, core, kp, parent);
+  return hipGetLastError();
+}
+
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s) {
   hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 65535)), dim3(256), 0,
                      s, n, parent);
@@ -270,6 +443,71 @@ hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
   hipLaunchKernelGGL(k_comp_scatter, dim3(grid_for(T, 256, 65535)), dim3(256),
                      0, s, T, k, cliques, kill, kp, comp_of, fv_of, coff, ccur,
                      fmem);
+  return hipGetLastError();
+}
+
+hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
+                              const uint32_t *kill, uint32_t kp,
+                              const uint32_t *comp_of, const uint32_t *fv_of,
+                              const uint64_t *coff, uint32_t *ccur,
+                              uint32_t *fmem, uint32_t *cslot, hipStream_t s) {
+  hipLaunchKernelGGL(k_comp_scatter2, dim3(grid_for(T, 256, 65535)), dim3(256),
+                     0, s, T, k, cliques, kill, kp, comp_of, fv_of, coff, ccur,
+                     fmem, cslot);
+  return hipGetLastError();
+}
+
+hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
+                              const uint32_t *kill, uint32_t kp,
+                              const uint32_t *comp_of, uint32_t *ccount,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_comp_count2, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
+                     s, T, cliques, k, kill, kp, comp_of, ccount);
+  return hipGetLastError();
+}
+
+hipError_t item_core_count(uint64_t nitems, const uint4 *items,
+                           const uint32_t *inc, uint32_t ik,
+                           const uint32_t *kill, uint32_t kp, uint32_t *cnt,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_item_core_count,
+                     dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
+                     nitems, items, inc, ik, kill, kp, cnt);
+  return hipGetLastError();
+}
+
+hipError_t item_core_write(uint64_t nitems, const uint4 *items,
+                           const uint64_t *item_out, const uint32_t *inc,
+                           uint32_t ik, const uint32_t *kill, uint32_t kp,
+                           const uint32_t *cliques, uint32_t k,
+                           const uint32_t *cslot, uint32_t *finc,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_item_core_write,
+                     dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
+                     nitems, items, item_out, inc, ik, kill, kp, cliques, k,
+                     cslot, finc);
+  return hipGetLastError();
+}
+
+hipError_t core_clique_degree(uint32_t n, const uint64_t *ioff,
+                              const uint32_t *inc, const uint32_t *kill,
+                              const uint32_t *core, uint32_t kp,
+                              const uint32_t *fv_of, uint32_t *fcd,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_core_cd, dim3(grid_for((size_t)n * kWave, 256, 16384)),
+                     dim3(256), 0, s, n, ioff, inc, kill, core, kp, fv_of, fcd);
+  return hipGetLastError();
+}
+
+hipError_t core_flow_incidence(uint32_t n, uint32_t k, const uint64_t *ioff,
+                               const uint32_t *inc, const uint32_t *kill,
+                               const uint32_t *core, uint32_t kp,
+                               const uint32_t *cliques, const uint32_t *cslot,
+                               const uint32_t *fv_of, const uint64_t *fioff,
+                               uint32_t *finc, hipStream_t s) {
+  hipLaunchKernelGGL(k_core_inc, dim3(grid_for((size_t)n * kWave, 256, 16384)),
+                     dim3(256), 0, s, n, k, ioff, inc, kill, core, kp, cliques,
+                     cslot, fv_of, fioff, finc);
   return hipGetLastError();
 }
 

@@ -97,19 +97,48 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
                         uint32_t *core, uint32_t *kill, uint32_t *next,
                         uint32_t *ncount, unsigned long long *killed,
                         hipStream_t s);
+// Whole peel in one cooperative launch (see peel.hip).  Control block:
+struct PeelCtl {
+  unsigned int bar_count, bar_gen;
+  unsigned int gmin[2];
+  unsigned int alive_n[2];
+  unsigned long long fpack[3];  // per frontier buffer: count<<40 | chunks
+  unsigned int nlevels, status;
+  unsigned long long removed;
+  unsigned long long rounds;
+  // leader-thread time split (wall_clock64 ticks): barriers, alive scans,
+  // frontier chunk processing, LDS flushes
+  unsigned long long t_bar, t_scan, t_proc, t_flush;
+};
+// Scratch: u32 6n (two alive lists, three frontiers, alive positions),
+// u64 3n (chunk starts).
+// Level tables (capacity n, zeroed here): level, removed, killed.
+// Requires n < 2^24.  Synchronises nothing; read ctl after the stream.
+hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
+                           const uint32_t *inc, const uint32_t *cliques,
+                           uint32_t *deg, uint32_t *core, uint32_t *kill,
+                           uint32_t *scratch32, uint64_t *scratch64,
+                           PeelCtl *ctl, uint32_t *lv_level,
+                           unsigned long long *lv_removed,
+                           unsigned long long *lv_killed, int device,
+                           unsigned *grid_out, hipStream_t s);
+
 // Exact sequential replay of one peel phase (level `level`); see peel.hip.
 struct ReplayOut {
   unsigned long long best_num, best_den;  // max intermediate density
   uint32_t steps, found;
 };
+// Collects the level's vertices, compacts their phase-clique incidence into
+// pinc/pioff (pinc holds at least k * k * (cliques killed at the level)
+// words, pioff M+1 entries) and replays the phase with one workgroup.
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,
-                        const uint32_t *cliques, uint64_t T_start,
-                        uint64_t N_start, uint32_t M, uint32_t *pv,
-                        uint32_t *pdeg, uint32_t *posmap, uint8_t *palive,
-                        uint32_t *rdead, uint32_t *pcount, ReplayOut *out,
-                        hipStream_t s);
+                        uint64_t T_start, uint64_t N_start, uint32_t M,
+                        uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
+                        uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
+                        uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
+                        DeviceArena &arena, hipStream_t s);
 
 // ---- core / prune / components (glue.hip) ----
 hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
@@ -124,6 +153,10 @@ hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s);
 hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
                             const uint32_t *kill, uint32_t kp,
                             uint32_t *parent, hipStream_t s);
+// Unions the endpoints of the marked (clique-covered) edges of core vertices.
+hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                          const uint8_t *mark, const uint32_t *core,
+                          uint32_t kp, uint32_t *parent, hipStream_t s);
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s);
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *kill, uint32_t kp, const uint32_t *off,
@@ -143,6 +176,44 @@ hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
                                const uint32_t *fv_of, const uint64_t *coff,
                                uint32_t *ccur, uint32_t *fmem,
                                hipStream_t s);
+// Contention-free versions used by the engine:
+//  comp_clique_totals  ccount[comp] += 1 per core clique (wave-aggregated)
+//  comp_cliques_build  fmem / cslot (flow index of each core clique)
+//  core_clique_degree  fcd[fv] = core cliques containing the vertex
+//  core_flow_incidence finc (slot ids) per core vertex, incidence order
+hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
+                              const uint32_t *kill, uint32_t kp,
+                              const uint32_t *comp_of, uint32_t *ccount,
+                              hipStream_t s);
+hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
+                              const uint32_t *kill, uint32_t kp,
+                              const uint32_t *comp_of, const uint32_t *fv_of,
+                              const uint64_t *coff, uint32_t *ccur,
+                              uint32_t *fmem, uint32_t *cslot, hipStream_t s);
+// Chunked per-vertex incidence work.  items[i] = {vertex, first entry
+// (hi 32 bits), first entry (lo 32 bits), entry count}; incidence entries
+// are ik words wide with the clique id first.
+hipError_t item_core_count(uint64_t nitems, const uint4 *items,
+                           const uint32_t *inc, uint32_t ik,
+                           const uint32_t *kill, uint32_t kp, uint32_t *cnt,
+                           hipStream_t s);
+hipError_t item_core_write(uint64_t nitems, const uint4 *items,
+                           const uint64_t *item_out, const uint32_t *inc,
+                           uint32_t ik, const uint32_t *kill, uint32_t kp,
+                           const uint32_t *cliques, uint32_t k,
+                           const uint32_t *cslot, uint32_t *finc,
+                           hipStream_t s);
+hipError_t core_clique_degree(uint32_t n, const uint64_t *ioff,
+                              const uint32_t *inc, const uint32_t *kill,
+                              const uint32_t *core, uint32_t kp,
+                              const uint32_t *fv_of, uint32_t *fcd,
+                              hipStream_t s);
+hipError_t core_flow_incidence(uint32_t n, uint32_t k, const uint64_t *ioff,
+                               const uint32_t *inc, const uint32_t *kill,
+                               const uint32_t *core, uint32_t kp,
+                               const uint32_t *cliques, const uint32_t *cslot,
+                               const uint32_t *fv_of, const uint64_t *fioff,
+                               uint32_t *finc, hipStream_t s);
 hipError_t flow_incidence(uint64_t TT, uint32_t k, const uint32_t *fmem,
                           const uint64_t *fioff, uint32_t *ficur,
                           uint32_t *finc, hipStream_t s);
@@ -171,6 +242,32 @@ struct FlowState {
 // (all capacities multiplied by U).  Writes misc[0] = F (scaled), reach[].
 hipError_t max_flow(const FlowProblem &P, const FlowState &S, long long cap_t,
                     long long unit, uint32_t max_rounds, hipStream_t s);
+
+// Grid-wide max flow for large components (flow_grid.hip), one cooperative
+// launch; same answer as max_flow.
+struct GridFlowCtl {
+  unsigned int bar_count, bar_gen;
+  unsigned long long lpack[2];  // vertex work lists: count<<40 | chunks
+  unsigned int cqn[2];          // clique BFS frontier sizes
+  unsigned long long active;
+  unsigned long long sink;      // flow into the sink (scaled)
+  unsigned int rounds, limit, relabels, pad;
+};
+struct GridFlowState {
+  long long *x;            // T*k
+  long long *rvs, *exv, *rem;  // V each
+  long long *exc;          // T
+  int *hv, *hmin;          // V
+  int *hc;                 // T
+  uint32_t *list0, *list1; // V each
+  uint64_t *lchunk0, *lchunk1;  // V each
+  uint32_t *cq0, *cq1;     // T each
+  GridFlowCtl *ctl;
+  int *reach;              // V
+};
+hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
+                         long long cap_t, long long unit, uint32_t max_rounds,
+                         int device, hipStream_t s);
 
 // Cliques of a component whose members all have reach != 0.
 hipError_t count_cliques_in_reach(uint32_t T, uint32_t k, const uint32_t *mem,

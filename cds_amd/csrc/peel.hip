@@ -24,18 +24,30 @@
 namespace cds {
 namespace {
 
+// Incidence entries are k words: the clique id, then its other k-1
+// members, so the peel streams the members instead of gathering them.
+// Slots are reserved with wave-aggregated cursor increments: consecutive
+// cliques share their lowest vertex (they come from the same listing item)
+// and hubs recur, so most lanes of a wave share one atomic.
 __global__ void k_build_inc(uint64_t T, uint32_t k,
                             const uint32_t *__restrict__ cl,
                             const uint64_t *__restrict__ ioff,
                             uint32_t *__restrict__ icur,
                             uint32_t *__restrict__ inc) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x)
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = base + threadIdx.x;
+    const bool in = c < T;
     for (uint32_t j = 0; j < k; ++j) {
-      const uint32_t x = cl[c * k + j];
-      const uint32_t pos = atomicAdd(&icur[x], 1u);
-      inc[ioff[x] + pos] = (uint32_t)c;
+      const uint32_t x = in ? cl[c * k + j] : 0;
+      const uint32_t pos = aggregated_reserve(icur, x, in);
+      if (!in) continue;
+      uint32_t *e = inc + (ioff[x] + pos) * k;
+      e[0] = (uint32_t)c;
+      for (uint32_t t = 0, o = 1; t < k; ++t)
+        if (t != j) e[o++] = cl[c * k + t];
     }
+  }
 }
 
 __global__ void k_peel_init_v(uint32_t n, const uint64_t *__restrict__ cd,
@@ -100,12 +112,14 @@ __global__ __launch_bounds__(256) void k_peel_process(
     const uint32_t v = frontier[f];
     const uint64_t e = ioff[v + 1];
     for (uint64_t q = ioff[v] + lane(); q < e; q += kWave) {
-      const uint32_t c = inc[q];
+      const uint32_t *en = inc + q * k;
+      const uint32_t c = en[0];
+      if (kill[c] != kAlive) continue;
       if (atomicCAS(&kill[c], kAlive, level) != kAlive) continue;
       ++nk;
-      for (uint32_t t = 0; t < k; ++t) {
-        const uint32_t u = cl[(uint64_t)c * k + t];
-        if (u == v || core[u] != kAlive) continue;
+      for (uint32_t t = 1; t < k; ++t) {
+        const uint32_t u = en[t];
+        if (core[u] != kAlive) continue;
         const uint32_t old = atomicSub(&deg[u], 1u);
         if (old == level + 1) {
           core[u] = level;
@@ -116,6 +130,257 @@ __global__ __launch_bounds__(256) void k_peel_process(
   }
   nk = wave_sum(nk);
   if (lane() == 0 && nk) atomicAdd(killed, (unsigned long long)nk);
+}
+
+// ---- persistent peel: every level in one cooperative launch ----
+//
+// All workgroups are co-resident (hipLaunchCooperativeKernel), so a grid
+// barrier is a counter plus a generation word.  Per level: (1) min degree
+// over the alive list, (2) the frontier (alive, degree <= level) and the
+// compacted alive list for the next level, (3) sub-rounds of frontier
+// processing until no vertex drops to the level.  Frontier work is split
+// into chunks of kPeelChunk incidence entries; one packed 64-bit atomic per
+// appended vertex (vertex count << 40 | chunk count) gives every vertex its
+// slot and its first chunk, so chunk i maps to its vertex by a binary search
+// and a hub's incidence list is spread over many waves.
+constexpr uint32_t kPeelChunk = 256;
+constexpr uint64_t kChunkMask = (1ull << 40) - 1;
+constexpr uint32_t kPeelLdsCap = 8192;  // alive vertices aggregated in LDS
+
+__device__ __forceinline__ void grid_sync(PeelCtl *ctl) {
+  // The workgroup barrier makes every wave's stores reach L2; one thread
+  // per workgroup then releases them device-wide (the fence writes back
+  // this XCD's L2) and, after the arrival, acquires (invalidates stale
+  // lines) for the whole workgroup, whose waves share the CU's caches.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t0 = wall_clock64();
+    __threadfence();
+    const unsigned gen =
+        __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(
+        &ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE,
+                               __HIP_MEMORY_SCOPE_AGENT) == gen)
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+    if (blockIdx.x == 0) ctl->t_bar += wall_clock64() - t0;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t ld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t chunks_of(uint64_t len) {
+  return (len + kPeelChunk - 1) / kPeelChunk;
+}
+
+// Appends v (with its chunk count) to frontier buffer b.
+__device__ __forceinline__ void frontier_push(PeelCtl *ctl, int b, uint32_t *F,
+                                              uint64_t *FC, uint32_t v,
+                                              uint64_t nch) {
+  const unsigned long long old =
+      atomicAdd(&ctl->fpack[b], (1ull << 40) + (unsigned long long)nch);
+  const uint32_t slot = (uint32_t)(old >> 40);
+  F[slot] = v;
+  FC[slot] = old & kChunkMask;
+}
+
+__global__ __launch_bounds__(256) void k_peel_persistent(
+    uint32_t n, uint32_t k, const uint64_t *__restrict__ ioff,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ cl,
+    uint32_t *deg, uint32_t *core, uint32_t *kill, uint32_t *A0, uint32_t *A1,
+    uint32_t *F0, uint32_t *F1, uint32_t *F2, uint64_t *FC0, uint64_t *FC1,
+    uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
+    unsigned long long *lv_removed, unsigned long long *lv_killed,
+    uint32_t *pos) {
+  // Once few vertices remain alive, degree decrements are summed per
+  // workgroup in LDS (indexed by position in the alive list) and applied
+  // with one atomic per vertex and sub-round; the vertex whose aggregated
+  // decrement crosses level+1 -> level joins the next sub-frontier.
+  __shared__ uint32_t lds_cnt[kPeelLdsCap];
+  for (uint32_t i = threadIdx.x; i < kPeelLdsCap; i += blockDim.x) lds_cnt[i] = 0;
+  __syncthreads();
+  uint32_t *A[2] = {A0, A1};
+  uint32_t *F[3] = {F0, F1, F2};
+  uint64_t *FC[3] = {FC0, FC1, FC2};
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gthreads = gridDim.x * blockDim.x;
+  const uint32_t gwave = gtid / kWave, gwaves = gthreads / kWave;
+  const bool leader = gtid == 0;
+  int ab = 0, gp = 0, fb = 0;
+  uint32_t lvl_idx = 0;
+  for (;;) {
+    // (1) minimum degree among alive vertices
+    unsigned long long ts = leader ? wall_clock64() : 0;
+    const uint32_t na = ld(&ctl->alive_n[ab]);
+    uint32_t m = kAlive;
+    for (uint32_t i = gtid; i < na; i += gthreads) {
+      const uint32_t v = A[ab][i];
+      if (ld(&core[v]) == kAlive) m = min(m, ld(&deg[v]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off, kWave));
+    if (lane() == 0 && m != kAlive) atomicMin(&ctl->gmin[gp], m);
+    if (leader) {
+      ctl->alive_n[1 - ab] = 0;
+      ctl->t_scan += wall_clock64() - ts;
+    }
+    grid_sync(ctl);
+    const uint32_t level = ld(&ctl->gmin[gp]);
+    if (level == kAlive) break;
+    if (leader) {
+      ctl->gmin[1 - gp] = kAlive;
+      ts = wall_clock64();
+    }
+    // (2) frontier + compacted alive list
+    for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gthreads) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t v = 0;
+      bool alive = false, take = false;
+      if (i < na) {
+        v = A[ab][i];
+        alive = ld(&core[v]) == kAlive;
+        take = alive && ld(&deg[v]) <= level;
+      }
+      if (take) {
+        core[v] = level;
+        frontier_push(ctl, fb, F[fb], FC[fb], v, chunks_of(ioff[v + 1] - ioff[v]));
+      }
+      const bool keep = alive && !take;
+      unsigned cnt;
+      const unsigned slot = wave_slot(keep, &cnt);
+      uint32_t start = 0;
+      if (lane() == 0 && cnt) start = atomicAdd(&ctl->alive_n[1 - ab], cnt);
+      start = __shfl(start, 0, kWave);
+      if (keep) {
+        A[1 - ab][start + slot] = v;
+        pos[v] = start + slot;
+      }
+    }
+    if (leader) ctl->t_scan += wall_clock64() - ts;
+    grid_sync(ctl);
+    const uint32_t na_next = ld(&ctl->alive_n[1 - ab]);
+    const bool agg = na_next <= kPeelLdsCap;
+    // (3) sub-rounds
+    unsigned long long removed = 0;
+    for (;;) {
+      const unsigned long long pack = __hip_atomic_load(
+          &ctl->fpack[fb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t fcount = (uint32_t)(pack >> 40);
+      const uint64_t nchunks = pack & kChunkMask;
+      if (fcount == 0) break;
+      removed += fcount;
+      const int nb = (fb + 1) % 3, rb = (fb + 2) % 3;
+      if (leader) ctl->fpack[rb] = 0;  // used as "next" one sub-round later
+      const uint32_t *Fc = F[fb];
+      const uint64_t *FCc = FC[fb];
+      uint32_t nk = 0;
+      unsigned long long tp = leader ? wall_clock64() : 0;
+      for (uint64_t ci = gwave; ci < nchunks; ci += gwaves) {
+        uint32_t lo = 0, hi = fcount - 1;  // last slot with FC <= ci
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (FCc[mid] <= ci)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const uint32_t v = Fc[lo];
+        const uint64_t b = ioff[v] + (ci - FCc[lo]) * kPeelChunk;
+        const uint64_t e = min(ioff[v + 1], b + kPeelChunk);
+        for (uint64_t q = b + lane(); q < e; q += kWave) {
+          const uint32_t *en = inc + q * k;
+          const uint32_t c = en[0];
+          if (ld(&kill[c]) != kAlive) continue;  // plain load before the CAS
+          if (atomicCAS(&kill[c], kAlive, level) != kAlive) continue;
+          ++nk;
+          for (uint32_t t = 1; t < k; ++t) {
+            const uint32_t u = en[t];
+            if (ld(&core[u]) != kAlive) continue;
+            if (agg) {
+              atomicAdd(&lds_cnt[pos[u]], 1u);
+              continue;
+            }
+            const uint32_t old = atomicSub(&deg[u], 1u);
+            if (old == level + 1) {
+              core[u] = level;
+              frontier_push(ctl, nb, F[nb], FC[nb], u,
+                            chunks_of(ioff[u + 1] - ioff[u]));
+            }
+          }
+        }
+      }
+      if (leader) {
+        const unsigned long long now = wall_clock64();
+        ctl->t_proc += now - tp;
+        tp = now;
+      }
+      if (agg) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < na_next; i += blockDim.x) {
+          const uint32_t cnt = lds_cnt[i];
+          if (!cnt) continue;
+          lds_cnt[i] = 0;
+          const uint32_t u = A[1 - ab][i];
+          const uint32_t old = atomicSub(&deg[u], cnt);
+          if (old > level && old - cnt <= level) {
+            core[u] = level;
+            frontier_push(ctl, nb, F[nb], FC[nb], u,
+                          chunks_of(ioff[u + 1] - ioff[u]));
+          }
+        }
+        __syncthreads();
+        if (leader) ctl->t_flush += wall_clock64() - tp;
+      }
+      nk = wave_sum(nk);
+      if (lane() == 0 && nk) atomicAdd(&lv_killed[lvl_idx], (unsigned long long)nk);
+      grid_sync(ctl);
+      fb = nb;
+      if (leader) ctl->rounds++;
+    }
+    if (leader) {
+      lv_level[lvl_idx] = level;
+      lv_removed[lvl_idx] = removed;
+      ctl->removed += removed;
+      ctl->fpack[fb] = 0;
+    }
+    ++lvl_idx;
+    ab = 1 - ab;
+    gp = 1 - gp;
+    grid_sync(ctl);
+  }
+  if (leader) ctl->nlevels = lvl_idx;
+}
+
+__global__ void k_iota(uint32_t n, uint32_t *a) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    a[i] = i;
+}
+
+__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t n) {
+  ctl->bar_count = 0;
+  ctl->bar_gen = 0;
+  ctl->gmin[0] = kAlive;
+  ctl->gmin[1] = kAlive;
+  ctl->alive_n[0] = n;
+  ctl->alive_n[1] = 0;
+  ctl->fpack[0] = ctl->fpack[1] = ctl->fpack[2] = 0;
+  ctl->nlevels = 0;
+  ctl->status = 0;
+  ctl->removed = 0;
+  ctl->rounds = 0;
+  ctl->t_bar = ctl->t_scan = ctl->t_proc = ctl->t_flush = 0;
 }
 
 // ---- exact replay of one level in the oracle's removal order ----
@@ -142,7 +407,7 @@ __global__ void k_collect_level(uint32_t n, uint32_t level,
 }
 
 __global__ __launch_bounds__(256) void k_phase_deg(
-    uint32_t level, const uint32_t *__restrict__ count,
+    uint32_t k, uint32_t level, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ pv, const uint64_t *__restrict__ ioff,
     const uint32_t *__restrict__ inc, const uint32_t *__restrict__ kill,
     uint32_t *__restrict__ pdeg, uint8_t *__restrict__ palive) {
@@ -153,11 +418,37 @@ __global__ __launch_bounds__(256) void k_phase_deg(
     const uint32_t v = pv[i];
     uint32_t d = 0;
     for (uint64_t q = ioff[v] + lane(); q < ioff[v + 1]; q += kWave)
-      d += kill[inc[q]] == level;
+      d += kill[inc[q * k]] == level;
     d = wave_sum(d);
     if (lane() == 0) {
       pdeg[i] = d;
       palive[i] = 1;
+    }
+  }
+}
+
+// Copies each phase vertex's phase-clique incidence entries (kill ==
+// level) to pinc at pioff[i], in incidence order (ballot compaction).
+__global__ __launch_bounds__(256) void k_phase_compact(
+    uint32_t k, uint32_t level, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ pv, const uint64_t *__restrict__ ioff,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ kill,
+    const uint64_t *__restrict__ pioff, uint32_t *__restrict__ pinc) {
+  const uint32_t M = *count;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       i < M; i += waves) {
+    const uint32_t v = pv[i];
+    uint64_t out = pioff[i];
+    const uint64_t e = ioff[v + 1];
+    for (uint64_t q0 = ioff[v]; q0 < e; q0 += kWave) {
+      const uint64_t q = q0 + lane();
+      const bool take = q < e && kill[inc[q * k]] == level;
+      unsigned cnt;
+      const unsigned slot = wave_slot(take, &cnt);
+      if (take)
+        for (uint32_t t = 0; t < k; ++t) pinc[(out + slot) * k + t] = inc[q * k + t];
+      out += cnt;
     }
   }
 }
@@ -181,9 +472,8 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay(
     uint32_t k, uint32_t level, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ pv, uint32_t *__restrict__ pdeg,
     const uint32_t *__restrict__ posmap, uint8_t *__restrict__ palive,
-    const uint32_t *__restrict__ core, const uint32_t *__restrict__ kill,
-    const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
-    const uint32_t *__restrict__ cl, uint32_t *__restrict__ rdead,
+    const uint32_t *__restrict__ core, const uint64_t *__restrict__ pioff,
+    const uint32_t *__restrict__ pinc, uint32_t *__restrict__ rdead,
     uint64_t T_start, uint64_t N_start, ReplayOut *out) {
   __shared__ uint64_t s_wmin[kReplayThreads / kWave];
   __shared__ uint64_t s_key;
@@ -224,14 +514,17 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay(
         }
       }
     }
-    const uint64_t e = ioff[v + 1];
-    for (uint64_t q = ioff[v] + tid; q < e; q += kReplayThreads) {
-      const uint32_t c = inc[q];
-      if (kill[c] != level || rdead[c] == level + 1) continue;
+    // v's phase cliques (compacted beforehand), those not killed yet
+    const uint32_t iv = posmap[v];
+    const uint64_t e = pioff[iv + 1];
+    for (uint64_t q = pioff[iv] + tid; q < e; q += kReplayThreads) {
+      const uint32_t *en = pinc + q * k;
+      const uint32_t c = en[0];
+      if (rdead[c] == level + 1) continue;
       rdead[c] = level + 1;
-      for (uint32_t t = 0; t < k; ++t) {
-        const uint32_t u = cl[(uint64_t)c * k + t];
-        if (u != v && core[u] == level) atomicSub(&pdeg[posmap[u]], 1u);
+      for (uint32_t t = 1; t < k; ++t) {
+        const uint32_t u = en[t];
+        if (core[u] == level) atomicSub(&pdeg[posmap[u]], 1u);
       }
     }
     __syncthreads();
@@ -296,26 +589,71 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
   return hipGetLastError();
 }
 
+hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
+                           const uint32_t *inc, const uint32_t *cliques,
+                           uint32_t *deg, uint32_t *core, uint32_t *kill,
+                           uint32_t *scratch32, uint64_t *scratch64,
+                           PeelCtl *ctl, uint32_t *lv_level,
+                           unsigned long long *lv_removed,
+                           unsigned long long *lv_killed, int device,
+                           unsigned *grid_out, hipStream_t s) {
+  if (n >= (1u << 24)) return hipErrorInvalidValue;
+  hipError_t e;
+  if ((e = hipMemsetAsync(lv_level, 0, (size_t)n * 4, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(lv_removed, 0, (size_t)n * 8, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(lv_killed, 0, (size_t)n * 8, s)) != hipSuccess) return e;
+  uint32_t *A0 = scratch32, *A1 = A0 + n, *F0 = A1 + n, *F1 = F0 + n,
+           *F2 = F1 + n, *pos = F2 + n;
+  uint64_t *FC0 = scratch64, *FC1 = FC0 + n, *FC2 = FC1 + n;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A0);
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  int per_cu = 0, cus = 0;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+           &per_cu, reinterpret_cast<const void *>(k_peel_persistent), 256, 0)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                 device)) != hipSuccess)
+    return e;
+  if (per_cu < 1) return hipErrorLaunchFailure;
+  unsigned grid = (unsigned)(cus * (per_cu < 2 ? per_cu : 2));
+  if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {
+    const unsigned want = (unsigned)atoi(gb);
+    if (want >= 1 && want < grid) grid = want;
+  }
+  if (grid_out) *grid_out = grid;
+  void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
+                  &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
+                  &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_peel_persistent),
+                                    dim3(grid), dim3(256), args, 0, s);
+}
+
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,
-                        const uint32_t *cliques, uint64_t T_start,
-                        uint64_t N_start, uint32_t M, uint32_t *pv,
-                        uint32_t *pdeg, uint32_t *posmap, uint8_t *palive,
-                        uint32_t *rdead, uint32_t *pcount, ReplayOut *out,
-                        hipStream_t s) {
+                        uint64_t T_start, uint64_t N_start, uint32_t M,
+                        uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
+                        uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
+                        uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
+                        DeviceArena &arena, hipStream_t s) {
   hipError_t e = hipMemsetAsync(pcount, 0, sizeof(uint32_t), s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_collect_level, dim3(grid_for(n, 256, 8192)), dim3(256),
                      0, s, n, level, core, pv, posmap, pcount);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_phase_deg, dim3(grid_for((size_t)M * kWave, 256, 8192)),
-                     dim3(256), 0, s, level, pcount, pv, ioff, inc, kill, pdeg,
-                     palive);
+  const unsigned g = grid_for((size_t)M * kWave, 256, 8192);
+  hipLaunchKernelGGL(k_phase_deg, dim3(g), dim3(256), 0, s, k, level, pcount,
+                     pv, ioff, inc, kill, pdeg, palive);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32(pdeg, pioff, M, arena)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_phase_compact, dim3(g), dim3(256), 0, s, k, level,
+                     pcount, pv, ioff, inc, kill, pioff, pinc);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_replay, dim3(1), dim3(kReplayThreads), 0, s, k, level,
-                     pcount, pv, pdeg, posmap, palive, core, kill, ioff, inc,
-                     cliques, rdead, T_start, N_start, out);
+                     pcount, pv, pdeg, posmap, palive, core, pioff, pinc,
+                     rdead, T_start, N_start, out);
   return hipGetLastError();
 }
 

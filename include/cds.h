@@ -126,6 +126,16 @@ typedef struct {
   double t_list_count_ms;           /* listing count kernel (HIP events) */
   double t_list_write_ms;           /* listing write kernel (HIP events) */
   uint64_t peak_device_bytes;       /* engine arena peak during the solve */
+  uint32_t peel_grid;               /* workgroups of the persistent peel */
+  uint32_t flow_launches;           /* exact max flows computed */
+  uint32_t density_certificates;    /* components proven their own densest */
+  uint32_t peel_distinct_levels;    /* distinct clique-core values */
+  uint64_t peel_alive_scanned;      /* sum over levels of alive vertices */
+  double peel_split_ms[4];          /* one peel thread: barrier, scans,
+                                       chunk processing, LDS flushes */
+  double t_incidence_ms;            /* clique-incidence build (HIP events) */
+  double t_peel_kernel_ms;          /* persistent peel launch (HIP events) */
+  double t_replay_ms;               /* exact phase replays (HIP events) */
   cds_times times;
 } cds_result;
 
@@ -177,6 +187,15 @@ void cds_result_free(cds_result *r);
 cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *dg, uint32_t k,
                              uint64_t *total, uint64_t *clique_degree,
                              double *kernel_ms);
+
+/* Clique-core stage only: listing, peel and densest-core extraction
+ * (cliqueCoreDecompose + generateDensestCore, cudaCode/main.cu:252-399;
+ * oracle cliqueCoreDecompose + locateDensestCore,
+ * cpuCode/c++/src/helpers.cpp:16-150, :710-788).  Fills the result up to
+ * core_size / core_edges / core_cliques / deleted_cliques (and the vertex
+ * arrays when asked); the flow fields stay zero. */
+cds_status cds_core_decompose(cds_engine *e, const cds_dgraph *dg,
+                              const cds_params *p, cds_result *out);
 
 /* Batched plain path: many independent graphs, solved with shared launches.
  * results[i] is identical to cds_solve on graphs[i] with params[i]. */
