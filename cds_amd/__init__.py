@@ -82,7 +82,7 @@ class CResult(ctypes.Structure):
         ("trace_alpha", f32p), ("trace_component", u32p),
         ("trace_margin", f64p), ("flow_networks_solved", ctypes.c_uint32),
         ("flow_sweeps", ctypes.c_uint64), ("peel_levels", ctypes.c_uint32),
-        ("replay_phases", ctypes.c_uint32),
+        ("replay_phases", ctypes.c_uint32), ("replay_grid_phases", ctypes.c_uint32),
         ("oracle_quirk_flags", ctypes.c_uint32), ("dag_arcs", ctypes.c_uint64),
         ("list_work", ctypes.c_uint64), ("max_out_degree", ctypes.c_uint32),
         ("list_items", ctypes.c_uint64), ("t_list_count_ms", ctypes.c_double),
@@ -94,7 +94,8 @@ class CResult(ctypes.Structure):
         ("peel_alive_scanned", ctypes.c_uint64),
         ("peel_split_ms", ctypes.c_double * 4),
         ("t_incidence_ms", ctypes.c_double), ("t_peel_kernel_ms", ctypes.c_double),
-        ("t_replay_ms", ctypes.c_double), ("times", CTimes)]
+        ("t_replay_ms", ctypes.c_double), ("t_replay_prep_ms", ctypes.c_double),
+        ("times", CTimes)]
 
 
 class COpts(ctypes.Structure):

@@ -97,6 +97,39 @@ __device__ __forceinline__ int64_t find_sorted64(const uint32_t *a, int64_t len,
   return (lo < len && a[lo] == key) ? lo : -1;
 }
 
+// Barrier across all workgroups of a cooperative launch (B: GridBarrier).
+// Arrivals go to one of eight group counters (blockIdx % 8); the last
+// arrival of a group bumps the top counter and the last group flips the
+// generation, so no counter sees more than gridDim/8 + 8 contending
+// atomics.  One thread per workgroup releases the workgroup's stores
+// device-wide before arriving (the fence writes back this XCD's L2) and
+// acquires after the flip; the workgroup barriers around it order the
+// other waves, which share the CU's caches.
+template <class B>
+__device__ __forceinline__ void grid_barrier(B *b) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned gen =
+        __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned grp = blockIdx.x & 7u;
+    const unsigned gsz = (gridDim.x + 7u - grp) >> 3;
+    const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
+    const unsigned a = __hip_atomic_fetch_add(&b->sub[grp * 32], 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if ((a + 1u) % gsz == 0u) {
+      const unsigned t = __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+      if ((t + 1u) % groups == 0u)
+        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+      __builtin_amdgcn_s_sleep(1);
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 65535u * 16u) {
   size_t g = (work + block - 1) / block;
   if (g < 1) g = 1;

@@ -36,6 +36,9 @@ struct cds_engine {
   // (CDS_GRID_FLOW_NODES overrides it, e.g. to exercise that kernel on
   // small graphs)
   uint64_t grid_nodes = 1ull << 17;
+  // phases whose compacted incidence holds at least this many entries per
+  // vertex are replayed on the whole GPU (CDS_REPLAY_GRID_WORK overrides)
+  uint64_t replay_grid_work = 2048;
 };
 
 struct cds_dgraph {
@@ -89,7 +92,7 @@ template <class T> struct DBuf {
 };
 
 struct Timer {
-  hipEvent_t ev[24];
+  hipEvent_t ev[24];  // 22, 23: spare pair for sub-stage timing
   int used = 0;
   hipStream_t s;
   explicit Timer(hipStream_t st) : s(st) {
@@ -99,7 +102,7 @@ struct Timer {
     for (auto &e : ev) (void)hipEventDestroy(e);
   }
   int mark() {
-    if (used >= 24) return 23;  // out of events: later marks alias the last
+    if (used >= 22) return 21;  // out of events: later marks alias the last
     (void)hipEventRecord(ev[used], s);
     return used++;
   }
@@ -209,6 +212,8 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
   }
   e->arena = new DeviceArena(e->stream);
   if (const char *gn = getenv("CDS_GRID_FLOW_NODES")) e->grid_nodes = strtoull(gn, nullptr, 10);
+  if (const char *rw = getenv("CDS_REPLAY_GRID_WORK"))
+    e->replay_grid_work = strtoull(rw, nullptr, 10);
   *out = e;
   return CDS_OK;
 }
@@ -450,6 +455,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint32_t> icur(&A), inc(&A), deg(&A), core(&A), kill(&A), fa(&A),
       fb(&A), cnt(&A);
   DBuf<unsigned long long> killed(&A);
+  DBuf<uint32_t> sdeg(&A);  // degree at the start of the removing level
   HIPCHK(ioff.alloc((size_t)n + 1));
   HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
   HIPCHK(icur.alloc(n));
@@ -479,11 +485,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(lrem.alloc(n));
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
+    HIPCHK(sdeg.alloc(n));
     unsigned grid = 0;
     const int ev_p0 = tm.mark();
     HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
                            kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
-                           e->device, &grid, st));
+                           sdeg.p, e->device, &grid, st));
     const int ev_p1 = tm.mark();
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));
@@ -613,14 +620,32 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         HIPCHK(rout.alloc(1));
         allocated = true;
       }
-      DBuf<uint32_t> pinc(&A);
-      HIPCHK(pinc.alloc(lv.killed * k * k));
+      DBuf<uint32_t> pinc(&A), partial(&A);
+      DBuf<GridBarrier> gbar(&A);
+      // Few vertices with many cliques each: replay on the whole GPU.
+      unsigned rgrid = 0;
+      if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
+        rgrid = replay_grid_blocks(e->device);
+      if (rgrid) {
+        HIPCHK(partial.alloc((size_t)rgrid * M));
+        HIPCHK(gbar.alloc(1));
+        R->replay_grid_phases++;
+      } else {
+        HIPCHK(pinc.alloc(lv.killed * k * k));
+      }
+      HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
                           T - kl_before[ii], n - rm_before[ii], (uint32_t)M,
                           pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
-                          pioff.p, pinc.p, rout.p, A, st));
+                          pioff.p, pinc.p, rout.p, partial.p, gbar.p, rgrid,
+                          sdeg.p, A, tm.ev[23], st));
       ReplayOut ro;
       CHK(readback(e, &ro, rout.p, 1));
+      {
+        float prep = 0;
+        (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
+        R->t_replay_prep_ms += prep;
+      }
       R->replay_phases++;
       if (ro.steps != M)
         return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",

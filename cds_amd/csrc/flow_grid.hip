@@ -37,30 +37,7 @@ constexpr uint32_t kGridRelabel = 24;
 constexpr uint64_t kMask40 = (1ull << 40) - 1;
 
 __device__ __forceinline__ void gsync(GridFlowCtl *ctl) {
-  // The workgroup barrier makes every wave's stores reach L2; one thread
-  // per workgroup then releases them device-wide (the fence writes back
-  // this XCD's L2) and, after the arrival, acquires (invalidates stale
-  // lines) for the whole workgroup, whose waves share the CU's caches.
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned gen =
-        __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(
-        &ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
-      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE,
-                               __HIP_MEMORY_SCOPE_AGENT) == gen)
-        __builtin_amdgcn_s_sleep(1);
-    }
-    __threadfence();
-  }
-  __syncthreads();
+  grid_barrier(&ctl->bar);
 }
 
 template <class T> __device__ __forceinline__ T ald(const T *p) {
@@ -435,8 +412,8 @@ __global__ __launch_bounds__(256) void k_grid_flow(FlowProblem P,
 }
 
 __global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
-  ctl->bar_count = 0;
-  ctl->bar_gen = 0;
+  unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
+  for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
   ctl->lpack[0] = ctl->lpack[1] = 0;
   ctl->cqn[0] = ctl->cqn[1] = 0;
   ctl->active = 0;

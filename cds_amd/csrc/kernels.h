@@ -98,11 +98,21 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
                         uint32_t *ncount, unsigned long long *killed,
                         hipStream_t s);
 // Whole peel in one cooperative launch (see peel.hip).  Control block:
+// Grid barrier state of a cooperative launch: a generation word and
+// arrival counters (one per group of workgroups blockIdx % 8, plus one for
+// the groups), each in its own 128-byte line.  Counters only grow; zero
+// them before the launch.
+struct GridBarrier {
+  unsigned int gen, pad0[31];
+  unsigned int top, pad1[31];
+  unsigned int sub[8 * 32];
+};
+
 struct PeelCtl {
-  unsigned int bar_count, bar_gen;
-  unsigned int gmin[2];
-  unsigned int alive_n[2];
-  unsigned long long fpack[3];  // per frontier buffer: count<<40 | chunks
+  GridBarrier bar;
+  unsigned int gmin[2], pad0[30];
+  unsigned int alive_n[2], pad1[30];
+  unsigned long long fpack[3], pad2[13];  // per frontier: count<<40 | chunks
   unsigned int nlevels, status;
   unsigned long long removed;
   unsigned long long rounds;
@@ -113,6 +123,7 @@ struct PeelCtl {
 // Scratch: u32 6n (two alive lists, three frontiers, alive positions),
 // u64 3n (chunk starts).
 // Level tables (capacity n, zeroed here): level, removed, killed.
+// sdeg[v] (n entries): v's degree at the start of the level removing it.
 // Requires n < 2^24.  Synchronises nothing; read ctl after the stream.
 hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
@@ -120,17 +131,21 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            uint32_t *scratch32, uint64_t *scratch64,
                            PeelCtl *ctl, uint32_t *lv_level,
                            unsigned long long *lv_removed,
-                           unsigned long long *lv_killed, int device,
-                           unsigned *grid_out, hipStream_t s);
+                           unsigned long long *lv_killed, uint32_t *sdeg,
+                           int device, unsigned *grid_out, hipStream_t s);
 
 // Exact sequential replay of one peel phase (level `level`); see peel.hip.
 struct ReplayOut {
   unsigned long long best_num, best_den;  // max intermediate density
   uint32_t steps, found;
 };
-// Collects the level's vertices, compacts their phase-clique incidence into
-// pinc/pioff (pinc holds at least k * k * (cliques killed at the level)
-// words, pioff M+1 entries) and replays the phase with one workgroup.
+// Replays level `level` (M vertices) in the oracle's removal order.
+// One workgroup (grid = 0): the phase-clique incidence is first compacted
+// into pinc/pioff (pinc >= k * k * cliques killed at the level, pioff M+1).
+// Whole GPU (grid workgroups, M <= kReplayGridMaxVertices): works on the
+// full incidence lists; partial holds grid * M words, bar one GridBarrier,
+// and sdeg (if not null) the degrees recorded by peel_persistent.
+// `prepared` (if not null) is recorded before the replay kernel.
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,
@@ -138,7 +153,12 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
                         uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
-                        DeviceArena &arena, hipStream_t s);
+                        uint32_t *partial, GridBarrier *bar, unsigned grid,
+                        const uint32_t *sdeg, DeviceArena &arena,
+                        hipEvent_t prepared, hipStream_t s);
+constexpr uint32_t kReplayGridMaxVertices = 16384;
+// Workgroups of the whole-GPU replay (one per CU), 0 if unavailable.
+unsigned replay_grid_blocks(int device);
 
 // ---- core / prune / components (glue.hip) ----
 hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
@@ -246,7 +266,7 @@ hipError_t max_flow(const FlowProblem &P, const FlowState &S, long long cap_t,
 // Grid-wide max flow for large components (flow_grid.hip), one cooperative
 // launch; same answer as max_flow.
 struct GridFlowCtl {
-  unsigned int bar_count, bar_gen;
+  GridBarrier bar;
   unsigned long long lpack[2];  // vertex work lists: count<<40 | chunks
   unsigned int cqn[2];          // clique BFS frontier sizes
   unsigned long long active;

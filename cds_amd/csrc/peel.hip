@@ -29,24 +29,66 @@ namespace {
 // Slots are reserved with wave-aggregated cursor increments: consecutive
 // cliques share their lowest vertex (they come from the same listing item)
 // and hubs recur, so most lanes of a wave share one atomic.
-__global__ void k_build_inc(uint64_t T, uint32_t k,
-                            const uint32_t *__restrict__ cl,
-                            const uint64_t *__restrict__ ioff,
-                            uint32_t *__restrict__ icur,
-                            uint32_t *__restrict__ inc) {
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t c = base + threadIdx.x;
-    const bool in = c < T;
-    for (uint32_t j = 0; j < k; ++j) {
-      const uint32_t x = in ? cl[c * k + j] : 0;
-      const uint32_t pos = aggregated_reserve(icur, x, in);
-      if (!in) continue;
-      uint32_t *e = inc + (ioff[x] + pos) * k;
+// Incidence build.  A workgroup takes a tile of kIncTile cliques and
+// aggregates its (vertex, clique) pairs in an LDS hash table keyed by
+// vertex: every pair gets its rank among the tile's pairs of that vertex
+// from an LDS atomic, then one global atomicAdd per distinct vertex of the
+// tile reserves a run of the vertex's incidence list.  Hub vertices, which
+// appear in most cliques of a power-law graph, thus cost one global atomic
+// per tile instead of one per clique.
+constexpr uint32_t kIncThreads = 256;
+constexpr uint32_t kIncTile = 512;                       // cliques per tile
+constexpr uint32_t kIncPairs = kIncTile * 4;             // k <= 4
+constexpr uint32_t kIncHash = 2 * kIncPairs;             // load <= 1/2
+constexpr uint32_t kIncPerThread = kIncPairs / kIncThreads;
+
+__global__ __launch_bounds__(kIncThreads) void k_build_inc(
+    uint64_t T, uint32_t k, const uint32_t *__restrict__ cl,
+    const uint64_t *__restrict__ ioff, uint32_t *__restrict__ icur,
+    uint32_t *__restrict__ inc) {
+  __shared__ uint32_t hkey[kIncHash];
+  __shared__ uint32_t hcnt[kIncHash];
+  const uint64_t tiles = (T + kIncTile - 1) / kIncTile;
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads) {
+      hkey[i] = kAlive;
+      hcnt[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t c0 = tile * kIncTile;
+    const uint32_t pairs = (uint32_t)min<uint64_t>(kIncTile, T - c0) * k;
+    uint32_t slot[kIncPerThread], rank[kIncPerThread];
+#pragma unroll
+    for (uint32_t r = 0; r < kIncPerThread; ++r) {
+      const uint32_t p = r * kIncThreads + threadIdx.x;
+      if (p >= pairs) break;
+      const uint32_t x = cl[c0 * k + p];
+      uint32_t h = (x * 2654435761u) & (kIncHash - 1);
+      for (;;) {
+        const uint32_t prev = atomicCAS(&hkey[h], kAlive, x);
+        if (prev == kAlive || prev == x) break;
+        h = (h + 1) & (kIncHash - 1);
+      }
+      slot[r] = h;
+      rank[r] = atomicAdd(&hcnt[h], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads)
+      if (hcnt[i]) hcnt[i] = atomicAdd(&icur[hkey[i]], hcnt[i]);  // -> base
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kIncPerThread; ++r) {
+      const uint32_t p = r * kIncThreads + threadIdx.x;
+      if (p >= pairs) break;
+      const uint32_t q = p / k, j = p - q * k;
+      const uint64_t c = c0 + q;
+      const uint32_t x = hkey[slot[r]];
+      uint32_t *e = inc + (ioff[x] + hcnt[slot[r]] + rank[r]) * k;
       e[0] = (uint32_t)c;
       for (uint32_t t = 0, o = 1; t < k; ++t)
         if (t != j) e[o++] = cl[c * k + t];
     }
+    __syncthreads();
   }
 }
 
@@ -143,37 +185,16 @@ __global__ __launch_bounds__(256) void k_peel_process(
 // appended vertex (vertex count << 40 | chunk count) gives every vertex its
 // slot and its first chunk, so chunk i maps to its vertex by a binary search
 // and a hub's incidence list is spread over many waves.
-constexpr uint32_t kPeelChunk = 256;
+constexpr uint32_t kPeelChunk = 64;     // one wave pass per chunk
+constexpr int kPeelThreads = 1024;
 constexpr uint64_t kChunkMask = (1ull << 40) - 1;
 constexpr uint32_t kPeelLdsCap = 8192;  // alive vertices aggregated in LDS
 
 __device__ __forceinline__ void grid_sync(PeelCtl *ctl) {
-  // The workgroup barrier makes every wave's stores reach L2; one thread
-  // per workgroup then releases them device-wide (the fence writes back
-  // this XCD's L2) and, after the arrival, acquires (invalidates stale
-  // lines) for the whole workgroup, whose waves share the CU's caches.
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned long long t0 = wall_clock64();
-    __threadfence();
-    const unsigned gen =
-        __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(
-        &ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
-      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE,
-                               __HIP_MEMORY_SCOPE_AGENT) == gen)
-        __builtin_amdgcn_s_sleep(1);
-    }
-    __threadfence();
-    if (blockIdx.x == 0) ctl->t_bar += wall_clock64() - t0;
-  }
-  __syncthreads();
+  const unsigned long long t0 =
+      (blockIdx.x == 0 && threadIdx.x == 0) ? wall_clock64() : 0;
+  grid_barrier(&ctl->bar);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->t_bar += wall_clock64() - t0;
 }
 
 __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
@@ -195,14 +216,14 @@ __device__ __forceinline__ void frontier_push(PeelCtl *ctl, int b, uint32_t *F,
   FC[slot] = old & kChunkMask;
 }
 
-__global__ __launch_bounds__(256) void k_peel_persistent(
+__global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     uint32_t n, uint32_t k, const uint64_t *__restrict__ ioff,
     const uint32_t *__restrict__ inc, const uint32_t *__restrict__ cl,
     uint32_t *deg, uint32_t *core, uint32_t *kill, uint32_t *A0, uint32_t *A1,
     uint32_t *F0, uint32_t *F1, uint32_t *F2, uint64_t *FC0, uint64_t *FC1,
     uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
     unsigned long long *lv_removed, unsigned long long *lv_killed,
-    uint32_t *pos) {
+    uint32_t *pos, uint32_t *sdeg) {
   // Once few vertices remain alive, degree decrements are summed per
   // workgroup in LDS (indexed by position in the alive list) and applied
   // with one atomic per vertex and sub-round; the vertex whose aggregated
@@ -226,7 +247,13 @@ __global__ __launch_bounds__(256) void k_peel_persistent(
     uint32_t m = kAlive;
     for (uint32_t i = gtid; i < na; i += gthreads) {
       const uint32_t v = A[ab][i];
-      if (ld(&core[v]) == kAlive) m = min(m, ld(&deg[v]));
+      if (ld(&core[v]) == kAlive) {
+        // degree at the start of the level; kept for the level that
+        // removes v (its clique count in the core C_level)
+        const uint32_t d = ld(&deg[v]);
+        sdeg[v] = d;
+        m = min(m, d);
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off, kWave));
@@ -348,16 +375,16 @@ __global__ __launch_bounds__(256) void k_peel_persistent(
       fb = nb;
       if (leader) ctl->rounds++;
     }
+    // The last sub-round's barrier already ordered every update of this
+    // level; the next min scan touches only buffers no block still reads.
     if (leader) {
       lv_level[lvl_idx] = level;
       lv_removed[lvl_idx] = removed;
       ctl->removed += removed;
-      ctl->fpack[fb] = 0;
     }
     ++lvl_idx;
     ab = 1 - ab;
     gp = 1 - gp;
-    grid_sync(ctl);
   }
   if (leader) ctl->nlevels = lvl_idx;
 }
@@ -369,8 +396,8 @@ __global__ void k_iota(uint32_t n, uint32_t *a) {
 }
 
 __global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t n) {
-  ctl->bar_count = 0;
-  ctl->bar_gen = 0;
+  unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
+  for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
   ctl->gmin[0] = kAlive;
   ctl->gmin[1] = kAlive;
   ctl->alive_n[0] = n;
@@ -424,6 +451,20 @@ __global__ __launch_bounds__(256) void k_phase_deg(
       pdeg[i] = d;
       palive[i] = 1;
     }
+  }
+}
+
+// Phase degrees from the degrees recorded at the start of the level.
+__global__ void k_phase_deg_snap(const uint32_t *__restrict__ count,
+                                 const uint32_t *__restrict__ pv,
+                                 const uint32_t *__restrict__ sdeg,
+                                 uint32_t *__restrict__ pdeg,
+                                 uint8_t *__restrict__ palive) {
+  const uint32_t M = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M;
+       i += gridDim.x * blockDim.x) {
+    pdeg[i] = sdeg[pv[i]];
+    palive[i] = 1;
   }
 }
 
@@ -537,13 +578,120 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay(
   }
 }
 
+// Whole-GPU replay for phases with few vertices (M <= kReplayLdsCap) and
+// many cliques per removal.  Every workgroup computes the same argmin from
+// pdeg/palive, takes a contiguous slice of the removed vertex's incidence
+// list, counts the decrements of the other phase members in LDS
+// and publishes them as one row of `partial` (G x M); after a barrier one
+// wave per phase vertex sums its column and updates pdeg.  Two grid
+// barriers per removal; the removal order and the recorded densities are
+// those of k_replay.
+constexpr uint32_t kReplayLdsCap = kReplayGridMaxVertices;
+
+__global__ __launch_bounds__(kReplayThreads) void k_replay_grid(
+    uint32_t k, uint32_t level, uint32_t M, const uint32_t *__restrict__ pv,
+    uint32_t *pdeg, const uint32_t *__restrict__ posmap, uint8_t *palive,
+    const uint32_t *__restrict__ core, const uint32_t *__restrict__ kill,
+    const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
+    uint32_t *rdead, uint32_t *partial, GridBarrier *bar, uint64_t T_start,
+    uint64_t N_start, ReplayOut *out) {
+  __shared__ uint32_t dec[kReplayLdsCap];
+  __shared__ uint64_t s_wmin[kReplayThreads / kWave];
+  __shared__ uint64_t s_key;
+  const uint32_t tid = threadIdx.x, wid = tid / kWave;
+  const uint32_t G = gridDim.x;
+  const bool boss = blockIdx.x == 0 && tid == 0;
+  for (uint32_t i = tid; i < M; i += kReplayThreads) dec[i] = 0;
+  uint64_t R = 0, best_num = 0, best_den = 1;
+  uint32_t found = 0, steps = 0;
+  for (uint32_t j = 0; j < M; ++j) {
+    uint64_t key = ~0ull;
+    for (uint32_t i = tid; i < M; i += kReplayThreads)
+      if (palive[i]) {
+        const uint64_t kk = ((uint64_t)pdeg[i] << 32) | pv[i];
+        key = kk < key ? kk : key;
+      }
+    key = wave_min64(key);
+    if (lane() == 0) s_wmin[wid] = key;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t m = s_wmin[0];
+      for (int w = 1; w < kReplayThreads / kWave; ++w)
+        m = s_wmin[w] < m ? s_wmin[w] : m;
+      s_key = m;
+    }
+    __syncthreads();
+    const uint64_t kk = s_key;
+    if (kk == ~0ull) break;  // inconsistent counts; reported via steps
+    const uint32_t v = (uint32_t)kk;
+    const uint32_t iv = posmap[v];
+    if (boss) {
+      R += kk >> 32;
+      ++steps;
+      if (j + 1 < M) {
+        const uint64_t num = T_start - R, den = N_start - (j + 1);
+        if (!found || frac_gt(num, den, best_num, best_den)) {
+          best_num = num;
+          best_den = den;
+          found = 1;
+        }
+      }
+    }
+    // v's whole incidence list, one contiguous slice per workgroup; the
+    // phase cliques are those killed at this level
+    const uint64_t e0 = ioff[v], e1 = ioff[v + 1];
+    const uint64_t per = (e1 - e0 + G - 1) / G;
+    const uint64_t b0 = e0 + per * blockIdx.x;
+    const uint64_t b1 = min(e1, b0 + per);
+    for (uint64_t q = b0 + tid; q < b1; q += kReplayThreads) {
+      const uint32_t *en = inc + q * k;
+      const uint32_t c = en[0];
+      if (kill[c] != level || rdead[c] == level + 1) continue;
+      rdead[c] = level + 1;
+      for (uint32_t t = 1; t < k; ++t) {
+        const uint32_t u = en[t];
+        if (core[u] == level) atomicAdd(&dec[posmap[u]], 1u);
+      }
+    }
+    __syncthreads();
+    uint32_t *row = partial + (size_t)blockIdx.x * M;
+    for (uint32_t i = tid; i < M; i += kReplayThreads) {
+      row[i] = dec[i];
+      dec[i] = 0;
+    }
+    grid_barrier(bar);
+    const uint32_t waves = G * (kReplayThreads / kWave);
+    for (uint32_t i = blockIdx.x * (kReplayThreads / kWave) + wid; i < M; i += waves) {
+      uint32_t sum = 0;
+      for (uint32_t b = lane(); b < G; b += kWave) sum += partial[(size_t)b * M + i];
+      sum = wave_sum(sum);
+      if (lane() == 0 && sum) pdeg[i] -= sum;
+    }
+    if (boss) palive[iv] = 0;
+    grid_barrier(bar);
+  }
+  if (boss) {
+    out->best_num = best_num;
+    out->best_den = best_den;
+    out->steps = steps;
+    out->found = found;
+  }
+}
+
+__global__ void k_zero_barrier(GridBarrier *b) {
+  unsigned *w = reinterpret_cast<unsigned *>(b);
+  for (size_t i = threadIdx.x; i < sizeof(GridBarrier) / sizeof(unsigned); i += blockDim.x)
+    w[i] = 0;
+}
+
 }  // namespace
 
 hipError_t build_incidence(uint64_t T, uint32_t k, const uint32_t *cliques,
                            const uint64_t *ioff, uint32_t *icur,
                            uint32_t *inc, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_inc, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
-                     s, T, k, cliques, ioff, icur, inc);
+  if (k > 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_build_inc, dim3(grid_for(T, kIncTile, 16384)),
+                     dim3(kIncThreads), 0, s, T, k, cliques, ioff, icur, inc);
   return hipGetLastError();
 }
 
@@ -595,8 +743,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            uint32_t *scratch32, uint64_t *scratch64,
                            PeelCtl *ctl, uint32_t *lv_level,
                            unsigned long long *lv_removed,
-                           unsigned long long *lv_killed, int device,
-                           unsigned *grid_out, hipStream_t s) {
+                           unsigned long long *lv_killed, uint32_t *sdeg,
+                           int device, unsigned *grid_out, hipStream_t s) {
   if (n >= (1u << 24)) return hipErrorInvalidValue;
   hipError_t e;
   if ((e = hipMemsetAsync(lv_level, 0, (size_t)n * 4, s)) != hipSuccess) return e;
@@ -610,14 +758,18 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
   if ((e = hipGetLastError()) != hipSuccess) return e;
   int per_cu = 0, cus = 0;
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-           &per_cu, reinterpret_cast<const void *>(k_peel_persistent), 256, 0)) !=
+           &per_cu, reinterpret_cast<const void *>(k_peel_persistent), kPeelThreads, 0)) !=
       hipSuccess)
     return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                  device)) != hipSuccess)
     return e;
   if (per_cu < 1) return hipErrorLaunchFailure;
-  unsigned grid = (unsigned)(cus * (per_cu < 2 ? per_cu : 2));
+  // 1024-thread workgroups on half the CUs: every grid barrier costs a
+  // release and an acquire fence per workgroup, and the levels of a power-
+  // law graph are too light to pay for more (R-MAT scale 20, k = 3: 410 ms
+  // with 128 workgroups, 459 ms with 256).
+  unsigned grid = (unsigned)(cus > 1 ? cus / 2 : 1);
   if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {
     const unsigned want = (unsigned)atoi(gb);
     if (want >= 1 && want < grid) grid = want;
@@ -625,9 +777,10 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
   if (grid_out) *grid_out = grid;
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
                   &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
-                  &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos};
+                  &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
+                  &sdeg};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_peel_persistent),
-                                    dim3(grid), dim3(256), args, 0, s);
+                                    dim3(grid), dim3(kPeelThreads), args, 0, s);
 }
 
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
@@ -637,24 +790,55 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
                         uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
-                        DeviceArena &arena, hipStream_t s) {
+                        uint32_t *partial, GridBarrier *bar, unsigned grid,
+                        const uint32_t *sdeg, DeviceArena &arena,
+                        hipEvent_t prepared, hipStream_t s) {
+  if (grid && M > kReplayLdsCap) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(pcount, 0, sizeof(uint32_t), s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_collect_level, dim3(grid_for(n, 256, 8192)), dim3(256),
                      0, s, n, level, core, pv, posmap, pcount);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const unsigned g = grid_for((size_t)M * kWave, 256, 8192);
-  hipLaunchKernelGGL(k_phase_deg, dim3(g), dim3(256), 0, s, k, level, pcount,
-                     pv, ioff, inc, kill, pdeg, palive);
+  if (grid && sdeg)
+    hipLaunchKernelGGL(k_phase_deg_snap, dim3(grid_for(M, 256, 4096)), dim3(256),
+                       0, s, pcount, pv, sdeg, pdeg, palive);
+  else
+    hipLaunchKernelGGL(k_phase_deg, dim3(g), dim3(256), 0, s, k, level, pcount,
+                       pv, ioff, inc, kill, pdeg, palive);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (grid) {
+    hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (prepared) (void)hipEventRecord(prepared, s);
+    void *args[] = {&k, &level, &M, &pv, &pdeg, (void *)&posmap, &palive,
+                    (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
+                    &rdead, &partial, &bar, &T_start, &N_start, &out};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_replay_grid),
+                                      dim3(grid), dim3(kReplayThreads), args, 0, s);
+  }
   if ((e = scan_u32(pdeg, pioff, M, arena)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_phase_compact, dim3(g), dim3(256), 0, s, k, level,
                      pcount, pv, ioff, inc, kill, pioff, pinc);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (prepared) (void)hipEventRecord(prepared, s);
   hipLaunchKernelGGL(k_replay, dim3(1), dim3(kReplayThreads), 0, s, k, level,
                      pcount, pv, pdeg, posmap, palive, core, pioff, pinc,
                      rdead, T_start, N_start, out);
   return hipGetLastError();
+}
+
+unsigned replay_grid_blocks(int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(k_replay_grid), kReplayThreads, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+      hipSuccess)
+    return 0;
+  return (unsigned)cus;
 }
 
 }  // namespace cds

@@ -117,6 +117,7 @@ typedef struct {
   uint64_t flow_sweeps;             /* push-relabel sweeps, all alphas */
   uint32_t peel_levels;             /* level-synchronous peel rounds */
   uint32_t replay_phases;           /* phases replayed in exact order */
+  uint32_t replay_grid_phases;      /* ... of which on the whole GPU */
   uint32_t oracle_quirk_flags;      /* bit2: oracle's int clique total wraps */
   /* listing statistics (algorithmic-bytes inputs) */
   uint64_t dag_arcs;                /* m */
@@ -136,6 +137,7 @@ typedef struct {
   double t_incidence_ms;            /* clique-incidence build (HIP events) */
   double t_peel_kernel_ms;          /* persistent peel launch (HIP events) */
   double t_replay_ms;               /* exact phase replays (HIP events) */
+  double t_replay_prep_ms;          /* ... of which phase incidence compaction */
   cds_times times;
 } cds_result;
 

@@ -1,9 +1,11 @@
-"""GPU parity of the whole-GPU flow kernel and of R-MAT inputs.
+"""GPU parity of the whole-GPU kernels and of R-MAT inputs.
 
 The engine switches to the cooperative grid-wide push-relabel for
 components above a node threshold; CDS_GRID_FLOW_NODES=0 forces it for
 every component, so the small graphs of the oracle comparison exercise it
-too (including components that need one flow per alpha query).
+too (including components that need one flow per alpha query).  Likewise
+CDS_REPLAY_GRID_WORK=0 sends every exact phase replay with at most 16384
+vertices to the whole-GPU replay kernel.
 """
 import os
 
@@ -17,17 +19,30 @@ from test_gpu_parity import _announce, compare
 pytestmark = pytest.mark.gpu
 
 
+def _engine_with(env):
+    """An engine created while `env` is set (the engine reads it once)."""
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return cds_amd.Engine(device=0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
 @pytest.fixture(scope="module")
 def grid_engine():
-    old = os.environ.get("CDS_GRID_FLOW_NODES")
-    os.environ["CDS_GRID_FLOW_NODES"] = "0"
-    try:
-        eng = cds_amd.Engine(device=0)
-    finally:
-        if old is None:
-            del os.environ["CDS_GRID_FLOW_NODES"]
-        else:
-            os.environ["CDS_GRID_FLOW_NODES"] = old
+    eng = _engine_with({"CDS_GRID_FLOW_NODES": "0"})
+    yield eng
+    eng.close()
+
+
+@pytest.fixture(scope="module")
+def replay_engine():
+    eng = _engine_with({"CDS_REPLAY_GRID_WORK": "0"})
     yield eng
     eng.close()
 
@@ -69,3 +84,35 @@ def test_rmat_parity(engine, scale):
     _announce(label)
     got = engine.solve(engine.upload(g), 3, vertex_arrays=True)
     compare(got, oracle_solve(g, 3), label)
+
+
+_REPLAYED = []
+
+
+@pytest.mark.parametrize("seed,k", [(s, k) for s in range(12) for k in (3, 4)])
+def test_grid_replay_small(replay_engine, seed, k):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(20, 400))
+    g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                               float(rng.uniform(3, 12)), int(rng.integers(0, 25)),
+                               0.8, seed=100 + seed)
+    label = f"grid replay chung_lu seed={100 + seed} n={n} k={k}"
+    _announce(label)
+    got = replay_engine.solve(replay_engine.upload(g), k, vertex_arrays=True)
+    compare(got, oracle_solve(g, k), label)
+    assert got["replay_grid_phases"] == got["replay_phases"], label
+    _REPLAYED.append(got["replay_grid_phases"])
+
+
+@pytest.mark.parametrize("scale", [9, 10])
+def test_grid_replay_rmat(replay_engine, scale):
+    g = cds_amd.Graph.rmat(scale, 16, seed=3)
+    label = f"grid replay rmat scale={scale} k=3"
+    _announce(label)
+    got = replay_engine.solve(replay_engine.upload(g), 3, vertex_arrays=True)
+    compare(got, oracle_solve(g, 3), label)
+    _REPLAYED.append(got["replay_grid_phases"])
+
+
+def test_grid_replay_was_exercised():
+    assert sum(_REPLAYED) > 0, "no phase went through the whole-GPU replay"
