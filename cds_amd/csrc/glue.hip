@@ -20,13 +20,6 @@
 namespace cds {
 namespace {
 
-__global__ void k_core_flags(uint32_t n, const uint32_t *__restrict__ core,
-                             uint32_t kp, uint32_t *__restrict__ flag) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += gridDim.x * blockDim.x)
-    flag[v] = core[v] >= kp;
-}
-
 __global__ __launch_bounds__(256) void k_core_edges(
     uint32_t n, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ adj, const uint32_t *__restrict__ core,
@@ -88,18 +81,6 @@ __device__ void uf_unite(uint32_t *p, uint32_t a, uint32_t b) {
     }
     // hook the larger root a under the smaller root b
     if (atomicCAS(&p[a], a, b) == a) return;
-  }
-}
-
-__global__ void k_uf_union(uint64_t T, uint32_t k,
-                           const uint32_t *__restrict__ cl,
-                           const uint32_t *__restrict__ kill, uint32_t kp,
-                           uint32_t *__restrict__ p) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] < kp) continue;
-    const uint32_t a = cl[c * k];
-    for (uint32_t j = 1; j < k; ++j) uf_unite(p, a, cl[c * k + j]);
   }
 }
 
@@ -250,112 +231,7 @@ __global__ __launch_bounds__(256) void k_item_core_write(
   }
 }
 
-// Core clique degree of every core vertex: incidence entries whose clique
-// lies in the core.  One wave per vertex, no atomics.
-__global__ __launch_bounds__(256) void k_core_cd(
-    uint32_t n, const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
-    const uint32_t *__restrict__ kill, const uint32_t *__restrict__ core,
-    uint32_t kp, const uint32_t *__restrict__ fv_of, uint32_t *__restrict__ fcd) {
-  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       v < n; v += waves) {
-    if (core[v] < kp) continue;
-    uint32_t d = 0;
-    for (uint64_t q = ioff[v] + lane(); q < ioff[v + 1]; q += kWave)
-      d += kill[inc[q]] >= kp;
-    d = wave_sum(d);
-    if (lane() == 0) fcd[fv_of[v]] = d;
-  }
-}
-
-// Flow incidence of every core vertex (slot ids f*k + j of its core
-// cliques), written in incidence order by ballot compaction; no atomics.
-__global__ __launch_bounds__(256) void k_core_inc(
-    uint32_t n, uint32_t k, const uint64_t *__restrict__ ioff,
-    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ kill,
-    const uint32_t *__restrict__ core, uint32_t kp,
-    const uint32_t *__restrict__ cl, const uint32_t *__restrict__ cslot,
-    const uint32_t *__restrict__ fv_of, const uint64_t *__restrict__ fioff,
-    uint32_t *__restrict__ finc) {
-  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       v < n; v += waves) {
-    if (core[v] < kp) continue;
-    uint64_t out = fioff[fv_of[v]];
-    const uint64_t e = ioff[v + 1];
-    for (uint64_t q0 = ioff[v]; q0 < e; q0 += kWave) {
-      const uint64_t q = q0 + lane();
-      bool take = false;
-      uint32_t slot = 0;
-      if (q < e) {
-        const uint32_t c = inc[q];
-        if (kill[c] >= kp) {
-          take = true;
-          uint32_t j = 0;
-          while (j + 1 < k && cl[(uint64_t)c * k + j] != v) ++j;
-          slot = cslot[c] * k + j;
-        }
-      }
-      unsigned cnt;
-      const unsigned pos = wave_slot(take, &cnt);
-      if (take) finc[out + pos] = slot;
-      out += cnt;
-    }
-  }
-}
-
-__global__ void k_comp_count(uint64_t T, uint32_t k,
-                             const uint32_t *__restrict__ cl,
-                             const uint32_t *__restrict__ kill, uint32_t kp,
-                             const uint32_t *__restrict__ comp_of,
-                             const uint32_t *__restrict__ fv_of,
-                             uint32_t *__restrict__ ccount,
-                             uint32_t *__restrict__ fcd) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] < kp) continue;
-    atomicAdd(&ccount[comp_of[cl[c * k]]], 1u);
-    for (uint32_t j = 0; j < k; ++j) atomicAdd(&fcd[fv_of[cl[c * k + j]]], 1u);
-  }
-}
-
-__global__ void k_comp_scatter(uint64_t T, uint32_t k,
-                               const uint32_t *__restrict__ cl,
-                               const uint32_t *__restrict__ kill, uint32_t kp,
-                               const uint32_t *__restrict__ comp_of,
-                               const uint32_t *__restrict__ fv_of,
-                               const uint64_t *__restrict__ coff,
-                               uint32_t *__restrict__ ccur,
-                               uint32_t *__restrict__ fmem) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] < kp) continue;
-    const uint32_t comp = comp_of[cl[c * k]];
-    const uint64_t slot = coff[comp] + atomicAdd(&ccur[comp], 1u);
-    for (uint32_t j = 0; j < k; ++j) fmem[slot * k + j] = fv_of[cl[c * k + j]];
-  }
-}
-
-__global__ void k_flow_inc(uint64_t TT, uint32_t k,
-                           const uint32_t *__restrict__ fmem,
-                           const uint64_t *__restrict__ fioff,
-                           uint32_t *__restrict__ ficur,
-                           uint32_t *__restrict__ finc) {
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-       s < TT * k; s += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = fmem[s];
-    finc[fioff[v] + atomicAdd(&ficur[v], 1u)] = (uint32_t)s;
-  }
-}
-
 }  // namespace
-
-hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
-                      uint32_t *flag, hipStream_t s) {
-  hipLaunchKernelGGL(k_core_flags, dim3(grid_for(n, 256, 65535)), dim3(256), 0,
-                     s, n, core, kp, flag);
-  return hipGetLastError();
-}
 
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,
@@ -380,14 +256,6 @@ hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
 hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s) {
   hipLaunchKernelGGL(k_uf_init, dim3(grid_for(n, 256, 65535)), dim3(256), 0, s,
                      n, parent);
-  return hipGetLastError();
-}
-
-hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
-                            const uint32_t *kill, uint32_t kp,
-                            uint32_t *parent, hipStream_t s) {
-  hipLaunchKernelGGL(k_uf_union, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
-                     s, T, k, cliques, kill, kp, parent);
   return hipGetLastError();
 }
 
@@ -420,29 +288,6 @@ hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_count_nonzero_u8, dim3(grid_for(n, 256, 8192)),
                      dim3(256), 0, s, n, a, out);
-  return hipGetLastError();
-}
-
-hipError_t comp_clique_count(uint64_t T, uint32_t k, const uint32_t *cliques,
-                             const uint32_t *kill, uint32_t kp,
-                             const uint32_t *comp_of, const uint32_t *fv_of,
-                             uint32_t *comp_cliques, uint32_t *fcd,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_comp_count, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
-                     s, T, k, cliques, kill, kp, comp_of, fv_of, comp_cliques,
-                     fcd);
-  return hipGetLastError();
-}
-
-hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
-                               const uint32_t *cliques, const uint32_t *kill,
-                               uint32_t kp, const uint32_t *comp_of,
-                               const uint32_t *fv_of, const uint64_t *coff,
-                               uint32_t *ccur, uint32_t *fmem,
-                               hipStream_t s) {
-  hipLaunchKernelGGL(k_comp_scatter, dim3(grid_for(T, 256, 65535)), dim3(256),
-                     0, s, T, k, cliques, kill, kp, comp_of, fv_of, coff, ccur,
-                     fmem);
   return hipGetLastError();
 }
 
@@ -486,36 +331,6 @@ hipError_t item_core_write(uint64_t nitems, const uint4 *items,
                      dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
                      nitems, items, item_out, inc, ik, kill, kp, cliques, k,
                      cslot, finc);
-  return hipGetLastError();
-}
-
-hipError_t core_clique_degree(uint32_t n, const uint64_t *ioff,
-                              const uint32_t *inc, const uint32_t *kill,
-                              const uint32_t *core, uint32_t kp,
-                              const uint32_t *fv_of, uint32_t *fcd,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_core_cd, dim3(grid_for((size_t)n * kWave, 256, 16384)),
-                     dim3(256), 0, s, n, ioff, inc, kill, core, kp, fv_of, fcd);
-  return hipGetLastError();
-}
-
-hipError_t core_flow_incidence(uint32_t n, uint32_t k, const uint64_t *ioff,
-                               const uint32_t *inc, const uint32_t *kill,
-                               const uint32_t *core, uint32_t kp,
-                               const uint32_t *cliques, const uint32_t *cslot,
-                               const uint32_t *fv_of, const uint64_t *fioff,
-                               uint32_t *finc, hipStream_t s) {
-  hipLaunchKernelGGL(k_core_inc, dim3(grid_for((size_t)n * kWave, 256, 16384)),
-                     dim3(256), 0, s, n, k, ioff, inc, kill, core, kp, cliques,
-                     cslot, fv_of, fioff, finc);
-  return hipGetLastError();
-}
-
-hipError_t flow_incidence(uint64_t TT, uint32_t k, const uint32_t *fmem,
-                          const uint64_t *fioff, uint32_t *ficur,
-                          uint32_t *finc, hipStream_t s) {
-  hipLaunchKernelGGL(k_flow_inc, dim3(grid_for(TT * k, 256, 65535)), dim3(256),
-                     0, s, TT, k, fmem, fioff, ficur, finc);
   return hipGetLastError();
 }
 

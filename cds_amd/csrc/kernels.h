@@ -160,9 +160,6 @@ constexpr uint32_t kReplayGridMaxVertices = 16384;
 // Workgroups of the whole-GPU replay (one per CU), 0 if unavailable.
 unsigned replay_grid_blocks(int device);
 
-// ---- core / prune / components (glue.hip) ----
-hipError_t core_flags(uint32_t n, const uint32_t *core, uint32_t kp,
-                      uint32_t *flag, hipStream_t s);
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,
                            uint32_t kp, unsigned long long *out,
@@ -170,9 +167,6 @@ hipError_t core_edge_count(uint32_t n, const uint32_t *off,
 hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
                          unsigned long long *out, hipStream_t s);
 hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s);
-hipError_t uf_union_cliques(uint64_t T, uint32_t k, const uint32_t *cliques,
-                            const uint32_t *kill, uint32_t kp,
-                            uint32_t *parent, hipStream_t s);
 // Unions the endpoints of the marked (clique-covered) edges of core vertices.
 hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
                           const uint8_t *mark, const uint32_t *core,
@@ -183,24 +177,9 @@ hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *adj, uint8_t *mark, hipStream_t s);
 hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
                             unsigned long long *out, hipStream_t s);
-// Per-component clique bucketing.  comp_of / local_of indexed by original
-// vertex id (valid for core vertices).
-hipError_t comp_clique_count(uint64_t T, uint32_t k, const uint32_t *cliques,
-                             const uint32_t *kill, uint32_t kp,
-                             const uint32_t *comp_of, const uint32_t *fv_of,
-                             uint32_t *comp_cliques, uint32_t *fcd,
-                             hipStream_t s);
-hipError_t comp_clique_scatter(uint64_t T, uint32_t k,
-                               const uint32_t *cliques, const uint32_t *kill,
-                               uint32_t kp, const uint32_t *comp_of,
-                               const uint32_t *fv_of, const uint64_t *coff,
-                               uint32_t *ccur, uint32_t *fmem,
-                               hipStream_t s);
-// Contention-free versions used by the engine:
-//  comp_clique_totals  ccount[comp] += 1 per core clique (wave-aggregated)
+// Per-component clique buckets (wave-aggregated reservations):
+//  comp_clique_totals  ccount[comp] += 1 per core clique
 //  comp_cliques_build  fmem / cslot (flow index of each core clique)
-//  core_clique_degree  fcd[fv] = core cliques containing the vertex
-//  core_flow_incidence finc (slot ids) per core vertex, incidence order
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, uint32_t *ccount,
@@ -223,20 +202,6 @@ hipError_t item_core_write(uint64_t nitems, const uint4 *items,
                            const uint32_t *cliques, uint32_t k,
                            const uint32_t *cslot, uint32_t *finc,
                            hipStream_t s);
-hipError_t core_clique_degree(uint32_t n, const uint64_t *ioff,
-                              const uint32_t *inc, const uint32_t *kill,
-                              const uint32_t *core, uint32_t kp,
-                              const uint32_t *fv_of, uint32_t *fcd,
-                              hipStream_t s);
-hipError_t core_flow_incidence(uint32_t n, uint32_t k, const uint64_t *ioff,
-                               const uint32_t *inc, const uint32_t *kill,
-                               const uint32_t *core, uint32_t kp,
-                               const uint32_t *cliques, const uint32_t *cslot,
-                               const uint32_t *fv_of, const uint64_t *fioff,
-                               uint32_t *finc, hipStream_t s);
-hipError_t flow_incidence(uint64_t TT, uint32_t k, const uint32_t *fmem,
-                          const uint64_t *fioff, uint32_t *ficur,
-                          uint32_t *finc, hipStream_t s);
 
 // ---- max flow (flow.hip) ----
 struct FlowProblem {
