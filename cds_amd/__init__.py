@@ -283,6 +283,25 @@ class Engine:
         finally:
             lib().cds_result_free(ctypes.byref(r))
 
+    def solve_batch(self, dgraphs, ks, vertex_arrays=False):
+        """cds_solve_batch: every graph solved with its own k (default
+        parameters, ub1 = ub2 = on); returns one result dict per graph."""
+        n = len(dgraphs)
+        if len(ks) != n:
+            raise ValueError("one k per graph")
+        if n == 0:
+            return []
+        hs = (ctypes.c_void_p * n)(*[d._h for d in dgraphs])
+        ps = (CParams * n)(*[CParams(int(k), 8000, 16000, 60000, 30000, -1, 1, 1,
+                                     int(vertex_arrays)) for k in ks])
+        rs = (CResult * n)()
+        _check(lib().cds_solve_batch(self._h, hs, ps, n, rs))
+        try:
+            return [result_to_dict(r) for r in rs]
+        finally:
+            for r in rs:
+                lib().cds_result_free(ctypes.byref(r))
+
     def core_decompose(self, dgraph, k, vertex_arrays=False):
         """Listing + clique-core peel + densest core only (no flow)."""
         p = CParams(k, 8000, 16000, 60000, 30000, -1, 1, 1, int(vertex_arrays))

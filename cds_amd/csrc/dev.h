@@ -130,6 +130,16 @@ __device__ __forceinline__ void grid_barrier(B *b) {
   __syncthreads();
 }
 
+// Launches a kernel that synchronises its workgroups with grid_barrier:
+// cooperatively (all workgroups co-resident) when there are several, as a
+// plain launch when there is one, which needs no co-residency and lets
+// independent single-workgroup solves overlap on other streams.
+inline hipError_t launch_grid(const void *fn, unsigned grid, unsigned threads,
+                              void **args, size_t lds, hipStream_t s) {
+  if (grid == 1) return hipLaunchKernel(fn, dim3(1), dim3(threads), args, lds, s);
+  return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, lds, s);
+}
+
 inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 65535u * 16u) {
   size_t g = (work + block - 1) / block;
   if (g < 1) g = 1;

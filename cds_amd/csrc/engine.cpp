@@ -13,10 +13,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
+#include <optional>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -32,13 +37,15 @@ struct cds_engine {
   bool profile = false;
   std::mutex mu;
   void *pinned = nullptr;  // small readbacks
-  // components with more flow nodes than this use the whole-GPU flow
-  // (CDS_GRID_FLOW_NODES overrides it, e.g. to exercise that kernel on
-  // small graphs)
-  uint64_t grid_nodes = 1ull << 17;
   // phases whose compacted incidence holds at least this many entries per
   // vertex are replayed on the whole GPU (CDS_REPLAY_GRID_WORK overrides)
   uint64_t replay_grid_work = 2048;
+  // workgroups of every max-flow launch; 0 = sized from the network
+  // (CDS_FLOW_BLOCKS overrides, e.g. to exercise several on small graphs)
+  unsigned flow_blocks = 0;
+  // batched solves: engines (own stream and pool) driven by host threads
+  std::vector<cds_engine *> workers;
+  std::mutex batch_mu;
 };
 
 struct cds_dgraph {
@@ -111,6 +118,60 @@ struct Timer {
     (void)hipEventElapsedTime(&t, ev[a], ev[b]);
     return t;
   }
+};
+
+// Cooperative launches need every workgroup co-resident.  Engines on one
+// device may run concurrently (batched solves), so at most one cooperative
+// kernel per device is in flight: the gate is held from the launch until
+// the engine's stream has drained.
+// Cooperative launches need every workgroup co-resident.  Engines on one
+// device may run concurrently (batched solves), so cooperative workgroups
+// in flight on a device are capped at its CU count: every cooperative
+// kernel here fits one 1024-thread workgroup per CU, so any set of them
+// within the cap is co-resident (other kernels only delay them).  A gate
+// holds its share from the launch until the engine's stream has drained.
+struct CoopBudget {
+  std::mutex mu;
+  std::condition_variable cv;
+  int cap = -1, avail = 0;  // cap -1: not yet read from the device
+};
+
+CoopBudget &coop_budget(int device) {
+  static CoopBudget b[64];
+  return b[device & 63];
+}
+
+struct CoopGate {
+  cds_engine *e;
+  int blocks;
+  CoopGate(cds_engine *eng, unsigned want) : e(eng), blocks((int)want) {
+    if (blocks <= 1) {  // one workgroup: a plain launch, nothing to reserve
+      blocks = 0;
+      return;
+    }
+    CoopBudget &b = coop_budget(e->device);
+    std::unique_lock<std::mutex> lk(b.mu);
+    if (b.cap < 0) {
+      int cus = 0;
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
+      b.cap = b.avail = cus > 0 ? cus : 1;
+    }
+    blocks = std::min(blocks, b.cap);
+    b.cv.wait(lk, [&] { return b.avail >= blocks; });
+    b.avail -= blocks;
+  }
+  ~CoopGate() {
+    if (blocks == 0) return;
+    (void)hipStreamSynchronize(e->stream);
+    CoopBudget &b = coop_budget(e->device);
+    {
+      std::lock_guard<std::mutex> lk(b.mu);
+      b.avail += blocks;
+    }
+    b.cv.notify_all();
+  }
+  CoopGate(const CoopGate &) = delete;
+  CoopGate &operator=(const CoopGate &) = delete;
 };
 
 template <class T>
@@ -211,7 +272,7 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
     return fail(CDS_ERR_HIP, "engine init failed: %s", hipGetErrorString(r));
   }
   e->arena = new DeviceArena(e->stream);
-  if (const char *gn = getenv("CDS_GRID_FLOW_NODES")) e->grid_nodes = strtoull(gn, nullptr, 10);
+  if (const char *fb = getenv("CDS_FLOW_BLOCKS")) e->flow_blocks = (unsigned)atoi(fb);
   if (const char *rw = getenv("CDS_REPLAY_GRID_WORK"))
     e->replay_grid_work = strtoull(rw, nullptr, 10);
   *out = e;
@@ -220,6 +281,8 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
 
 extern "C" void cds_engine_destroy(cds_engine *e) {
   if (!e) return;
+  for (cds_engine *w : e->workers) cds_engine_destroy(w);
+  e->workers.clear();
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   delete e->arena;
@@ -486,11 +549,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
     HIPCHK(sdeg.alloc(n));
-    unsigned grid = 0;
+    const unsigned grid = peel_grid_blocks(n, T, k, e->device);
+    if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     const int ev_p0 = tm.mark();
-    HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
-                           kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
-                           sdeg.p, e->device, &grid, st));
+    {
+      CoopGate gate(e, grid);
+      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
+                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
+                             sdeg.p, grid, st));
+    }
     const int ev_p1 = tm.mark();
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));
@@ -633,12 +700,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       } else {
         HIPCHK(pinc.alloc(lv.killed * k * k));
       }
+      std::optional<CoopGate> gate;
+      if (rgrid) gate.emplace(e, rgrid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
                           T - kl_before[ii], n - rm_before[ii], (uint32_t)M,
                           pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
                           pioff.p, pinc.p, rout.p, partial.p, gbar.p, rgrid,
                           sdeg.p, A, tm.ev[23], st));
+      gate.reset();
       ReplayOut ro;
       CHK(readback(e, &ro, rout.p, 1));
       {
@@ -894,10 +964,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     maxV = std::max(maxV, c.V);
     maxT = std::max(maxT, c.T);
   }
-  DBuf<long long> fx(&A), frvs(&A), fex(&A), fdt(&A), fda(&A);
-  DBuf<int> fh(&A), freach(&A);
-  DBuf<uint32_t> fq0(&A), fq1(&A);
-  DBuf<unsigned long long> fmisc(&A);
+  DBuf<int> freach(&A);
   bool flow_alloc = false;
   std::vector<int> res, reach_h;
   // Exact max flow of component ci with sink capacity capT (all
@@ -921,82 +988,59 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     P.ioff = fioff.p + C.vbase;
     P.inc = finc.p;
     P.cd = fcd.p + C.vbase;
-    if ((uint64_t)C.V + C.T > e->grid_nodes) {
-      // whole-GPU push-relabel for a large component
-      DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
-      DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
-      DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
-      DBuf<uint64_t> glc0(&A), glc1(&A);
-      DBuf<GridFlowCtl> gctl(&A);
-      HIPCHK(gx.alloc(C.T * k));
-      HIPCHK(grvs.alloc(C.V));
-      HIPCHK(gexv.alloc(C.V));
-      HIPCHK(grem.alloc(C.V));
-      HIPCHK(gexc.alloc(C.T));
-      HIPCHK(ghv.alloc(C.V));
-      HIPCHK(ghmin.alloc(C.V));
-      HIPCHK(ghc.alloc(C.T));
-      HIPCHK(gl0.alloc(C.V));
-      HIPCHK(gl1.alloc(C.V));
-      HIPCHK(glc0.alloc(C.V));
-      HIPCHK(glc1.alloc(C.V));
-      HIPCHK(gq0.alloc(C.T));
-      HIPCHK(gq1.alloc(C.T));
-      HIPCHK(gctl.alloc(1));
-      GridFlowState G;
-      G.x = gx.p;
-      G.rvs = grvs.p;
-      G.exv = gexv.p;
-      G.rem = grem.p;
-      G.exc = gexc.p;
-      G.hv = ghv.p;
-      G.hmin = ghmin.p;
-      G.hc = ghc.p;
-      G.list0 = gl0.p;
-      G.list1 = gl1.p;
-      G.lchunk0 = glc0.p;
-      G.lchunk1 = glc1.p;
-      G.cq0 = gq0.p;
-      G.cq1 = gq1.p;
-      G.ctl = gctl.p;
-      G.reach = freach.p;
-      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, e->device, st));
-      GridFlowCtl hc;
-      CHK(readback(e, &hc, gctl.p, 1));
-      if (hc.limit)
-        return fail(CDS_ERR_NOT_CONVERGED,
-                    "grid max flow of component %u hit %u rounds", ci, hc.rounds);
-      sweeps += hc.rounds;
-      R->flow_launches++;
-      *F = (long long)hc.sink;
-      return CDS_OK;
+    // push-relabel on as many workgroups as the network needs (one for a
+    // small component); one cooperative launch in flight per device
+    DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
+    DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
+    DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
+    DBuf<uint64_t> glc0(&A), glc1(&A);
+    DBuf<GridFlowCtl> gctl(&A);
+    HIPCHK(gx.alloc(C.T * k));
+    HIPCHK(grvs.alloc(C.V));
+    HIPCHK(gexv.alloc(C.V));
+    HIPCHK(grem.alloc(C.V));
+    HIPCHK(gexc.alloc(C.T));
+    HIPCHK(ghv.alloc(C.V));
+    HIPCHK(ghmin.alloc(C.V));
+    HIPCHK(ghc.alloc(C.T));
+    HIPCHK(gl0.alloc(C.V));
+    HIPCHK(gl1.alloc(C.V));
+    HIPCHK(glc0.alloc(C.V));
+    HIPCHK(glc1.alloc(C.V));
+    HIPCHK(gq0.alloc(C.T));
+    HIPCHK(gq1.alloc(C.T));
+    HIPCHK(gctl.alloc(1));
+    GridFlowState G;
+    G.x = gx.p;
+    G.rvs = grvs.p;
+    G.exv = gexv.p;
+    G.rem = grem.p;
+    G.exc = gexc.p;
+    G.hv = ghv.p;
+    G.hmin = ghmin.p;
+    G.hc = ghc.p;
+    G.list0 = gl0.p;
+    G.list1 = gl1.p;
+    G.lchunk0 = glc0.p;
+    G.lchunk1 = glc1.p;
+    G.cq0 = gq0.p;
+    G.cq1 = gq1.p;
+    G.ctl = gctl.p;
+    G.reach = freach.p;
+    {
+      const unsigned fgrid = flow_grid_blocks(P, e->device, e->flow_blocks);
+      if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
+      CoopGate gate(e, fgrid);
+      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, fgrid, st));
     }
-    FlowState S;
-    S.x = fx.p;
-    S.rvs = frvs.p;
-    S.excess = fex.p;
-    S.h = fh.p;
-    S.dtgt = fdt.p;
-    S.damt = fda.p;
-    S.queue0 = fq0.p;
-    S.queue1 = fq1.p;
-    S.misc = fmisc.p;
-    S.reach = freach.p;
-    // One workgroup sweeps all nodes per round; the round budget keeps a
-    // launch to roughly 2e10 node visits (tens of seconds) so an oversized
-    // component fails with NOT_CONVERGED instead of stalling.
-    const uint64_t nodes = (uint64_t)C.V + C.T;
-    const uint32_t max_rounds = (uint32_t)std::max<uint64_t>(
-        64, std::min<uint64_t>(2000000, 20000000000ull / nodes));
-    HIPCHK(max_flow(P, S, capT, U, max_rounds, st));
-    unsigned long long misc[4];
-    CHK(readback(e, misc, fmisc.p, 4));
-    if (misc[2])
+    GridFlowCtl hc;
+    CHK(readback(e, &hc, gctl.p, 1));
+    if (hc.limit)
       return fail(CDS_ERR_NOT_CONVERGED,
-                  "max flow of component %u hit the round limit", ci);
-    sweeps += misc[1];
+                  "grid max flow of component %u hit %u rounds", ci, hc.rounds);
+    sweeps += hc.rounds;
     R->flow_launches++;
-    *F = (long long)misc[0];
+    *F = (long long)hc.sink;
     return CDS_OK;
   };
   for (uint32_t ci = 0; ci < ncomp; ++ci) {
@@ -1022,16 +1066,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     int whole = -1;  // 1: the component is its own densest subgraph
     while ((ub - lb) > bias) {
       if (!flow_alloc) {
-        HIPCHK(fx.alloc(maxT * k));
-        HIPCHK(frvs.alloc(maxV));
-        HIPCHK(fex.alloc(maxV + maxT));
-        HIPCHK(fdt.alloc(maxV + maxT));
-        HIPCHK(fda.alloc(maxV + maxT));
-        HIPCHK(fh.alloc(maxV + maxT));
         HIPCHK(freach.alloc(maxV));
-        HIPCHK(fq0.alloc(maxV + maxT + 1));
-        HIPCHK(fq1.alloc(maxV + maxT + 1));
-        HIPCHK(fmisc.alloc(8));
         flow_alloc = true;
       }
       searched = true;
@@ -1213,11 +1248,60 @@ extern "C" cds_status cds_solve_batch(cds_engine *e,
   if (!e || (!graphs && count) || (!params && count) || (!results && count))
     return fail(CDS_ERR_INVALID, "null argument");
   for (size_t i = 0; i < count; ++i) {
-    cds_status s = cds_solve(e, graphs[i], &params[i], &results[i]);
-    if (s != CDS_OK) {
-      for (size_t j = 0; j < i; ++j) cds_result_free(&results[j]);
-      return s;
-    }
+    if (!graphs[i]) return fail(CDS_ERR_INVALID, "graph %zu is null", i);
+    memset(&results[i], 0, sizeof(cds_result));
   }
+  if (count == 0) return CDS_OK;
+  std::lock_guard<std::mutex> lk(e->batch_mu);
+  // Solves are independent; W engines on the same device, each with its own
+  // stream, run them concurrently from W host threads, taking graphs from a
+  // shared counter (largest first when sizes differ, so the tail is short).
+  size_t W = std::min<size_t>(count, 8);
+  if (const char *bw = getenv("CDS_BATCH_WORKERS")) {
+    const long v = atol(bw);
+    if (v >= 1) W = std::min<size_t>(count, (size_t)v);
+  }
+  while (e->workers.size() < W) {
+    cds_opts o;
+    memset(&o, 0, sizeof o);
+    o.device = e->device;
+    o.profile_kernels = e->profile ? 1 : 0;
+    cds_engine *w = nullptr;
+    cds_status st = cds_engine_create(&o, &w);
+    if (st != CDS_OK) return st;
+    w->replay_grid_work = e->replay_grid_work;
+    w->flow_blocks = e->flow_blocks;
+    e->workers.push_back(w);
+  }
+  std::vector<size_t> order(count);
+  for (size_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return graphs[a]->m > graphs[b]->m;
+  });
+  std::vector<cds_status> st(count, CDS_OK);
+  std::vector<std::string> msg(count);
+  std::atomic<size_t> next{0};
+  auto work = [&](cds_engine *w) {
+    (void)hipSetDevice(w->device);
+    for (;;) {
+      const size_t j = next.fetch_add(1);
+      if (j >= count) break;
+      const size_t i = order[j];
+      st[i] = cds_solve(w, graphs[i], &params[i], &results[i]);
+      if (st[i] != CDS_OK) msg[i] = cds_last_error();
+    }
+  };
+  std::vector<std::thread> threads;
+  for (size_t t = 1; t < W; ++t) threads.emplace_back(work, e->workers[t]);
+  work(e->workers[0]);
+  for (auto &t : threads) t.join();
+  for (size_t i = 0; i < count; ++i)
+    if (st[i] != CDS_OK) {
+      const cds_status first = st[i];
+      const std::string m = msg[i];
+      for (size_t j = 0; j < count; ++j)
+        if (st[j] == CDS_OK) cds_result_free(&results[j]);
+      return fail(first, "batch item %zu: %s", i, m.c_str());
+    }
   return CDS_OK;
 }

@@ -1,10 +1,29 @@
-// (c) Exact max flow of one large component on the whole GPU.
+// (c) Exact max flow of one component's vertex/clique network.
 //
-// Same network, orientation and exactness as flow.hip (reversed network,
-// int64 residuals scaled by U, answer = maximum preflow + "can reach the
-// sink" side), for components too big for one workgroup: a dense core of a
-// scale-20 R-MAT graph has ~1e3 vertices and ~1e8 cliques, so vertex nodes
-// have incidence lists of 1e5..1e7 arcs and clique nodes have k.
+// Reference GPU path: createFlowNetwork / preFlow / pushRelabel /
+// globalRelabel (cudaCode/src/helpers.cu:1173-1492), double capacities and
+// a cooperative grid.  Oracle: createFlownetwork + edmondsKarp
+// (cpuCode/c++/src/helpers.cpp:1302-1405), float capacities.
+//
+// Network (oracle orientation): s->v cap cd_v, v->t cap fl(alpha*k),
+// v->clique cap 1, clique->v cap k-1.  Every capacity is multiplied by
+// U = 2^sc so fl(alpha*k) is an integer; flows are int64 and exact.
+//
+// The kernel solves the REVERSED network (source t, sink s) because the
+// oracle's answer set is the vertices reachable from s in the residual of
+// a maximum flow, i.e. the minimal min-cut side, and in the reversed
+// network that set is exactly "can reach the sink" after a maximum
+// PREflow, which a backward BFS from the sink (the global relabel)
+// computes.  The arcs t->v start saturated (excess fl(alpha*k)*U at every
+// vertex) and are never used again.
+//
+// Representation is implicit: per (clique, member slot) one int64 x = net
+// flow vertex->clique in the reversed network, so residual(v->c) =
+// (k-1)U - x and residual(c->v) = U + x; per vertex rvs = residual(v->sink).
+// A dense core of a scale-20 R-MAT graph has ~1e3 vertices and ~1e8
+// cliques, so vertex nodes have incidence lists of 1e5..1e7 arcs and
+// clique nodes have k; small components run the same kernel on one
+// workgroup.
 //
 // One cooperative launch.  A round is four phases separated by grid
 // barriers:
@@ -173,7 +192,9 @@ __device__ unsigned long long global_relabel(const G &g) {
   return total;
 }
 
-__global__ __launch_bounds__(256) void k_grid_flow(FlowProblem P,
+constexpr int kGridFlowThreads = 1024;
+
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
                                                    GridFlowState S,
                                                    long long cap_t, long long U,
                                                    uint32_t max_rounds) {
@@ -422,28 +443,42 @@ __global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
 
 }  // namespace
 
+unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {
+  int per_cu = 0, cus = 0;
+  const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(k_grid_flow), kGridFlowThreads,
+          lds) != hipSuccess ||
+      per_cu < 1)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+      hipSuccess)
+    return 0;
+  // One 1024-thread workgroup per 64k (clique, member) arcs, at most half
+  // the CUs: each of the four grid barriers of a round costs a fence pair
+  // per workgroup, so small networks run on one workgroup.
+  const uint64_t cap = cus > 1 ? (uint64_t)cus / 2 : 1;
+  const uint64_t arcs = (uint64_t)P.T * P.k;
+  uint64_t want = (arcs + 65535) / 65536;
+  if (want > cap) want = cap;
+  if (want < 1) want = 1;
+  if (blocks >= 1 && blocks <= cap) want = blocks;
+  return (unsigned)want;
+}
+
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
-                         int device, hipStream_t s) {
+                         unsigned grid, hipStream_t s) {
+  if (grid < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_grid_ctl_init, dim3(1), dim3(1), 0, s, S.ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  int per_cu = 0, cus = 0;
   const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-           &per_cu, reinterpret_cast<const void *>(k_grid_flow), 256, lds)) !=
-      hipSuccess)
-    return e;
-  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                                 device)) != hipSuccess)
-    return e;
-  if (per_cu < 1) return hipErrorLaunchFailure;
-  const unsigned grid = (unsigned)(cus * (per_cu < 2 ? per_cu : 2));
   FlowProblem p = P;
   GridFlowState st = S;
   void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_grid_flow),
-                                    dim3(grid), dim3(256), args, lds, s);
+  return launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid,
+                     kGridFlowThreads, args, lds, s);
 }
 
 }  // namespace cds

@@ -97,7 +97,6 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
                         uint32_t *core, uint32_t *kill, uint32_t *next,
                         uint32_t *ncount, unsigned long long *killed,
                         hipStream_t s);
-// Whole peel in one cooperative launch (see peel.hip).  Control block:
 // Grid barrier state of a cooperative launch: a generation word and
 // arrival counters (one per group of workgroups blockIdx % 8, plus one for
 // the groups), each in its own 128-byte line.  Counters only grow; zero
@@ -108,6 +107,7 @@ struct GridBarrier {
   unsigned int sub[8 * 32];
 };
 
+// Whole peel in one cooperative launch (see peel.hip).  Control block:
 struct PeelCtl {
   GridBarrier bar;
   unsigned int gmin[2], pad0[30];
@@ -132,7 +132,9 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            PeelCtl *ctl, uint32_t *lv_level,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
-                           int device, unsigned *grid_out, hipStream_t s);
+                           unsigned grid, hipStream_t s);
+// Workgroups for peel_persistent on this graph (0: cannot launch).
+unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device);
 
 // Exact sequential replay of one peel phase (level `level`); see peel.hip.
 struct ReplayOut {
@@ -157,7 +159,7 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *sdeg, DeviceArena &arena,
                         hipEvent_t prepared, hipStream_t s);
 constexpr uint32_t kReplayGridMaxVertices = 16384;
-// Workgroups of the whole-GPU replay (one per CU), 0 if unavailable.
+// Workgroups of the whole-GPU replay (half the CUs), 0 if unavailable.
 unsigned replay_grid_blocks(int device);
 
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
@@ -203,7 +205,7 @@ hipError_t item_core_write(uint64_t nitems, const uint4 *items,
                            const uint32_t *cslot, uint32_t *finc,
                            hipStream_t s);
 
-// ---- max flow (flow.hip) ----
+// ---- max flow (flow_grid.hip) and set accounting (flow.hip) ----
 struct FlowProblem {
   uint32_t k, V, T;       // vertices, cliques of this component
   uint32_t vbase;         // flow-vertex id of local vertex 0
@@ -213,23 +215,9 @@ struct FlowProblem {
   const uint32_t *inc;    // slot ids (subtract sbase) of each vertex
   const uint32_t *cd;     // V clique degrees
 };
-struct FlowState {
-  long long *x;       // T*k: flow vertex->clique on slot (reversed network)
-  long long *rvs;     // V: residual vertex->sink
-  long long *excess;  // V+T
-  int *h;             // V+T heights
-  long long *dtgt, *damt;     // V+T per-round decisions
-  uint32_t *queue0, *queue1;  // V+T BFS queues
-  unsigned long long *misc;   // [0]=F, [1]=rounds, [2]=limit hit, [3]=BFS
-  int *reach;                 // V: 1 if on the source side of the min cut
-};
-// Max flow of one component at scaled sink capacity `cap_t` and unit `U`
-// (all capacities multiplied by U).  Writes misc[0] = F (scaled), reach[].
-hipError_t max_flow(const FlowProblem &P, const FlowState &S, long long cap_t,
-                    long long unit, uint32_t max_rounds, hipStream_t s);
 
-// Grid-wide max flow for large components (flow_grid.hip), one cooperative
-// launch; same answer as max_flow.
+// Exact max flow of one component in one cooperative launch on as many
+// workgroups as its size needs.
 struct GridFlowCtl {
   GridBarrier bar;
   unsigned long long lpack[2];  // vertex work lists: count<<40 | chunks
@@ -252,7 +240,10 @@ struct GridFlowState {
 };
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
-                         int device, hipStream_t s);
+                         unsigned grid, hipStream_t s);
+// Workgroups for grid_max_flow on P (0: cannot launch); `blocks` != 0
+// overrides the size-based choice.
+unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks);
 
 // Cliques of a component whose members all have reach != 0.
 hipError_t count_cliques_in_reach(uint32_t T, uint32_t k, const uint32_t *mem,

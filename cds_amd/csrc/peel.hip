@@ -18,6 +18,9 @@
 // The oracle's max peel-prefix density depends on its exact removal order
 // only inside levels above ceil(max level-boundary density) (the engine
 // proves which); replay_phase re-runs those levels in the oracle's order.
+#include <algorithm>
+#include <cstdlib>
+
 #include "dev.h"
 #include "kernels.h"
 
@@ -744,8 +747,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            PeelCtl *ctl, uint32_t *lv_level,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
-                           int device, unsigned *grid_out, hipStream_t s) {
-  if (n >= (1u << 24)) return hipErrorInvalidValue;
+                           unsigned grid, hipStream_t s) {
+  if (n >= (1u << 24) || grid < 1) return hipErrorInvalidValue;
   hipError_t e;
   if ((e = hipMemsetAsync(lv_level, 0, (size_t)n * 4, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(lv_removed, 0, (size_t)n * 8, s)) != hipSuccess) return e;
@@ -756,31 +759,12 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
   hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A0);
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, n);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  int per_cu = 0, cus = 0;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-           &per_cu, reinterpret_cast<const void *>(k_peel_persistent), kPeelThreads, 0)) !=
-      hipSuccess)
-    return e;
-  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                                 device)) != hipSuccess)
-    return e;
-  if (per_cu < 1) return hipErrorLaunchFailure;
-  // 1024-thread workgroups on half the CUs: every grid barrier costs a
-  // release and an acquire fence per workgroup, and the levels of a power-
-  // law graph are too light to pay for more (R-MAT scale 20, k = 3: 410 ms
-  // with 128 workgroups, 459 ms with 256).
-  unsigned grid = (unsigned)(cus > 1 ? cus / 2 : 1);
-  if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {
-    const unsigned want = (unsigned)atoi(gb);
-    if (want >= 1 && want < grid) grid = want;
-  }
-  if (grid_out) *grid_out = grid;
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
                   &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
                   &sdeg};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_peel_persistent),
-                                    dim3(grid), dim3(kPeelThreads), args, 0, s);
+  return launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
+                     kPeelThreads, args, 0, s);
 }
 
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
@@ -814,8 +798,8 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
     void *args[] = {&k, &level, &M, &pv, &pdeg, (void *)&posmap, &palive,
                     (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
                     &rdead, &partial, &bar, &T_start, &N_start, &out};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_replay_grid),
-                                      dim3(grid), dim3(kReplayThreads), args, 0, s);
+    return launch_grid(reinterpret_cast<const void *>(k_replay_grid), grid,
+                       kReplayThreads, args, 0, s);
   }
   if ((e = scan_u32(pdeg, pioff, M, arena)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_phase_compact, dim3(g), dim3(256), 0, s, k, level,
@@ -828,10 +812,10 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
   return hipGetLastError();
 }
 
-unsigned replay_grid_blocks(int device) {
+// Co-resident workgroups of a cooperative kernel: 0 if it cannot run.
+static unsigned coop_cus(const void *fn, int threads, int device) {
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(k_replay_grid), kReplayThreads, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) !=
           hipSuccess ||
       per_cu < 1)
     return 0;
@@ -839,6 +823,32 @@ unsigned replay_grid_blocks(int device) {
       hipSuccess)
     return 0;
   return (unsigned)cus;
+}
+
+unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device) {
+  const unsigned cus = coop_cus(reinterpret_cast<const void *>(k_peel_persistent),
+                                kPeelThreads, device);
+  if (cus == 0) return 0;
+  // 1024-thread workgroups, at most half the CUs: every grid barrier costs
+  // a release and an acquire fence per workgroup, and the levels of a
+  // power-law graph are too light to pay for more (R-MAT scale 20, k = 3:
+  // 410 ms with 128 workgroups, 459 ms with 256).  Smaller graphs get one
+  // workgroup per 32768 vertices or 2^22 incidence entries, so most run
+  // on one workgroup as a plain launch.
+  const uint64_t cap = cus > 1 ? cus / 2 : 1;
+  uint64_t want = std::max<uint64_t>((n + 32767) / 32768, (k * T + (1u << 22) - 1) >> 22);
+  want = std::min(std::max<uint64_t>(want, 1), cap);
+  if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {
+    const long v = atol(gb);
+    if (v >= 1 && (uint64_t)v <= cap) want = (uint64_t)v;
+  }
+  return (unsigned)want;
+}
+
+unsigned replay_grid_blocks(int device) {
+  const unsigned cus = coop_cus(reinterpret_cast<const void *>(k_replay_grid),
+                                kReplayThreads, device);
+  return cus > 1 ? cus / 2 : cus;
 }
 
 }  // namespace cds

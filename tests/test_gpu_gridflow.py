@@ -1,11 +1,11 @@
 """GPU parity of the whole-GPU kernels and of R-MAT inputs.
 
-The engine switches to the cooperative grid-wide push-relabel for
-components above a node threshold; CDS_GRID_FLOW_NODES=0 forces it for
-every component, so the small graphs of the oracle comparison exercise it
-too (including components that need one flow per alpha query).  Likewise
-CDS_REPLAY_GRID_WORK=0 sends every exact phase replay with at most 16384
-vertices to the whole-GPU replay kernel.
+The max flow runs on as many workgroups as a component's size needs, one
+for the small graphs of the oracle comparison; CDS_FLOW_BLOCKS=3 forces
+three, so the grid barriers and cross-workgroup pushes of that kernel are
+checked on every component (including components that need one flow per
+alpha query).  Likewise CDS_REPLAY_GRID_WORK=0 sends every exact phase
+replay with at most 16384 vertices to the whole-GPU replay kernel.
 """
 import os
 
@@ -35,7 +35,7 @@ def _engine_with(env):
 
 @pytest.fixture(scope="module")
 def grid_engine():
-    eng = _engine_with({"CDS_GRID_FLOW_NODES": "0"})
+    eng = _engine_with({"CDS_FLOW_BLOCKS": "3"})
     yield eng
     eng.close()
 
