@@ -108,6 +108,9 @@ __device__ __forceinline__ int64_t find_sorted64(const uint32_t *a, int64_t len,
 template <class B>
 __device__ __forceinline__ void grid_barrier(B *b) {
   __syncthreads();
+  // One workgroup: its waves share the CU's caches, so the workgroup
+  // barrier (with its workgroup-scope fence) is the whole barrier.
+  if (gridDim.x == 1) return;
   if (threadIdx.x == 0) {
     __threadfence();
     const unsigned gen =

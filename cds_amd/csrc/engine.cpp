@@ -174,10 +174,25 @@ struct CoopGate {
   CoopGate &operator=(const CoopGate &) = delete;
 };
 
+// The engine's pinned host block: the first kPinnedScratch bytes are free
+// for callers, the rest stages small readbacks (a copy into pageable
+// memory goes through the runtime's shared staging buffer instead).
+constexpr size_t kPinnedScratch = 4096;
+constexpr size_t kPinnedBytes = 64 * 1024;
+
 template <class T>
 cds_status readback(cds_engine *e, T *host, const T *dev, size_t count) {
-  HIPCHK(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost,
-                        e->stream));
+  const size_t bytes = count * sizeof(T);
+  char *stage = static_cast<char *>(e->pinned) + kPinnedScratch;
+  const bool in_pinned = (char *)host >= (char *)e->pinned &&
+                         (char *)host < (char *)e->pinned + kPinnedBytes;
+  if (bytes <= kPinnedBytes - kPinnedScratch && !in_pinned) {
+    HIPCHK(hipMemcpyAsync(stage, dev, bytes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    memcpy(host, stage, bytes);
+    return CDS_OK;
+  }
+  HIPCHK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   return CDS_OK;
 }
@@ -234,6 +249,230 @@ struct Component {
   uint32_t first = 0;    // smallest original vertex id
 };
 
+// One max flow the search needs: component `comp`, sink capacity capT and
+// unit U (every capacity scaled by U).
+struct FlowRequest {
+  uint32_t comp = 0;
+  long long capT = 0, U = 0;
+};
+
+// The oracle's dynamicExact (cpuCode/c++/src/helpers.cpp:1120-1254) for one
+// graph, as a state machine so one driver can run it alone and another can
+// advance many graphs in lockstep with batched flows.  The arithmetic is the
+// oracle's: float lb/ub/alpha/bias, cumulative ub over components (UB1/UB2),
+// decision |F - kT| < 1e-2f evaluated exactly on the scaled integer flow,
+// strict improvement of the float density.  A component first gets one
+// flow at its own density kT/V (capacities scaled by V); if that saturates,
+// the component is its own densest subgraph and every alpha query is
+// answered without a flow.
+class ExactSearch {
+ public:
+  ExactSearch(uint32_t k, bool ub1, bool ub2, const std::vector<Component> &comps,
+              const uint32_t *fcd, double core_density, uint64_t kmax)
+      : k_(k), ub1_(ub1), ub2_(ub2), comps_(comps), fcd_(fcd) {
+    const uint32_t nc = (uint32_t)comps.size();
+    std::vector<double> cdens(nc);
+    for (uint32_t c = 0; c < nc; ++c)
+      cdens[c] = (double)(int64_t)comps[c].T / ((double)comps[c].V);
+    // initial lower bound: the densest component, in float, then the core
+    // density if its ceiling is larger (helpers.cpp:1128-1143)
+    uint32_t current = 0;
+    for (uint32_t c = 0; c < nc; ++c)
+      if (lower_ < cdens[c]) {
+        lower_ = (float)cdens[c];
+        current = c;
+      }
+    if (std::ceil(lower_) < std::ceil(core_density)) lower_ = (float)core_density;
+    upper_ = (float)(int64_t)kmax;
+    best_density = lower_;
+    best_comp = current;
+    best_cliques = nc ? comps[current].T : 0;
+    best_size = nc ? comps[current].V : 0;
+    for (auto d : cdens)
+      if (paper_lower_bound < d) paper_lower_bound = d;
+  }
+
+  // Advances the search until it needs a flow (true, *req filled) or ends
+  // (false).  A failure leaves *err != CDS_OK (message already set).
+  bool next(FlowRequest *req, cds_status *err) {
+    *err = CDS_OK;
+    for (;;) {
+      switch (state_) {
+        case kCompStart: {
+          if (ci_ >= comps_.size()) {
+            state_ = kDone;
+            return false;
+          }
+          const Component &C = comps_[ci_];
+          if (ub1_) {
+            ub1_val = ub1_value(k_, C.T);
+            if (ub1_val < upper_) upper_ = ub1_val;
+          }
+          if (ub2_) {
+            ub2_val = ub2_value(k_, C.T, fcd_ + C.vbase, C.V);
+            if (ub2_val < upper_) upper_ = ub2_val;
+          }
+          // exact(): helpers.cpp:1256-1300
+          lb_ = lower_;
+          ub_ = upper_;
+          alpha_ = (lb_ + ub_) * 0.5f;
+          const int32_t vv = (int32_t)((uint32_t)C.V * (uint32_t)(C.V - 1));
+          bias_ = 1.0f / (float)vv;
+          if (bias_ < 1e-2f) bias_ = 1e-2f;
+          target_ = (float)(int64_t)(C.T * k_);
+          res.assign(C.V, 1);
+          res_cliques_ = C.T;
+          searched_ = false;
+          whole_ = -1;
+          state_ = kLoop;
+          break;
+        }
+        case kLoop: {
+          if (!((ub_ - lb_) > bias_)) {
+            finish_component();
+            break;
+          }
+          const Component &C = comps_[ci_];
+          searched_ = true;
+          const float cap_t = alpha_ * (float)k_;
+          int e2 = 0;
+          std::frexp(cap_t, &e2);
+          sc_ = std::max(0, 24 - e2);
+          if (sc_ > 40) {
+            *err = fail(CDS_ERR_RANGE, "alpha %g too small for exact scaling", alpha_);
+            return false;
+          }
+          U_ = 1ll << sc_;
+          capT_ = (long long)std::ldexp((double)cap_t, sc_);
+          if (whole_ < 0) {
+            req->comp = ci_;
+            req->capT = (long long)(C.T * k_);
+            req->U = (long long)C.V;
+            state_ = kWaitCert;
+            return true;
+          }
+          if (whole_ == 1) {
+            __int128 g = ((__int128)(C.T * k_) << sc_) - (__int128)capT_ * C.V;
+            if (g < 0) g = 0;
+            decide(((__int128)(C.T * k_) << sc_) - g, false);
+            break;
+          }
+          req->comp = ci_;
+          req->capT = capT_;
+          req->U = U_;
+          state_ = kWaitFlow;
+          return true;
+        }
+        default:
+          *err = fail(CDS_ERR_INTERNAL, "exact search advanced while waiting");
+          return false;
+      }
+    }
+  }
+
+  // Result of the last request.  Returns true if the search now needs the
+  // min-cut side of that flow (feed_reach), false if it can advance.
+  bool feed_flow(long long F) {
+    const Component &C = comps_[ci_];
+    if (state_ == kWaitCert) {
+      // back to the loop with the same alpha: answered analytically if the
+      // component is its own densest subgraph, else by its own flow
+      whole_ = (__int128)F == (__int128)(C.T * k_) * C.V ? 1 : 0;
+      if (whole_) certificates++;
+      state_ = kLoop;
+      return false;
+    }
+    flows++;
+    return decide(F, true);
+  }
+
+  // reach[v] != 0 for the component's min-cut side; `cliques` inside it.
+  void feed_reach(const int *reach, uint64_t cliques) {
+    const Component &C = comps_[ci_];
+    for (uint32_t v = 0; v < C.V; ++v) res[v] = reach[v] ? 1 : -1;
+    res_cliques_ = cliques;
+    alpha_ = (lb_ + ub_) * 0.5f;
+    state_ = kLoop;
+  }
+
+  bool done() const { return state_ == kDone; }
+
+  // results
+  float ub1_val = 0, ub2_val = 0;
+  double best_density = 0, paper_lower_bound = 0;
+  uint32_t best_comp = 0, best_size = 0;
+  uint64_t best_cliques = 0;
+  std::vector<int> best_res;  // empty = the whole component
+  std::vector<int> res;
+  std::vector<float> tr_lb, tr_ub, tr_alpha;
+  std::vector<uint32_t> tr_comp;
+  std::vector<double> tr_margin;
+  uint32_t flows_solved = 0, certificates = 0, flows = 0;
+
+ private:
+  enum State { kCompStart, kLoop, kWaitCert, kWaitFlow, kWaitReach, kDone };
+
+  // Decision for the current alpha from F (scaled by 2^sc).  Returns true
+  // if the min-cut side of the flow is needed.
+  bool decide(__int128 F, bool from_flow) {
+    const Component &C = comps_[ci_];
+    const bool eq = flow_equal(F, target_, sc_);
+    tr_comp.push_back(ci_);
+    tr_lb.push_back(lb_);
+    tr_ub.push_back(ub_);
+    tr_alpha.push_back(alpha_);
+    tr_margin.push_back((double)(C.T * k_) - std::ldexp((double)F, -sc_));
+    if (eq) {
+      ub_ = alpha_;
+    } else {
+      lb_ = alpha_;
+      if (from_flow) {
+        state_ = kWaitReach;
+        return true;
+      }
+      for (uint32_t v = 0; v < C.V; ++v) res[v] = 1;
+      res_cliques_ = C.T;
+    }
+    alpha_ = (lb_ + ub_) * 0.5f;
+    state_ = kLoop;
+    return false;
+  }
+
+  // result accounting of one component (helpers.cpp:1210-1252)
+  void finish_component() {
+    const Component &C = comps_[ci_];
+    if (searched_) flows_solved++;
+    uint32_t vcount = 0;
+    for (uint32_t v = 0; v < C.V; ++v) vcount += res[v] != -1;
+    if (vcount == 0) vcount = C.V;
+    const float density = (float)(int64_t)res_cliques_ / (float)vcount;
+    if (density > best_density) {
+      best_density = density;
+      lower_ = density;
+      best_comp = ci_;
+      best_res = res;
+      best_size = vcount;
+      best_cliques = res_cliques_;
+    }
+    ++ci_;
+    state_ = kCompStart;
+  }
+
+  uint32_t k_;
+  bool ub1_, ub2_;
+  const std::vector<Component> &comps_;
+  const uint32_t *fcd_;
+  float lower_ = 0.0f, upper_ = 0.0f;
+  State state_ = kCompStart;
+  uint32_t ci_ = 0;
+  float lb_ = 0, ub_ = 0, alpha_ = 0, bias_ = 0, target_ = 0;
+  int sc_ = 0;
+  long long U_ = 0, capT_ = 0;
+  uint64_t res_cliques_ = 0;
+  bool searched_ = false;
+  int whole_ = -1;
+};
+
 }  // namespace
 
 // ===========================================================================
@@ -264,7 +503,7 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
       uint64_t keep = ~0ull;  // keep freed blocks cached in the pool
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
-    r = hipHostMalloc(&e->pinned, 4096, hipHostMallocDefault);
+    r = hipHostMalloc(&e->pinned, kPinnedBytes, hipHostMallocDefault);
   }
   if (r != hipSuccess) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -445,6 +684,71 @@ struct Level {
   uint32_t level;
   uint64_t removed, killed;
 };
+
+// Peel summary of one graph from its level table (ascending levels).
+// The state before levels[i] is the clique core C_level; the oracle's
+// density rows include each of them (helpers.cpp:125-134), so rho starts
+// as the largest boundary density and k' = ceil(rho).  Levels above k'
+// may hold intermediate states of higher density: `replay` lists (in
+// descending level order) those with at least two vertices whose bound
+// ceil((T_end + (M-1) l) / (N_end + M - 1)) exceeds that k'.
+struct PeelSummary {
+  std::vector<uint64_t> rm_before, kl_before;
+  double rho = 0;
+  int64_t kp = 0;
+  uint64_t kmax = 0;
+  std::vector<size_t> replay;
+};
+
+cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
+                            uint64_t T, PeelSummary *S) {
+  S->rm_before.assign(levels.size(), 0);
+  S->kl_before.assign(levels.size(), 0);
+  uint64_t r = 0, kk = 0;
+  for (size_t i = 0; i < levels.size(); ++i) {
+    S->rm_before[i] = r;
+    S->kl_before[i] = kk;
+    r += levels[i].removed;
+    kk += levels[i].killed;
+    if (levels[i].removed) S->kmax = std::max<uint64_t>(S->kmax, levels[i].level);
+  }
+  if (kk != T || r != n)
+    return fail(CDS_ERR_INTERNAL, "peel removed %llu of %llu vertices, killed %llu of %llu cliques",
+                (unsigned long long)r, (unsigned long long)n,
+                (unsigned long long)kk, (unsigned long long)T);
+  S->rho = 0;
+  for (size_t i = 0; i < levels.size(); ++i) {
+    const uint64_t N = n - S->rm_before[i], Tr = T - S->kl_before[i];
+    const double d = (double)(int64_t)Tr / static_cast<double>(N);
+    if (i == 0 || d > S->rho) S->rho = d;
+  }
+  S->kp = (int64_t)std::ceil(S->rho);
+  S->replay.clear();
+  for (size_t ii = levels.size(); ii-- > 0;) {
+    const Level &lv = levels[ii];
+    if ((int64_t)lv.level <= S->kp) break;
+    const uint64_t M = lv.removed;
+    if (M < 2) continue;
+    const uint64_t N_end = n - S->rm_before[ii] - M;
+    const uint64_t T_end = T - S->kl_before[ii] - lv.killed;
+    // f <= (T_end + x*l) / (N_end + x) for x = M-1 remaining phase vertices
+    const unsigned __int128 num =
+        (unsigned __int128)T_end + (unsigned __int128)(M - 1) * lv.level;
+    const unsigned __int128 den = N_end + M - 1;
+    const unsigned __int128 ceil_ub = (num + den - 1) / den;
+    if ((int64_t)ceil_ub <= S->kp) continue;
+    S->replay.push_back(ii);
+  }
+  return CDS_OK;
+}
+
+// Folds one replayed phase's best intermediate density into rho / k'.
+void apply_replay(PeelSummary *S, const ReplayOut &ro) {
+  if (!ro.found) return;
+  const double d = (double)(int64_t)ro.best_num / static_cast<double>(ro.best_den);
+  if (d > S->rho) S->rho = d;
+  S->kp = std::max(S->kp, (int64_t)std::ceil(d));
+}
 
 }  // namespace
 
@@ -632,30 +936,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     R->peel_alive_scanned = scanned;
   }
 
-  // Boundary densities: the state before level levels[i] is the clique core
-  // C_level; the oracle's rows include each of them (helpers.cpp:125-134).
-  std::vector<uint64_t> rm_before(levels.size()), kl_before(levels.size());
-  {
-    uint64_t r = 0, kk = 0;
-    for (size_t i = 0; i < levels.size(); ++i) {
-      rm_before[i] = r;
-      kl_before[i] = kk;
-      r += levels[i].removed;
-      kk += levels[i].killed;
-    }
-    if (kk != T)
-      return fail(CDS_ERR_INTERNAL, "peel killed %llu of %llu cliques",
-                  (unsigned long long)kk, (unsigned long long)T);
-  }
-  double rho = 0;
-  for (size_t i = 0; i < levels.size(); ++i) {
-    const uint64_t N = n - rm_before[i], Tr = T - kl_before[i];
-    const double d = (double)(int64_t)Tr / static_cast<double>(N);
-    if (i == 0 || d > rho) rho = d;
-  }
-  int64_t kp = (int64_t)std::ceil(rho);
-  // Levels above ceil(max boundary density) may hold intermediate states of
-  // higher density; replay those whose bound can exceed the current k'.
+  PeelSummary PS;
+  CHK(summarize_levels(levels, n, T, &PS));
+  const std::vector<uint64_t> &rm_before = PS.rm_before, &kl_before = PS.kl_before;
   {
     DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
     DBuf<uint64_t> pioff(&A);
@@ -663,18 +946,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     DBuf<ReplayOut> rout(&A);
     bool allocated = false;
     const int ev_r0 = tm.mark();
-    for (size_t ii = levels.size(); ii-- > 0;) {
+    for (size_t ii : PS.replay) {
       const Level &lv = levels[ii];
-      if ((int64_t)lv.level <= kp) break;
       const uint64_t M = lv.removed;
-      if (M < 2) continue;
-      const uint64_t N_end = n - rm_before[ii] - M;
-      const uint64_t T_end = T - kl_before[ii] - lv.killed;
-      // f <= (T_end + x*l) / (N_end + x) for x = M-1 remaining phase vertices
-      const unsigned __int128 num = (unsigned __int128)T_end + (unsigned __int128)(M - 1) * lv.level;
-      const unsigned __int128 den = N_end + M - 1;
-      const unsigned __int128 ceil_ub = (num + den - 1) / den;
-      if ((int64_t)ceil_ub <= kp) continue;
       if (!allocated) {
         HIPCHK(pv.alloc(n));
         HIPCHK(pdeg.alloc(n));
@@ -720,21 +994,16 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       if (ro.steps != M)
         return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
                     lv.level, ro.steps, (unsigned long long)M);
-      if (ro.found) {
-        const double d = (double)(int64_t)ro.best_num / static_cast<double>(ro.best_den);
-        if (d > rho) rho = d;
-        kp = std::max(kp, (int64_t)std::ceil(d));
-      }
+      apply_replay(&PS, ro);
     }
     const int ev_r1 = tm.mark();
     HIPCHK(hipStreamSynchronize(st));
     R->t_replay_ms = tm.ms(ev_r0, ev_r1);
   }
-  R->peel_max_density = rho;
+  R->peel_max_density = PS.rho;
+  const int64_t kp = PS.kp;
   R->k_prime = (uint64_t)kp;
-  uint64_t kmax = 0;
-  for (auto &lv : levels)
-    if (lv.removed) kmax = std::max<uint64_t>(kmax, lv.level);
+  const uint64_t kmax = PS.kmax;
   R->kmax = kmax;
   const int ev_peel = tm.mark();
 
@@ -929,33 +1198,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   const int ev_cc = tm.mark();
 
   // ---- (c) dynamicExact (helpers.cpp:1120-1254) ----
-  // Initial lower bound: the densest component, in float, then the core
-  // density if its ceiling is larger (helpers.cpp:1128-1143).
-  std::vector<double> cdens(ncomp);
-  for (uint32_t c = 0; c < ncomp; ++c)
-    cdens[c] = (double)(int64_t)comps[c].T / ((double)comps[c].V);
-  float lower = 0.0f;
-  uint32_t current = 0;
-  for (uint32_t c = 0; c < ncomp; ++c)
-    if (lower < cdens[c]) {
-      lower = (float)cdens[c];
-      current = c;
-    }
-  if (std::ceil(lower) < std::ceil(R->core_density)) lower = (float)R->core_density;
-  float upper = (float)(int64_t)kmax;
-  double best_density = lower;
-  uint32_t best_comp = current;
-  std::vector<int> best_res;  // empty = the whole component
-  uint64_t best_cliques = comps[current].T;
-  uint32_t best_size = comps[current].V;
-  double lbp = 0;
-  for (auto d : cdens)
-    if (lbp < d) lbp = d;
-  R->paper_lower_bound = lbp;
-
-  std::vector<float> tr_lb, tr_ub, tr_alpha;
-  std::vector<uint32_t> tr_comp;
-  std::vector<double> tr_margin;
+  ExactSearch search(k, p->ub1 != 0, p->ub2 != 0, comps, hfcd.data(),
+                     R->core_density, kmax);
+  R->paper_lower_bound = search.paper_lower_bound;
   uint64_t sweeps = 0;
   // flow state, sized for the largest component
   uint32_t maxV = 0;
@@ -966,7 +1211,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   }
   DBuf<int> freach(&A);
   bool flow_alloc = false;
-  std::vector<int> res, reach_h;
+  DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
+  DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
+  DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
+  DBuf<uint64_t> glc0(&A), glc1(&A);
+  DBuf<GridFlowCtl> gctl(&A);
+  std::vector<int> reach_h;
   // Exact max flow of component ci with sink capacity capT (all
   // capacities in units of 1/U); leaves the min-cut side in freach.
   auto run_flow = [&](uint32_t ci, long long capT, long long U,
@@ -989,27 +1239,25 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     P.inc = finc.p;
     P.cd = fcd.p + C.vbase;
     // push-relabel on as many workgroups as the network needs (one for a
-    // small component); one cooperative launch in flight per device
-    DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
-    DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
-    DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
-    DBuf<uint64_t> glc0(&A), glc1(&A);
-    DBuf<GridFlowCtl> gctl(&A);
-    HIPCHK(gx.alloc(C.T * k));
-    HIPCHK(grvs.alloc(C.V));
-    HIPCHK(gexv.alloc(C.V));
-    HIPCHK(grem.alloc(C.V));
-    HIPCHK(gexc.alloc(C.T));
-    HIPCHK(ghv.alloc(C.V));
-    HIPCHK(ghmin.alloc(C.V));
-    HIPCHK(ghc.alloc(C.T));
-    HIPCHK(gl0.alloc(C.V));
-    HIPCHK(gl1.alloc(C.V));
-    HIPCHK(glc0.alloc(C.V));
-    HIPCHK(glc1.alloc(C.V));
-    HIPCHK(gq0.alloc(C.T));
-    HIPCHK(gq1.alloc(C.T));
-    HIPCHK(gctl.alloc(1));
+    // small component); state sized for the largest component, allocated
+    // at the first flow of the solve
+    if (!gx.p) {
+      HIPCHK(gx.alloc(maxT * k));
+      HIPCHK(grvs.alloc(maxV));
+      HIPCHK(gexv.alloc(maxV));
+      HIPCHK(grem.alloc(maxV));
+      HIPCHK(gexc.alloc(maxT));
+      HIPCHK(ghv.alloc(maxV));
+      HIPCHK(ghmin.alloc(maxV));
+      HIPCHK(ghc.alloc(maxT));
+      HIPCHK(gl0.alloc(maxV));
+      HIPCHK(gl1.alloc(maxV));
+      HIPCHK(glc0.alloc(maxV));
+      HIPCHK(glc1.alloc(maxV));
+      HIPCHK(gq0.alloc(maxT));
+      HIPCHK(gq1.alloc(maxT));
+      HIPCHK(gctl.alloc(1));
+    }
     GridFlowState G;
     G.x = gx.p;
     G.rvs = grvs.p;
@@ -1043,119 +1291,46 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     *F = (long long)hc.sink;
     return CDS_OK;
   };
-  for (uint32_t ci = 0; ci < ncomp; ++ci) {
-    const Component &C = comps[ci];
-    if (p->ub1) {
-      R->ub1_val = ub1_value(k, C.T);
-      if (R->ub1_val < upper) upper = R->ub1_val;
-    }
-    if (p->ub2) {
-      R->ub2_val = ub2_value(k, C.T, hfcd.data() + C.vbase, C.V);
-      if (R->ub2_val < upper) upper = R->ub2_val;
-    }
-    // exact(): helpers.cpp:1256-1300
-    float lb = lower, ub = upper;
-    float alpha = (lb + ub) * 0.5f;
-    const int32_t vv = (int32_t)((uint32_t)C.V * (uint32_t)(C.V - 1));
-    float bias = 1.0f / (float)vv;
-    if (bias < 1e-2f) bias = 1e-2f;
-    const float target = (float)(int64_t)(C.T * k);
-    res.assign(C.V, 1);
-    uint64_t res_cliques = C.T;
-    bool searched = false;
-    int whole = -1;  // 1: the component is its own densest subgraph
-    while ((ub - lb) > bias) {
+  {
+    FlowRequest rq;
+    cds_status serr = CDS_OK;
+    while (search.next(&rq, &serr)) {
       if (!flow_alloc) {
         HIPCHK(freach.alloc(maxV));
         flow_alloc = true;
       }
-      searched = true;
-      const float cap_t = alpha * (float)k;
-      int e2 = 0;
-      std::frexp(cap_t, &e2);
-      const int sc = std::max(0, 24 - e2);
-      if (sc > 40)
-        return fail(CDS_ERR_RANGE, "alpha %g too small for exact scaling", alpha);
-      const long long U = 1ll << sc;
-      const long long capT = (long long)std::ldexp((double)cap_t, sc);
-      if (whole < 0) {
-        // One flow at the component's own density c0 = kT/V (capacities
-        // scaled by V): if it saturates, no subset is denser, g(c) =
-        // max(0, kT - cV) for every c and the minimal maximiser is the whole
-        // component below c0 and empty above it.
-        long long F0 = 0;
-        CHK(run_flow(ci, (long long)(C.T * k), (long long)C.V, &F0));
-        whole = (__int128)F0 == (__int128)(C.T * k) * C.V ? 1 : 0;
-        if (whole) R->density_certificates++;
+      long long F = 0;
+      CHK(run_flow(rq.comp, rq.capT, rq.U, &F));
+      if (search.feed_flow(F)) {
+        const Component &C = comps[rq.comp];
+        reach_h.resize(C.V);
+        CHK(readback(e, reach_h.data(), freach.p, C.V));
+        HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
+                                      C.vbase, freach.p, acc.p, st));
+        unsigned long long cc = 0;
+        CHK(readback(e, &cc, acc.p, 1));
+        search.feed_reach(reach_h.data(), cc);
       }
-      __int128 F = 0;
-      bool from_flow = false;
-      if (whole == 1) {
-        __int128 g = ((__int128)(C.T * k) << sc) - (__int128)capT * C.V;
-        if (g < 0) g = 0;
-        F = ((__int128)(C.T * k) << sc) - g;
-      } else {
-        long long Ff = 0;
-        CHK(run_flow(ci, capT, U, &Ff));
-        F = Ff;
-        from_flow = true;
-      }
-      const bool eq = flow_equal(F, target, sc);
-      tr_comp.push_back(ci);
-      tr_lb.push_back(lb);
-      tr_ub.push_back(ub);
-      tr_alpha.push_back(alpha);
-      tr_margin.push_back((double)(C.T * k) - std::ldexp((double)F, -sc));
-      if (eq) {
-        ub = alpha;
-      } else {
-        lb = alpha;
-        if (!from_flow) {
-          for (uint32_t v = 0; v < C.V; ++v) res[v] = 1;
-          res_cliques = C.T;
-        } else {
-          reach_h.resize(C.V);
-          CHK(readback(e, reach_h.data(), freach.p, C.V));
-          for (uint32_t v = 0; v < C.V; ++v) res[v] = reach_h[v] ? 1 : -1;
-          HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
-                                        C.vbase, freach.p, acc.p, st));
-          unsigned long long cc = 0;
-          CHK(readback(e, &cc, acc.p, 1));
-          res_cliques = cc;
-        }
-      }
-      alpha = (lb + ub) * 0.5f;
     }
-    if (searched) R->flow_networks_solved++;
-    // count motifs and vertices in the result (helpers.cpp:1210-1252)
-    uint32_t vcount = 0;
-    for (uint32_t v = 0; v < C.V; ++v) vcount += res[v] != -1;
-    uint64_t ccount_res = res_cliques;
-    if (vcount == 0) vcount = C.V;
-    const float density = (float)(int64_t)ccount_res / (float)vcount;
-    if (density > best_density) {
-      best_density = density;
-      lower = density;
-      best_comp = ci;
-      best_res = res;
-      best_size = vcount;
-      best_cliques = ccount_res;
-    }
+    if (serr != CDS_OK) return serr;
   }
+  R->ub1_val = search.ub1_val;
+  R->ub2_val = search.ub2_val;
+  R->flow_networks_solved = search.flows_solved;
+  R->density_certificates = search.certificates;
   R->flow_sweeps = sweeps;
   const int ev_exact = tm.mark();
 
   // ---- result ----
-  R->density = (float)best_density;
-  R->size = best_size;
+  R->density = (float)search.best_density;
+  R->size = search.best_size;
   std::vector<uint32_t> verts;
   {
-    const Component &C = comps[best_comp];
+    const Component &C = comps[search.best_comp];
     for (uint32_t v = 0; v < C.V; ++v)
-      if (best_res.empty() || best_res[v] != -1)
+      if (search.best_res.empty() || search.best_res[v] != -1)
         verts.push_back(fv_orig[C.vbase + v]);
   }
-  (void)best_cliques;
   R->set_size = (uint32_t)verts.size();
   R->vertices = host_alloc<uint32_t>(verts.size());
   memcpy(R->vertices, verts.data(), verts.size() * 4);
@@ -1172,7 +1347,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     CHK(readback(e, &sc, acc.p, 1));
     R->set_cliques = sc;
   }
-  const size_t tl = tr_alpha.size();
+  const size_t tl = search.tr_alpha.size();
   R->trace_len = (uint32_t)tl;
   R->trace_lb = host_alloc<float>(tl);
   R->trace_ub = host_alloc<float>(tl);
@@ -1180,11 +1355,11 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   R->trace_component = host_alloc<uint32_t>(tl);
   R->trace_margin = host_alloc<double>(tl);
   for (size_t i = 0; i < tl; ++i) {
-    R->trace_lb[i] = tr_lb[i];
-    R->trace_ub[i] = tr_ub[i];
-    R->trace_alpha[i] = tr_alpha[i];
-    R->trace_component[i] = tr_comp[i];
-    R->trace_margin[i] = tr_margin[i];
+    R->trace_lb[i] = search.tr_lb[i];
+    R->trace_ub[i] = search.tr_ub[i];
+    R->trace_alpha[i] = search.tr_alpha[i];
+    R->trace_component[i] = search.tr_comp[i];
+    R->trace_margin[i] = search.tr_margin[i];
   }
   if (p->want_vertex_arrays) {
     R->clique_degree = host_alloc<uint64_t>(n);

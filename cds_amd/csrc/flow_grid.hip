@@ -55,10 +55,6 @@ constexpr uint32_t kGridLdsVertices = 4096;  // LDS excess accumulator, 32 KB
 constexpr uint32_t kGridRelabel = 24;
 constexpr uint64_t kMask40 = (1ull << 40) - 1;
 
-__device__ __forceinline__ void gsync(GridFlowCtl *ctl) {
-  grid_barrier(&ctl->bar);
-}
-
 template <class T> __device__ __forceinline__ T ald(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -80,7 +76,16 @@ struct G {
   GridFlowCtl *ctl;
   uint32_t gtid, gthreads, gwave, gwaves;
   bool leader;
+  bool block;  // the problem owns one workgroup of a batched launch
 };
+
+// Barrier over the threads working on one problem.
+__device__ __forceinline__ void gsync(const G &g) {
+  if (g.block)
+    __syncthreads();
+  else
+    grid_barrier(&g.ctl->bar);
+}
 
 __device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
   return (g.ioff[v + 1] - g.ioff[v] + kGridChunk - 1) / kGridChunk;
@@ -125,13 +130,13 @@ __device__ unsigned long long global_relabel(const G &g) {
     ctl->cqn[1] = 0;
     ctl->active = 0;
   }
-  gsync(ctl);
+  gsync(g);
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
     if (g.rvs[v] > 0) {
       g.hv[v] = 1;
       list_push(g, 0, v);
     }
-  gsync(ctl);
+  gsync(g);
   int vb = 0, cb = 0, d = 1;
   for (;;) {
     // vertices at distance d -> cliques at d+1 (arc clique->v residual U+x)
@@ -143,7 +148,7 @@ __device__ unsigned long long global_relabel(const G &g) {
       ctl->cqn[cb] = 0;
       ctl->lpack[1 - vb] = 0;
     }
-    gsync(ctl);
+    gsync(g);
     for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
       uint32_t v;
       uint64_t lo, hi;
@@ -157,7 +162,7 @@ __device__ unsigned long long global_relabel(const G &g) {
           g.cq[cb][atomicAdd(&ctl->cqn[cb], 1u)] = c;
       }
     }
-    gsync(ctl);
+    gsync(g);
     // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x)
     const uint32_t nc = ald(&ctl->cqn[cb]);
     for (uint32_t i = g.gtid; i < nc; i += g.gthreads) {
@@ -170,7 +175,7 @@ __device__ unsigned long long global_relabel(const G &g) {
         if (atomicCAS(&g.hv[v], g.INF, d + 2) == g.INF) list_push(g, 1 - vb, v);
       }
     }
-    gsync(ctl);
+    gsync(g);
     vb = 1 - vb;
     cb = 1 - cb;
     d += 2;
@@ -182,23 +187,22 @@ __device__ unsigned long long global_relabel(const G &g) {
     act += g.exc[c] > 0 && g.hc[c] < g.INF;
   act = wave_sum64(act);
   if (lane() == 0 && act) atomicAdd(&ctl->active, act);
-  gsync(ctl);
+  gsync(g);
   const unsigned long long total = ald(&ctl->active);
   if (g.leader) {
     ctl->lpack[0] = 0;
     ctl->lpack[1] = 0;
   }
-  gsync(ctl);
+  gsync(g);
   return total;
 }
 
 constexpr int kGridFlowThreads = 1024;
+constexpr int kFlowBatchThreads = 512;
 
-__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
-                                                   GridFlowState S,
-                                                   long long cap_t, long long U,
-                                                   uint32_t max_rounds) {
-  G g;
+// Problem fields of G; the caller sets the thread geometry.
+__device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
+                                           const GridFlowState &S, long long U) {
   g.k = P.k;
   g.V = P.V;
   g.T = P.T;
@@ -224,15 +228,16 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   g.cq[0] = S.cq0;
   g.cq[1] = S.cq1;
   g.ctl = S.ctl;
-  g.gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  g.gthreads = gridDim.x * blockDim.x;
-  g.gwave = g.gtid / kWave;
-  g.gwaves = g.gthreads / kWave;
-  g.leader = g.gtid == 0;
+}
+
+// The max flow of one problem by the threads of g (a whole cooperative
+// grid, or one workgroup of a batched launch).  lds_ex (LDS, V entries if
+// lds_v) accumulates the excess cliques push to vertices.
+__device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
+                         long long cap_t, long long U, uint32_t max_rounds,
+                         long long *lds_ex, bool lds_v) {
   GridFlowCtl *ctl = S.ctl;
   const uint64_t slots = (uint64_t)g.T * g.k;
-  extern __shared__ long long lds_ex[];
-  const bool lds_v = g.V <= kGridLdsVertices;
 
   for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
   for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
     ctl->limit = 0;
     ctl->relabels = 0;
   }
-  gsync(ctl);
+  gsync(g);
   unsigned long long active = global_relabel(g);
   uint32_t rounds = 0, relabels = 1, since = 0;
   unsigned long long sink = 0;
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
         if (a) atomicAdd((unsigned long long *)&g.exv[i], (unsigned long long)a);
       }
     }
-    gsync(ctl);
+    gsync(g);
     // 2. vertices push to the sink; active vertices post their chunks
     for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
       long long e = g.exv[v];
@@ -333,7 +338,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
     }
     act = wave_sum64(act);
     if (lane() == 0 && act) atomicAdd(&ctl->active, act);
-    gsync(ctl);
+    gsync(g);
     const unsigned long long pack = ald(&ctl->lpack[0]);
     const uint32_t nv = (uint32_t)(pack >> 40);
     const uint64_t nch = pack & kMask40;
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
         before += __shfl(incl, kWave - 1, kWave);
       }
     }
-    gsync(ctl);
+    gsync(g);
     // 4. remainders become excess; relabel vertices that kept some
     for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
       const uint32_t v = g.list[0][i];
@@ -411,7 +416,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
         g.hv[v] = hm >= g.INF - 1 ? g.INF : hm + 1;
       }
     }
-    gsync(ctl);
+    gsync(g);
     ++rounds;
     ++since;
     if (any == 0 || since >= kGridRelabel) {
@@ -432,6 +437,68 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   }
 }
 
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
+                                                   GridFlowState S,
+                                                   long long cap_t, long long U,
+                                                   uint32_t max_rounds) {
+  G g;
+  flow_setup(g, P, S, U);
+  g.gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  g.gthreads = gridDim.x * blockDim.x;
+  g.gwave = g.gtid / kWave;
+  g.gwaves = g.gthreads / kWave;
+  g.leader = g.gtid == 0;
+  g.block = false;
+  extern __shared__ long long lds_ex[];
+  flow_run(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
+}
+
+// Many independent problems, one workgroup each.  Each problem's control
+// block is initialised here; lds_cap = LDS excess slots per workgroup.
+__global__ __launch_bounds__(kFlowBatchThreads) void k_flow_batch(
+    const FlowBatchItem *__restrict__ items, uint32_t max_rounds,
+    uint32_t lds_cap) {
+  const FlowBatchItem &it = items[blockIdx.x];
+  G g;
+  flow_setup(g, it.P, it.S, it.U);
+  g.gtid = threadIdx.x;
+  g.gthreads = blockDim.x;
+  g.gwave = threadIdx.x / kWave;
+  g.gwaves = blockDim.x / kWave;
+  g.leader = threadIdx.x == 0;
+  g.block = true;
+  if (g.leader) {
+    GridFlowCtl *ctl = it.S.ctl;
+    ctl->lpack[0] = ctl->lpack[1] = 0;
+    ctl->cqn[0] = ctl->cqn[1] = 0;
+    ctl->active = 0;
+    ctl->sink = 0;
+  }
+  __syncthreads();
+  extern __shared__ long long lds_ex[];
+  flow_run(g, it.P, it.S, it.capT, it.U, max_rounds, lds_ex, g.V <= lds_cap);
+}
+
+// Cliques of each batched problem with every member on its min-cut side.
+__global__ void k_flow_batch_count(const FlowBatchItem *__restrict__ items,
+                                   unsigned long long *out) {
+  const FlowBatchItem &it = items[blockIdx.x];
+  unsigned long long c = 0;
+  for (uint32_t t = threadIdx.x; t < it.P.T; t += blockDim.x) {
+    bool all = true;
+    for (uint32_t j = 0; j < it.P.k; ++j)
+      all &= it.S.reach[it.P.mem[(uint64_t)t * it.P.k + j] - it.P.vbase] != 0;
+    c += all;
+  }
+  c = wave_sum64(c);
+  __shared__ unsigned long long s_c;
+  if (threadIdx.x == 0) s_c = 0;
+  __syncthreads();
+  if (lane() == 0 && c) atomicAdd(&s_c, c);
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = s_c;
+}
+
 __global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
   unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
   for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
@@ -442,6 +509,21 @@ __global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
 }
 
 }  // namespace
+
+hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
+                      uint32_t max_rounds, uint32_t max_v,
+                      unsigned long long *reach_cliques, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
+  hipLaunchKernelGGL(k_flow_batch, dim3(count), dim3(kFlowBatchThreads),
+                     (size_t)lds_cap * sizeof(long long), s, items, max_rounds,
+                     lds_cap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_flow_batch_count, dim3(count), dim3(256), 0, s, items,
+                     reach_cliques);
+  return hipGetLastError();
+}
 
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {
   int per_cu = 0, cus = 0;

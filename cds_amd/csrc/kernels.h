@@ -141,6 +141,26 @@ struct ReplayOut {
   unsigned long long best_num, best_den;  // max intermediate density
   uint32_t steps, found;
 };
+// One phase of replay_phases_multi: level, its vertices at positions
+// [base, base + M) of the shared position arrays, and the state C_level.
+struct ReplayPhase {
+  uint32_t level, base, M, pad;
+  unsigned long long T_start, N_start;
+};
+// Replays many phases at once, one workgroup each (phases of different
+// graphs, or of different levels: disjoint vertices and cliques).  pv holds
+// npos positions (each phase's vertices ascending), posmap maps a vertex to
+// its position, sdeg gives the level-start degrees; pdeg/palive/pioff are
+// per position (pioff npos+1); pinc >= k * sum(pdeg) words.  out[p] per phase.
+hipError_t replay_phases_multi(uint32_t k, uint32_t nphases,
+                               const ReplayPhase *phases, uint32_t npos,
+                               const uint32_t *pv, const uint32_t *sdeg,
+                               uint32_t *pdeg, const uint32_t *posmap,
+                               uint8_t *palive, const uint32_t *core,
+                               const uint32_t *kill, const uint64_t *ioff,
+                               const uint32_t *inc, uint64_t *pioff,
+                               uint32_t *pinc, uint32_t *rdead, ReplayOut *out,
+                               DeviceArena &arena, hipStream_t s);
 // Replays level `level` (M vertices) in the oracle's removal order.
 // One workgroup (grid = 0): the phase-clique incidence is first compacted
 // into pinc/pioff (pinc >= k * k * cliques killed at the level, pioff M+1).
@@ -205,6 +225,51 @@ hipError_t item_core_write(uint64_t nitems, const uint4 *items,
                            const uint32_t *cslot, uint32_t *finc,
                            hipStream_t s);
 
+// ---- batched solves: the disjoint union of many graphs (batch.hip) ----
+struct UnionPart {
+  const uint32_t *off, *adj;  // the graph's own CSR (device)
+  uint32_t n, vbase, abase, pad;  // its vertices / first vertex / first slot
+};
+// Union CSR (off n_total+1, adj a_total) and gid[v] = graph of v.
+hipError_t union_build(uint32_t B, const UnionPart *parts, uint32_t n_total,
+                       uint32_t a_total, uint32_t *off, uint32_t *adj,
+                       uint32_t *gid, hipStream_t s);
+// Per graph: sum and max of clique degrees (outputs zeroed by the caller).
+hipError_t graph_clique_degrees(uint32_t n, const uint32_t *gid, const uint64_t *cd,
+                                unsigned long long *sum, unsigned long long *mx,
+                                hipStream_t s);
+// Level runs: vertices sorted by (graph, core) (skey/sval, n each; key/val
+// scratch n each); ukey[r], counts[r] for each distinct (graph, core) pair
+// in ascending order, *nruns (device) their number.
+hipError_t level_runs(uint32_t n, const uint32_t *gid, const uint32_t *core,
+                      uint64_t *key, uint32_t *val, uint64_t *skey, uint32_t *sval,
+                      uint64_t *ukey, uint32_t *counts, uint32_t *nruns,
+                      DeviceArena &arena, hipStream_t s);
+// killed[r] = cliques whose (graph, kill level) is run r.
+hipError_t run_killed(uint64_t T, uint32_t k, const uint32_t *cl,
+                      const uint32_t *kill, const uint32_t *gid,
+                      const uint64_t *ukeys, uint32_t nruns, uint32_t *killed,
+                      hipStream_t s);
+// posmap[sval[i]] = i.
+hipError_t positions_of(uint32_t n, const uint32_t *sval, uint32_t *posmap,
+                        hipStream_t s);
+// core01[v] = core[v] >= kp[graph of v]; kill01[c] likewise with the graph
+// of the clique; both are then compared against a threshold of 1.
+hipError_t densest_flags(uint32_t n, uint64_t T, uint32_t k, const uint32_t *cl,
+                         const uint32_t *gid, const uint32_t *core,
+                         const uint32_t *kill, const uint32_t *kp,
+                         uint32_t *core01, uint32_t *kill01, hipStream_t s);
+// Per graph: adjacency entries between flagged vertices, and marked entries
+// (mark may be null).  Outputs zeroed by the caller.
+hipError_t graph_edge_counts(uint32_t n, const uint32_t *gid, const uint32_t *off,
+                             const uint32_t *adj, const uint32_t *flag,
+                             const uint8_t *mark, unsigned long long *core_e,
+                             unsigned long long *kept_e, hipStream_t s);
+// Per graph: cliques whose members all have mask != 0 (out zeroed by caller).
+hipError_t graph_cliques_in_mask(uint64_t T, uint32_t k, const uint32_t *cl,
+                                 const uint32_t *gid, const uint8_t *mask,
+                                 uint32_t *out, hipStream_t s);
+
 // ---- max flow (flow_grid.hip) and set accounting (flow.hip) ----
 struct FlowProblem {
   uint32_t k, V, T;       // vertices, cliques of this component
@@ -238,6 +303,18 @@ struct GridFlowState {
   GridFlowCtl *ctl;
   int *reach;              // V
 };
+// Many independent max flows in one launch, one workgroup each (same
+// answer as grid_max_flow); every item's S.ctl is initialised by the
+// kernel.  max_v = largest P.V (sizes the LDS excess accumulator);
+// reach_cliques[i] = cliques of item i inside its min-cut side.
+struct FlowBatchItem {
+  FlowProblem P;
+  GridFlowState S;
+  long long capT, U;
+};
+hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
+                      uint32_t max_rounds, uint32_t max_v,
+                      unsigned long long *reach_cliques, hipStream_t s);
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
                          unsigned grid, hipStream_t s);

@@ -511,23 +511,27 @@ constexpr int kReplayThreads = 1024;
 // (T_start cliques, N_start vertices): repeatedly removes the alive phase
 // vertex with the smallest (current degree, id), kills its alive phase
 // cliques and decrements their other phase members; records the largest
-// density of the intermediate states (after 1..M-1 removals).
-__global__ __launch_bounds__(kReplayThreads) void k_replay(
-    uint32_t k, uint32_t level, const uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ pv, uint32_t *__restrict__ pdeg,
-    const uint32_t *__restrict__ posmap, uint8_t *__restrict__ palive,
-    const uint32_t *__restrict__ core, const uint64_t *__restrict__ pioff,
-    const uint32_t *__restrict__ pinc, uint32_t *__restrict__ rdead,
-    uint64_t T_start, uint64_t N_start, ReplayOut *out) {
+// density of the intermediate states (after 1..M-1 removals).  The phase's
+// vertices are pv[base .. base+M); pdeg, palive and pioff are indexed by
+// the same positions and posmap maps a vertex to its position.
+__device__ void replay_one(uint32_t k, uint32_t level, uint32_t base,
+                           uint32_t M, const uint32_t *__restrict__ pv,
+                           uint32_t *__restrict__ pdeg,
+                           const uint32_t *__restrict__ posmap,
+                           uint8_t *__restrict__ palive,
+                           const uint32_t *__restrict__ core,
+                           const uint64_t *__restrict__ pioff,
+                           const uint32_t *__restrict__ pinc,
+                           uint32_t *__restrict__ rdead, uint64_t T_start,
+                           uint64_t N_start, ReplayOut *out) {
   __shared__ uint64_t s_wmin[kReplayThreads / kWave];
   __shared__ uint64_t s_key;
-  const uint32_t M = *count;
   const uint32_t tid = threadIdx.x, wid = tid / kWave;
   uint64_t R = 0, best_num = 0, best_den = 1;
   uint32_t found = 0, steps = 0;
   for (uint32_t j = 0; j < M; ++j) {
     uint64_t key = ~0ull;
-    for (uint32_t i = tid; i < M; i += kReplayThreads)
+    for (uint32_t i = base + tid; i < base + M; i += kReplayThreads)
       if (palive[i]) {
         const uint64_t kk = ((uint64_t)pdeg[i] << 32) | pv[i];
         key = kk < key ? kk : key;
@@ -578,6 +582,68 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay(
     out->best_den = best_den;
     out->steps = steps;
     out->found = found;
+  }
+}
+
+__global__ __launch_bounds__(kReplayThreads) void k_replay(
+    uint32_t k, uint32_t level, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ pv, uint32_t *__restrict__ pdeg,
+    const uint32_t *__restrict__ posmap, uint8_t *__restrict__ palive,
+    const uint32_t *__restrict__ core, const uint64_t *__restrict__ pioff,
+    const uint32_t *__restrict__ pinc, uint32_t *__restrict__ rdead,
+    uint64_t T_start, uint64_t N_start, ReplayOut *out) {
+  replay_one(k, level, 0, *count, pv, pdeg, posmap, palive, core, pioff, pinc,
+             rdead, T_start, N_start, out);
+}
+
+// Many independent phases (of different graphs, or different levels of one
+// graph: their vertex and clique sets are disjoint), one workgroup each.
+__global__ __launch_bounds__(kReplayThreads) void k_replay_multi(
+    uint32_t k, const ReplayPhase *__restrict__ phases,
+    const uint32_t *__restrict__ pv, uint32_t *__restrict__ pdeg,
+    const uint32_t *__restrict__ posmap, uint8_t *__restrict__ palive,
+    const uint32_t *__restrict__ core, const uint64_t *__restrict__ pioff,
+    const uint32_t *__restrict__ pinc, uint32_t *__restrict__ rdead,
+    ReplayOut *out) {
+  const ReplayPhase ph = phases[blockIdx.x];
+  replay_one(k, ph.level, ph.base, ph.M, pv, pdeg, posmap, palive, core, pioff,
+             pinc, rdead, ph.T_start, ph.N_start, out + blockIdx.x);
+}
+
+// Phase set-up for k_replay_multi: pdeg from the level-start degrees,
+// palive = 1, then the phase cliques of every position are compacted.
+__global__ void k_phase_init_multi(const ReplayPhase *__restrict__ phases,
+                                   const uint32_t *__restrict__ pv,
+                                   const uint32_t *__restrict__ sdeg,
+                                   uint32_t *__restrict__ pdeg,
+                                   uint8_t *__restrict__ palive) {
+  const ReplayPhase ph = phases[blockIdx.x];
+  for (uint32_t i = ph.base + threadIdx.x; i < ph.base + ph.M; i += blockDim.x) {
+    pdeg[i] = sdeg[pv[i]];
+    palive[i] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_phase_compact_multi(
+    uint32_t k, const ReplayPhase *__restrict__ phases,
+    const uint32_t *__restrict__ pv, const uint64_t *__restrict__ ioff,
+    const uint32_t *__restrict__ inc, const uint32_t *__restrict__ kill,
+    const uint64_t *__restrict__ pioff, uint32_t *__restrict__ pinc) {
+  const ReplayPhase ph = phases[blockIdx.x];
+  const uint32_t waves = blockDim.x / kWave;
+  for (uint32_t i = ph.base + threadIdx.x / kWave; i < ph.base + ph.M; i += waves) {
+    const uint32_t v = pv[i];
+    uint64_t out = pioff[i];
+    const uint64_t e = ioff[v + 1];
+    for (uint64_t q0 = ioff[v]; q0 < e; q0 += kWave) {
+      const uint64_t q = q0 + lane();
+      const bool take = q < e && kill[inc[q * k]] == ph.level;
+      unsigned cnt;
+      const unsigned slot = wave_slot(take, &cnt);
+      if (take)
+        for (uint32_t t = 0; t < k; ++t) pinc[(out + slot) * k + t] = inc[q * k + t];
+      out += cnt;
+    }
   }
 }
 
@@ -809,6 +875,31 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
   hipLaunchKernelGGL(k_replay, dim3(1), dim3(kReplayThreads), 0, s, k, level,
                      pcount, pv, pdeg, posmap, palive, core, pioff, pinc,
                      rdead, T_start, N_start, out);
+  return hipGetLastError();
+}
+
+hipError_t replay_phases_multi(uint32_t k, uint32_t nphases,
+                               const ReplayPhase *phases, uint32_t npos,
+                               const uint32_t *pv, const uint32_t *sdeg,
+                               uint32_t *pdeg, const uint32_t *posmap,
+                               uint8_t *palive, const uint32_t *core,
+                               const uint32_t *kill, const uint64_t *ioff,
+                               const uint32_t *inc, uint64_t *pioff,
+                               uint32_t *pinc, uint32_t *rdead, ReplayOut *out,
+                               DeviceArena &arena, hipStream_t s) {
+  if (nphases == 0) return hipSuccess;
+  hipError_t e = hipMemsetAsync(pdeg, 0, (size_t)npos * sizeof(uint32_t), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_phase_init_multi, dim3(nphases), dim3(256), 0, s, phases,
+                     pv, sdeg, pdeg, palive);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32(pdeg, pioff, npos, arena)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_phase_compact_multi, dim3(nphases), dim3(256), 0, s, k,
+                     phases, pv, ioff, inc, kill, pioff, pinc);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_replay_multi, dim3(nphases), dim3(kReplayThreads), 0, s, k,
+                     phases, pv, pdeg, posmap, palive, core, pioff, pinc, rdead,
+                     out);
   return hipGetLastError();
 }
 

@@ -92,7 +92,10 @@ typedef struct {
   uint32_t *core;                   /* n entries: clique-core numbers (opt.) */
   uint64_t kmax;                    /* Denseset_core_kmax (max clique core) */
   uint64_t k_prime;                 /* ceil(max peel-prefix density) */
-  double peel_max_density;          /* max peel-prefix density (rho-hat) */
+  double peel_max_density;          /* max peel-prefix density over the level
+                                       boundaries and the replayed levels
+                                       (levels that cannot raise k_prime are
+                                       not replayed); ceil = k_prime */
   double core_density;              /* Denseset_core_density */
   uint32_t core_size;               /* Densest_core_size */
   uint64_t core_edges;              /* Densest_core_edges_count */
