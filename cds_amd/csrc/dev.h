@@ -105,19 +105,22 @@ __device__ __forceinline__ int64_t find_sorted64(const uint32_t *a, int64_t len,
 // device-wide before arriving (the fence writes back this XCD's L2) and
 // acquires after the flip; the workgroup barriers around it order the
 // other waves, which share the CU's caches.
+// group_barrier is the same over the `nblocks` co-resident workgroups of
+// one problem inside a larger launch (bid = this workgroup's index among
+// them).
 template <class B>
-__device__ __forceinline__ void grid_barrier(B *b) {
+__device__ __forceinline__ void group_barrier(B *b, unsigned nblocks, unsigned bid) {
   __syncthreads();
   // One workgroup: its waves share the CU's caches, so the workgroup
   // barrier (with its workgroup-scope fence) is the whole barrier.
-  if (gridDim.x == 1) return;
+  if (nblocks == 1) return;
   if (threadIdx.x == 0) {
     __threadfence();
     const unsigned gen =
         __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned grp = blockIdx.x & 7u;
-    const unsigned gsz = (gridDim.x + 7u - grp) >> 3;
-    const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
+    const unsigned grp = bid & 7u;
+    const unsigned gsz = (nblocks + 7u - grp) >> 3;
+    const unsigned groups = nblocks < 8u ? nblocks : 8u;
     const unsigned a = __hip_atomic_fetch_add(&b->sub[grp * 32], 1u, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
     if ((a + 1u) % gsz == 0u) {
@@ -131,6 +134,11 @@ __device__ __forceinline__ void grid_barrier(B *b) {
     __threadfence();
   }
   __syncthreads();
+}
+
+template <class B>
+__device__ __forceinline__ void grid_barrier(B *b) {
+  group_barrier(b, gridDim.x, blockIdx.x);
 }
 
 // Launches a kernel that synchronises its workgroups with grid_barrier:

@@ -13,15 +13,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <optional>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -43,9 +42,6 @@ struct cds_engine {
   // workgroups of every max-flow launch; 0 = sized from the network
   // (CDS_FLOW_BLOCKS overrides, e.g. to exercise several on small graphs)
   unsigned flow_blocks = 0;
-  // batched solves: engines (own stream and pool) driven by host threads
-  std::vector<cds_engine *> workers;
-  std::mutex batch_mu;
 };
 
 struct cds_dgraph {
@@ -124,8 +120,8 @@ struct Timer {
 // device may run concurrently (batched solves), so at most one cooperative
 // kernel per device is in flight: the gate is held from the launch until
 // the engine's stream has drained.
-// Cooperative launches need every workgroup co-resident.  Engines on one
-// device may run concurrently (batched solves), so cooperative workgroups
+// Cooperative launches need every workgroup co-resident.  Several engines
+// may share a device (one per host thread), so cooperative workgroups
 // in flight on a device are capped at its CU count: every cooperative
 // kernel here fits one 1024-thread workgroup per CU, so any set of them
 // within the cap is co-resident (other kernels only delay them).  A gate
@@ -520,8 +516,6 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
 
 extern "C" void cds_engine_destroy(cds_engine *e) {
   if (!e) return;
-  for (cds_engine *w : e->workers) cds_engine_destroy(w);
-  e->workers.clear();
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   delete e->arena;
@@ -1382,6 +1376,723 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   return CDS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched solve of B graphs with the same (k, ub1, ub2) as one disjoint
+// union: one DAG, listing, incidence and persistent peel over the union;
+// per-graph level tables from a (graph, core) sort; every replay phase of
+// every graph in one launch; per-graph k' applied through 0/1 flags so the
+// prune, union-find and flow-network kernels run once with threshold 1;
+// then one ExactSearch per graph, advanced in lockstep, with all pending
+// flows of a step batched (one workgroup per small component in one plain
+// launch, larger components on several workgroups each in cooperative
+// launches).  Results equal cds_solve on each graph.
+// ---------------------------------------------------------------------------
+
+cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
+                       const std::vector<const cds_params *> &ps,
+                       const std::vector<cds_result *> &rs) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  DeviceArena &A = *e->arena;
+  hipStream_t st = e->stream;
+  const uint32_t B = (uint32_t)gs.size();
+  const uint32_t k = ps[0]->k;
+  const bool ub1 = ps[0]->ub1 != 0, ub2 = ps[0]->ub2 != 0;
+  for (uint32_t b = 0; b < B; ++b) {
+    cds_result *R = rs[b];
+    memset(R, 0, sizeof(*R));
+    R->k = k;
+    R->n = gs[b]->n;
+    R->m = gs[b]->m;
+    R->max_degree = gs[b]->max_degree;
+    if (gs[b]->n == 0) return fail(CDS_ERR_INVALID, "batch item %u: empty graph", b);
+  }
+  Timer tm(st);
+  const int ev_start = tm.mark();
+
+  // ---- the union CSR ----
+  std::vector<uint32_t> vbase(B + 1, 0);
+  std::vector<uint64_t> abase(B + 1, 0);
+  for (uint32_t b = 0; b < B; ++b) {
+    vbase[b + 1] = vbase[b] + gs[b]->n;
+    abase[b + 1] = abase[b] + gs[b]->host_off[gs[b]->n];
+  }
+  const uint32_t n = vbase[B];
+  if (abase[B] >= 0xFFFFFFFFull || n >= (1u << 24))
+    return fail(CDS_ERR_RANGE, "batch union too large (%u vertices)", n);
+  cds_dgraph U;
+  U.n = n;
+  U.m = abase[B] / 2;
+  U.host_off.resize((size_t)n + 1);
+  for (uint32_t b = 0; b < B; ++b) {
+    U.max_degree = std::max(U.max_degree, gs[b]->max_degree);
+    for (uint32_t i = 0; i < gs[b]->n; ++i)
+      U.host_off[vbase[b] + i] = (uint32_t)abase[b] + gs[b]->host_off[i];
+  }
+  U.host_off[n] = (uint32_t)abase[B];
+  DBuf<uint32_t> uoff(&A), uadj(&A), gid(&A);
+  {
+    std::vector<UnionPart> parts(B);
+    for (uint32_t b = 0; b < B; ++b)
+      parts[b] = UnionPart{gs[b]->off, gs[b]->adj, gs[b]->n, vbase[b],
+                           (uint32_t)abase[b], 0};
+    DBuf<UnionPart> dparts(&A);
+    HIPCHK(dparts.alloc(B));
+    HIPCHK(hipMemcpyAsync(dparts.p, parts.data(), B * sizeof(UnionPart),
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(uoff.alloc((size_t)n + 1));
+    HIPCHK(uadj.alloc(abase[B]));
+    HIPCHK(gid.alloc(n));
+    HIPCHK(union_build(B, dparts.p, n, (uint32_t)abase[B], uoff.p, uadj.p, gid.p, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  U.off = uoff.p;
+  U.adj = uadj.p;
+  std::vector<uint32_t> hgid(n);
+  for (uint32_t b = 0; b < B; ++b)
+    std::fill(hgid.begin() + vbase[b], hgid.begin() + vbase[b + 1], b);
+
+  // ---- (a) DAG + listing ----
+  Listing L(&A);
+  CHK(run_dag(e, &U, L));
+  const int ev_dag = tm.mark();
+  CHK(run_listing(e, &U, k, true, L, &tm));
+  const int ev_list = tm.mark();
+  const uint64_t T = L.T;
+  std::vector<uint64_t> cd(n);
+  CHK(readback(e, cd.data(), L.cd.p, n));
+  std::vector<uint64_t> Tb(B, 0);
+  {
+    uint64_t sum_cd = 0, max_cd = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      uint64_t s = 0;
+      for (uint32_t v = vbase[b]; v < vbase[b + 1]; ++v) {
+        s += cd[v];
+        max_cd = std::max(max_cd, cd[v]);
+      }
+      if (s % k)
+        return fail(CDS_ERR_INTERNAL, "batch item %u: clique degrees sum %llu", b,
+                    (unsigned long long)s);
+      Tb[b] = s / k;
+      sum_cd += s;
+      if (s >= (1ull << 31)) rs[b]->oracle_quirk_flags |= 4;
+    }
+    if (sum_cd != T * k)
+      return fail(CDS_ERR_INTERNAL, "listing invariant: sum cd %llu != k*T %llu",
+                  (unsigned long long)sum_cd, (unsigned long long)(T * k));
+    if (max_cd >= 0xFFFFFFFFull)
+      return fail(CDS_ERR_RANGE, "a clique degree exceeds 32 bits");
+  }
+  for (uint32_t b = 0; b < B; ++b) {
+    rs[b]->total_cliques = Tb[b];
+    rs[b]->dag_arcs = gs[b]->m;
+    rs[b]->list_work = L.work;
+    rs[b]->max_out_degree = L.max_out;
+    rs[b]->list_items = L.items_n;
+    rs[b]->t_list_count_ms = L.t_count;
+    rs[b]->t_list_write_ms = L.t_write;
+  }
+
+  // ---- (b) incidence + peel over the union ----
+  DBuf<uint64_t> ioff(&A);
+  DBuf<uint32_t> icur(&A), inc(&A), deg(&A), core(&A), kill(&A), sdeg(&A);
+  HIPCHK(ioff.alloc((size_t)n + 1));
+  HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
+  HIPCHK(icur.alloc(n));
+  HIPCHK(icur.zero());
+  HIPCHK(inc.alloc(T * k * k));
+  HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
+  icur.reset();
+  HIPCHK(deg.alloc(n));
+  HIPCHK(core.alloc(n));
+  HIPCHK(kill.alloc(T));
+  HIPCHK(sdeg.alloc(n));
+  HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
+  uint32_t peel_grid = 0, peel_rounds = 0;
+  {
+    DBuf<uint32_t> s32(&A), lvl(&A);
+    DBuf<uint64_t> s64(&A);
+    DBuf<unsigned long long> lrem(&A), lkil(&A);
+    DBuf<PeelCtl> ctl(&A);
+    HIPCHK(s32.alloc(6ull * n));
+    HIPCHK(s64.alloc(3ull * n));
+    HIPCHK(lvl.alloc(n));
+    HIPCHK(lrem.alloc(n));
+    HIPCHK(lkil.alloc(n));
+    HIPCHK(ctl.alloc(1));
+    peel_grid = peel_grid_blocks(n, T, k, e->device);
+    if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
+    {
+      CoopGate gate(e, peel_grid);
+      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
+                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
+                             sdeg.p, peel_grid, st));
+    }
+    PeelCtl hc;
+    CHK(readback(e, &hc, ctl.p, 1));
+    if (hc.removed != n)
+      return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %u", hc.removed, n);
+    peel_rounds = (uint32_t)hc.rounds;
+  }
+  deg.reset();
+
+  // ---- per-graph level tables: runs of (graph, core) ----
+  DBuf<uint64_t> lkey(&A), skey(&A), ukey(&A);
+  DBuf<uint32_t> lval(&A), sval(&A), rcount(&A), nruns_d(&A), rkilled(&A);
+  HIPCHK(lkey.alloc(n));
+  HIPCHK(lval.alloc(n));
+  HIPCHK(skey.alloc(n));
+  HIPCHK(sval.alloc(n));
+  HIPCHK(ukey.alloc(n));
+  HIPCHK(rcount.alloc(n));
+  HIPCHK(nruns_d.alloc(1));
+  HIPCHK(level_runs(n, gid.p, core.p, lkey.p, lval.p, skey.p, sval.p, ukey.p,
+                    rcount.p, nruns_d.p, A, st));
+  lkey.reset();
+  lval.reset();
+  uint32_t nruns = 0;
+  CHK(readback(e, &nruns, nruns_d.p, 1));
+  std::vector<uint64_t> hukey(nruns);
+  std::vector<uint32_t> hcount(nruns), hkilled(nruns);
+  CHK(readback(e, hukey.data(), ukey.p, nruns));
+  CHK(readback(e, hcount.data(), rcount.p, nruns));
+  HIPCHK(rkilled.alloc(nruns));
+  HIPCHK(run_killed(T, k, L.cliques.p, kill.p, gid.p, ukey.p, nruns, rkilled.p, st));
+  CHK(readback(e, hkilled.data(), rkilled.p, nruns));
+  std::vector<std::vector<Level>> levels(B);
+  std::vector<std::vector<uint32_t>> run_pos(B);  // first sorted position
+  {
+    uint32_t pos = 0;
+    for (uint32_t r = 0; r < nruns; ++r) {
+      const uint32_t b = (uint32_t)(hukey[r] >> 32);
+      if (b >= B) return fail(CDS_ERR_INTERNAL, "level run of graph %u", b);
+      levels[b].push_back({(uint32_t)hukey[r], hcount[r], hkilled[r]});
+      run_pos[b].push_back(pos);
+      pos += hcount[r];
+    }
+  }
+  std::vector<PeelSummary> PS(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    cds_status s = summarize_levels(levels[b], gs[b]->n, Tb[b], &PS[b]);
+    if (s != CDS_OK) return s;
+  }
+
+  // ---- exact replays: every candidate phase of every graph at once ----
+  {
+    std::vector<ReplayPhase> phases;
+    std::vector<uint32_t> phase_graph;
+    uint64_t pinc_words = 0;
+    for (uint32_t b = 0; b < B; ++b)
+      for (size_t ii : PS[b].replay) {
+        const Level &lv = levels[b][ii];
+        ReplayPhase ph;
+        ph.level = lv.level;
+        ph.base = run_pos[b][ii];
+        ph.M = (uint32_t)lv.removed;
+        ph.pad = 0;
+        ph.T_start = Tb[b] - PS[b].kl_before[ii];
+        ph.N_start = gs[b]->n - PS[b].rm_before[ii];
+        phases.push_back(ph);
+        phase_graph.push_back(b);
+        pinc_words += lv.killed * k * k;
+      }
+    if (!phases.empty()) {
+      const uint32_t np = (uint32_t)phases.size();
+      DBuf<ReplayPhase> dph(&A);
+      DBuf<uint32_t> pdeg(&A), posmap(&A), pinc(&A), rdead(&A);
+      DBuf<uint8_t> palive(&A);
+      DBuf<uint64_t> pioff(&A);
+      DBuf<ReplayOut> rout(&A);
+      HIPCHK(dph.alloc(np));
+      HIPCHK(hipMemcpyAsync(dph.p, phases.data(), np * sizeof(ReplayPhase),
+                            hipMemcpyHostToDevice, st));
+      HIPCHK(pdeg.alloc(n));
+      HIPCHK(posmap.alloc(n));
+      HIPCHK(palive.alloc(n));
+      HIPCHK(pioff.alloc((size_t)n + 1));
+      HIPCHK(pinc.alloc(pinc_words));
+      HIPCHK(rdead.alloc(T));
+      HIPCHK(rdead.zero());
+      HIPCHK(rout.alloc(np));
+      HIPCHK(positions_of(n, sval.p, posmap.p, st));
+      HIPCHK(replay_phases_multi(k, np, dph.p, n, sval.p, sdeg.p, pdeg.p, posmap.p,
+                                 palive.p, core.p, kill.p, ioff.p, inc.p, pioff.p,
+                                 pinc.p, rdead.p, rout.p, A, st));
+      std::vector<ReplayOut> ro(np);
+      CHK(readback(e, ro.data(), rout.p, np));
+      for (uint32_t i = 0; i < np; ++i) {
+        if (ro[i].steps != phases[i].M)
+          return fail(CDS_ERR_INTERNAL, "batch item %u: replay of level %u ran %u of %u steps",
+                      phase_graph[i], phases[i].level, ro[i].steps, phases[i].M);
+        apply_replay(&PS[phase_graph[i]], ro[i]);
+        rs[phase_graph[i]]->replay_phases++;
+      }
+    }
+  }
+  const int ev_peel = tm.mark();
+
+  // ---- densest cores: per-graph k' turned into 0/1 flags ----
+  std::vector<uint32_t> kpb(B);
+  std::vector<uint32_t> core_n(B);
+  std::vector<uint64_t> core_T(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    cds_result *R = rs[b];
+    const PeelSummary &S = PS[b];
+    R->peel_max_density = S.rho;
+    R->k_prime = (uint64_t)S.kp;
+    R->kmax = S.kmax;
+    R->peel_levels = peel_rounds;
+    R->peel_grid = peel_grid;
+    R->peel_distinct_levels = (uint32_t)levels[b].size();
+    size_t cl = levels[b].size();
+    for (size_t i = 0; i < levels[b].size(); ++i)
+      if ((int64_t)levels[b][i].level >= S.kp) {
+        cl = i;
+        break;
+      }
+    if (cl == levels[b].size())
+      return fail(CDS_ERR_INTERNAL, "batch item %u: empty densest core", b);
+    core_n[b] = (uint32_t)(gs[b]->n - S.rm_before[cl]);
+    core_T[b] = Tb[b] - S.kl_before[cl];
+    R->core_size = core_n[b];
+    R->core_cliques = core_T[b];
+    R->deleted_cliques = Tb[b] - core_T[b];
+    R->core_density = (double)(int64_t)core_T[b] / static_cast<double>(core_n[b]);
+    if (S.kp > 0xFFFFFFFFll) return fail(CDS_ERR_RANGE, "k' exceeds 32 bits");
+    kpb[b] = (uint32_t)S.kp;
+  }
+  DBuf<uint32_t> dkp(&A), core01(&A), kill01(&A);
+  HIPCHK(dkp.alloc(B));
+  HIPCHK(hipMemcpyAsync(dkp.p, kpb.data(), B * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(core01.alloc(n));
+  HIPCHK(kill01.alloc(T));
+  HIPCHK(densest_flags(n, T, k, L.cliques.p, gid.p, core.p, kill.p, dkp.p, core01.p,
+                       kill01.p, st));
+  const int ev_core = tm.mark();
+
+  // ---- prune + per-graph edge counts ----
+  DBuf<uint8_t> mark(&A);
+  HIPCHK(mark.alloc(abase[B]));
+  HIPCHK(mark.zero());
+  HIPCHK(prune_mark(T, k, L.cliques.p, kill01.p, 1, uoff.p, uadj.p, mark.p, st));
+  {
+    DBuf<unsigned long long> ce(&A), ke(&A);
+    HIPCHK(ce.alloc(B));
+    HIPCHK(ke.alloc(B));
+    HIPCHK(ce.zero());
+    HIPCHK(ke.zero());
+    HIPCHK(graph_edge_counts(n, gid.p, uoff.p, uadj.p, core01.p, mark.p, ce.p, ke.p, st));
+    std::vector<unsigned long long> hce(B), hke(B);
+    CHK(readback(e, hce.data(), ce.p, B));
+    CHK(readback(e, hke.data(), ke.p, B));
+    for (uint32_t b = 0; b < B; ++b) {
+      rs[b]->core_edges = hce[b] / 2;
+      rs[b]->pruned_edges = hke[b] / 2;
+    }
+  }
+  const int ev_prune = tm.mark();
+
+  // ---- components (oracle order: graph, then smallest vertex) ----
+  DBuf<uint32_t> parent(&A);
+  HIPCHK(parent.alloc(n));
+  HIPCHK(uf_init(n, parent.p, st));
+  HIPCHK(uf_union_edges(n, uoff.p, uadj.p, mark.p, core01.p, 1, parent.p, st));
+  HIPCHK(uf_flatten(n, parent.p, st));
+  std::vector<uint32_t> hpar(n), hflag(n);
+  CHK(readback(e, hpar.data(), parent.p, n));
+  CHK(readback(e, hflag.data(), core01.p, n));
+  uint32_t core_total = 0;
+  for (uint32_t b = 0; b < B; ++b) core_total += core_n[b];
+  std::vector<uint32_t> comp_of(n, 0), fv_of(n, 0), fv_orig(core_total);
+  std::vector<Component> comps;
+  std::vector<uint32_t> comp_first(B + 1, 0);
+  {
+    std::vector<uint32_t> cidx(n, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+      comp_first[b] = (uint32_t)comps.size();
+      for (uint32_t v = vbase[b]; v < vbase[b + 1]; ++v) {
+        if (!hflag[v]) continue;
+        if (hpar[v] == v) {
+          cidx[v] = (uint32_t)comps.size();
+          Component c;
+          c.first = v;
+          comps.push_back(c);
+        }
+        const uint32_t ci = cidx[hpar[v]];
+        comp_of[v] = ci;
+        comps[ci].V++;
+      }
+    }
+    comp_first[B] = (uint32_t)comps.size();
+    uint32_t base = 0;
+    for (auto &c : comps) {
+      c.vbase = base;
+      base += c.V;
+    }
+    if (base != core_total)
+      return fail(CDS_ERR_INTERNAL, "component sizes %u != cores %u", base, core_total);
+    std::vector<uint32_t> fill(comps.size(), 0);
+    for (uint32_t v = 0; v < n; ++v) {
+      if (!hflag[v]) continue;
+      const uint32_t ci = comp_of[v];
+      fv_of[v] = comps[ci].vbase + fill[ci]++;
+      fv_orig[fv_of[v]] = v;
+    }
+  }
+  const uint32_t ncomp = (uint32_t)comps.size();
+  uint64_t core_T_total = 0;
+  for (uint32_t b = 0; b < B; ++b) {
+    rs[b]->num_components = comp_first[b + 1] - comp_first[b];
+    core_T_total += core_T[b];
+  }
+  DBuf<uint32_t> d_comp_of(&A), d_fv_of(&A), ccount(&A), fcd(&A), ccur(&A), fmem(&A),
+      finc(&A);
+  DBuf<uint64_t> coff(&A), fioff(&A);
+  HIPCHK(d_comp_of.alloc(n));
+  HIPCHK(d_fv_of.alloc(n));
+  CHK(upload(e, d_comp_of.p, comp_of.data(), n));
+  CHK(upload(e, d_fv_of.p, fv_of.data(), n));
+  HIPCHK(ccount.alloc(ncomp));
+  HIPCHK(ccount.zero());
+  HIPCHK(fcd.alloc(core_total));
+  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, ccount.p, st));
+  constexpr uint32_t kItem = 4096;
+  std::vector<uint4> items;
+  std::vector<uint32_t> item_fv;
+  {
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      const uint64_t len = cd[v];
+      if (hflag[v])
+        for (uint64_t s0 = 0; s0 < len; s0 += kItem) {
+          const uint64_t at = pos + s0;
+          items.push_back(make_uint4(v, (uint32_t)(at >> 32), (uint32_t)at,
+                                     (uint32_t)std::min<uint64_t>(kItem, len - s0)));
+          item_fv.push_back(fv_of[v]);
+        }
+      pos += len;
+    }
+  }
+  const uint64_t nitems = items.size();
+  DBuf<uint4> d_items(&A);
+  DBuf<uint32_t> d_icnt(&A);
+  DBuf<uint64_t> d_iout(&A);
+  HIPCHK(d_items.alloc(nitems));
+  HIPCHK(d_icnt.alloc(nitems));
+  HIPCHK(d_iout.alloc(nitems));
+  CHK(upload(e, d_items.p, items.data(), nitems));
+  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, kill01.p, 1, d_icnt.p, st));
+  std::vector<uint32_t> hcc(ncomp), hfcd(core_total, 0), hicnt(nitems);
+  std::vector<uint64_t> hiout(nitems), hfioff((size_t)core_total + 1, 0);
+  CHK(readback(e, hcc.data(), ccount.p, ncomp));
+  CHK(readback(e, hicnt.data(), d_icnt.p, nitems));
+  for (uint64_t i = 0; i < nitems; ++i) hfcd[item_fv[i]] += hicnt[i];
+  for (uint32_t f = 0; f < core_total; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
+  {
+    std::vector<uint64_t> cursor(hfioff.begin(), hfioff.end() - 1);
+    for (uint64_t i = 0; i < nitems; ++i) {
+      hiout[i] = cursor[item_fv[i]];
+      cursor[item_fv[i]] += hicnt[i];
+    }
+  }
+  CHK(upload(e, fcd.p, hfcd.data(), core_total));
+  CHK(upload(e, d_iout.p, hiout.data(), nitems));
+  std::vector<uint64_t> hcoff((size_t)ncomp + 1, 0);
+  for (uint32_t c = 0; c < ncomp; ++c) {
+    comps[c].T = hcc[c];
+    comps[c].cbase = hcoff[c];
+    hcoff[c + 1] = hcoff[c] + hcc[c];
+  }
+  if (hcoff[ncomp] != core_T_total)
+    return fail(CDS_ERR_INTERNAL, "bucketed %llu of %llu core cliques",
+                (unsigned long long)hcoff[ncomp], (unsigned long long)core_T_total);
+  if (core_T_total * k >= 0xFFFFFFFFull)
+    return fail(CDS_ERR_RANGE, "core clique slots exceed 32 bits");
+  HIPCHK(coff.alloc((size_t)ncomp + 1));
+  CHK(upload(e, coff.p, hcoff.data(), ncomp + 1));
+  HIPCHK(ccur.alloc(ncomp));
+  HIPCHK(ccur.zero());
+  HIPCHK(fmem.alloc(core_T_total * k));
+  {
+    DBuf<uint32_t> cslot(&A);
+    HIPCHK(cslot.alloc(T));
+    HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, d_fv_of.p,
+                              coff.p, ccur.p, fmem.p, cslot.p, st));
+    HIPCHK(fioff.alloc((size_t)core_total + 1));
+    CHK(upload(e, fioff.p, hfioff.data(), core_total + 1));
+    HIPCHK(finc.alloc(core_T_total * k));
+    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, kill01.p, 1,
+                           L.cliques.p, k, cslot.p, finc.p, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  inc.reset();
+  const int ev_cc = tm.mark();
+
+  // ---- (c) one ExactSearch per graph, flows batched per step ----
+  std::vector<std::vector<Component>> gcomps(B);
+  for (uint32_t b = 0; b < B; ++b)
+    gcomps[b].assign(comps.begin() + comp_first[b], comps.begin() + comp_first[b + 1]);
+  std::vector<std::unique_ptr<ExactSearch>> search(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    search[b].reset(new ExactSearch(k, ub1, ub2, gcomps[b], hfcd.data(),
+                                    rs[b]->core_density, PS[b].kmax));
+    rs[b]->paper_lower_bound = search[b]->paper_lower_bound;
+  }
+  std::vector<uint64_t> sweeps(B, 0);
+  std::vector<char> active(B, 1);
+  DBuf<unsigned long long> acc(&A);
+  HIPCHK(acc.alloc(2));
+  for (;;) {
+    struct Pending {
+      uint32_t b;
+      FlowRequest rq;
+      const Component *C;
+    };
+    std::vector<Pending> pend;
+    for (uint32_t b = 0; b < B; ++b) {
+      if (!active[b]) continue;
+      FlowRequest rq;
+      cds_status serr = CDS_OK;
+      if (search[b]->next(&rq, &serr)) {
+        pend.push_back({b, rq, &gcomps[b][rq.comp]});
+      } else {
+        if (serr != CDS_OK) return serr;
+        active[b] = 0;
+      }
+    }
+    if (pend.empty()) break;
+    // state slices for every pending flow
+    const size_t np = pend.size();
+    std::vector<uint64_t> xo(np + 1, 0), vo(np + 1, 0), to(np + 1, 0);
+    uint32_t max_v = 0;
+    for (size_t i = 0; i < np; ++i) {
+      const Component &C = *pend[i].C;
+      uint32_t maxcd = 0;
+      for (uint32_t v = 0; v < C.V; ++v) maxcd = std::max(maxcd, hfcd[C.vbase + v]);
+      if ((__int128)maxcd * pend[i].rq.U >= ((__int128)1 << 62) ||
+          (__int128)pend[i].rq.capT * C.V >= ((__int128)1 << 62) ||
+          (__int128)C.T * k * pend[i].rq.U >= ((__int128)1 << 62))
+        return fail(CDS_ERR_RANGE, "batch item %u: component exceeds the int64 flow range",
+                    pend[i].b);
+      xo[i + 1] = xo[i] + C.T * k;
+      vo[i + 1] = vo[i] + C.V;
+      to[i + 1] = to[i] + C.T;
+      max_v = std::max(max_v, C.V);
+    }
+    DBuf<long long> fx(&A), frvs(&A), fexv(&A), frem(&A), fexc(&A);
+    DBuf<int> fhv(&A), fhmin(&A), fhc(&A), freach(&A);
+    DBuf<uint32_t> fl0(&A), fl1(&A), fq0(&A), fq1(&A);
+    DBuf<uint64_t> flc0(&A), flc1(&A);
+    DBuf<GridFlowCtl> fctl(&A);
+    DBuf<FlowBatchItem> fitems(&A);
+    DBuf<unsigned long long> fcnt(&A);
+    HIPCHK(fx.alloc(xo[np]));
+    HIPCHK(frvs.alloc(vo[np]));
+    HIPCHK(fexv.alloc(vo[np]));
+    HIPCHK(frem.alloc(vo[np]));
+    HIPCHK(fexc.alloc(to[np]));
+    HIPCHK(fhv.alloc(vo[np]));
+    HIPCHK(fhmin.alloc(vo[np]));
+    HIPCHK(fhc.alloc(to[np]));
+    HIPCHK(freach.alloc(vo[np]));
+    HIPCHK(fl0.alloc(vo[np]));
+    HIPCHK(fl1.alloc(vo[np]));
+    HIPCHK(flc0.alloc(vo[np]));
+    HIPCHK(flc1.alloc(vo[np]));
+    HIPCHK(fq0.alloc(to[np]));
+    HIPCHK(fq1.alloc(to[np]));
+    HIPCHK(fctl.alloc(np));
+    HIPCHK(fcnt.alloc(np));
+    std::vector<FlowBatchItem> all(np);
+    const unsigned cap = flow_batch_capacity(max_v, e->device);
+    if (cap == 0) return fail(CDS_ERR_HIP, "the batched max flow cannot be launched");
+    for (size_t i = 0; i < np; ++i) {
+      const Component &C = *pend[i].C;
+      FlowBatchItem &it = all[i];
+      it.P.k = k;
+      it.P.V = C.V;
+      it.P.T = (uint32_t)C.T;
+      it.P.vbase = C.vbase;
+      it.P.sbase = C.cbase * k;
+      it.P.mem = fmem.p + C.cbase * k;
+      it.P.ioff = fioff.p + C.vbase;
+      it.P.inc = finc.p;
+      it.P.cd = fcd.p + C.vbase;
+      it.S.x = fx.p + xo[i];
+      it.S.rvs = frvs.p + vo[i];
+      it.S.exv = fexv.p + vo[i];
+      it.S.rem = frem.p + vo[i];
+      it.S.exc = fexc.p + to[i];
+      it.S.hv = fhv.p + vo[i];
+      it.S.hmin = fhmin.p + vo[i];
+      it.S.hc = fhc.p + to[i];
+      it.S.list0 = fl0.p + vo[i];
+      it.S.list1 = fl1.p + vo[i];
+      it.S.lchunk0 = flc0.p + vo[i];
+      it.S.lchunk1 = flc1.p + vo[i];
+      it.S.cq0 = fq0.p + to[i];
+      it.S.cq1 = fq1.p + to[i];
+      it.S.ctl = fctl.p + i;
+      it.S.reach = freach.p + vo[i];
+      it.capT = pend[i].rq.capT;
+      it.U = pend[i].rq.U;
+      // one workgroup per 64k arcs, at most half the cooperative capacity
+      const uint64_t want = (C.T * k + 65535) / 65536;
+      it.nblocks = (uint32_t)std::max<uint64_t>(
+          1, std::min<uint64_t>(e->flow_blocks ? e->flow_blocks : want, std::max(1u, cap / 2)));
+      it.first_block = 0;
+    }
+    HIPCHK(fctl.zero());
+    // launches: every single-workgroup problem together (a plain launch),
+    // then the multi-workgroup ones packed into cooperative launches
+    std::vector<std::vector<size_t>> groups;
+    {
+      std::vector<size_t> singles, multis;
+      for (size_t i = 0; i < np; ++i) (all[i].nblocks == 1 ? singles : multis).push_back(i);
+      if (!singles.empty()) groups.push_back(singles);
+      std::vector<size_t> cur;
+      uint32_t used = 0;
+      for (size_t i : multis) {
+        if (used + all[i].nblocks > cap) {
+          groups.push_back(cur);
+          cur.clear();
+          used = 0;
+        }
+        cur.push_back(i);
+        used += all[i].nblocks;
+      }
+      if (!cur.empty()) groups.push_back(cur);
+    }
+    HIPCHK(fitems.alloc(np));
+    std::vector<unsigned long long> hcnt(np, 0);
+    size_t item_at = 0;
+    for (const auto &grp : groups) {
+      std::vector<FlowBatchItem> li;
+      uint32_t blocks = 0, mv = 0;
+      bool coop = false;
+      for (size_t i : grp) {
+        FlowBatchItem it = all[i];
+        it.first_block = blocks;
+        blocks += it.nblocks;
+        coop |= it.nblocks > 1;
+        mv = std::max(mv, it.P.V);
+        li.push_back(it);
+      }
+      FlowBatchItem *dli = fitems.p + item_at;
+      HIPCHK(hipMemcpyAsync(dli, li.data(), li.size() * sizeof(FlowBatchItem),
+                            hipMemcpyHostToDevice, st));
+      {
+        CoopGate gate(e, coop ? blocks : 1);
+        HIPCHK(flow_batch(dli, (uint32_t)li.size(), blocks, coop, kGridFlowRounds, mv,
+                          fcnt.p + item_at, st));
+      }
+      std::vector<unsigned long long> sc(li.size());
+      CHK(readback(e, sc.data(), fcnt.p + item_at, li.size()));
+      for (size_t j = 0; j < grp.size(); ++j) hcnt[grp[j]] = sc[j];
+      item_at += li.size();
+    }
+    std::vector<GridFlowCtl> hctl(np);
+    std::vector<int> hreach(vo[np]);
+    CHK(readback(e, hctl.data(), fctl.p, np));
+    CHK(readback(e, hreach.data(), freach.p, vo[np]));
+    for (size_t i = 0; i < np; ++i) {
+      const uint32_t b = pend[i].b;
+      if (hctl[i].limit)
+        return fail(CDS_ERR_NOT_CONVERGED, "batch item %u: max flow hit %u rounds", b,
+                    hctl[i].rounds);
+      sweeps[b] += hctl[i].rounds;
+      rs[b]->flow_launches++;
+      if (search[b]->feed_flow((long long)hctl[i].sink))
+        search[b]->feed_reach(hreach.data() + vo[i], hcnt[i]);
+    }
+  }
+  const int ev_exact = tm.mark();
+
+  // ---- results ----
+  std::vector<std::vector<uint32_t>> verts(B);
+  std::vector<uint8_t> hm(n, 0);
+  for (uint32_t b = 0; b < B; ++b) {
+    ExactSearch &S = *search[b];
+    cds_result *R = rs[b];
+    R->ub1_val = S.ub1_val;
+    R->ub2_val = S.ub2_val;
+    R->flow_networks_solved = S.flows_solved;
+    R->density_certificates = S.certificates;
+    R->flow_sweeps = sweeps[b];
+    R->density = (float)S.best_density;
+    R->size = S.best_size;
+    if (!gcomps[b].empty()) {
+      const Component &C = gcomps[b][S.best_comp];
+      for (uint32_t v = 0; v < C.V; ++v)
+        if (S.best_res.empty() || S.best_res[v] != -1) {
+          const uint32_t u = fv_orig[C.vbase + v];
+          hm[u] = 1;
+          verts[b].push_back(u - vbase[b]);
+        }
+    }
+  }
+  {
+    DBuf<uint8_t> mask(&A);
+    DBuf<uint32_t> inset(&A);
+    HIPCHK(mask.alloc(n));
+    CHK(upload(e, mask.p, hm.data(), n));
+    HIPCHK(inset.alloc(B));
+    HIPCHK(inset.zero());
+    HIPCHK(graph_cliques_in_mask(T, k, L.cliques.p, gid.p, mask.p, inset.p, st));
+    std::vector<uint32_t> hin(B);
+    CHK(readback(e, hin.data(), inset.p, B));
+    for (uint32_t b = 0; b < B; ++b) rs[b]->set_cliques = hin[b];
+  }
+  std::vector<uint32_t> hcore;
+  bool any_arrays = false;
+  for (uint32_t b = 0; b < B; ++b) any_arrays |= ps[b]->want_vertex_arrays != 0;
+  if (any_arrays) {
+    hcore.resize(n);
+    CHK(readback(e, hcore.data(), core.p, n));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  const double total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0)
+          .count();
+  for (uint32_t b = 0; b < B; ++b) {
+    ExactSearch &S = *search[b];
+    cds_result *R = rs[b];
+    R->set_size = (uint32_t)verts[b].size();
+    R->vertices = host_alloc<uint32_t>(verts[b].size());
+    memcpy(R->vertices, verts[b].data(), verts[b].size() * 4);
+    const size_t tl = S.tr_alpha.size();
+    R->trace_len = (uint32_t)tl;
+    R->trace_lb = host_alloc<float>(tl);
+    R->trace_ub = host_alloc<float>(tl);
+    R->trace_alpha = host_alloc<float>(tl);
+    R->trace_component = host_alloc<uint32_t>(tl);
+    R->trace_margin = host_alloc<double>(tl);
+    for (size_t i = 0; i < tl; ++i) {
+      R->trace_lb[i] = S.tr_lb[i];
+      R->trace_ub[i] = S.tr_ub[i];
+      R->trace_alpha[i] = S.tr_alpha[i];
+      R->trace_component[i] = S.tr_comp[i];
+      R->trace_margin[i] = S.tr_margin[i];
+    }
+    if (ps[b]->want_vertex_arrays) {
+      const uint32_t nb = gs[b]->n;
+      R->clique_degree = host_alloc<uint64_t>(nb);
+      memcpy(R->clique_degree, cd.data() + vbase[b], (size_t)nb * 8);
+      R->core = host_alloc<uint32_t>(nb);
+      memcpy(R->core, hcore.data() + vbase[b], (size_t)nb * 4);
+    }
+    R->times.dag = tm.ms(ev_start, ev_dag);
+    R->times.listing = tm.ms(ev_dag, ev_list);
+    R->times.core_decompose = tm.ms(ev_list, ev_peel);
+    R->times.densest_core = tm.ms(ev_peel, ev_core);
+    R->times.prune = tm.ms(ev_core, ev_prune);
+    R->times.components = tm.ms(ev_prune, ev_cc);
+    R->times.exact = tm.ms(ev_cc, ev_exact);
+    R->times.total = total_ms;
+    R->peak_device_bytes = A.peak_bytes();
+  }
+  return CDS_OK;
+}
+
 }  // namespace
 
 extern "C" cds_status cds_solve(cds_engine *e, const cds_dgraph *g,
@@ -1423,60 +2134,76 @@ extern "C" cds_status cds_solve_batch(cds_engine *e,
   if (!e || (!graphs && count) || (!params && count) || (!results && count))
     return fail(CDS_ERR_INVALID, "null argument");
   for (size_t i = 0; i < count; ++i) {
-    if (!graphs[i]) return fail(CDS_ERR_INVALID, "graph %zu is null", i);
     memset(&results[i], 0, sizeof(cds_result));
+    if (!graphs[i]) return fail(CDS_ERR_INVALID, "batch item %zu: null graph", i);
+    if (params[i].k < 2 || params[i].k > 4)
+      return fail(CDS_ERR_UNSUPPORTED, "batch item %zu: k=%u: the device lister covers k=2..4",
+                  i, params[i].k);
+    if (params[i].early_stop != -1)
+      return fail(CDS_ERR_UNSUPPORTED, "batch item %zu: only earlyStop = -1 is supported", i);
   }
   if (count == 0) return CDS_OK;
-  std::lock_guard<std::mutex> lk(e->batch_mu);
-  // Solves are independent; W engines on the same device, each with its own
-  // stream, run them concurrently from W host threads, taking graphs from a
-  // shared counter (largest first when sizes differ, so the tail is short).
-  size_t W = std::min<size_t>(count, 8);
-  if (const char *bw = getenv("CDS_BATCH_WORKERS")) {
-    const long v = atol(bw);
-    if (v >= 1) W = std::min<size_t>(count, (size_t)v);
-  }
-  while (e->workers.size() < W) {
-    cds_opts o;
-    memset(&o, 0, sizeof o);
-    o.device = e->device;
-    o.profile_kernels = e->profile ? 1 : 0;
-    cds_engine *w = nullptr;
-    cds_status st = cds_engine_create(&o, &w);
-    if (st != CDS_OK) return st;
-    w->replay_grid_work = e->replay_grid_work;
-    w->flow_blocks = e->flow_blocks;
-    e->workers.push_back(w);
-  }
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  // Group by (k, ub1, ub2); cut each group into unions of bounded size
+  // (vertices below the persistent peel's 2^24, memory bounded by edges).
+  constexpr uint64_t kUnionVertices = 1u << 22, kUnionEdges = 8u << 20;
   std::vector<size_t> order(count);
   for (size_t i = 0; i < count; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return graphs[a]->m > graphs[b]->m;
+    const cds_params &x = params[a], &y = params[b];
+    if (x.k != y.k) return x.k < y.k;
+    if ((x.ub1 != 0) != (y.ub1 != 0)) return (x.ub1 != 0) < (y.ub1 != 0);
+    return (x.ub2 != 0) < (y.ub2 != 0);
   });
-  std::vector<cds_status> st(count, CDS_OK);
-  std::vector<std::string> msg(count);
-  std::atomic<size_t> next{0};
-  auto work = [&](cds_engine *w) {
-    (void)hipSetDevice(w->device);
-    for (;;) {
-      const size_t j = next.fetch_add(1);
-      if (j >= count) break;
-      const size_t i = order[j];
-      st[i] = cds_solve(w, graphs[i], &params[i], &results[i]);
-      if (st[i] != CDS_OK) msg[i] = cds_last_error();
-    }
+  auto same_group = [&](size_t a, size_t b) {
+    return params[a].k == params[b].k && (params[a].ub1 != 0) == (params[b].ub1 != 0) &&
+           (params[a].ub2 != 0) == (params[b].ub2 != 0);
   };
-  std::vector<std::thread> threads;
-  for (size_t t = 1; t < W; ++t) threads.emplace_back(work, e->workers[t]);
-  work(e->workers[0]);
-  for (auto &t : threads) t.join();
-  for (size_t i = 0; i < count; ++i)
-    if (st[i] != CDS_OK) {
-      const cds_status first = st[i];
-      const std::string m = msg[i];
-      for (size_t j = 0; j < count; ++j)
-        if (st[j] == CDS_OK) cds_result_free(&results[j]);
-      return fail(first, "batch item %zu: %s", i, m.c_str());
+  cds_status status = CDS_OK;
+  std::vector<char> done(count, 0);
+  size_t i = 0;
+  while (i < count && status == CDS_OK) {
+    std::vector<const cds_dgraph *> gs;
+    std::vector<const cds_params *> ps;
+    std::vector<cds_result *> rs;
+    std::vector<size_t> idx;
+    uint64_t nv = 0, ne = 0;
+    size_t j = i;
+    while (j < count && same_group(order[i], order[j])) {
+      const cds_dgraph *g = graphs[order[j]];
+      if (!gs.empty() && (nv + g->n > kUnionVertices || ne + g->m > kUnionEdges)) break;
+      if (g->n >= kUnionVertices) {
+        if (!gs.empty()) break;  // a huge graph is solved on its own
+        status = solve_impl(e, g, &params[order[j]], &results[order[j]], false);
+        if (status != CDS_OK) status = fail(status, "batch item %zu: %s", order[j], cds_last_error());
+        done[order[j]] = status == CDS_OK;
+        ++j;
+        break;
+      }
+      gs.push_back(g);
+      ps.push_back(&params[order[j]]);
+      rs.push_back(&results[order[j]]);
+      idx.push_back(order[j]);
+      nv += g->n;
+      ne += g->m;
+      ++j;
     }
+    if (status == CDS_OK && !gs.empty()) {
+      status = solve_union(e, gs, ps, rs);
+      if (status == CDS_OK) {
+        for (size_t x : idx) done[x] = 1;
+      } else {
+        for (cds_result *r : rs) cds_result_free(r);
+      }
+    }
+    i = j;
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (status != CDS_OK) {
+    for (size_t x = 0; x < count; ++x)
+      if (done[x]) cds_result_free(&results[x]);
+    return status;
+  }
   return CDS_OK;
 }

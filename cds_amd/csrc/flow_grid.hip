@@ -76,15 +76,12 @@ struct G {
   GridFlowCtl *ctl;
   uint32_t gtid, gthreads, gwave, gwaves;
   bool leader;
-  bool block;  // the problem owns one workgroup of a batched launch
+  uint32_t nblocks, bid;  // workgroups on this problem, this one's index
 };
 
-// Barrier over the threads working on one problem.
+// Barrier over the workgroups working on one problem.
 __device__ __forceinline__ void gsync(const G &g) {
-  if (g.block)
-    __syncthreads();
-  else
-    grid_barrier(&g.ctl->bar);
+  group_barrier(&g.ctl->bar, g.nblocks, g.bid);
 }
 
 __device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
@@ -448,33 +445,38 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   g.gwave = g.gtid / kWave;
   g.gwaves = g.gthreads / kWave;
   g.leader = g.gtid == 0;
-  g.block = false;
+  g.nblocks = gridDim.x;
+  g.bid = blockIdx.x;
   extern __shared__ long long lds_ex[];
   flow_run(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
 }
 
-// Many independent problems, one workgroup each.  Each problem's control
-// block is initialised here; lds_cap = LDS excess slots per workgroup.
+// Many independent problems in one launch; item i owns workgroups
+// [first_block, first_block + nblocks) (items ascending by first_block).
+// A problem on several workgroups synchronises them through its own
+// control block's barrier, so such launches are cooperative.  Control
+// blocks are zeroed by the caller; lds_cap = LDS excess slots per workgroup.
 __global__ __launch_bounds__(kFlowBatchThreads) void k_flow_batch(
-    const FlowBatchItem *__restrict__ items, uint32_t max_rounds,
+    const FlowBatchItem *__restrict__ items, uint32_t count, uint32_t max_rounds,
     uint32_t lds_cap) {
-  const FlowBatchItem &it = items[blockIdx.x];
+  uint32_t lo = 0, hi = count - 1;  // last item with first_block <= blockIdx
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (items[mid].first_block <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const FlowBatchItem &it = items[lo];
   G g;
   flow_setup(g, it.P, it.S, it.U);
-  g.gtid = threadIdx.x;
-  g.gthreads = blockDim.x;
-  g.gwave = threadIdx.x / kWave;
-  g.gwaves = blockDim.x / kWave;
-  g.leader = threadIdx.x == 0;
-  g.block = true;
-  if (g.leader) {
-    GridFlowCtl *ctl = it.S.ctl;
-    ctl->lpack[0] = ctl->lpack[1] = 0;
-    ctl->cqn[0] = ctl->cqn[1] = 0;
-    ctl->active = 0;
-    ctl->sink = 0;
-  }
-  __syncthreads();
+  g.nblocks = it.nblocks;
+  g.bid = blockIdx.x - it.first_block;
+  g.gtid = g.bid * blockDim.x + threadIdx.x;
+  g.gthreads = g.nblocks * blockDim.x;
+  g.gwave = g.gtid / kWave;
+  g.gwaves = g.gthreads / kWave;
+  g.leader = g.gtid == 0;
   extern __shared__ long long lds_ex[];
   flow_run(g, it.P, it.S, it.capT, it.U, max_rounds, lds_ex, g.V <= lds_cap);
 }
@@ -511,18 +513,41 @@ __global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
 }  // namespace
 
 hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
-                      uint32_t max_rounds, uint32_t max_v,
-                      unsigned long long *reach_cliques, hipStream_t s) {
+                      uint32_t total_blocks, bool cooperative, uint32_t max_rounds,
+                      uint32_t max_v, unsigned long long *reach_cliques,
+                      hipStream_t s) {
   if (count == 0) return hipSuccess;
-  const uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
-  hipLaunchKernelGGL(k_flow_batch, dim3(count), dim3(kFlowBatchThreads),
-                     (size_t)lds_cap * sizeof(long long), s, items, max_rounds,
-                     lds_cap);
-  hipError_t e = hipGetLastError();
+  uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
+  const size_t lds = (size_t)lds_cap * sizeof(long long);
+  hipError_t e;
+  if (cooperative) {
+    void *args[] = {(void *)&items, &count, &max_rounds, &lds_cap};
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_flow_batch),
+                                   dim3(total_blocks), dim3(kFlowBatchThreads), args,
+                                   lds, s);
+  } else {
+    hipLaunchKernelGGL(k_flow_batch, dim3(total_blocks), dim3(kFlowBatchThreads), lds,
+                       s, items, count, max_rounds, lds_cap);
+    e = hipGetLastError();
+  }
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_flow_batch_count, dim3(count), dim3(256), 0, s, items,
                      reach_cliques);
   return hipGetLastError();
+}
+
+unsigned flow_batch_capacity(uint32_t max_v, int device) {
+  const uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(k_flow_batch), kFlowBatchThreads,
+          (size_t)lds_cap * sizeof(long long)) != hipSuccess ||
+      per_cu < 1)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+      hipSuccess)
+    return 0;
+  return (unsigned)cus;
 }
 
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {

@@ -303,18 +303,26 @@ struct GridFlowState {
   GridFlowCtl *ctl;
   int *reach;              // V
 };
-// Many independent max flows in one launch, one workgroup each (same
-// answer as grid_max_flow); every item's S.ctl is initialised by the
-// kernel.  max_v = largest P.V (sizes the LDS excess accumulator);
+// Many independent max flows in one launch (same answers as
+// grid_max_flow): item i runs on workgroups [first_block, first_block +
+// nblocks), items ascending by first_block, total_blocks in all.  Items on
+// several workgroups need `cooperative` (all co-resident: total_blocks <=
+// flow_batch_capacity).  Every item's S.ctl must be zeroed before the
+// launch.  max_v = largest P.V (sizes the LDS excess accumulator);
 // reach_cliques[i] = cliques of item i inside its min-cut side.
 struct FlowBatchItem {
   FlowProblem P;
   GridFlowState S;
   long long capT, U;
+  uint32_t first_block, nblocks;
 };
 hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
-                      uint32_t max_rounds, uint32_t max_v,
-                      unsigned long long *reach_cliques, hipStream_t s);
+                      uint32_t total_blocks, bool cooperative, uint32_t max_rounds,
+                      uint32_t max_v, unsigned long long *reach_cliques,
+                      hipStream_t s);
+// Workgroups of one cooperative flow_batch launch that are surely
+// co-resident (one per CU), 0 if the kernel cannot launch.
+unsigned flow_batch_capacity(uint32_t max_v, int device);
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
                          unsigned grid, hipStream_t s);

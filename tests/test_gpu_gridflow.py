@@ -116,3 +116,22 @@ def test_grid_replay_rmat(replay_engine, scale):
 
 def test_grid_replay_was_exercised():
     assert sum(_REPLAYED) > 0, "no phase went through the whole-GPU replay"
+
+
+def test_batched_flows_on_several_workgroups(grid_engine):
+    """cds_solve_batch with three workgroups per max flow: the cooperative
+    batched launch and its per-problem barriers, against the oracle."""
+    rng = np.random.default_rng(4242)
+    graphs, ks = [], []
+    for i in range(10):
+        n = int(rng.integers(40, 400))
+        graphs.append(cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                                             float(rng.uniform(4, 12)),
+                                             int(rng.integers(8, 30)), 0.85,
+                                             seed=4242 + i))
+        ks.append(3 if i % 3 else 4)
+    _announce("batched flows on three workgroups each")
+    got = grid_engine.solve_batch([grid_engine.upload(g) for g in graphs], ks,
+                                  vertex_arrays=True)
+    for i, (g, k, r) in enumerate(zip(graphs, ks, got)):
+        compare(r, oracle_solve(g, k), f"batched multi-workgroup flow item {i}")
