@@ -1905,6 +1905,9 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     std::vector<FlowBatchItem> all(np);
     const unsigned cap = flow_batch_capacity(max_v, e->device);
     if (cap == 0) return fail(CDS_ERR_HIP, "the batched max flow cannot be launched");
+    // cooperative launches stay within half the CUs, like every other
+    // cooperative kernel here
+    const unsigned coop_cap = std::max(1u, cap / 2);
     for (size_t i = 0; i < np; ++i) {
       const Component &C = *pend[i].C;
       FlowBatchItem &it = all[i];
@@ -1935,15 +1938,16 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       it.S.reach = freach.p + vo[i];
       it.capT = pend[i].rq.capT;
       it.U = pend[i].rq.U;
-      // one workgroup per 64k arcs, at most half the cooperative capacity
+      // one workgroup per 64k arcs, at most a quarter of the CUs
       const uint64_t want = (C.T * k + 65535) / 65536;
       it.nblocks = (uint32_t)std::max<uint64_t>(
-          1, std::min<uint64_t>(e->flow_blocks ? e->flow_blocks : want, std::max(1u, cap / 2)));
+          1, std::min<uint64_t>(e->flow_blocks ? e->flow_blocks : want, std::max(1u, coop_cap / 2)));
       it.first_block = 0;
     }
     HIPCHK(fctl.zero());
-    // launches: every single-workgroup problem together (a plain launch),
-    // then the multi-workgroup ones packed into cooperative launches
+    // launches: every single-workgroup problem together in one plain
+    // launch, then the multi-workgroup ones packed into cooperative
+    // launches of at most coop_cap workgroups
     std::vector<std::vector<size_t>> groups;
     {
       std::vector<size_t> singles, multis;
@@ -1952,7 +1956,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       std::vector<size_t> cur;
       uint32_t used = 0;
       for (size_t i : multis) {
-        if (used + all[i].nblocks > cap) {
+        if (used + all[i].nblocks > coop_cap) {
           groups.push_back(cur);
           cur.clear();
           used = 0;
@@ -1977,6 +1981,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
         mv = std::max(mv, it.P.V);
         li.push_back(it);
       }
+      coop = coop && blocks > 1;
       FlowBatchItem *dli = fitems.p + item_at;
       HIPCHK(hipMemcpyAsync(dli, li.data(), li.size() * sizeof(FlowBatchItem),
                             hipMemcpyHostToDevice, st));

@@ -70,9 +70,11 @@ struct G {
   const uint32_t *inc;
   long long *x, *rvs, *exv, *exc, *rem;
   int *hv, *hc, *hmin;
-  uint32_t *list[2];       // vertex work lists (active vertices / BFS)
-  uint64_t *lchunk[2];     // their chunk starts
-  uint32_t *cq[2];         // clique BFS frontiers
+  // Double buffers are separate members selected with ?: (an array member
+  // indexed at run time would live in scratch memory).
+  uint32_t *list0, *list1;      // vertex work lists (active vertices / BFS)
+  uint64_t *lchunk0, *lchunk1;  // their chunk starts
+  uint32_t *cq0, *cq1;          // clique BFS frontiers
   GridFlowCtl *ctl;
   uint32_t gtid, gthreads, gwave, gwaves;
   bool leader;
@@ -84,6 +86,16 @@ __device__ __forceinline__ void gsync(const G &g) {
   group_barrier(&g.ctl->bar, g.nblocks, g.bid);
 }
 
+__device__ __forceinline__ uint32_t *glist(const G &g, int b) {
+  return b ? g.list1 : g.list0;
+}
+__device__ __forceinline__ uint64_t *glchunk(const G &g, int b) {
+  return b ? g.lchunk1 : g.lchunk0;
+}
+__device__ __forceinline__ uint32_t *gcq(const G &g, int b) {
+  return b ? g.cq1 : g.cq0;
+}
+
 __device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
   return (g.ioff[v + 1] - g.ioff[v] + kGridChunk - 1) / kGridChunk;
 }
@@ -91,8 +103,8 @@ __device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
 __device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
   const unsigned long long old =
       atomicAdd(&g.ctl->lpack[b], (1ull << 40) + (unsigned long long)nchunks(g, v));
-  g.list[b][old >> 40] = v;
-  g.lchunk[b][old >> 40] = old & kMask40;
+  glist(g, b)[old >> 40] = v;
+  glchunk(g, b)[old >> 40] = old & kMask40;
 }
 
 // Maps work chunk ci of list b (count entries) to (vertex, incidence range).
@@ -102,13 +114,13 @@ __device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
   uint32_t a = 0, z = count - 1;
   while (a < z) {
     const uint32_t mid = (a + z + 1) >> 1;
-    if (g.lchunk[b][mid] <= ci)
+    if (glchunk(g, b)[mid] <= ci)
       a = mid;
     else
       z = mid - 1;
   }
-  *v = g.list[b][a];
-  *lo = g.ioff[*v] + (ci - g.lchunk[b][a]) * kGridChunk;
+  *v = glist(g, b)[a];
+  *lo = g.ioff[*v] + (ci - glchunk(g, b)[a]) * kGridChunk;
   const uint64_t e = g.ioff[*v + 1];
   *hi = *lo + kGridChunk < e ? *lo + kGridChunk : e;
 }
@@ -156,14 +168,14 @@ __device__ unsigned long long global_relabel(const G &g) {
         const uint32_t c = (uint32_t)(s / g.k);
         if (ald(&g.hc[c]) != g.INF) continue;
         if (atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF)
-          g.cq[cb][atomicAdd(&ctl->cqn[cb], 1u)] = c;
+          gcq(g, cb)[atomicAdd(&ctl->cqn[cb], 1u)] = c;
       }
     }
     gsync(g);
     // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x)
     const uint32_t nc = ald(&ctl->cqn[cb]);
     for (uint32_t i = g.gtid; i < nc; i += g.gthreads) {
-      const uint32_t c = g.cq[cb][i];
+      const uint32_t c = gcq(g, cb)[i];
       const uint64_t base = (uint64_t)c * g.k;
       for (uint32_t j = 0; j < g.k; ++j) {
         if ((long long)(g.k - 1) * g.U - g.x[base + j] <= 0) continue;
@@ -218,12 +230,12 @@ __device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
   g.hv = S.hv;
   g.hc = S.hc;
   g.hmin = S.hmin;
-  g.list[0] = S.list0;
-  g.list[1] = S.list1;
-  g.lchunk[0] = S.lchunk0;
-  g.lchunk[1] = S.lchunk1;
-  g.cq[0] = S.cq0;
-  g.cq[1] = S.cq1;
+  g.list0 = S.list0;
+  g.list1 = S.list1;
+  g.lchunk0 = S.lchunk0;
+  g.lchunk1 = S.lchunk1;
+  g.cq0 = S.cq0;
+  g.cq1 = S.cq1;
   g.ctl = S.ctl;
 }
 
@@ -404,7 +416,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     gsync(g);
     // 4. remainders become excess; relabel vertices that kept some
     for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
-      const uint32_t v = g.list[0][i];
+      const uint32_t v = g.list0[i];
       long long r = g.rem[v];
       if (r < 0) r = 0;
       g.exv[v] = r;
