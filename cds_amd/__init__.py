@@ -103,6 +103,21 @@ class COpts(ctypes.Structure):
                 ("profile_kernels", ctypes.c_uint32)]
 
 
+class CServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
+                ("max_wait_us", ctypes.c_uint32), ("large_vertices", ctypes.c_uint32)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in
+                ("submitted", "admitted", "completed", "failed", "batches",
+                 "max_batch_seen", "peak_admitted_bytes", "queued")]
+
+
+class CRequestTimes(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("service_ms", ctypes.c_double)]
+
+
 # Every function include/cds.h declares, with its ctypes signature.
 _SIGNATURES = {
     "cds_last_error": (ctypes.c_char_p, []),
@@ -114,6 +129,7 @@ _SIGNATURES = {
     "cds_graph_free": (None, [ctypes.POINTER(CGraph)]),
     "cds_engine_create": (ctypes.c_int, [ctypes.POINTER(COpts), ctypes.POINTER(ctypes.c_void_p)]),
     "cds_engine_destroy": (None, [ctypes.c_void_p]),
+    "cds_engine_device": (ctypes.c_int, [ctypes.c_void_p]),
     "cds_graph_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CGraph), ctypes.POINTER(ctypes.c_void_p)]),
     "cds_dgraph_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "cds_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
@@ -121,6 +137,13 @@ _SIGNATURES = {
     "cds_count_cliques": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, f64p]),
     "cds_core_decompose": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
     "cds_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(CParams), ctypes.c_size_t, ctypes.POINTER(CResult)]),
+    "cds_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerOpts), ctypes.POINTER(ctypes.c_void_p)]),
+    "cds_server_destroy": (None, [ctypes.c_void_p]),
+    "cds_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CGraph), ctypes.POINTER(CParams), ctypes.POINTER(ctypes.c_uint64)]),
+    "cds_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "cds_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(CResult), ctypes.POINTER(CRequestTimes)]),
+    "cds_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "cds_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerStats)]),
     "cds_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
     "cds_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
 }
@@ -346,3 +369,63 @@ class DeviceGraph:
         if getattr(self, "_h", None) and self._engine._h:
             lib().cds_dgraph_free(self._engine._h, self._h)
             self._h = None
+
+
+class Server:
+    """cds_server: a request loop over one engine with HBM-budget admission
+    and continuous batching.  submit() is thread-safe; wait() returns the
+    result dict with the request's queue_ms / service_ms."""
+
+    def __init__(self, engine, hbm_budget_bytes=0, max_batch=1024, max_wait_us=0,
+                 large_vertices=65536):
+        self.engine = engine
+        o = CServerOpts(int(hbm_budget_bytes), int(max_batch), int(max_wait_us),
+                        int(large_vertices))
+        h = ctypes.c_void_p()
+        _check(lib().cds_server_create(engine._h, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, graph, k, ub1=True, ub2=True, vertex_arrays=False):
+        p = CParams(int(k), 8000, 16000, 60000, 30000, -1, int(ub1), int(ub2),
+                    int(vertex_arrays))
+        t = ctypes.c_uint64()
+        _check(lib().cds_submit(self._h, ctypes.byref(graph._g), ctypes.byref(p),
+                                ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        _check(lib().cds_poll(self._h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket, timeout_s=None):
+        r = CResult()
+        tm = CRequestTimes()
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        _check(lib().cds_wait(self._h, ticket, us, ctypes.byref(r), ctypes.byref(tm)))
+        try:
+            d = result_to_dict(r)
+        finally:
+            lib().cds_result_free(ctypes.byref(r))
+        d["queue_ms"] = tm.queue_ms
+        d["service_ms"] = tm.service_ms
+        return d
+
+    def cancel(self, ticket):
+        _check(lib().cds_cancel(self._h, ticket))
+
+    def stats(self):
+        st = CServerStats()
+        _check(lib().cds_server_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in CServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().cds_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

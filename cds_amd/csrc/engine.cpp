@@ -514,6 +514,8 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
   return CDS_OK;
 }
 
+extern "C" int cds_engine_device(const cds_engine *e) { return e ? e->device : -1; }
+
 extern "C" void cds_engine_destroy(cds_engine *e) {
   if (!e) return;
   (void)hipSetDevice(e->device);

@@ -170,6 +170,7 @@ void cds_graph_free(cds_graph *g);
  *      cudaCode/src/common.cpp:4-12 and gpuMemoryAllocation.cu) ---- */
 cds_status cds_engine_create(const cds_opts *opts, cds_engine **out);
 void cds_engine_destroy(cds_engine *e);
+int cds_engine_device(const cds_engine *e);
 
 /* Upload a host graph to HBM once; solves then start from device memory
  * (replaces memoryAllocationGraph, cudaCode/src/gpuMemoryAllocation.cu:5-27). */
@@ -207,6 +208,59 @@ cds_status cds_core_decompose(cds_engine *e, const cds_dgraph *dg,
 cds_status cds_solve_batch(cds_engine *e, const cds_dgraph *const *graphs,
                            const cds_params *params, size_t count,
                            cds_result *results);
+
+/* ---- serving (no counterpart in the reference, which solves one graph
+ *      per process): a request loop over one engine with HBM-budget
+ *      admission and continuous batching.  Requests submitted while a batch
+ *      is on the GPU join the next batch; a request's answer equals
+ *      cds_solve on its graph whatever it is batched with. ---- */
+typedef struct cds_server cds_server;
+typedef uint64_t cds_ticket;
+
+typedef struct {
+  uint64_t hbm_budget_bytes; /* estimated device bytes of all admitted,
+                                unfinished requests; 0 = unlimited */
+  uint32_t max_batch;        /* requests per batch (0 = 1024) */
+  uint32_t max_wait_us;      /* after the first arrival, wait this long for
+                                more before launching (0 = launch at once) */
+  uint32_t large_vertices;   /* graphs with at least this many vertices go to
+                                the large lane, one at a time (0 = 65536) */
+} cds_server_opts;
+
+typedef struct {
+  uint64_t submitted, admitted, completed, failed, batches;
+  uint64_t max_batch_seen, peak_admitted_bytes, queued;
+} cds_server_stats;
+
+typedef struct {
+  double queue_ms;   /* submit -> admitted into a batch */
+  double service_ms; /* admitted -> completed */
+} cds_request_times;
+
+/* Two lanes, each a loop thread with its own engine: the batch lane (on
+ * `e`) admits and batches graphs below large_vertices; the large lane (an
+ * engine the server creates on e's device) solves bigger graphs one at a
+ * time.  `e` must outlive the server and should not be driven by other
+ * threads meanwhile (calls would queue on its lock). */
+cds_status cds_server_create(cds_engine *e, const cds_server_opts *opts,
+                             cds_server **out);
+/* Stops the loop (queued requests end CDS_ERR_CANCELLED) and frees results
+ * nobody waited for. */
+void cds_server_destroy(cds_server *s);
+/* Thread-safe.  The graph is copied; the caller keeps its arrays.
+ * CDS_ERR_OOM if the request alone exceeds the budget. */
+cds_status cds_submit(cds_server *s, const cds_graph *g, const cds_params *p,
+                      cds_ticket *out);
+cds_status cds_poll(cds_server *s, cds_ticket t, int *done);
+/* Waits (timeout_us < 0: forever) and hands the result over (free it with
+ * cds_result_free); CDS_ERR_BUSY on timeout, else the request's status.
+ * times (may be NULL) receives its queueing and service times.  A ticket
+ * can be waited for once. */
+cds_status cds_wait(cds_server *s, cds_ticket t, int64_t timeout_us,
+                    cds_result *out, cds_request_times *times);
+/* Removes a queued request (CDS_ERR_BUSY once it is on the GPU). */
+cds_status cds_cancel(cds_server *s, cds_ticket t);
+cds_status cds_server_get_stats(cds_server *s, cds_server_stats *out);
 
 /* ---- generators (seeded synthetic inputs; not part of the reference) ---- */
 cds_status cds_gen_chung_lu(uint32_t n, double exponent, double mean_degree,

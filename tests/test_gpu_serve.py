@@ -1,0 +1,117 @@
+"""GPU tests of the serving loop (cds_server): continuous batching with
+HBM-budget admission.
+
+Each request's result must be the same alone (cds_solve on its own) and
+in the mix (admitted into shared batches with other requests of any k).
+"""
+import math
+import threading
+
+import numpy as np
+import pytest
+
+import cds_amd
+from test_gpu_batch import same
+from test_gpu_parity import _announce
+
+pytestmark = pytest.mark.gpu
+
+
+def request_graphs(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(math.exp(rng.uniform(math.log(200), math.log(4000))))
+        g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                                   float(rng.uniform(4, 24)),
+                                   int(rng.integers(8, 61)), 0.9,
+                                   seed=seed * 1000 + i)
+        k = int(rng.choice([2, 3, 4], p=[0.3, 0.5, 0.2]))
+        out.append((g, k))
+    return out
+
+
+@pytest.fixture(scope="module")
+def serve_engine():
+    eng = cds_amd.Engine(device=0)
+    yield eng
+    eng.close()
+
+
+def test_results_same_alone_and_in_the_mix(engine, serve_engine):
+    reqs = request_graphs(36, seed=21)
+    # large_vertices=1500 sends the bigger graphs to the one-at-a-time lane
+    srv = cds_amd.Server(serve_engine, max_batch=16, large_vertices=1500)
+    tickets = [None] * len(reqs)
+
+    def producer(part):
+        for i in range(part, len(reqs), 4):
+            g, k = reqs[i]
+            tickets[i] = srv.submit(g, k)
+
+    _announce(f"serve {len(reqs)} requests from 4 threads")
+    threads = [threading.Thread(target=producer, args=(p,)) for p in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    served = [srv.wait(t, timeout_s=120) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    assert stats["batches"] <= len(reqs)
+    for i, ((g, k), got) in enumerate(zip(reqs, served)):
+        alone = engine.solve(engine.upload(g), k)
+        diff = same(got, alone)
+        assert diff is None, f"request {i} (n={g.n}, k={k}): {diff} differs in the mix"
+        assert got["queue_ms"] >= 0 and got["service_ms"] >= 0
+
+
+def test_budget_admits_one_at_a_time(engine, serve_engine):
+    reqs = request_graphs(6, seed=22)
+    # a budget that holds any one of these requests but no two of them
+    est = [16 * g.n + 24 * g.m + k * g.m * (4 * k + 4 * k * k + 8) + (1 << 20)
+           for g, k in reqs]
+    budget = max(est) + min(est) - 1
+    srv = cds_amd.Server(serve_engine, hbm_budget_bytes=budget)
+    _announce("serve under a one-request HBM budget")
+    tickets = [srv.submit(g, k) for g, k in reqs]
+    served = [srv.wait(t, timeout_s=120) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    assert stats["peak_admitted_bytes"] <= budget
+    for (g, k), got in zip(reqs, served):
+        assert same(got, engine.solve(engine.upload(g), k)) is None
+
+
+def test_request_over_budget_is_refused(serve_engine):
+    g, k = request_graphs(1, seed=23)[0]
+    srv = cds_amd.Server(serve_engine, hbm_budget_bytes=1 << 20)
+    with pytest.raises(cds_amd.CdsError) as ei:
+        srv.submit(g, k)
+    assert ei.value.status == 4  # CDS_ERR_OOM
+    srv.close()
+
+
+def test_cancel_and_unknown_ticket(serve_engine):
+    reqs = request_graphs(12, seed=24)
+    srv = cds_amd.Server(serve_engine, max_batch=1)
+    tickets = [srv.submit(g, k) for g, k in reqs]
+    last = tickets[-1]
+    cancelled = True
+    try:
+        srv.cancel(last)
+    except cds_amd.CdsError as ex:  # already on the GPU or done
+        assert ex.status == 8
+        cancelled = False
+    for t in tickets[:-1]:
+        srv.wait(t, timeout_s=120)
+    if cancelled:
+        with pytest.raises(cds_amd.CdsError) as ei:
+            srv.wait(last, timeout_s=10)
+        assert ei.value.status == 9  # CDS_ERR_CANCELLED
+    else:
+        srv.wait(last, timeout_s=120)
+    with pytest.raises(cds_amd.CdsError):
+        srv.poll(123456789)
+    srv.close()
