@@ -99,13 +99,22 @@ def bench_batched(args):
     dgs = [eng.upload(g) for g in graphs]
     eng.solve_batch(dgs[:64], ks[:64])  # warm-up
     t0 = time.perf_counter()
-    plain = [eng.solve(d, k) for d, k in zip(dgs, ks)]
+    plain = []
+    for i, (d, k) in enumerate(zip(dgs, ks)):
+        plain.append(eng.solve(d, k))
+        if (i + 1) % 512 == 0:
+            print(f"[bench] plain {i + 1}: {time.perf_counter() - t0:.2f}s",
+                  file=sys.stderr, flush=True)
     t_plain = time.perf_counter() - t0
     print(f"[bench] plain: {t_plain:.2f}s", file=sys.stderr, flush=True)
     times = []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        got = eng.solve_batch(dgs, ks)
+        got = []
+        for c in range(0, len(dgs), args.chunk):
+            got += eng.solve_batch(dgs[c:c + args.chunk], ks[c:c + args.chunk])
+            print(f"[bench] batched step {i}: {c + args.chunk} graphs "
+                  f"{time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
         times.append(time.perf_counter() - t0)
         print(f"[bench] batched step {i}: {times[-1]:.2f}s", file=sys.stderr, flush=True)
     mism = sum(1 for a, b in zip(got, plain) if not same_answer(a, b))
@@ -121,7 +130,8 @@ def bench_batched(args):
                                "log-uniform [200, 20000], exponent [2.1, 3.0], mean "
                                "degree [4, 24], planted G(s, 0.9) s in [8, 60], k = 3 "
                                "(70%) / 4 (30%)",
-                   "graphs": len(graphs), "vertices": n_total, "seed": args.seed},
+                   "graphs": len(graphs), "graphs_per_call": args.chunk,
+                   "vertices": n_total, "seed": args.seed},
         "plain_solves_per_s": len(graphs) / t_plain,
         "batched_vs_plain": t_plain / t_batch,
         "mismatches_vs_plain": mism,
@@ -363,6 +373,7 @@ def main():
                     choices=sorted(WORKLOADS) + ["c4", "c5"],
                     help="c4: batched form; c5: served form")
     ap.add_argument("--graphs", type=int, default=4096, help="c4: graphs")
+    ap.add_argument("--chunk", type=int, default=512, help="c4: graphs per batched call")
     ap.add_argument("--duration", type=float, default=20.0, help="c5: seconds per load")
     ap.add_argument("--loads", default="20,60,180", help="c5: offered requests/s")
     ap.add_argument("--seed", type=int, default=4)

@@ -257,10 +257,22 @@ struct FlowRequest {
 // advance many graphs in lockstep with batched flows.  The arithmetic is the
 // oracle's: float lb/ub/alpha/bias, cumulative ub over components (UB1/UB2),
 // decision |F - kT| < 1e-2f evaluated exactly on the scaled integer flow,
-// strict improvement of the float density.  A component first gets one
-// flow at its own density kT/V (capacities scaled by V); if that saturates,
-// the component is its own densest subgraph and every alpha query is
-// answered without a flow.
+// strict improvement of the float density, result set = the minimal
+// min-cut side of the flow at the last "not equal" alpha.
+//
+// The max flow at sink capacity c per vertex is F(c) = kT - h(c) with
+// h(c) = max over vertex sets S of k T(S) - c |S|: the upper envelope of one
+// line per set, convex and piecewise linear, and its minimal maximisers are
+// nested.  Before a component's binary search the envelope is built
+// exactly: starting from the lines of the empty set and the whole
+// component, one flow at the crossing c = k dT / dV of two adjacent known
+// lines (capacities scaled by dV, so every capacity is an integer) either
+// returns a set strictly above both (inserted between them; its size and
+// clique count come from the flow's min-cut side) or proves the envelope
+// there is just those two lines.  A component with r envelope sets costs
+// 2r - 3 flows (one, the certificate, when it is its own densest subgraph)
+// instead of one flow per alpha step; every alpha query is then answered
+// from the envelope with the same integer arithmetic as a flow.
 class ExactSearch {
  public:
   ExactSearch(uint32_t k, bool ub1, bool ub2, const std::vector<Component> &comps,
@@ -318,46 +330,80 @@ class ExactSearch {
           target_ = (float)(int64_t)(C.T * k_);
           res.assign(C.V, 1);
           res_cliques_ = C.T;
-          searched_ = false;
-          whole_ = -1;
-          state_ = kLoop;
+          if (!((ub_ - lb_) > bias_)) {
+            finish_component(false);
+            break;
+          }
+          // envelope: the empty set and the whole component
+          chain_.clear();
+          chain_.push_back({0, 0, {}});
+          chain_.push_back({C.T, C.V, {}});
+          confirmed_.assign(1, 0);
+          state_ = kEnvelope;
           break;
+        }
+        case kEnvelope: {
+          size_t i = 0;
+          while (i < confirmed_.size() && confirmed_[i]) ++i;
+          if (i == confirmed_.size()) {
+            if (chain_.size() == 2) certificates++;
+            state_ = kLoop;
+            break;
+          }
+          const EnvSet &a = chain_[i], &b = chain_[i + 1];
+          seg_ = i;
+          req->comp = ci_;
+          req->capT = (long long)(k_ * (b.T - a.T));
+          req->U = (long long)(b.V - a.V);
+          state_ = kWaitFlow;
+          return true;
         }
         case kLoop: {
           if (!((ub_ - lb_) > bias_)) {
-            finish_component();
+            finish_component(true);
             break;
           }
           const Component &C = comps_[ci_];
-          searched_ = true;
           const float cap_t = alpha_ * (float)k_;
           int e2 = 0;
           std::frexp(cap_t, &e2);
-          sc_ = std::max(0, 24 - e2);
-          if (sc_ > 40) {
+          const int sc = std::max(0, 24 - e2);
+          if (sc > 40) {
             *err = fail(CDS_ERR_RANGE, "alpha %g too small for exact scaling", alpha_);
             return false;
           }
-          U_ = 1ll << sc_;
-          capT_ = (long long)std::ldexp((double)cap_t, sc_);
-          if (whole_ < 0) {
-            req->comp = ci_;
-            req->capT = (long long)(C.T * k_);
-            req->U = (long long)C.V;
-            state_ = kWaitCert;
-            return true;
+          // h(c) from the envelope, c = fl(alpha k) scaled by 2^sc; the
+          // minimal maximiser is the smallest set attaining it
+          const __int128 capT = (__int128)std::ldexp((double)cap_t, sc);
+          __int128 h = 0;
+          size_t arg = 0;
+          for (size_t j = 1; j < chain_.size(); ++j) {
+            const __int128 v = ((__int128)(chain_[j].T * k_) << sc) - capT * chain_[j].V;
+            if (v > h) {
+              h = v;
+              arg = j;
+            }
           }
-          if (whole_ == 1) {
-            __int128 g = ((__int128)(C.T * k_) << sc_) - (__int128)capT_ * C.V;
-            if (g < 0) g = 0;
-            decide(((__int128)(C.T * k_) << sc_) - g, false);
-            break;
+          const __int128 F = ((__int128)(C.T * k_) << sc) - h;
+          const bool eq = flow_equal(F, target_, sc);
+          tr_comp.push_back(ci_);
+          tr_lb.push_back(lb_);
+          tr_ub.push_back(ub_);
+          tr_alpha.push_back(alpha_);
+          tr_margin.push_back((double)(C.T * k_) - std::ldexp((double)F, -sc));
+          if (eq) {
+            ub_ = alpha_;
+          } else {
+            lb_ = alpha_;
+            const EnvSet &s = chain_[arg];
+            if (s.reach.empty())  // the whole component
+              res.assign(C.V, 1);
+            else
+              for (uint32_t v = 0; v < C.V; ++v) res[v] = s.reach[v] ? 1 : -1;
+            res_cliques_ = s.T;
           }
-          req->comp = ci_;
-          req->capT = capT_;
-          req->U = U_;
-          state_ = kWaitFlow;
-          return true;
+          alpha_ = (lb_ + ub_) * 0.5f;
+          break;
         }
         default:
           *err = fail(CDS_ERR_INTERNAL, "exact search advanced while waiting");
@@ -370,25 +416,43 @@ class ExactSearch {
   // min-cut side of that flow (feed_reach), false if it can advance.
   bool feed_flow(long long F) {
     const Component &C = comps_[ci_];
-    if (state_ == kWaitCert) {
-      // back to the loop with the same alpha: answered analytically if the
-      // component is its own densest subgraph, else by its own flow
-      whole_ = (__int128)F == (__int128)(C.T * k_) * C.V ? 1 : 0;
-      if (whole_) certificates++;
-      state_ = kLoop;
-      return false;
-    }
     flows++;
-    return decide(F, true);
+    const EnvSet &a = chain_[seg_];
+    const __int128 p = (__int128)k_ * (chain_[seg_ + 1].T - a.T);
+    const __int128 q = (__int128)(chain_[seg_ + 1].V - a.V);
+    // h at the crossing (scaled by q) versus the two known lines there
+    last_h_ = (__int128)(C.T * k_) * q - (__int128)F;
+    const __int128 known = (__int128)(a.T * k_) * q - p * a.V;
+    if (last_h_ > known) {
+      state_ = kWaitReach;
+      return true;
+    }
+    confirmed_[seg_] = 1;
+    state_ = kEnvelope;
+    return false;
   }
 
-  // reach[v] != 0 for the component's min-cut side; `cliques` inside it.
-  void feed_reach(const int *reach, uint64_t cliques) {
+  // reach[v] != 0 for the flow's min-cut side; `cliques` inside it.
+  cds_status feed_reach(const int *reach, uint64_t cliques) {
     const Component &C = comps_[ci_];
-    for (uint32_t v = 0; v < C.V; ++v) res[v] = reach[v] ? 1 : -1;
-    res_cliques_ = cliques;
-    alpha_ = (lb_ + ub_) * 0.5f;
-    state_ = kLoop;
+    EnvSet s;
+    s.T = cliques;
+    s.V = 0;
+    s.reach.assign(reach, reach + C.V);
+    for (uint32_t v = 0; v < C.V; ++v) s.V += reach[v] != 0;
+    const EnvSet &a = chain_[seg_], &b = chain_[seg_ + 1];
+    const __int128 p = (__int128)k_ * (b.T - a.T);
+    const __int128 q = (__int128)(b.V - a.V);
+    if (!(s.V > a.V && s.V < b.V) ||
+        (__int128)(s.T * k_) * q - p * s.V != last_h_)
+      return fail(CDS_ERR_INTERNAL,
+                  "component %u: envelope set (%llu cliques, %u vertices) inconsistent",
+                  ci_, (unsigned long long)s.T, s.V);
+    chain_.insert(chain_.begin() + seg_ + 1, std::move(s));
+    confirmed_[seg_] = 0;
+    confirmed_.insert(confirmed_.begin() + seg_ + 1, 0);
+    state_ = kEnvelope;
+    return CDS_OK;
   }
 
   bool done() const { return state_ == kDone; }
@@ -406,38 +470,20 @@ class ExactSearch {
   uint32_t flows_solved = 0, certificates = 0, flows = 0;
 
  private:
-  enum State { kCompStart, kLoop, kWaitCert, kWaitFlow, kWaitReach, kDone };
+  enum State { kCompStart, kEnvelope, kWaitFlow, kWaitReach, kLoop, kDone };
 
-  // Decision for the current alpha from F (scaled by 2^sc).  Returns true
-  // if the min-cut side of the flow is needed.
-  bool decide(__int128 F, bool from_flow) {
-    const Component &C = comps_[ci_];
-    const bool eq = flow_equal(F, target_, sc_);
-    tr_comp.push_back(ci_);
-    tr_lb.push_back(lb_);
-    tr_ub.push_back(ub_);
-    tr_alpha.push_back(alpha_);
-    tr_margin.push_back((double)(C.T * k_) - std::ldexp((double)F, -sc_));
-    if (eq) {
-      ub_ = alpha_;
-    } else {
-      lb_ = alpha_;
-      if (from_flow) {
-        state_ = kWaitReach;
-        return true;
-      }
-      for (uint32_t v = 0; v < C.V; ++v) res[v] = 1;
-      res_cliques_ = C.T;
-    }
-    alpha_ = (lb_ + ub_) * 0.5f;
-    state_ = kLoop;
-    return false;
-  }
+  // one line of the envelope: a vertex set, its cliques and size, and its
+  // membership (empty for the empty set and the whole component)
+  struct EnvSet {
+    uint64_t T;
+    uint32_t V;
+    std::vector<int> reach;
+  };
 
   // result accounting of one component (helpers.cpp:1210-1252)
-  void finish_component() {
+  void finish_component(bool searched) {
     const Component &C = comps_[ci_];
-    if (searched_) flows_solved++;
+    if (searched) flows_solved++;
     uint32_t vcount = 0;
     for (uint32_t v = 0; v < C.V; ++v) vcount += res[v] != -1;
     if (vcount == 0) vcount = C.V;
@@ -462,11 +508,11 @@ class ExactSearch {
   State state_ = kCompStart;
   uint32_t ci_ = 0;
   float lb_ = 0, ub_ = 0, alpha_ = 0, bias_ = 0, target_ = 0;
-  int sc_ = 0;
-  long long U_ = 0, capT_ = 0;
   uint64_t res_cliques_ = 0;
-  bool searched_ = false;
-  int whole_ = -1;
+  std::vector<EnvSet> chain_;  // ascending size
+  std::vector<char> confirmed_;  // per adjacent pair of chain_
+  size_t seg_ = 0;
+  __int128 last_h_ = 0;
 };
 
 }  // namespace
@@ -1305,7 +1351,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                                       C.vbase, freach.p, acc.p, st));
         unsigned long long cc = 0;
         CHK(readback(e, &cc, acc.p, 1));
-        search.feed_reach(reach_h.data(), cc);
+        CHK(search.feed_reach(reach_h.data(), cc));
       }
     }
     if (serr != CDS_OK) return serr;
@@ -2009,7 +2055,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       sweeps[b] += hctl[i].rounds;
       rs[b]->flow_launches++;
       if (search[b]->feed_flow((long long)hctl[i].sink))
-        search[b]->feed_reach(hreach.data() + vo[i], hcnt[i]);
+        CHK(search[b]->feed_reach(hreach.data() + vo[i], hcnt[i]));
     }
   }
   const int ev_exact = tm.mark();
