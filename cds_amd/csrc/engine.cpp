@@ -269,10 +269,11 @@ struct FlowRequest {
 // lines (capacities scaled by dV, so every capacity is an integer) either
 // returns a set strictly above both (inserted between them; its size and
 // clique count come from the flow's min-cut side) or proves the envelope
-// there is just those two lines.  A component with r envelope sets costs
-// 2r - 3 flows (one, the certificate, when it is its own densest subgraph)
-// instead of one flow per alpha step; every alpha query is then answered
-// from the envelope with the same integer arithmetic as a flow.
+// there is just those two lines.  Only segments whose lines can be maximal
+// inside the range of c the search will query are refined, so a component
+// costs a few flows (one, the certificate, when it is its own densest
+// subgraph) instead of one per alpha step; every alpha query is then
+// answered from the envelope with the same integer arithmetic as a flow.
 class ExactSearch {
  public:
   ExactSearch(uint32_t k, bool ub1, bool ub2, const std::vector<Component> &comps,
@@ -334,17 +335,27 @@ class ExactSearch {
             finish_component(false);
             break;
           }
-          // envelope: the empty set and the whole component
+          // envelope: the empty set (tight for every c above the densest
+          // density) and the whole component (tight at c = 0)
           chain_.clear();
-          chain_.push_back({0, 0, {}});
-          chain_.push_back({C.T, C.V, {}});
+          chain_.push_back({0, 0, {}, HUGE_VAL});
+          chain_.push_back({C.T, C.V, {}, 0.0});
           confirmed_.assign(1, 0);
+          // the search only asks for c = fl(alpha k) with lb <= alpha <= ub
+          c_lo_ = (double)(lb_ * (float)k_);
+          c_hi_ = (double)(ub_ * (float)k_);
           state_ = kEnvelope;
           break;
         }
         case kEnvelope: {
+          // A segment's unknown lines can only be maximal between the
+          // points where its two known lines are tight; segments whose span
+          // misses the queried range need no flow.
           size_t i = 0;
-          while (i < confirmed_.size() && confirmed_[i]) ++i;
+          while (i < confirmed_.size() &&
+                 (confirmed_[i] || chain_[i].tight < c_lo_ * (1 - 1e-12) ||
+                  chain_[i + 1].tight > c_hi_ * (1 + 1e-12)))
+            ++i;
           if (i == confirmed_.size()) {
             if (chain_.size() == 2) certificates++;
             state_ = kLoop;
@@ -443,6 +454,7 @@ class ExactSearch {
     const EnvSet &a = chain_[seg_], &b = chain_[seg_ + 1];
     const __int128 p = (__int128)k_ * (b.T - a.T);
     const __int128 q = (__int128)(b.V - a.V);
+    s.tight = (double)p / (double)q;  // found as the maximiser at this c
     if (!(s.V > a.V && s.V < b.V) ||
         (__int128)(s.T * k_) * q - p * s.V != last_h_)
       return fail(CDS_ERR_INTERNAL,
@@ -478,6 +490,7 @@ class ExactSearch {
     uint64_t T;
     uint32_t V;
     std::vector<int> reach;
+    double tight;  // a c at which the set is a maximiser
   };
 
   // result accounting of one component (helpers.cpp:1210-1252)
@@ -513,6 +526,7 @@ class ExactSearch {
   std::vector<char> confirmed_;  // per adjacent pair of chain_
   size_t seg_ = 0;
   __int128 last_h_ = 0;
+  double c_lo_ = 0, c_hi_ = 0;
 };
 
 }  // namespace
