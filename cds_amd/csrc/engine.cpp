@@ -270,10 +270,11 @@ struct FlowRequest {
 // returns a set strictly above both (inserted between them; its size and
 // clique count come from the flow's min-cut side) or proves the envelope
 // there is just those two lines.  Only segments whose lines can be maximal
-// inside the range of c the search will query are refined, so a component
-// costs a few flows (one, the certificate, when it is its own densest
-// subgraph) instead of one per alpha step; every alpha query is then
-// answered from the envelope with the same integer arithmetic as a flow.
+// inside the range of c the search will query are refined, all of them in
+// one round of independent flows, so a component costs a few flows (one,
+// the certificate, when it is its own densest subgraph) instead of one per
+// alpha step; every alpha query is then answered from the envelope with the
+// same integer arithmetic as a flow.
 class ExactSearch {
  public:
   ExactSearch(uint32_t k, bool ub1, bool ub2, const std::vector<Component> &comps,
@@ -301,10 +302,12 @@ class ExactSearch {
       if (paper_lower_bound < d) paper_lower_bound = d;
   }
 
-  // Advances the search until it needs a flow (true, *req filled) or ends
-  // (false).  A failure leaves *err != CDS_OK (message already set).
-  bool next(FlowRequest *req, cds_status *err) {
+  // Advances the search until it needs flows (true: *reqs holds one round
+  // of independent flows, each to be answered with feed_flow / feed_reach
+  // by its index) or ends (false).  A failure leaves *err != CDS_OK.
+  bool next(std::vector<FlowRequest> *reqs, cds_status *err) {
     *err = CDS_OK;
+    reqs->clear();
     for (;;) {
       switch (state_) {
         case kCompStart: {
@@ -347,26 +350,47 @@ class ExactSearch {
           state_ = kEnvelope;
           break;
         }
+        case kRound: {
+          // insert the sets the last round found (right to left, so the
+          // earlier segments keep their indices) and mark the rest final
+          for (size_t r = round_.size(); r-- > 0;) {
+            Probe &pr = round_[r];
+            if (pr.found) {
+              chain_.insert(chain_.begin() + pr.seg + 1, std::move(pr.set));
+              confirmed_.insert(confirmed_.begin() + pr.seg + 1, 0);
+            } else {
+              confirmed_[pr.seg] = 1;
+            }
+          }
+          round_.clear();
+          state_ = kEnvelope;
+          break;
+        }
         case kEnvelope: {
           // A segment's unknown lines can only be maximal between the
           // points where its two known lines are tight; segments whose span
-          // misses the queried range need no flow.
-          size_t i = 0;
-          while (i < confirmed_.size() &&
-                 (confirmed_[i] || chain_[i].tight < c_lo_ * (1 - 1e-12) ||
-                  chain_[i + 1].tight > c_hi_ * (1 + 1e-12)))
-            ++i;
-          if (i == confirmed_.size()) {
+          // misses the queried range need no flow.  All others are probed
+          // in one round.
+          for (size_t i = 0; i < confirmed_.size(); ++i) {
+            if (confirmed_[i] || chain_[i].tight < c_lo_ * (1 - 1e-12) ||
+                chain_[i + 1].tight > c_hi_ * (1 + 1e-12))
+              continue;
+            const EnvSet &a = chain_[i], &b = chain_[i + 1];
+            FlowRequest rq;
+            rq.comp = ci_;
+            rq.capT = (long long)(k_ * (b.T - a.T));
+            rq.U = (long long)(b.V - a.V);
+            reqs->push_back(rq);
+            Probe pr;
+            pr.seg = i;
+            round_.push_back(std::move(pr));
+          }
+          if (reqs->empty()) {
             if (chain_.size() == 2) certificates++;
             state_ = kLoop;
             break;
           }
-          const EnvSet &a = chain_[i], &b = chain_[i + 1];
-          seg_ = i;
-          req->comp = ci_;
-          req->capT = (long long)(k_ * (b.T - a.T));
-          req->U = (long long)(b.V - a.V);
-          state_ = kWaitFlow;
+          state_ = kRound;
           return true;
         }
         case kLoop: {
@@ -423,47 +447,39 @@ class ExactSearch {
     }
   }
 
-  // Result of the last request.  Returns true if the search now needs the
-  // min-cut side of that flow (feed_reach), false if it can advance.
-  bool feed_flow(long long F) {
+  // Result of request i of the current round.  Returns true if the search
+  // also needs that flow's min-cut side (feed_reach).
+  bool feed_flow(size_t i, long long F) {
     const Component &C = comps_[ci_];
     flows++;
-    const EnvSet &a = chain_[seg_];
-    const __int128 p = (__int128)k_ * (chain_[seg_ + 1].T - a.T);
-    const __int128 q = (__int128)(chain_[seg_ + 1].V - a.V);
+    Probe &pr = round_[i];
+    const EnvSet &a = chain_[pr.seg], &b = chain_[pr.seg + 1];
+    const __int128 p = (__int128)k_ * (b.T - a.T);
+    const __int128 q = (__int128)(b.V - a.V);
     // h at the crossing (scaled by q) versus the two known lines there
-    last_h_ = (__int128)(C.T * k_) * q - (__int128)F;
+    pr.h = (__int128)(C.T * k_) * q - (__int128)F;
     const __int128 known = (__int128)(a.T * k_) * q - p * a.V;
-    if (last_h_ > known) {
-      state_ = kWaitReach;
-      return true;
-    }
-    confirmed_[seg_] = 1;
-    state_ = kEnvelope;
-    return false;
+    pr.found = pr.h > known;
+    return pr.found;
   }
 
-  // reach[v] != 0 for the flow's min-cut side; `cliques` inside it.
-  cds_status feed_reach(const int *reach, uint64_t cliques) {
+  // reach[v] != 0 for request i's min-cut side; `cliques` inside it.
+  cds_status feed_reach(size_t i, const int *reach, uint64_t cliques) {
     const Component &C = comps_[ci_];
-    EnvSet s;
+    Probe &pr = round_[i];
+    EnvSet &s = pr.set;
     s.T = cliques;
     s.V = 0;
     s.reach.assign(reach, reach + C.V);
     for (uint32_t v = 0; v < C.V; ++v) s.V += reach[v] != 0;
-    const EnvSet &a = chain_[seg_], &b = chain_[seg_ + 1];
+    const EnvSet &a = chain_[pr.seg], &b = chain_[pr.seg + 1];
     const __int128 p = (__int128)k_ * (b.T - a.T);
     const __int128 q = (__int128)(b.V - a.V);
     s.tight = (double)p / (double)q;  // found as the maximiser at this c
-    if (!(s.V > a.V && s.V < b.V) ||
-        (__int128)(s.T * k_) * q - p * s.V != last_h_)
+    if (!(s.V > a.V && s.V < b.V) || (__int128)(s.T * k_) * q - p * s.V != pr.h)
       return fail(CDS_ERR_INTERNAL,
                   "component %u: envelope set (%llu cliques, %u vertices) inconsistent",
                   ci_, (unsigned long long)s.T, s.V);
-    chain_.insert(chain_.begin() + seg_ + 1, std::move(s));
-    confirmed_[seg_] = 0;
-    confirmed_.insert(confirmed_.begin() + seg_ + 1, 0);
-    state_ = kEnvelope;
     return CDS_OK;
   }
 
@@ -482,15 +498,15 @@ class ExactSearch {
   uint32_t flows_solved = 0, certificates = 0, flows = 0;
 
  private:
-  enum State { kCompStart, kEnvelope, kWaitFlow, kWaitReach, kLoop, kDone };
+  enum State { kCompStart, kEnvelope, kRound, kLoop, kDone };
 
   // one line of the envelope: a vertex set, its cliques and size, and its
   // membership (empty for the empty set and the whole component)
   struct EnvSet {
-    uint64_t T;
-    uint32_t V;
+    uint64_t T = 0;
+    uint32_t V = 0;
     std::vector<int> reach;
-    double tight;  // a c at which the set is a maximiser
+    double tight = 0;  // a c at which the set is a maximiser
   };
 
   // result accounting of one component (helpers.cpp:1210-1252)
@@ -524,8 +540,15 @@ class ExactSearch {
   uint64_t res_cliques_ = 0;
   std::vector<EnvSet> chain_;  // ascending size
   std::vector<char> confirmed_;  // per adjacent pair of chain_
-  size_t seg_ = 0;
-  __int128 last_h_ = 0;
+  // one flow of the current round: the chain segment it probes and what
+  // it found there
+  struct Probe {
+    size_t seg = 0;
+    __int128 h = 0;
+    bool found = false;
+    EnvSet set;
+  };
+  std::vector<Probe> round_;
   double c_lo_ = 0, c_hi_ = 0;
 };
 
@@ -1348,24 +1371,27 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     return CDS_OK;
   };
   {
-    FlowRequest rq;
+    std::vector<FlowRequest> round;
     cds_status serr = CDS_OK;
-    while (search.next(&rq, &serr)) {
+    while (search.next(&round, &serr)) {
       if (!flow_alloc) {
         HIPCHK(freach.alloc(maxV));
         flow_alloc = true;
       }
-      long long F = 0;
-      CHK(run_flow(rq.comp, rq.capT, rq.U, &F));
-      if (search.feed_flow(F)) {
-        const Component &C = comps[rq.comp];
-        reach_h.resize(C.V);
-        CHK(readback(e, reach_h.data(), freach.p, C.V));
-        HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
-                                      C.vbase, freach.p, acc.p, st));
-        unsigned long long cc = 0;
-        CHK(readback(e, &cc, acc.p, 1));
-        CHK(search.feed_reach(reach_h.data(), cc));
+      for (size_t i = 0; i < round.size(); ++i) {
+        const FlowRequest &rq = round[i];
+        long long F = 0;
+        CHK(run_flow(rq.comp, rq.capT, rq.U, &F));
+        if (search.feed_flow(i, F)) {
+          const Component &C = comps[rq.comp];
+          reach_h.resize(C.V);
+          CHK(readback(e, reach_h.data(), freach.p, C.V));
+          HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
+                                        C.vbase, freach.p, acc.p, st));
+          unsigned long long cc = 0;
+          CHK(readback(e, &cc, acc.p, 1));
+          CHK(search.feed_reach(i, reach_h.data(), cc));
+        }
       }
     }
     if (serr != CDS_OK) return serr;
@@ -1906,16 +1932,18 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   for (;;) {
     struct Pending {
       uint32_t b;
+      size_t i;  // index in the graph's round
       FlowRequest rq;
       const Component *C;
     };
     std::vector<Pending> pend;
+    std::vector<FlowRequest> round;
     for (uint32_t b = 0; b < B; ++b) {
       if (!active[b]) continue;
-      FlowRequest rq;
       cds_status serr = CDS_OK;
-      if (search[b]->next(&rq, &serr)) {
-        pend.push_back({b, rq, &gcomps[b][rq.comp]});
+      if (search[b]->next(&round, &serr)) {
+        for (size_t i = 0; i < round.size(); ++i)
+          pend.push_back({b, i, round[i], &gcomps[b][round[i].comp]});
       } else {
         if (serr != CDS_OK) return serr;
         active[b] = 0;
@@ -2068,8 +2096,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
                     hctl[i].rounds);
       sweeps[b] += hctl[i].rounds;
       rs[b]->flow_launches++;
-      if (search[b]->feed_flow((long long)hctl[i].sink))
-        CHK(search[b]->feed_reach(hreach.data() + vo[i], hcnt[i]));
+      if (search[b]->feed_flow(pend[i].i, (long long)hctl[i].sink))
+        CHK(search[b]->feed_reach(pend[i].i, hreach.data() + vo[i], hcnt[i]));
     }
   }
   const int ev_exact = tm.mark();
