@@ -189,11 +189,23 @@ __device__ unsigned long long global_relabel(const G &g) {
     cb = 1 - cb;
     d += 2;
   }
+  // Active nodes; the active cliques also become the clique work list of
+  // the rounds that follow (cq0, count cqn[0]; cq1 starts empty).
+  if (g.leader) {
+    ctl->cqn[0] = 0;
+    ctl->cqn[1] = 0;
+  }
+  gsync(g);
   unsigned long long act = 0;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
     act += g.exv[v] > 0 && g.hv[v] < g.INF;
-  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads)
-    act += g.exc[c] > 0 && g.hc[c] < g.INF;
+  for (uint32_t w0 = g.gtid - lane(); w0 < g.T; w0 += g.gthreads) {
+    const uint32_t c = w0 + lane();
+    const bool on = c < g.T && g.exc[c] > 0 && g.hc[c] < g.INF;
+    act += on;
+    const uint32_t slot = aggregated_reserve(&ctl->cqn[0], 0, on);
+    if (on) g.cq0[slot] = c;
+  }
   act = wave_sum64(act);
   if (lane() == 0 && act) atomicAdd(&ctl->active, act);
   gsync(g);
@@ -262,6 +274,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
   }
   gsync(g);
   unsigned long long active = global_relabel(g);
+  int wb = 0;  // clique work list of this round (global_relabel fills cq0)
   uint32_t rounds = 0, relabels = 1, since = 0;
   unsigned long long sink = 0;
   bool limit = false;
@@ -281,41 +294,55 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) lds_ex[i] = 0;
       __syncthreads();
     }
+    // Only cliques on the work list (those holding excess) are visited;
+    // one that keeps excess moves to the next list, and phase 3 appends a
+    // clique when its excess goes from zero to positive, so every clique
+    // is on a list at most once.
     unsigned long long act = 0;
-    for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) {
-      long long e = g.exc[c];
-      const int hcc = g.hc[c];
-      if (e <= 0 || hcc >= g.INF) continue;
-      ++act;
-      const uint64_t base = (uint64_t)c * g.k;
-      int hmin = g.INF;
-      for (uint32_t j = 0; j < g.k && e > 0; ++j) {
-        const long long res = U + g.x[base + j];
-        if (res <= 0) continue;
-        const uint32_t v = g.mem[base + j] - g.vbase;
-        const int hvv = ald(&g.hv[v]);
-        if (hvv < hcc) {
-          const long long d = e < res ? e : res;
-          g.x[base + j] -= d;
-          e -= d;
-          if (lds_v)
-            atomicAdd((unsigned long long *)&lds_ex[v], (unsigned long long)d);
-          else
-            atomicAdd((unsigned long long *)&g.exv[v], (unsigned long long)d);
-          if (d < res) hmin = min(hmin, hvv);
-        } else {
-          hmin = min(hmin, hvv);
+    const uint32_t nwl = ald(&ctl->cqn[wb]);
+    const uint32_t *wl = gcq(g, wb);
+    uint32_t *nx = gcq(g, 1 - wb);
+    for (uint32_t w0 = g.gtid - lane(); w0 < nwl; w0 += g.gthreads) {
+      const uint32_t i = w0 + lane();
+      bool keep = false;  // still holds excess: onto the next list
+      const uint32_t c = i < nwl ? wl[i] : 0;
+      long long e = i < nwl ? g.exc[c] : 0;
+      const int hcc = i < nwl ? g.hc[c] : g.INF;
+      if (e > 0 && hcc < g.INF) {
+        ++act;
+        const uint64_t base = (uint64_t)c * g.k;
+        int hmin = g.INF;
+        for (uint32_t j = 0; j < g.k && e > 0; ++j) {
+          const long long res = U + g.x[base + j];
+          if (res <= 0) continue;
+          const uint32_t v = g.mem[base + j] - g.vbase;
+          const int hvv = ald(&g.hv[v]);
+          if (hvv < hcc) {
+            const long long d = e < res ? e : res;
+            g.x[base + j] -= d;
+            e -= d;
+            if (lds_v)
+              atomicAdd((unsigned long long *)&lds_ex[v], (unsigned long long)d);
+            else
+              atomicAdd((unsigned long long *)&g.exv[v], (unsigned long long)d);
+            if (d < res) hmin = min(hmin, hvv);
+          } else {
+            hmin = min(hmin, hvv);
+          }
+        }
+        g.exc[c] = e;
+        if (e > 0) {
+          // all lower members are saturated; recompute over residual arcs
+          hmin = g.INF;
+          for (uint32_t j = 0; j < g.k; ++j)
+            if (U + g.x[base + j] > 0)
+              hmin = min(hmin, ald(&g.hv[g.mem[base + j] - g.vbase]));
+          g.hc[c] = hmin >= g.INF - 1 ? g.INF : hmin + 1;
+          keep = g.hc[c] < g.INF;
         }
       }
-      g.exc[c] = e;
-      if (e > 0) {
-        // all lower members are saturated; recompute over residual arcs
-        hmin = g.INF;
-        for (uint32_t j = 0; j < g.k; ++j)
-          if (U + g.x[base + j] > 0)
-            hmin = min(hmin, ald(&g.hv[g.mem[base + j] - g.vbase]));
-        g.hc[c] = hmin >= g.INF - 1 ? g.INF : hmin + 1;
-      }
+      const uint32_t slot = aggregated_reserve(&ctl->cqn[1 - wb], 0, keep);
+      if (keep) nx[slot] = c;
     }
     if (lds_v) {
       __syncthreads();
@@ -325,7 +352,10 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       }
     }
     gsync(g);
-    // 2. vertices push to the sink; active vertices post their chunks
+    // 2. vertices push to the sink; active vertices post their chunks.
+    // Every workgroup has read this round's clique list count: it is
+    // cleared for the round after next.
+    if (g.leader) ctl->cqn[wb] = 0;
     for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
       long long e = g.exv[v];
       if (e <= 0 || g.hv[v] >= g.INF) {
@@ -406,10 +436,14 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         const long long start = before + incl - ra;
         long long take = avail - start;
         take = take < 0 ? 0 : (take < ra ? take : ra);
+        bool fresh = false;  // the clique's excess goes from zero to positive
         if (take > 0) {
           g.x[s] += take;
-          atomicAdd((unsigned long long *)&g.exc[s / g.k], (unsigned long long)take);
+          fresh = atomicAdd((unsigned long long *)&g.exc[s / g.k],
+                            (unsigned long long)take) == 0;
         }
+        const uint32_t slot = aggregated_reserve(&ctl->cqn[1 - wb], 0, fresh);
+        if (fresh) nx[slot] = (uint32_t)(s / g.k);
         before += __shfl(incl, kWave - 1, kWave);
       }
     }
@@ -428,8 +462,10 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     gsync(g);
     ++rounds;
     ++since;
+    wb = 1 - wb;
     if (any == 0 || since >= kGridRelabel) {
       active = global_relabel(g);
+      wb = 0;
       ++relabels;
       since = 0;
     }
