@@ -196,7 +196,6 @@ def bench_serving(args):
     cds_server (continuous batching, HBM budget 200 GB) and through the
     plain one-graph-at-a-time path on the same stream."""
     import queue
-    import threading
     import numpy as np
     import cds_amd
 
@@ -212,19 +211,27 @@ def bench_serving(args):
         # served
         srv = cds_amd.Server(eng, hbm_budget_bytes=200 << 30)
         tickets = [None] * len(stream)
-        ready = threading.Event()
 
         def submit_served(i, g, k):
             tickets[i] = srv.submit(g, k)
-            if i == len(stream) - 1:
-                ready.set()
+
+        served = [None] * len(stream)
+
+        def progress(path, i):
+            if (i + 1) % 250 == 0:
+                print(f"[bench] load {rate}/s {path}: {i + 1}/{len(stream)} done",
+                      file=sys.stderr, flush=True)
 
         def wait_served(done_at, t0):
             for i in range(len(stream)):
                 while tickets[i] is None:
                     time.sleep(0.0005)
-                srv.wait(tickets[i])
+                try:
+                    served[i] = srv.wait(tickets[i])
+                except cds_amd.CdsError as ex:  # refused: the request's answer is its status
+                    served[i] = ex.status
                 done_at[i] = time.perf_counter() - t0
+                progress("served", i)
 
         lat_s, wall_s = run_stream(stream, submit_served, wait_served)
         stats = srv.stats()
@@ -235,14 +242,27 @@ def bench_serving(args):
         def submit_plain(i, g, k):
             q.put((i, g, k))
 
+        plain = [None] * len(stream)
+        n_done = [0]
+
         def wait_plain(done_at, t0):
             for _ in range(len(stream)):
                 i, g, k = q.get()
-                eng.solve(eng.upload(g), k)  # the device graph is freed here
+                try:
+                    plain[i] = eng.solve(eng.upload(g), k)  # device graph freed here
+                except cds_amd.CdsError as ex:
+                    plain[i] = ex.status
                 done_at[i] = time.perf_counter() - t0
+                progress("plain", n_done[0])
+                n_done[0] += 1
 
         lat_p, wall_p = run_stream(stream, submit_plain, wait_plain)
+        refused = sum(isinstance(x, int) for x in served)
+        mismatch = sum(1 for a, b in zip(served, plain)
+                       if isinstance(a, int) != isinstance(b, int)
+                       or (a != b if isinstance(a, int) else not same_answer(a, b)))
         row = {"offered_per_s": rate, "requests": len(stream),
+               "refused": refused, "mismatches_served_vs_plain": mismatch,
                "served_solves_per_s": len(stream) / wall_s,
                "served_p50_ms": percentile(lat_s, 50),
                "served_p99_ms": percentile(lat_s, 99),

@@ -126,6 +126,9 @@ _SIGNATURES = {
     "cds_graph_load_adjlist": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CGraph)]),
     "cds_graph_from_csr": (ctypes.c_int, [ctypes.c_uint32, u32p, u32p, ctypes.POINTER(CGraph)]),
     "cds_graph_write_adjlist": (ctypes.c_int, [ctypes.POINTER(CGraph), ctypes.c_char_p]),
+    "cds_result_json": (ctypes.c_size_t, [ctypes.POINTER(CResult), ctypes.c_char_p, ctypes.c_size_t]),
+    "cds_graph_save_csr": (ctypes.c_int, [ctypes.POINTER(CGraph), ctypes.c_char_p]),
+    "cds_graph_load_csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CGraph)]),
     "cds_graph_free": (None, [ctypes.POINTER(CGraph)]),
     "cds_engine_create": (ctypes.c_int, [ctypes.POINTER(COpts), ctypes.POINTER(ctypes.c_void_p)]),
     "cds_engine_destroy": (None, [ctypes.c_void_p]),
@@ -190,6 +193,13 @@ class Graph:
         return g
 
     @classmethod
+    def load_csr(cls, path):
+        """Graph from a binary CSR file written by save_csr."""
+        g = cls()
+        _check(lib().cds_graph_load_csr(os.fsencode(path), ctypes.byref(g._g)))
+        return g
+
+    @classmethod
     def from_csr(cls, offsets, neighbors):
         offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
         neighbors = np.ascontiguousarray(neighbors, dtype=np.uint32)
@@ -251,6 +261,9 @@ class Graph:
     def write(self, path):
         _check(lib().cds_graph_write_adjlist(ctypes.byref(self._g), os.fsencode(path)))
 
+    def save_csr(self, path):
+        _check(lib().cds_graph_save_csr(ctypes.byref(self._g), os.fsencode(path)))
+
     def __del__(self):
         if _lib is not None and self._g.offsets:
             _lib.cds_graph_free(ctypes.byref(self._g))
@@ -262,7 +275,15 @@ def _arr(ptr, count, dtype):
     return np.ctypeslib.as_array(ptr, (count,)).astype(dtype, copy=True)
 
 
-def result_to_dict(r):
+def result_json(r):
+    """cds_result_json of a CResult: the result record as a JSON string."""
+    need = lib().cds_result_json(ctypes.byref(r), None, 0)
+    buf = ctypes.create_string_buffer(need + 1)
+    lib().cds_result_json(ctypes.byref(r), buf, need + 1)
+    return buf.value.decode()
+
+
+def result_to_dict(r, with_json=False):
     d = {f: getattr(r, f) for f, _ in CResult._fields_
          if f not in ("clique_degree", "core", "vertices", "trace_lb", "trace_ub",
                       "trace_alpha", "trace_component", "trace_margin", "times",
@@ -279,6 +300,8 @@ def result_to_dict(r):
     d["times"] = {f: getattr(r.times, f) for f, _ in CTimes._fields_}
     d["peel_split_ms"] = dict(zip(("barrier", "scan", "process", "flush"),
                                   list(r.peel_split_ms)))
+    if with_json:
+        d["json"] = result_json(r)
     return d
 
 
@@ -296,13 +319,13 @@ class Engine:
 
     def solve(self, dgraph, k, p_size=8000, cp_size=16000, gl_buffer_size=60000,
               partition_size=30000, early_stop=-1, ub1=True, ub2=True,
-              vertex_arrays=False):
+              vertex_arrays=False, with_json=False):
         p = CParams(k, p_size, cp_size, gl_buffer_size, partition_size,
                     early_stop, int(ub1), int(ub2), int(vertex_arrays))
         r = CResult()
         _check(lib().cds_solve(self._h, dgraph._h, ctypes.byref(p), ctypes.byref(r)))
         try:
-            return result_to_dict(r)
+            return result_to_dict(r, with_json)
         finally:
             lib().cds_result_free(ctypes.byref(r))
 

@@ -12,16 +12,59 @@
 // reference's fixed partitions; this engine sizes its buffers from the
 // graph, so they are accepted and validated but change nothing.  earlyStop
 // must be -1 (exact search); other values are rejected.
+//
+// Environment (none changes the printed report):
+//   CDS_CSR_CACHE=<dir>  keep a binary CSR copy of each text graph in <dir>,
+//                        keyed by file name, size and modification time, and
+//                        load it instead of parsing the text again
+//   CDS_JSON=<path>      also write the result as one JSON object to <path>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/cds.h"
 
 static int die(const char *what) {
   fprintf(stderr, "cds_main: %s: %s\n", what, cds_last_error());
   return 2;
+}
+
+static cds_status load_graph(const char *path, cds_graph *g) {
+  const char *dir = getenv("CDS_CSR_CACHE");
+  struct stat st;
+  if (!dir || !*dir || stat(path, &st) != 0) return cds_graph_load_adjlist(path, g);
+  const char *base = strrchr(path, '/');
+  base = base ? base + 1 : path;
+  const std::string cache = std::string(dir) + "/" + base + "." +
+                            std::to_string((long long)st.st_size) + "." +
+                            std::to_string((long long)st.st_mtime) + ".csr";
+  if (cds_graph_load_csr(cache.c_str(), g) == CDS_OK) return CDS_OK;
+  const cds_status s = cds_graph_load_adjlist(path, g);
+  if (s == CDS_OK) {
+    // written under a temporary name and renamed, so a reader never sees a
+    // partial file; a cache that cannot be written is skipped silently
+    const std::string tmp = cache + ".tmp" + std::to_string((long long)getpid());
+    if (cds_graph_save_csr(g, tmp.c_str()) == CDS_OK)
+      rename(tmp.c_str(), cache.c_str());
+    else
+      unlink(tmp.c_str());
+  }
+  return s;
+}
+
+static void write_json(const cds_result *r) {
+  const char *path = getenv("CDS_JSON");
+  if (!path || !*path) return;
+  std::vector<char> buf(cds_result_json(r, nullptr, 0) + 1);
+  cds_result_json(r, buf.data(), buf.size());
+  FILE *f = fopen(path, "w");
+  if (!f || fprintf(f, "%s\n", buf.data()) < 0 || fclose(f) != 0)
+    fprintf(stderr, "cds_main: cannot write %s\n", path);
 }
 
 int main(int argc, const char *argv[]) {
@@ -46,7 +89,7 @@ int main(int argc, const char *argv[]) {
   }
   cds_graph g;
   memset(&g, 0, sizeof g);
-  if (cds_graph_load_adjlist(argv[1], &g) != CDS_OK) return die("load");
+  if (load_graph(argv[1], &g) != CDS_OK) return die("load");
   printf("n=%u, m=%llu\n", g.n,
          (unsigned long long)(g.m_header ? g.m_header : g.m));
   printf("Max Degree=%u\n", g.max_degree);
@@ -91,6 +134,7 @@ int main(int argc, const char *argv[]) {
   printf("\n");
   printf("Density_exact=%llu/%u\n", (unsigned long long)r.set_cliques,
          r.set_size);
+  write_json(&r);
   cds_result_free(&r);
   cds_dgraph_free(e, dg);
   cds_engine_destroy(e);

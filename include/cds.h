@@ -165,6 +165,15 @@ cds_status cds_graph_from_csr(uint32_t n, const uint32_t *offsets,
                               const uint32_t *neighbors, cds_graph *out);
 cds_status cds_graph_write_adjlist(const cds_graph *g, const char *path);
 void cds_graph_free(cds_graph *g);
+/* Binary CSR file (no counterpart in the reference, whose loader re-parses
+ * the text on every run): the cds_graph as stored, for reloading a large
+ * graph without parsing.  Layout, little-endian: magic "CDSCSR\0\1",
+ * u32 n, u32 max_degree, u64 m, u64 m_header, u32 offsets[n+1],
+ * u32 neighbors[2m].  Loading checks every invariant of cds_graph (monotone
+ * offsets, ids < n, strictly ascending lists, no self loops, symmetry) and
+ * fails with CDS_ERR_IO otherwise. */
+cds_status cds_graph_save_csr(const cds_graph *g, const char *path);
+cds_status cds_graph_load_csr(const char *path, cds_graph *out);
 
 /* ---- engine (replaces the global device structs of
  *      cudaCode/src/common.cpp:4-12 and gpuMemoryAllocation.cu) ---- */
@@ -183,6 +192,13 @@ void cds_dgraph_free(cds_engine *e, cds_dgraph *dg);
 cds_status cds_solve(cds_engine *e, const cds_dgraph *dg,
                      const cds_params *p, cds_result *out);
 void cds_result_free(cds_result *r);
+/* The result as one JSON object (answer with the exact fraction, core and
+ * search statistics, the density-search trace, stage and kernel times).
+ * The reference prints only stdout lines (cudaCode/main.cu:1069-1104,
+ * cpuCode/c++/src/helpers.cpp:1580-1588); this is their machine-readable
+ * form.  snprintf convention: writes at most cap-1 bytes and a NUL, returns
+ * the full length (call with cap 0 to size the buffer). */
+size_t cds_result_json(const cds_result *r, char *buf, size_t cap);
 
 /* Stage entry points (diagnostics and per-stage parity).
  * cds_count_cliques: T and per-vertex clique degrees
