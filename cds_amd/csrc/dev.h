@@ -67,6 +67,24 @@ __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
   return v;
 }
 
+// Last index i < count with key[i] <= x, for non-decreasing key[] with
+// key[0] <= x.  Wave-cooperative 64-ary search: each step every lane loads
+// one sample, so the dependent loads number ceil(log64(count)) instead of
+// log2(count).  Every lane of the wave calls it with the same arguments.
+__device__ __forceinline__ uint32_t wave_last_le(const uint64_t *key,
+                                                 uint32_t count, uint64_t x) {
+  uint32_t lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const uint32_t step = (hi - lo) / kWave + 1;  // ceil((hi-lo+1)/64)
+    const uint32_t i = lo + lane() * step;
+    const uint64_t b = __ballot(i <= hi && key[i] <= x);
+    const uint32_t j = 63 - __clzll(b);  // lane 0 samples lo: always set
+    lo += j * step;
+    hi = min(hi, lo + step - 1);
+  }
+  return lo;
+}
+
 // Index of `key` in the ascending array a[0..len), or -1.
 __device__ __forceinline__ int find_sorted(const uint32_t *a, int len,
                                            uint32_t key) {

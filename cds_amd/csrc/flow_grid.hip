@@ -108,17 +108,11 @@ __device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
 }
 
 // Maps work chunk ci of list b (count entries) to (vertex, incidence range).
+// Called by whole waves with a wave-uniform ci.
 __device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
                                          uint64_t ci, uint32_t *v, uint64_t *lo,
                                          uint64_t *hi) {
-  uint32_t a = 0, z = count - 1;
-  while (a < z) {
-    const uint32_t mid = (a + z + 1) >> 1;
-    if (glchunk(g, b)[mid] <= ci)
-      a = mid;
-    else
-      z = mid - 1;
-  }
+  const uint32_t a = wave_last_le(glchunk(g, b), count, ci);
   *v = glist(g, b)[a];
   *lo = g.ioff[*v] + (ci - glchunk(g, b)[a]) * kGridChunk;
   const uint64_t e = g.ioff[*v + 1];

@@ -317,14 +317,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
       uint32_t nk = 0;
       unsigned long long tp = leader ? wall_clock64() : 0;
       for (uint64_t ci = gwave; ci < nchunks; ci += gwaves) {
-        uint32_t lo = 0, hi = fcount - 1;  // last slot with FC <= ci
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi + 1) >> 1;
-          if (FCc[mid] <= ci)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
+        const uint32_t lo = wave_last_le(FCc, fcount, ci);  // ci's vertex slot
         const uint32_t v = Fc[lo];
         const uint64_t b = ioff[v] + (ci - FCc[lo]) * kPeelChunk;
         const uint64_t e = min(ioff[v + 1], b + kPeelChunk);
