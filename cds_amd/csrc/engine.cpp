@@ -236,6 +236,12 @@ bool flow_equal(__int128 f_scaled, float target, int sc) {
 
 // Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
 constexpr uint32_t kGridFlowRounds = 20000;
+constexpr uint32_t kFlowTraceRounds = 4096;
+
+static bool flow_trace_on() {
+  const char *v = getenv("CDS_FLOW_TRACE");
+  return v && *v && *v != '0';
+}
 
 struct Component {
   uint32_t V = 0;        // vertices
@@ -1293,7 +1299,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
   DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
   DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
-  DBuf<uint64_t> glc0(&A), glc1(&A);
+  DBuf<uint64_t> glc0(&A), glc1(&A), garc(&A);
+  DBuf<uint32_t> gastop(&A);
   DBuf<GridFlowCtl> gctl(&A);
   std::vector<int> reach_h;
   // Exact max flow of component ci with sink capacity capT (all
@@ -1335,6 +1342,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       HIPCHK(glc1.alloc(maxV));
       HIPCHK(gq0.alloc(maxT));
       HIPCHK(gq1.alloc(maxT));
+      HIPCHK(garc.alloc(maxV));
+      HIPCHK(gastop.alloc(maxV));
       HIPCHK(gctl.alloc(1));
     }
     GridFlowState G;
@@ -1354,6 +1363,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     G.cq1 = gq1.p;
     G.ctl = gctl.p;
     G.reach = freach.p;
+    G.arc = garc.p;
+    G.astop = gastop.p;
+    // CDS_FLOW_TRACE=1 prints one line per push-relabel round to stderr
+    DBuf<unsigned long long> ftrace(&A);
+    if (flow_trace_on()) {
+      HIPCHK(ftrace.alloc(4ull * kFlowTraceRounds));
+      G.trace = ftrace.p;
+      G.trace_cap = kFlowTraceRounds;
+    }
     {
       const unsigned fgrid = flow_grid_blocks(P, e->device, e->flow_blocks);
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
@@ -1368,6 +1386,17 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     sweeps += hc.rounds;
     R->flow_launches++;
     *F = (long long)hc.sink;
+    if (G.trace) {
+      const uint32_t nr = std::min(hc.rounds, kFlowTraceRounds);
+      std::vector<unsigned long long> tr(4ull * nr);
+      CHK(readback(e, tr.data(), G.trace, tr.size()));
+      fprintf(stderr, "[flow] component %u V=%u T=%u rounds=%u relabels=%u\n", ci,
+              P.V, P.T, hc.rounds, hc.relabels);
+      for (uint32_t r = 0; r < nr; ++r)
+        fprintf(stderr, "[flow] round %u cliques %llu vertices %llu chunks %llu active %llu%s\n",
+                r, tr[4 * r], tr[4 * r + 1], tr[4 * r + 2], tr[4 * r + 3] & ~(1ull << 63),
+                (tr[4 * r + 3] >> 63) ? " (after relabel)" : "");
+    }
     return CDS_OK;
   };
   {
@@ -1970,8 +1999,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     }
     DBuf<long long> fx(&A), frvs(&A), fexv(&A), frem(&A), fexc(&A);
     DBuf<int> fhv(&A), fhmin(&A), fhc(&A), freach(&A);
-    DBuf<uint32_t> fl0(&A), fl1(&A), fq0(&A), fq1(&A);
-    DBuf<uint64_t> flc0(&A), flc1(&A);
+    DBuf<uint32_t> fl0(&A), fl1(&A), fq0(&A), fq1(&A), fastop(&A);
+    DBuf<uint64_t> flc0(&A), flc1(&A), farc(&A);
     DBuf<GridFlowCtl> fctl(&A);
     DBuf<FlowBatchItem> fitems(&A);
     DBuf<unsigned long long> fcnt(&A);
@@ -1988,6 +2017,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(fl1.alloc(vo[np]));
     HIPCHK(flc0.alloc(vo[np]));
     HIPCHK(flc1.alloc(vo[np]));
+    HIPCHK(farc.alloc(vo[np]));
+    HIPCHK(fastop.alloc(vo[np]));
     HIPCHK(fq0.alloc(to[np]));
     HIPCHK(fq1.alloc(to[np]));
     HIPCHK(fctl.alloc(np));
@@ -2026,6 +2057,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       it.S.cq1 = fq1.p + to[i];
       it.S.ctl = fctl.p + i;
       it.S.reach = freach.p + vo[i];
+      it.S.arc = farc.p + vo[i];
+      it.S.astop = fastop.p + vo[i];
       it.capT = pend[i].rq.capT;
       it.U = pend[i].rq.U;
       // one workgroup per 64k arcs, at most a quarter of the CUs

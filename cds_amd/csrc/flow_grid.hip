@@ -53,6 +53,15 @@ namespace {
 constexpr uint32_t kGridChunk = 512;
 constexpr uint32_t kGridLdsVertices = 4096;  // LDS excess accumulator, 32 KB
 constexpr uint32_t kGridRelabel = 24;
+// Current-arc windows of the vertex discharge, in chunks.  After a global
+// relabel a vertex scans its whole list; afterwards the window starts at
+// its current arc and is sized from the chunks the last discharge used
+// (twice that, at least kArcWindow0), doubling while the excess outlasts
+// it; after a relabel it restarts at chunk 0 with kArcWindowRelabel.
+constexpr uint32_t kArcWindow0 = 4;
+constexpr uint32_t kArcWindowRelabel = 64;
+constexpr uint32_t kArcWindowMax = 1u << 24;  // more than any list has
+constexpr uint64_t kArcFull = (uint64_t)kArcWindowMax << 32;  // chunk 0, whole list
 constexpr uint64_t kMask40 = (1ull << 40) - 1;
 
 template <class T> __device__ __forceinline__ T ald(const T *p) {
@@ -75,6 +84,8 @@ struct G {
   uint32_t *list0, *list1;      // vertex work lists (active vertices / BFS)
   uint64_t *lchunk0, *lchunk1;  // their chunk starts
   uint32_t *cq0, *cq1;          // clique BFS frontiers
+  uint64_t *arc;                // per vertex: current chunk | window << 32
+  uint32_t *astop;              // per vertex: first unsaturated window chunk
   GridFlowCtl *ctl;
   uint32_t gtid, gthreads, gwave, gwaves;
   bool leader;
@@ -107,14 +118,46 @@ __device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
   glchunk(g, b)[old >> 40] = old & kMask40;
 }
 
-// Maps work chunk ci of list b (count entries) to (vertex, incidence range).
-// Called by whole waves with a wave-uniform ci.
+// Appends v with `chunks` work chunks to vertex list b.
+__device__ __forceinline__ void list_push_n(const G &g, int b, uint32_t v,
+                                            uint32_t chunks) {
+  const unsigned long long old =
+      atomicAdd(&g.ctl->lpack[b], (1ull << 40) + (unsigned long long)chunks);
+  glist(g, b)[old >> 40] = v;
+  glchunk(g, b)[old >> 40] = old & kMask40;
+}
+
+// First chunk and chunk count a vertex with excess scans this round: its
+// current-arc window, or the whole list once the current arc has passed
+// the end (a full scan, after which it relabels).
+__device__ __forceinline__ void discharge_range(const G &g, uint32_t v,
+                                                uint32_t *first, uint32_t *count) {
+  const uint32_t nch = (uint32_t)nchunks(g, v);
+  const uint64_t a = g.arc[v];
+  const uint32_t cur = (uint32_t)a, win = (uint32_t)(a >> 32);
+  if (cur >= nch) {
+    *first = 0;
+    *count = nch;
+  } else {
+    *first = cur;
+    *count = min(win, nch - cur);
+  }
+}
+
+// Maps work chunk ci of list b (count entries) to (vertex, incidence range);
+// `windowed`: the vertex's chunks start at its discharge range.  Called by
+// whole waves with a wave-uniform ci.
 __device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
                                          uint64_t ci, uint32_t *v, uint64_t *lo,
-                                         uint64_t *hi) {
+                                         uint64_t *hi, bool windowed = false,
+                                         uint32_t *vchunk = nullptr) {
   const uint32_t a = wave_last_le(glchunk(g, b), count, ci);
   *v = glist(g, b)[a];
-  *lo = g.ioff[*v] + (ci - glchunk(g, b)[a]) * kGridChunk;
+  uint32_t first = 0, n_;
+  if (windowed) discharge_range(g, *v, &first, &n_);
+  const uint32_t j = first + (uint32_t)(ci - glchunk(g, b)[a]);
+  if (vchunk) *vchunk = j;
+  *lo = g.ioff[*v] + (uint64_t)j * kGridChunk;
   const uint64_t e = g.ioff[*v + 1];
   *hi = *lo + kGridChunk < e ? *lo + kGridChunk : e;
 }
@@ -191,8 +234,10 @@ __device__ unsigned long long global_relabel(const G &g) {
   }
   gsync(g);
   unsigned long long act = 0;
-  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
     act += g.exv[v] > 0 && g.hv[v] < g.INF;
+    g.arc[v] = kArcFull;  // new heights: every arc may be admissible again
+  }
   for (uint32_t w0 = g.gtid - lane(); w0 < g.T; w0 += g.gthreads) {
     const uint32_t c = w0 + lane();
     const bool on = c < g.T && g.exc[c] > 0 && g.hc[c] < g.INF;
@@ -242,6 +287,8 @@ __device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
   g.lchunk1 = S.lchunk1;
   g.cq0 = S.cq0;
   g.cq1 = S.cq1;
+  g.arc = S.arc;
+  g.astop = S.astop;
   g.ctl = S.ctl;
 }
 
@@ -259,6 +306,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
     g.rvs[v] = (long long)P.cd[v] * U;
     g.exv[v] = cap_t;
+    g.arc[v] = kArcFull;
   }
   if (g.leader) {
     ctl->sink = 0;
@@ -367,7 +415,12 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       g.exv[v] = e;
       g.rem[v] = e;
       g.hmin[v] = g.INF;
-      if (e > 0) list_push(g, 0, v);
+      if (e > 0) {
+        uint32_t first, count;
+        discharge_range(g, v, &first, &count);
+        g.astop[v] = 0xFFFFFFFFu;
+        list_push_n(g, 0, v, count);
+      }
     }
     act = wave_sum64(act);
     if (lane() == 0 && act) atomicAdd(&ctl->active, act);
@@ -376,6 +429,13 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     const uint32_t nv = (uint32_t)(pack >> 40);
     const uint64_t nch = pack & kMask40;
     const unsigned long long any = ald(&ctl->active);
+    if (S.trace && g.leader && rounds < S.trace_cap) {
+      unsigned long long *t = S.trace + 4ull * rounds;
+      t[0] = nwl;
+      t[1] = nv;
+      t[2] = nch;
+      t[3] = any | (since == 0 ? 1ull << 63 : 0);  // top bit: after a global relabel
+    }
     // 3. chunked discharge of active vertices over their incidence lists:
     // pass 1 sums the chunk's admissible residual (and the lowest
     // non-admissible height), one atomic reserves that much of the vertex's
@@ -385,7 +445,8 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
       uint32_t v;
       uint64_t lo, hi;
-      chunk_of(g, 0, nv, ci, &v, &lo, &hi);
+      uint32_t vchunk;
+      chunk_of(g, 0, nv, ci, &v, &lo, &hi, true, &vchunk);
       const int hvv = g.hv[v];
       int hm = g.INF;
       long long radm = 0;
@@ -411,6 +472,8 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         avail = old < 0 ? 0 : (old < radm ? old : radm);
       }
       avail = __shfl(avail, 0, kWave);
+      // this chunk keeps admissible residual: the current arc stops here
+      if (lane() == 0 && avail < radm) atomicMin(&g.astop[v], vchunk);
       long long before = 0;
       for (uint64_t p0 = lo; p0 < hi && before < avail; p0 += kWave) {
         const uint64_t p = p0 + lane();
@@ -442,16 +505,43 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       }
     }
     gsync(g);
-    // 4. remainders become excess; relabel vertices that kept some
+    // 4. remainders become excess, and the current arc moves.  Arcs before
+    // the current arc are saturated or not admissible, and stay so until
+    // the vertex relabels: pushes into the vertex come from higher cliques
+    // and heights only grow.
+    //  - excess left after a scan of the whole list: relabel (hmin covers
+    //    every residual arc); the current arc restarts at chunk 0;
+    //  - excess left after a window: every admissible arc in it is
+    //    saturated, so the current arc moves past it and the window
+    //    doubles; past the end of the list, the next round scans the whole
+    //    list and relabels;
+    //  - excess used up: the current arc moves to the first chunk that kept
+    //    admissible residual, and the window becomes twice what was used.
     for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
       const uint32_t v = g.list0[i];
       long long r = g.rem[v];
       if (r < 0) r = 0;
       g.exv[v] = r;
-      if (r > 0) {
+      uint32_t first, count;
+      discharge_range(g, v, &first, &count);
+      const uint32_t nch = (uint32_t)nchunks(g, v);
+      const uint32_t win = (uint32_t)(g.arc[v] >> 32);
+      uint32_t cur, nwin;
+      if (r > 0 && first == 0 && count == nch) {
         const int hm = g.hmin[v];
         g.hv[v] = hm >= g.INF - 1 ? g.INF : hm + 1;
+        cur = 0;
+        nwin = kArcWindowRelabel;
+      } else if (r > 0) {
+        cur = first + count;  // == nch: the whole list next round
+        nwin = min(2 * win, kArcWindowMax);
+      } else {
+        const uint32_t stop = g.astop[v];
+        cur = stop < first + count ? stop : first + count;
+        const uint32_t used = cur - first + 1;
+        nwin = min(max(2 * used, kArcWindow0), kArcWindowMax);
       }
+      g.arc[v] = (uint64_t)cur | (uint64_t)nwin << 32;
     }
     gsync(g);
     ++rounds;

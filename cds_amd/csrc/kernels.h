@@ -302,6 +302,13 @@ struct GridFlowState {
   uint32_t *cq0, *cq1;     // T each
   GridFlowCtl *ctl;
   int *reach;              // V
+  uint64_t *arc;           // V: current-arc window (first chunk | chunks << 32)
+  uint32_t *astop;         // V: first chunk of the window left unsaturated
+  // Optional per-round record (nullptr: none): 4 words per round, the
+  // clique work-list size, active vertices, their chunks and the round's
+  // active-node count; trace_cap rounds at most.
+  unsigned long long *trace = nullptr;
+  uint32_t trace_cap = 0;
 };
 // Many independent max flows in one launch (same answers as
 // grid_max_flow): item i runs on workgroups [first_block, first_block +
