@@ -62,6 +62,65 @@ constexpr uint32_t kArcWindow0 = 4;
 constexpr uint32_t kArcWindowRelabel = 64;
 constexpr uint32_t kArcWindowMax = 1u << 24;  // more than any list has
 constexpr uint64_t kArcFull = (uint64_t)kArcWindowMax << 32;  // chunk 0, whole list
+constexpr int kChunkSteps = kGridChunk / kWave;  // wave passes per chunk
+constexpr int kMaxFlowWaves = 16;                // 1024-thread workgroups
+
+// List appends are aggregated so a counter that millions of nodes append
+// to sees few atomics.  wave_append: the items one wave gathered over up to
+// N passes (one per lane and pass), one atomic.  All lanes call it.
+template <int N>
+__device__ __forceinline__ void wave_append(uint32_t *n, uint32_t *out,
+                                            const bool (&f)[N],
+                                            const uint32_t (&v)[N]) {
+  uint32_t mine[N], total = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint64_t b = __ballot(f[i]);
+    mine[i] = total + __popcll(b & lanes_below());
+    total += __popcll(b);
+  }
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane() == 0) base = atomicAdd(n, total);
+  base = __shfl(base, 0, kWave);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (f[i]) out[base + mine[i]] = v[i];
+}
+
+// block_append: the same over a whole workgroup, one atomic per call;
+// app = kMaxFlowWaves + 1 words of LDS.  Every thread of the workgroup
+// calls it (it synchronises the workgroup).
+template <int N>
+__device__ __forceinline__ void block_append(uint32_t *app, uint32_t *n,
+                                             uint32_t *out, const bool (&f)[N],
+                                             const uint32_t (&v)[N]) {
+  uint32_t mine[N], total = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint64_t b = __ballot(f[i]);
+    mine[i] = total + __popcll(b & lanes_below());
+    total += __popcll(b);
+  }
+  const uint32_t w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  if (lane() == 0) app[w] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < nw; ++i) {
+      const uint32_t t = app[i];
+      app[i] = run;
+      run += t;
+    }
+    app[kMaxFlowWaves] = run ? atomicAdd(n, run) : 0;
+  }
+  __syncthreads();
+  const uint32_t base = app[kMaxFlowWaves] + app[w];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (f[i]) out[base + mine[i]] = v[i];
+  __syncthreads();  // app is reused by the next call
+}
 constexpr uint64_t kMask40 = (1ull << 40) - 1;
 
 template <class T> __device__ __forceinline__ T ald(const T *p) {
@@ -86,6 +145,7 @@ struct G {
   uint32_t *cq0, *cq1;          // clique BFS frontiers
   uint64_t *arc;                // per vertex: current chunk | window << 32
   uint32_t *astop;              // per vertex: first unsaturated window chunk
+  uint32_t *app;                // LDS: block_append's wave offsets
   GridFlowCtl *ctl;
   uint32_t gtid, gthreads, gwave, gwaves;
   bool leader;
@@ -199,14 +259,23 @@ __device__ unsigned long long global_relabel(const G &g) {
       uint32_t v;
       uint64_t lo, hi;
       chunk_of(g, vb, nv, ci, &v, &lo, &hi);
-      for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+      bool f[kChunkSteps];
+      uint32_t cs[kChunkSteps];
+#pragma unroll
+      for (int it = 0; it < kChunkSteps; ++it) {
+        const uint64_t p = lo + (uint64_t)it * kWave + lane();
+        f[it] = false;
+        cs[it] = 0;
+        if (p >= hi) continue;
         const uint64_t s = g.inc[p] - g.sbase;
         if (g.U + g.x[s] <= 0) continue;
         const uint32_t c = (uint32_t)(s / g.k);
-        if (ald(&g.hc[c]) != g.INF) continue;
-        if (atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF)
-          gcq(g, cb)[atomicAdd(&ctl->cqn[cb], 1u)] = c;
+        // a plain load filters: a stale INF only costs a failed CAS
+        if (g.hc[c] != g.INF) continue;
+        f[it] = atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF;
+        cs[it] = c;
       }
+      wave_append(&ctl->cqn[cb], gcq(g, cb), f, cs);
     }
     gsync(g);
     // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x)
@@ -217,7 +286,7 @@ __device__ unsigned long long global_relabel(const G &g) {
       for (uint32_t j = 0; j < g.k; ++j) {
         if ((long long)(g.k - 1) * g.U - g.x[base + j] <= 0) continue;
         const uint32_t v = g.mem[base + j] - g.vbase;
-        if (ald(&g.hv[v]) != g.INF) continue;
+        if (g.hv[v] != g.INF) continue;  // plain filter, as above
         if (atomicCAS(&g.hv[v], g.INF, d + 2) == g.INF) list_push(g, 1 - vb, v);
       }
     }
@@ -238,12 +307,19 @@ __device__ unsigned long long global_relabel(const G &g) {
     act += g.exv[v] > 0 && g.hv[v] < g.INF;
     g.arc[v] = kArcFull;  // new heights: every arc may be admissible again
   }
-  for (uint32_t w0 = g.gtid - lane(); w0 < g.T; w0 += g.gthreads) {
-    const uint32_t c = w0 + lane();
-    const bool on = c < g.T && g.exc[c] > 0 && g.hc[c] < g.INF;
-    act += on;
-    const uint32_t slot = aggregated_reserve(&ctl->cqn[0], 0, on);
-    if (on) g.cq0[slot] = c;
+  constexpr int kScan = 8;  // cliques per thread and block_append
+  for (uint64_t t0 = (uint64_t)g.bid * blockDim.x * kScan; t0 < g.T;
+       t0 += (uint64_t)g.nblocks * blockDim.x * kScan) {
+    bool on[kScan];
+    uint32_t cs[kScan];
+#pragma unroll
+    for (int it = 0; it < kScan; ++it) {
+      const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+      on[it] = c < g.T && g.exc[c] > 0 && g.hc[c] < g.INF;
+      cs[it] = (uint32_t)c;
+      act += on[it];
+    }
+    block_append(g.app, &ctl->cqn[0], g.cq0, on, cs);
   }
   act = wave_sum64(act);
   if (lane() == 0 && act) atomicAdd(&ctl->active, act);
@@ -325,6 +401,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       limit = true;
       break;
     }
+    const unsigned long long tr0 = (S.trace && g.leader) ? wall_clock64() : 0;
     if (g.leader) {
       ctl->lpack[0] = 0;
       ctl->active = 0;
@@ -344,9 +421,15 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     const uint32_t nwl = ald(&ctl->cqn[wb]);
     const uint32_t *wl = gcq(g, wb);
     uint32_t *nx = gcq(g, 1 - wb);
-    for (uint32_t w0 = g.gtid - lane(); w0 < nwl; w0 += g.gthreads) {
-      const uint32_t i = w0 + lane();
-      bool keep = false;  // still holds excess: onto the next list
+    constexpr int kTile = 4;  // list entries per thread and block_append
+    for (uint64_t t0 = (uint64_t)g.bid * blockDim.x * kTile; t0 < nwl;
+         t0 += (uint64_t)g.nblocks * blockDim.x * kTile) {
+     bool keeps[kTile];  // still holds excess: onto the next list
+     uint32_t cs[kTile];
+#pragma unroll
+     for (int it = 0; it < kTile; ++it) {
+      const uint64_t i = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+      bool keep = false;
       const uint32_t c = i < nwl ? wl[i] : 0;
       long long e = i < nwl ? g.exc[c] : 0;
       const int hcc = i < nwl ? g.hc[c] : g.INF;
@@ -358,7 +441,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
           const long long res = U + g.x[base + j];
           if (res <= 0) continue;
           const uint32_t v = g.mem[base + j] - g.vbase;
-          const int hvv = ald(&g.hv[v]);
+          const int hvv = g.hv[v];  // heights change only in phase 4
           if (hvv < hcc) {
             const long long d = e < res ? e : res;
             g.x[base + j] -= d;
@@ -378,13 +461,15 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
           hmin = g.INF;
           for (uint32_t j = 0; j < g.k; ++j)
             if (U + g.x[base + j] > 0)
-              hmin = min(hmin, ald(&g.hv[g.mem[base + j] - g.vbase]));
+              hmin = min(hmin, g.hv[g.mem[base + j] - g.vbase]);
           g.hc[c] = hmin >= g.INF - 1 ? g.INF : hmin + 1;
           keep = g.hc[c] < g.INF;
         }
       }
-      const uint32_t slot = aggregated_reserve(&ctl->cqn[1 - wb], 0, keep);
-      if (keep) nx[slot] = c;
+      keeps[it] = keep;
+      cs[it] = c;
+     }
+     block_append(g.app, &ctl->cqn[1 - wb], nx, keeps, cs);
     }
     if (lds_v) {
       __syncthreads();
@@ -394,6 +479,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       }
     }
     gsync(g);
+    const unsigned long long tr1 = (S.trace && g.leader) ? wall_clock64() : 0;
     // 2. vertices push to the sink; active vertices post their chunks.
     // Every workgroup has read this round's clique list count: it is
     // cleared for the round after next.
@@ -429,13 +515,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     const uint32_t nv = (uint32_t)(pack >> 40);
     const uint64_t nch = pack & kMask40;
     const unsigned long long any = ald(&ctl->active);
-    if (S.trace && g.leader && rounds < S.trace_cap) {
-      unsigned long long *t = S.trace + 4ull * rounds;
-      t[0] = nwl;
-      t[1] = nv;
-      t[2] = nch;
-      t[3] = any | (since == 0 ? 1ull << 63 : 0);  // top bit: after a global relabel
-    }
+
     // 3. chunked discharge of active vertices over their incidence lists:
     // pass 1 sums the chunk's admissible residual (and the lowest
     // non-admissible height), one atomic reserves that much of the vertex's
@@ -475,7 +555,14 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       // this chunk keeps admissible residual: the current arc stops here
       if (lane() == 0 && avail < radm) atomicMin(&g.astop[v], vchunk);
       long long before = 0;
-      for (uint64_t p0 = lo; p0 < hi && before < avail; p0 += kWave) {
+      bool fresh[kChunkSteps];  // the clique's excess goes from zero to positive
+      uint32_t fc[kChunkSteps];
+#pragma unroll
+      for (int it = 0; it < kChunkSteps; ++it) {
+        fresh[it] = false;
+        fc[it] = 0;
+        const uint64_t p0 = lo + (uint64_t)it * kWave;
+        if (p0 >= hi || before >= avail) continue;  // wave-uniform
         const uint64_t p = p0 + lane();
         long long ra = 0;
         uint64_t s = 0;
@@ -493,18 +580,18 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         const long long start = before + incl - ra;
         long long take = avail - start;
         take = take < 0 ? 0 : (take < ra ? take : ra);
-        bool fresh = false;  // the clique's excess goes from zero to positive
         if (take > 0) {
           g.x[s] += take;
-          fresh = atomicAdd((unsigned long long *)&g.exc[s / g.k],
-                            (unsigned long long)take) == 0;
+          fc[it] = (uint32_t)(s / g.k);
+          fresh[it] = atomicAdd((unsigned long long *)&g.exc[fc[it]],
+                                (unsigned long long)take) == 0;
         }
-        const uint32_t slot = aggregated_reserve(&ctl->cqn[1 - wb], 0, fresh);
-        if (fresh) nx[slot] = (uint32_t)(s / g.k);
         before += __shfl(incl, kWave - 1, kWave);
       }
+      wave_append(&ctl->cqn[1 - wb], nx, fresh, fc);
     }
     gsync(g);
+    const unsigned long long tr2 = (S.trace && g.leader) ? wall_clock64() : 0;
     // 4. remainders become excess, and the current arc moves.  Arcs before
     // the current arc are saturated or not admissible, and stay so until
     // the vertex relabels: pushes into the vertex come from higher cliques
@@ -544,6 +631,8 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       g.arc[v] = (uint64_t)cur | (uint64_t)nwin << 32;
     }
     gsync(g);
+    const unsigned long long tr3 = (S.trace && g.leader) ? wall_clock64() : 0;
+    const bool after_relabel = since == 0;
     ++rounds;
     ++since;
     wb = 1 - wb;
@@ -552,6 +641,17 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       wb = 0;
       ++relabels;
       since = 0;
+    }
+    if (S.trace && g.leader && rounds - 1 < S.trace_cap) {
+      unsigned long long *t = S.trace + 8ull * (rounds - 1);
+      t[0] = nwl;
+      t[1] = nv;
+      t[2] = nch;
+      t[3] = any | (after_relabel ? 1ull << 63 : 0);  // top bit: first after a relabel
+      t[4] = tr1 - tr0;  // phase 1 (wall_clock64 ticks)
+      t[5] = tr2 - tr1;  // phases 2 and 3
+      t[6] = tr3 - tr2;  // phase 4
+      t[7] = wall_clock64() - tr3;  // global relabel, if one ran
     }
   }
   if (!limit) global_relabel(g);
@@ -579,6 +679,8 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   g.leader = g.gtid == 0;
   g.nblocks = gridDim.x;
   g.bid = blockIdx.x;
+  __shared__ uint32_t app[kMaxFlowWaves + 1];
+  g.app = app;
   extern __shared__ long long lds_ex[];
   flow_run(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
 }
@@ -609,6 +711,8 @@ __global__ __launch_bounds__(kFlowBatchThreads) void k_flow_batch(
   g.gwave = g.gtid / kWave;
   g.gwaves = g.gthreads / kWave;
   g.leader = g.gtid == 0;
+  __shared__ uint32_t app[kMaxFlowWaves + 1];
+  g.app = app;
   extern __shared__ long long lds_ex[];
   flow_run(g, it.P, it.S, it.capT, it.U, max_rounds, lds_ex, g.V <= lds_cap);
 }
