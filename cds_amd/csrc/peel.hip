@@ -232,6 +232,15 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
   // with one atomic per vertex and sub-round; the vertex whose aggregated
   // decrement crosses level+1 -> level joins the next sub-frontier.
   __shared__ uint32_t lds_cnt[kPeelLdsCap];
+  // Per-wave partials for workgroup-level aggregation: counters every
+  // workgroup updates (level minimum, killed cliques, frontier and alive
+  // list reservations) see one atomic per workgroup instead of per wave.
+  constexpr int kPeelWaves = kPeelThreads / kWave;
+  __shared__ uint32_t sh_min[kPeelWaves], sh_an[kPeelWaves], sh_fn[kPeelWaves];
+  __shared__ unsigned long long sh_kill[kPeelWaves], sh_fc[kPeelWaves];
+  __shared__ unsigned long long sh_fbase;
+  __shared__ uint32_t sh_abase;
+  const uint32_t wid = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
   for (uint32_t i = threadIdx.x; i < kPeelLdsCap; i += blockDim.x) lds_cnt[i] = 0;
   __syncthreads();
   uint32_t *A[2] = {A0, A1};
@@ -260,7 +269,13 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off, kWave));
-    if (lane() == 0 && m != kAlive) atomicMin(&ctl->gmin[gp], m);
+    if (lane() == 0) sh_min[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t bm = kAlive;
+      for (uint32_t w = 0; w < nwaves; ++w) bm = min(bm, sh_min[w]);
+      if (bm != kAlive) atomicMin(&ctl->gmin[gp], bm);
+    }
     if (leader) {
       ctl->alive_n[1 - ab] = 0;
       ctl->t_scan += wall_clock64() - ts;
@@ -282,19 +297,55 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
         alive = ld(&core[v]) == kAlive;
         take = alive && ld(&deg[v]) <= level;
       }
+      const bool keep = alive && !take;
+      // inclusive wave scans: frontier count and chunks, alive count
+      unsigned long long fch = take ? chunks_of(ioff[v + 1] - ioff[v]) : 0;
+      uint32_t fcnt = take ? 1u : 0u, acnt = keep ? 1u : 0u;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned long long t = __shfl_up(fch, off, kWave);
+        const uint32_t tf = __shfl_up(fcnt, off, kWave);
+        const uint32_t ta = __shfl_up(acnt, off, kWave);
+        if ((int)lane() >= off) {
+          fch += t;
+          fcnt += tf;
+          acnt += ta;
+        }
+      }
+      if (lane() == kWave - 1) {
+        sh_fc[wid] = fch;
+        sh_fn[wid] = fcnt;
+        sh_an[wid] = acnt;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long rc = 0;
+        uint32_t rf = 0, ra = 0;
+        for (uint32_t w = 0; w < nwaves; ++w) {
+          const unsigned long long c = sh_fc[w];
+          const uint32_t f = sh_fn[w], a = sh_an[w];
+          sh_fc[w] = rc;
+          sh_fn[w] = rf;
+          sh_an[w] = ra;
+          rc += c;
+          rf += f;
+          ra += a;
+        }
+        sh_fbase = rf ? atomicAdd(&ctl->fpack[fb], ((unsigned long long)rf << 40) + rc) : 0;
+        sh_abase = ra ? atomicAdd(&ctl->alive_n[1 - ab], ra) : 0;
+      }
+      __syncthreads();
       if (take) {
         core[v] = level;
-        frontier_push(ctl, fb, F[fb], FC[fb], v, chunks_of(ioff[v + 1] - ioff[v]));
+        const uint32_t slot = (uint32_t)(sh_fbase >> 40) + sh_fn[wid] + fcnt - 1;
+        F[fb][slot] = v;
+        FC[fb][slot] = (sh_fbase & kChunkMask) + sh_fc[wid] + fch -
+                       chunks_of(ioff[v + 1] - ioff[v]);
       }
-      const bool keep = alive && !take;
-      unsigned cnt;
-      const unsigned slot = wave_slot(keep, &cnt);
-      uint32_t start = 0;
-      if (lane() == 0 && cnt) start = atomicAdd(&ctl->alive_n[1 - ab], cnt);
-      start = __shfl(start, 0, kWave);
       if (keep) {
-        A[1 - ab][start + slot] = v;
-        pos[v] = start + slot;
+        const uint32_t slot = sh_abase + sh_an[wid] + acnt - 1;
+        A[1 - ab][slot] = v;
+        pos[v] = slot;
       }
     }
     if (leader) ctl->t_scan += wall_clock64() - ts;
@@ -366,7 +417,13 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
         if (leader) ctl->t_flush += wall_clock64() - tp;
       }
       nk = wave_sum(nk);
-      if (lane() == 0 && nk) atomicAdd(&lv_killed[lvl_idx], (unsigned long long)nk);
+      if (lane() == 0) sh_kill[wid] = nk;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long bk = 0;
+        for (uint32_t w = 0; w < nwaves; ++w) bk += sh_kill[w];
+        if (bk) atomicAdd(&lv_killed[lvl_idx], bk);
+      }
       grid_sync(ctl);
       fb = nb;
       if (leader) ctl->rounds++;
