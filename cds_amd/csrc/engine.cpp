@@ -1207,7 +1207,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(ccount.alloc(ncomp));
   HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_n));
-  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
+  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, ncomp,
                             ccount.p, st));
   // Work items over the core vertices' incidence lists, cut into chunks so
   // hub lists spread over many waves.
@@ -1272,7 +1272,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     DBuf<uint32_t> cslot(&A);  // flow index of each core clique
     HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
-                              d_fv_of.p, coff.p, ccur.p, fmem.p, cslot.p, st));
+                              d_fv_of.p, ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
     HIPCHK(fioff.alloc((size_t)core_n + 1));
     CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
     HIPCHK(finc.alloc(core_T * k));
@@ -1368,7 +1368,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     // CDS_FLOW_TRACE=1 prints one line per push-relabel round to stderr
     DBuf<unsigned long long> ftrace(&A);
     if (flow_trace_on()) {
-      HIPCHK(ftrace.alloc(4ull * kFlowTraceRounds));
+      HIPCHK(ftrace.alloc(8ull * kFlowTraceRounds));
       G.trace = ftrace.p;
       G.trace_cap = kFlowTraceRounds;
     }
@@ -1388,14 +1388,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     *F = (long long)hc.sink;
     if (G.trace) {
       const uint32_t nr = std::min(hc.rounds, kFlowTraceRounds);
-      std::vector<unsigned long long> tr(4ull * nr);
+      std::vector<unsigned long long> tr(8ull * nr);
       CHK(readback(e, tr.data(), G.trace, tr.size()));
       fprintf(stderr, "[flow] component %u V=%u T=%u rounds=%u relabels=%u\n", ci,
               P.V, P.T, hc.rounds, hc.relabels);
-      for (uint32_t r = 0; r < nr; ++r)
-        fprintf(stderr, "[flow] round %u cliques %llu vertices %llu chunks %llu active %llu%s\n",
-                r, tr[4 * r], tr[4 * r + 1], tr[4 * r + 2], tr[4 * r + 3] & ~(1ull << 63),
-                (tr[4 * r + 3] >> 63) ? " (after relabel)" : "");
+      int khz = 100000;
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device);
+      const double us = 1e3 / (khz > 0 ? khz : 100000);  // microseconds per tick
+      for (uint32_t r = 0; r < nr; ++r) {
+        const unsigned long long *t = &tr[8ull * r];
+        fprintf(stderr,
+                "[flow] round %u cliques %llu vertices %llu chunks %llu active %llu "
+                "us %.1f %.1f %.1f %.1f%s\n",
+                r, t[0], t[1], t[2], t[3] & ~(1ull << 63), t[4] * us, t[5] * us,
+                t[6] * us, t[7] * us, (t[3] >> 63) ? " (after relabel)" : "");
+      }
     }
     return CDS_OK;
   };
@@ -1871,7 +1878,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(ccount.alloc(ncomp));
   HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_total));
-  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, ccount.p, st));
+  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, ncomp, ccount.p,
+                            st));
   constexpr uint32_t kItem = 4096;
   std::vector<uint4> items;
   std::vector<uint32_t> item_fv;
@@ -1933,7 +1941,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     DBuf<uint32_t> cslot(&A);
     HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, d_fv_of.p,
-                              coff.p, ccur.p, fmem.p, cslot.p, st));
+                              ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
     HIPCHK(fioff.alloc((size_t)core_total + 1));
     CHK(upload(e, fioff.p, hfioff.data(), core_total + 1));
     HIPCHK(finc.alloc(core_T_total * k));

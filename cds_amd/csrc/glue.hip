@@ -139,41 +139,101 @@ __global__ void k_count_nonzero_u8(size_t n, const uint8_t *__restrict__ a,
 }
 
 // Cliques of each component (wave-aggregated counters).
-__global__ void k_comp_count2(uint64_t T, const uint32_t *__restrict__ cl,
-                              uint32_t k, const uint32_t *__restrict__ kill,
-                              uint32_t kp, const uint32_t *__restrict__ comp_of,
-                              uint32_t *__restrict__ ccount) {
+// With at most kCompLds components the per-component counts of a
+// workgroup are summed in LDS, so a component holding most core cliques
+// sees one global atomic per workgroup (count) or per tile (slots).
+constexpr uint32_t kCompLds = 1024;
+constexpr int kCompThreads = 256;
+constexpr int kCompTile = 8;  // cliques per thread and tile
+
+__global__ __launch_bounds__(kCompThreads) void k_comp_count2(
+    uint64_t T, const uint32_t *__restrict__ cl, uint32_t k,
+    const uint32_t *__restrict__ kill, uint32_t kp,
+    const uint32_t *__restrict__ comp_of, uint32_t ncomp,
+    uint32_t *__restrict__ ccount) {
+  __shared__ uint32_t cnt[kCompLds];
+  const bool lds = ncomp <= kCompLds;
+  if (lds) {
+    for (uint32_t i = threadIdx.x; i < ncomp; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+  }
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
        base += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t c = base + threadIdx.x;
     const bool in = c < T && kill[c] >= kp;
     const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
-    (void)aggregated_reserve(ccount, comp, in);
+    (void)aggregated_reserve(lds ? cnt : ccount, comp, in);
+  }
+  if (lds) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ncomp; i += blockDim.x)
+      if (cnt[i]) atomicAdd(&ccount[i], cnt[i]);
   }
 }
 
 // Places every core clique in its component's range of the flow arrays:
 // fmem (flow-vertex ids of the members) and cslot[c] = its flow index.
-__global__ void k_comp_scatter2(uint64_t T, uint32_t k,
-                                const uint32_t *__restrict__ cl,
-                                const uint32_t *__restrict__ kill, uint32_t kp,
-                                const uint32_t *__restrict__ comp_of,
-                                const uint32_t *__restrict__ fv_of,
-                                const uint64_t *__restrict__ coff,
-                                uint32_t *__restrict__ ccur,
-                                uint32_t *__restrict__ fmem,
-                                uint32_t *__restrict__ cslot) {
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t c = base + threadIdx.x;
-    const bool in = c < T && kill[c] >= kp;
-    const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
-    const uint32_t slot = aggregated_reserve(ccur, comp, in);
-    if (in) {
-      const uint64_t f = coff[comp] + slot;
-      cslot[c] = (uint32_t)f;
-      for (uint32_t j = 0; j < k; ++j) fmem[f * k + j] = fv_of[cl[c * k + j]];
+__device__ __forceinline__ void comp_place(uint64_t c, uint64_t f, uint32_t k,
+                                           const uint32_t *__restrict__ cl,
+                                           const uint32_t *__restrict__ fv_of,
+                                           uint32_t *__restrict__ fmem,
+                                           uint32_t *__restrict__ cslot) {
+  cslot[c] = (uint32_t)f;
+  for (uint32_t j = 0; j < k; ++j) fmem[f * k + j] = fv_of[cl[c * k + j]];
+}
+
+__global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
+    uint64_t T, uint32_t k, const uint32_t *__restrict__ cl,
+    const uint32_t *__restrict__ kill, uint32_t kp,
+    const uint32_t *__restrict__ comp_of, const uint32_t *__restrict__ fv_of,
+    uint32_t ncomp, const uint64_t *__restrict__ coff,
+    uint32_t *__restrict__ ccur, uint32_t *__restrict__ fmem,
+    uint32_t *__restrict__ cslot) {
+  __shared__ uint32_t cnt[kCompLds], cbase[kCompLds];
+  if (ncomp > kCompLds) {
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+         base += (uint64_t)gridDim.x * blockDim.x) {
+      const uint64_t c = base + threadIdx.x;
+      const bool in = c < T && kill[c] >= kp;
+      const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
+      const uint32_t slot = aggregated_reserve(ccur, comp, in);
+      if (in) comp_place(c, coff[comp] + slot, k, cl, fv_of, fmem, cslot);
     }
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < ncomp; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  // tiles of blockDim * kCompTile cliques: slots within the tile from LDS
+  // counters, then one global reservation per component present
+  const uint64_t tile = (uint64_t)blockDim.x * kCompTile;
+  for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < T;
+       t0 += (uint64_t)gridDim.x * tile) {
+    uint32_t comp[kCompTile], local[kCompTile];
+    bool in[kCompTile];
+#pragma unroll
+    for (int it = 0; it < kCompTile; ++it) {
+      const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+      in[it] = c < T && kill[c] >= kp;
+      comp[it] = in[it] ? comp_of[cl[c * k]] : 0;
+      local[it] = aggregated_reserve(cnt, comp[it], in[it]);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ncomp; i += blockDim.x) {
+      const uint32_t n = cnt[i];
+      if (n) {
+        cbase[i] = atomicAdd(&ccur[i], n);
+        cnt[i] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kCompTile; ++it) {
+      if (!in[it]) continue;
+      const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+      comp_place(c, coff[comp[it]] + cbase[comp[it]] + local[it], k, cl, fv_of,
+                 fmem, cslot);
+    }
+    __syncthreads();  // cbase is rewritten by the next tile
   }
 }
 
@@ -294,20 +354,25 @@ hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
 hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, const uint32_t *fv_of,
-                              const uint64_t *coff, uint32_t *ccur,
-                              uint32_t *fmem, uint32_t *cslot, hipStream_t s) {
-  hipLaunchKernelGGL(k_comp_scatter2, dim3(grid_for(T, 256, 65535)), dim3(256),
-                     0, s, T, k, cliques, kill, kp, comp_of, fv_of, coff, ccur,
-                     fmem, cslot);
+                              uint32_t ncomp, const uint64_t *coff,
+                              uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
+                              hipStream_t s) {
+  const unsigned grid = ncomp > kCompLds
+                            ? grid_for(T, kCompThreads, 65535)
+                            : grid_for(T, kCompThreads * kCompTile, 4096);
+  hipLaunchKernelGGL(k_comp_scatter2, dim3(grid), dim3(kCompThreads), 0, s, T, k,
+                     cliques, kill, kp, comp_of, fv_of, ncomp, coff, ccur, fmem,
+                     cslot);
   return hipGetLastError();
 }
 
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
-                              const uint32_t *comp_of, uint32_t *ccount,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_comp_count2, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
-                     s, T, cliques, k, kill, kp, comp_of, ccount);
+                              const uint32_t *comp_of, uint32_t ncomp,
+                              uint32_t *ccount, hipStream_t s) {
+  const unsigned grid = grid_for(T, kCompThreads, ncomp > kCompLds ? 65535 : 4096);
+  hipLaunchKernelGGL(k_comp_count2, dim3(grid), dim3(kCompThreads), 0, s, T,
+                     cliques, k, kill, kp, comp_of, ncomp, ccount);
   return hipGetLastError();
 }
 

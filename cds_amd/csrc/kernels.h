@@ -199,18 +199,20 @@ hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *adj, uint8_t *mark, hipStream_t s);
 hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
                             unsigned long long *out, hipStream_t s);
-// Per-component clique buckets (wave-aggregated reservations):
+// Per-component clique buckets of ncomp components (reservations summed
+// per workgroup in LDS when ncomp is small, else wave-aggregated):
 //  comp_clique_totals  ccount[comp] += 1 per core clique
 //  comp_cliques_build  fmem / cslot (flow index of each core clique)
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
-                              const uint32_t *comp_of, uint32_t *ccount,
-                              hipStream_t s);
+                              const uint32_t *comp_of, uint32_t ncomp,
+                              uint32_t *ccount, hipStream_t s);
 hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, const uint32_t *fv_of,
-                              const uint64_t *coff, uint32_t *ccur,
-                              uint32_t *fmem, uint32_t *cslot, hipStream_t s);
+                              uint32_t ncomp, const uint64_t *coff,
+                              uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
+                              hipStream_t s);
 // Chunked per-vertex incidence work.  items[i] = {vertex, first entry
 // (hi 32 bits), first entry (lo 32 bits), entry count}; incidence entries
 // are ik words wide with the clique id first.
@@ -304,9 +306,10 @@ struct GridFlowState {
   int *reach;              // V
   uint64_t *arc;           // V: current-arc window (first chunk | chunks << 32)
   uint32_t *astop;         // V: first chunk of the window left unsaturated
-  // Optional per-round record (nullptr: none): 4 words per round, the
-  // clique work-list size, active vertices, their chunks and the round's
-  // active-node count; trace_cap rounds at most.
+  // Optional per-round record (nullptr: none): 8 words per round, the
+  // clique work-list size, active vertices, their chunks, the round's
+  // active-node count, and the wall_clock64 ticks of phase 1, phases 2-3,
+  // phase 4 and the global relabel after it; trace_cap rounds at most.
   unsigned long long *trace = nullptr;
   uint32_t trace_cap = 0;
 };
