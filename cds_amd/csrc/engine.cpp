@@ -42,6 +42,11 @@ struct cds_engine {
   // workgroups of every max-flow launch; 0 = sized from the network
   // (CDS_FLOW_BLOCKS overrides, e.g. to exercise several on small graphs)
   unsigned flow_blocks = 0;
+  // Peel in coarse steps of [m, m + m >> peel_shift] and re-peel exactly
+  // only the steps the answer depends on: -1 = when k*T >= kCoarseMinSlots,
+  // 0 = never, 1 = always (CDS_PEEL_COARSE, CDS_PEEL_COARSE_SHIFT).
+  int peel_coarse = -1;
+  uint32_t peel_shift = 3;
 };
 
 struct cds_dgraph {
@@ -111,6 +116,7 @@ struct Timer {
   }
   double ms(int a, int b) {
     float t = 0;
+    (void)hipEventSynchronize(ev[b]);  // a mark may still be in flight
     (void)hipEventElapsedTime(&t, ev[a], ev[b]);
     return t;
   }
@@ -236,6 +242,8 @@ bool flow_equal(__int128 f_scaled, float target, int sc) {
 
 // Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
 constexpr uint32_t kGridFlowRounds = 20000;
+// Incidence entries (k*T) from which the plain path peels in coarse steps.
+constexpr uint64_t kCoarseMinSlots = 1ull << 24;
 constexpr uint32_t kFlowTraceRounds = 4096;
 
 static bool flow_trace_on() {
@@ -599,6 +607,11 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
   if (const char *fb = getenv("CDS_FLOW_BLOCKS")) e->flow_blocks = (unsigned)atoi(fb);
   if (const char *rw = getenv("CDS_REPLAY_GRID_WORK"))
     e->replay_grid_work = strtoull(rw, nullptr, 10);
+  if (const char *pc = getenv("CDS_PEEL_COARSE")) e->peel_coarse = atoi(pc) ? 1 : 0;
+  if (const char *ps = getenv("CDS_PEEL_COARSE_SHIFT")) {
+    const int v = atoi(ps);
+    if (v >= 1 && v <= 16) e->peel_shift = (uint32_t)v;
+  }
   *out = e;
   return CDS_OK;
 }
@@ -766,29 +779,41 @@ cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
 }
 
 struct Level {
-  uint32_t level;
+  uint32_t level;  // core value of the step's vertices (an exact level, or
+                   // the top of a coarse step)
   uint64_t removed, killed;
+  uint32_t lo = kAliveHost;  // coarse step: the minimum degree that opened it
+  bool coarse() const { return lo != kAliveHost && lo < level; }
 };
 
 // Peel summary of one graph from its level table (ascending levels).
 // The state before levels[i] is the clique core C_level; the oracle's
 // density rows include each of them (helpers.cpp:125-134), so rho starts
-// as the largest boundary density and k' = ceil(rho).  Levels above k'
-// may hold intermediate states of higher density: `replay` lists (in
-// descending level order) those with at least two vertices whose bound
-// ceil((T_end + (M-1) l) / (N_end + M - 1)) exceeds that k'.
+// as the largest boundary density and k' = ceil(rho).  Inside a level l
+// every removed vertex has degree <= l, so its intermediate rows stay below
+// (T_end + x l) / (N_end + x) for the x <= M-1 vertices still to go; only
+// levels above k' can exceed it.  `replay` lists (in descending order) the
+// exact levels with at least two vertices whose bound
+// ceil((T_end + (M-1) l) / (N_end + M - 1)) exceeds k'.  A coarse step (its
+// vertices have core numbers in [lo, level]) obeys the same bound with
+// l = level; `refine` lists (descending) the coarse steps that must be
+// re-peeled exactly: those whose bound exceeds k', those straddling k'
+// (lo < k' <= level: the densest core would split them), the last step
+// (kmax is its exact top level), and every coarse step when all core
+// numbers are wanted.
 struct PeelSummary {
   std::vector<uint64_t> rm_before, kl_before;
   double rho = 0;
   int64_t kp = 0;
   uint64_t kmax = 0;
-  std::vector<size_t> replay;
+  std::vector<size_t> replay, refine;
 };
 
 cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
-                            uint64_t T, PeelSummary *S) {
+                            uint64_t T, PeelSummary *S, bool refine_all = false) {
   S->rm_before.assign(levels.size(), 0);
   S->kl_before.assign(levels.size(), 0);
+  S->kmax = 0;
   uint64_t r = 0, kk = 0;
   for (size_t i = 0; i < levels.size(); ++i) {
     S->rm_before[i] = r;
@@ -809,9 +834,17 @@ cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
   }
   S->kp = (int64_t)std::ceil(S->rho);
   S->replay.clear();
+  S->refine.clear();
   for (size_t ii = levels.size(); ii-- > 0;) {
     const Level &lv = levels[ii];
-    if ((int64_t)lv.level <= S->kp) break;
+    if (lv.coarse()) {
+      const bool straddle = (int64_t)lv.lo < S->kp && S->kp <= (int64_t)lv.level;
+      if (refine_all || straddle || ii + 1 == levels.size()) {
+        S->refine.push_back(ii);
+        continue;
+      }
+    }
+    if ((int64_t)lv.level <= S->kp) continue;
     const uint64_t M = lv.removed;
     if (M < 2) continue;
     const uint64_t N_end = n - S->rm_before[ii] - M;
@@ -822,7 +855,7 @@ cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
     const unsigned __int128 den = N_end + M - 1;
     const unsigned __int128 ceil_ub = (num + den - 1) / den;
     if ((int64_t)ceil_ub <= S->kp) continue;
-    S->replay.push_back(ii);
+    (lv.coarse() ? S->refine : S->replay).push_back(ii);
   }
   return CDS_OK;
 }
@@ -925,15 +958,60 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   std::vector<Level> levels;
   uint64_t removed_total = 0;
   uint32_t peel_rounds = 0;
+  // persistent-peel state, kept for the exact re-peels of coarse steps
+  DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A), ralive(&A), rcount(&A);
+  DBuf<uint64_t> s64(&A);
+  DBuf<unsigned long long> lrem(&A), lkil(&A);
+  DBuf<PeelCtl> ctl(&A);
+  const bool coarse_peel =
+      e->peel_coarse == 1 || (e->peel_coarse == -1 && (uint64_t)k * T >= kCoarseMinSlots);
+  const uint32_t cshift = (n < (1u << 24) && coarse_peel) ? e->peel_shift : 0;
+  R->peel_coarse_shift = cshift;
+  int wall_khz = 0;
+  if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) !=
+      hipSuccess)
+    wall_khz = 0;
+  // One persistent launch over `alive0` (null: every vertex) appending its
+  // steps to `out`; returns the launch's control block.
+  auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift, unsigned grid,
+                      uint64_t want_removed, std::vector<Level> *out,
+                      PeelCtl *hc) -> cds_status {
+    {
+      CoopGate gate(e, grid);
+      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
+                             s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p, sdeg.p,
+                             alive0, na0, shift, grid, st));
+    }
+    CHK(readback(e, hc, ctl.p, 1));
+    if (hc->removed != want_removed || hc->nlevels == 0 || hc->nlevels > n)
+      return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %llu (%u levels)",
+                  hc->removed, (unsigned long long)want_removed, hc->nlevels);
+    std::vector<uint32_t> hl(hc->nlevels), hm(hc->nlevels);
+    std::vector<unsigned long long> hr(hc->nlevels), hk(hc->nlevels);
+    CHK(readback(e, hl.data(), lvl.p, hc->nlevels));
+    CHK(readback(e, hm.data(), lmin.p, hc->nlevels));
+    CHK(readback(e, hr.data(), lrem.p, hc->nlevels));
+    CHK(readback(e, hk.data(), lkil.p, hc->nlevels));
+    for (uint32_t i = 0; i < hc->nlevels; ++i) {
+      Level lv{hl[i], hr[i], hk[i]};
+      if (hm[i] < hl[i]) lv.lo = hm[i];
+      out->push_back(lv);
+    }
+    peel_rounds += (uint32_t)hc->rounds;
+    if (wall_khz > 0) {
+      R->peel_split_ms[0] += (double)hc->t_bar / wall_khz;
+      R->peel_split_ms[1] += (double)hc->t_scan / wall_khz;
+      R->peel_split_ms[2] += (double)hc->t_proc / wall_khz;
+      R->peel_split_ms[3] += (double)hc->t_flush / wall_khz;
+    }
+    return CDS_OK;
+  };
   if (n < (1u << 24)) {
-    // every level in one cooperative launch
-    DBuf<uint32_t> s32(&A), lvl(&A);
-    DBuf<uint64_t> s64(&A);
-    DBuf<unsigned long long> lrem(&A), lkil(&A);
-    DBuf<PeelCtl> ctl(&A);
+    // every level (or coarse step) in one cooperative launch
     HIPCHK(s32.alloc(6ull * n));
     HIPCHK(s64.alloc(3ull * n));
     HIPCHK(lvl.alloc(n));
+    HIPCHK(lmin.alloc(n));
     HIPCHK(lrem.alloc(n));
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
@@ -941,37 +1019,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const unsigned grid = peel_grid_blocks(n, T, k, e->device);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     const int ev_p0 = tm.mark();
-    {
-      CoopGate gate(e, grid);
-      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
-                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
-                             sdeg.p, grid, st));
-    }
-    const int ev_p1 = tm.mark();
     PeelCtl hc;
-    CHK(readback(e, &hc, ctl.p, 1));
-    if (hc.removed != n || hc.nlevels == 0 || hc.nlevels > n)
-      return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %u (%u levels)",
-                  hc.removed, n, hc.nlevels);
-    std::vector<uint32_t> hl(hc.nlevels);
-    std::vector<unsigned long long> hr(hc.nlevels), hk(hc.nlevels);
-    CHK(readback(e, hl.data(), lvl.p, hc.nlevels));
-    CHK(readback(e, hr.data(), lrem.p, hc.nlevels));
-    CHK(readback(e, hk.data(), lkil.p, hc.nlevels));
-    for (uint32_t i = 0; i < hc.nlevels; ++i) levels.push_back({hl[i], hr[i], hk[i]});
+    CHK(run_peel(nullptr, 0, cshift, grid, n, &levels, &hc));
+    const int ev_p1 = tm.mark();
     R->t_peel_kernel_ms = tm.ms(ev_p0, ev_p1);
     removed_total = n;
-    peel_rounds = (uint32_t)hc.rounds;
     R->peel_grid = grid;
-    int wall_khz = 0;
-    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate,
-                              e->device) == hipSuccess &&
-        wall_khz > 0) {
-      R->peel_split_ms[0] = (double)hc.t_bar / wall_khz;
-      R->peel_split_ms[1] = (double)hc.t_scan / wall_khz;
-      R->peel_split_ms[2] = (double)hc.t_proc / wall_khz;
-      R->peel_split_ms[3] = (double)hc.t_flush / wall_khz;
-    }
   }
   HIPCHK(fa.alloc(n));
   HIPCHK(fb.alloc(n));
@@ -1010,28 +1063,85 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   fa.reset();
   fb.reset();
   R->t_incidence_ms = tm.ms(ev_inc0, ev_inc1);
-  R->peel_levels = peel_rounds;
-  R->peel_distinct_levels = (uint32_t)levels.size();
-  {
-    uint64_t alive = n, scanned = 0;
-    for (auto &lv : levels) {
-      scanned += alive;
-      alive -= lv.removed;
-    }
-    R->peel_alive_scanned = scanned;
-  }
 
+  // Exact re-peel of coarse step i: its vertices and cliques become alive
+  // again (degrees from sdeg), later steps are parked, and the step's
+  // entry is replaced by its exact levels.
+  auto refine_step = [&](size_t i) -> cds_status {
+    const Level lv = levels[i];
+    if (!ralive.p) {
+      HIPCHK(ralive.alloc(n));
+      HIPCHK(rcount.alloc(1));
+    }
+    HIPCHK(refine_prepare(n, T, lv.level, core.p, deg.p, sdeg.p, kill.p, ralive.p,
+                          rcount.p, st));
+    uint32_t na = 0;
+    CHK(readback(e, &na, rcount.p, 1));
+    if (na != lv.removed)
+      return fail(CDS_ERR_INTERNAL, "coarse step %u holds %u of %llu vertices", lv.level,
+                  na, (unsigned long long)lv.removed);
+    const unsigned grid = peel_grid_blocks(na, lv.killed, k, e->device);
+    if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
+    std::vector<Level> exact;
+    PeelCtl hc;
+    CHK(run_peel(ralive.p, na, 0, grid, na, &exact, &hc));
+    HIPCHK(refine_restore(n, core.p, deg.p, st));
+    uint64_t kl = 0;
+    for (const Level &x : exact) {
+      kl += x.killed;
+      if (x.level < lv.lo || x.level > lv.level)
+        return fail(CDS_ERR_INTERNAL, "coarse step [%u, %u] re-peeled to level %u", lv.lo,
+                    lv.level, x.level);
+    }
+    if (kl != lv.killed)
+      return fail(CDS_ERR_INTERNAL, "coarse step %u killed %llu of %llu cliques", lv.level,
+                  (unsigned long long)kl, (unsigned long long)lv.killed);
+    levels.erase(levels.begin() + (ptrdiff_t)i);
+    levels.insert(levels.begin() + (ptrdiff_t)i, exact.begin(), exact.end());
+    R->peel_refined_steps++;
+    return CDS_OK;
+  };
+
+  // Coarse steps are refined and exact levels replayed until neither list
+  // has work: each changes the table or rho, so the summary is recomputed.
   PeelSummary PS;
-  CHK(summarize_levels(levels, n, T, &PS));
-  const std::vector<uint64_t> &rm_before = PS.rm_before, &kl_before = PS.kl_before;
+  std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
+  const bool all_cores = p->want_vertex_arrays != 0;
+  auto summarize = [&]() -> cds_status {
+    CHK(summarize_levels(levels, n, T, &PS, all_cores));
+    for (const auto &r : replayed) apply_replay(&PS, r.second);
+    return CDS_OK;
+  };
+  CHK(summarize());
   {
     DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
     DBuf<uint64_t> pioff(&A);
     DBuf<uint8_t> palive(&A);
     DBuf<ReplayOut> rout(&A);
     bool allocated = false;
+    double refine_ms = 0;
     const int ev_r0 = tm.mark();
-    for (size_t ii : PS.replay) {
+    for (;;) {
+      if (!PS.refine.empty()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        CHK(refine_step(PS.refine.front()));
+        HIPCHK(hipStreamSynchronize(st));
+        refine_ms += std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - t0)
+                         .count();
+        CHK(summarize());
+        continue;
+      }
+      size_t ii = SIZE_MAX;
+      for (size_t cand : PS.replay) {
+        bool done = false;
+        for (const auto &r : replayed) done |= r.first == levels[cand].level;
+        if (!done) {
+          ii = cand;
+          break;
+        }
+      }
+      if (ii == SIZE_MAX) break;
       const Level &lv = levels[ii];
       const uint64_t M = lv.removed;
       if (!allocated) {
@@ -1063,7 +1173,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       if (rgrid) gate.emplace(e, rgrid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
-                          T - kl_before[ii], n - rm_before[ii], (uint32_t)M,
+                          T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
                           pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
                           pioff.p, pinc.p, rout.p, partial.p, gbar.p, rgrid,
                           sdeg.p, A, tm.ev[23], st));
@@ -1079,11 +1189,24 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       if (ro.steps != M)
         return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
                     lv.level, ro.steps, (unsigned long long)M);
+      replayed.push_back({lv.level, ro});
       apply_replay(&PS, ro);
     }
     const int ev_r1 = tm.mark();
     HIPCHK(hipStreamSynchronize(st));
-    R->t_replay_ms = tm.ms(ev_r0, ev_r1);
+    R->t_replay_ms = std::max(0.0, tm.ms(ev_r0, ev_r1) - refine_ms);
+    R->t_peel_kernel_ms += refine_ms;
+  }
+  const std::vector<uint64_t> &rm_before = PS.rm_before, &kl_before = PS.kl_before;
+  R->peel_levels = peel_rounds;
+  R->peel_distinct_levels = (uint32_t)levels.size();
+  {
+    uint64_t alive = n, scanned = 0;
+    for (auto &lv : levels) {
+      scanned += alive;
+      alive -= lv.removed;
+    }
+    R->peel_alive_scanned = scanned;
   }
   R->peel_max_density = PS.rho;
   const int64_t kp = PS.kp;
@@ -1633,13 +1756,14 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
   uint32_t peel_grid = 0, peel_rounds = 0;
   {
-    DBuf<uint32_t> s32(&A), lvl(&A);
+    DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A);
     DBuf<uint64_t> s64(&A);
     DBuf<unsigned long long> lrem(&A), lkil(&A);
     DBuf<PeelCtl> ctl(&A);
     HIPCHK(s32.alloc(6ull * n));
     HIPCHK(s64.alloc(3ull * n));
     HIPCHK(lvl.alloc(n));
+    HIPCHK(lmin.alloc(n));
     HIPCHK(lrem.alloc(n));
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
@@ -1648,8 +1772,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     {
       CoopGate gate(e, peel_grid);
       HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
-                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lrem.p, lkil.p,
-                             sdeg.p, peel_grid, st));
+                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p,
+                             sdeg.p, nullptr, 0, 0, peel_grid, st));
     }
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));

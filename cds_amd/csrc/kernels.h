@@ -122,17 +122,38 @@ struct PeelCtl {
 };
 // Scratch: u32 6n (two alive lists, three frontiers, alive positions),
 // u64 3n (chunk starts).
-// Level tables (capacity n, zeroed here): level, removed, killed.
-// sdeg[v] (n entries): v's degree at the start of the level removing it.
+// Level tables (capacity n, zeroed here), one entry per step: level (the
+// core value given to the step's vertices and cliques), lv_min (the
+// minimum degree that opened it; == level for an exact level), removed,
+// killed.  cshift = 0 peels exact levels; cshift > 0 coarse steps covering
+// [m, m + (m >> cshift)].
+// sdeg[v] (n entries): v's degree at the start of the step removing it.
+// alive0 (na0 vertices, or null for all n): the vertices alive at the
+// start; every other vertex must have core != kAlive and every clique
+// containing one must be dead or never reached.
 // Requires n < 2^24.  Synchronises nothing; read ctl after the stream.
 hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
                            uint32_t *deg, uint32_t *core, uint32_t *kill,
                            uint32_t *scratch32, uint64_t *scratch64,
-                           PeelCtl *ctl, uint32_t *lv_level,
+                           PeelCtl *ctl, uint32_t *lv_level, uint32_t *lv_min,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
+                           const uint32_t *alive0, uint32_t na0, uint32_t cshift,
                            unsigned grid, hipStream_t s);
+// Prepares the exact re-peel of one coarse step whose vertices and cliques
+// carry the value `level`: those vertices become alive (core = kAlive,
+// degree = sdeg, listed in alive[], count in *count), vertices of later
+// steps (core > level) are parked (core = kParked, their core value kept
+// in deg) and the step's cliques become alive again.
+constexpr uint32_t kParked = 0xFFFFFFFEu;
+hipError_t refine_prepare(uint32_t n, uint64_t T, uint32_t level,
+                          uint32_t *core, uint32_t *deg, const uint32_t *sdeg,
+                          uint32_t *kill, uint32_t *alive, uint32_t *count,
+                          hipStream_t s);
+// Undoes the parking after the re-peel.
+hipError_t refine_restore(uint32_t n, uint32_t *core, const uint32_t *deg,
+                          hipStream_t s);
 // Workgroups for peel_persistent on this graph (0: cannot launch).
 unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device);
 

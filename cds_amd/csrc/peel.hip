@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     uint32_t *F0, uint32_t *F1, uint32_t *F2, uint64_t *FC0, uint64_t *FC1,
     uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
     unsigned long long *lv_removed, unsigned long long *lv_killed,
-    uint32_t *pos, uint32_t *sdeg) {
+    uint32_t *pos, uint32_t *sdeg, uint32_t *lv_min, uint32_t cshift) {
   // Once few vertices remain alive, degree decrements are summed per
   // workgroup in LDS (indexed by position in the alive list) and applied
   // with one atomic per vertex and sub-round; the vertex whose aggregated
@@ -281,8 +281,15 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
       ctl->t_scan += wall_clock64() - ts;
     }
     grid_sync(ctl);
-    const uint32_t level = ld(&ctl->gmin[gp]);
-    if (level == kAlive) break;
+    // The step removes every alive vertex whose degree drops to `level`:
+    // the minimum degree (one exact level), or with cshift > 0 the range
+    // [min, min + min >> cshift] at once (a coarse step: its start and end
+    // are exact clique cores, the core numbers inside it are not).
+    const uint32_t mlev = ld(&ctl->gmin[gp]);
+    if (mlev == kAlive) break;
+    const uint32_t level =
+        (cshift && mlev < kParked - 1) ? mlev + min(mlev >> cshift, kParked - 1 - mlev)
+                                       : mlev;
     if (leader) {
       ctl->gmin[1 - gp] = kAlive;
       ts = wall_clock64();
@@ -432,6 +439,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     // level; the next min scan touches only buffers no block still reads.
     if (leader) {
       lv_level[lvl_idx] = level;
+      lv_min[lvl_idx] = mlev;
       lv_removed[lvl_idx] = removed;
       ctl->removed += removed;
     }
@@ -442,18 +450,58 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
   if (leader) ctl->nlevels = lvl_idx;
 }
 
+// ---- exact re-peel of one coarse step (refine_prepare / restore) ----
+
+__global__ void k_refine_vertices(uint32_t n, uint32_t level,
+                                  uint32_t *__restrict__ core,
+                                  uint32_t *__restrict__ deg,
+                                  const uint32_t *__restrict__ sdeg,
+                                  uint32_t *__restrict__ alive,
+                                  uint32_t *__restrict__ count) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t v = base + threadIdx.x;
+    const uint32_t c = v < n ? core[v] : 0;
+    const bool mine = v < n && c == level;
+    if (v < n && c > level) {  // a later step: parked, value kept in deg
+      deg[v] = c;
+      core[v] = kParked;
+    }
+    if (mine) {
+      core[v] = kAlive;
+      deg[v] = sdeg[v];  // its degree when the step began
+    }
+    const uint32_t slot = aggregated_reserve(count, 0, mine);
+    if (mine) alive[slot] = v;
+  }
+}
+
+__global__ void k_refine_cliques(uint64_t T, uint32_t level,
+                                 uint32_t *__restrict__ kill) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x)
+    if (kill[c] == level) kill[c] = kAlive;
+}
+
+__global__ void k_refine_restore(uint32_t n, uint32_t *__restrict__ core,
+                                 const uint32_t *__restrict__ deg) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x)
+    if (core[v] == kParked) core[v] = deg[v];
+}
+
 __global__ void k_iota(uint32_t n, uint32_t *a) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x)
     a[i] = i;
 }
 
-__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t n) {
+__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t alive) {
   unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
   for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
   ctl->gmin[0] = kAlive;
   ctl->gmin[1] = kAlive;
-  ctl->alive_n[0] = n;
+  ctl->alive_n[0] = alive;
   ctl->alive_n[1] = 0;
   ctl->fpack[0] = ctl->fpack[1] = ctl->fpack[2] = 0;
   ctl->nlevels = 0;
@@ -856,29 +904,59 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
   return hipGetLastError();
 }
 
+hipError_t refine_prepare(uint32_t n, uint64_t T, uint32_t level,
+                          uint32_t *core, uint32_t *deg, const uint32_t *sdeg,
+                          uint32_t *kill, uint32_t *alive, uint32_t *count,
+                          hipStream_t s) {
+  if (level >= kParked) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(count, 0, 4, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_refine_vertices, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
+                     n, level, core, deg, sdeg, alive, count);
+  hipLaunchKernelGGL(k_refine_cliques, dim3(grid_for(T, 256, 16384)), dim3(256), 0, s,
+                     T, level, kill);
+  return hipGetLastError();
+}
+
+hipError_t refine_restore(uint32_t n, uint32_t *core, const uint32_t *deg,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_restore, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
+                     n, core, deg);
+  return hipGetLastError();
+}
+
 hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
                            uint32_t *deg, uint32_t *core, uint32_t *kill,
                            uint32_t *scratch32, uint64_t *scratch64,
-                           PeelCtl *ctl, uint32_t *lv_level,
+                           PeelCtl *ctl, uint32_t *lv_level, uint32_t *lv_min,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
+                           const uint32_t *alive0, uint32_t na0, uint32_t cshift,
                            unsigned grid, hipStream_t s) {
-  if (n >= (1u << 24) || grid < 1) return hipErrorInvalidValue;
+  if (n >= (1u << 24) || grid < 1 || cshift > 31) return hipErrorInvalidValue;
+  const uint32_t na = alive0 ? na0 : n;
   hipError_t e;
   if ((e = hipMemsetAsync(lv_level, 0, (size_t)n * 4, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(lv_min, 0, (size_t)n * 4, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(lv_removed, 0, (size_t)n * 8, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(lv_killed, 0, (size_t)n * 8, s)) != hipSuccess) return e;
   uint32_t *A0 = scratch32, *A1 = A0 + n, *F0 = A1 + n, *F1 = F0 + n,
            *F2 = F1 + n, *pos = F2 + n;
   uint64_t *FC0 = scratch64, *FC1 = FC0 + n, *FC2 = FC1 + n;
-  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A0);
-  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, n);
+  if (alive0) {
+    if (na && (e = hipMemcpyAsync(A0, alive0, (size_t)na * 4, hipMemcpyDeviceToDevice,
+                                  s)) != hipSuccess)
+      return e;
+  } else {
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A0);
+  }
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, na);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
                   &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
-                  &sdeg};
+                  &sdeg, &lv_min, &cshift};
   return launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
                      kPeelThreads, args, 0, s);
 }

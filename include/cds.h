@@ -133,7 +133,11 @@ typedef struct {
   uint32_t peel_grid;               /* workgroups of the persistent peel */
   uint32_t flow_launches;           /* exact max flows computed */
   uint32_t density_certificates;    /* components proven their own densest */
-  uint32_t peel_distinct_levels;    /* distinct clique-core values */
+  uint32_t peel_distinct_levels;    /* level-table entries: exact levels,
+                                       plus coarse steps left unrefined */
+  uint32_t peel_coarse_shift;       /* 0: exact levels; s: first pass in
+                                       steps [m, m + m>>s] */
+  uint32_t peel_refined_steps;      /* coarse steps re-peeled exactly */
   uint64_t peel_alive_scanned;      /* sum over levels of alive vertices */
   double peel_split_ms[4];          /* one peel thread: barrier, scans,
                                        chunk processing, LDS flushes */
