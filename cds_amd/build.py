@@ -7,6 +7,7 @@ written.
 """
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -40,6 +41,27 @@ def _run(cmd):
     return r
 
 
+# Kernels launched cooperatively (all workgroups co-resident, spinning on
+# grid barriers) must not spill to scratch memory: the build fails if the
+# compiler reports any scratch for them.
+COOPERATIVE_KERNELS = ("k_peel_persistent", "k_grid_flow", "k_flow_batch",
+                       "k_replay_grid")
+
+
+def _check_scratch(src, stderr):
+    name = None
+    for line in stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            continue
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name and int(m.group(1)) > 0 and \
+                any(k in name for k in COOPERATIVE_KERNELS):
+            raise RuntimeError(f"{src}: cooperative kernel {name} uses "
+                               f"{m.group(1)} bytes/lane of scratch")
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
@@ -48,14 +70,17 @@ def build(verbose=False):
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *DEVICE_ONLY,
+                     "-Rpass-analysis=kernel-resource-usage",
                      "-c", os.path.join(CSRC, src), "-o", obj])
     for src in HOST_SOURCES + ["main.cpp"]:
         obj = os.path.join(BUILD, src + ".o")
         jobs.append(["g++", *COMMON, *HOST_ONLY, "-c", os.path.join(CSRC, src),
                      "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        for r in ex.map(_run, jobs):
-            if verbose and (r.stdout or r.stderr):
+        for cmd, r in zip(jobs, ex.map(_run, jobs)):
+            if "-Rpass-analysis=kernel-resource-usage" in cmd:
+                _check_scratch(cmd[-3], r.stderr)
+            elif verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
     objs = [os.path.join(BUILD, s + ".o") for s in HIP_SOURCES + HOST_SOURCES]
     tmp = LIB + ".tmp"

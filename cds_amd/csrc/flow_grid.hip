@@ -62,6 +62,9 @@ constexpr uint32_t kArcWindow0 = 4;
 constexpr uint32_t kArcWindowRelabel = 64;
 constexpr uint32_t kArcWindowMax = 1u << 24;  // more than any list has
 constexpr uint64_t kArcFull = (uint64_t)kArcWindowMax << 32;  // chunk 0, whole list
+// After a global relabel in mid-solve the excess left is small: windows
+// restart at chunk 0 with 256 chunks instead of the whole list.
+constexpr uint64_t kArcAfterRelabel = (uint64_t)256 << 32;
 constexpr int kChunkSteps = kGridChunk / kWave;  // wave passes per chunk
 constexpr int kMaxFlowWaves = 16;                // 1024-thread workgroups
 
@@ -225,7 +228,7 @@ __device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
 // Exact BFS distances to the sink over reversed residual arcs; returns the
 // number of active nodes (excess > 0 and finite height).  Uses list buffer
 // 0/1 for vertex frontiers and cq[0]/cq[1] for clique frontiers.
-__device__ unsigned long long global_relabel(const G &g) {
+__device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
   GridFlowCtl *ctl = g.ctl;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
   for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.hc[c] = g.INF;
@@ -255,6 +258,33 @@ __device__ unsigned long long global_relabel(const G &g) {
       ctl->lpack[1 - vb] = 0;
     }
     gsync(g);
+    // Direction choice: when the frontier's incidence is large against the
+    // clique count, every unlabelled clique checks its own members instead
+    // (a sequential pass over the cliques, no atomics); otherwise the
+    // frontier vertices scan their incidence.
+    if (nch * kGridChunk >= (uint64_t)g.T / 4) {
+      constexpr int kPull = 2;
+      for (uint64_t t0 = (uint64_t)g.bid * blockDim.x * kPull; t0 < g.T;
+           t0 += (uint64_t)g.nblocks * blockDim.x * kPull) {
+        bool f[kPull];
+        uint32_t cs[kPull];
+#pragma unroll
+        for (int it = 0; it < kPull; ++it) {
+          const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+          f[it] = false;
+          cs[it] = (uint32_t)c;
+          if (c >= g.T || g.hc[c] != g.INF) continue;
+          const uint64_t base = c * g.k;
+          for (uint32_t j = 0; j < g.k; ++j)
+            if (g.hv[g.mem[base + j] - g.vbase] == d && g.U + g.x[base + j] > 0) {
+              f[it] = true;
+              break;
+            }
+          if (f[it]) g.hc[c] = d + 1;
+        }
+        block_append(g.app, &ctl->cqn[cb], gcq(g, cb), f, cs);
+      }
+    } else
     for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
       uint32_t v;
       uint64_t lo, hi;
@@ -268,10 +298,11 @@ __device__ unsigned long long global_relabel(const G &g) {
         cs[it] = 0;
         if (p >= hi) continue;
         const uint64_t s = g.inc[p] - g.sbase;
-        if (g.U + g.x[s] <= 0) continue;
         const uint32_t c = (uint32_t)(s / g.k);
-        // a plain load filters: a stale INF only costs a failed CAS
+        // a plain load filters: a stale INF only costs a failed CAS; most
+        // cliques met after the first level are labelled already
         if (g.hc[c] != g.INF) continue;
+        if (g.U + g.x[s] <= 0) continue;
         f[it] = atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF;
         cs[it] = c;
       }
@@ -305,7 +336,7 @@ __device__ unsigned long long global_relabel(const G &g) {
   unsigned long long act = 0;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
     act += g.exv[v] > 0 && g.hv[v] < g.INF;
-    g.arc[v] = kArcFull;  // new heights: every arc may be admissible again
+    g.arc[v] = arc_reset;  // new heights: every arc may be admissible again
   }
   constexpr int kScan = 8;  // cliques per thread and block_append
   for (uint64_t t0 = (uint64_t)g.bid * blockDim.x * kScan; t0 < g.T;
@@ -391,7 +422,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     ctl->relabels = 0;
   }
   gsync(g);
-  unsigned long long active = global_relabel(g);
+  unsigned long long active = global_relabel(g, kArcFull);
   int wb = 0;  // clique work list of this round (global_relabel fills cq0)
   uint32_t rounds = 0, relabels = 1, since = 0;
   unsigned long long sink = 0;
@@ -637,7 +668,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     ++since;
     wb = 1 - wb;
     if (any == 0 || since >= kGridRelabel) {
-      active = global_relabel(g);
+      active = global_relabel(g, kArcAfterRelabel);
       wb = 0;
       ++relabels;
       since = 0;
@@ -654,7 +685,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       t[7] = wall_clock64() - tr3;  // global relabel, if one ran
     }
   }
-  if (!limit) global_relabel(g);
+  if (!limit) global_relabel(g, kArcFull);
   sink = wave_sum64(sink);
   if (lane() == 0 && sink) atomicAdd(&ctl->sink, sink);
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
