@@ -263,7 +263,7 @@ __device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
     // (a sequential pass over the cliques, no atomics); otherwise the
     // frontier vertices scan their incidence.
     if (nch * kGridChunk >= (uint64_t)g.T / 4) {
-      constexpr int kPull = 2;
+      constexpr int kPull = 1;
       for (uint64_t t0 = (uint64_t)g.bid * blockDim.x * kPull; t0 < g.T;
            t0 += (uint64_t)g.nblocks * blockDim.x * kPull) {
         bool f[kPull];
@@ -309,8 +309,31 @@ __device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
       wave_append(&ctl->cqn[cb], gcq(g, cb), f, cs);
     }
     gsync(g);
-    // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x)
+    // cliques at d+1 -> vertices at d+2 (arc v->clique residual (k-1)U-x);
+    // a large clique frontier is pulled: each unlabelled vertex scans its
+    // own incidence for a clique at d+1 and stops at the first one
     const uint32_t nc = ald(&ctl->cqn[cb]);
+    if ((uint64_t)nc * 4 >= g.T) {
+      for (uint32_t v = g.gwave; v < g.V; v += g.gwaves) {
+        if (g.hv[v] != g.INF) continue;  // wave-uniform
+        const uint64_t e = g.ioff[v + 1];
+        for (uint64_t p0 = g.ioff[v]; p0 < e; p0 += kWave) {
+          const uint64_t p = p0 + lane();
+          bool hit = false;
+          if (p < e) {
+            const uint64_t s = g.inc[p] - g.sbase;
+            hit = g.hc[s / g.k] == d + 1 && (long long)(g.k - 1) * g.U - g.x[s] > 0;
+          }
+          if (__ballot(hit)) {
+            if (lane() == 0) {
+              g.hv[v] = d + 2;
+              list_push(g, 1 - vb, v);
+            }
+            break;
+          }
+        }
+      }
+    } else
     for (uint32_t i = g.gtid; i < nc; i += g.gthreads) {
       const uint32_t c = gcq(g, cb)[i];
       const uint64_t base = (uint64_t)c * g.k;
