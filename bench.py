@@ -451,23 +451,32 @@ def main():
     def avg(key):
         return sum(x[key] for x in results) / len(results)
 
-    # per-kernel HIP-event times measured inside the engine on its stream
+    # per-kernel HIP-event times measured inside the engine on its stream;
+    # the peel runs as one first pass plus one launch per re-peeled coarse
+    # step, so its per-launch figures are the solve's totals over launches
     cnt_ms = avg("t_list_count_ms")
-    peel_ms = avg("t_peel_kernel_ms")
+    peel_launches = avg("peel_launches") or 1
     list_ms = sum(x["times"]["dag"] + x["times"]["listing"] for x in results) / len(results)
     kernels = {
-        "k_list3": {"avg_launch_ms": cnt_ms,
-                    "algorithmic_bytes": list3_bytes(g.n, r["dag_arcs"], r["list_work"])},
-        "k_peel_persistent": {"avg_launch_ms": peel_ms,
-                              "algorithmic_bytes": peel_bytes(g.n, T, w["k"])},
-        "k_build_inc": {"avg_launch_ms": avg("t_incidence_ms"),
+        "k_list3": {"launches_per_solve": 1, "total_ms_per_solve": cnt_ms,
+                    "algorithmic_bytes_per_solve":
+                        list3_bytes(g.n, r["dag_arcs"], r["list_work"])},
+        "k_peel_persistent": {"launches_per_solve": peel_launches,
+                              "total_ms_per_solve": avg("t_peel_kernel_ms"),
+                              "algorithmic_bytes_per_solve": peel_bytes(g.n, T, w["k"])},
+        "k_build_inc": {"launches_per_solve": 1,
+                        "total_ms_per_solve": avg("t_incidence_ms"),
                         # read the cliques, write k entries of k words each
-                        "algorithmic_bytes": 4 * w["k"] * T + 4 * w["k"] * w["k"] * T},
+                        "algorithmic_bytes_per_solve":
+                            4 * w["k"] * T + 4 * w["k"] * w["k"] * T},
     }
     for kv in kernels.values():
-        t = kv["avg_launch_ms"]
-        kv["achieved_GBps"] = kv["algorithmic_bytes"] / (t / 1000.0) / 1e9 if t > 0 else None
-    dom = max(kernels, key=lambda name: kernels[name]["avg_launch_ms"])
+        t = kv["total_ms_per_solve"]
+        kv["avg_launch_ms"] = t / kv["launches_per_solve"]
+        kv["algorithmic_bytes"] = kv["algorithmic_bytes_per_solve"] / kv["launches_per_solve"]
+        kv["achieved_GBps"] = (kv["algorithmic_bytes_per_solve"] / (t / 1000.0) / 1e9
+                               if t > 0 else None)
+    dom = max(kernels, key=lambda name: kernels[name]["total_ms_per_solve"])
     alg = kernels[dom]["algorithmic_bytes"]
     dom_ms = kernels[dom]["avg_launch_ms"]
     achieved = kernels[dom]["achieved_GBps"]
@@ -505,9 +514,11 @@ def main():
                      "traffic": read_traffic(dom, args.workload),
                      "algorithmic_bytes": alg, "avg_launch_ms": dom_ms},
         "kernels": kernels,
-        "peel": {"levels": r["peel_distinct_levels"], "sub_rounds": r["peel_levels"],
-                 "workgroups": r["peel_grid"], "replay_phases": r["replay_phases"],
-                 "replay_ms": avg("t_replay_ms")},
+        "peel": {"level_table_entries": r["peel_distinct_levels"],
+                 "sub_rounds": r["peel_levels"], "workgroups": r["peel_grid"],
+                 "coarse_shift": r["peel_coarse_shift"],
+                 "refined_steps": r["peel_refined_steps"],
+                 "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms")},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -93,6 +93,7 @@ class CResult(ctypes.Structure):
         ("peel_distinct_levels", ctypes.c_uint32),
         ("peel_coarse_shift", ctypes.c_uint32),
         ("peel_refined_steps", ctypes.c_uint32),
+        ("peel_launches", ctypes.c_uint32),
         ("peel_alive_scanned", ctypes.c_uint64),
         ("peel_split_ms", ctypes.c_double * 4),
         ("t_incidence_ms", ctypes.c_double), ("t_peel_kernel_ms", ctypes.c_double),

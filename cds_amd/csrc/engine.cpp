@@ -973,15 +973,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     wall_khz = 0;
   // One persistent launch over `alive0` (null: every vertex) appending its
   // steps to `out`; returns the launch's control block.
+  // every launch is timed with the spare event pair; R->t_peel_kernel_ms
+  // sums them
   auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift, unsigned grid,
                       uint64_t want_removed, std::vector<Level> *out,
                       PeelCtl *hc) -> cds_status {
     {
       CoopGate gate(e, grid);
+      HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
                              s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p, sdeg.p,
                              alive0, na0, shift, grid, st));
+      HIPCHK(hipEventRecord(tm.ev[23], st));
     }
+    R->t_peel_kernel_ms += tm.ms(22, 23);
+    R->peel_launches++;
     CHK(readback(e, hc, ctl.p, 1));
     if (hc->removed != want_removed || hc->nlevels == 0 || hc->nlevels > n)
       return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %llu (%u levels)",
@@ -1018,11 +1024,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(sdeg.alloc(n));
     const unsigned grid = peel_grid_blocks(n, T, k, e->device);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
-    const int ev_p0 = tm.mark();
     PeelCtl hc;
     CHK(run_peel(nullptr, 0, cshift, grid, n, &levels, &hc));
-    const int ev_p1 = tm.mark();
-    R->t_peel_kernel_ms = tm.ms(ev_p0, ev_p1);
     removed_total = n;
     R->peel_grid = grid;
   }
@@ -1195,7 +1198,6 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const int ev_r1 = tm.mark();
     HIPCHK(hipStreamSynchronize(st));
     R->t_replay_ms = std::max(0.0, tm.ms(ev_r0, ev_r1) - refine_ms);
-    R->t_peel_kernel_ms += refine_ms;
   }
   const std::vector<uint64_t> &rm_before = PS.rm_before, &kl_before = PS.kl_before;
   R->peel_levels = peel_rounds;

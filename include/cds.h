@@ -138,11 +138,14 @@ typedef struct {
   uint32_t peel_coarse_shift;       /* 0: exact levels; s: first pass in
                                        steps [m, m + m>>s] */
   uint32_t peel_refined_steps;      /* coarse steps re-peeled exactly */
+  uint32_t peel_launches;           /* persistent peel launches (first pass
+                                       + re-peels) */
   uint64_t peel_alive_scanned;      /* sum over levels of alive vertices */
   double peel_split_ms[4];          /* one peel thread: barrier, scans,
                                        chunk processing, LDS flushes */
   double t_incidence_ms;            /* clique-incidence build (HIP events) */
-  double t_peel_kernel_ms;          /* persistent peel launch (HIP events) */
+  double t_peel_kernel_ms;          /* persistent peel launches, summed
+                                       (HIP events) */
   double t_replay_ms;               /* exact phase replays (HIP events) */
   double t_replay_prep_ms;          /* ... of which phase incidence compaction */
   cds_times times;
