@@ -1265,6 +1265,10 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     return CDS_OK;
   }
 
+  // The flow networks index core clique slots with 32 bits: a core beyond
+  // that is refused here, before pruning and bucketing it.
+  if (core_T * k >= 0xFFFFFFFFull)
+    return fail(CDS_ERR_RANGE, "core clique slots exceed 32 bits");
   // ---- prune ----
   DBuf<uint8_t> mark(&A);
   {
