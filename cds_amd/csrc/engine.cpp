@@ -1164,7 +1164,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       // Few vertices with many cliques each: replay on the whole GPU.
       unsigned rgrid = 0;
       if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
-        rgrid = replay_grid_blocks(e->device);
+        rgrid = replay_grid_blocks(e->device, lv.killed * k / M);
       if (rgrid) {
         HIPCHK(partial.alloc((size_t)rgrid * M));
         HIPCHK(gbar.alloc(1));

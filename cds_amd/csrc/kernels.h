@@ -201,7 +201,9 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         hipEvent_t prepared, hipStream_t s);
 constexpr uint32_t kReplayGridMaxVertices = 16384;
 // Workgroups of the whole-GPU replay (half the CUs), 0 if unavailable.
-unsigned replay_grid_blocks(int device);
+// Workgroups for the whole-GPU replay of a phase whose removals touch about
+// entries_per_step incidence entries each (CDS_REPLAY_BLOCKS overrides).
+unsigned replay_grid_blocks(int device, uint64_t entries_per_step);
 
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,
