@@ -975,15 +975,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // steps to `out`; returns the launch's control block.
   // every launch is timed with the spare event pair; R->t_peel_kernel_ms
   // sums them
-  auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift, unsigned grid,
-                      uint64_t want_removed, std::vector<Level> *out,
-                      PeelCtl *hc) -> cds_status {
+  auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift,
+                      uint32_t max_level, unsigned grid, uint64_t want_removed,
+                      std::vector<Level> *out, PeelCtl *hc) -> cds_status {
     {
       CoopGate gate(e, grid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
                              s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p, sdeg.p,
-                             alive0, na0, shift, grid, st));
+                             alive0, na0, shift, max_level, grid, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
     }
     R->t_peel_kernel_ms += tm.ms(22, 23);
@@ -1025,7 +1025,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const unsigned grid = peel_grid_blocks(n, T, k, e->device);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     PeelCtl hc;
-    CHK(run_peel(nullptr, 0, cshift, grid, n, &levels, &hc));
+    CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
     removed_total = n;
     R->peel_grid = grid;
   }
@@ -1085,9 +1085,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                   na, (unsigned long long)lv.removed);
     const unsigned grid = peel_grid_blocks(na, lv.killed, k, e->device);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
+    // Exact levels at once: re-peeling a step in narrower steps first and
+    // refining those re-processes the step's cliques at every depth, which
+    // measured slower on R-MAT scale 20 k=3 and scale 17 k=4.
     std::vector<Level> exact;
     PeelCtl hc;
-    CHK(run_peel(ralive.p, na, 0, grid, na, &exact, &hc));
+    CHK(run_peel(ralive.p, na, 0, lv.level, grid, na, &exact, &hc));
     HIPCHK(refine_restore(n, core.p, deg.p, st));
     uint64_t kl = 0;
     for (const Level &x : exact) {
@@ -1779,7 +1782,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       CoopGate gate(e, peel_grid);
       HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
                              kill.p, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p,
-                             sdeg.p, nullptr, 0, 0, peel_grid, st));
+                             sdeg.p, nullptr, 0, 0, kAliveHost, peel_grid, st));
     }
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));

@@ -126,7 +126,8 @@ struct PeelCtl {
 // core value given to the step's vertices and cliques), lv_min (the
 // minimum degree that opened it; == level for an exact level), removed,
 // killed.  cshift = 0 peels exact levels; cshift > 0 coarse steps covering
-// [m, m + (m >> cshift)].
+// [m, min(m + (m >> cshift), max_level)] (max_level bounds the values given
+// out when the vertices' core numbers are known not to exceed it).
 // sdeg[v] (n entries): v's degree at the start of the step removing it.
 // alive0 (na0 vertices, or null for all n): the vertices alive at the
 // start; every other vertex must have core != kAlive and every clique
@@ -140,7 +141,7 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
                            const uint32_t *alive0, uint32_t na0, uint32_t cshift,
-                           unsigned grid, hipStream_t s);
+                           uint32_t max_level, unsigned grid, hipStream_t s);
 // Prepares the exact re-peel of one coarse step whose vertices and cliques
 // carry the value `level`: those vertices become alive (core = kAlive,
 // degree = sdeg, listed in alive[], count in *count), vertices of later

@@ -226,7 +226,8 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     uint32_t *F0, uint32_t *F1, uint32_t *F2, uint64_t *FC0, uint64_t *FC1,
     uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
     unsigned long long *lv_removed, unsigned long long *lv_killed,
-    uint32_t *pos, uint32_t *sdeg, uint32_t *lv_min, uint32_t cshift) {
+    uint32_t *pos, uint32_t *sdeg, uint32_t *lv_min, uint32_t cshift,
+    uint32_t max_level) {
   // Once few vertices remain alive, degree decrements are summed per
   // workgroup in LDS (indexed by position in the alive list) and applied
   // with one atomic per vertex and sub-round; the vertex whose aggregated
@@ -288,8 +289,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     const uint32_t mlev = ld(&ctl->gmin[gp]);
     if (mlev == kAlive) break;
     const uint32_t level =
-        (cshift && mlev < kParked - 1) ? mlev + min(mlev >> cshift, kParked - 1 - mlev)
-                                       : mlev;
+        (cshift && mlev < max_level) ? mlev + min(mlev >> cshift, max_level - mlev) : mlev;
     if (leader) {
       ctl->gmin[1 - gp] = kAlive;
       ts = wall_clock64();
@@ -934,8 +934,9 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
                            const uint32_t *alive0, uint32_t na0, uint32_t cshift,
-                           unsigned grid, hipStream_t s) {
+                           uint32_t max_level, unsigned grid, hipStream_t s) {
   if (n >= (1u << 24) || grid < 1 || cshift > 31) return hipErrorInvalidValue;
+  if (max_level >= kParked) max_level = kParked - 1;
   const uint32_t na = alive0 ? na0 : n;
   hipError_t e;
   if ((e = hipMemsetAsync(lv_level, 0, (size_t)n * 4, s)) != hipSuccess) return e;
@@ -957,7 +958,7 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
                   &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
-                  &sdeg, &lv_min, &cshift};
+                  &sdeg, &lv_min, &cshift, &max_level};
   return launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
                      kPeelThreads, args, 0, s);
 }
