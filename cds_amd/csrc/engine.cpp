@@ -44,9 +44,11 @@ struct cds_engine {
   unsigned flow_blocks = 0;
   // Peel in coarse steps of [m, m + m >> peel_shift] and re-peel exactly
   // only the steps the answer depends on: -1 = when k*T >= kCoarseMinSlots,
-  // 0 = never, 1 = always (CDS_PEEL_COARSE, CDS_PEEL_COARSE_SHIFT).
+  // 0 = never, 1 = always (CDS_PEEL_COARSE, CDS_PEEL_COARSE_SHIFT).  Shift 1
+  // (steps [m, 1.5 m]) measured fastest of 1..5 on R-MAT scale 20 k=3,
+  // scale 18 k=3 and scale 17 k=4.
   int peel_coarse = -1;
-  uint32_t peel_shift = 3;
+  uint32_t peel_shift = 1;
 };
 
 struct cds_dgraph {
