@@ -50,10 +50,18 @@ struct Request {
 
 // Device bytes a request may hold while it is solved: CSR and DAG, the
 // listing work items, and clique storage (cliques, incidence, kill levels)
-// for an assumed k * m cliques.  An estimate, used only for admission.
-uint64_t estimate_bytes(uint32_t n, uint64_t m, uint32_t k) {
-  const uint64_t T = (uint64_t)k * m;
+// for T cliques -- at submission an assumed T = k * m.  An estimate, used
+// only for admission.
+uint64_t estimate_bytes(uint32_t n, uint64_t m, uint32_t k, uint64_t T = 0) {
+  if (T == 0) T = (uint64_t)k * m;
   return 16ull * n + 24ull * m + T * (4ull * k + 4ull * k * k + 8ull) + (1ull << 20);
+}
+
+// Once the clique count is known (large lane): the same plus the flow
+// network of a densest core of up to T cliques (int64 residual, member and
+// incidence slot per member; excess, height, two list entries per clique).
+uint64_t estimate_counted_bytes(uint32_t n, uint64_t m, uint32_t k, uint64_t T) {
+  return estimate_bytes(n, m, k, T) + T * (16ull * k + 20ull);
 }
 
 }  // namespace
@@ -76,6 +84,7 @@ struct cds_server {
   cds_ticket next_id = 1;
   bool stop = false;
   uint64_t in_flight_bytes = 0;  // admitted and not yet completed
+  uint64_t reserve_bytes = 0;    // large lane waiting to grow its admission
   cds_server_stats stats;
 };
 
@@ -120,11 +129,13 @@ void serve(cds_server *s, Lane *L) {
         if (L->queue.empty() || batch.size() >= cap) break;
         std::shared_ptr<Request> r = L->queue.front();  // may be cancelled while we wait
         if (s->opts.hbm_budget_bytes &&
-            s->in_flight_bytes + bytes + r->bytes > s->opts.hbm_budget_bytes) {
+            s->in_flight_bytes + s->reserve_bytes + bytes + r->bytes >
+                s->opts.hbm_budget_bytes) {
           if (!batch.empty()) break;  // the rest waits for this batch
           // nothing admitted yet: wait until the other lane frees HBM
           s->cv_budget.wait(lk, [&] {
-            return s->stop || s->in_flight_bytes + r->bytes <= s->opts.hbm_budget_bytes;
+            return s->stop || s->in_flight_bytes + s->reserve_bytes + r->bytes <=
+                                  s->opts.hbm_budget_bytes;
           });
           if (s->stop) break;
           continue;
@@ -166,6 +177,44 @@ void serve(cds_server *s, Lane *L) {
       if (st != CDS_OK) {
         complete(s, r, st, cds_last_error());
         continue;
+      }
+      if (!L->batched && s->opts.hbm_budget_bytes) {
+        // A large graph's clique count can exceed the submission estimate
+        // by orders of magnitude (dense cores): count exactly, then refuse
+        // it or grow its admission before solving.
+        uint64_t T = 0;
+        cds_status cs = cds_count_cliques(e, dgs[i], r->params.k, &T, nullptr, nullptr);
+        uint64_t need = 0;
+        if (cs == CDS_OK) {
+          need = estimate_counted_bytes(r->n, g.m, r->params.k, T);
+          if (need > s->opts.hbm_budget_bytes)
+            cs = fail(CDS_ERR_OOM, "request needs ~%llu bytes for %llu cliques, budget %llu",
+                      (unsigned long long)need, (unsigned long long)T,
+                      (unsigned long long)s->opts.hbm_budget_bytes);
+        }
+        if (cs != CDS_OK) {
+          complete(s, r, cs, cds_last_error());
+          continue;
+        }
+        if (need > r->bytes) {
+          std::unique_lock<std::mutex> lk(s->mu);
+          const uint64_t extra = need - r->bytes;
+          s->reserve_bytes += extra;
+          s->cv_budget.wait(lk, [&] {
+            return s->stop ||
+                   s->in_flight_bytes + extra <= s->opts.hbm_budget_bytes;
+          });
+          s->reserve_bytes -= extra;
+          if (s->stop) {
+            lk.unlock();
+            complete(s, r, CDS_ERR_CANCELLED, "server is stopping");
+            continue;
+          }
+          s->in_flight_bytes += extra;
+          bytes += extra;
+          if (s->in_flight_bytes > s->stats.peak_admitted_bytes)
+            s->stats.peak_admitted_bytes = s->in_flight_bytes;
+        }
       }
       ok.push_back(r);
       gptr.push_back(dgs[i]);

@@ -115,3 +115,30 @@ def test_cancel_and_unknown_ticket(serve_engine):
     with pytest.raises(cds_amd.CdsError):
         srv.poll(123456789)
     srv.close()
+
+
+def test_large_lane_counts_before_admitting(serve_engine):
+    """A graph whose clique count dwarfs the submission estimate (a dense
+    planted community, k = 4) is counted exactly in the large lane and
+    refused when the counted need exceeds the budget."""
+    g = cds_amd.Graph.chung_lu(3000, 2.5, 6.0, 60, 0.95, seed=25)
+    k = 4
+    est = 16 * g.n + 24 * g.m + k * g.m * (4 * k + 4 * k * k + 8) + (1 << 20)
+    eng_T = serve_engine.count_cliques(serve_engine.upload(g), k)[0]
+    counted = (16 * g.n + 24 * g.m + eng_T * (4 * k + 4 * k * k + 8) + (1 << 20)
+               + eng_T * (16 * k + 20))
+    assert counted > est + (1 << 20)
+    srv = cds_amd.Server(serve_engine, hbm_budget_bytes=est + 1, large_vertices=1000)
+    _announce("large-lane request over budget once counted")
+    t = srv.submit(g, k)  # admitted on the estimate
+    with pytest.raises(cds_amd.CdsError) as ei:
+        srv.wait(t, timeout_s=120)
+    assert ei.value.status == 4 and "cliques" in str(ei.value)
+    # the same request fits a budget that covers the counted need
+    srv.close()
+    srv = cds_amd.Server(serve_engine, hbm_budget_bytes=counted + (1 << 20),
+                         large_vertices=1000)
+    got = srv.wait(srv.submit(g, k), timeout_s=120)
+    srv.close()
+    alone = serve_engine.solve(serve_engine.upload(g), k)
+    assert same(got, alone) is None
