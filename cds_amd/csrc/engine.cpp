@@ -1373,7 +1373,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(d_icnt.alloc(nitems));
   HIPCHK(d_iout.alloc(nitems));
   CHK(upload(e, d_items.p, items.data(), nitems));
-  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, kill.p, kpu, d_icnt.p, st));
+  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, core.p, kpu, d_icnt.p, st));
   std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0), hicnt(nitems);
   std::vector<uint64_t> hiout(nitems), hfioff(core_n + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
@@ -1413,7 +1413,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(fioff.alloc((size_t)core_n + 1));
     CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
     HIPCHK(finc.alloc(core_T * k));
-    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, kill.p, kpu,
+    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, core.p, kpu,
                            L.cliques.p, k, cslot.p, finc.p, st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -2043,7 +2043,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(d_icnt.alloc(nitems));
   HIPCHK(d_iout.alloc(nitems));
   CHK(upload(e, d_items.p, items.data(), nitems));
-  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, kill01.p, 1, d_icnt.p, st));
+  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, core01.p, 1, d_icnt.p, st));
   std::vector<uint32_t> hcc(ncomp), hfcd(core_total, 0), hicnt(nitems);
   std::vector<uint64_t> hiout(nitems), hfioff((size_t)core_total + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
@@ -2083,7 +2083,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(fioff.alloc((size_t)core_total + 1));
     CHK(upload(e, fioff.p, hfioff.data(), core_total + 1));
     HIPCHK(finc.alloc(core_T_total * k));
-    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, kill01.p, 1,
+    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, core01.p, 1,
                            L.cliques.p, k, cslot.p, finc.p, st));
     HIPCHK(hipStreamSynchronize(st));
   }

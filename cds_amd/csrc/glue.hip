@@ -241,17 +241,26 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
 // thousand entries, so hubs spread over many waves.  Pass 1 counts the
 // core cliques of each item; pass 2 writes each item's flow slots at a
 // host-scanned offset, in incidence order.
+__device__ __forceinline__ bool entry_in_core(const uint32_t *__restrict__ en,
+                                              uint32_t ik,
+                                              const uint32_t *__restrict__ core,
+                                              uint32_t kp) {
+  bool in = true;
+  for (uint32_t t = 1; t < ik; ++t) in &= core[en[t]] >= kp;
+  return in;
+}
+
 __global__ __launch_bounds__(256) void k_item_core_count(
     uint64_t nitems, const uint4 *__restrict__ items,
     const uint32_t *__restrict__ inc, uint32_t ik,
-    const uint32_t *__restrict__ kill, uint32_t kp, uint32_t *__restrict__ cnt) {
+    const uint32_t *__restrict__ core, uint32_t kp, uint32_t *__restrict__ cnt) {
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
   for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
        it < nitems; it += waves) {
     const uint4 t = items[it];
     const uint64_t lo = ((uint64_t)t.y << 32) | t.z, hi = lo + t.w;
     uint32_t d = 0;
-    for (uint64_t q = lo + lane(); q < hi; q += kWave) d += kill[inc[q * ik]] >= kp;
+    for (uint64_t q = lo + lane(); q < hi; q += kWave) d += entry_in_core(inc + q * ik, ik, core, kp);
     d = wave_sum(d);
     if (lane() == 0) cnt[it] = d;
   }
@@ -260,7 +269,7 @@ __global__ __launch_bounds__(256) void k_item_core_count(
 __global__ __launch_bounds__(256) void k_item_core_write(
     uint64_t nitems, const uint4 *__restrict__ items,
     const uint64_t *__restrict__ item_out, const uint32_t *__restrict__ inc,
-    uint32_t ik, const uint32_t *__restrict__ kill, uint32_t kp,
+    uint32_t ik, const uint32_t *__restrict__ core, uint32_t kp,
     const uint32_t *__restrict__ cl, uint32_t k,
     const uint32_t *__restrict__ cslot, uint32_t *__restrict__ finc) {
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
@@ -276,7 +285,7 @@ __global__ __launch_bounds__(256) void k_item_core_write(
       uint32_t slot = 0;
       if (q < hi) {
         const uint32_t c = inc[q * ik];
-        if (kill[c] >= kp) {
+        if (entry_in_core(inc + q * ik, ik, core, kp)) {
           take = true;
           uint32_t j = 0;
           while (j + 1 < k && cl[(uint64_t)c * k + j] != v) ++j;
@@ -378,23 +387,23 @@ hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
 
 hipError_t item_core_count(uint64_t nitems, const uint4 *items,
                            const uint32_t *inc, uint32_t ik,
-                           const uint32_t *kill, uint32_t kp, uint32_t *cnt,
+                           const uint32_t *core, uint32_t kp, uint32_t *cnt,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_item_core_count,
                      dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
-                     nitems, items, inc, ik, kill, kp, cnt);
+                     nitems, items, inc, ik, core, kp, cnt);
   return hipGetLastError();
 }
 
 hipError_t item_core_write(uint64_t nitems, const uint4 *items,
                            const uint64_t *item_out, const uint32_t *inc,
-                           uint32_t ik, const uint32_t *kill, uint32_t kp,
+                           uint32_t ik, const uint32_t *core, uint32_t kp,
                            const uint32_t *cliques, uint32_t k,
                            const uint32_t *cslot, uint32_t *finc,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_item_core_write,
                      dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
-                     nitems, items, item_out, inc, ik, kill, kp, cliques, k,
+                     nitems, items, item_out, inc, ik, core, kp, cliques, k,
                      cslot, finc);
   return hipGetLastError();
 }
