@@ -1187,7 +1187,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                           T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
                           pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
                           pioff.p, pinc.p, rout.p, partial.p, gbar.p, rgrid,
-                          sdeg.p, A, tm.ev[23], st));
+                          sdeg.p, A, tm.ev[23], (uint64_t)PS.kp,
+                          replay_nstar(k, (uint64_t)PS.kp), st));
       gate.reset();
       ReplayOut ro;
       CHK(readback(e, &ro, rout.p, 1));
@@ -1197,7 +1198,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         R->t_replay_prep_ms += prep;
       }
       R->replay_phases++;
-      if (ro.steps != M)
+      if (ro.steps > M || (!ro.stopped && ro.steps != M))
         return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
                     lv.level, ro.steps, (unsigned long long)M);
       replayed.push_back({lv.level, ro});

@@ -162,6 +162,7 @@ unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device);
 struct ReplayOut {
   unsigned long long best_num, best_den;  // max intermediate density
   uint32_t steps, found;
+  uint32_t stopped, pad;  // 1: ended early, the rows left cannot exceed kp
 };
 // One phase of replay_phases_multi: level, its vertices at positions
 // [base, base + M) of the shared position arrays, and the state C_level.
@@ -199,7 +200,14 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
                         uint32_t *partial, GridBarrier *bar, unsigned grid,
                         const uint32_t *sdeg, DeviceArena &arena,
-                        hipEvent_t prepared, hipStream_t s);
+                        hipEvent_t prepared, uint64_t kp, uint64_t nstar,
+                        hipStream_t s);
+// Early end of a whole-GPU replay (nstar = 0: none).  Once T cliques
+// remain on N vertices, every later row holds at most T cliques and at most
+// C(N'-1, k-1)/k density on its N' < N vertices; nstar is the least N' with
+// C(N'-1, k-1) > k * kp, so the replay may stop when N <= nstar or
+// T <= kp * nstar: no later row can exceed kp.
+uint64_t replay_nstar(uint32_t k, uint64_t kp);
 constexpr uint32_t kReplayGridMaxVertices = 16384;
 // Workgroups of the whole-GPU replay (half the CUs), 0 if unavailable.
 // Workgroups for the whole-GPU replay of a phase whose removals touch about

@@ -680,6 +680,7 @@ __device__ void replay_one(uint32_t k, uint32_t level, uint32_t base,
     out->best_den = best_den;
     out->steps = steps;
     out->found = found;
+    out->stopped = 0;
   }
 }
 
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_grid(
     const uint32_t *__restrict__ core, const uint32_t *__restrict__ kill,
     const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
     uint32_t *rdead, uint32_t *partial, GridBarrier *bar, uint64_t T_start,
-    uint64_t N_start, ReplayOut *out) {
+    uint64_t N_start, ReplayOut *out, uint64_t kp, uint64_t nstar) {
   __shared__ uint32_t dec[kReplayLdsCap];
   __shared__ uint64_t s_wmin[kReplayThreads / kWave];
   __shared__ uint64_t s_key;
@@ -803,6 +804,8 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_grid(
           best_den = den;
           found = 1;
         }
+        // no later row can exceed kp: end after this removal
+        if (nstar && (den <= nstar || num <= kp * nstar)) out->stopped = 1;
       }
     }
     // v's whole incidence list, one contiguous slice per workgroup; the
@@ -837,6 +840,8 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_grid(
     }
     if (boss) palive[iv] = 0;
     grid_barrier(bar);
+    if (__hip_atomic_load(&out->stopped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;  // the whole grid reads the same flag after the barrier
   }
   if (boss) {
     out->best_num = best_num;
@@ -972,9 +977,12 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
                         uint32_t *partial, GridBarrier *bar, unsigned grid,
                         const uint32_t *sdeg, DeviceArena &arena,
-                        hipEvent_t prepared, hipStream_t s) {
+                        hipEvent_t prepared, uint64_t kp, uint64_t nstar,
+                        hipStream_t s) {
   if (grid && M > kReplayLdsCap) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(pcount, 0, sizeof(uint32_t), s);
+  if (e != hipSuccess) return e;
+  if ((e = hipMemsetAsync(out, 0, sizeof(ReplayOut), s)) != hipSuccess) return e;
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_collect_level, dim3(grid_for(n, 256, 8192)), dim3(256),
                      0, s, n, level, core, pv, posmap, pcount);
@@ -993,7 +1001,7 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
     if (prepared) (void)hipEventRecord(prepared, s);
     void *args[] = {&k, &level, &M, &pv, &pdeg, (void *)&posmap, &palive,
                     (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
-                    &rdead, &partial, &bar, &T_start, &N_start, &out};
+                    &rdead, &partial, &bar, &T_start, &N_start, &out, &kp, &nstar};
     return launch_grid(reinterpret_cast<const void *>(k_replay_grid), grid,
                        kReplayThreads, args, 0, s);
   }
@@ -1064,6 +1072,19 @@ unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device) {
     if (v >= 1 && (uint64_t)v <= cap) want = (uint64_t)v;
   }
   return (unsigned)want;
+}
+
+uint64_t replay_nstar(uint32_t k, uint64_t kp) {
+  if (k < 2) return 0;
+  const double lim = (double)k * (double)kp;
+  // C(N-1, k-1) for N = k, k+1, ...: C(m, r) grows as m increases
+  double c = 1;  // C(k-1, k-1)
+  for (uint64_t N = k; N < (1ull << 32); ++N) {
+    if (c > lim) return N;
+    const double m = (double)(N - 1);  // C(m+1, r) = C(m, r) (m+1) / (m+1-r)
+    c = c * (m + 1) / (m + 1 - (double)(k - 1));
+  }
+  return 0;
 }
 
 unsigned replay_grid_blocks(int device, uint64_t entries_per_step) {
