@@ -1301,7 +1301,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const size_t na = g->host_off[n];
     HIPCHK(mark.alloc(na));
     HIPCHK(mark.zero());
-    HIPCHK(prune_mark(T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st));
+    if (core_n <= kDenseCoreMax) {
+      DBuf<uint32_t> cidx(&A), ccount(&A), cmat(&A);
+      HIPCHK(cidx.alloc(n));
+      HIPCHK(ccount.alloc(1));
+      HIPCHK(cmat.alloc((size_t)core_n * core_n));
+      HIPCHK(prune_mark_dense(T, k, L.cliques.p, kill.p, kpu, n, core.p, g->off, g->adj,
+                              core_n, cidx.p, ccount.p, cmat.p, mark.p, st));
+      uint32_t counted = 0;
+      CHK(readback(e, &counted, ccount.p, 1));
+      if (counted != core_n)
+        return fail(CDS_ERR_INTERNAL, "dense prune indexed %u of %u core vertices", counted,
+                    core_n);
+    } else {
+      HIPCHK(prune_mark(T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st));
+    }
     HIPCHK(count_nonzero_u8(na, mark.p, acc.p, st));
     unsigned long long pe = 0;
     CHK(readback(e, &pe, acc.p, 1));

@@ -128,6 +128,69 @@ __global__ void k_prune_mark(uint64_t T, uint32_t k,
   }
 }
 
+// Dense prune for a small densest core: cidx numbers the core vertices,
+// cmat[i * cn + j] holds the CSR position of the edge between core
+// vertices i and j (or kNoEdge), so every ordered member pair of a core
+// clique is one lookup in a matrix that stays in cache instead of a binary
+// search through a hub's adjacency list.
+constexpr uint32_t kNoEdge = 0xFFFFFFFFu;
+
+__global__ void k_core_index(uint32_t n, const uint32_t *__restrict__ core,
+                             uint32_t kp, uint32_t *__restrict__ cidx,
+                             uint32_t *__restrict__ count) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t v = base + threadIdx.x;
+    const bool in = v < n && core[v] >= kp;
+    const uint32_t slot = aggregated_reserve(count, 0, in);
+    if (v < n) cidx[v] = in ? slot : kNoEdge;
+  }
+}
+
+__global__ void k_fill_edges(uint64_t n, uint32_t *__restrict__ a) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = kNoEdge;
+}
+
+__global__ void k_core_edge_matrix(uint32_t n, const uint32_t *__restrict__ off,
+                                   const uint32_t *__restrict__ adj,
+                                   const uint32_t *__restrict__ cidx, uint32_t cn,
+                                   uint32_t *__restrict__ cmat) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    const uint32_t iu = cidx[u];
+    if (iu >= cn) continue;  // not a core vertex (wave-uniform)
+    for (uint32_t p = off[u] + lane(); p < off[u + 1]; p += kWave) {
+      const uint32_t iv = cidx[adj[p]];
+      if (iv < cn) cmat[(uint64_t)iu * cn + iv] = p;
+    }
+  }
+}
+
+__global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
+                                   const uint32_t *__restrict__ cl,
+                                   const uint32_t *__restrict__ kill, uint32_t kp,
+                                   const uint32_t *__restrict__ cidx, uint32_t cn,
+                                   const uint32_t *__restrict__ cmat,
+                                   uint8_t *__restrict__ mark) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] < kp) continue;
+    for (uint32_t i = 0; i < k; ++i) {
+      const uint32_t ci = cidx[cl[c * k + i]];
+      if (ci >= cn) continue;  // cannot happen for a core clique
+      for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t cj = cidx[cl[c * k + j]];
+        if (i == j || cj >= cn) continue;
+        const uint32_t pos = cmat[(uint64_t)ci * cn + cj];
+        if (pos != kNoEdge) mark[pos] = 1;
+      }
+    }
+  }
+}
+
 __global__ void k_count_nonzero_u8(size_t n, const uint8_t *__restrict__ a,
                                    unsigned long long *out) {
   unsigned long long c = 0;
@@ -340,6 +403,26 @@ hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s) {
   hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 65535)), dim3(256), 0,
                      s, n, parent);
+  return hipGetLastError();
+}
+
+hipError_t prune_mark_dense(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *kill, uint32_t kp, uint32_t n,
+                            const uint32_t *core, const uint32_t *off,
+                            const uint32_t *adj, uint32_t cn, uint32_t *cidx,
+                            uint32_t *count, uint32_t *cmat, uint8_t *mark,
+                            hipStream_t s) {
+  hipError_t e = hipMemsetAsync(count, 0, 4, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_core_index, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
+                     core, kp, cidx, count);
+  const uint64_t cells = (uint64_t)cn * cn;
+  hipLaunchKernelGGL(k_fill_edges, dim3(grid_for(cells, 256, 16384)), dim3(256), 0, s,
+                     cells, cmat);
+  hipLaunchKernelGGL(k_core_edge_matrix, dim3(grid_for((size_t)n * kWave, 256, 8192)),
+                     dim3(256), 0, s, n, off, adj, cidx, cn, cmat);
+  hipLaunchKernelGGL(k_prune_mark_dense, dim3(grid_for(T, 256, 65535)), dim3(256), 0, s,
+                     T, k, cliques, kill, kp, cidx, cn, cmat, mark);
   return hipGetLastError();
 }
 

@@ -226,6 +226,16 @@ hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
                           const uint8_t *mark, const uint32_t *core,
                           uint32_t kp, uint32_t *parent, hipStream_t s);
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s);
+// Same marks for a densest core of cn <= kDenseCoreMax vertices (those
+// with core >= kp) through a cn x cn matrix of edge positions; cidx (n),
+// count (1) and cmat (cn * cn) are scratch.
+constexpr uint32_t kDenseCoreMax = 4096;
+hipError_t prune_mark_dense(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *kill, uint32_t kp, uint32_t n,
+                            const uint32_t *core, const uint32_t *off,
+                            const uint32_t *adj, uint32_t cn, uint32_t *cidx,
+                            uint32_t *count, uint32_t *cmat, uint8_t *mark,
+                            hipStream_t s);
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *kill, uint32_t kp, const uint32_t *off,
                       const uint32_t *adj, uint8_t *mark, hipStream_t s);
