@@ -605,7 +605,12 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
     delete e;
     return fail(CDS_ERR_HIP, "engine init failed: %s", hipGetErrorString(r));
   }
-  e->arena = new DeviceArena(e->stream);
+  {
+    // large blocks are cached up to 40 % of the device's memory
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) total_b = 0;
+    e->arena = new DeviceArena(e->stream, total_b / 10 * 4);
+  }
   // Freed stream-ordered memory stays in the device pool between solves:
   // returning ~100 GB to the driver after a large solve and mapping it
   // again for the next one cost up to 1.5 s of wall time per solve.
@@ -634,7 +639,8 @@ extern "C" void cds_engine_destroy(cds_engine *e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  delete e->arena;
+  delete e->arena;  // frees its cached blocks on the stream
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
   {
     // hand the pool's cached memory back (other engines on the device
     // re-grow it on demand)

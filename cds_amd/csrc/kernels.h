@@ -12,20 +12,37 @@ namespace cds {
 
 constexpr uint32_t kAliveHost = 0xFFFFFFFFu;  // same value as kAlive (dev.h)
 
-// Stream-ordered device allocations (hipMallocAsync on the engine's pool).
+// Stream-ordered device allocations (hipMallocAsync on the device pool).
+// Blocks of kArenaCacheMin bytes or more are kept on release, up to
+// cache_cap bytes, and handed out again to a request of at least half
+// their size: freeing and re-mapping tens of GB per solve cost seconds
+// (hipFreeAsync of a 13 GB block blocked the host for 1.2-1.8 s).  All
+// use is on the arena's one stream, so a reused block is stream-ordered.
+// A failed allocation empties the cache and retries once.
+constexpr size_t kArenaCacheMin = 64ull << 20;
 class DeviceArena {
  public:
-  explicit DeviceArena(hipStream_t s) : stream_(s) {}
+  explicit DeviceArena(hipStream_t s, size_t cache_cap = 0)
+      : stream_(s), cache_cap_(cache_cap) {}
+  ~DeviceArena();
   hipError_t alloc(void **p, size_t bytes);
   void release(void *p);
+  void trim();  // frees every cached block
   size_t live_bytes() const { return live_; }
   size_t peak_bytes() const { return peak_; }
+  size_t cached_bytes() const { return cached_; }
   hipStream_t stream() const { return stream_; }
 
  private:
+  struct Block {
+    void *p;
+    size_t size;  // bytes requested
+    size_t real;  // bytes of the underlying allocation
+  };
   hipStream_t stream_;
-  size_t live_ = 0, peak_ = 0;
-  std::vector<std::pair<void *, size_t>> sizes_;
+  size_t cache_cap_;
+  size_t live_ = 0, peak_ = 0, cached_ = 0;
+  std::vector<Block> live_blocks_, cache_;
 };
 
 // ---- scan / reduce (scan.hip) ----

@@ -11,23 +11,61 @@
 
 namespace cds {
 
+DeviceArena::~DeviceArena() { trim(); }
+
+void DeviceArena::trim() {
+  for (const Block &b : cache_) (void)hipFreeAsync(b.p, stream_);
+  cache_.clear();
+  cached_ = 0;
+}
+
 hipError_t DeviceArena::alloc(void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
-  hipError_t e = hipMallocAsync(p, bytes, stream_);
-  if (e != hipSuccess) return e;
+  size_t real = bytes;
+  *p = nullptr;
+  if (bytes >= kArenaCacheMin) {
+    // smallest cached block that fits and is not more than twice as large
+    size_t best = cache_.size();
+    for (size_t i = 0; i < cache_.size(); ++i)
+      if (cache_[i].real >= bytes && cache_[i].real / 2 <= bytes &&
+          (best == cache_.size() || cache_[i].real < cache_[best].real))
+        best = i;
+    if (best < cache_.size()) {
+      *p = cache_[best].p;
+      real = cache_[best].real;
+      cached_ -= real;
+      cache_[best] = cache_.back();
+      cache_.pop_back();
+    }
+  }
+  if (!*p) {
+    hipError_t e = hipMallocAsync(p, bytes, stream_);
+    if (e != hipSuccess && !cache_.empty()) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMallocAsync(p, bytes, stream_);
+    }
+    if (e != hipSuccess) return e;
+  }
   live_ += bytes;
   if (live_ > peak_) peak_ = live_;
-  sizes_.emplace_back(*p, bytes);
+  live_blocks_.push_back({*p, bytes, real});
   return hipSuccess;
 }
 
 void DeviceArena::release(void *p) {
   if (!p) return;
-  for (size_t i = 0; i < sizes_.size(); ++i)
-    if (sizes_[i].first == p) {
-      live_ -= sizes_[i].second;
-      sizes_[i] = sizes_.back();
-      sizes_.pop_back();
+  for (size_t i = 0; i < live_blocks_.size(); ++i)
+    if (live_blocks_[i].p == p) {
+      const Block b = live_blocks_[i];
+      live_ -= b.size;
+      live_blocks_[i] = live_blocks_.back();
+      live_blocks_.pop_back();
+      if (b.real >= kArenaCacheMin && cached_ + b.real <= cache_cap_) {
+        cache_.push_back(b);
+        cached_ += b.real;
+        return;
+      }
       break;
     }
   (void)hipFreeAsync(p, stream_);
