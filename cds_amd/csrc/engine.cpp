@@ -986,6 +986,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // persistent-peel state, kept for the exact re-peels of coarse steps
   DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A), ralive(&A), rcount(&A);
   DBuf<uint64_t> s64(&A);
+  DBuf<uint32_t> stamp(&A);
+  uint32_t sbase = 1;  // first sub-round of the next persistent launch
   DBuf<unsigned long long> lrem(&A), lkil(&A);
   DBuf<PeelCtl> ctl(&A);
   const bool coarse_peel =
@@ -1006,14 +1008,18 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     {
       CoopGate gate(e, grid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
-      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
-                             s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p, sdeg.p,
+      HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
+                             stamp.p, sbase, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
+                             lkil.p, sdeg.p,
                              alive0, na0, shift, max_level, grid, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
     }
     R->t_peel_kernel_ms += tm.ms(22, 23);
     R->peel_launches++;
     CHK(readback(e, hc, ctl.p, 1));
+    if (hc->rounds >= (1ull << 30) || sbase + hc->rounds + 1 >= (1u << 30))
+      return fail(CDS_ERR_INTERNAL, "persistent peel: %llu sub-rounds", hc->rounds);
+    sbase += (uint32_t)hc->rounds + 1;
     if (hc->removed != want_removed || hc->nlevels == 0 || hc->nlevels > n)
       return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %llu (%u levels)",
                   hc->removed, (unsigned long long)want_removed, hc->nlevels);
@@ -1047,6 +1053,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
     HIPCHK(sdeg.alloc(n));
+    HIPCHK(stamp.alloc(n));
     const unsigned grid = peel_grid_blocks(n, T, k, e->device);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     PeelCtl hc;
@@ -1819,12 +1826,15 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(lrem.alloc(n));
     HIPCHK(lkil.alloc(n));
     HIPCHK(ctl.alloc(1));
+    DBuf<uint32_t> stamp(&A);
+    HIPCHK(stamp.alloc(n));
     peel_grid = peel_grid_blocks(n, T, k, e->device);
     if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     {
       CoopGate gate(e, peel_grid);
-      HIPCHK(peel_persistent(n, k, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
-                             kill.p, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p, lkil.p,
+      HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
+                             kill.p, stamp.p, 1, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
+                             lkil.p,
                              sdeg.p, nullptr, 0, 0, kAliveHost, peel_grid, st));
     }
     PeelCtl hc;

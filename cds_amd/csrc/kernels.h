@@ -149,11 +149,17 @@ struct PeelCtl {
 // alive0 (na0 vertices, or null for all n): the vertices alive at the
 // start; every other vertex must have core != kAlive and every clique
 // containing one must be dead or never reached.
+// kill[] (T words): every clique alive (kAlive) before the launch is set
+// after it to the level that killed it.  stamp[] (n words) holds each
+// vertex's sub-round stamp; sbase is the launch's first sub-round: 1 for the
+// first launch on a graph, above every stamp of earlier launches (their
+// sub-round counts + 1) for a re-peel.
 // Requires n < 2^24.  Synchronises nothing; read ctl after the stream.
-hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
+hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
                            uint32_t *deg, uint32_t *core, uint32_t *kill,
-                           uint32_t *scratch32, uint64_t *scratch64,
+                           uint32_t *stamp, uint32_t sbase, uint32_t *scratch32,
+                           uint64_t *scratch64,
                            PeelCtl *ctl, uint32_t *lv_level, uint32_t *lv_min,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,

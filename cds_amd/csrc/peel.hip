@@ -191,6 +191,20 @@ __global__ __launch_bounds__(256) void k_peel_process(
 // appended vertex (vertex count << 40 | chunk count) gives every vertex its
 // slot and its first chunk, so chunk i maps to its vertex by a binary search
 // and a hub's incidence list is spread over many waves.
+// Cliques are claimed without per-clique state.  Every vertex gets a stamp,
+// the sub-round whose frontier it joins (stamps grow over the launch).  An
+// entry (v; c, other members) processed in sub-round r is dead if another
+// member has a stamp below r (c was claimed then), and is claimed by v iff
+// no other member with stamp r has a smaller id, so exactly one member of
+// each frontier claims each live clique.  Stamps written during sub-round r
+// (r + 1, for vertices joining the next frontier) compare like unstamped
+// ones, so the race with them decides nothing.  The stamp reads are k-1
+// random 4-byte loads into an n-word array that stays in cache; the CAS on
+// a T-word kill array they replace cost one HBM line per entry (measured 57
+// bytes of traffic per entry at R-MAT scale 20, k = 3; a one-bit-per-clique
+// set with atomicOr was slower still, consecutive clique ids of a hub
+// serialising on one word).  Kill words are derived after the launch
+// (k_kill_from_core).
 constexpr uint32_t kPeelChunk = 64;     // one wave pass per chunk
 constexpr int kPeelThreads = 1024;
 constexpr uint64_t kChunkMask = (1ull << 40) - 1;
@@ -228,7 +242,8 @@ __device__ __forceinline__ void frontier_push(PeelCtl *ctl, int b, uint32_t *F,
 __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     uint32_t n, uint32_t k, const uint64_t *__restrict__ ioff,
     const uint32_t *__restrict__ inc, const uint32_t *__restrict__ cl,
-    uint32_t *deg, uint32_t *core, uint32_t *kill, uint32_t *A0, uint32_t *A1,
+    uint32_t *deg, uint32_t *core, uint32_t *stamp, uint32_t sbase, uint32_t *A0,
+    uint32_t *A1,
     uint32_t *F0, uint32_t *F1, uint32_t *F2, uint64_t *FC0, uint64_t *FC1,
     uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
     unsigned long long *lv_removed, unsigned long long *lv_killed,
@@ -259,6 +274,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
   const bool leader = gtid == 0;
   int ab = 0, gp = 0, fb = 0;
   uint32_t lvl_idx = 0;
+  uint32_t sr = sbase;  // current sub-round, identical in every thread
   for (;;) {
     // (1) minimum degree among alive vertices
     unsigned long long ts = leader ? wall_clock64() : 0;
@@ -350,6 +366,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
       __syncthreads();
       if (take) {
         core[v] = level;
+        stamp[v] = sr;
         const uint32_t slot = (uint32_t)(sh_fbase >> 40) + sh_fn[wid] + fcnt - 1;
         F[fb][slot] = v;
         FC[fb][slot] = (sh_fbase & kChunkMask) + sh_fc[wid] + fch -
@@ -387,9 +404,13 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
         const uint64_t e = min(ioff[v + 1], b + kPeelChunk);
         for (uint64_t q = b + lane(); q < e; q += kWave) {
           const uint32_t *en = inc + q * k;
-          const uint32_t c = en[0];
-          if (ld(&kill[c]) != kAlive) continue;  // plain load before the CAS
-          if (atomicCAS(&kill[c], kAlive, level) != kAlive) continue;
+          bool claim = true;
+          for (uint32_t t = 1; t < k; ++t) {
+            const uint32_t u = en[t];
+            const uint32_t su = ld(&stamp[u]);
+            claim = claim && su >= sr && (su != sr || u > v);
+          }
+          if (!claim) continue;
           ++nk;
           for (uint32_t t = 1; t < k; ++t) {
             const uint32_t u = en[t];
@@ -401,6 +422,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
             const uint32_t old = atomicSub(&deg[u], 1u);
             if (old == level + 1) {
               core[u] = level;
+              stamp[u] = sr + 1;
               frontier_push(ctl, nb, F[nb], FC[nb], u,
                             chunks_of(ioff[u + 1] - ioff[u]));
             }
@@ -422,6 +444,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
           const uint32_t old = atomicSub(&deg[u], cnt);
           if (old > level && old - cnt <= level) {
             core[u] = level;
+            stamp[u] = sr + 1;
             frontier_push(ctl, nb, F[nb], FC[nb], u,
                           chunks_of(ioff[u + 1] - ioff[u]));
           }
@@ -439,6 +462,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
       }
       grid_sync(ctl);
       fb = nb;
+      ++sr;
       if (leader) ctl->rounds++;
     }
     // The last sub-round's barrier already ordered every update of this
@@ -494,6 +518,36 @@ __global__ void k_refine_restore(uint32_t n, uint32_t *__restrict__ core,
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x)
     if (core[v] == kParked) core[v] = deg[v];
+}
+
+// Stamps before a launch: vertices the launch may meet alive (alive, or
+// parked in a later coarse step) compare as never removed; vertices removed
+// by earlier launches keep stamps below the launch's first sub-round.
+__global__ void k_stamp_reset(uint32_t n, const uint32_t *__restrict__ core,
+                              uint32_t *__restrict__ stamp, uint32_t first) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x) {
+    const uint32_t c = core[v];
+    if (c == kAlive || c == kParked) stamp[v] = kAlive;
+    else if (first) stamp[v] = 0;
+  }
+}
+
+// After a persistent peel: every clique still marked alive in kill[] was
+// claimed during the launch, at the level its first member was removed,
+// which is the smallest core value among its members (parked vertices of
+// later steps carry kParked, above every level).
+__global__ void k_kill_from_core(uint64_t T, uint32_t k,
+                                 const uint32_t *__restrict__ cl,
+                                 const uint32_t *__restrict__ core,
+                                 uint32_t *__restrict__ kill) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    if (kill[c] != kAlive) continue;
+    uint32_t m = kAlive;
+    for (uint32_t j = 0; j < k; ++j) m = min(m, core[cl[c * k + j]]);
+    kill[c] = m;
+  }
 }
 
 __global__ void k_iota(uint32_t n, uint32_t *a) {
@@ -937,10 +991,11 @@ hipError_t refine_restore(uint32_t n, uint32_t *core, const uint32_t *deg,
   return hipGetLastError();
 }
 
-hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
+hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
                            uint32_t *deg, uint32_t *core, uint32_t *kill,
-                           uint32_t *scratch32, uint64_t *scratch64,
+                           uint32_t *stamp, uint32_t sbase, uint32_t *scratch32,
+                           uint64_t *scratch64,
                            PeelCtl *ctl, uint32_t *lv_level, uint32_t *lv_min,
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
@@ -966,12 +1021,21 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, const uint64_t *ioff,
   }
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, na);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (sbase < 1 || sbase > kParked / 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_stamp_reset, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
+                     core, stamp, (uint32_t)(sbase == 1));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
-                  &deg, &core, &kill, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
+                  &deg, &core, &stamp, &sbase, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
                   &sdeg, &lv_min, &cshift, &max_level};
-  return launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
-                     kPeelThreads, args, 0, s);
+  if ((e = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
+                       kPeelThreads, args, 0, s)) != hipSuccess)
+    return e;
+  if (!T) return hipSuccess;
+  hipLaunchKernelGGL(k_kill_from_core, dim3(grid_for(T, 256, 16384)), dim3(256), 0, s,
+                     T, k, cliques, core, kill);
+  return hipGetLastError();
 }
 
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
