@@ -103,7 +103,29 @@ class CResult(ctypes.Structure):
 
 class COpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64),
-                ("profile_kernels", ctypes.c_uint32)]
+                ("profile_kernels", ctypes.c_uint32), ("tune_cache", ctypes.c_char_p)]
+
+
+TUNING_FIELDS = ("list_threads", "list_chunk", "peel_blocks", "peel_coarse_shift",
+                 "replay_blocks", "flow_blocks")
+
+
+class CTuning(ctypes.Structure):
+    _fields_ = [("list_threads", ctypes.c_uint32), ("list_chunk", ctypes.c_uint32),
+                ("peel_blocks", ctypes.c_uint32), ("peel_coarse_shift", ctypes.c_int32),
+                ("replay_blocks", ctypes.c_uint32), ("flow_blocks", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in TUNING_FIELDS}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls(0, 0, 0, -1, 0, 0)
+        for f, v in d.items():
+            if f not in TUNING_FIELDS:
+                raise ValueError(f"unknown tuning field {f}")
+            setattr(t, f, int(v))
+        return t
 
 
 class CServerOpts(ctypes.Structure):
@@ -150,6 +172,11 @@ _SIGNATURES = {
     "cds_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(CResult), ctypes.POINTER(CRequestTimes)]),
     "cds_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "cds_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerStats)]),
+    "cds_engine_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CTuning)]),
+    "cds_engine_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(CTuning)]),
+    "cds_engine_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "cds_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.c_char_p, ctypes.POINTER(CTuning), f64p, f64p]),
+    "cds_tuning_lookup": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(CTuning)]),
     "cds_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
     "cds_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(CGraph)]),
 }
@@ -169,7 +196,7 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        for i, cls in enumerate((CGraph, CParams, CResult, COpts)):
+        for i, cls in enumerate((CGraph, CParams, CResult, COpts, CTuning)):
             if L.cds_struct_size(i) != ctypes.sizeof(cls):
                 raise EngineUnavailable(
                     f"ABI mismatch for {cls.__name__}: C {L.cds_struct_size(i)}"
@@ -181,6 +208,17 @@ def lib():
 def _check(status):
     if status != 0:
         raise CdsError(status, lib().cds_last_error().decode())
+
+
+def tuning_lookup(path, arch, cus, k, m):
+    """cds_tuning_lookup: the cache entry nearest to (arch, cus, k, m), as a
+    dict, or None if no entry is within 2 edge classes."""
+    t = CTuning()
+    st = lib().cds_tuning_lookup(os.fsencode(path), arch.encode(), cus, k, m, ctypes.byref(t))
+    if st == 1:  # CDS_ERR_INVALID: no entry near enough
+        return None
+    _check(st)
+    return t.as_dict()
 
 
 class Graph:
@@ -311,11 +349,38 @@ def result_to_dict(r, with_json=False):
 class Engine:
     """One engine per GPU (cds_engine): owns a HIP stream and device memory."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, profile_kernels=False):
-        opts = COpts(device, hbm_budget_bytes, 1 if profile_kernels else 0)
+    def __init__(self, device=0, hbm_budget_bytes=0, profile_kernels=False,
+                 tune_cache=None):
+        opts = COpts(device, hbm_budget_bytes, 1 if profile_kernels else 0,
+                     os.fsencode(tune_cache) if tune_cache else None)
         h = ctypes.c_void_p()
         _check(lib().cds_engine_create(ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
+
+    def set_tuning(self, tuning=None):
+        """Default launch shapes (dict of TUNING_FIELDS; None: built-in rules)."""
+        t = CTuning.from_dict(tuning) if tuning is not None else None
+        _check(lib().cds_engine_set_tuning(self._h, ctypes.byref(t) if t else None))
+
+    def get_tuning(self, k, m):
+        t = CTuning()
+        _check(lib().cds_engine_get_tuning(self._h, k, m, ctypes.byref(t)))
+        return t.as_dict()
+
+    def load_tuning(self, path):
+        n = ctypes.c_uint32()
+        _check(lib().cds_engine_load_tuning(self._h, os.fsencode(path), ctypes.byref(n)))
+        return n.value
+
+    def autotune(self, dgraph, k, path=None):
+        """cds_autotune on one graph; returns (tuning dict, best ms, first ms)."""
+        p = CParams(k, 8000, 16000, 60000, 30000, -1, 1, 1, 0)
+        t = CTuning()
+        best, base = ctypes.c_double(), ctypes.c_double()
+        _check(lib().cds_autotune(self._h, dgraph._h, ctypes.byref(p),
+                                  os.fsencode(path) if path else None, ctypes.byref(t),
+                                  ctypes.byref(best), ctypes.byref(base)))
+        return t.as_dict(), best.value, base.value
 
     def upload(self, graph):
         return DeviceGraph(self, graph)

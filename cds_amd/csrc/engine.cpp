@@ -25,6 +25,7 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "tune.h"
 
 using namespace cds;
 
@@ -49,7 +50,38 @@ struct cds_engine {
   // scale 18 k=3 and scale 17 k=4.
   int peel_coarse = -1;
   uint32_t peel_shift = 1;
+  // launch shapes: device identity for the tune cache, the default tuning,
+  // the loaded cache entries, and the tuning of the solve in progress
+  std::string arch;
+  uint32_t cus = 0;
+  cds_tuning deflt{0, 0, 0, -1, 0, 0};
+  std::vector<TuneEntry> tuned;
+  cds_tuning cur{0, 0, 0, -1, 0, 0};
+  bool forced = false;  // autotune sweep: `cur` is set by the sweep
 };
+
+namespace {
+// The tuning a solve of (k, m) uses: the nearest cache entry of k, else the
+// engine default.  Called under the engine lock.
+cds_tuning tuning_for(const cds_engine *e, uint32_t k, uint64_t m) {
+  if (const TuneEntry *t = nearest_entry(e->tuned, e->arch, e->cus, k, m)) return t->t;
+  return e->deflt;
+}
+void begin_solve(cds_engine *e, uint32_t k, uint64_t m) {
+  if (!e->forced) e->cur = tuning_for(e, k, m);
+}
+ListShape list_shape(const cds_engine *e) {
+  ListShape s;
+  if (e->cur.list_threads) {
+    s.threads = e->cur.list_threads;
+    s.chunk = e->cur.list_chunk;
+  }
+  return s;
+}
+unsigned flow_blocks_of(const cds_engine *e) {
+  return e->cur.flow_blocks ? e->cur.flow_blocks : e->flow_blocks;
+}
+}  // namespace
 
 struct cds_dgraph {
   uint32_t n = 0;
@@ -629,7 +661,72 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
     const int v = atoi(ps);
     if (v >= 1 && v <= 16) e->peel_shift = (uint32_t)v;
   }
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) {
+      std::string a = prop.gcnArchName;
+      e->arch = a.substr(0, a.find(':'));
+      e->cus = (uint32_t)prop.multiProcessorCount;
+    }
+    const char *tc = (opts && opts->tune_cache) ? opts->tune_cache : getenv("CDS_TUNE_CACHE");
+    if (tc && *tc) {
+      std::vector<TuneEntry> all;
+      const cds_status ts = read_tune_file(tc, &all, true);
+      if (ts != CDS_OK) {
+        cds_engine_destroy(e);
+        return ts;
+      }
+      for (const TuneEntry &t : all)
+        if (t.arch == e->arch && t.cus == e->cus) e->tuned.push_back(t);
+    }
+  }
   *out = e;
+  return CDS_OK;
+}
+
+namespace cds {
+void copy_tuning(cds_engine *dst, cds_engine *src) {
+  std::lock_guard<std::mutex> lk(src->mu);
+  dst->deflt = src->deflt;
+  dst->tuned = src->tuned;
+}
+}  // namespace cds
+
+extern "C" cds_status cds_engine_set_tuning(cds_engine *e, const cds_tuning *t) {
+  if (!e) return fail(CDS_ERR_INVALID, "null engine");
+  if (t && !tuning_valid(*t)) return fail(CDS_ERR_INVALID, "tuning out of range");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->deflt = t ? *t : cds_tuning{0, 0, 0, -1, 0, 0};
+  return CDS_OK;
+}
+
+extern "C" cds_status cds_engine_get_tuning(cds_engine *e, uint32_t k, uint64_t m,
+                                            cds_tuning *out) {
+  if (!e || !out) return fail(CDS_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(e->mu);
+  *out = tuning_for(e, k, m);
+  return CDS_OK;
+}
+
+extern "C" cds_status cds_engine_load_tuning(cds_engine *e, const char *path,
+                                             uint32_t *loaded) {
+  if (!e || !path) return fail(CDS_ERR_INVALID, "null argument");
+  std::vector<TuneEntry> all;
+  CHK(read_tune_file(path, &all, false));
+  std::lock_guard<std::mutex> lk(e->mu);
+  uint32_t cnt = 0;
+  for (const TuneEntry &t : all) {
+    if (t.arch != e->arch || t.cus != e->cus) continue;
+    bool replaced = false;
+    for (TuneEntry &x : e->tuned)
+      if (x.k == t.k && x.cls == t.cls) {
+        x = t;
+        replaced = true;
+      }
+    if (!replaced) e->tuned.push_back(t);
+    ++cnt;
+  }
+  if (loaded) *loaded = cnt;
   return CDS_OK;
 }
 
@@ -773,15 +870,16 @@ cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
                 kList4LdsCap, L.max_out);
   HIPCHK(L.ipv.alloc(n));
   HIPCHK(L.item_off.alloc((size_t)n + 1));
-  HIPCHK(list_item_counts(n, L.doff.p, L.ipv.p, e->stream));
+  const ListShape shape = list_shape(e);
+  HIPCHK(list_item_counts(n, shape.chunk, L.doff.p, L.ipv.p, e->stream));
   HIPCHK(scan_u32(L.ipv.p, L.item_off.p, n, A));
   CHK(readback(e, &L.items_n, L.item_off.p + n, 1));
   HIPCHK(L.items.alloc(L.items_n));
-  HIPCHK(list_item_fill(n, L.doff.p, L.item_off.p, L.items.p, e->stream));
+  HIPCHK(list_item_fill(n, shape.chunk, L.doff.p, L.item_off.p, L.items.p, e->stream));
   HIPCHK(L.item_count.alloc(L.items_n));
   HIPCHK(L.item_base.alloc(L.items_n + 1));
   int t0 = tm->mark();
-  HIPCHK(list_cliques(k, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+  HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
                       L.item_count.p, L.cd.p, nullptr, nullptr, e->stream));
   int t1 = tm->mark();
   HIPCHK(scan_u32(L.item_count.p, L.item_base.p, L.items_n, A));
@@ -793,7 +891,7 @@ cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
                   (unsigned long long)L.T);
     HIPCHK(L.cliques.alloc(L.T * k));
     int t2 = tm->mark();
-    HIPCHK(list_cliques(k, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+    HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
                         nullptr, nullptr, L.item_base.p, L.cliques.p,
                         e->stream));
     int t3 = tm->mark();
@@ -904,6 +1002,7 @@ extern "C" cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *g,
     return fail(CDS_ERR_UNSUPPORTED, "k=%u: the device lister covers k=2..4", k);
   std::lock_guard<std::mutex> lk(e->mu);
   HIPCHK(hipSetDevice(e->device));
+  begin_solve(e, k, g->m);
   Timer tm(e->stream);
   Listing L(e->arena);
   CHK(run_dag(e, g, L));
@@ -924,6 +1023,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   hipStream_t st = e->stream;
   const uint32_t n = g->n, k = p->k;
   memset(R, 0, sizeof(*R));
+  begin_solve(e, k, g->m);
   R->k = k;
   R->n = n;
   R->m = g->m;
@@ -991,8 +1091,13 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<unsigned long long> lrem(&A), lkil(&A);
   DBuf<PeelCtl> ctl(&A);
   const bool coarse_peel =
-      e->peel_coarse == 1 || (e->peel_coarse == -1 && (uint64_t)k * T >= kCoarseMinSlots);
-  const uint32_t cshift = (n < (1u << 24) && coarse_peel) ? e->peel_shift : 0;
+      e->cur.peel_coarse_shift > 0 ||
+      (e->cur.peel_coarse_shift < 0 &&
+       (e->peel_coarse == 1 || (e->peel_coarse == -1 && (uint64_t)k * T >= kCoarseMinSlots)));
+  const uint32_t cshift =
+      (n < (1u << 24) && coarse_peel)
+          ? (e->cur.peel_coarse_shift > 0 ? (uint32_t)e->cur.peel_coarse_shift : e->peel_shift)
+          : 0;
   R->peel_coarse_shift = cshift;
   int wall_khz = 0;
   if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) !=
@@ -1054,7 +1159,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(ctl.alloc(1));
     HIPCHK(sdeg.alloc(n));
     HIPCHK(stamp.alloc(n));
-    const unsigned grid = peel_grid_blocks(n, T, k, e->device);
+    const unsigned grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     PeelCtl hc;
     CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
@@ -1118,7 +1223,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     // as wide as the first pass: the re-peel is bound by its cliques'
     // kills (measured 127 ms against 157 ms with a grid sized by the
     // step's own clique count on R-MAT scale 20, k = 3)
-    const unsigned grid = peel_grid_blocks(n, T, k, e->device);
+    const unsigned grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     // Exact levels at once: re-peeling a step in narrower steps first and
     // refining those re-processes the step's cliques at every depth, which
@@ -1202,7 +1307,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       // Few vertices with many cliques each: replay on the whole GPU.
       unsigned rgrid = 0;
       if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
-        rgrid = replay_grid_blocks(e->device, lv.killed * k / M);
+        rgrid = replay_grid_blocks(e->device, lv.killed * k / M, e->cur.replay_blocks);
       if (rgrid) {
         HIPCHK(partial.alloc((size_t)rgrid * M));
         HIPCHK(gbar.alloc(1));
@@ -1555,7 +1660,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       G.trace_cap = kFlowTraceRounds;
     }
     {
-      const unsigned fgrid = flow_grid_blocks(P, e->device, e->flow_blocks);
+      const unsigned fgrid = flow_grid_blocks(P, e->device, flow_blocks_of(e));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
       HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, fgrid, st));
@@ -1735,6 +1840,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       U.host_off[vbase[b] + i] = (uint32_t)abase[b] + gs[b]->host_off[i];
   }
   U.host_off[n] = (uint32_t)abase[B];
+  begin_solve(e, k, U.m);
   DBuf<uint32_t> uoff(&A), uadj(&A), gid(&A);
   {
     std::vector<UnionPart> parts(B);
@@ -1828,7 +1934,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(ctl.alloc(1));
     DBuf<uint32_t> stamp(&A);
     HIPCHK(stamp.alloc(n));
-    peel_grid = peel_grid_blocks(n, T, k, e->device);
+    peel_grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
     if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     {
       CoopGate gate(e, peel_grid);
@@ -2258,7 +2364,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       // one workgroup per 64k arcs, at most a quarter of the CUs
       const uint64_t want = (C.T * k + 65535) / 65536;
       it.nblocks = (uint32_t)std::max<uint64_t>(
-          1, std::min<uint64_t>(e->flow_blocks ? e->flow_blocks : want, std::max(1u, coop_cap / 2)));
+          1, std::min<uint64_t>(flow_blocks_of(e) ? flow_blocks_of(e) : want,
+                                std::max(1u, coop_cap / 2)));
       it.first_block = 0;
     }
     HIPCHK(fctl.zero());
@@ -2527,5 +2634,163 @@ extern "C" cds_status cds_solve_batch(cds_engine *e,
       if (done[x]) cds_result_free(&results[x]);
     return status;
   }
+  return CDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: coordinate descent over the launch shapes on one graph.  Each
+// field in turn takes each candidate value with the others fixed; a
+// candidate is kept if its best-of-reps solve time beats the incumbent's by
+// 2 %.  Every candidate solve must give the first solve's answer.
+// ---------------------------------------------------------------------------
+namespace {
+
+bool same_answer(const cds_result &a, const cds_result &b) {
+  if (a.total_cliques != b.total_cliques || a.kmax != b.kmax || a.k_prime != b.k_prime ||
+      a.core_size != b.core_size || a.core_cliques != b.core_cliques ||
+      a.num_components != b.num_components || a.size != b.size ||
+      a.set_cliques != b.set_cliques || a.set_size != b.set_size ||
+      memcmp(&a.density, &b.density, sizeof(float)) != 0)
+    return false;
+  for (uint32_t i = 0; i < a.set_size; ++i)
+    if (a.vertices[i] != b.vertices[i]) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" cds_status cds_autotune(cds_engine *e, const cds_dgraph *g, const cds_params *p,
+                                   const char *path, cds_tuning *best, double *best_ms,
+                                   double *base_ms) {
+  if (!e || !g || !p) return fail(CDS_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  cds_params q = *p;
+  q.want_vertex_arrays = 0;
+  const uint32_t k = q.k;
+  struct Forced {
+    cds_engine *e;
+    ~Forced() { e->forced = false; }
+  } forced{e};
+  e->forced = true;
+  cds_tuning cur = tuning_for(e, k, g->m);
+  cds_result ref;
+  memset(&ref, 0, sizeof ref);
+  // first solve: warms the arena and fixes the answer every candidate must give
+  e->cur = cur;
+  CHK(solve_impl(e, g, &q, &ref, false));
+  struct RefFree {
+    cds_result *r;
+    ~RefFree() { cds_result_free(r); }
+  } ref_free{&ref};
+  const double first_ms = std::max(ref.times.total, 1e-3);
+  // best of `reps` solves, about 0.3 s of work per candidate (at least 2)
+  const int reps = (int)std::min(8.0, std::max(2.0, std::ceil(300.0 / first_ms)));
+  auto timed = [&](const cds_tuning &t, double *ms) -> cds_status {
+    e->cur = t;
+    double bestt = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      cds_result R;
+      memset(&R, 0, sizeof R);
+      const cds_status st = solve_impl(e, g, &q, &R, false);
+      if (st != CDS_OK) {
+        cds_result_free(&R);
+        return st;
+      }
+      const bool same = same_answer(R, ref);
+      bestt = std::min(bestt, R.times.total);
+      cds_result_free(&R);
+      if (!same)
+        return fail(CDS_ERR_INTERNAL,
+                    "autotune: shape (%u,%u) peel %u/%d replay %u flow %u changed the answer",
+                    t.list_threads, t.list_chunk, t.peel_blocks, t.peel_coarse_shift,
+                    t.replay_blocks, t.flow_blocks);
+    }
+    *ms = bestt;
+    return CDS_OK;
+  };
+  double cur_ms = 0;
+  CHK(timed(cur, &cur_ms));
+  const double start_ms = cur_ms;
+  const unsigned cap = std::max(1u, e->cus / 2);
+  const unsigned rule_peel = peel_grid_blocks(g->n, ref.total_cliques, k, e->device, 0);
+  auto try_values = [&](auto set, const std::vector<cds_tuning> &cands) -> cds_status {
+    (void)set;
+    for (const cds_tuning &t : cands) {
+      if (memcmp(&t, &cur, sizeof t) == 0 || !tuning_valid(t)) continue;
+      double ms = 0;
+      CHK(timed(t, &ms));
+      if (ms < cur_ms * 0.98) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+    return CDS_OK;
+  };
+  std::vector<cds_tuning> c;
+  // listing tile shape (k = 3, 4)
+  if (k >= 3) {
+    c.clear();
+    const uint32_t shapes[5][2] = {{0, 0}, {256, 32}, {512, 64}, {512, 32}, {128, 32}};
+    for (auto &sh : shapes) {
+      cds_tuning t = cur;
+      t.list_threads = sh[0];
+      t.list_chunk = sh[1];
+      c.push_back(t);
+    }
+    CHK(try_values(0, c));
+  }
+  // peel: coarse steps, then width (narrow widths only where the rule is
+  // narrow: a large graph on a few workgroups takes seconds per solve)
+  c.clear();
+  for (int sh : {-1, 0, 1, 2, 3}) {
+    cds_tuning t = cur;
+    t.peel_coarse_shift = sh;
+    c.push_back(t);
+  }
+  CHK(try_values(0, c));
+  c.clear();
+  for (unsigned b : {0u, 1u, 4u, 16u, cap / 4, cap / 2, cap}) {
+    if (b > cap || (b != 0 && b < rule_peel / 4)) continue;
+    cds_tuning t = cur;
+    t.peel_blocks = b;
+    c.push_back(t);
+  }
+  CHK(try_values(0, c));
+  c.clear();
+  for (unsigned b : {0u, 16u, cap / 4, cap / 2, cap}) {
+    if (b > cap) continue;
+    cds_tuning t = cur;
+    t.replay_blocks = b;
+    c.push_back(t);
+  }
+  CHK(try_values(0, c));
+  c.clear();
+  for (unsigned b : {0u, 1u, 8u, cap / 4, cap / 2}) {
+    if (b > cap) continue;
+    cds_tuning t = cur;
+    t.flow_blocks = b;
+    c.push_back(t);
+  }
+  CHK(try_values(0, c));
+  // keep the winner for this (k, class) in the engine and the cache file
+  TuneEntry ent;
+  ent.arch = e->arch;
+  ent.cus = e->cus;
+  ent.k = k;
+  ent.cls = edge_class(g->m);
+  ent.t = cur;
+  ent.best_ms = cur_ms;
+  bool replaced = false;
+  for (TuneEntry &x : e->tuned)
+    if (x.k == ent.k && x.cls == ent.cls) {
+      x = ent;
+      replaced = true;
+    }
+  if (!replaced) e->tuned.push_back(ent);
+  if (path) CHK(merge_tune_file(path, ent));
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+  if (base_ms) *base_ms = start_ms;
   return CDS_OK;
 }

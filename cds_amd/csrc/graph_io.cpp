@@ -84,6 +84,7 @@ extern "C" size_t cds_struct_size(int which) {
     case 1: return sizeof(cds_params);
     case 2: return sizeof(cds_result);
     case 3: return sizeof(cds_opts);
+    case 4: return sizeof(cds_tuning);
     default: return 0;
   }
 }

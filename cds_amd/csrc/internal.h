@@ -13,6 +13,9 @@ namespace cds {
 cds_status fail(cds_status st, const char *fmt, ...)
     __attribute__((format(printf, 2, 3)));
 
+// Copies the default tuning and the loaded tune-cache entries (engine.cpp).
+void copy_tuning(cds_engine *dst, cds_engine *src);
+
 // Allocates a host array with malloc (so the C ABI can free it with free()).
 template <class T> T *host_alloc(size_t count) {
   if (count == 0) count = 1;

@@ -66,27 +66,32 @@ hipError_t reduce_max_u32(const uint32_t *in, size_t n, uint32_t *out,
                           hipStream_t s);
 
 // ---- listing (list.hip) ----
-// Work items: (u, first position in N+(u)); each covers up to
-// kListChunk out-neighbours of u.
-constexpr uint32_t kListChunk = 64;
+// Work items: (u, first position in N+(u)); each covers up to `chunk`
+// out-neighbours of u.  The tile shape (workgroup lanes, chunk) is one of
+// (256, 64) [default], (256, 32), (512, 64), (512, 32), (128, 32).
+struct ListShape {
+  uint32_t threads = 256;
+  uint32_t chunk = 64;
+};
 // N+(u) is staged in LDS up to this many entries (k = 3: list + counters,
 // 64000 B; larger lists are searched in global memory).  k = 4 needs three
 // LDS arrays and requires every out-degree to fit.
 constexpr uint32_t kListLdsCap = 8000;
 constexpr uint32_t kList4LdsCap = 4000;
-hipError_t list_item_counts(uint32_t n, const uint32_t *doff,
+hipError_t list_item_counts(uint32_t n, uint32_t chunk, const uint32_t *doff,
                             uint32_t *items_per_vertex, hipStream_t s);
-hipError_t list_item_fill(uint32_t n, const uint32_t *doff,
+hipError_t list_item_fill(uint32_t n, uint32_t chunk, const uint32_t *doff,
                           const uint64_t *item_off, uint2 *items,
                           hipStream_t s);
 // k = 3 or 4.  Count mode (out == nullptr): item_count[i] = cliques found by
 // item i, clique_degree += per-vertex contributions.  Write mode: cliques
-// written AoS (k ids each) at out + k * item_base[i].
-hipError_t list_cliques(uint32_t k, uint64_t num_items, const uint2 *items,
-                        const uint32_t *doff, const uint32_t *dadj,
-                        uint32_t max_out_degree, uint32_t *item_count,
-                        uint64_t *clique_degree, const uint64_t *item_base,
-                        uint32_t *out, hipStream_t s);
+// written AoS (k ids each) at out + k * item_base[i].  hipErrorInvalidValue
+// for a shape not listed above.
+hipError_t list_cliques(uint32_t k, ListShape shape, uint64_t num_items,
+                        const uint2 *items, const uint32_t *doff,
+                        const uint32_t *dadj, uint32_t max_out_degree,
+                        uint32_t *item_count, uint64_t *clique_degree,
+                        const uint64_t *item_base, uint32_t *out, hipStream_t s);
 // k = 2: every DAG arc is a clique.
 hipError_t list_edges(uint32_t n, const uint32_t *doff, const uint32_t *dadj,
                       uint64_t *clique_degree, uint32_t *out, hipStream_t s);
@@ -179,7 +184,9 @@ hipError_t refine_prepare(uint32_t n, uint64_t T, uint32_t level,
 hipError_t refine_restore(uint32_t n, uint32_t *core, const uint32_t *deg,
                           hipStream_t s);
 // Workgroups for peel_persistent on this graph (0: cannot launch).
-unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device);
+// force >= 1: that many (at most the cap) instead of the sizing rule.
+unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device,
+                          unsigned force = 0);
 
 // Exact sequential replay of one peel phase (level `level`); see peel.hip.
 struct ReplayOut {
@@ -235,7 +242,7 @@ constexpr uint32_t kReplayGridMaxVertices = 16384;
 // Workgroups of the whole-GPU replay (half the CUs), 0 if unavailable.
 // Workgroups for the whole-GPU replay of a phase whose removals touch about
 // entries_per_step incidence entries each (CDS_REPLAY_BLOCKS overrides).
-unsigned replay_grid_blocks(int device, uint64_t entries_per_step);
+unsigned replay_grid_blocks(int device, uint64_t entries_per_step, unsigned force = 0);
 
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,

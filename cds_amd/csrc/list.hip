@@ -7,10 +7,10 @@
 // found exactly once, from its lowest-ranked vertex u, as u < v < w (< x)
 // along DAG arcs.
 //
-// Work decomposition: an item is (u, j0): up to kListChunk consecutive
-// out-neighbours v of u, so a hub's out-list is spread over many
-// workgroups.  One 256-thread workgroup per item stages the sorted N+(u)
-// in LDS and flattens the item's work sum_v d+(v) across its 256 lanes
+// Work decomposition: an item is (u, j0): up to `chunk` consecutive
+// out-neighbours v of u (32 or 64), so a hub's out-list is spread over many
+// workgroups.  One workgroup (256 or 512 lanes) per item stages the sorted
+// N+(u) in LDS and flattens the item's work sum_v d+(v) across its lanes
 // (coalesced reads of each N+(v)); membership of w in N+(u) is a binary
 // search in LDS that also yields w's position, so per-vertex clique
 // degrees are accumulated in LDS counters indexed by position and flushed
@@ -25,41 +25,43 @@
 namespace cds {
 namespace {
 
-constexpr int kListThreads = 256;
-constexpr int kListWaves = kListThreads / kWave;
-
-__global__ void k_item_counts(uint32_t n, const uint32_t *__restrict__ doff,
+__global__ void k_item_counts(uint32_t n, uint32_t chunk,
+                              const uint32_t *__restrict__ doff,
                               uint32_t *__restrict__ ipv) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     const uint32_t d = doff[v + 1] - doff[v];
-    ipv[v] = (d + kListChunk - 1) / kListChunk;
+    ipv[v] = (d + chunk - 1) / chunk;
   }
 }
 
-__global__ void k_item_fill(uint32_t n, const uint32_t *__restrict__ doff,
+__global__ void k_item_fill(uint32_t n, uint32_t chunk,
+                            const uint32_t *__restrict__ doff,
                             const uint64_t *__restrict__ ioff,
                             uint2 *__restrict__ items) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     const uint32_t d = doff[v + 1] - doff[v];
-    const uint32_t nb = (d + kListChunk - 1) / kListChunk;
+    const uint32_t nb = (d + chunk - 1) / chunk;
     const uint64_t base = ioff[v];
     for (uint32_t t = 0; t < nb; ++t)
-      items[base + t] = make_uint2(v, t * kListChunk);
+      items[base + t] = make_uint2(v, t * chunk);
   }
 }
 
+template <int CHUNK, int WAVES>
 struct ChunkTable {
-  uint32_t pre[kListChunk + 1];  // prefix sums of d+(v) over the chunk
-  uint32_t vid[kListChunk];
-  uint32_t vbase[kListChunk];
-  uint32_t wave_found[kListWaves];
+  static_assert(CHUNK <= kWave, "the chunk scan is one wave");
+  uint32_t pre[CHUNK + 1];  // prefix sums of d+(v) over the chunk
+  uint32_t vid[CHUNK];
+  uint32_t vbase[CHUNK];
+  uint32_t wave_found[WAVES];
   uint32_t cursor;
 };
 
 // First wave: load the chunk's v ids and scan their out-degrees.
-__device__ __forceinline__ void load_chunk(ChunkTable &t, const uint32_t *dadj,
+template <class Table>
+__device__ __forceinline__ void load_chunk(Table &t, const uint32_t *dadj,
                                            const uint32_t *doff, uint32_t ub,
                                            uint32_t j0, uint32_t jn) {
   if (threadIdx.x < kWave) {
@@ -86,8 +88,9 @@ __device__ __forceinline__ void load_chunk(ChunkTable &t, const uint32_t *dadj,
 }
 
 // Finishes an item in count mode: item total, cd[u], flush LDS counters.
+template <int WAVES, class Table>
 __device__ __forceinline__ void finish_count(
-    ChunkTable &t, uint32_t found, uint64_t it, uint32_t u, bool staged,
+    Table &t, uint32_t found, uint64_t it, uint32_t u, bool staged,
     const uint32_t *su, const uint32_t *scnt, uint32_t du,
     uint32_t *item_count, unsigned long long *cd) {
   found = wave_sum(found);
@@ -95,7 +98,7 @@ __device__ __forceinline__ void finish_count(
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t tot = 0;
-    for (int w = 0; w < kListWaves; ++w) tot += t.wave_found[w];
+    for (int w = 0; w < WAVES; ++w) tot += t.wave_found[w];
     item_count[it] = tot;
     if (tot) atomicAdd(&cd[u], (unsigned long long)tot);
   }
@@ -106,7 +109,8 @@ __device__ __forceinline__ void finish_count(
     }
 }
 
-__global__ __launch_bounds__(kListThreads) void k_list3(
+template <int THREADS, int CHUNK>
+__global__ __launch_bounds__(THREADS) void k_list3(
     uint64_t num_items, const uint2 *__restrict__ items,
     const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dadj,
     uint32_t lds_cap, uint32_t *__restrict__ item_count,
@@ -115,13 +119,14 @@ __global__ __launch_bounds__(kListThreads) void k_list3(
   extern __shared__ uint32_t dyn[];
   uint32_t *su = dyn;             // N+(u), sorted
   uint32_t *scnt = dyn + lds_cap; // clique-degree counters by position
-  __shared__ ChunkTable t;
+  constexpr int WAVES = THREADS / kWave;
+  __shared__ ChunkTable<CHUNK, WAVES> t;
   const bool write = out != nullptr;
   for (uint64_t it = blockIdx.x; it < num_items; it += gridDim.x) {
     const uint2 item = items[it];
     const uint32_t u = item.x, j0 = item.y;
     const uint32_t ub = doff[u], du = doff[u + 1] - ub;
-    const uint32_t jn = min(du - j0, (uint32_t)kListChunk);
+    const uint32_t jn = min(du - j0, (uint32_t)CHUNK);
     const bool staged = du <= lds_cap;
     if (staged)
       for (uint32_t i = threadIdx.x; i < du; i += blockDim.x) {
@@ -161,13 +166,14 @@ __global__ __launch_bounds__(kListThreads) void k_list3(
         atomicAdd(&cd[t.vid[lo]], 1ull);
       }
     }
-    if (!write) finish_count(t, found, it, u, staged, su, scnt, du, item_count, cd);
+    if (!write) finish_count<WAVES>(t, found, it, u, staged, su, scnt, du, item_count, cd);
     __syncthreads();
   }
 }
 
 // k = 4.  Requires du <= lds_cap for every u (checked by the host).
-__global__ __launch_bounds__(kListThreads) void k_list4(
+template <int THREADS, int CHUNK>
+__global__ __launch_bounds__(THREADS) void k_list4(
     uint64_t num_items, const uint2 *__restrict__ items,
     const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dadj,
     uint32_t lds_cap, uint32_t *__restrict__ item_count,
@@ -178,7 +184,8 @@ __global__ __launch_bounds__(kListThreads) void k_list4(
   uint32_t *scnt = dyn + lds_cap;
   uint32_t *sbits = dyn + 2 * lds_cap;               // lds_cap/32 + 1 words
   uint32_t *plist = sbits + (lds_cap / 32 + 1);      // lds_cap
-  __shared__ ChunkTable t;
+  constexpr int WAVES = THREADS / kWave;
+  __shared__ ChunkTable<CHUNK, WAVES> t;
   __shared__ uint32_t s_np;
   const bool write = out != nullptr;
   const uint32_t wid = threadIdx.x / kWave;
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(kListThreads) void k_list4(
     const uint2 item = items[it];
     const uint32_t u = item.x, j0 = item.y;
     const uint32_t ub = doff[u], du = doff[u + 1] - ub;
-    const uint32_t jn = min(du - j0, (uint32_t)kListChunk);
+    const uint32_t jn = min(du - j0, (uint32_t)CHUNK);
     for (uint32_t i = threadIdx.x; i < du; i += blockDim.x) {
       su[i] = dadj[ub + i];
       scnt[i] = 0;
@@ -212,7 +219,7 @@ __global__ __launch_bounds__(kListThreads) void k_list4(
       }
       __syncthreads();
       const uint32_t np = s_np;
-      for (uint32_t q = wid; q < np; q += kListWaves) {
+      for (uint32_t q = wid; q < np; q += WAVES) {
         const uint32_t pw = plist[q];
         const uint32_t w = su[pw];
         const uint32_t wb = doff[w], dw = doff[w + 1] - wb;
@@ -237,7 +244,7 @@ __global__ __launch_bounds__(kListThreads) void k_list4(
       }
       __syncthreads();
     }
-    if (!write) finish_count(t, found, it, u, true, su, scnt, du, item_count, cd);
+    if (!write) finish_count<WAVES>(t, found, it, u, true, su, scnt, du, item_count, cd);
     __syncthreads();
   }
 }
@@ -265,33 +272,35 @@ __global__ __launch_bounds__(256) void k_list2(
 
 }  // namespace
 
-hipError_t list_item_counts(uint32_t n, const uint32_t *doff,
+hipError_t list_item_counts(uint32_t n, uint32_t chunk, const uint32_t *doff,
                             uint32_t *items_per_vertex, hipStream_t s) {
+  if (chunk != 32 && chunk != 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_item_counts, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
-                     s, n, doff, items_per_vertex);
+                     s, n, chunk, doff, items_per_vertex);
   return hipGetLastError();
 }
 
-hipError_t list_item_fill(uint32_t n, const uint32_t *doff,
+hipError_t list_item_fill(uint32_t n, uint32_t chunk, const uint32_t *doff,
                           const uint64_t *item_off, uint2 *items,
                           hipStream_t s) {
+  if (chunk != 32 && chunk != 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_item_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
-                     s, n, doff, item_off, items);
+                     s, n, chunk, doff, item_off, items);
   return hipGetLastError();
 }
 
-hipError_t list_cliques(uint32_t k, uint64_t num_items, const uint2 *items,
-                        const uint32_t *doff, const uint32_t *dadj,
-                        uint32_t max_out_degree, uint32_t *item_count,
-                        uint64_t *clique_degree, const uint64_t *item_base,
-                        uint32_t *out, hipStream_t s) {
-  if (num_items == 0) return hipSuccess;
+namespace {
+template <int THREADS, int CHUNK>
+hipError_t launch_list(uint32_t k, uint64_t num_items, const uint2 *items,
+                       const uint32_t *doff, const uint32_t *dadj,
+                       uint32_t max_out_degree, uint32_t *item_count,
+                       unsigned long long *cd, const uint64_t *item_base,
+                       uint32_t *out, hipStream_t s) {
   const unsigned grid = (unsigned)(num_items < 16384 ? num_items : 16384);
-  auto *cd = reinterpret_cast<unsigned long long *>(clique_degree);
   if (k == 3) {
     const uint32_t cap = max_out_degree < kListLdsCap ? max_out_degree : kListLdsCap;
     const size_t lds = 2ull * (cap ? cap : 1) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_list3, dim3(grid), dim3(kListThreads), lds, s,
+    hipLaunchKernelGGL((k_list3<THREADS, CHUNK>), dim3(grid), dim3(THREADS), lds, s,
                        num_items, items, doff, dadj, cap ? cap : 1, item_count,
                        cd, item_base, out);
     return hipGetLastError();
@@ -300,11 +309,32 @@ hipError_t list_cliques(uint32_t k, uint64_t num_items, const uint2 *items,
     const uint32_t cap = max_out_degree ? max_out_degree : 1;
     if (cap > kList4LdsCap) return hipErrorInvalidValue;
     const size_t lds = (3ull * cap + cap / 32 + 1) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_list4, dim3(grid), dim3(kListThreads), lds, s,
+    hipLaunchKernelGGL((k_list4<THREADS, CHUNK>), dim3(grid), dim3(THREADS), lds, s,
                        num_items, items, doff, dadj, cap, item_count, cd,
                        item_base, out);
     return hipGetLastError();
   }
+  return hipErrorInvalidValue;
+}
+}  // namespace
+
+hipError_t list_cliques(uint32_t k, ListShape shape, uint64_t num_items,
+                        const uint2 *items, const uint32_t *doff,
+                        const uint32_t *dadj, uint32_t max_out_degree,
+                        uint32_t *item_count, uint64_t *clique_degree,
+                        const uint64_t *item_base, uint32_t *out, hipStream_t s) {
+  if (num_items == 0) return hipSuccess;
+  auto *cd = reinterpret_cast<unsigned long long *>(clique_degree);
+#define CDS_LIST_SHAPE(TH, CH)                                                      \
+  if (shape.threads == TH && shape.chunk == CH)                                     \
+    return launch_list<TH, CH>(k, num_items, items, doff, dadj, max_out_degree,     \
+                               item_count, cd, item_base, out, s);
+  CDS_LIST_SHAPE(256, 64)
+  CDS_LIST_SHAPE(256, 32)
+  CDS_LIST_SHAPE(512, 64)
+  CDS_LIST_SHAPE(512, 32)
+  CDS_LIST_SHAPE(128, 32)
+#undef CDS_LIST_SHAPE
   return hipErrorInvalidValue;
 }
 

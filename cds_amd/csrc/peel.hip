@@ -1124,7 +1124,8 @@ static unsigned coop_cus(const void *fn, int threads, int device) {
   return (unsigned)cus;
 }
 
-unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device) {
+unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device,
+                          unsigned force) {
   const unsigned cus = coop_cus(reinterpret_cast<const void *>(k_peel_persistent),
                                 kPeelThreads, device);
   if (cus == 0) return 0;
@@ -1141,6 +1142,7 @@ unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device) {
     const long v = atol(gb);
     if (v >= 1 && (uint64_t)v <= cap) want = (uint64_t)v;
   }
+  if (force >= 1) want = std::min<uint64_t>(force, cap);
   return (unsigned)want;
 }
 
@@ -1157,7 +1159,7 @@ uint64_t replay_nstar(uint32_t k, uint64_t kp) {
   return 0;
 }
 
-unsigned replay_grid_blocks(int device, uint64_t entries_per_step) {
+unsigned replay_grid_blocks(int device, uint64_t entries_per_step, unsigned force) {
   const unsigned cus = coop_cus(reinterpret_cast<const void *>(k_replay_grid),
                                 kReplayThreads, device);
   const unsigned cap = cus > 1 ? cus / 2 : cus;
@@ -1169,6 +1171,7 @@ unsigned replay_grid_blocks(int device, uint64_t entries_per_step) {
   // workgroup.
   uint64_t want = (entries_per_step + kReplayEntriesPerBlock - 1) / kReplayEntriesPerBlock;
   if (const char *e = getenv("CDS_REPLAY_BLOCKS")) want = strtoull(e, nullptr, 10);
+  if (force >= 1) want = force;
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
 }
 

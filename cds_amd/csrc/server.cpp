@@ -29,6 +29,7 @@
 
 #include "internal.h"
 
+using cds::copy_tuning;
 using cds::fail;
 
 namespace {
@@ -274,6 +275,7 @@ extern "C" cds_status cds_server_create(cds_engine *e, const cds_server_opts *o,
   cds_engine *large = nullptr;
   const cds_status st = cds_engine_create(&eo, &large);
   if (st != CDS_OK) return st;
+  copy_tuning(large, e);
   auto *s = new cds_server();
   memset(&s->opts, 0, sizeof s->opts);
   if (o) s->opts = *o;

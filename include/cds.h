@@ -155,7 +155,26 @@ typedef struct {
   int device;               /* HIP device ordinal */
   uint64_t hbm_budget_bytes;/* 0 = no budget */
   uint32_t profile_kernels; /* record per-kernel HIP-event times */
+  const char *tune_cache;   /* autotune cache file loaded at creation (may be
+                               NULL; the CDS_TUNE_CACHE environment variable
+                               names one otherwise); a missing file is not an
+                               error */
 } cds_opts;
+
+/* Launch shapes of the stages that have a choice (no counterpart in the
+ * reference, whose shapes are compile-time: BLOCK_SIZE / warps per block,
+ * cudaCode/inc/common.h).  Zero in a field means "the engine's rule". */
+typedef struct {
+  uint32_t list_threads;      /* listing workgroup lanes: 128, 256 or 512 */
+  uint32_t list_chunk;        /* out-neighbours per listing work item: 32 or
+                                 64 ((256, 64) by default; 128 lanes need 32) */
+  uint32_t peel_blocks;       /* persistent-peel workgroups (<= half the CUs) */
+  int32_t peel_coarse_shift;  /* -1 rule (coarse steps from 2^28 incidence
+                                 entries), 0 exact levels, s in 1..8: coarse
+                                 steps [m, m + m >> s] */
+  uint32_t replay_blocks;     /* whole-GPU replay workgroups */
+  uint32_t flow_blocks;       /* max-flow workgroups per network */
+} cds_tuning;
 
 typedef struct cds_engine cds_engine;
 typedef struct cds_dgraph cds_dgraph; /* device-resident graph */
@@ -163,7 +182,7 @@ typedef struct cds_dgraph cds_dgraph; /* device-resident graph */
 const char *cds_last_error(void);
 const char *cds_version(void);
 /* sizeof of the ABI structs (0 cds_graph, 1 cds_params, 2 cds_result,
- * 3 cds_opts) so foreign bindings can verify their layouts. */
+ * 3 cds_opts, 4 cds_tuning) so foreign bindings can verify their layouts. */
 size_t cds_struct_size(int which);
 
 /* ---- graph I/O (replaces Graph::Graph, cudaCode/src/graph.cpp:7-40) ---- */
@@ -187,6 +206,33 @@ cds_status cds_graph_load_csr(const char *path, cds_graph *out);
 cds_status cds_engine_create(const cds_opts *opts, cds_engine **out);
 void cds_engine_destroy(cds_engine *e);
 int cds_engine_device(const cds_engine *e);
+
+/* ---- autotuner (no counterpart in the reference) ----
+ * Tunings are kept per (device architecture, CU count, k, edge class), the
+ * edge class being floor(log2(m)); a solve uses the entry of its k whose
+ * class is nearest to its graph's (within 2), else the engine's default.
+ * cds_engine_set_tuning sets the default (NULL: the built-in rules).
+ * cds_engine_get_tuning reports what a solve of (k, m) would use. */
+cds_status cds_engine_set_tuning(cds_engine *e, const cds_tuning *t);
+cds_status cds_engine_get_tuning(cds_engine *e, uint32_t k, uint64_t m,
+                                 cds_tuning *out);
+/* Loads every entry of the cache file made for this engine's device
+ * (returns the number loaded in *loaded, may be NULL). */
+cds_status cds_engine_load_tuning(cds_engine *e, const char *path,
+                                  uint32_t *loaded);
+/* Sweeps the shapes one field at a time on graph dg (each candidate solved
+ * and timed; every candidate's answer must equal the first solve's, else
+ * CDS_ERR_INTERNAL), keeps the fastest for (k, class of dg) in the engine
+ * and, when path is not NULL, merges it into that cache file (written to a
+ * temporary name, then renamed).  best / best_ms / base_ms may be NULL. */
+cds_status cds_autotune(cds_engine *e, const cds_dgraph *dg, const cds_params *p,
+                        const char *path, cds_tuning *best, double *best_ms,
+                        double *base_ms);
+/* Host-only lookup in a cache file, for tools and tests: the entry for
+ * (arch, cus, k) nearest to class floor(log2(m)); CDS_ERR_INVALID if none
+ * is within 2 classes, CDS_ERR_IO if the file is unreadable or malformed. */
+cds_status cds_tuning_lookup(const char *path, const char *arch, uint32_t cus,
+                             uint32_t k, uint64_t m, cds_tuning *out);
 
 /* Upload a host graph to HBM once; solves then start from device memory
  * (replaces memoryAllocationGraph, cudaCode/src/gpuMemoryAllocation.cu:5-27). */
