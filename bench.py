@@ -95,7 +95,7 @@ def bench_batched(args):
     ks = [3 if rng.random() < 0.7 else 4 for _ in graphs]
     print(f"[bench] generated {len(graphs)} graphs in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    eng = cds_amd.Engine(device=0)
+    eng = cds_amd.Engine(device=0, tune_cache="" if args.untuned else None)
     dgs = [eng.upload(g) for g in graphs]
     eng.solve_batch(dgs[:64], ks[:64])  # warm-up
     t0 = time.perf_counter()
@@ -200,7 +200,7 @@ def bench_serving(args):
     import cds_amd
 
     loads = [float(x) for x in args.loads.split(",")]
-    eng = cds_amd.Engine(device=0)
+    eng = cds_amd.Engine(device=0, tune_cache="" if args.untuned else None)
     rows = []
     for li, rate in enumerate(loads):
         rng = np.random.default_rng(args.seed + li)
@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--loads", default="20,60,180", help="c5: offered requests/s")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--untuned", action="store_true",
+                    help="built-in launch-shape rules instead of cds_amd/tune_cache.txt")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the "
                          "reference point that scales rocprofv3 byte counters)")
@@ -432,7 +434,7 @@ def main():
     g = make_graph(w)
     print(f"[bench] generated n={g.n} m={g.m} in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    eng = cds_amd.Engine(device=0)
+    eng = cds_amd.Engine(device=0, tune_cache="" if args.untuned else None)
     dg = eng.upload(g)
     for i in range(args.warmup):
         r = eng.solve(dg, w["k"])
@@ -516,11 +518,16 @@ def main():
                      "traffic": read_traffic(dom, args.workload),
                      "algorithmic_bytes": alg, "avg_launch_ms": dom_ms},
         "kernels": kernels,
+        "tuning": dict(eng.get_tuning(w["k"], g.m),
+                       cache=os.path.relpath(cds_amd.TUNE_CACHE, ROOT)
+                       if os.path.exists(cds_amd.TUNE_CACHE) else None),
         "peel": {"level_table_entries": r["peel_distinct_levels"],
                  "sub_rounds": r["peel_levels"], "workgroups": r["peel_grid"],
                  "coarse_shift": r["peel_coarse_shift"],
                  "refined_steps": r["peel_refined_steps"],
-                 "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms")},
+                 "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms"),
+                 # one thread's wall-clock split of the persistent launches
+                 "split_ms": r["peel_split_ms"]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

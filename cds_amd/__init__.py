@@ -17,6 +17,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcds_amd.so")
+# launch shapes found by `python -m cds_amd.tune` (loaded by Engine by default)
+TUNE_CACHE = os.path.join(HERE, "tune_cache.txt")
 CLI_PATH = os.path.join(HERE, "bin", "cds_main")
 
 STATUS = {
@@ -351,6 +353,10 @@ class Engine:
 
     def __init__(self, device=0, hbm_budget_bytes=0, profile_kernels=False,
                  tune_cache=None):
+        # tune_cache None: the shipped cache (cds_amd/tune_cache.txt) if present;
+        # "": none (the CDS_TUNE_CACHE environment variable still applies)
+        if tune_cache is None and os.path.exists(TUNE_CACHE):
+            tune_cache = TUNE_CACHE
         opts = COpts(device, hbm_budget_bytes, 1 if profile_kernels else 0,
                      os.fsencode(tune_cache) if tune_cache else None)
         h = ctypes.c_void_p()

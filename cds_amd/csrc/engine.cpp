@@ -50,6 +50,7 @@ struct cds_engine {
   // scale 18 k=3 and scale 17 k=4.
   int peel_coarse = -1;
   uint32_t peel_shift = 1;
+  bool peel_shift_env = false;
   // launch shapes: device identity for the tune cache, the default tuning,
   // the loaded cache entries, and the tuning of the solve in progress
   std::string arch;
@@ -79,7 +80,7 @@ ListShape list_shape(const cds_engine *e) {
   return s;
 }
 unsigned flow_blocks_of(const cds_engine *e) {
-  return e->cur.flow_blocks ? e->cur.flow_blocks : e->flow_blocks;
+  return e->flow_blocks ? e->flow_blocks : e->cur.flow_blocks;  // CDS_FLOW_BLOCKS first
 }
 }  // namespace
 
@@ -659,7 +660,10 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
   if (const char *pc = getenv("CDS_PEEL_COARSE")) e->peel_coarse = atoi(pc) ? 1 : 0;
   if (const char *ps = getenv("CDS_PEEL_COARSE_SHIFT")) {
     const int v = atoi(ps);
-    if (v >= 1 && v <= 16) e->peel_shift = (uint32_t)v;
+    if (v >= 1 && v <= 16) {
+      e->peel_shift = (uint32_t)v;
+      e->peel_shift_env = true;
+    }
   }
   {
     hipDeviceProp_t prop;
@@ -1090,14 +1094,18 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   uint32_t sbase = 1;  // first sub-round of the next persistent launch
   DBuf<unsigned long long> lrem(&A), lkil(&A);
   DBuf<PeelCtl> ctl(&A);
-  const bool coarse_peel =
-      e->cur.peel_coarse_shift > 0 ||
-      (e->cur.peel_coarse_shift < 0 &&
-       (e->peel_coarse == 1 || (e->peel_coarse == -1 && (uint64_t)k * T >= kCoarseMinSlots)));
-  const uint32_t cshift =
-      (n < (1u << 24) && coarse_peel)
-          ? (e->cur.peel_coarse_shift > 0 ? (uint32_t)e->cur.peel_coarse_shift : e->peel_shift)
-          : 0;
+  // CDS_PEEL_COARSE / CDS_PEEL_COARSE_SHIFT (tests, diagnostics) win over
+  // the tuning, which wins over the rule (coarse steps of [m, 1.5 m] from
+  // kCoarseMinSlots incidence entries)
+  bool coarse_peel = (uint64_t)k * T >= kCoarseMinSlots;
+  uint32_t shift = 1;
+  if (e->cur.peel_coarse_shift >= 0) {
+    coarse_peel = e->cur.peel_coarse_shift > 0;
+    shift = coarse_peel ? (uint32_t)e->cur.peel_coarse_shift : 1;
+  }
+  if (e->peel_coarse != -1) coarse_peel = e->peel_coarse == 1;
+  if (e->peel_shift_env) shift = e->peel_shift;
+  const uint32_t cshift = (n < (1u << 24) && coarse_peel) ? shift : 0;
   R->peel_coarse_shift = cshift;
   int wall_khz = 0;
   if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) !=
@@ -2709,8 +2717,20 @@ extern "C" cds_status cds_autotune(cds_engine *e, const cds_dgraph *g, const cds
     *ms = bestt;
     return CDS_OK;
   };
+  // CDS_TUNE_VERBOSE=1: one stderr line per candidate (progress of long sweeps)
+  const bool verbose = [] {
+    const char *v = getenv("CDS_TUNE_VERBOSE");
+    return v && atoi(v) != 0;
+  }();
+  auto report = [&](const cds_tuning &t, double ms) {
+    if (verbose)
+      fprintf(stderr, "[cds_autotune] k=%u m=%llu (%u,%u) peel %u/%d replay %u flow %u: %.3f ms\n",
+              k, (unsigned long long)g->m, t.list_threads, t.list_chunk, t.peel_blocks,
+              t.peel_coarse_shift, t.replay_blocks, t.flow_blocks, ms);
+  };
   double cur_ms = 0;
   CHK(timed(cur, &cur_ms));
+  report(cur, cur_ms);
   const double start_ms = cur_ms;
   const unsigned cap = std::max(1u, e->cus / 2);
   const unsigned rule_peel = peel_grid_blocks(g->n, ref.total_cliques, k, e->device, 0);
@@ -2720,6 +2740,7 @@ extern "C" cds_status cds_autotune(cds_engine *e, const cds_dgraph *g, const cds
       if (memcmp(&t, &cur, sizeof t) == 0 || !tuning_valid(t)) continue;
       double ms = 0;
       CHK(timed(t, &ms));
+      report(t, ms);
       if (ms < cur_ms * 0.98) {
         cur = t;
         cur_ms = ms;

@@ -213,6 +213,34 @@ constexpr uint64_t kChunkMask = (1ull << 40) - 1;
 // to 107 ms, the coarse steps reaching the aggregated regime sooner
 constexpr uint32_t kPeelLdsCap = 32768;
 
+// While many vertices are alive, degree decrements are aggregated per
+// workgroup and sub-round in an LDS hash table over the same 128 KB: keys
+// (vertex + 1, 0 = empty) in the first half, counts in the second, linear
+// probing over kPeelHashProbe slots.  Decrements of a power-law graph
+// concentrate on hub vertices, which recur across a workgroup's chunks, so
+// most reach HBM as one atomic per (hub, workgroup, sub-round).  A
+// decrement that finds no slot goes to the global degree directly.  Every
+// atomic on deg[u] applies its count at once, so exactly one of them takes
+// it from above the level to at most the level, and that one appends u.
+constexpr uint32_t kPeelHashSlots = kPeelLdsCap / 2;
+constexpr uint32_t kPeelHashProbe = 8;
+
+__device__ __forceinline__ bool hash_add(uint32_t *table, uint32_t u) {
+  const uint32_t key = u + 1;
+  uint32_t h = (key * 2654435761u) >> 18;  // 14 bits: kPeelHashSlots
+  for (uint32_t p = 0; p < kPeelHashProbe; ++p) {
+    uint32_t cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == 0) cur = atomicCAS(&table[h], 0u, key);
+    if (cur == 0 || cur == key) {
+      atomicAdd(&table[kPeelHashSlots + h], 1u);
+      return true;
+    }
+    h = (h + 1) & (kPeelHashSlots - 1);
+  }
+  return false;
+}
+
 __device__ __forceinline__ void grid_sync(PeelCtl *ctl) {
   const unsigned long long t0 =
       (blockIdx.x == 0 && threadIdx.x == 0) ? wall_clock64() : 0;
@@ -248,7 +276,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
     uint64_t *FC2, PeelCtl *ctl, uint32_t *lv_level,
     unsigned long long *lv_removed, unsigned long long *lv_killed,
     uint32_t *pos, uint32_t *sdeg, uint32_t *lv_min, uint32_t cshift,
-    uint32_t max_level) {
+    uint32_t max_level, uint32_t hagg) {
   // Once few vertices remain alive, degree decrements are summed per
   // workgroup in LDS (indexed by position in the alive list) and applied
   // with one atomic per vertex and sub-round; the vertex whose aggregated
@@ -419,6 +447,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
               atomicAdd(&lds_cnt[pos[u]], 1u);
               continue;
             }
+            if (hagg && hash_add(lds_cnt, u)) continue;
             const uint32_t old = atomicSub(&deg[u], 1u);
             if (old == level + 1) {
               core[u] = level;
@@ -433,6 +462,27 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
         const unsigned long long now = wall_clock64();
         ctl->t_proc += now - tp;
         tp = now;
+      }
+      if (hagg && !agg) {
+        // apply the workgroup's aggregated decrements, one atomic per vertex
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kPeelHashSlots; i += blockDim.x) {
+          const uint32_t key = lds_cnt[i];
+          if (!key) continue;
+          const uint32_t cnt = lds_cnt[kPeelHashSlots + i];
+          lds_cnt[i] = 0;
+          lds_cnt[kPeelHashSlots + i] = 0;
+          const uint32_t u = key - 1;
+          const uint32_t old = atomicSub(&deg[u], cnt);
+          if (old > level && old - cnt <= level) {
+            core[u] = level;
+            stamp[u] = sr + 1;
+            frontier_push(ctl, nb, F[nb], FC[nb], u,
+                          chunks_of(ioff[u + 1] - ioff[u]));
+          }
+        }
+        __syncthreads();
+        if (leader) ctl->t_flush += wall_clock64() - tp;
       }
       if (agg) {
         __syncthreads();
@@ -1022,13 +1072,18 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, na);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (sbase < 1 || sbase > kParked / 2) return hipErrorInvalidValue;
+  // LDS hash aggregation of decrements (CDS_PEEL_HASH=0 turns it off)
+  static const uint32_t hagg = [] {
+    const char *v = getenv("CDS_PEEL_HASH");
+    return (uint32_t)(v ? atoi(v) != 0 : 1);
+  }();
   hipLaunchKernelGGL(k_stamp_reset, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
                      core, stamp, (uint32_t)(sbase == 1));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   void *args[] = {&n,  &k,  (void *)&ioff, (void *)&inc, (void *)&cliques,
                   &deg, &core, &stamp, &sbase, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
-                  &sdeg, &lv_min, &cshift, &max_level};
+                  &sdeg, &lv_min, &cshift, &max_level, (void *)&hagg};
   if ((e = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
                        kPeelThreads, args, 0, s)) != hipSuccess)
     return e;
@@ -1138,11 +1193,11 @@ unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device,
   const uint64_t cap = cus > 1 ? cus / 2 : 1;
   uint64_t want = std::max<uint64_t>((n + 32767) / 32768, (k * T + (1u << 22) - 1) >> 22);
   want = std::min(std::max<uint64_t>(want, 1), cap);
-  if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {
+  if (force >= 1) want = std::min<uint64_t>(force, cap);
+  if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {  // diagnostics: wins
     const long v = atol(gb);
     if (v >= 1 && (uint64_t)v <= cap) want = (uint64_t)v;
   }
-  if (force >= 1) want = std::min<uint64_t>(force, cap);
   return (unsigned)want;
 }
 
@@ -1170,8 +1225,8 @@ unsigned replay_grid_blocks(int device, uint64_t entries_per_step, unsigned forc
   // unless a removal touches fewer than kReplayEntriesPerBlock entries per
   // workgroup.
   uint64_t want = (entries_per_step + kReplayEntriesPerBlock - 1) / kReplayEntriesPerBlock;
-  if (const char *e = getenv("CDS_REPLAY_BLOCKS")) want = strtoull(e, nullptr, 10);
   if (force >= 1) want = force;
+  if (const char *e = getenv("CDS_REPLAY_BLOCKS")) want = strtoull(e, nullptr, 10);
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
 }
 
