@@ -937,7 +937,8 @@ struct PeelSummary {
 };
 
 cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
-                            uint64_t T, PeelSummary *S, bool refine_all = false) {
+                            uint64_t T, PeelSummary *S, bool refine_all = false,
+                            double rho_replayed = 0) {
   S->rm_before.assign(levels.size(), 0);
   S->kl_before.assign(levels.size(), 0);
   S->kmax = 0;
@@ -959,6 +960,10 @@ cds_status summarize_levels(const std::vector<Level> &levels, uint64_t n,
     const double d = (double)(int64_t)Tr / static_cast<double>(N);
     if (i == 0 || d > S->rho) S->rho = d;
   }
+  // densities of rows inside levels replayed so far count before the
+  // refine and replay lists are drawn up: a step straddling the raised k'
+  // must be re-peeled
+  if (rho_replayed > S->rho) S->rho = rho_replayed;
   S->kp = (int64_t)std::ceil(S->rho);
   S->replay.clear();
   S->refine.clear();
@@ -1262,9 +1267,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
   const bool all_cores = p->want_vertex_arrays != 0;
   auto summarize = [&]() -> cds_status {
-    CHK(summarize_levels(levels, n, T, &PS, all_cores));
-    for (const auto &r : replayed) apply_replay(&PS, r.second);
-    return CDS_OK;
+    double rho_r = 0;
+    for (const auto &r : replayed)
+      if (r.second.found)
+        rho_r = std::max(rho_r, (double)(int64_t)r.second.best_num /
+                                    static_cast<double>(r.second.best_den));
+    return summarize_levels(levels, n, T, &PS, all_cores, rho_r);
   };
   CHK(summarize());
   {
@@ -1345,7 +1353,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
                     lv.level, ro.steps, (unsigned long long)M);
       replayed.push_back({lv.level, ro});
-      apply_replay(&PS, ro);
+      // a replay can raise k': the refine list (steps straddling k') and
+      // the replay candidates are recomputed against the new value
+      CHK(summarize());
     }
     const int ev_r1 = tm.mark();
     HIPCHK(hipStreamSynchronize(st));
