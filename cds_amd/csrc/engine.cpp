@@ -1514,49 +1514,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(fcd.alloc(core_n));
   HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, ncomp,
                             ccount.p, st));
-  // Work items over the core vertices' incidence lists, cut into chunks so
-  // hub lists spread over many waves.
-  constexpr uint32_t kItem = 4096;
-  std::vector<uint4> items;
-  std::vector<uint32_t> item_fv;
-  {
-    uint64_t pos = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-      const uint64_t len = cd[v];
-      if (hcore[v] >= kpu)
-        for (uint64_t s0 = 0; s0 < len; s0 += kItem) {
-          const uint64_t at = pos + s0;
-          items.push_back(make_uint4(v, (uint32_t)(at >> 32), (uint32_t)at,
-                                     (uint32_t)std::min<uint64_t>(kItem, len - s0)));
-          item_fv.push_back(fv_of[v]);
-        }
-      pos += len;
-    }
-  }
-  const uint64_t nitems = items.size();
-  DBuf<uint4> d_items(&A);
-  DBuf<uint32_t> d_icnt(&A);
-  DBuf<uint64_t> d_iout(&A);
-  HIPCHK(d_items.alloc(nitems));
-  HIPCHK(d_icnt.alloc(nitems));
-  HIPCHK(d_iout.alloc(nitems));
-  CHK(upload(e, d_items.p, items.data(), nitems));
-  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, core.p, kpu, d_icnt.p, st));
-  std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0), hicnt(nitems);
-  std::vector<uint64_t> hiout(nitems), hfioff(core_n + 1, 0);
+  std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0);
+  std::vector<uint64_t> hfioff(core_n + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
-  CHK(readback(e, hicnt.data(), d_icnt.p, nitems));
-  for (uint64_t i = 0; i < nitems; ++i) hfcd[item_fv[i]] += hicnt[i];
-  for (uint32_t f = 0; f < core_n; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
-  {
-    std::vector<uint64_t> cursor(hfioff.begin(), hfioff.end() - 1);
-    for (uint64_t i = 0; i < nitems; ++i) {
-      hiout[i] = cursor[item_fv[i]];
-      cursor[item_fv[i]] += hicnt[i];
-    }
-  }
-  CHK(upload(e, fcd.p, hfcd.data(), core_n));
-  CHK(upload(e, d_iout.p, hiout.data(), nitems));
   std::vector<uint64_t> hcoff(ncomp + 1, 0);
   for (uint32_t c = 0; c < ncomp; ++c) {
     comps[c].T = hcc[c];
@@ -1578,13 +1538,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
                               d_fv_of.p, ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
-    HIPCHK(fioff.alloc((size_t)core_n + 1));
-    CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
-    HIPCHK(finc.alloc(core_T * k));
-    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, core.p, kpu,
-                           L.cliques.p, k, cslot.p, finc.p, st));
-    HIPCHK(hipStreamSynchronize(st));
   }
+  // flow incidence: per core vertex, its (clique slot, position) pairs
+  HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st));
+  CHK(readback(e, hfcd.data(), fcd.p, core_n));
+  for (uint32_t f = 0; f < core_n; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
+  if (hfioff[core_n] != core_T * k)
+    return fail(CDS_ERR_INTERNAL, "flow incidence holds %llu of %llu pairs",
+                (unsigned long long)hfioff[core_n], (unsigned long long)(core_T * k));
+  HIPCHK(fioff.alloc((size_t)core_n + 1));
+  CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
+  HIPCHK(finc.alloc(core_T * k));
+  HIPCHK(ficur.alloc(core_n));
+  HIPCHK(flow_incidence_fill(core_T, k, fmem.p, fioff.p, core_n, ficur.p, finc.p, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ficur.reset();
   const int ev_cc = tm.mark();
 
   // ---- (c) dynamicExact (helpers.cpp:1120-1254) ----
@@ -2190,47 +2158,9 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(fcd.alloc(core_total));
   HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, ncomp, ccount.p,
                             st));
-  constexpr uint32_t kItem = 4096;
-  std::vector<uint4> items;
-  std::vector<uint32_t> item_fv;
-  {
-    uint64_t pos = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-      const uint64_t len = cd[v];
-      if (hflag[v])
-        for (uint64_t s0 = 0; s0 < len; s0 += kItem) {
-          const uint64_t at = pos + s0;
-          items.push_back(make_uint4(v, (uint32_t)(at >> 32), (uint32_t)at,
-                                     (uint32_t)std::min<uint64_t>(kItem, len - s0)));
-          item_fv.push_back(fv_of[v]);
-        }
-      pos += len;
-    }
-  }
-  const uint64_t nitems = items.size();
-  DBuf<uint4> d_items(&A);
-  DBuf<uint32_t> d_icnt(&A);
-  DBuf<uint64_t> d_iout(&A);
-  HIPCHK(d_items.alloc(nitems));
-  HIPCHK(d_icnt.alloc(nitems));
-  HIPCHK(d_iout.alloc(nitems));
-  CHK(upload(e, d_items.p, items.data(), nitems));
-  HIPCHK(item_core_count(nitems, d_items.p, inc.p, k, core01.p, 1, d_icnt.p, st));
-  std::vector<uint32_t> hcc(ncomp), hfcd(core_total, 0), hicnt(nitems);
-  std::vector<uint64_t> hiout(nitems), hfioff((size_t)core_total + 1, 0);
+  std::vector<uint32_t> hcc(ncomp), hfcd(core_total, 0);
+  std::vector<uint64_t> hfioff((size_t)core_total + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
-  CHK(readback(e, hicnt.data(), d_icnt.p, nitems));
-  for (uint64_t i = 0; i < nitems; ++i) hfcd[item_fv[i]] += hicnt[i];
-  for (uint32_t f = 0; f < core_total; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
-  {
-    std::vector<uint64_t> cursor(hfioff.begin(), hfioff.end() - 1);
-    for (uint64_t i = 0; i < nitems; ++i) {
-      hiout[i] = cursor[item_fv[i]];
-      cursor[item_fv[i]] += hicnt[i];
-    }
-  }
-  CHK(upload(e, fcd.p, hfcd.data(), core_total));
-  CHK(upload(e, d_iout.p, hiout.data(), nitems));
   std::vector<uint64_t> hcoff((size_t)ncomp + 1, 0);
   for (uint32_t c = 0; c < ncomp; ++c) {
     comps[c].T = hcc[c];
@@ -2252,11 +2182,22 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, d_fv_of.p,
                               ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
-    HIPCHK(fioff.alloc((size_t)core_total + 1));
-    CHK(upload(e, fioff.p, hfioff.data(), core_total + 1));
-    HIPCHK(finc.alloc(core_T_total * k));
-    HIPCHK(item_core_write(nitems, d_items.p, d_iout.p, inc.p, k, core01.p, 1,
-                           L.cliques.p, k, cslot.p, finc.p, st));
+  }
+  HIPCHK(flow_incidence_degrees(core_T_total, k, fmem.p, core_total, fcd.p, st));
+  CHK(readback(e, hfcd.data(), fcd.p, core_total));
+  for (uint32_t f = 0; f < core_total; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
+  if (hfioff[core_total] != core_T_total * k)
+    return fail(CDS_ERR_INTERNAL, "flow incidence holds %llu of %llu pairs",
+                (unsigned long long)hfioff[core_total],
+                (unsigned long long)(core_T_total * k));
+  HIPCHK(fioff.alloc((size_t)core_total + 1));
+  CHK(upload(e, fioff.p, hfioff.data(), core_total + 1));
+  HIPCHK(finc.alloc(core_T_total * k));
+  {
+    DBuf<uint32_t> ficur(&A);
+    HIPCHK(ficur.alloc(core_total));
+    HIPCHK(flow_incidence_fill(core_T_total, k, fmem.p, fioff.p, core_total, ficur.p,
+                               finc.p, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   inc.reset();

@@ -300,66 +300,95 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
   }
 }
 
-// Chunked variants: an item is (vertex, incidence range) of at most a few
-// thousand entries, so hubs spread over many waves.  Pass 1 counts the
-// core cliques of each item; pass 2 writes each item's flow slots at a
-// host-scanned offset, in incidence order.
-__device__ __forceinline__ bool entry_in_core(const uint32_t *__restrict__ en,
-                                              uint32_t ik,
-                                              const uint32_t *__restrict__ core,
-                                              uint32_t kp) {
-  bool in = true;
-  for (uint32_t t = 1; t < ik; ++t) in &= core[en[t]] >= kp;
-  return in;
-}
+// Flow-network incidence of the core, from its clique bucket fmem (T'
+// cliques of k flow-vertex ids each): every (clique slot s, position j) pair
+// belongs to vertex fmem[s*k+j], whose list holds s*k+j.  A workgroup takes
+// a tile of kFlowTile cliques and aggregates its pairs in an LDS hash keyed
+// by vertex (a core is a few hub-like vertices in most cliques), so each
+// distinct vertex of a tile costs one global atomic: in the count pass its
+// tile total, in the fill pass the reservation of a run of its list.  The
+// order of a vertex's list depends on the order the reservations land in;
+// the max flow and its minimal min-cut side do not.
+constexpr uint32_t kFlowThreads = 1024;
+constexpr uint32_t kFlowTile = 2048;                 // cliques per tile
+constexpr uint32_t kFlowPairs = kFlowTile * 4;       // k <= 4
+constexpr uint32_t kFlowHash = 2 * kFlowPairs;       // load <= 1/2
+constexpr uint32_t kFlowPerThread = kFlowPairs / kFlowThreads;
 
-__global__ __launch_bounds__(256) void k_item_core_count(
-    uint64_t nitems, const uint4 *__restrict__ items,
-    const uint32_t *__restrict__ inc, uint32_t ik,
-    const uint32_t *__restrict__ core, uint32_t kp, uint32_t *__restrict__ cnt) {
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       it < nitems; it += waves) {
-    const uint4 t = items[it];
-    const uint64_t lo = ((uint64_t)t.y << 32) | t.z, hi = lo + t.w;
-    uint32_t d = 0;
-    for (uint64_t q = lo + lane(); q < hi; q += kWave) d += entry_in_core(inc + q * ik, ik, core, kp);
-    d = wave_sum(d);
-    if (lane() == 0) cnt[it] = d;
+// Inserts the tile's pairs into the hash; slot[] / rank[] per pair (rank:
+// position among the tile's pairs of the same vertex).
+__device__ __forceinline__ void flow_tile_hash(const uint32_t *__restrict__ fmem,
+                                               uint64_t p0, uint32_t pairs,
+                                               uint32_t *hkey, uint32_t *hcnt,
+                                               uint32_t *slot, uint32_t *rank) {
+#pragma unroll
+  for (uint32_t r = 0; r < kFlowPerThread; ++r) {
+    const uint32_t p = r * kFlowThreads + threadIdx.x;
+    if (p >= pairs) break;
+    const uint32_t x = fmem[p0 + p];
+    uint32_t h = (x * 2654435761u) & (kFlowHash - 1);
+    for (;;) {  // at most pairs < kFlowHash keys: a free slot exists
+      const uint32_t prev = atomicCAS(&hkey[h], kAlive, x);
+      if (prev == kAlive || prev == x) break;
+      h = (h + 1) & (kFlowHash - 1);
+    }
+    slot[r] = h;
+    rank[r] = atomicAdd(&hcnt[h], 1u);
   }
 }
 
-__global__ __launch_bounds__(256) void k_item_core_write(
-    uint64_t nitems, const uint4 *__restrict__ items,
-    const uint64_t *__restrict__ item_out, const uint32_t *__restrict__ inc,
-    uint32_t ik, const uint32_t *__restrict__ core, uint32_t kp,
-    const uint32_t *__restrict__ cl, uint32_t k,
-    const uint32_t *__restrict__ cslot, uint32_t *__restrict__ finc) {
-  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       it < nitems; it += waves) {
-    const uint4 t = items[it];
-    const uint32_t v = t.x;
-    const uint64_t lo = ((uint64_t)t.y << 32) | t.z, hi = lo + t.w;
-    uint64_t out = item_out[it];
-    for (uint64_t q0 = lo; q0 < hi; q0 += kWave) {
-      const uint64_t q = q0 + lane();
-      bool take = false;
-      uint32_t slot = 0;
-      if (q < hi) {
-        const uint32_t c = inc[q * ik];
-        if (entry_in_core(inc + q * ik, ik, core, kp)) {
-          take = true;
-          uint32_t j = 0;
-          while (j + 1 < k && cl[(uint64_t)c * k + j] != v) ++j;
-          slot = cslot[c] * k + j;
-        }
-      }
-      unsigned n_;
-      const unsigned p = wave_slot(take, &n_);
-      if (take) finc[out + p] = slot;
-      out += n_;
+__global__ __launch_bounds__(kFlowThreads) void k_flow_deg(
+    uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem,
+    uint32_t *__restrict__ fcd) {
+  __shared__ uint32_t hkey[kFlowHash];
+  __shared__ uint32_t hcnt[kFlowHash];
+  const uint64_t tiles = (Tc + kFlowTile - 1) / kFlowTile;
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads) {
+      hkey[i] = kAlive;
+      hcnt[i] = 0;
     }
+    __syncthreads();
+    const uint64_t c0 = tile * kFlowTile;
+    const uint32_t pairs = (uint32_t)min<uint64_t>(kFlowTile, Tc - c0) * k;
+    uint32_t slot[kFlowPerThread], rank[kFlowPerThread];
+    flow_tile_hash(fmem, c0 * k, pairs, hkey, hcnt, slot, rank);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads)
+      if (hcnt[i]) atomicAdd(&fcd[hkey[i]], hcnt[i]);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
+    uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem,
+    const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
+    uint32_t *__restrict__ finc) {
+  __shared__ uint32_t hkey[kFlowHash];
+  __shared__ uint32_t hcnt[kFlowHash];
+  const uint64_t tiles = (Tc + kFlowTile - 1) / kFlowTile;
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads) {
+      hkey[i] = kAlive;
+      hcnt[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t c0 = tile * kFlowTile;
+    const uint32_t pairs = (uint32_t)min<uint64_t>(kFlowTile, Tc - c0) * k;
+    uint32_t slot[kFlowPerThread], rank[kFlowPerThread];
+    flow_tile_hash(fmem, c0 * k, pairs, hkey, hcnt, slot, rank);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads)
+      if (hcnt[i]) hcnt[i] = atomicAdd(&fcur[hkey[i]], hcnt[i]);  // -> run base
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kFlowPerThread; ++r) {
+      const uint32_t p = r * kFlowThreads + threadIdx.x;
+      if (p >= pairs) break;
+      const uint32_t x = hkey[slot[r]];
+      finc[fioff[x] + hcnt[slot[r]] + rank[r]] = (uint32_t)(c0 * k + p);
+    }
+    __syncthreads();
   }
 }
 
@@ -468,26 +497,24 @@ hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
   return hipGetLastError();
 }
 
-hipError_t item_core_count(uint64_t nitems, const uint4 *items,
-                           const uint32_t *inc, uint32_t ik,
-                           const uint32_t *core, uint32_t kp, uint32_t *cnt,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_item_core_count,
-                     dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
-                     nitems, items, inc, ik, core, kp, cnt);
+hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
+                                  uint32_t nv, uint32_t *fcd, hipStream_t s) {
+  if (k > 4) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(fcd, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
+  if (e != hipSuccess || Tc == 0) return e;
+  hipLaunchKernelGGL(k_flow_deg, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
+                     0, s, Tc, k, fmem, fcd);
   return hipGetLastError();
 }
 
-hipError_t item_core_write(uint64_t nitems, const uint4 *items,
-                           const uint64_t *item_out, const uint32_t *inc,
-                           uint32_t ik, const uint32_t *core, uint32_t kp,
-                           const uint32_t *cliques, uint32_t k,
-                           const uint32_t *cslot, uint32_t *finc,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_item_core_write,
-                     dim3(grid_for(nitems * kWave, 256, 16384)), dim3(256), 0, s,
-                     nitems, items, item_out, inc, ik, core, kp, cliques, k,
-                     cslot, finc);
+hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
+                               const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
+                               uint32_t *finc, hipStream_t s) {
+  if (k > 4) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(fcur, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
+  if (e != hipSuccess || Tc == 0) return e;
+  hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
+                     0, s, Tc, k, fmem, fioff, fcur, finc);
   return hipGetLastError();
 }
 

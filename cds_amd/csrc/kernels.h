@@ -285,22 +285,16 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               uint32_t ncomp, const uint64_t *coff,
                               uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
                               hipStream_t s);
-// Chunked per-vertex incidence work over core vertices.  items[i] =
-// {vertex, first entry (hi 32 bits), first entry (lo 32 bits), entry
-// count}; incidence entries are ik words wide, the clique id then its other
-// members.  A clique is in the core iff every other member has
-// core >= kp (the item's own vertex has), which reads the vertex-sized core
-// array instead of the clique-sized kill levels.
-hipError_t item_core_count(uint64_t nitems, const uint4 *items,
-                           const uint32_t *inc, uint32_t ik,
-                           const uint32_t *core, uint32_t kp, uint32_t *cnt,
-                           hipStream_t s);
-hipError_t item_core_write(uint64_t nitems, const uint4 *items,
-                           const uint64_t *item_out, const uint32_t *inc,
-                           uint32_t ik, const uint32_t *core, uint32_t kp,
-                           const uint32_t *cliques, uint32_t k,
-                           const uint32_t *cslot, uint32_t *finc,
-                           hipStream_t s);
+// Flow-network incidence from the core's clique bucket fmem (Tc cliques,
+// k flow-vertex ids each, nv flow vertices): fcd[v] = number of pairs of
+// vertex v (zeroed here); then, with fioff the exclusive prefix sum of fcd,
+// finc[fioff[v] ..) = the pair indices s*k+j whose vertex is v (fcur is
+// nv words of scratch, zeroed here).
+hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
+                                  uint32_t nv, uint32_t *fcd, hipStream_t s);
+hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
+                               const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
+                               uint32_t *finc, hipStream_t s);
 
 // ---- batched solves: the disjoint union of many graphs (batch.hip) ----
 struct UnionPart {
