@@ -497,13 +497,66 @@ hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
   return hipGetLastError();
 }
 
+// Cores of at most kFlowDirect vertices (the usual case: a densest core is
+// small): counters indexed by vertex in LDS, one contiguous clique range
+// per workgroup, one global atomic per (workgroup, vertex).  The fill pass
+// counts its range, reserves one run per vertex, then sweeps the range
+// again placing each pair at its run's LDS cursor.
+constexpr uint32_t kFlowDirect = 32768;
+
+__device__ __forceinline__ void flow_range(uint64_t Tc, uint64_t *lo, uint64_t *hi) {
+  const uint64_t per = (Tc + gridDim.x - 1) / gridDim.x;
+  *lo = min<uint64_t>(Tc, (uint64_t)blockIdx.x * per);
+  *hi = min<uint64_t>(Tc, *lo + per);
+}
+
+__global__ __launch_bounds__(kFlowThreads) void k_flow_deg_direct(
+    uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem, uint32_t nv,
+    uint32_t *__restrict__ fcd) {
+  __shared__ uint32_t cnt[kFlowDirect];
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) cnt[i] = 0;
+  __syncthreads();
+  uint64_t lo, hi;
+  flow_range(Tc, &lo, &hi);
+  for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads)
+    atomicAdd(&cnt[fmem[p]], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
+    if (cnt[i]) atomicAdd(&fcd[i], cnt[i]);
+}
+
+__global__ __launch_bounds__(kFlowThreads) void k_flow_fill_direct(
+    uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem, uint32_t nv,
+    const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
+    uint32_t *__restrict__ finc) {
+  __shared__ uint32_t cnt[kFlowDirect];
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) cnt[i] = 0;
+  __syncthreads();
+  uint64_t lo, hi;
+  flow_range(Tc, &lo, &hi);
+  for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads)
+    atomicAdd(&cnt[fmem[p]], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
+    if (cnt[i]) cnt[i] = atomicAdd(&fcur[i], cnt[i]);  // -> this range's run
+  __syncthreads();
+  for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads) {
+    const uint32_t x = fmem[p];
+    finc[fioff[x] + atomicAdd(&cnt[x], 1u)] = (uint32_t)p;
+  }
+}
+
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                   uint32_t nv, uint32_t *fcd, hipStream_t s) {
   if (k > 4) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(fcd, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
-  hipLaunchKernelGGL(k_flow_deg, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
-                     0, s, Tc, k, fmem, fcd);
+  if (nv <= kFlowDirect)
+    hipLaunchKernelGGL(k_flow_deg_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
+                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fcd);
+  else
+    hipLaunchKernelGGL(k_flow_deg, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
+                       0, s, Tc, k, fmem, fcd);
   return hipGetLastError();
 }
 
@@ -513,8 +566,12 @@ hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
   if (k > 4) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(fcur, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
-  hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
-                     0, s, Tc, k, fmem, fioff, fcur, finc);
+  if (nv <= kFlowDirect)
+    hipLaunchKernelGGL(k_flow_fill_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
+                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc);
+  else
+    hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
+                       0, s, Tc, k, fmem, fioff, fcur, finc);
   return hipGetLastError();
 }
 
