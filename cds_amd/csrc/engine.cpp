@@ -1132,6 +1132,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                              alive0, na0, shift, max_level, grid, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
     }
+    HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     R->t_peel_kernel_ms += tm.ms(22, 23);
     R->peel_launches++;
     CHK(readback(e, hc, ctl.p, 1));
@@ -1929,6 +1930,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
                              lkil.p,
                              sdeg.p, nullptr, 0, 0, kAliveHost, peel_grid, st));
     }
+    HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));
     if (hc.removed != n)
@@ -2722,8 +2724,8 @@ extern "C" cds_status cds_autotune(cds_engine *e, const cds_dgraph *g, const cds
   }
   CHK(try_values(0, c));
   c.clear();
-  for (unsigned b : {0u, 1u, 4u, 16u, cap / 4, cap / 2, cap}) {
-    if (b > cap || (b != 0 && b < rule_peel / 4)) continue;
+  for (unsigned b : {0u, 1u, 4u, 16u, cap / 4, cap / 2, cap, 2 * cap}) {
+    if (b > 2 * cap || (b != 0 && b < rule_peel / 4)) continue;
     cds_tuning t = cur;
     t.peel_blocks = b;
     c.push_back(t);

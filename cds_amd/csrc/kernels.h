@@ -154,12 +154,15 @@ struct PeelCtl {
 // alive0 (na0 vertices, or null for all n): the vertices alive at the
 // start; every other vertex must have core != kAlive and every clique
 // containing one must be dead or never reached.
-// kill[] (T words): every clique alive (kAlive) before the launch is set
-// after it to the level that killed it.  stamp[] (n words) holds each
+// kill[] (T words) is not touched: after the launch, peel_derive_kill sets
+// every clique still marked alive (kAlive) to the level that killed it, the
+// smallest core value among its members.  stamp[] (n words) holds each
 // vertex's sub-round stamp; sbase is the launch's first sub-round: 1 for the
 // first launch on a graph, above every stamp of earlier launches (their
 // sub-round counts + 1) for a re-peel.
 // Requires n < 2^24.  Synchronises nothing; read ctl after the stream.
+hipError_t peel_derive_kill(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *core, uint32_t *kill, hipStream_t s);
 hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *ioff,
                            const uint32_t *inc, const uint32_t *cliques,
                            uint32_t *deg, uint32_t *core, uint32_t *kill,

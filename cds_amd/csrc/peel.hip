@@ -1084,9 +1084,22 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
                   &deg, &core, &stamp, &sbase, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
                   &sdeg, &lv_min, &cshift, &max_level, (void *)&hagg};
-  if ((e = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
-                       kPeelThreads, args, 0, s)) != hipSuccess)
-    return e;
+  // The kernel adapts to any width (gridDim drives its loops and barriers).
+  // A cooperative launch over every CU can be refused while another process
+  // holds the device; it is then retried at half the width, twice at most.
+  hipError_t r = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
+                             kPeelThreads, args, 0, s);
+  for (int retry = 0; r != hipSuccess && grid > 1 && retry < 2; ++retry) {
+    (void)hipGetLastError();
+    grid = (grid + 1) / 2;
+    r = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
+                    kPeelThreads, args, 0, s);
+  }
+  return r;
+}
+
+hipError_t peel_derive_kill(uint64_t T, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *core, uint32_t *kill, hipStream_t s) {
   if (!T) return hipSuccess;
   hipLaunchKernelGGL(k_kill_from_core, dim3(grid_for(T, 256, 16384)), dim3(256), 0, s,
                      T, k, cliques, core, kill);
@@ -1184,19 +1197,22 @@ unsigned peel_grid_blocks(uint32_t n, uint64_t T, uint32_t k, int device,
   const unsigned cus = coop_cus(reinterpret_cast<const void *>(k_peel_persistent),
                                 kPeelThreads, device);
   if (cus == 0) return 0;
-  // 1024-thread workgroups, at most half the CUs: every grid barrier costs
-  // a release and an acquire fence per workgroup, and the levels of a
-  // power-law graph are too light to pay for more (R-MAT scale 20, k = 3:
-  // 410 ms with 128 workgroups, 459 ms with 256).  Smaller graphs get one
-  // workgroup per 32768 vertices or 2^22 incidence entries, so most run
-  // on one workgroup as a plain launch.
-  const uint64_t cap = cus > 1 ? cus / 2 : 1;
+  // 1024-thread workgroups, up to one per CU.  Every grid barrier costs a
+  // release and an acquire fence per workgroup, but once claims and
+  // decrements are aggregated the sub-rounds' chunk processing dominates
+  // and scales with the CUs (R-MAT scale 20, k = 3: 60 ms with 128
+  // workgroups, 42 with 192, 34 with 256; barriers 1.1 -> 1.6 ms).
+  // Smaller graphs get one workgroup per 32768 vertices or 2^22 incidence
+  // entries, so most run on one workgroup as a plain launch.
+  const uint64_t cap = cus;
   uint64_t want = std::max<uint64_t>((n + 32767) / 32768, (k * T + (1u << 22) - 1) >> 22);
   want = std::min(std::max<uint64_t>(want, 1), cap);
-  if (force >= 1) want = std::min<uint64_t>(force, cap);
+  // an explicit width (tuning, CDS_PEEL_BLOCKS) may take every CU: one
+  // 1024-thread workgroup per CU is co-resident
+  if (force >= 1) want = std::min<uint64_t>(force, cus);
   if (const char *gb = getenv("CDS_PEEL_BLOCKS")) {  // diagnostics: wins
     const long v = atol(gb);
-    if (v >= 1 && (uint64_t)v <= cap) want = (uint64_t)v;
+    if (v >= 1 && (uint64_t)v <= cus) want = (uint64_t)v;
   }
   return (unsigned)want;
 }
