@@ -858,10 +858,11 @@ unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
       hipSuccess)
     return 0;
-  // One 1024-thread workgroup per 64k (clique, member) arcs, at most half
-  // the CUs: each of the four grid barriers of a round costs a fence pair
-  // per workgroup, so small networks run on one workgroup.
-  const uint64_t cap = cus > 1 ? (uint64_t)cus / 2 : 1;
+  // One 1024-thread workgroup per 64k (clique, member) arcs, up to one per
+  // CU: each of the four grid barriers of a round costs a fence pair per
+  // workgroup, so small networks run on one workgroup, while on large ones
+  // the vertex discharges dominate and scale with the CUs.
+  const uint64_t cap = cus > 1 ? (uint64_t)cus : 1;
   const uint64_t arcs = (uint64_t)P.T * P.k;
   uint64_t want = (arcs + 65535) / 65536;
   if (want > cap) want = cap;
@@ -881,8 +882,18 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
   FlowProblem p = P;
   GridFlowState st = S;
   void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
-  return launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid,
-                     kGridFlowThreads, args, lds, s);
+  // The kernel adapts to any width (gridDim drives its loops and barriers).
+  // A cooperative launch over every CU can be refused while another process
+  // holds the device; it is then retried at half the width, twice at most.
+  hipError_t r = launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid,
+                             kGridFlowThreads, args, lds, s);
+  for (int retry = 0; r != hipSuccess && grid > 1 && retry < 2; ++retry) {
+    (void)hipGetLastError();
+    grid = (grid + 1) / 2;
+    r = launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid, kGridFlowThreads,
+                    args, lds, s);
+  }
+  return r;
 }
 
 }  // namespace cds
