@@ -32,19 +32,19 @@ namespace {
 // Slots are reserved with wave-aggregated cursor increments: consecutive
 // cliques share their lowest vertex (they come from the same listing item)
 // and hubs recur, so most lanes of a wave share one atomic.
-// Incidence build.  A workgroup takes a tile of kIncTile cliques and
+// Incidence build.  A workgroup takes a tile of kIncPairs / k cliques and
 // aggregates its (vertex, clique) pairs in an LDS hash table keyed by
 // vertex: every pair gets its rank among the tile's pairs of that vertex
 // from an LDS atomic, then one global atomicAdd per distinct vertex of the
 // tile reserves a run of the vertex's incidence list.  Hub vertices, which
 // appear in most cliques of a power-law graph, thus cost one global atomic
 // per tile instead of one per clique.
-// 2048-clique tiles (128 KB of LDS, one workgroup per CU): larger tiles
-// share more reservations; on R-MAT scale 20 k=3 the build took 71 ms with
-// 512-clique tiles, 41 ms with 1024 and 29 ms with 2048.
+// Tiles of 8192 (vertex, clique) pairs, 8192/k cliques (128 KB of LDS, one
+// workgroup per CU): larger tiles share more reservations; on R-MAT scale
+// 20 k=3 the build took 71 ms with 512-clique tiles, 41 ms with 1024 and
+// 29 ms with 2048.
 constexpr uint32_t kIncThreads = 1024;
-constexpr uint32_t kIncTile = 2048;                      // cliques per tile
-constexpr uint32_t kIncPairs = kIncTile * 4;             // k <= 4
+constexpr uint32_t kIncPairs = 8192;                     // pairs per tile: 8192/k cliques
 constexpr uint32_t kIncHash = 2 * kIncPairs;             // load <= 1/2
 constexpr uint32_t kIncPerThread = kIncPairs / kIncThreads;
 
@@ -54,15 +54,16 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
     uint32_t *__restrict__ inc) {
   __shared__ uint32_t hkey[kIncHash];
   __shared__ uint32_t hcnt[kIncHash];
-  const uint64_t tiles = (T + kIncTile - 1) / kIncTile;
+  const uint32_t tile_c = kIncPairs / k;  // cliques per tile: kIncPairs pairs
+  const uint64_t tiles = (T + tile_c - 1) / tile_c;
   for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads) {
       hkey[i] = kAlive;
       hcnt[i] = 0;
     }
     __syncthreads();
-    const uint64_t c0 = tile * kIncTile;
-    const uint32_t pairs = (uint32_t)min<uint64_t>(kIncTile, T - c0) * k;
+    const uint64_t c0 = tile * tile_c;
+    const uint32_t pairs = (uint32_t)min<uint64_t>(tile_c, T - c0) * k;
     uint32_t slot[kIncPerThread], rank[kIncPerThread];
 #pragma unroll
     for (uint32_t r = 0; r < kIncPerThread; ++r) {
@@ -973,7 +974,8 @@ hipError_t build_incidence(uint64_t T, uint32_t k, const uint32_t *cliques,
                            const uint64_t *ioff, uint32_t *icur,
                            uint32_t *inc, hipStream_t s) {
   if (k > 4) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_build_inc, dim3(grid_for(T, kIncTile, 16384)),
+  if (k < 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_build_inc, dim3(grid_for(T, kIncPairs / k, 16384)),
                      dim3(kIncThreads), 0, s, T, k, cliques, ioff, icur, inc);
   return hipGetLastError();
 }
