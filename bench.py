@@ -300,6 +300,21 @@ def peel_bytes(n, T, k):
     return 4 * k * T + (4 * k + 8) * T + 8 * (k - 1) * T + 12 * n
 
 
+def flow_bytes(r, k):
+    """Algorithmic HBM bytes of the whole-network max flows of one solve, from
+    the work the rounds did: per clique work-list visit its excess and
+    height (12 B) and its k arcs (member id 4 B, residual 8 B, member height
+    4 B); per vertex incidence arc posted for discharge its clique slot
+    (4 B), residual (8 B) and clique height (4 B); per global relabel pass
+    every arc once (16 B, the per-pass figure of a BFS over the network);
+    per round the vertex state (excess 8 B, height 4 B).  The per-sweep
+    figure 16|E| + 12|V| would count every arc in every round, which the
+    work lists avoid."""
+    arcs = k * r["core_cliques"]
+    return (r["flow_clique_visits"] * (12 + 16 * k) + r["flow_arc_visits"] * 16
+            + r["flow_relabels"] * 16 * arcs + r["flow_sweeps"] * 12 * r["core_size"])
+
+
 def percentile(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -484,6 +499,9 @@ def main():
         "k_peel_persistent": {"launches_per_solve": peel_launches,
                               "total_ms_per_solve": avg("t_peel_kernel_ms"),
                               "algorithmic_bytes_per_solve": peel_bytes(g.n, T, w["k"])},
+        "k_grid_flow": {"launches_per_solve": avg("flow_launches") or 1,
+                        "total_ms_per_solve": avg("t_flow_kernel_ms"),
+                        "algorithmic_bytes_per_solve": flow_bytes(r, w["k"])},
         "k_build_inc": {"launches_per_solve": 1,
                         "total_ms_per_solve": avg("t_incidence_ms"),
                         # read the cliques, write k entries of k words each

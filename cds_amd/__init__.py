@@ -100,6 +100,8 @@ class CResult(ctypes.Structure):
         ("peel_split_ms", ctypes.c_double * 4),
         ("t_incidence_ms", ctypes.c_double), ("t_peel_kernel_ms", ctypes.c_double),
         ("t_replay_ms", ctypes.c_double), ("t_replay_prep_ms", ctypes.c_double),
+        ("t_flow_kernel_ms", ctypes.c_double), ("flow_clique_visits", ctypes.c_uint64),
+        ("flow_arc_visits", ctypes.c_uint64), ("flow_relabels", ctypes.c_uint64),
         ("times", CTimes)]
 
 

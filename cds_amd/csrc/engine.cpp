@@ -1650,10 +1650,16 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       const unsigned fgrid = flow_grid_blocks(P, e->device, flow_blocks_of(e));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
+      HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, fgrid, st));
+      HIPCHK(hipEventRecord(tm.ev[23], st));
     }
     GridFlowCtl hc;
     CHK(readback(e, &hc, gctl.p, 1));
+    R->t_flow_kernel_ms += tm.ms(22, 23);
+    R->flow_clique_visits += hc.clique_visits;
+    R->flow_arc_visits += hc.chunk_visits * kFlowChunkArcs;
+    R->flow_relabels += hc.relabels;
     if (hc.limit)
       return fail(CDS_ERR_NOT_CONVERGED,
                   "grid max flow of component %u hit %u rounds", ci, hc.rounds);

@@ -50,7 +50,7 @@
 namespace cds {
 namespace {
 
-constexpr uint32_t kGridChunk = 512;
+constexpr uint32_t kGridChunk = kFlowChunkArcs;
 constexpr uint32_t kGridLdsVertices = 4096;  // LDS excess accumulator, 32 KB
 // Rounds between global relabels: a relabel reads the whole network, so
 // large networks relabel rarely (measured on R-MAT cores of 2.3e8 / 7.2e8
@@ -455,7 +455,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
   unsigned long long active = global_relabel(g, kArcFull);
   int wb = 0;  // clique work list of this round (global_relabel fills cq0)
   uint32_t rounds = 0, relabels = 1, since = 0;
-  unsigned long long sink = 0;
+  unsigned long long sink = 0, wl_visits = 0, ch_visits = 0;
   bool limit = false;
   while (active) {
     if (rounds >= max_rounds) {
@@ -703,6 +703,8 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       ++relabels;
       since = 0;
     }
+    wl_visits += nwl;
+    ch_visits += nch;
     if (S.trace && g.leader && rounds - 1 < S.trace_cap) {
       unsigned long long *t = S.trace + 8ull * (rounds - 1);
       t[0] = nwl;
@@ -724,6 +726,8 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     ctl->rounds = rounds;
     ctl->limit = limit ? 1 : 0;
     ctl->relabels = relabels;
+    ctl->clique_visits = wl_visits;
+    ctl->chunk_visits = ch_visits;
   }
 }
 

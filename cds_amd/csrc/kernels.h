@@ -364,7 +364,12 @@ struct GridFlowCtl {
   unsigned long long active;
   unsigned long long sink;      // flow into the sink (scaled)
   unsigned int rounds, limit, relabels, pad;
+  // work done, summed over rounds: clique work-list entries visited in
+  // phase 1, incidence chunks posted by active vertices in phase 2
+  unsigned long long clique_visits, chunk_visits;
 };
+// Arcs per vertex incidence chunk of the max-flow kernels.
+constexpr uint32_t kFlowChunkArcs = 512;
 struct GridFlowState {
   long long *x;            // T*k
   long long *rvs, *exv, *rem;  // V each

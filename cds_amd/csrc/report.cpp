@@ -142,6 +142,9 @@ std::string render(const cds_result *r) {
   j.u64("flow_launches", r->flow_launches);
   j.u64("density_certificates", r->density_certificates);
   j.u64("flow_sweeps", r->flow_sweeps);
+  j.u64("flow_clique_visits", r->flow_clique_visits);
+  j.u64("flow_arc_visits", r->flow_arc_visits);
+  j.u64("flow_relabels", r->flow_relabels);
   j.open("trace", '[');
   for (uint32_t i = 0; i < r->trace_len; ++i) {
     j.open(nullptr, '{');

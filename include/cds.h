@@ -148,6 +148,13 @@ typedef struct {
                                        (HIP events) */
   double t_replay_ms;               /* exact phase replays (HIP events) */
   double t_replay_prep_ms;          /* ... of which phase incidence compaction */
+  double t_flow_kernel_ms;          /* whole-network max-flow launches, summed
+                                       (HIP events) */
+  uint64_t flow_clique_visits;      /* clique work-list entries visited by
+                                       those flows (phase 1, all rounds) */
+  uint64_t flow_arc_visits;         /* vertex incidence arcs posted for
+                                       discharge (phase 2, all rounds) */
+  uint64_t flow_relabels;           /* global relabel passes of those flows */
   cds_times times;
 } cds_result;
 
@@ -162,8 +169,9 @@ typedef struct {
 } cds_opts;
 
 /* Launch shapes of the stages that have a choice (no counterpart in the
- * reference, whose shapes are compile-time: BLOCK_SIZE / warps per block,
- * cudaCode/inc/common.h).  Zero in a field means "the engine's rule". */
+ * reference, whose shapes are compile-time: BLK_NUMS = 216 blocks of
+ * BLK_DIM = 1024 threads, cudaCode/inc/common.h:30-36).  Zero in a field
+ * means "the engine's rule". */
 typedef struct {
   uint32_t list_threads;      /* listing workgroup lanes: 128, 256 or 512 */
   uint32_t list_chunk;        /* out-neighbours per listing work item: 32 or
