@@ -54,14 +54,20 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
     uint32_t *__restrict__ inc) {
   __shared__ uint32_t hkey[kIncHash];
   __shared__ uint32_t hcnt[kIncHash];
+  // occupied slots of the tile, so the reservation pass and the reset visit
+  // the tile's distinct vertices only (a tile holds at most kIncPairs)
+  __shared__ uint16_t used[kIncPairs];
+  __shared__ uint32_t nused;
+  static_assert(kIncHash <= 65536, "slot indices are 16-bit");
   const uint32_t tile_c = kIncPairs / k;  // cliques per tile: kIncPairs pairs
   const uint64_t tiles = (T + tile_c - 1) / tile_c;
+  for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads) {
+    hkey[i] = kAlive;
+    hcnt[i] = 0;
+  }
+  if (threadIdx.x == 0) nused = 0;
+  __syncthreads();
   for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads) {
-      hkey[i] = kAlive;
-      hcnt[i] = 0;
-    }
-    __syncthreads();
     const uint64_t c0 = tile * tile_c;
     const uint32_t pairs = (uint32_t)min<uint64_t>(tile_c, T - c0) * k;
     uint32_t slot[kIncPerThread], rank[kIncPerThread];
@@ -73,15 +79,22 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
       uint32_t h = (x * 2654435761u) & (kIncHash - 1);
       for (;;) {
         const uint32_t prev = atomicCAS(&hkey[h], kAlive, x);
-        if (prev == kAlive || prev == x) break;
+        if (prev == kAlive) {
+          used[atomicAdd(&nused, 1u)] = (uint16_t)h;
+          break;
+        }
+        if (prev == x) break;
         h = (h + 1) & (kIncHash - 1);
       }
       slot[r] = h;
       rank[r] = atomicAdd(&hcnt[h], 1u);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kIncHash; i += kIncThreads)
-      if (hcnt[i]) hcnt[i] = atomicAdd(&icur[hkey[i]], hcnt[i]);  // -> base
+    const uint32_t nu = nused;
+    for (uint32_t i = threadIdx.x; i < nu; i += kIncThreads) {
+      const uint32_t h = used[i];
+      hcnt[h] = atomicAdd(&icur[hkey[h]], hcnt[h]);  // -> base
+    }
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < kIncPerThread; ++r) {
@@ -95,6 +108,13 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
       for (uint32_t t = 0, o = 1; t < k; ++t)
         if (t != j) e[o++] = cl[c * k + t];
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nu; i += kIncThreads) {
+      const uint32_t h = used[i];
+      hkey[h] = kAlive;
+      hcnt[h] = 0;
+    }
+    if (threadIdx.x == 0) nused = 0;
     __syncthreads();
   }
 }
