@@ -432,6 +432,11 @@ __device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
 // The max flow of one problem by the threads of g (a whole cooperative
 // grid, or one workgroup of a batched launch).  lds_ex (LDS, V entries if
 // lds_v) accumulates the excess cliques push to vertices.
+// kCacheArcs: the discharge's pass 1 keeps its admissible arcs in
+// registers for pass 2 (the whole-grid kernel, one workgroup per CU); the
+// batched kernel re-reads them and keeps its registers for two workgroups
+// per CU.
+template <bool kCacheArcs>
 __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long U, uint32_t max_rounds,
                          long long *lds_ex, bool lds_v) {
@@ -582,7 +587,10 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
     // non-admissible height), one atomic reserves that much of the vertex's
     // remaining excess, pass 2 hands the reservation to the admissible arcs
     // in chunk order (wave prefix sums).  A slot belongs to one chunk and
-    // clique heights do not change in this phase, so both passes agree.
+    // clique heights do not change in this phase, so both passes agree, and
+    // with kCacheArcs pass 1 keeps each lane's admissible residuals and
+    // slots in registers for pass 2 instead of re-reading them (random
+    // lines, long evicted).
     for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
       uint32_t v;
       uint64_t lo, hi;
@@ -591,15 +599,38 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
       const int hvv = g.hv[v];
       int hm = g.INF;
       long long radm = 0;
-      for (uint64_t p = lo + lane(); p < hi; p += kWave) {
-        const uint64_t s = g.inc[p] - g.sbase;
-        const long long res = (long long)(g.k - 1) * U - g.x[s];
-        if (res <= 0) continue;
-        const int hcc = g.hc[s / g.k];
-        if (hcc < hvv)
-          radm += res;
-        else
-          hm = min(hm, hcc);
+      long long rav[kCacheArcs ? kChunkSteps : 1];
+      uint32_t sv[kCacheArcs ? kChunkSteps : 1];
+      if constexpr (kCacheArcs) {
+#pragma unroll
+        for (int it = 0; it < kChunkSteps; ++it) {
+          rav[it] = 0;
+          sv[it] = 0;
+          const uint64_t p = lo + (uint64_t)it * kWave + lane();
+          if (p >= hi) continue;
+          const uint64_t s = g.inc[p] - g.sbase;
+          const long long res = (long long)(g.k - 1) * U - g.x[s];
+          if (res <= 0) continue;
+          const int hcc = g.hc[s / g.k];
+          if (hcc < hvv) {
+            radm += res;
+            rav[it] = res;
+            sv[it] = (uint32_t)s;
+          } else {
+            hm = min(hm, hcc);
+          }
+        }
+      } else {
+        for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+          const uint64_t s = g.inc[p] - g.sbase;
+          const long long res = (long long)(g.k - 1) * U - g.x[s];
+          if (res <= 0) continue;
+          const int hcc = g.hc[s / g.k];
+          if (hcc < hvv)
+            radm += res;
+          else
+            hm = min(hm, hcc);
+        }
       }
       radm = (long long)wave_sum64((uint64_t)radm);
 #pragma unroll
@@ -624,13 +655,18 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         fc[it] = 0;
         const uint64_t p0 = lo + (uint64_t)it * kWave;
         if (p0 >= hi || before >= avail) continue;  // wave-uniform
-        const uint64_t p = p0 + lane();
         long long ra = 0;
         uint64_t s = 0;
-        if (p < hi) {
-          s = g.inc[p] - g.sbase;
-          const long long res = (long long)(g.k - 1) * U - g.x[s];
-          if (res > 0 && g.hc[s / g.k] < hvv) ra = res;
+        if constexpr (kCacheArcs) {
+          ra = rav[it];
+          s = sv[it];
+        } else {
+          const uint64_t p = p0 + lane();
+          if (p < hi) {
+            s = g.inc[p] - g.sbase;
+            const long long res = (long long)(g.k - 1) * U - g.x[s];
+            if (res > 0 && g.hc[s / g.k] < hvv) ra = res;
+          }
         }
         long long incl = ra;
 #pragma unroll
@@ -747,7 +783,7 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   __shared__ uint32_t app[kMaxFlowWaves + 1];
   g.app = app;
   extern __shared__ long long lds_ex[];
-  flow_run(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
+  flow_run<true>(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
 }
 
 // Many independent problems in one launch; item i owns workgroups
@@ -779,7 +815,7 @@ __global__ __launch_bounds__(kFlowBatchThreads) void k_flow_batch(
   __shared__ uint32_t app[kMaxFlowWaves + 1];
   g.app = app;
   extern __shared__ long long lds_ex[];
-  flow_run(g, it.P, it.S, it.capT, it.U, max_rounds, lds_ex, g.V <= lds_cap);
+  flow_run<false>(g, it.P, it.S, it.capT, it.U, max_rounds, lds_ex, g.V <= lds_cap);
 }
 
 // Cliques of each batched problem with every member on its min-cut side.
