@@ -426,7 +426,7 @@ def main():
                     choices=sorted(WORKLOADS) + ["c4", "c5"],
                     help="c4: batched form; c5: served form")
     ap.add_argument("--graphs", type=int, default=4096, help="c4: graphs")
-    ap.add_argument("--chunk", type=int, default=512, help="c4: graphs per batched call")
+    ap.add_argument("--chunk", type=int, default=4096, help="c4: graphs per batched call")
     ap.add_argument("--duration", type=float, default=20.0, help="c5: seconds per load")
     ap.add_argument("--loads", default="20,60,180", help="c5: offered requests/s")
     ap.add_argument("--seed", type=int, default=4)

@@ -1927,7 +1927,12 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     HIPCHK(ctl.alloc(1));
     DBuf<uint32_t> stamp(&A);
     HIPCHK(stamp.alloc(n));
-    peel_grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
+    // A union peels exact levels of every graph at once, thousands of
+    // barrier-ended sub-rounds: the width tuned for one graph of the same
+    // edge class does not transfer.  The built-in rule capped at 64
+    // workgroups measured best (4096-graph mix: k=3 unions 79 / 55 ms with
+    // 64 workgroups, 112 / 85 ms with 256).
+    peel_grid = std::min(peel_grid_blocks(n, T, k, e->device, 0), 64u);
     if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     {
       CoopGate gate(e, peel_grid);
