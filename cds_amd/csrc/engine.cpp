@@ -1905,7 +1905,11 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(icur.alloc(n));
   HIPCHK(icur.zero());
   HIPCHK(inc.alloc(T * k * k));
+  HIPCHK(hipEventRecord(tm.ev[22], st));
   HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
+  HIPCHK(hipEventRecord(tm.ev[23], st));
+  // sub-stage kernel times of the union (the same for every graph in it)
+  double t_inc = tm.ms(22, 23), t_peel = 0, t_replay = 0;
   icur.reset();
   HIPCHK(deg.alloc(n));
   HIPCHK(core.alloc(n));
@@ -1936,11 +1940,14 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     {
       CoopGate gate(e, peel_grid);
+      HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
                              kill.p, stamp.p, 1, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
                              lkil.p,
                              sdeg.p, nullptr, 0, 0, kAliveHost, peel_grid, st));
+      HIPCHK(hipEventRecord(tm.ev[23], st));
     }
+    t_peel = tm.ms(22, 23);
     HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));
@@ -2029,9 +2036,12 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       HIPCHK(rdead.zero());
       HIPCHK(rout.alloc(np));
       HIPCHK(positions_of(n, sval.p, posmap.p, st));
+      HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(replay_phases_multi(k, np, dph.p, n, sval.p, sdeg.p, pdeg.p, posmap.p,
                                  palive.p, core.p, kill.p, ioff.p, inc.p, pioff.p,
                                  pinc.p, rdead.p, rout.p, A, st));
+      HIPCHK(hipEventRecord(tm.ev[23], st));
+      t_replay = tm.ms(22, 23);
       std::vector<ReplayOut> ro(np);
       CHK(readback(e, ro.data(), rout.p, np));
       for (uint32_t i = 0; i < np; ++i) {
@@ -2044,6 +2054,11 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     }
   }
   const int ev_peel = tm.mark();
+  for (uint32_t b = 0; b < B; ++b) {
+    rs[b]->t_incidence_ms = t_inc;
+    rs[b]->t_peel_kernel_ms = t_peel;
+    rs[b]->t_replay_ms = t_replay;
+  }
 
   // ---- densest cores: per-graph k' turned into 0/1 flags ----
   std::vector<uint32_t> kpb(B);
