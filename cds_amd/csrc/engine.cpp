@@ -1998,10 +1998,12 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     if (s != CDS_OK) return s;
   }
 
-  // ---- exact replays: every candidate phase of every graph at once ----
+  // ---- exact replays: every candidate phase of every graph at once; a
+  // phase with few vertices and many cliques each (the plain path's rule)
+  // runs on the whole GPU instead of one workgroup ----
   {
-    std::vector<ReplayPhase> phases;
-    std::vector<uint32_t> phase_graph;
+    std::vector<ReplayPhase> phases, heavy;
+    std::vector<uint32_t> phase_graph, heavy_graph;
     uint64_t pinc_words = 0;
     for (uint32_t b = 0; b < B; ++b)
       for (size_t ii : PS[b].replay) {
@@ -2013,27 +2015,35 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
         ph.pad = 0;
         ph.T_start = Tb[b] - PS[b].kl_before[ii];
         ph.N_start = gs[b]->n - PS[b].rm_before[ii];
+        if (ph.M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * ph.M) {
+          heavy.push_back(ph);
+          heavy_graph.push_back(b);
+          continue;
+        }
         phases.push_back(ph);
         phase_graph.push_back(b);
         pinc_words += lv.killed * k * k;
       }
+    DBuf<uint32_t> pdeg(&A), posmap(&A), rdead(&A);
+    DBuf<uint8_t> palive(&A);
+    if (!phases.empty() || !heavy.empty()) {
+      HIPCHK(pdeg.alloc(n));
+      HIPCHK(posmap.alloc(n));
+      HIPCHK(palive.alloc(n));
+      HIPCHK(rdead.alloc(T));
+      HIPCHK(rdead.zero());
+    }
     if (!phases.empty()) {
       const uint32_t np = (uint32_t)phases.size();
       DBuf<ReplayPhase> dph(&A);
-      DBuf<uint32_t> pdeg(&A), posmap(&A), pinc(&A), rdead(&A);
-      DBuf<uint8_t> palive(&A);
+      DBuf<uint32_t> pinc(&A);
       DBuf<uint64_t> pioff(&A);
       DBuf<ReplayOut> rout(&A);
       HIPCHK(dph.alloc(np));
       HIPCHK(hipMemcpyAsync(dph.p, phases.data(), np * sizeof(ReplayPhase),
                             hipMemcpyHostToDevice, st));
-      HIPCHK(pdeg.alloc(n));
-      HIPCHK(posmap.alloc(n));
-      HIPCHK(palive.alloc(n));
       HIPCHK(pioff.alloc((size_t)n + 1));
       HIPCHK(pinc.alloc(pinc_words));
-      HIPCHK(rdead.alloc(T));
-      HIPCHK(rdead.zero());
       HIPCHK(rout.alloc(np));
       HIPCHK(positions_of(n, sval.p, posmap.p, st));
       HIPCHK(hipEventRecord(tm.ev[22], st));
@@ -2041,7 +2051,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
                                  palive.p, core.p, kill.p, ioff.p, inc.p, pioff.p,
                                  pinc.p, rdead.p, rout.p, A, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
-      t_replay = tm.ms(22, 23);
+      t_replay += tm.ms(22, 23);
       std::vector<ReplayOut> ro(np);
       CHK(readback(e, ro.data(), rout.p, np));
       for (uint32_t i = 0; i < np; ++i) {
@@ -2051,6 +2061,43 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
         apply_replay(&PS[phase_graph[i]], ro[i]);
         rs[phase_graph[i]]->replay_phases++;
       }
+    }
+    for (size_t i = 0; i < heavy.size(); ++i) {
+      // one phase at a time on the whole GPU; its vertices are a run of the
+      // (graph, core)-sorted list, its cliques its own graph's
+      const ReplayPhase &ph = heavy[i];
+      const uint32_t b = heavy_graph[i];
+      uint64_t killed = 0;
+      for (const Level &lv : levels[b])
+        if (lv.level == ph.level) killed = lv.killed;
+      const unsigned grid = replay_grid_blocks(e->device, killed * k / ph.M,
+                                               e->cur.replay_blocks);
+      DBuf<uint32_t> partial(&A), pcount(&A);
+      DBuf<GridBarrier> gbar(&A);
+      DBuf<ReplayOut> hout(&A);
+      HIPCHK(partial.alloc((size_t)grid * ph.M));
+      HIPCHK(pcount.alloc(1));
+      HIPCHK(gbar.alloc(1));
+      HIPCHK(hout.alloc(1));
+      const uint64_t kp = (uint64_t)PS[b].kp;
+      {
+        CoopGate gate(e, grid);
+        HIPCHK(hipEventRecord(tm.ev[22], st));
+        HIPCHK(replay_phase_list(k, ph.level, sval.p + ph.base, ph.M, core.p, kill.p, ioff.p,
+                                 inc.p, ph.T_start, ph.N_start, pdeg.p, posmap.p, palive.p,
+                                 rdead.p, pcount.p, hout.p, partial.p, gbar.p, grid, sdeg.p,
+                                 kp, replay_nstar(k, kp), st));
+        HIPCHK(hipEventRecord(tm.ev[23], st));
+      }
+      t_replay += tm.ms(22, 23);
+      ReplayOut ro;
+      CHK(readback(e, &ro, hout.p, 1));
+      if (ro.steps > ph.M || (!ro.stopped && ro.steps != ph.M))
+        return fail(CDS_ERR_INTERNAL, "batch item %u: replay of level %u ran %u of %u steps",
+                    b, ph.level, ro.steps, ph.M);
+      apply_replay(&PS[b], ro);
+      rs[b]->replay_phases++;
+      rs[b]->replay_grid_phases++;
     }
   }
   const int ev_peel = tm.mark();

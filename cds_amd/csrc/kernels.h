@@ -224,6 +224,18 @@ hipError_t replay_phases_multi(uint32_t k, uint32_t nphases,
 // full incidence lists; partial holds grid * M words, bar one GridBarrier,
 // and sdeg (if not null) the degrees recorded by peel_persistent.
 // `prepared` (if not null) is recorded before the replay kernel.
+// Whole-GPU replay of one phase given its vertex list pv[0, M) (ascending),
+// for a phase of one graph inside a disjoint union, where the level value
+// does not identify the phase: posmap entries of pv's vertices are set to
+// their positions; sdeg holds the degrees at the level's start.
+hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uint32_t M,
+                             const uint32_t *core, const uint32_t *kill,
+                             const uint64_t *ioff, const uint32_t *inc,
+                             uint64_t T_start, uint64_t N_start, uint32_t *pdeg,
+                             uint32_t *posmap, uint8_t *palive, uint32_t *rdead,
+                             uint32_t *pcount, ReplayOut *out, uint32_t *partial,
+                             GridBarrier *bar, unsigned grid, const uint32_t *sdeg,
+                             uint64_t kp, uint64_t nstar, hipStream_t s);
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,

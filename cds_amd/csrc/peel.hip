@@ -1128,6 +1128,39 @@ hipError_t peel_derive_kill(uint64_t T, uint32_t k, const uint32_t *cliques,
   return hipGetLastError();
 }
 
+__global__ void k_posmap_local(uint32_t M, const uint32_t *__restrict__ pv,
+                               uint32_t *__restrict__ posmap) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M;
+       i += gridDim.x * blockDim.x)
+    posmap[pv[i]] = i;
+}
+
+hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uint32_t M,
+                             const uint32_t *core, const uint32_t *kill,
+                             const uint64_t *ioff, const uint32_t *inc,
+                             uint64_t T_start, uint64_t N_start, uint32_t *pdeg,
+                             uint32_t *posmap, uint8_t *palive, uint32_t *rdead,
+                             uint32_t *pcount, ReplayOut *out, uint32_t *partial,
+                             GridBarrier *bar, unsigned grid, const uint32_t *sdeg,
+                             uint64_t kp, uint64_t nstar, hipStream_t s) {
+  if (grid < 1 || M == 0 || M > kReplayLdsCap) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(pcount), M, 1, s);
+  if (e != hipSuccess) return e;
+  if ((e = hipMemsetAsync(out, 0, sizeof(ReplayOut), s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_posmap_local, dim3(grid_for(M, 256, 4096)), dim3(256), 0, s, M, pv,
+                     posmap);
+  hipLaunchKernelGGL(k_phase_deg_snap, dim3(grid_for(M, 256, 4096)), dim3(256), 0, s,
+                     pcount, pv, sdeg, pdeg, palive);
+  hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  uint32_t *pvm = const_cast<uint32_t *>(pv);  // the kernel reads it only
+  void *args[] = {&k, &level, &M, &pvm, &pdeg, (void *)&posmap, &palive,
+                  (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
+                  &rdead, &partial, &bar, &T_start, &N_start, &out, &kp, &nstar};
+  return launch_grid(reinterpret_cast<const void *>(k_replay_grid), grid, kReplayThreads,
+                     args, 0, s);
+}
+
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,
