@@ -281,6 +281,17 @@ constexpr uint32_t kGridFlowRounds = 20000;
 constexpr uint64_t kCoarseMinSlots = 1ull << 24;
 constexpr uint32_t kFlowTraceRounds = 4096;
 
+// Whole-GPU replay of heavy phases inside batched unions: off unless
+// CDS_UNION_GRID_REPLAY=1.  A served stream (batch lane and large lane
+// running side by side) stalled with it on; the cause is not known.
+static bool union_grid_replay() {
+  static const bool on = [] {
+    const char *v = getenv("CDS_UNION_GRID_REPLAY");
+    return v && *v && *v != '0';
+  }();
+  return on;
+}
+
 static bool flow_trace_on() {
   const char *v = getenv("CDS_FLOW_TRACE");
   return v && *v && *v != '0';
@@ -2015,7 +2026,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
         ph.pad = 0;
         ph.T_start = Tb[b] - PS[b].kl_before[ii];
         ph.N_start = gs[b]->n - PS[b].rm_before[ii];
-        if (ph.M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * ph.M) {
+        if (union_grid_replay() && ph.M <= kReplayGridMaxVertices &&
+            lv.killed * k >= e->replay_grid_work * ph.M) {
           heavy.push_back(ph);
           heavy_graph.push_back(b);
           continue;
