@@ -59,6 +59,10 @@ struct cds_engine {
   std::vector<TuneEntry> tuned;
   cds_tuning cur{0, 0, 0, -1, 0, 0};
   bool forced = false;  // autotune sweep: `cur` is set by the sweep
+  // Engines that share a device with another engine (the server's lanes)
+  // keep every cooperative kernel within half the CUs, so neither lane's
+  // kernel needs a CU the other lane's kernels may hold.
+  bool coop_half = false;
 };
 
 namespace {
@@ -70,6 +74,12 @@ cds_tuning tuning_for(const cds_engine *e, uint32_t k, uint64_t m) {
 }
 void begin_solve(cds_engine *e, uint32_t k, uint64_t m) {
   if (!e->forced) e->cur = tuning_for(e, k, m);
+}
+// Cooperative width of this engine: `grid` capped at half the CUs when the
+// engine shares its device.
+unsigned coop_width(const cds_engine *e, unsigned grid) {
+  if (!e->coop_half || e->cus < 2) return grid;
+  return std::min(grid, std::max(1u, e->cus / 2));
 }
 ListShape list_shape(const cds_engine *e) {
   ListShape s;
@@ -700,6 +710,10 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
 }
 
 namespace cds {
+void share_device(cds_engine *e, bool shared) {
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->coop_half = shared;
+}
 void copy_tuning(cds_engine *dst, cds_engine *src) {
   std::lock_guard<std::mutex> lk(src->mu);
   dst->deflt = src->deflt;
@@ -1184,7 +1198,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(ctl.alloc(1));
     HIPCHK(sdeg.alloc(n));
     HIPCHK(stamp.alloc(n));
-    const unsigned grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
+    const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     PeelCtl hc;
     CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
@@ -1248,7 +1262,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     // as wide as the first pass: the re-peel is bound by its cliques'
     // kills (measured 127 ms against 157 ms with a grid sized by the
     // step's own clique count on R-MAT scale 20, k = 3)
-    const unsigned grid = peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks);
+    const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
     if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     // Exact levels at once: re-peeling a step in narrower steps first and
     // refining those re-processes the step's cliques at every depth, which
@@ -1335,7 +1349,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       // Few vertices with many cliques each: replay on the whole GPU.
       unsigned rgrid = 0;
       if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
-        rgrid = replay_grid_blocks(e->device, lv.killed * k / M, e->cur.replay_blocks);
+        rgrid = coop_width(e, replay_grid_blocks(e->device, lv.killed * k / M,
+                                                 e->cur.replay_blocks));
       if (rgrid) {
         HIPCHK(partial.alloc((size_t)rgrid * M));
         HIPCHK(gbar.alloc(1));
@@ -1658,7 +1673,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       G.trace_cap = kFlowTraceRounds;
     }
     {
-      const unsigned fgrid = flow_grid_blocks(P, e->device, flow_blocks_of(e));
+      const unsigned fgrid = coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e)));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
@@ -1947,7 +1962,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     // edge class does not transfer.  The built-in rule capped at 64
     // workgroups measured best (4096-graph mix: k=3 unions 79 / 55 ms with
     // 64 workgroups, 112 / 85 ms with 256).
-    peel_grid = std::min(peel_grid_blocks(n, T, k, e->device, 0), 64u);
+    peel_grid = coop_width(e, std::min(peel_grid_blocks(n, T, k, e->device, 0), 64u));
     if (peel_grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
     {
       CoopGate gate(e, peel_grid);
@@ -2082,8 +2097,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       uint64_t killed = 0;
       for (const Level &lv : levels[b])
         if (lv.level == ph.level) killed = lv.killed;
-      const unsigned grid = replay_grid_blocks(e->device, killed * k / ph.M,
-                                               e->cur.replay_blocks);
+      const unsigned grid = coop_width(
+          e, replay_grid_blocks(e->device, killed * k / ph.M, e->cur.replay_blocks));
       DBuf<uint32_t> partial(&A), pcount(&A);
       DBuf<GridBarrier> gbar(&A);
       DBuf<ReplayOut> hout(&A);

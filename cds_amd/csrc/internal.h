@@ -15,6 +15,9 @@ cds_status fail(cds_status st, const char *fmt, ...)
 
 // Copies the default tuning and the loaded tune-cache entries (engine.cpp).
 void copy_tuning(cds_engine *dst, cds_engine *src);
+// Marks an engine as sharing its device with another one, or not
+// (engine.cpp): shared engines keep cooperative kernels within half the CUs.
+void share_device(cds_engine *e, bool shared);
 
 // Allocates a host array with malloc (so the C ABI can free it with free()).
 template <class T> T *host_alloc(size_t count) {

@@ -30,6 +30,7 @@
 #include "internal.h"
 
 using cds::copy_tuning;
+using cds::share_device;
 using cds::fail;
 
 namespace {
@@ -276,6 +277,8 @@ extern "C" cds_status cds_server_create(cds_engine *e, const cds_server_opts *o,
   const cds_status st = cds_engine_create(&eo, &large);
   if (st != CDS_OK) return st;
   copy_tuning(large, e);
+  share_device(large, true);
+  share_device(e, true);
   auto *s = new cds_server();
   memset(&s->opts, 0, sizeof s->opts);
   if (o) s->opts = *o;
@@ -302,8 +305,12 @@ extern "C" void cds_server_destroy(cds_server *s) {
   s->cv_budget.notify_all();
   for (Lane &L : s->lanes)
     if (L.loop.joinable()) L.loop.join();
-  for (Lane &L : s->lanes)
-    if (L.owned) cds_engine_destroy(L.engine);
+  for (Lane &L : s->lanes) {
+    if (L.owned)
+      cds_engine_destroy(L.engine);
+    else
+      share_device(L.engine, false);  // the caller's engine: full width again
+  }
   for (auto &kv : s->all)
     if (kv.second->done && kv.second->status == CDS_OK && !kv.second->taken)
       cds_result_free(&kv.second->result);
