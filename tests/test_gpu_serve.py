@@ -142,3 +142,28 @@ def test_large_lane_counts_before_admitting(serve_engine):
     srv.close()
     alone = serve_engine.solve(serve_engine.upload(g), k)
     assert same(got, alone) is None
+
+
+def test_server_lanes_keep_cooperative_kernels_within_half_the_cus():
+    """While a server runs, its lanes (two engines on one device) launch
+    cooperative kernels on at most half the CUs; the caller's engine gets
+    its full width back when the server ends.  R-MAT scale 18, k = 3: the
+    shipped tuning peels it on one workgroup per CU."""
+    g = cds_amd.Graph.rmat(18, 16, 0.57, 0.19, 0.19, seed=18)
+    _announce("server lane width, R-MAT scale 18 k=3")
+    eng = cds_amd.Engine(device=0)
+    try:
+        alone = eng.solve(eng.upload(g), 3)
+        full = alone["peel_grid"]
+        srv = cds_amd.Server(eng, hbm_budget_bytes=64 << 30)
+        try:
+            served = srv.wait(srv.submit(g, 3))
+        finally:
+            srv.close()
+        assert served["peel_grid"] <= max(1, full // 2) or full == 1
+        assert same(served, alone) is None
+        again = eng.solve(eng.upload(g), 3)
+        assert again["peel_grid"] == full
+        assert same(again, alone) is None
+    finally:
+        eng.close()
