@@ -93,8 +93,25 @@ int main(int argc, const char *argv[]) {
   printf("n=%u, m=%llu\n", g.n,
          (unsigned long long)(g.m_header ? g.m_header : g.m));
   printf("Max Degree=%u\n", g.max_degree);
+  // launch shapes: CDS_TUNE_CACHE if set, else the tuned cache shipped
+  // beside the binary (cds_amd/tune_cache.txt for cds_amd/bin/cds_main)
+  cds_opts o;
+  memset(&o, 0, sizeof o);
+  std::string shipped;
+  if (!getenv("CDS_TUNE_CACHE")) {
+    char exe[4096];
+    const ssize_t len = readlink("/proc/self/exe", exe, sizeof exe - 1);
+    if (len > 0) {
+      exe[len] = 0;
+      std::string dir(exe);
+      dir = dir.substr(0, dir.rfind('/'));
+      shipped = dir + "/../tune_cache.txt";
+      struct stat st;
+      if (stat(shipped.c_str(), &st) == 0) o.tune_cache = shipped.c_str();
+    }
+  }
   cds_engine *e = nullptr;
-  if (cds_engine_create(nullptr, &e) != CDS_OK) return die("engine");
+  if (cds_engine_create(&o, &e) != CDS_OK) return die("engine");
   cds_dgraph *dg = nullptr;
   if (cds_graph_upload(e, &g, &dg) != CDS_OK) return die("upload");
   cds_result r;
