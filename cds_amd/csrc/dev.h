@@ -169,6 +169,21 @@ inline hipError_t launch_grid(const void *fn, unsigned grid, unsigned threads,
   return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, lds, s);
 }
 
+// launch_grid for kernels whose work split and barriers follow gridDim
+// alone: a refused cooperative launch (this happens while another process
+// holds the device) is retried at half the width, down to one workgroup,
+// whose plain launch needs no co-residency.
+inline hipError_t launch_grid_shrinking(const void *fn, unsigned grid, unsigned threads,
+                                        void **args, size_t lds, hipStream_t s) {
+  hipError_t r = launch_grid(fn, grid, threads, args, lds, s);
+  while (r != hipSuccess && grid > 1) {
+    (void)hipGetLastError();
+    grid = grid > 2 ? grid / 2 : 1;
+    r = launch_grid(fn, grid, threads, args, lds, s);
+  }
+  return r;
+}
+
 inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 65535u * 16u) {
   size_t g = (work + block - 1) / block;
   if (g < 1) g = 1;

@@ -922,18 +922,8 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
   FlowProblem p = P;
   GridFlowState st = S;
   void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
-  // The kernel adapts to any width (gridDim drives its loops and barriers).
-  // A cooperative launch over every CU can be refused while another process
-  // holds the device; it is then retried at half the width, twice at most.
-  hipError_t r = launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid,
-                             kGridFlowThreads, args, lds, s);
-  for (int retry = 0; r != hipSuccess && grid > 1 && retry < 2; ++retry) {
-    (void)hipGetLastError();
-    grid = (grid + 1) / 2;
-    r = launch_grid(reinterpret_cast<const void *>(k_grid_flow), grid, kGridFlowThreads,
-                    args, lds, s);
-  }
-  return r;
+  return launch_grid_shrinking(reinterpret_cast<const void *>(k_grid_flow), grid,
+                               kGridFlowThreads, args, lds, s);
 }
 
 }  // namespace cds

@@ -185,7 +185,10 @@ __global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
         const uint32_t cj = cidx[cl[c * k + j]];
         if (i == j || cj >= cn) continue;
         const uint32_t pos = cmat[(uint64_t)ci * cn + cj];
-        if (pos != kNoEdge) mark[pos] = 1;
+        // a core edge is in many core cliques: store its mark once (a
+        // cached read instead of a partial-line write per clique pair;
+        // threads racing on the same mark all store 1)
+        if (pos != kNoEdge && !mark[pos]) mark[pos] = 1;
       }
     }
   }

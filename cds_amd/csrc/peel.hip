@@ -1106,18 +1106,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
                   &deg, &core, &stamp, &sbase, &A0, &A1, &F0, &F1, &F2, &FC0, &FC1,
                   &FC2, &ctl, &lv_level, &lv_removed, &lv_killed, &pos,
                   &sdeg, &lv_min, &cshift, &max_level, (void *)&hagg};
-  // The kernel adapts to any width (gridDim drives its loops and barriers).
-  // A cooperative launch over every CU can be refused while another process
-  // holds the device; it is then retried at half the width, twice at most.
-  hipError_t r = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
-                             kPeelThreads, args, 0, s);
-  for (int retry = 0; r != hipSuccess && grid > 1 && retry < 2; ++retry) {
-    (void)hipGetLastError();
-    grid = (grid + 1) / 2;
-    r = launch_grid(reinterpret_cast<const void *>(k_peel_persistent), grid,
-                    kPeelThreads, args, 0, s);
-  }
-  return r;
+  return launch_grid_shrinking(reinterpret_cast<const void *>(k_peel_persistent), grid,
+                               kPeelThreads, args, 0, s);
 }
 
 hipError_t peel_derive_kill(uint64_t T, uint32_t k, const uint32_t *cliques,
@@ -1157,8 +1147,8 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
   void *args[] = {&k, &level, &M, &pvm, &pdeg, (void *)&posmap, &palive,
                   (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
                   &rdead, &partial, &bar, &T_start, &N_start, &out, &kp, &nstar};
-  return launch_grid(reinterpret_cast<const void *>(k_replay_grid), grid, kReplayThreads,
-                     args, 0, s);
+  return launch_grid_shrinking(reinterpret_cast<const void *>(k_replay_grid), grid,
+                               kReplayThreads, args, 0, s);
 }
 
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
@@ -1195,8 +1185,8 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
     void *args[] = {&k, &level, &M, &pv, &pdeg, (void *)&posmap, &palive,
                     (void *)&core, (void *)&kill, (void *)&ioff, (void *)&inc,
                     &rdead, &partial, &bar, &T_start, &N_start, &out, &kp, &nstar};
-    return launch_grid(reinterpret_cast<const void *>(k_replay_grid), grid,
-                       kReplayThreads, args, 0, s);
+    return launch_grid_shrinking(reinterpret_cast<const void *>(k_replay_grid), grid,
+                                 kReplayThreads, args, 0, s);
   }
   if ((e = scan_u32(pdeg, pioff, M, arena)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_phase_compact, dim3(g), dim3(256), 0, s, k, level,
