@@ -177,7 +177,7 @@ struct Component {
   // FLOW_FLOAT_EK keeps the reference's string-keyed map (its iteration order
   // fixes the flow network's node order); FLOW_EXACT keeps a plain list.
   std::unordered_map<std::string, std::vector<int>> clique_map;
-  std::vector<std::vector<int>> cliques;  // members (local ids)
+  std::vector<int> cliques;  // FLOW_EXACT: k members (local ids) per clique
   std::vector<long> clique_degree;
   long total_cliques = 0;
   double density = 0;
@@ -222,7 +222,7 @@ struct Result {
 // (cliqueEnumerationFast, helpers.cpp:317-347 + listCliques :373-511).
 // --------------------------------------------------------------------------
 static void list_full(const std::vector<std::vector<ui>> &adj, ui k, Result &R,
-                      std::vector<std::vector<ui>> *members) {
+                      std::vector<ui> *members) {
   const ui n = adj.size();
   R.clique_degree.assign(n, 0);
   R.total_cliques = 0;
@@ -233,10 +233,9 @@ static void list_full(const std::vector<std::vector<ui>> &adj, ui k, Result &R,
     R.clique_degree[a]++;
     R.clique_degree[b]++;
     if (members) {
-      std::vector<ui> c(p);
-      c.push_back(a);
-      c.push_back(b);
-      members->push_back(std::move(c));
+      members->insert(members->end(), p.begin(), p.end());
+      members->push_back(a);
+      members->push_back(b);
     }
   });
 }
@@ -284,7 +283,7 @@ static void neighbourhood_counts(const std::vector<std::vector<ui>> &adj, ui k,
 }
 
 static void peel(const std::vector<std::vector<ui>> &adj, ui k,
-                 const std::vector<std::vector<ui>> &members, int mode,
+                 const std::vector<ui> &members, int mode,
                  Result &R) {
   const ui n = adj.size();
   std::vector<long> cur(R.clique_degree);
@@ -338,33 +337,59 @@ static void peel(const std::vector<std::vector<ui>> &adj, ui k,
   }
 
   // PEEL_FAST: the same sequence.  argmin over unmarked vertices of
-  // (current clique degree, id) comes from an ordered set; the cliques killed
-  // by removing v are the alive cliques on v's incidence list.
-  std::vector<std::vector<ui>> inc(n);
-  for (ui c = 0; c < members.size(); ++c)
-    for (ui x : members[c]) inc[x].push_back(c);
-  std::vector<char> alive(members.size(), 1);
-  std::set<std::pair<long, ui>> pq;
-  for (ui v = 0; v < n; ++v) pq.emplace(cur[v], v);
+  // (current clique degree, id) comes from a tournament tree; the cliques
+  // killed by removing v are the alive cliques on v's incidence list (a CSR
+  // whose row lengths are the initial clique degrees).  Members and
+  // incidence are flat arrays: k + k 32-bit words per clique.
+  const uint64_t T = members.size() / k;
+  std::vector<uint64_t> ioff(n + 1, 0);
+  for (ui v = 0; v < n; ++v) ioff[v + 1] = ioff[v] + (uint64_t)R.clique_degree[v];
+  std::vector<ui> inc(ioff[n]);
+  {
+    std::vector<uint64_t> fill(ioff.begin(), ioff.end() - 1);
+    for (uint64_t c = 0; c < T; ++c)
+      for (ui j = 0; j < k; ++j) inc[fill[members[c * k + j]]++] = (ui)c;
+  }
+  std::vector<char> alive(T, 1);
+  // Tournament tree over (degree, id) keys; a removed vertex holds the
+  // largest key.
+  typedef unsigned __int128 key_t;
+  const key_t kGone = ~(key_t)0;
+  size_t leaves = 1;
+  while (leaves < n) leaves <<= 1;
+  std::vector<key_t> tree(2 * leaves, kGone);
+  auto key_of = [&](ui v) { return ((key_t)(uint64_t)cur[v] << 32) | v; };
+  for (ui v = 0; v < n; ++v) tree[leaves + v] = key_of(v);
+  for (size_t i = leaves - 1; i >= 1; --i) tree[i] = std::min(tree[2 * i], tree[2 * i + 1]);
+  auto set_leaf = [&](ui v, key_t key) {
+    size_t i = leaves + v;
+    tree[i] = key;
+    for (i >>= 1; i >= 1; i >>= 1) {
+      const key_t m = std::min(tree[2 * i], tree[2 * i + 1]);
+      if (tree[i] == m) break;
+      tree[i] = m;
+    }
+  };
   for (ui i = 1; i <= n; ++i) {
-    auto it = pq.begin();
+    const key_t top = tree[1];
+    const ui index = (ui)(uint64_t)top;
     // Above 0xFFFFFF the reference selects index 0 even if it is already
     // removed (helpers.cpp:93-100); that regime is flagged, not emulated.
-    if (it->first > 0xFFFFFF) R.quirk_flags |= 1;
-    const ui index = it->second;
-    pq.erase(it);
+    if ((long)(uint64_t)(top >> 32) > 0xFFFFFF) R.quirk_flags |= 1;
+    set_leaf(index, kGone);
     R.removal_vertex[i] = index;
     R.removal_value[i] = cur[index];
     long del = 0;
-    for (ui c : inc[index]) {
+    for (uint64_t e = ioff[index]; e < ioff[index + 1]; ++e) {
+      const ui c = inc[e];
       if (!alive[c]) continue;
       alive[c] = 0;
       ++del;
-      for (ui x : members[c]) {
+      for (ui j = 0; j < k; ++j) {
+        const ui x = members[(uint64_t)c * k + j];
         if (x == index || mark[x]) continue;
-        pq.erase({cur[x], x});
         cur[x]--;
-        pq.emplace(cur[x], x);
+        set_leaf(x, key_of(x));
       }
     }
     cur[index] = 0;
@@ -423,52 +448,75 @@ static void components(const std::vector<std::vector<ui>> &core_graph, ui k,
                        int flow_mode, Result &R) {
   const ui cn = core_graph.size();
   // Re-list with string keys: the map's iteration order is what the
-  // reference's flow network is built from.
+  // reference's flow network is built from.  FLOW_EXACT keeps the cliques
+  // as a flat list of k members each (order-independent from here on).
   std::unordered_map<std::string, std::vector<int>> clique_data;
-  std::vector<std::vector<int>> clique_list;
+  std::vector<int> clique_list;
   {
     Lister L;
     L.list_all(core_graph, k, [&](const std::vector<ui> &p, ui a, ui b) {
-      std::vector<int> arr(k + 1);
-      for (ui x = 0; x < p.size(); ++x) arr[x] = p[x];
-      arr[k - 2] = a;
-      arr[k - 1] = b;
-      arr[k] = 1;
       if (flow_mode == FLOW_FLOAT_EK) {
+        std::vector<int> arr(k + 1);
+        for (ui x = 0; x < p.size(); ++x) arr[x] = p[x];
+        arr[k - 2] = a;
+        arr[k - 1] = b;
+        arr[k] = 1;
         std::string key;
         for (ui x : p) key += std::to_string(x) + " ";
         key += std::to_string(a) + " " + std::to_string(b);
         clique_data[key] = arr;
       } else {
-        clique_list.push_back(arr);
+        for (ui x : p) clique_list.push_back((int)x);
+        clique_list.push_back((int)a);
+        clique_list.push_back((int)b);
       }
     });
   }
+  const size_t nflat = clique_list.size() / k;
   if (flow_mode == FLOW_FLOAT_EK) {
     R.core_cliques = clique_data.size();
   } else {
-    R.core_cliques = clique_list.size();
+    R.core_cliques = nflat;
   }
-  // Prune: keep core edges covered by at least one core clique.
-  std::vector<std::vector<ui>> valid(cn);
-  auto mark_clique = [&](const std::vector<int> &arr) {
+  // Prune: keep core edges covered by at least one core clique.  Covered
+  // pairs are marked in a bit matrix when the core is small enough, else in
+  // sorted per-vertex lists; both hold the same set.
+  const bool dense = cn <= 32768;
+  std::vector<uint64_t> bits(dense ? ((size_t)cn * cn + 63) / 64 : 0);
+  std::vector<std::vector<ui>> valid(dense ? 0 : cn);
+  auto mark_pair = [&](int a, int b) {
+    if (dense) {
+      const size_t i = (size_t)a * cn + b;
+      bits[i >> 6] |= 1ull << (i & 63);
+    } else {
+      valid[a].push_back(b);
+    }
+  };
+  auto mark_clique = [&](const int *arr) {
     for (ui i = 0; i < k; ++i)
       for (ui j = 0; j < k; ++j)
-        if (i != j) valid[arr[i]].push_back(arr[j]);
+        if (i != j) mark_pair(arr[i], arr[j]);
   };
   if (flow_mode == FLOW_FLOAT_EK)
-    for (auto &e : clique_data) mark_clique(e.second);
+    for (auto &e : clique_data) mark_clique(e.second.data());
   else
-    for (auto &c : clique_list) mark_clique(c);
+    for (size_t c = 0; c < nflat; ++c) mark_clique(&clique_list[c * k]);
   for (auto &v : valid) {
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
   }
+  auto covered = [&](ui a, ui b) {
+    if (dense) {
+      const size_t i = (size_t)a * cn + b;
+      return (bits[i >> 6] >> (i & 63)) & 1;
+    }
+    return (uint64_t)std::binary_search(valid[a].begin(), valid[a].end(), b);
+  };
   std::vector<std::vector<ui>> pruned(cn);
   int total_edges = 0;
   for (ui i = 0; i < cn; ++i)
     for (ui u : core_graph[i])
-      if (std::binary_search(valid[i].begin(), valid[i].end(), u)) {
+      if (covered(i, u)) {
         pruned[i].push_back(u);
         total_edges++;
       }
@@ -502,27 +550,18 @@ static void components(const std::vector<std::vector<ui>> &core_graph, ui k,
   }
   for (ui i = 0; i < cn; ++i)
     R.comps[status[i] - 1].reverse_map[old_to_new[i]] = i;
-  auto add_clique = [&](const std::string *key, const std::vector<int> &arr0) {
-    std::vector<int> arr(arr0);
-    const int cid = status[arr[0]] - 1;
-    for (ui i = 0; i < k; ++i) arr[i] = old_to_new[arr[i]];
-    Component &C = R.comps[cid];
-    if (key) {
-      if (ncomp == 1)
-        C.clique_map[*key] = arr0;  // single component keeps the ids as is
-      else
-        C.clique_map[*key] = arr;
-    } else {
-      C.cliques.push_back(arr);
-    }
-  };
   if (flow_mode == FLOW_FLOAT_EK) {
     if (ncomp == 1) {
       // One component: conComp.cliqueData is a copy of the whole map
       // (helpers.cpp:1026-1044).
       R.comps[0].clique_map = clique_data;
     } else {
-      for (auto &e : clique_data) add_clique(&e.first, e.second);
+      for (auto &e : clique_data) {
+        std::vector<int> arr(e.second);
+        const int cid = status[arr[0]] - 1;
+        for (ui i = 0; i < k; ++i) arr[i] = old_to_new[arr[i]];
+        R.comps[cid].clique_map[e.first] = arr;
+      }
     }
     for (auto &C : R.comps) {
       for (auto &e : C.clique_map) {
@@ -531,12 +570,16 @@ static void components(const std::vector<std::vector<ui>> &core_graph, ui k,
       }
     }
   } else {
-    for (auto &c : clique_list) add_clique(nullptr, c);
-    for (auto &C : R.comps)
-      for (auto &c : C.cliques) {
-        for (ui i = 0; i < k; ++i) C.clique_degree[c[i]] += 1;
-        C.total_cliques += 1;
+    for (size_t c = 0; c < nflat; ++c) {
+      const int *arr = &clique_list[c * k];
+      Component &C = R.comps[status[arr[0]] - 1];
+      for (ui i = 0; i < k; ++i) {
+        const int x = (int)old_to_new[arr[i]];
+        C.cliques.push_back(x);
+        C.clique_degree[x] += 1;
       }
+      C.total_cliques += 1;
+    }
   }
   for (auto &C : R.comps)
     C.density = (double)C.total_cliques / ((double)C.size);
@@ -651,48 +694,83 @@ static void exact_float(std::vector<int> &res, Component &C, ui k, float lb,
 }
 
 // Exact max flow (Dinic, int64) on the same network with every capacity
-// scaled by 2^s.  Returns F * 2^s and leaves `reach` = nodes reachable from
-// the source in the final residual graph (the minimal min-cut side).
+// scaled by 2^s.  maxflow returns F * 2^s and leaves `level` >= 0 exactly on
+// the nodes reachable from the source in the final residual graph (the
+// minimal min-cut side, the same for every maximum flow).
+//
+// Arcs live in a CSR built from an arc list; each arc stores its reverse.
+// A pair of opposite arcs u->v (cap a) and v->u (cap b) is one arc pair
+// with residuals a and b: the same maximum flow value and the same residual
+// reachability as two separate arcs with zero-capacity reverses (u->v has
+// residual capacity unless it carries a net flow of a, in both forms).
 struct Dinic {
-  struct E {
-    int to;
-    int64_t cap;
+  struct Arc {
+    int u, v;
+    int64_t a, b;
   };
-  std::vector<E> edges;
-  std::vector<std::vector<int>> g;
-  std::vector<int> level, it;
-  explicit Dinic(int n) : g(n), level(n), it(n) {}
-  void add(int u, int v, int64_t c) {
-    g[u].push_back(edges.size());
-    edges.push_back({v, c});
-    g[v].push_back(edges.size());
-    edges.push_back({u, 0});
+  int n = 0;
+  std::vector<Arc> list;
+  std::vector<uint64_t> head;  // n + 1
+  std::vector<int> to;
+  std::vector<uint64_t> rev;
+  std::vector<int64_t> cap, cap0;
+  std::vector<int> level;
+  std::vector<uint64_t> it;
+  std::vector<int> queue;
+  explicit Dinic(int nodes) : n(nodes) {}
+  void add(int u, int v, int64_t a, int64_t b = 0) { list.push_back({u, v, a, b}); }
+  void build() {
+    head.assign(n + 1, 0);
+    for (auto &e : list) {
+      head[e.u + 1]++;
+      head[e.v + 1]++;
+    }
+    for (int i = 0; i < n; ++i) head[i + 1] += head[i];
+    const uint64_t m = head[n];
+    to.resize(m);
+    rev.resize(m);
+    cap0.resize(m);
+    std::vector<uint64_t> pos(head.begin(), head.end() - 1);
+    for (auto &e : list) {
+      const uint64_t i = pos[e.u]++, j = pos[e.v]++;
+      to[i] = e.v;
+      to[j] = e.u;
+      rev[i] = j;
+      rev[j] = i;
+      cap0[i] = e.a;
+      cap0[j] = e.b;
+    }
+    list.clear();
+    list.shrink_to_fit();
+    cap = cap0;
+    level.assign(n, -1);
+    it.assign(n, 0);
   }
+  void reset() { std::copy(cap0.begin(), cap0.end(), cap.begin()); }
   bool bfs(int s, int t) {
     std::fill(level.begin(), level.end(), -1);
-    std::queue<int> q;
+    queue.clear();
     level[s] = 0;
-    q.push(s);
-    while (!q.empty()) {
-      int u = q.front();
-      q.pop();
-      for (int id : g[u])
-        if (edges[id].cap > 0 && level[edges[id].to] < 0) {
-          level[edges[id].to] = level[u] + 1;
-          q.push(edges[id].to);
+    queue.push_back(s);
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+      const int u = queue[qi];
+      for (uint64_t e = head[u]; e < head[u + 1]; ++e)
+        if (cap[e] > 0 && level[to[e]] < 0) {
+          level[to[e]] = level[u] + 1;
+          queue.push_back(to[e]);
         }
     }
     return level[t] >= 0;
   }
   int64_t dfs(int u, int t, int64_t f) {
     if (u == t) return f;
-    for (int &i = it[u]; i < (int)g[u].size(); ++i) {
-      E &e = edges[g[u][i]];
-      if (e.cap > 0 && level[e.to] == level[u] + 1) {
-        int64_t d = dfs(e.to, t, std::min(f, e.cap));
+    for (uint64_t &e = it[u]; e < head[u + 1]; ++e) {
+      const int v = to[e];
+      if (cap[e] > 0 && level[v] == level[u] + 1) {
+        const int64_t d = dfs(v, t, std::min(f, cap[e]));
         if (d > 0) {
-          e.cap -= d;
-          edges[g[u][i] ^ 1].cap += d;
+          cap[e] -= d;
+          cap[rev[e]] += d;
           return d;
         }
       }
@@ -702,10 +780,10 @@ struct Dinic {
   int64_t maxflow(int s, int t) {
     int64_t flow = 0;
     while (bfs(s, t)) {
-      std::fill(it.begin(), it.end(), 0);
+      for (int i = 0; i < n; ++i) it[i] = head[i];
       while (int64_t f = dfs(s, t, INT64_MAX)) flow += f;
     }
-    return flow;
+    return flow;  // the last bfs left the residual reachability in level
   }
 };
 
@@ -731,28 +809,31 @@ static void exact_int(std::vector<int> &res, Component &C, ui k, float lb,
   long maxcd = 0;
   for (long d : C.clique_degree) maxcd = std::max(maxcd, d);
   if (maxcd >= (1l << 24)) R.quirk_flags |= 2;
+  if (!((ub - lb) > bias)) return;
+  // The network's shape is the same for every alpha: build it once with
+  // the clique and source capacities for unit U = 1, then scale them and
+  // set the sink arcs per alpha.
+  Dinic D(V + T + 2);
+  for (int c = 0; c < T; ++c)
+    for (ui i = 0; i < k; ++i) D.add(V + c, C.cliques[(size_t)c * k + i], (int64_t)k - 1, 1);
+  for (int v = 0; v < V; ++v) D.add(source, v, C.clique_degree[v]);
+  std::vector<uint64_t> sink_arc(V);
+  for (int v = 0; v < V; ++v) D.add(v, sink, 0);
+  D.build();
+  for (int v = 0; v < V; ++v)
+    for (uint64_t e = D.head[v]; e < D.head[v + 1]; ++e)
+      if (D.to[e] == sink) sink_arc[v] = e;
+  const std::vector<int64_t> unit_caps = D.cap0;
   while ((ub - lb) > bias) {
     const float cap_t = alpha * k;
     int e2;
     std::frexp(cap_t, &e2);
     int s = std::max(0, 24 - e2);
-    const int64_t U = (int64_t)1 << s;
     const int64_t ct = (int64_t)std::ldexp((double)cap_t, s);
-    Dinic D(V + T + 2);
-    int node = V;
-    for (auto &c : C.cliques) {
-      for (ui i = 0; i < k; ++i) {
-        D.add(node, c[i], (int64_t)(k - 1) * U);
-        D.add(c[i], node, U);
-      }
-      ++node;
-    }
-    for (int v = 0; v < V; ++v) {
-      D.add(source, v, C.clique_degree[v] * U);
-      D.add(v, sink, ct);
-    }
+    for (size_t e = 0; e < unit_caps.size(); ++e) D.cap0[e] = unit_caps[e] << s;
+    for (int v = 0; v < V; ++v) D.cap0[sink_arc[v]] = ct;
+    D.reset();
     const int64_t F = D.maxflow(source, sink);
-    D.bfs(source, sink);  // residual reachability from the source
     R.tr_comp.push_back(cid);
     R.tr_lb.push_back(lb);
     R.tr_ub.push_back(ub);
@@ -831,16 +912,16 @@ static void dynamic_exact(ui k, int flow_mode, bool ub1, bool ub2, Result &R) {
     if (R.tr_alpha.size() > before) R.flow_networks++;
     long clique_count = 0;
     ui vertex_count = 0;
-    auto count_clique = [&](const std::vector<int> &t) {
+    auto count_clique = [&](const int *t, long weight) {
       ui i = 0;
       for (; i < k; ++i)
         if (res[t[i]] == -1) break;
-      if (i == k) clique_count += t.size() > k ? t[k] : 1;
+      if (i == k) clique_count += weight;
     };
     if (flow_mode == FLOW_FLOAT_EK)
-      for (auto &e : C.clique_map) count_clique(e.second);
+      for (auto &e : C.clique_map) count_clique(e.second.data(), e.second[k]);
     else
-      for (auto &c : C.cliques) count_clique(c);
+      for (size_t c = 0; c < C.cliques.size() / k; ++c) count_clique(&C.cliques[c * k], 1);
     for (int i = 0; i < C.size; i++)
       if (res[i] != -1) vertex_count++;
     if (vertex_count == 0) vertex_count = C.size;
@@ -886,7 +967,7 @@ static void solve(const std::vector<std::vector<ui>> &adj, ui k, int peel_mode,
   unsigned long long e2 = 0;
   for (auto &l : adj) e2 += l.size();
   R.m = e2 / 2;
-  std::vector<std::vector<ui>> members;
+  std::vector<ui> members;
   list_full(adj, k, R, peel_mode == PEEL_FAST ? &members : nullptr);
   auto t1 = clk::now();
   peel(adj, k, members, peel_mode, R);
@@ -1062,16 +1143,14 @@ long long oracle_component_flow(int V, int T, ui k, const int *members,
   Dinic D(V + T + 2);
   const int source = V + T, sink = source + 1;
   for (int c = 0; c < T; ++c)
-    for (ui i = 0; i < k; ++i) {
-      D.add(V + c, members[c * k + i], (long long)(k - 1) * unit);
-      D.add(members[c * k + i], V + c, unit);
-    }
+    for (ui i = 0; i < k; ++i)
+      D.add(V + c, members[c * k + i], (long long)(k - 1) * unit, unit);
   for (int v = 0; v < V; ++v) {
     D.add(source, v, cd[v] * unit);
     D.add(v, sink, cap_t);
   }
+  D.build();
   long long F = D.maxflow(source, sink);
-  D.bfs(source, sink);
   for (int v = 0; v < V; ++v) reach[v] = D.level[v] >= 0;
   return F;
 }
