@@ -12,6 +12,7 @@ every call raises `EngineUnavailable`.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -102,7 +103,11 @@ class CResult(ctypes.Structure):
         ("t_replay_ms", ctypes.c_double), ("t_replay_prep_ms", ctypes.c_double),
         ("t_flow_kernel_ms", ctypes.c_double), ("flow_clique_visits", ctypes.c_uint64),
         ("flow_arc_visits", ctypes.c_uint64), ("flow_relabels", ctypes.c_uint64),
-        ("times", CTimes)]
+        ("times", CTimes),
+        ("flow_grid", ctypes.c_uint32), ("replay_grid", ctypes.c_uint32),
+        ("coop_shrinks", ctypes.c_uint32), ("fragile_decisions", ctypes.c_uint32),
+        ("trace_fragile", ctypes.POINTER(ctypes.c_uint8)),
+        ("trace_drift", ctypes.POINTER(ctypes.c_double))]
 
 
 class COpts(ctypes.Structure):
@@ -332,7 +337,7 @@ def result_to_dict(r, with_json=False):
     d = {f: getattr(r, f) for f, _ in CResult._fields_
          if f not in ("clique_degree", "core", "vertices", "trace_lb", "trace_ub",
                       "trace_alpha", "trace_component", "trace_margin", "times",
-                      "peel_split_ms")}
+                      "peel_split_ms", "trace_fragile", "trace_drift")}
     d["vertices"] = _arr(r.vertices, r.set_size, np.uint32).tolist()
     d["clique_degree"] = _arr(r.clique_degree, r.n, np.uint64) if r.clique_degree else None
     d["core"] = _arr(r.core, r.n, np.uint32) if r.core else None
@@ -342,6 +347,8 @@ def result_to_dict(r, with_json=False):
     d["trace_alpha"] = _arr(r.trace_alpha, n, np.float32).tolist()
     d["trace_component"] = _arr(r.trace_component, n, np.uint32).tolist()
     d["trace_margin"] = _arr(r.trace_margin, n, np.float64).tolist()
+    d["trace_fragile"] = _arr(r.trace_fragile, n, np.uint8).tolist()
+    d["trace_drift"] = _arr(r.trace_drift, n, np.float64).tolist()
     d["times"] = {f: getattr(r.times, f) for f, _ in CTimes._fields_}
     d["peel_split_ms"] = dict(zip(("barrier", "scan", "process", "flush"),
                                   list(r.peel_split_ms)))
@@ -355,9 +362,11 @@ class Engine:
 
     def __init__(self, device=0, hbm_budget_bytes=0, profile_kernels=False,
                  tune_cache=None):
-        # tune_cache None: the shipped cache (cds_amd/tune_cache.txt) if present;
+        # tune_cache None: the file CDS_TUNE_CACHE names, else the shipped
+        # cache (cds_amd/tune_cache.txt) if present, the order cds_main uses;
         # "": none (the CDS_TUNE_CACHE environment variable still applies)
-        if tune_cache is None and os.path.exists(TUNE_CACHE):
+        if tune_cache is None and not os.environ.get("CDS_TUNE_CACHE") and \
+                os.path.exists(TUNE_CACHE):
             tune_cache = TUNE_CACHE
         opts = COpts(device, hbm_budget_bytes, 1 if profile_kernels else 0,
                      os.fsencode(tune_cache) if tune_cache else None)
@@ -445,12 +454,22 @@ class Engine:
         return T.value, cd, ms.value
 
     def close(self):
+        """Destroys the engine (its stream and device memory).  Call it
+        before the interpreter exits: at shutdown __del__ makes no HIP call,
+        the runtime and any profiler attached to it may already be gone."""
         if getattr(self, "_h", None):
             lib().cds_engine_destroy(self._h)
             self._h = None
 
-    def __del__(self):
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
         self.close()
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
 
 class DeviceGraph:
@@ -464,10 +483,15 @@ class DeviceGraph:
         self.n = graph.n
         self.m = graph.m
 
-    def __del__(self):
+    def free(self):
+        """Releases the graph's device memory (its engine must be open)."""
         if getattr(self, "_h", None) and self._engine._h:
             lib().cds_dgraph_free(self._engine._h, self._h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.free()
 
 
 class Server:
@@ -524,6 +548,8 @@ class Server:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 
 namespace cds {
 
@@ -123,40 +124,87 @@ __device__ __forceinline__ int64_t find_sorted64(const uint32_t *a, int64_t len,
 // device-wide before arriving (the fence writes back this XCD's L2) and
 // acquires after the flip; the workgroup barriers around it order the
 // other waves, which share the CU's caches.
+// The wait is bounded: after b->limit wall-clock ticks (0: the default)
+// the workgroup sets b->abort and ends (s_endpgm, every wave of it, right
+// after the workgroup barrier); a workgroup that finds abort set ends the
+// same way.  So a launch whose workgroups are not all resident finishes
+// instead of hanging, its results invalid and b->abort set for the host.
 // group_barrier is the same over the `nblocks` co-resident workgroups of
 // one problem inside a larger launch (bid = this workgroup's index among
 // them).
+constexpr unsigned long long kBarrierDefaultTicks = 6000000000ull;  // 60 s at 100 MHz
+
+// Thread 0's part of group_barrier: arrive, then wait for the generation
+// to flip; returns 1 if the wait ran out or another workgroup's did.  Kept
+// out of line: inlined at every barrier of the flow kernel it raised
+// register pressure into scratch spills.
+template <class B>
+__device__ __noinline__ unsigned barrier_arrive_wait(B *b, unsigned nblocks, unsigned bid) {
+  if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 1;
+  __threadfence();
+  const unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned grp = bid & 7u;
+  const unsigned gsz = (nblocks + 7u - grp) >> 3;
+  const unsigned groups = nblocks < 8u ? nblocks : 8u;
+  const unsigned a = __hip_atomic_fetch_add(&b->sub[grp * 32], 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  if ((a + 1u) % gsz == 0u) {
+    const unsigned t =
+        __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((t + 1u) % groups == 0u)
+      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const unsigned long long t0 = wall_clock64();
+  unsigned ab = 0;
+  while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+    if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      ab = 1;
+      break;
+    }
+    const unsigned long long lim =
+        __hip_atomic_load(&b->limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wall_clock64() - t0 > (lim ? lim : kBarrierDefaultTicks)) {
+      if (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ab = 1;
+      }
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __threadfence();
+  return ab;
+}
+
 template <class B>
 __device__ __forceinline__ void group_barrier(B *b, unsigned nblocks, unsigned bid) {
+  __shared__ unsigned s_abort;
   __syncthreads();
   // One workgroup: its waves share the CU's caches, so the workgroup
   // barrier (with its workgroup-scope fence) is the whole barrier.
   if (nblocks == 1) return;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned gen =
-        __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned grp = bid & 7u;
-    const unsigned gsz = (nblocks + 7u - grp) >> 3;
-    const unsigned groups = nblocks < 8u ? nblocks : 8u;
-    const unsigned a = __hip_atomic_fetch_add(&b->sub[grp * 32], 1u, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    if ((a + 1u) % gsz == 0u) {
-      const unsigned t = __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-      if ((t + 1u) % groups == 0u)
-        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
-      __builtin_amdgcn_s_sleep(1);
-    __threadfence();
-  }
+  if (threadIdx.x == 0) s_abort = barrier_arrive_wait(b, nblocks, bid);
   __syncthreads();
+  if (s_abort) __builtin_amdgcn_endpgm();
 }
 
 template <class B>
 __device__ __forceinline__ void grid_barrier(B *b) {
   group_barrier(b, gridDim.x, blockIdx.x);
+}
+
+// Tick limit of one barrier wait: CDS_BARRIER_TIMEOUT_US (microseconds)
+// if set, else 0 (the kernels' default, 60 s).  Read at every launch.
+inline unsigned long long barrier_limit_ticks() {
+  const char *v = getenv("CDS_BARRIER_TIMEOUT_US");
+  if (!v || !*v) return 0;
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+      khz <= 0)
+    khz = 100000;
+  const unsigned long long t = strtoull(v, nullptr, 10) * (unsigned long long)khz / 1000ull;
+  return t ? t : 1;
 }
 
 // Launches a kernel that synchronises its workgroups with grid_barrier:
@@ -169,17 +217,25 @@ inline hipError_t launch_grid(const void *fn, unsigned grid, unsigned threads,
   return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, lds, s);
 }
 
+// The errors with which the runtime refuses a cooperative launch it cannot
+// make co-resident (too many workgroups, or the device busy with another
+// process); any other launch error is a real failure.
+inline bool cooperative_refusal(hipError_t e) {
+  return e == hipErrorCooperativeLaunchTooLarge || e == hipErrorIllegalState;
+}
+
 // launch_grid for kernels whose work split and barriers follow gridDim
-// alone: a refused cooperative launch (this happens while another process
-// holds the device) is retried at half the width, down to one workgroup,
-// whose plain launch needs no co-residency.
-inline hipError_t launch_grid_shrinking(const void *fn, unsigned grid, unsigned threads,
+// alone: a refused cooperative launch is retried at half the width, down
+// to one workgroup, whose plain launch needs no co-residency.  *grid is
+// the width asked for on entry and the width launched on return; any
+// other error is returned as is.
+inline hipError_t launch_grid_shrinking(const void *fn, unsigned *grid, unsigned threads,
                                         void **args, size_t lds, hipStream_t s) {
-  hipError_t r = launch_grid(fn, grid, threads, args, lds, s);
-  while (r != hipSuccess && grid > 1) {
+  hipError_t r = launch_grid(fn, *grid, threads, args, lds, s);
+  while (cooperative_refusal(r) && *grid > 1) {
     (void)hipGetLastError();
-    grid = grid > 2 ? grid / 2 : 1;
-    r = launch_grid(fn, grid, threads, args, lds, s);
+    *grid = *grid > 2 ? *grid / 2 : 1;
+    r = launch_grid(fn, *grid, threads, args, lds, s);
   }
   return r;
 }

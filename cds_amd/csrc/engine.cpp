@@ -167,20 +167,22 @@ struct Timer {
   }
 };
 
-// Cooperative launches need every workgroup co-resident.  Engines on one
-// device may run concurrently (batched solves), so at most one cooperative
-// kernel per device is in flight: the gate is held from the launch until
-// the engine's stream has drained.
 // Cooperative launches need every workgroup co-resident.  Several engines
-// may share a device (one per host thread), so cooperative workgroups
-// in flight on a device are capped at its CU count: every cooperative
-// kernel here fits one 1024-thread workgroup per CU, so any set of them
-// within the cap is co-resident (other kernels only delay them).  A gate
-// holds its share from the launch until the engine's stream has drained.
+// may share a device (one per host thread, or the server's two lanes), so
+// the cooperative workgroups in flight on a device are budgeted per XCD:
+// the dispatcher deals a launch's workgroups round-robin over the XCDs
+// (eight on MI355X), so a width-G launch places at most ceil(G / 8) on any
+// one XCD whatever XCD it starts on.  Every cooperative kernel here runs
+// at most one workgroup per CU, so launches whose ceil(G / 8) sum to at
+// most CUs / 8 fit on every XCD at once (other kernels only delay them).
+// A gate holds its share from the launch until the engine's stream has
+// drained.  Widths that are multiples of 8 lose nothing to the rounding.
+constexpr int kXcds = 8;
+
 struct CoopBudget {
   std::mutex mu;
   std::condition_variable cv;
-  int cap = -1, avail = 0;  // cap -1: not yet read from the device
+  int cap = -1, avail = 0;  // per-XCD workgroup slots; -1: not yet read
 };
 
 CoopBudget &coop_budget(int device) {
@@ -190,30 +192,27 @@ CoopBudget &coop_budget(int device) {
 
 struct CoopGate {
   cds_engine *e;
-  int blocks;
-  CoopGate(cds_engine *eng, unsigned want) : e(eng), blocks((int)want) {
-    if (blocks <= 1) {  // one workgroup: a plain launch, nothing to reserve
-      blocks = 0;
-      return;
-    }
+  int slots;  // per-XCD workgroups reserved
+  CoopGate(cds_engine *eng, unsigned want) : e(eng), slots(0) {
+    if (want <= 1) return;  // one workgroup: a plain launch, nothing to reserve
     CoopBudget &b = coop_budget(e->device);
     std::unique_lock<std::mutex> lk(b.mu);
     if (b.cap < 0) {
       int cus = 0;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
-      b.cap = b.avail = cus > 0 ? cus : 1;
+      b.cap = b.avail = std::max(1, cus / kXcds);
     }
-    blocks = std::min(blocks, b.cap);
-    b.cv.wait(lk, [&] { return b.avail >= blocks; });
-    b.avail -= blocks;
+    slots = std::min((int)((want + kXcds - 1) / kXcds), b.cap);
+    b.cv.wait(lk, [&] { return b.avail >= slots; });
+    b.avail -= slots;
   }
   ~CoopGate() {
-    if (blocks == 0) return;
+    if (slots == 0) return;
     (void)hipStreamSynchronize(e->stream);
     CoopBudget &b = coop_budget(e->device);
     {
       std::lock_guard<std::mutex> lk(b.mu);
-      b.avail += blocks;
+      b.avail += slots;
     }
     b.cv.notify_all();
   }
@@ -283,6 +282,22 @@ bool flow_equal(__int128 f_scaled, float target, int sc) {
   __int128 d = f_scaled - t;
   if (d < 0) d = -d;
   return d * ((__int128)1 << 29) < (__int128)5368709 << sc;
+}
+
+// Records one cooperative launch: *field keeps the smallest width launched,
+// coop_shrinks counts launches the runtime made run narrower than asked.
+void note_width(cds_result *R, uint32_t *field, unsigned asked, unsigned launched) {
+  if (launched < asked) R->coop_shrinks++;
+  if (*field == 0 || launched < *field) *field = launched;
+}
+
+// A cooperative kernel ended because a grid barrier wait ran out.
+cds_status barrier_failed(const char *kernel, unsigned grid) {
+  return fail(CDS_ERR_HIP,
+              "%s on %u workgroups: a grid barrier wait ran out (the workgroups were "
+              "not all resident at once, or one stalled); CDS_BARRIER_TIMEOUT_US sets "
+              "the limit",
+              kernel, grid);
 }
 
 // Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
@@ -829,6 +844,8 @@ extern "C" void cds_result_free(cds_result *r) {
   free(r->trace_alpha);
   free(r->trace_component);
   free(r->trace_margin);
+  free(r->trace_fragile);
+  free(r->trace_drift);
   memset(r, 0, sizeof(*r));
 }
 
@@ -1148,19 +1165,22 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift,
                       uint32_t max_level, unsigned grid, uint64_t want_removed,
                       std::vector<Level> *out, PeelCtl *hc) -> cds_status {
+    unsigned launched = grid;
     {
       CoopGate gate(e, grid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
       HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
                              stamp.p, sbase, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
                              lkil.p, sdeg.p,
-                             alive0, na0, shift, max_level, grid, st));
+                             alive0, na0, shift, max_level, &launched, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
     }
+    note_width(R, &R->peel_grid, grid, launched);
+    CHK(readback(e, hc, ctl.p, 1));
+    if (hc->bar.abort) return barrier_failed("persistent peel", launched);
     HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     R->t_peel_kernel_ms += tm.ms(22, 23);
     R->peel_launches++;
-    CHK(readback(e, hc, ctl.p, 1));
     if (hc->rounds >= (1ull << 30) || sbase + hc->rounds + 1 >= (1u << 30))
       return fail(CDS_ERR_INTERNAL, "persistent peel: %llu sub-rounds", hc->rounds);
     sbase += (uint32_t)hc->rounds + 1;
@@ -1203,7 +1223,6 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     PeelCtl hc;
     CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
     removed_total = n;
-    R->peel_grid = grid;
   }
   HIPCHK(fa.alloc(n));
   HIPCHK(fb.alloc(n));
@@ -1361,15 +1380,22 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       std::optional<CoopGate> gate;
       if (rgrid) gate.emplace(e, rgrid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
+      const unsigned rasked = rgrid;
       HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
                           T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
                           pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
-                          pioff.p, pinc.p, rout.p, partial.p, gbar.p, rgrid,
+                          pioff.p, pinc.p, rout.p, partial.p, gbar.p, &rgrid,
                           sdeg.p, A, tm.ev[23], (uint64_t)PS.kp,
                           replay_nstar(k, (uint64_t)PS.kp), st));
       gate.reset();
+      if (rasked) note_width(R, &R->replay_grid, rasked, rgrid);
       ReplayOut ro;
       CHK(readback(e, &ro, rout.p, 1));
+      if (rgrid > 1) {
+        unsigned ab = 0;
+        CHK(readback(e, &ab, &gbar.p->abort, 1));
+        if (ab) return barrier_failed("whole-GPU replay", rgrid);
+      }
       {
         float prep = 0;
         (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
@@ -1672,16 +1698,20 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       G.trace = ftrace.p;
       G.trace_cap = kFlowTraceRounds;
     }
+    unsigned launched = 0;
     {
       const unsigned fgrid = coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e)));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
+      launched = fgrid;
       HIPCHK(hipEventRecord(tm.ev[22], st));
-      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, fgrid, st));
+      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, &launched, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
+      note_width(R, &R->flow_grid, fgrid, launched);
     }
     GridFlowCtl hc;
     CHK(readback(e, &hc, gctl.p, 1));
+    if (hc.bar.abort) return barrier_failed("whole-network max flow", launched);
     R->t_flow_kernel_ms += tm.ms(22, 23);
     R->flow_clique_visits += hc.clique_visits;
     R->flow_arc_visits += hc.chunk_visits * kFlowChunkArcs;
@@ -1942,7 +1972,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(kill.alloc(T));
   HIPCHK(sdeg.alloc(n));
   HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
-  uint32_t peel_grid = 0, peel_rounds = 0;
+  uint32_t peel_grid = 0, peel_rounds = 0, coop_shrinks = 0, replay_grid = 0;
   {
     DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A);
     DBuf<uint64_t> s64(&A);
@@ -1967,16 +1997,19 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     {
       CoopGate gate(e, peel_grid);
       HIPCHK(hipEventRecord(tm.ev[22], st));
+      const unsigned asked = peel_grid;
       HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p,
                              kill.p, stamp.p, 1, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
                              lkil.p,
-                             sdeg.p, nullptr, 0, 0, kAliveHost, peel_grid, st));
+                             sdeg.p, nullptr, 0, 0, kAliveHost, &peel_grid, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
+      if (peel_grid < asked) coop_shrinks++;
     }
-    t_peel = tm.ms(22, 23);
-    HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     PeelCtl hc;
     CHK(readback(e, &hc, ctl.p, 1));
+    if (hc.bar.abort) return barrier_failed("persistent peel", peel_grid);
+    t_peel = tm.ms(22, 23);
+    HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
     if (hc.removed != n)
       return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %u", hc.removed, n);
     peel_rounds = (uint32_t)hc.rounds;
@@ -2097,7 +2130,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       uint64_t killed = 0;
       for (const Level &lv : levels[b])
         if (lv.level == ph.level) killed = lv.killed;
-      const unsigned grid = coop_width(
+      unsigned grid = coop_width(
           e, replay_grid_blocks(e->device, killed * k / ph.M, e->cur.replay_blocks));
       DBuf<uint32_t> partial(&A), pcount(&A);
       DBuf<GridBarrier> gbar(&A);
@@ -2110,15 +2143,23 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       {
         CoopGate gate(e, grid);
         HIPCHK(hipEventRecord(tm.ev[22], st));
+        const unsigned asked = grid;
         HIPCHK(replay_phase_list(k, ph.level, sval.p + ph.base, ph.M, core.p, kill.p, ioff.p,
                                  inc.p, ph.T_start, ph.N_start, pdeg.p, posmap.p, palive.p,
-                                 rdead.p, pcount.p, hout.p, partial.p, gbar.p, grid, sdeg.p,
+                                 rdead.p, pcount.p, hout.p, partial.p, gbar.p, &grid, sdeg.p,
                                  kp, replay_nstar(k, kp), st));
         HIPCHK(hipEventRecord(tm.ev[23], st));
+        if (grid < asked) coop_shrinks++;
+        if (replay_grid == 0 || grid < replay_grid) replay_grid = grid;
       }
       t_replay += tm.ms(22, 23);
       ReplayOut ro;
       CHK(readback(e, &ro, hout.p, 1));
+      if (grid > 1) {
+        unsigned ab = 0;
+        CHK(readback(e, &ab, &gbar.p->abort, 1));
+        if (ab) return barrier_failed("whole-GPU replay", grid);
+      }
       if (ro.steps > ph.M || (!ro.stopped && ro.steps != ph.M))
         return fail(CDS_ERR_INTERNAL, "batch item %u: replay of level %u ran %u of %u steps",
                     b, ph.level, ro.steps, ph.M);
@@ -2146,6 +2187,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     R->kmax = S.kmax;
     R->peel_levels = peel_rounds;
     R->peel_grid = peel_grid;
+    R->replay_grid = replay_grid;
+    R->coop_shrinks = coop_shrinks;
     R->peel_distinct_levels = (uint32_t)levels[b].size();
     size_t cl = levels[b].size();
     for (size_t i = 0; i < levels[b].size(); ++i)
@@ -2486,6 +2529,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     CHK(readback(e, hreach.data(), freach.p, vo[np]));
     for (size_t i = 0; i < np; ++i) {
       const uint32_t b = pend[i].b;
+      if (hctl[i].bar.abort) return barrier_failed("batched max flow", all[i].nblocks);
       if (hctl[i].limit)
         return fail(CDS_ERR_NOT_CONVERGED, "batch item %u: max flow hit %u rounds", b,
                     hctl[i].rounds);

@@ -838,9 +838,10 @@ __global__ void k_flow_batch_count(const FlowBatchItem *__restrict__ items,
   if (threadIdx.x == 0) out[blockIdx.x] = s_c;
 }
 
-__global__ void k_grid_ctl_init(GridFlowCtl *ctl) {
+__global__ void k_grid_ctl_init(GridFlowCtl *ctl, unsigned long long limit) {
   unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
   for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
+  ctl->bar.limit = limit;
   ctl->lpack[0] = ctl->lpack[1] = 0;
   ctl->cqn[0] = ctl->cqn[1] = 0;
   ctl->active = 0;
@@ -913,9 +914,9 @@ unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {
 
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
-                         unsigned grid, hipStream_t s) {
-  if (grid < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_grid_ctl_init, dim3(1), dim3(1), 0, s, S.ctl);
+                         unsigned *grid, hipStream_t s) {
+  if (*grid < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_grid_ctl_init, dim3(1), dim3(1), 0, s, S.ctl, barrier_limit_ticks());
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;

@@ -122,11 +122,17 @@ hipError_t peel_process(uint32_t k, uint32_t level, const uint32_t *frontier,
 // Grid barrier state of a cooperative launch: a generation word and
 // arrival counters (one per group of workgroups blockIdx % 8, plus one for
 // the groups), each in its own 128-byte line.  Counters only grow; zero
-// them before the launch.
+// them before the launch.  A wait lasts at most `limit` wall-clock ticks
+// (0: kBarrierDefaultTicks); a workgroup whose wait runs out sets `abort`
+// and ends, and so does every workgroup that reaches a barrier after that.
+// The host reads `abort` after the launch and fails the call instead of
+// hanging when workgroups are not co-resident.
 struct GridBarrier {
   unsigned int gen, pad0[31];
   unsigned int top, pad1[31];
   unsigned int sub[8 * 32];
+  unsigned int abort, pad2[31];
+  unsigned long long limit, pad3[15];
 };
 
 // Whole peel in one cooperative launch (see peel.hip).  Control block:
@@ -172,7 +178,7 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
                            const uint32_t *alive0, uint32_t na0, uint32_t cshift,
-                           uint32_t max_level, unsigned grid, hipStream_t s);
+                           uint32_t max_level, unsigned *grid, hipStream_t s);
 // Prepares the exact re-peel of one coarse step whose vertices and cliques
 // carry the value `level`: those vertices become alive (core = kAlive,
 // degree = sdeg, listed in alive[], count in *count), vertices of later
@@ -234,7 +240,7 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
                              uint64_t T_start, uint64_t N_start, uint32_t *pdeg,
                              uint32_t *posmap, uint8_t *palive, uint32_t *rdead,
                              uint32_t *pcount, ReplayOut *out, uint32_t *partial,
-                             GridBarrier *bar, unsigned grid, const uint32_t *sdeg,
+                             GridBarrier *bar, unsigned *grid, const uint32_t *sdeg,
                              uint64_t kp, uint64_t nstar, hipStream_t s);
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
@@ -243,7 +249,7 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
                         uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
-                        uint32_t *partial, GridBarrier *bar, unsigned grid,
+                        uint32_t *partial, GridBarrier *bar, unsigned *grid,
                         const uint32_t *sdeg, DeviceArena &arena,
                         hipEvent_t prepared, uint64_t kp, uint64_t nstar,
                         hipStream_t s);
@@ -424,7 +430,7 @@ hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
 unsigned flow_batch_capacity(uint32_t max_v, int device);
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
-                         unsigned grid, hipStream_t s);
+                         unsigned *grid, hipStream_t s);
 // Workgroups for grid_max_flow on P (0: cannot launch); `blocks` != 0
 // overrides the size-based choice.
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks);

@@ -627,9 +627,10 @@ __global__ void k_iota(uint32_t n, uint32_t *a) {
     a[i] = i;
 }
 
-__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t alive) {
+__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t alive, unsigned long long limit) {
   unsigned *bw = reinterpret_cast<unsigned *>(&ctl->bar);
   for (size_t i = 0; i < sizeof(GridBarrier) / sizeof(unsigned); ++i) bw[i] = 0;
+  ctl->bar.limit = limit;
   ctl->gmin[0] = kAlive;
   ctl->gmin[1] = kAlive;
   ctl->alive_n[0] = alive;
@@ -982,10 +983,12 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_grid(
   }
 }
 
-__global__ void k_zero_barrier(GridBarrier *b) {
+__global__ void k_zero_barrier(GridBarrier *b, unsigned long long limit) {
   unsigned *w = reinterpret_cast<unsigned *>(b);
   for (size_t i = threadIdx.x; i < sizeof(GridBarrier) / sizeof(unsigned); i += blockDim.x)
     w[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) b->limit = limit;
 }
 
 }  // namespace
@@ -1072,8 +1075,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
                            unsigned long long *lv_removed,
                            unsigned long long *lv_killed, uint32_t *sdeg,
                            const uint32_t *alive0, uint32_t na0, uint32_t cshift,
-                           uint32_t max_level, unsigned grid, hipStream_t s) {
-  if (n >= (1u << 24) || grid < 1 || cshift > 31) return hipErrorInvalidValue;
+                           uint32_t max_level, unsigned *grid, hipStream_t s) {
+  if (n >= (1u << 24) || *grid < 1 || cshift > 31) return hipErrorInvalidValue;
   if (max_level >= kParked) max_level = kParked - 1;
   const uint32_t na = alive0 ? na0 : n;
   hipError_t e;
@@ -1091,7 +1094,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
   } else {
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, A0);
   }
-  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, na);
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, na,
+                     barrier_limit_ticks());
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (sbase < 1 || sbase > kParked / 2) return hipErrorInvalidValue;
   // LDS hash aggregation of decrements (CDS_PEEL_HASH=0 turns it off)
@@ -1131,9 +1135,9 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
                              uint64_t T_start, uint64_t N_start, uint32_t *pdeg,
                              uint32_t *posmap, uint8_t *palive, uint32_t *rdead,
                              uint32_t *pcount, ReplayOut *out, uint32_t *partial,
-                             GridBarrier *bar, unsigned grid, const uint32_t *sdeg,
+                             GridBarrier *bar, unsigned *grid, const uint32_t *sdeg,
                              uint64_t kp, uint64_t nstar, hipStream_t s) {
-  if (grid < 1 || M == 0 || M > kReplayLdsCap) return hipErrorInvalidValue;
+  if (*grid < 1 || M == 0 || M > kReplayLdsCap) return hipErrorInvalidValue;
   hipError_t e = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(pcount), M, 1, s);
   if (e != hipSuccess) return e;
   if ((e = hipMemsetAsync(out, 0, sizeof(ReplayOut), s)) != hipSuccess) return e;
@@ -1141,7 +1145,7 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
                      posmap);
   hipLaunchKernelGGL(k_phase_deg_snap, dim3(grid_for(M, 256, 4096)), dim3(256), 0, s,
                      pcount, pv, sdeg, pdeg, palive);
-  hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar);
+  hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar, barrier_limit_ticks());
   if ((e = hipGetLastError()) != hipSuccess) return e;
   uint32_t *pvm = const_cast<uint32_t *>(pv);  // the kernel reads it only
   void *args[] = {&k, &level, &M, &pvm, &pdeg, (void *)&posmap, &palive,
@@ -1158,11 +1162,11 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         uint32_t *pv, uint32_t *pdeg, uint32_t *posmap,
                         uint8_t *palive, uint32_t *rdead, uint32_t *pcount,
                         uint64_t *pioff, uint32_t *pinc, ReplayOut *out,
-                        uint32_t *partial, GridBarrier *bar, unsigned grid,
+                        uint32_t *partial, GridBarrier *bar, unsigned *grid,
                         const uint32_t *sdeg, DeviceArena &arena,
                         hipEvent_t prepared, uint64_t kp, uint64_t nstar,
                         hipStream_t s) {
-  if (grid && M > kReplayLdsCap) return hipErrorInvalidValue;
+  if (*grid && M > kReplayLdsCap) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(pcount, 0, sizeof(uint32_t), s);
   if (e != hipSuccess) return e;
   if ((e = hipMemsetAsync(out, 0, sizeof(ReplayOut), s)) != hipSuccess) return e;
@@ -1171,15 +1175,15 @@ hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                      0, s, n, level, core, pv, posmap, pcount);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const unsigned g = grid_for((size_t)M * kWave, 256, 8192);
-  if (grid && sdeg)
+  if (*grid && sdeg)
     hipLaunchKernelGGL(k_phase_deg_snap, dim3(grid_for(M, 256, 4096)), dim3(256),
                        0, s, pcount, pv, sdeg, pdeg, palive);
   else
     hipLaunchKernelGGL(k_phase_deg, dim3(g), dim3(256), 0, s, k, level, pcount,
                        pv, ioff, inc, kill, pdeg, palive);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (grid) {
-    hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar);
+  if (*grid) {
+    hipLaunchKernelGGL(k_zero_barrier, dim3(1), dim3(256), 0, s, bar, barrier_limit_ticks());
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (prepared) (void)hipEventRecord(prepared, s);
     void *args[] = {&k, &level, &M, &pv, &pdeg, (void *)&posmap, &palive,

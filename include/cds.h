@@ -156,6 +156,27 @@ typedef struct {
                                        discharge (phase 2, all rounds) */
   uint64_t flow_relabels;           /* global relabel passes of those flows */
   cds_times times;
+  /* Cooperative launches (persistent peel, whole-network max flow,
+   * whole-GPU replay) keep all their workgroups resident.  peel_grid above,
+   * flow_grid and replay_grid are the widths they actually ran at (the
+   * smallest over the solve's launches of that kernel; 0: none ran);
+   * coop_shrinks counts launches that ran narrower than asked because the
+   * runtime refused the cooperative launch (another process on the GPU). */
+  uint32_t flow_grid;
+  uint32_t replay_grid;
+  uint32_t coop_shrinks;
+  /* Float fragility of the density search.  The reference decides each
+   * alpha step with a float Edmonds-Karp (cpuCode/c++/src/helpers.cpp:
+   * 1279-1290, |F - kT| < 1e-2f); this engine decides exactly.
+   * trace_drift[i] bounds how far a float flow sum can stray from the exact
+   * F at step i; trace_fragile[i] = 1 when the exact margin
+   * trace_margin[i] lies within that bound of the 1e-2 threshold, i.e. the
+   * reference may take the other branch there.  fragile_decisions counts
+   * them; when it is 0 the reference's float search takes the same
+   * decisions as this result. */
+  uint32_t fragile_decisions;
+  uint8_t *trace_fragile;           /* trace_len entries */
+  double *trace_drift;              /* trace_len entries */
 } cds_result;
 
 typedef struct {

@@ -1123,6 +1123,28 @@ char *oracle_solve_json(ui n, const ui *offsets, const ui *neighbors, ui k,
   return out;
 }
 
+// Listing and the fast peel only: the removal rows (vertex, value) and the
+// cliques left after each row, as JSON (free with oracle_free).
+char *oracle_peel_rows_json(ui n, const ui *offsets, const ui *neighbors, ui k) {
+  auto adj = adj_from_csr(n, offsets, neighbors);
+  Result R;
+  R.k = k;
+  R.n = n;
+  std::vector<ui> members;
+  list_full(adj, k, R, &members);
+  peel(adj, k, members, PEEL_FAST, R);
+  Json j;
+  j.s = "{";
+  j.inum("total_cliques", R.total_cliques);
+  j.arr("removal_vertex", R.removal_vertex);
+  j.arr("removal_value", R.removal_value);
+  j.arr("row_remaining", R.row_remaining);
+  j.s += "}";
+  char *out = (char *)malloc(j.s.size() + 1);
+  memcpy(out, j.s.c_str(), j.s.size() + 1);
+  return out;
+}
+
 // Listing only: T and per-vertex clique degrees.
 long oracle_count_cliques(ui n, const ui *offsets, const ui *neighbors, ui k,
                           long *clique_degree) {

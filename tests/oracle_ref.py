@@ -30,6 +30,8 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int]
         L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_peel_rows_json.restype = ctypes.c_void_p
+        L.oracle_peel_rows_json.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32]
         L.oracle_count_cliques.restype = ctypes.c_long
         L.oracle_count_cliques.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_long)]
@@ -48,6 +50,17 @@ def oracle_solve(graph, k, peel=PEEL_FAST, flow=FLOW_EXACT, ub1=True, ub2=True,
     g = graph._g
     p = lib().oracle_solve_json(g.n, g.offsets, g.neighbors, k, peel, flow,
                                 int(ub1), int(ub2), int(rows))
+    try:
+        return json.loads(ctypes.string_at(p).decode())
+    finally:
+        lib().oracle_free(p)
+
+
+def oracle_peel_rows(graph, k):
+    """Listing + fast peel only: total_cliques, removal_vertex,
+    removal_value (rows 1..n, index 0 unused) and row_remaining."""
+    g = graph._g
+    p = lib().oracle_peel_rows_json(g.n, g.offsets, g.neighbors, k)
     try:
         return json.loads(ctypes.string_at(p).decode())
     finally:
