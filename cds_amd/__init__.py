@@ -172,6 +172,7 @@ _SIGNATURES = {
     "cds_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
     "cds_result_free": (None, [ctypes.POINTER(CResult)]),
     "cds_count_cliques": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, f64p]),
+    "cds_count_cliques_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, u64p, u64p, f64p]),
     "cds_core_decompose": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CParams), ctypes.POINTER(CResult)]),
     "cds_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(CParams), ctypes.c_size_t, ctypes.POINTER(CResult)]),
     "cds_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerOpts), ctypes.POINTER(ctypes.c_void_p)]),
@@ -449,6 +450,17 @@ class Engine:
         ms = ctypes.c_double()
         cd = np.zeros(dgraph.n, dtype=np.uint64) if clique_degree else None
         _check(lib().cds_count_cliques(
+            self._h, dgraph._h, k, ctypes.byref(T),
+            cd.ctypes.data_as(u64p) if cd is not None else None, ctypes.byref(ms)))
+        return T.value, cd, ms.value
+
+    def count_cliques_noalloc(self, dgraph, k, clique_degree=True):
+        """cds_count_cliques_noalloc: count_cliques with no device allocation
+        (scratch in the library's static device array)."""
+        T = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        cd = np.zeros(dgraph.n, dtype=np.uint64) if clique_degree else None
+        _check(lib().cds_count_cliques_noalloc(
             self._h, dgraph._h, k, ctypes.byref(T),
             cd.ctypes.data_as(u64p) if cd is not None else None, ctypes.byref(ms)))
         return T.value, cd, ms.value

@@ -669,6 +669,8 @@ extern "C" cds_status cds_engine_create(const cds_opts *opts, cds_engine **out) 
     }
     r = hipHostMalloc(&e->pinned, kPinnedBytes, hipHostMallocDefault);
   }
+  // this device's copy of the zero-allocation scratch, looked up once
+  if (r == hipSuccess) r = noalloc_attach(e->device);
   if (r != hipSuccess) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1062,6 +1064,115 @@ extern "C" cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *g,
   *total = L.T;
   if (clique_degree) CHK(readback(e, clique_degree, L.cd.p, g->n));
   if (kernel_ms) *kernel_ms = L.t_count;
+  return CDS_OK;
+}
+
+// No counterpart in the reference: cds_count_cliques for callers that
+// cannot allocate.  Every scratch buffer is a slice of this device's static
+// scratch array (its address recorded at engine creation); results reach
+// the caller through the engine's pinned block.  Nothing here allocates:
+// no hipMalloc, no arena, no events.
+extern "C" cds_status cds_count_cliques_noalloc(cds_engine *e, const cds_dgraph *g,
+                                                uint32_t k, uint64_t *total,
+                                                uint64_t *clique_degree,
+                                                double *kernel_ms) {
+  if (!e || !g || !total) return fail(CDS_ERR_INVALID, "null argument");
+  if (k < 2 || k > 4)
+    return fail(CDS_ERR_UNSUPPORTED, "k=%u: the device lister covers k=2..4", k);
+  std::lock_guard<std::mutex> lk(e->mu);
+  HIPCHK(hipSetDevice(e->device));
+  char *base = static_cast<char *>(noalloc_scratch(e->device));
+  if (!base) return fail(CDS_ERR_INTERNAL, "no scratch recorded for device %d", e->device);
+  begin_solve(e, k, g->m);
+  const ListShape shape = list_shape(e);
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  hipStream_t st = e->stream;
+  // slices, each 256-byte aligned; items_n <= n + m / chunk bounds the items
+  const uint64_t items_max = (uint64_t)n + m / shape.chunk + 1;
+  const size_t ws = scan_ws_words(std::max<uint64_t>(n, 1));
+  size_t used = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t at = used;
+    used += (bytes + 255) & ~(size_t)255;
+    return at;
+  };
+  const size_t o_dcnt = carve(4ull * n), o_doff = carve(4ull * (n + 1)),
+               o_dadj = carve(4ull * m), o_maxd = carve(4), o_cd = carve(8ull * n),
+               o_ipv = carve(4ull * n), o_ioff = carve(8ull * (n + 1)),
+               o_items = carve(8ull * items_max), o_icount = carve(4ull * items_max),
+               o_tsum = carve(8), o_ws32 = carve(4 * ws), o_ws64 = carve(8 * ws);
+  if (used > kNoallocScratchBytes)
+    return fail(CDS_ERR_RANGE,
+                "graph needs %zu bytes of scratch (n=%u, m=%llu); the zero-allocation "
+                "count holds %zu",
+                used, n, (unsigned long long)m, (size_t)kNoallocScratchBytes);
+  auto *dcnt = reinterpret_cast<uint32_t *>(base + o_dcnt);
+  auto *doff = reinterpret_cast<uint32_t *>(base + o_doff);
+  auto *dadj = reinterpret_cast<uint32_t *>(base + o_dadj);
+  auto *maxd = reinterpret_cast<uint32_t *>(base + o_maxd);
+  auto *cd = reinterpret_cast<uint64_t *>(base + o_cd);
+  auto *ipv = reinterpret_cast<uint32_t *>(base + o_ipv);
+  auto *ioff = reinterpret_cast<uint64_t *>(base + o_ioff);
+  auto *items = reinterpret_cast<uint2 *>(base + o_items);
+  auto *icount = reinterpret_cast<uint32_t *>(base + o_icount);
+  auto *tsum = reinterpret_cast<unsigned long long *>(base + o_tsum);
+  auto *ws32 = reinterpret_cast<uint32_t *>(base + o_ws32);
+  auto *ws64 = reinterpret_cast<uint64_t *>(base + o_ws64);
+  uint32_t *hp = static_cast<uint32_t *>(e->pinned);  // the caller part of the block
+  auto *hp64 = reinterpret_cast<uint64_t *>(static_cast<char *>(e->pinned) + 8);
+  // orientation, DAG offsets and lists, largest out-degree
+  HIPCHK(dag_count(n, g->off, g->adj, dcnt, st));
+  HIPCHK(scan_u32_32_ws(dcnt, doff, n, ws32, ws, st));
+  HIPCHK(dag_fill(n, g->off, g->adj, doff, dadj, st));
+  HIPCHK(reduce_max_u32(dcnt, n, maxd, st));
+  HIPCHK(fill_u64(cd, n, 0, st));
+  HIPCHK(hipMemcpyAsync(hp, maxd, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const uint32_t max_out = hp[0];
+  uint64_t T = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (k == 2) {
+    HIPCHK(list_edges(n, doff, dadj, cd, nullptr, st));
+    T = m;
+  } else {
+    if (k == 4 && max_out > kList4LdsCap)
+      return fail(CDS_ERR_UNSUPPORTED,
+                  "k=4 listing needs every DAG out-degree <= %u (max is %u)", kList4LdsCap,
+                  max_out);
+    HIPCHK(list_item_counts(n, shape.chunk, doff, ipv, st));
+    HIPCHK(scan_u32_ws(ipv, ioff, n, ws64, ws, st));
+    HIPCHK(hipMemcpyAsync(hp64, ioff + n, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint64_t items_n = hp64[0];
+    if (items_n > items_max)
+      return fail(CDS_ERR_INTERNAL, "%llu listing items above the bound %llu",
+                  (unsigned long long)items_n, (unsigned long long)items_max);
+    HIPCHK(list_item_fill(n, shape.chunk, doff, ioff, items, st));
+    HIPCHK(list_cliques(k, shape, items_n, items, doff, dadj, max_out, icount, cd, nullptr,
+                        nullptr, st));
+    HIPCHK(sum_u32(icount, items_n, tsum, st));
+    HIPCHK(hipMemcpyAsync(hp64, tsum, 8, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  if (k != 2) T = hp64[0];
+  if (kernel_ms)
+    *kernel_ms = std::chrono::duration<double, std::milli>(
+                     std::chrono::steady_clock::now() - t0)
+                     .count();
+  *total = T;
+  if (clique_degree) {
+    // through the pinned block, past its first kPinnedScratch bytes
+    const size_t per = (kPinnedBytes - kPinnedScratch) / sizeof(uint64_t);
+    auto *stage =
+        reinterpret_cast<uint64_t *>(static_cast<char *>(e->pinned) + kPinnedScratch);
+    for (size_t at = 0; at < n; at += per) {
+      const size_t c = std::min<size_t>(per, n - at);
+      HIPCHK(hipMemcpyAsync(stage, cd + at, c * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      memcpy(clique_degree + at, stage, c * 8);
+    }
+  }
   return CDS_OK;
 }
 

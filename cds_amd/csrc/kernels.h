@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../include/cds.h"
+
 namespace cds {
 
 constexpr uint32_t kAliveHost = 0xFFFFFFFFu;  // same value as kAlive (dev.h)
@@ -54,6 +56,27 @@ hipError_t scan_u64(const uint64_t *in, uint64_t *out, size_t n,
 // Same with 32-bit output (caller guarantees total < 2^32).
 hipError_t scan_u32_32(const uint32_t *in, uint32_t *out, size_t n,
                        DeviceArena &a);
+
+// The same scans with their block sums in a caller-given workspace of
+// scan_ws_words(n) elements of the output type (no arena, no allocation).
+size_t scan_ws_words(size_t n);
+hipError_t scan_u32_ws(const uint32_t *in, uint64_t *out, size_t n, uint64_t *ws,
+                       size_t ws_words, hipStream_t s);
+hipError_t scan_u32_32_ws(const uint32_t *in, uint32_t *out, size_t n, uint32_t *ws,
+                          size_t ws_words, hipStream_t s);
+
+// ---- zero-allocation scratch (noalloc.hip) ----
+// Bytes of the static device array behind cds_count_cliques_noalloc.
+constexpr size_t kNoallocScratchBytes = CDS_NOALLOC_SCRATCH_BYTES;
+// Records this device's address of the array (hipGetSymbolAddress); call
+// once per device with that device current.
+hipError_t noalloc_attach(int device);
+// The address noalloc_attach recorded for `device` (null before it ran).
+void *noalloc_scratch(int device);
+// a[0..n) = v, by a kernel.
+hipError_t fill_u64(uint64_t *a, size_t n, uint64_t v, hipStream_t s);
+// *out = sum of in[0..n) (out is cleared by a kernel first).
+hipError_t sum_u32(const uint32_t *in, size_t n, unsigned long long *out, hipStream_t s);
 
 // ---- DAG (dag.hip) ----
 // Degree-ordered orientation: u -> v iff (deg u, u) < (deg v, v).

@@ -168,6 +168,41 @@ hipError_t scan_impl(const Tin *in, Tout *out, size_t n, DeviceArena &a) {
   return e;
 }
 
+template <typename Tout>
+__global__ void k_zero_one(Tout *out) {
+  *out = 0;
+}
+
+// scan_impl with its block sums in a caller-given workspace instead of the
+// arena (scan_ws_words(n) words of Tout): bsum (nb), bscan (nb + 1), then
+// the workspace of the recursive scan of the block sums.
+template <typename Tin, typename Tout>
+hipError_t scan_ws_impl(const Tin *in, Tout *out, size_t n, Tout *ws, size_t ws_words,
+                        hipStream_t s) {
+  if (ws_words < scan_ws_words(n)) return hipErrorInvalidValue;
+  if (n == 0) {
+    hipLaunchKernelGGL((k_zero_one<Tout>), dim3(1), dim3(1), 0, s, out);
+    return hipGetLastError();
+  }
+  const size_t nb = (n + kScanPerBlock - 1) / kScanPerBlock;
+  Tout *bsum = ws;
+  hipLaunchKernelGGL((k_scan_local<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0,
+                     s, in, out, bsum, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_set_total<Tout>), dim3(1), dim3(1), 0, s, out, n, bsum);
+    return hipGetLastError();
+  }
+  Tout *bscan = ws + nb;
+  e = scan_ws_impl<Tout, Tout>(bsum, bscan, nb, ws + 2 * nb + 1, ws_words - 2 * nb - 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_scan_add<Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, out,
+                     bscan, n);
+  hipLaunchKernelGGL((k_set_total<Tout>), dim3(1), dim3(1), 0, s, out, n, bscan + nb);
+  return hipGetLastError();
+}
+
 __global__ void k_reduce_max_u32(const uint32_t *__restrict__ in, size_t n,
                                  uint32_t *out) {
   uint32_t m = 0;
@@ -192,6 +227,20 @@ hipError_t scan_u64(const uint64_t *in, uint64_t *out, size_t n,
 hipError_t scan_u32_32(const uint32_t *in, uint32_t *out, size_t n,
                        DeviceArena &a) {
   return scan_impl<uint32_t, uint32_t>(in, out, n, a);
+}
+
+size_t scan_ws_words(size_t n) {
+  const size_t nb = (n + kScanPerBlock - 1) / kScanPerBlock;
+  if (nb <= 1) return 1;
+  return 2 * nb + 1 + scan_ws_words(nb);
+}
+hipError_t scan_u32_ws(const uint32_t *in, uint64_t *out, size_t n, uint64_t *ws,
+                       size_t ws_words, hipStream_t s) {
+  return scan_ws_impl<uint32_t, uint64_t>(in, out, n, ws, ws_words, s);
+}
+hipError_t scan_u32_32_ws(const uint32_t *in, uint32_t *out, size_t n, uint32_t *ws,
+                          size_t ws_words, hipStream_t s) {
+  return scan_ws_impl<uint32_t, uint32_t>(in, out, n, ws, ws_words, s);
 }
 
 hipError_t reduce_max_u32(const uint32_t *in, size_t n, uint32_t *out,

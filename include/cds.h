@@ -292,6 +292,22 @@ cds_status cds_count_cliques(cds_engine *e, const cds_dgraph *dg, uint32_t k,
                              uint64_t *total, uint64_t *clique_degree,
                              double *kernel_ms);
 
+/* cds_count_cliques without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as cds_count_cliques; every scratch buffer is
+ * a slice of one static device array of CDS_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends), and results come back through the
+ * engine's pinned block.  A graph whose scratch (about 20n + 4m + 12
+ * (n + m/64) bytes) exceeds the array returns CDS_ERR_RANGE.  Calls on one
+ * GPU share the array: calls that overlap on one GPU (several engines on
+ * one device, driven from several threads) race.  kernel_ms is host wall
+ * time around the listing (no events are created). */
+#define CDS_NOALLOC_SCRATCH_BYTES (64ull << 20)
+cds_status cds_count_cliques_noalloc(cds_engine *e, const cds_dgraph *dg, uint32_t k,
+                                     uint64_t *total, uint64_t *clique_degree,
+                                     double *kernel_ms);
+
 /* Clique-core stage only: listing, peel and densest-core extraction
  * (cliqueCoreDecompose + generateDensestCore, cudaCode/main.cu:252-399;
  * oracle cliqueCoreDecompose + locateDensestCore,
