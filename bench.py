@@ -340,19 +340,67 @@ def read_traffic(kernel, workload):
     return None
 
 
-CPU_SAMPLE_SCALE = 8  # the reference needs ~2 s per graph here, ~2 min at scale 9
-CPU_SAMPLE_SECONDS = 12.0  # graphs are added (new seeds) until this much CPU time
+def host_cpu():
+    """The host's CPU model (lscpu) and core counts."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
+        model = m.group(1).strip() if m else None
+    except Exception:
+        pass
+    return dict(model=model, nproc=os.cpu_count(),
+                affinity=len(os.sched_getaffinity(0)))
+
+
+def _threads_available():
+    """Cores this process may use: OMP_NUM_THREADS when set (the GPU host
+    sets it to the share of one GPU), else the affinity mask."""
+    v = os.environ.get("OMP_NUM_THREADS")
+    if v and v.isdigit() and int(v) > 0:
+        return int(v)
+    return len(os.sched_getaffinity(0))
+
+
+def _run_reference(cmd, timeout):
+    """Runs a reference binary; returns (wall s, stdout, status) with status
+    'ok', 'timeout' or 'exit <code>'."""
+    t0 = time.time()
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return time.time() - t0, "", "timeout"
+    return time.time() - t0, r.stdout, "ok" if r.returncode == 0 else f"exit {r.returncode}"
+
+
+def _printed_answer(out):
+    dens = re.search(r"^Density:\s*(\S+)", out, re.M)
+    size = re.search(r"^Size:\s*(\d+)", out, re.M)
+    verts = re.search(r"^densest subgraph:\s*(.*)$", out, re.M)
+    return (dens.group(1) if dens else None, int(size.group(1)) if size else None,
+            [int(x) for x in verts.group(1).split()] if verts else None)
+
+
+C1_REPS = 3          # serial reference solves of C1 (about 1.7 s each)
+C2_CAP_S = 30.0      # wall cap for the serial reference on the scale-20 graph
 
 
 def cpu_baseline(tmpdir, eng):
-    """The reference's own CPU pipeline (oracle/_ref/dsd, built from
-    cpuCode/c++) timed on this host, one core, on a bounded sample: full
-    solves of R-MAT graphs with the same generator parameters at scale 8,
-    one seed after another until about CPU_SAMPLE_SECONDS of work (its float
-    Edmonds-Karp over string-keyed cliques and O(n^2) peel make larger
-    scales take minutes to hours).  The GPU is timed on the same sample
-    graphs for a like-for-like ratio."""
+    """The reference's CPU paths timed on this host, in this run.
+
+    * cpu_baseline proper: cpuCode/c++ (oracle/_ref/dsd, built from the
+      reference's sources), one core, C1_REPS full solves of C1 (Chung-Lu
+      n=10,000, the reference-runnable configuration), with its printed
+      answer compared to the engine's solve of the same graph.
+    * cpuCode/parallel (oracle/_ref/dsd_parallel, its sources with a main
+      that takes the thread count) on C1 at 1 thread, at its built-in 2
+      threads and at every available core; a run that aborts is reported
+      with its exit status (its connectedComponentDecompose overflows a
+      heap buffer on C1 with more than one thread, helpers.cpp:1396).
+    * the serial reference on the scale-20 graph under a C2_CAP_S wall cap:
+      it does not finish (O(n^2) peel), so the speedup there is a bound."""
     dsd = os.path.join(ROOT, "oracle", "_ref", "dsd")
+    dsdp = os.path.join(ROOT, "oracle", "_ref", "dsd_parallel")
     if not os.path.exists(dsd):
         return None
     import cds_amd
@@ -360,46 +408,74 @@ def cpu_baseline(tmpdir, eng):
     motif = os.path.join(tmpdir, "motif3.txt")
     with open(motif, "w") as f:
         f.write("3 1 3\n0 1\n0 2\n1 2\n")
-    scale = CPU_SAMPLE_SCALE
-    cpu_ms, gpu_ms, same, seeds, differ = 0.0, 0.0, 0, [], []
-    seed = 2
-    while cpu_ms < CPU_SAMPLE_SECONDS * 1000 and len(seeds) < 32:
-        g = cds_amd.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=seed)
-        path = os.path.join(tmpdir, f"rmat{scale}_{seed}.txt")
-        g.write(path)
-        t0 = time.time()
-        r = subprocess.run([dsd, path, motif, "1", "1"], capture_output=True,
-                           text=True, timeout=120)
-        wall = time.time() - t0
-        m = re.search(r"Execution_time:\s*([0-9.eE+-]+)", r.stdout)
-        cpu_ms += float(m.group(1)) if m else wall * 1000
-        dg = eng.upload(g)
-        eng.solve(dg, 3)
-        reps, t1 = 5, time.perf_counter()
-        for _ in range(reps):
-            gr = eng.solve(dg, 3)
-        gpu_ms += (time.perf_counter() - t1) * 1000 / reps
-        dens = re.search(r"^Density:\s*(\S+)", r.stdout, re.M)
-        # the reference prints Density with 6 significant digits
-        gd = f"{float(gr['density']):g}"
-        if dens is not None and dens.group(1) == gd:
-            same += 1
-        else:
-            differ.append({"seed": seed, "reference": dens.group(1) if dens else None,
-                           "gpu": gd, "gpu_exact": f"{gr['set_cliques']}/{gr['set_size']}"})
-        seeds.append(seed)
-        seed += 1
-    nsolves = len(seeds)
-    return dict(value=1000.0 * nsolves / cpu_ms, unit="solves/s", cores=1, kind="reference",
-                sample=f"{nsolves} full solves of R-MAT scale {scale} edge factor 16 "
-                       f"graphs (same a,b,c,d, seeds {seeds[0]}..{seeds[-1]}), k=3, by "
-                       f"cpuCode/c++ built from source (-O3 -march=x86-64-v3 "
-                       f"-ffast-math): {cpu_ms:.0f} ms",
-                gpu_same_sample_solves_per_s=1000.0 * nsolves / gpu_ms,
-                # equal printed Density on `same` of the graphs; a difference is
-                # the reference's float Edmonds-Karp taking another branch than
-                # the exact decision (see the oracle's float mode)
-                same_density_printed=f"{same}/{nsolves}", density_differs=differ)
+    c1 = make_graph(WORKLOADS["c1"])
+    path = os.path.join(tmpdir, "c1.txt")
+    c1.write(path)
+    times, answers = [], set()
+    out = ""
+    for _ in range(C1_REPS):
+        wall, out, st = _run_reference([dsd, path, motif, "1", "1"], 120)
+        if st != "ok":
+            return dict(error=f"cpuCode/c++ on C1: {st}")
+        m = re.search(r"Execution_time:\s*([0-9.eE+-]+)", out)
+        times.append(float(m.group(1)) / 1000.0 if m else wall)
+        answers.add(repr(_printed_answer(out)))
+    cpu_s = sorted(times)[len(times) // 2]
+    dens, size, verts = _printed_answer(out)
+    dg = eng.upload(c1)
+    gr = eng.solve(dg, 3)
+    reps, t1 = 20, time.perf_counter()
+    for _ in range(reps):
+        gr = eng.solve(dg, 3)
+    gpu_s = (time.perf_counter() - t1) / reps
+    dg.free()
+    gpu_answer = (f"{float(gr['density']):g}", gr["size"], list(gr["vertices"]))
+    res = dict(value=1.0 / cpu_s, unit="solves/s", cores=1, kind="reference",
+               sample=f"C1 (Chung-Lu n=10000, exponent 2.5, mean degree 10, planted "
+                      f"G(40,0.9), seed 1, k=3): median of {C1_REPS} full solves by "
+                      f"cpuCode/c++ built from source (-O3 -march=x86-64-v3 "
+                      f"-ffast-math), {cpu_s * 1000:.0f} ms each",
+               host=host_cpu(),
+               gpu_same_graph_solves_per_s=1.0 / gpu_s,
+               speedup_same_graph=cpu_s / gpu_s,
+               same_answer=(dens, size, verts) == gpu_answer and len(answers) == 1,
+               reference_answer={"density": dens, "size": size},
+               gpu_answer={"density": gpu_answer[0], "size": gpu_answer[1],
+                           "exact": f"{gr['set_cliques']}/{gr['set_size']}"})
+    if os.path.exists(dsdp):
+        par = []
+        for t in sorted({1, 2, _threads_available()}):
+            wall, pout, st = _run_reference([dsdp, path, motif, "1", "1", str(t)], 120)
+            m = re.search(r"Execution_time:\s*([0-9.eE+-]+)", pout)
+            entry = dict(threads=t, status=st)
+            if st == "ok":
+                entry["seconds"] = float(m.group(1)) / 1000.0 if m else wall
+                entry["solves_per_s"] = 1.0 / entry["seconds"]
+            par.append(entry)
+        res["cpuCode_parallel_C1"] = par
+    return res
+
+
+def cpu_c2_bound(tmpdir, graph, gpu_ms):
+    """The serial reference on the scale-20 graph under a wall cap: a lower
+    bound on the plain path's speedup there."""
+    dsd = os.path.join(ROOT, "oracle", "_ref", "dsd")
+    if not os.path.exists(dsd):
+        return None
+    motif = os.path.join(tmpdir, "motif3.txt")
+    with open(motif, "w") as f:
+        f.write("3 1 3\n0 1\n0 2\n1 2\n")
+    path = os.path.join(tmpdir, "c2.txt")
+    graph.write(path)
+    wall, out, st = _run_reference([dsd, path, motif, "1", "1"], C2_CAP_S)
+    res = dict(cores=1, kind="reference", cap_s=C2_CAP_S, status=st)
+    if st == "timeout":
+        res["seconds"] = f"> {C2_CAP_S:g}"
+        res["speedup_lower_bound"] = C2_CAP_S * 1000.0 / gpu_ms
+    elif st == "ok":
+        res["seconds"] = wall
+        res["speedup"] = wall * 1000.0 / gpu_ms
+    return res
 
 
 def cpu_listing_rate(graph, k):
@@ -583,6 +659,8 @@ def main():
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as td:
             out["cpu_baseline"] = cpu_baseline(td, eng)
+            if args.workload == "c2":
+                out["cpu_c2_bound"] = cpu_c2_bound(td, g, out["ms_per_step"])
         if w["k"] == 3:
             out["cpu_listing"] = cpu_listing_rate(g, w["k"])
             # the oracle's count of the same graph must equal the engine's
