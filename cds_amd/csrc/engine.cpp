@@ -300,6 +300,14 @@ cds_status barrier_failed(const char *kernel, unsigned grid) {
               kernel, grid);
 }
 
+// Scale of the float Edmonds-Karp drift bound (float_ek_drift): over 1110
+// alpha steps of 147 seeded graphs (R-MAT scale 8 and Chung-Lu, k = 2..4),
+// |F_float - F_exact| stayed below 0.92 u (kT)^1.5, u = 2^-24, the size of
+// a random walk of about kT augmentations each rounding by about u kT; the
+// bound doubles that.  The worst case of float Edmonds-Karp is far larger,
+// so the flag is calibrated, not proven.
+constexpr double kFloatDriftScale = 2.0;
+
 // Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
 constexpr uint32_t kGridFlowRounds = 20000;
 // Incidence entries (k*T) from which the plain path peels in coarse steps.
@@ -636,6 +644,48 @@ class ExactSearch {
   std::vector<Probe> round_;
   double c_lo_ = 0, c_hi_ = 0;
 };
+
+// Bound on how far the reference's float Edmonds-Karp flow sum
+// (cpuCode/c++/src/helpers.cpp:1343-1405: float residuals and a float
+// running sum over the augmentations) strays from the exact flow on a
+// component with T cliques: kFloatDriftScale u (kT)^1.5, u = 2^-24.  The
+// decision there is |F - kT| < 1e-2f (helpers.cpp:1279-1290), so a step
+// whose exact margin lies within this bound of 1e-2 may go the other way
+// in the reference.
+double float_ek_drift(uint32_t k, uint64_t T) {
+  const double kt = (double)k * (double)T;
+  return kFloatDriftScale * std::ldexp(kt * std::sqrt(kt), -24);
+}
+
+// Copies the search trace into the result, with each step's drift bound
+// and fragility flag.
+template <class Comps>
+void fill_trace(cds_result *R, const ExactSearch &S, uint32_t k, const Comps &comps) {
+  const size_t tl = S.tr_alpha.size();
+  R->trace_len = (uint32_t)tl;
+  R->trace_lb = host_alloc<float>(tl);
+  R->trace_ub = host_alloc<float>(tl);
+  R->trace_alpha = host_alloc<float>(tl);
+  R->trace_component = host_alloc<uint32_t>(tl);
+  R->trace_margin = host_alloc<double>(tl);
+  R->trace_fragile = host_alloc<uint8_t>(tl);
+  R->trace_drift = host_alloc<double>(tl);
+  R->fragile_decisions = 0;
+  for (size_t i = 0; i < tl; ++i) {
+    R->trace_lb[i] = S.tr_lb[i];
+    R->trace_ub[i] = S.tr_ub[i];
+    R->trace_alpha[i] = S.tr_alpha[i];
+    R->trace_component[i] = S.tr_comp[i];
+    R->trace_margin[i] = S.tr_margin[i];
+    const double d = float_ek_drift(k, comps[S.tr_comp[i]].T);
+    R->trace_drift[i] = d;
+    // the float decision |F + e - kT| < 1e-2 with |e| <= d differs from
+    // the exact one (margin m = kT - F >= 0) only if |m - 1e-2| <= d
+    const bool fr = std::fabs(S.tr_margin[i] - (double)1e-2f) <= d;
+    R->trace_fragile[i] = fr ? 1 : 0;
+    R->fragile_decisions += fr;
+  }
+}
 
 }  // namespace
 
@@ -1912,20 +1962,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     CHK(readback(e, &sc, acc.p, 1));
     R->set_cliques = sc;
   }
-  const size_t tl = search.tr_alpha.size();
-  R->trace_len = (uint32_t)tl;
-  R->trace_lb = host_alloc<float>(tl);
-  R->trace_ub = host_alloc<float>(tl);
-  R->trace_alpha = host_alloc<float>(tl);
-  R->trace_component = host_alloc<uint32_t>(tl);
-  R->trace_margin = host_alloc<double>(tl);
-  for (size_t i = 0; i < tl; ++i) {
-    R->trace_lb[i] = search.tr_lb[i];
-    R->trace_ub[i] = search.tr_ub[i];
-    R->trace_alpha[i] = search.tr_alpha[i];
-    R->trace_component[i] = search.tr_comp[i];
-    R->trace_margin[i] = search.tr_margin[i];
-  }
+  fill_trace(R, search, k, comps);
   if (p->want_vertex_arrays) {
     R->clique_degree = host_alloc<uint64_t>(n);
     memcpy(R->clique_degree, cd.data(), n * 8);
@@ -2704,20 +2741,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     R->set_size = (uint32_t)verts[b].size();
     R->vertices = host_alloc<uint32_t>(verts[b].size());
     memcpy(R->vertices, verts[b].data(), verts[b].size() * 4);
-    const size_t tl = S.tr_alpha.size();
-    R->trace_len = (uint32_t)tl;
-    R->trace_lb = host_alloc<float>(tl);
-    R->trace_ub = host_alloc<float>(tl);
-    R->trace_alpha = host_alloc<float>(tl);
-    R->trace_component = host_alloc<uint32_t>(tl);
-    R->trace_margin = host_alloc<double>(tl);
-    for (size_t i = 0; i < tl; ++i) {
-      R->trace_lb[i] = S.tr_lb[i];
-      R->trace_ub[i] = S.tr_ub[i];
-      R->trace_alpha[i] = S.tr_alpha[i];
-      R->trace_component[i] = S.tr_comp[i];
-      R->trace_margin[i] = S.tr_margin[i];
-    }
+    fill_trace(R, S, k, gcomps[b]);
     if (ps[b]->want_vertex_arrays) {
       const uint32_t nb = gs[b]->n;
       R->clique_degree = host_alloc<uint64_t>(nb);

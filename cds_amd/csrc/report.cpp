@@ -145,6 +145,9 @@ std::string render(const cds_result *r) {
   j.u64("flow_clique_visits", r->flow_clique_visits);
   j.u64("flow_arc_visits", r->flow_arc_visits);
   j.u64("flow_relabels", r->flow_relabels);
+  // steps whose exact decision the reference's float Edmonds-Karp may take
+  // the other way (cds.h: trace_fragile)
+  j.u64("fragile_decisions", r->fragile_decisions);
   j.open("trace", '[');
   for (uint32_t i = 0; i < r->trace_len; ++i) {
     j.open(nullptr, '{');
@@ -153,9 +156,19 @@ std::string render(const cds_result *r) {
     j.f64("ub", r->trace_ub ? r->trace_ub[i] : NAN);
     j.f64("alpha", r->trace_alpha ? r->trace_alpha[i] : NAN);
     j.f64("margin", r->trace_margin ? r->trace_margin[i] : NAN);
+    j.f64("drift_bound", r->trace_drift ? r->trace_drift[i] : NAN);
+    j.u64("fragile", r->trace_fragile ? r->trace_fragile[i] : 0);
     j.close('}');
   }
   j.close(']');
+  j.close('}');
+  // widths the cooperative kernels ran at; shrinks: launches the runtime
+  // refused at the width asked and that ran narrower
+  j.open("cooperative", '{');
+  j.u64("peel_workgroups", r->peel_grid);
+  j.u64("flow_workgroups", r->flow_grid);
+  j.u64("replay_workgroups", r->replay_grid);
+  j.u64("shrinks", r->coop_shrinks);
   j.close('}');
   j.open("times_ms", '{');
   j.f64("upload", r->times.upload);

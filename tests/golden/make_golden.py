@@ -11,6 +11,11 @@ Outputs: reference_outputs.json = the reference's own pipeline
 (oracle/_ref/dsd_harness, i.e. cpuCode/c++/src/*.cpp unmodified) on each
 graph for k = 2, 3, 4: clique count and degrees, core numbers, densest core,
 pruning, components, bounds, final density, size and vertex set.
+
+reference_fragile.json = the same on float-fragile graphs (R-MAT scale 8,
+edge factor 16, seeds 7, 8 and 16, k = 3), where the reference's float
+Edmonds-Karp (cpuCode/c++/src/helpers.cpp:1343-1405) decides an alpha step
+the other way than exact arithmetic and prints another densest subgraph.
 """
 import json
 import os
@@ -47,7 +52,25 @@ def seeded_graphs():
     return out
 
 
+FRAGILE = [(7, 3), (8, 3), (16, 3)]
+
+
+def fragile_graphs():
+    return [(f"rmat8_{seed}", cds_amd.Graph.rmat(8, 16, seed=seed), k)
+            for seed, k in FRAGILE]
+
+
 def main():
+    fragile = {}
+    os.makedirs(os.path.join(HERE, "graphs"), exist_ok=True)
+    for name, g, k in fragile_graphs():
+        path = os.path.join(HERE, "graphs", name + ".txt")
+        g.write(path)
+        ref = reference_solve(path, k)
+        fragile[f"{name}/k{k}"] = {"graph": os.path.relpath(path, HERE), "k": k,
+                                   **{key: ref[key] for key in KEYS}}
+    with open(os.path.join(HERE, "reference_fragile.json"), "w") as f:
+        json.dump(fragile, f, indent=0, sort_keys=True, allow_nan=True)
     fixtures = {}
     files = [("ref_data_test", os.path.join(HERE, "ref_data_test.txt")),
              ("ref_log_graph", os.path.join(HERE, "ref_log_graph.txt")),

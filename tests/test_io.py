@@ -179,6 +179,12 @@ def test_result_json_renders_every_section():
     r.trace_alpha = ctypes.cast(al, ctypes.POINTER(ctypes.c_float))
     r.trace_component = ctypes.cast(comp, ctypes.POINTER(ctypes.c_uint32))
     r.trace_margin = ctypes.cast(mar, ctypes.POINTER(ctypes.c_double))
+    frag = (ctypes.c_uint8 * 2)(0, 1)
+    drift = (ctypes.c_double * 2)(0.5, 0.25)
+    r.trace_fragile = ctypes.cast(frag, ctypes.POINTER(ctypes.c_uint8))
+    r.trace_drift = ctypes.cast(drift, ctypes.POINTER(ctypes.c_double))
+    r.fragile_decisions = 1
+    r.peel_grid, r.flow_grid, r.replay_grid, r.coop_shrinks = 256, 128, 64, 1
     r.times.total = 12.5
     r.ub1_val = float("inf")
     text = cds_amd.result_json(r)
@@ -188,7 +194,11 @@ def test_result_json_renders_every_section():
     assert d["k"] == 3 and d["total_cliques"] == 7
     assert d["search"]["ub1"] is None
     assert d["search"]["trace"][1] == {"component": 0, "lb": 1.0, "ub": 1.5,
-                                       "alpha": 1.25, "margin": None}
+                                       "alpha": 1.25, "margin": None,
+                                       "drift_bound": 0.25, "fragile": 1}
+    assert d["search"]["fragile_decisions"] == 1
+    assert d["cooperative"] == {"peel_workgroups": 256, "flow_workgroups": 128,
+                                "replay_workgroups": 64, "shrinks": 1}
     assert d["times_ms"]["total"] == 12.5
     # snprintf convention: a short buffer is cut and NUL-terminated
     buf = ctypes.create_string_buffer(10)

@@ -243,3 +243,72 @@ def test_exact_flow_matches_float_flow_when_not_fragile(seed):
             i = n_same
             assert fl["trace_alpha"][i] == ex["trace_alpha"][i]
             assert abs(fl["trace_margin"][i] - ex["trace_margin"][i]) > 1e-3
+
+
+# ---- float fragility -------------------------------------------------------
+
+with open(os.path.join(GOLDEN, "reference_fragile.json")) as f:
+    FRAGILE = json.load(f)
+
+
+def float_drift_bound(k, T):
+    """The engine's bound on |F_float - F_exact| for a component of T
+    k-cliques (engine.cpp float_ek_drift): 2 * 2^-24 * (kT)^1.5."""
+    return 2.0 * 2.0 ** -24 * (k * T) ** 1.5
+
+
+def first_divergence(fl, ex):
+    """Index of the first alpha step the float and exact searches decide
+    differently (same alpha and component up to there), or None."""
+    for i, (a, b) in enumerate(zip(fl["trace_alpha"], ex["trace_alpha"])):
+        if a != b or fl["trace_component"][i] != ex["trace_component"][i]:
+            return None
+        if fl["trace_equal"][i] != ex["trace_equal"][i]:
+            return i
+    return None
+
+
+@pytest.mark.parametrize("key", sorted(FRAGILE))
+def test_fragile_fixtures(key):
+    """On these graphs the reference (built from its sources) prints another
+    densest subgraph than exact arithmetic gives: the oracle's float mode
+    reproduces the reference's answer, the exact mode differs, and the step
+    where they part has exact margin 0 and a float drift the bound covers."""
+    fx = FRAGILE[key]
+    g = cds_amd.Graph.load(os.path.join(GOLDEN, fx["graph"]))
+    fl = O.oracle_solve(g, fx["k"], flow=O.FLOW_FLOAT_EK)
+    ex = O.oracle_solve(g, fx["k"], flow=O.FLOW_EXACT)
+    assert not [f for f in FIELDS if not same(fl[f], fx[f])]
+    assert (ex["density"], ex["vertices"]) != (fx["density"], fx["vertices"])
+    i = first_divergence(fl, ex)
+    assert i is not None
+    T = ex["component_cliques"][ex["trace_component"][i]]
+    drift = abs(fl["trace_margin"][i] - ex["trace_margin"][i])
+    assert ex["trace_margin"][i] == 0 and drift >= 0.01
+    assert drift <= float_drift_bound(fx["k"], T)
+    assert abs(ex["trace_margin"][i] - 0.01) <= float_drift_bound(fx["k"], T)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_float_drift_bound_on_fresh_graphs(seed):
+    """The drift bound, calibrated on other seeds, covers every alpha step
+    both searches take on fresh graphs, so every step where the reference's
+    float search would part from the exact one is flagged."""
+    specs = [cds_amd.Graph.rmat(8, 16, seed=200 + seed)]
+    rng = np.random.default_rng(seed + 4000)
+    specs.append(cds_amd.Graph.chung_lu(int(rng.integers(60, 300)),
+                                        float(rng.uniform(2.1, 3.0)),
+                                        float(rng.uniform(4, 14)),
+                                        int(rng.integers(8, 35)), 0.85, seed=seed + 4000))
+    for g, k in ((specs[0], 3), (specs[1], 3), (specs[1], 4)):
+        fl = O.oracle_solve(g, k, flow=O.FLOW_FLOAT_EK)
+        ex = O.oracle_solve(g, k, flow=O.FLOW_EXACT)
+        for i, (a, b) in enumerate(zip(fl["trace_alpha"], ex["trace_alpha"])):
+            if a != b or fl["trace_component"][i] != ex["trace_component"][i]:
+                break
+            T = ex["component_cliques"][ex["trace_component"][i]]
+            d = float_drift_bound(k, T)
+            assert abs(fl["trace_margin"][i] - ex["trace_margin"][i]) <= d
+            if fl["trace_equal"][i] != ex["trace_equal"][i]:
+                assert abs(ex["trace_margin"][i] - 0.01) <= d
+                break
