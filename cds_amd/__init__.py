@@ -107,7 +107,9 @@ class CResult(ctypes.Structure):
         ("flow_grid", ctypes.c_uint32), ("replay_grid", ctypes.c_uint32),
         ("coop_shrinks", ctypes.c_uint32), ("fragile_decisions", ctypes.c_uint32),
         ("trace_fragile", ctypes.POINTER(ctypes.c_uint8)),
-        ("trace_drift", ctypes.POINTER(ctypes.c_double))]
+        ("trace_drift", ctypes.POINTER(ctypes.c_double)),
+        ("prune_tau", ctypes.c_uint64), ("prune_vertices", ctypes.c_uint32),
+        ("prune_passes", ctypes.c_uint32), ("dense_peel", ctypes.c_uint32)]
 
 
 class COpts(ctypes.Structure):

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libcds_amd.so")
 BIN = os.path.join(HERE, "bin", "cds_main")
 
 HIP_SOURCES = ["scan.hip", "dag.hip", "list.hip", "peel.hip", "glue.hip", "flow.hip",
-               "flow_grid.hip", "batch.hip", "noalloc.hip"]
+               "flow_grid.hip", "batch.hip", "noalloc.hip", "prune.hip"]
 HOST_SOURCES = ["graph_io.cpp", "engine.cpp", "server.cpp", "report.cpp", "tune.cpp"]
 ARCH = os.environ.get("CDS_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

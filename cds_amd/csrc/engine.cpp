@@ -941,6 +941,8 @@ cds_status run_dag(cds_engine *e, const cds_dgraph *g, Listing &L) {
   return CDS_OK;
 }
 
+cds_status run_listing_write(cds_engine *e, uint32_t k, Listing &L, Timer *tm);
+
 cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
                        bool materialise, Listing &L, Timer *tm) {
   DeviceArena &A = *e->arena;
@@ -983,19 +985,24 @@ cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
   HIPCHK(scan_u32(L.item_count.p, L.item_base.p, L.items_n, A));
   CHK(readback(e, &L.T, L.item_base.p + L.items_n, 1));
   L.t_count = tm->ms(t0, t1);
-  if (materialise) {
-    if (L.T >= 0xFFFFFFFFull || L.T * k >= (1ull << 40))
-      return fail(CDS_ERR_RANGE, "%llu cliques exceed the 32-bit clique ids",
-                  (unsigned long long)L.T);
-    HIPCHK(L.cliques.alloc(L.T * k));
-    int t2 = tm->mark();
-    HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
-                        nullptr, nullptr, L.item_base.p, L.cliques.p,
-                        e->stream));
-    int t3 = tm->mark();
-    HIPCHK(hipStreamSynchronize(e->stream));
-    L.t_write = tm->ms(t2, t3);
-  }
+  if (materialise) CHK(run_listing_write(e, k, L, tm));
+  return CDS_OK;
+}
+
+// The write pass of a listing counted by run_listing (k >= 3): every clique,
+// k vertex ids each, at its item's offset.
+cds_status run_listing_write(cds_engine *e, uint32_t k, Listing &L, Timer *tm) {
+  if (L.T >= 0xFFFFFFFFull || L.T * k >= (1ull << 40))
+    return fail(CDS_ERR_RANGE, "%llu cliques exceed the 32-bit clique ids",
+                (unsigned long long)L.T);
+  const ListShape shape = list_shape(e);
+  HIPCHK(L.cliques.alloc(L.T * k));
+  int t2 = tm->mark();
+  HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+                      nullptr, nullptr, L.item_base.p, L.cliques.p, e->stream));
+  int t3 = tm->mark();
+  HIPCHK(hipStreamSynchronize(e->stream));
+  L.t_write = tm->ms(t2, t3);
   return CDS_OK;
 }
 
@@ -1228,8 +1235,101 @@ extern "C" cds_status cds_count_cliques_noalloc(cds_engine *e, const cds_dgraph 
 
 namespace {
 
+// Peel stage of a small dense graph (n <= kDenseMaxVertices): every row of
+// the oracle's sequential peel (cpuCode/c++/src/helpers.cpp:16-150) from
+// the bitset kernels of prune.hip, so k' is exact without replays.  Fills
+// the exact level table, its summary (rho and k' over all rows), core[]
+// and the kill level of every listed clique.
+cds_status dense_peel_stage(cds_engine *e, const cds_dgraph *g, const Listing &L, uint32_t k,
+                            cds_result *R, DBuf<uint32_t> *core, DBuf<uint32_t> *kill,
+                            std::vector<Level> *levels, PeelSummary *PS) {
+  DeviceArena &A = *e->arena;
+  hipStream_t st = e->stream;
+  const uint32_t n = g->n;
+  const uint64_t T = L.T;
+  if (n > kDenseMaxVertices)
+    return fail(CDS_ERR_RANGE, "dense peel of %u vertices (at most %u)", n, kDenseMaxVertices);
+  const uint32_t W = (n + 63) / 64;
+  DBuf<unsigned long long> bits(&A), alive(&A), cur(&A), rval(&A);
+  DBuf<uint32_t> stepv(&A), rv(&A);
+  HIPCHK(bits.alloc((size_t)n * W));
+  HIPCHK(alive.alloc(W));
+  HIPCHK(cur.alloc(n));
+  HIPCHK(rval.alloc(n));
+  HIPCHK(stepv.alloc(1));
+  HIPCHK(rv.alloc(n));
+  const auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(dense_peel(n, k, g->off, g->adj, L.cd.p, bits.p, alive.p, cur.p, stepv.p, rv.p,
+                    rval.p, st));
+  std::vector<uint32_t> hrv(n);
+  std::vector<unsigned long long> hval(n);
+  CHK(readback(e, hrv.data(), rv.p, n));
+  CHK(readback(e, hval.data(), rval.p, n));
+  R->t_peel_kernel_ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count();
+  // rows 0..n-1 of the oracle: T_rem / (n - i) after i removals
+  std::vector<uint32_t> hcore(n, kAliveHost);
+  uint64_t run = 0, trem = T;
+  double rho = (double)(int64_t)T / static_cast<double>(n);
+  levels->clear();
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t v = hrv[i];
+    const uint64_t val = hval[i];
+    if (v >= n || hcore[v] != kAliveHost || val > trem)
+      return fail(CDS_ERR_INTERNAL, "dense peel row %u: vertex %u, %llu of %llu cliques", i,
+                  v, (unsigned long long)val, (unsigned long long)trem);
+    if (val > 0xFFFFFF) R->oracle_quirk_flags |= 1;  // the oracle's sentinel regime
+    run = std::max<uint64_t>(run, val);
+    hcore[v] = (uint32_t)run;
+    if (levels->empty() || levels->back().level != run) levels->push_back({(uint32_t)run, 0, 0});
+    levels->back().removed++;
+    levels->back().killed += val;
+    trem -= val;
+    if (i + 1 < n) rho = std::max(rho, (double)(int64_t)trem / static_cast<double>(n - i - 1));
+  }
+  if (trem != 0)
+    return fail(CDS_ERR_INTERNAL, "dense peel left %llu cliques", (unsigned long long)trem);
+  CHK(summarize_levels(*levels, n, T, PS));
+  PS->rho = std::max(PS->rho, rho);
+  PS->kp = (int64_t)std::ceil(PS->rho);
+  PS->replay.clear();
+  PS->refine.clear();
+  HIPCHK(core->alloc(n));
+  CHK(upload(e, core->p, hcore.data(), n));
+  HIPCHK(kill->alloc(T));
+  // every clique alive (0xFFFFFFFF), then its level from its members' cores
+  HIPCHK(hipMemsetAsync(kill->p, 0xFF, T * sizeof(uint32_t), st));
+  HIPCHK(peel_derive_kill(T, k, L.cliques.p, core->p, kill->p, st));
+  HIPCHK(hipStreamSynchronize(st));  // hcore is read by the copy above
+  R->peel_levels = n;
+  R->peel_launches = 0;
+  return CDS_OK;
+}
+
+// How solve_impl treats its graph: the whole input, or the clique core the
+// pruned path cut it to (peeled with the persistent peel or, for a dense
+// core, on bitsets).  Only the whole input may be pruned.
+enum SolveMode { kSolveInput, kSolveCore, kSolveDenseCore };
+
+cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
+                        cds_result *R, bool core_only, Listing &L0);
+
+// Whether a graph with T k-cliques is solved on its clique core instead of
+// whole: its clique slots pass 32 bits, or its cliques and their
+// incidence would not fit the device (CDS_CORE_PRUNE=1/0 forces it).
+bool needs_prune(uint32_t k, uint64_t T) {
+  if (const char *f = getenv("CDS_CORE_PRUNE")) return atoi(f) != 0;
+  if ((unsigned __int128)k * T >= 0xFFFFFFFFull) return true;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  const unsigned __int128 need = (unsigned __int128)T * (4ull * k + 4ull * k * k + 16);
+  return need > (unsigned __int128)free_b / 10 * 8;
+}
+
 cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
-                      cds_result *R, bool core_only) {
+                      cds_result *R, bool core_only, SolveMode mode = kSolveInput) {
+  const bool dense = mode == kSolveDenseCore;
   const auto wall0 = std::chrono::steady_clock::now();
   DeviceArena &A = *e->arena;
   hipStream_t st = e->stream;
@@ -1248,7 +1348,12 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   Listing L(&A);
   CHK(run_dag(e, g, L));
   const int ev_dag = tm.mark();
-  CHK(run_listing(e, g, k, true, L, &tm));
+  CHK(run_listing(e, g, k, k == 2, L, &tm));
+  if (k > 2) {
+    if (mode == kSolveInput && needs_prune(k, L.T))
+      return solve_pruned(e, g, p, R, core_only, L);
+    CHK(run_listing_write(e, k, L, &tm));
+  }
   const int ev_list = tm.mark();
   const uint64_t T = L.T;
   R->total_cliques = T;
@@ -1273,307 +1378,313 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   if (sum_cd >= (1ull << 31)) R->oracle_quirk_flags |= 4;  // int total wraps
 
   // ---- (b) peel ----
-  DBuf<uint64_t> ioff(&A);
-  DBuf<uint32_t> icur(&A), inc(&A), deg(&A), core(&A), kill(&A), fa(&A),
-      fb(&A), cnt(&A);
-  DBuf<unsigned long long> killed(&A);
-  DBuf<uint32_t> sdeg(&A);  // degree at the start of the removing level
-  HIPCHK(ioff.alloc((size_t)n + 1));
-  HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
-  HIPCHK(icur.alloc(n));
-  HIPCHK(icur.zero());
-  HIPCHK(inc.alloc(T * k * k));  // entries: clique id + other k-1 members
-  const int ev_inc0 = tm.mark();
-  HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
-  const int ev_inc1 = tm.mark();
-  icur.reset();
-  HIPCHK(deg.alloc(n));
-  HIPCHK(core.alloc(n));
-  HIPCHK(kill.alloc(T));
-  HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
-  uint32_t *hp = (uint32_t *)e->pinned;
+  DBuf<uint32_t> core(&A), kill(&A);
   std::vector<Level> levels;
-  uint64_t removed_total = 0;
-  uint32_t peel_rounds = 0;
-  // persistent-peel state, kept for the exact re-peels of coarse steps
-  DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A), ralive(&A), rcount(&A);
-  DBuf<uint64_t> s64(&A);
-  DBuf<uint32_t> stamp(&A);
-  uint32_t sbase = 1;  // first sub-round of the next persistent launch
-  DBuf<unsigned long long> lrem(&A), lkil(&A);
-  DBuf<PeelCtl> ctl(&A);
-  // CDS_PEEL_COARSE / CDS_PEEL_COARSE_SHIFT (tests, diagnostics) win over
-  // the tuning, which wins over the rule (coarse steps of [m, 1.5 m] from
-  // kCoarseMinSlots incidence entries)
-  bool coarse_peel = (uint64_t)k * T >= kCoarseMinSlots;
-  uint32_t shift = 1;
-  if (e->cur.peel_coarse_shift >= 0) {
-    coarse_peel = e->cur.peel_coarse_shift > 0;
-    shift = coarse_peel ? (uint32_t)e->cur.peel_coarse_shift : 1;
-  }
-  if (e->peel_coarse != -1) coarse_peel = e->peel_coarse == 1;
-  if (e->peel_shift_env) shift = e->peel_shift;
-  const uint32_t cshift = (n < (1u << 24) && coarse_peel) ? shift : 0;
-  R->peel_coarse_shift = cshift;
-  int wall_khz = 0;
-  if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) !=
-      hipSuccess)
-    wall_khz = 0;
-  // One persistent launch over `alive0` (null: every vertex) appending its
-  // steps to `out`; returns the launch's control block.
-  // every launch is timed with the spare event pair; R->t_peel_kernel_ms
-  // sums them
-  auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift,
-                      uint32_t max_level, unsigned grid, uint64_t want_removed,
-                      std::vector<Level> *out, PeelCtl *hc) -> cds_status {
-    unsigned launched = grid;
-    {
-      CoopGate gate(e, grid);
-      HIPCHK(hipEventRecord(tm.ev[22], st));
-      HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
-                             stamp.p, sbase, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
-                             lkil.p, sdeg.p,
-                             alive0, na0, shift, max_level, &launched, st));
-      HIPCHK(hipEventRecord(tm.ev[23], st));
-    }
-    note_width(R, &R->peel_grid, grid, launched);
-    CHK(readback(e, hc, ctl.p, 1));
-    if (hc->bar.abort) return barrier_failed("persistent peel", launched);
-    HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
-    R->t_peel_kernel_ms += tm.ms(22, 23);
-    R->peel_launches++;
-    if (hc->rounds >= (1ull << 30) || sbase + hc->rounds + 1 >= (1u << 30))
-      return fail(CDS_ERR_INTERNAL, "persistent peel: %llu sub-rounds", hc->rounds);
-    sbase += (uint32_t)hc->rounds + 1;
-    if (hc->removed != want_removed || hc->nlevels == 0 || hc->nlevels > n)
-      return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %llu (%u levels)",
-                  hc->removed, (unsigned long long)want_removed, hc->nlevels);
-    std::vector<uint32_t> hl(hc->nlevels), hm(hc->nlevels);
-    std::vector<unsigned long long> hr(hc->nlevels), hk(hc->nlevels);
-    CHK(readback(e, hl.data(), lvl.p, hc->nlevels));
-    CHK(readback(e, hm.data(), lmin.p, hc->nlevels));
-    CHK(readback(e, hr.data(), lrem.p, hc->nlevels));
-    CHK(readback(e, hk.data(), lkil.p, hc->nlevels));
-    for (uint32_t i = 0; i < hc->nlevels; ++i) {
-      Level lv{hl[i], hr[i], hk[i]};
-      if (hm[i] < hl[i]) lv.lo = hm[i];
-      out->push_back(lv);
-    }
-    peel_rounds += (uint32_t)hc->rounds;
-    if (wall_khz > 0) {
-      R->peel_split_ms[0] += (double)hc->t_bar / wall_khz;
-      R->peel_split_ms[1] += (double)hc->t_scan / wall_khz;
-      R->peel_split_ms[2] += (double)hc->t_proc / wall_khz;
-      R->peel_split_ms[3] += (double)hc->t_flush / wall_khz;
-    }
-    return CDS_OK;
-  };
-  if (n < (1u << 24)) {
-    // every level (or coarse step) in one cooperative launch
-    HIPCHK(s32.alloc(6ull * n));
-    HIPCHK(s64.alloc(3ull * n));
-    HIPCHK(lvl.alloc(n));
-    HIPCHK(lmin.alloc(n));
-    HIPCHK(lrem.alloc(n));
-    HIPCHK(lkil.alloc(n));
-    HIPCHK(ctl.alloc(1));
-    HIPCHK(sdeg.alloc(n));
-    HIPCHK(stamp.alloc(n));
-    const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
-    if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
-    PeelCtl hc;
-    CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
-    removed_total = n;
-  }
-  HIPCHK(fa.alloc(n));
-  HIPCHK(fb.alloc(n));
-  HIPCHK(cnt.alloc(4));  // [0] min, [1] count A, [2] count B
-  HIPCHK(killed.alloc(1));
-  while (removed_total < n) {
-    HIPCHK(peel_min_alive(n, deg.p, core.p, cnt.p, st));
-    HIPCHK(hipMemsetAsync(cnt.p + 1, 0, 8, st));
-    HIPCHK(killed.zero());
-    CHK(readback(e, hp, cnt.p, 1));
-    const uint32_t level = hp[0];
-    if (level == kAliveHost)
-      return fail(CDS_ERR_INTERNAL, "peel: no alive vertex but %llu removed",
-                  (unsigned long long)removed_total);
-    HIPCHK(peel_frontier(n, level, deg.p, core.p, fa.p, cnt.p + 1, st));
-    CHK(readback(e, hp + 1, cnt.p + 1, 1));
-    uint32_t f = hp[1];
-    uint64_t removed = 0;
-    uint32_t *cur = fa.p, *nxt = fb.p, *ccur = cnt.p + 1, *cnxt = cnt.p + 2;
-    while (f > 0) {
-      removed += f;
-      HIPCHK(hipMemsetAsync(cnxt, 0, 4, st));
-      HIPCHK(peel_process(k, level, cur, ccur, f, ioff.p, inc.p, L.cliques.p,
-                          deg.p, core.p, kill.p, nxt, cnxt, killed.p, st));
-      CHK(readback(e, hp + 2, cnxt, 1));
-      f = hp[2];
-      std::swap(cur, nxt);
-      std::swap(ccur, cnxt);
-      ++peel_rounds;
-    }
-    unsigned long long kl = 0;
-    CHK(readback(e, &kl, killed.p, 1));
-    levels.push_back({level, removed, kl});
-    removed_total += removed;
-  }
-  fa.reset();
-  fb.reset();
-  R->t_incidence_ms = tm.ms(ev_inc0, ev_inc1);
-
-  // Exact re-peel of coarse step i: its vertices and cliques become alive
-  // again (degrees from sdeg), later steps are parked, and the step's
-  // entry is replaced by its exact levels.
-  auto refine_step = [&](size_t i) -> cds_status {
-    const Level lv = levels[i];
-    if (!ralive.p) {
-      HIPCHK(ralive.alloc(n));
-      HIPCHK(rcount.alloc(1));
-    }
-    HIPCHK(refine_prepare(n, T, lv.level, core.p, deg.p, sdeg.p, kill.p, ralive.p,
-                          rcount.p, st));
-    uint32_t na = 0;
-    CHK(readback(e, &na, rcount.p, 1));
-    if (na != lv.removed)
-      return fail(CDS_ERR_INTERNAL, "coarse step %u holds %u of %llu vertices", lv.level,
-                  na, (unsigned long long)lv.removed);
-    // as wide as the first pass: the re-peel is bound by its cliques'
-    // kills (measured 127 ms against 157 ms with a grid sized by the
-    // step's own clique count on R-MAT scale 20, k = 3)
-    const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
-    if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
-    // Exact levels at once: re-peeling a step in narrower steps first and
-    // refining those re-processes the step's cliques at every depth, which
-    // measured slower on R-MAT scale 20 k=3 and scale 17 k=4.
-    std::vector<Level> exact;
-    PeelCtl hc;
-    CHK(run_peel(ralive.p, na, 0, lv.level, grid, na, &exact, &hc));
-    HIPCHK(refine_restore(n, core.p, deg.p, st));
-    uint64_t kl = 0;
-    for (const Level &x : exact) {
-      kl += x.killed;
-      if (x.level < lv.lo || x.level > lv.level)
-        return fail(CDS_ERR_INTERNAL, "coarse step [%u, %u] re-peeled to level %u", lv.lo,
-                    lv.level, x.level);
-    }
-    if (kl != lv.killed)
-      return fail(CDS_ERR_INTERNAL, "coarse step %u killed %llu of %llu cliques", lv.level,
-                  (unsigned long long)kl, (unsigned long long)lv.killed);
-    levels.erase(levels.begin() + (ptrdiff_t)i);
-    levels.insert(levels.begin() + (ptrdiff_t)i, exact.begin(), exact.end());
-    R->peel_refined_steps++;
-    return CDS_OK;
-  };
-
-  // Coarse steps are refined and exact levels replayed until neither list
-  // has work: each changes the table or rho, so the summary is recomputed.
   PeelSummary PS;
-  std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
-  const bool all_cores = p->want_vertex_arrays != 0;
-  auto summarize = [&]() -> cds_status {
-    double rho_r = 0;
-    for (const auto &r : replayed)
-      if (r.second.found)
-        rho_r = std::max(rho_r, (double)(int64_t)r.second.best_num /
-                                    static_cast<double>(r.second.best_den));
-    return summarize_levels(levels, n, T, &PS, all_cores, rho_r);
-  };
-  CHK(summarize());
-  {
-    DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
-    DBuf<uint64_t> pioff(&A);
-    DBuf<uint8_t> palive(&A);
-    DBuf<ReplayOut> rout(&A);
-    bool allocated = false;
-    double refine_ms = 0;
-    const int ev_r0 = tm.mark();
-    for (;;) {
-      if (!PS.refine.empty()) {
-        const auto t0 = std::chrono::steady_clock::now();
-        CHK(refine_step(PS.refine.front()));
-        HIPCHK(hipStreamSynchronize(st));
-        refine_ms += std::chrono::duration<double, std::milli>(
-                         std::chrono::steady_clock::now() - t0)
-                         .count();
-        CHK(summarize());
-        continue;
-      }
-      size_t ii = SIZE_MAX;
-      for (size_t cand : PS.replay) {
-        bool done = false;
-        for (const auto &r : replayed) done |= r.first == levels[cand].level;
-        if (!done) {
-          ii = cand;
-          break;
-        }
-      }
-      if (ii == SIZE_MAX) break;
-      const Level &lv = levels[ii];
-      const uint64_t M = lv.removed;
-      if (!allocated) {
-        HIPCHK(pv.alloc(n));
-        HIPCHK(pdeg.alloc(n));
-        HIPCHK(posmap.alloc(n));
-        HIPCHK(palive.alloc(n));
-        HIPCHK(rdead.alloc(T));
-        HIPCHK(rdead.zero());
-        HIPCHK(pc.alloc(1));
-        HIPCHK(pioff.alloc((size_t)n + 1));
-        HIPCHK(rout.alloc(1));
-        allocated = true;
-      }
-      DBuf<uint32_t> pinc(&A), partial(&A);
-      DBuf<GridBarrier> gbar(&A);
-      // Few vertices with many cliques each: replay on the whole GPU.
-      unsigned rgrid = 0;
-      if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
-        rgrid = coop_width(e, replay_grid_blocks(e->device, lv.killed * k / M,
-                                                 e->cur.replay_blocks));
-      if (rgrid) {
-        HIPCHK(partial.alloc((size_t)rgrid * M));
-        HIPCHK(gbar.alloc(1));
-        R->replay_grid_phases++;
-      } else {
-        HIPCHK(pinc.alloc(lv.killed * k * k));
-      }
-      std::optional<CoopGate> gate;
-      if (rgrid) gate.emplace(e, rgrid);
-      HIPCHK(hipEventRecord(tm.ev[22], st));
-      const unsigned rasked = rgrid;
-      HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
-                          T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
-                          pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
-                          pioff.p, pinc.p, rout.p, partial.p, gbar.p, &rgrid,
-                          sdeg.p, A, tm.ev[23], (uint64_t)PS.kp,
-                          replay_nstar(k, (uint64_t)PS.kp), st));
-      gate.reset();
-      if (rasked) note_width(R, &R->replay_grid, rasked, rgrid);
-      ReplayOut ro;
-      CHK(readback(e, &ro, rout.p, 1));
-      if (rgrid > 1) {
-        unsigned ab = 0;
-        CHK(readback(e, &ab, &gbar.p->abort, 1));
-        if (ab) return barrier_failed("whole-GPU replay", rgrid);
-      }
-      {
-        float prep = 0;
-        (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
-        R->t_replay_prep_ms += prep;
-      }
-      R->replay_phases++;
-      if (ro.steps > M || (!ro.stopped && ro.steps != M))
-        return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
-                    lv.level, ro.steps, (unsigned long long)M);
-      replayed.push_back({lv.level, ro});
-      // a replay can raise k': the refine list (steps straddling k') and
-      // the replay candidates are recomputed against the new value
-      CHK(summarize());
+  uint32_t peel_rounds = 0;
+  if (dense) {
+    // a dense core: the exact sequential peel on adjacency bitsets
+    CHK(dense_peel_stage(e, g, L, k, R, &core, &kill, &levels, &PS));
+  } else {
+    DBuf<uint64_t> ioff(&A);
+    DBuf<uint32_t> icur(&A), inc(&A), deg(&A), fa(&A),
+        fb(&A), cnt(&A);
+    DBuf<unsigned long long> killed(&A);
+    DBuf<uint32_t> sdeg(&A);  // degree at the start of the removing level
+    HIPCHK(ioff.alloc((size_t)n + 1));
+    HIPCHK(scan_u64(L.cd.p, ioff.p, n, A));
+    HIPCHK(icur.alloc(n));
+    HIPCHK(icur.zero());
+    HIPCHK(inc.alloc(T * k * k));  // entries: clique id + other k-1 members
+    const int ev_inc0 = tm.mark();
+    HIPCHK(build_incidence(T, k, L.cliques.p, ioff.p, icur.p, inc.p, st));
+    const int ev_inc1 = tm.mark();
+    icur.reset();
+    HIPCHK(deg.alloc(n));
+    HIPCHK(core.alloc(n));
+    HIPCHK(kill.alloc(T));
+    HIPCHK(peel_init(n, T, L.cd.p, deg.p, core.p, kill.p, st));
+    uint32_t *hp = (uint32_t *)e->pinned;
+    uint64_t removed_total = 0;
+    // persistent-peel state, kept for the exact re-peels of coarse steps
+    DBuf<uint32_t> s32(&A), lvl(&A), lmin(&A), ralive(&A), rcount(&A);
+    DBuf<uint64_t> s64(&A);
+    DBuf<uint32_t> stamp(&A);
+    uint32_t sbase = 1;  // first sub-round of the next persistent launch
+    DBuf<unsigned long long> lrem(&A), lkil(&A);
+    DBuf<PeelCtl> ctl(&A);
+    // CDS_PEEL_COARSE / CDS_PEEL_COARSE_SHIFT (tests, diagnostics) win over
+    // the tuning, which wins over the rule (coarse steps of [m, 1.5 m] from
+    // kCoarseMinSlots incidence entries)
+    bool coarse_peel = (uint64_t)k * T >= kCoarseMinSlots;
+    uint32_t shift = 1;
+    if (e->cur.peel_coarse_shift >= 0) {
+      coarse_peel = e->cur.peel_coarse_shift > 0;
+      shift = coarse_peel ? (uint32_t)e->cur.peel_coarse_shift : 1;
     }
-    const int ev_r1 = tm.mark();
-    HIPCHK(hipStreamSynchronize(st));
-    R->t_replay_ms = std::max(0.0, tm.ms(ev_r0, ev_r1) - refine_ms);
+    if (e->peel_coarse != -1) coarse_peel = e->peel_coarse == 1;
+    if (e->peel_shift_env) shift = e->peel_shift;
+    const uint32_t cshift = (n < (1u << 24) && coarse_peel) ? shift : 0;
+    R->peel_coarse_shift = cshift;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) !=
+        hipSuccess)
+      wall_khz = 0;
+    // One persistent launch over `alive0` (null: every vertex) appending its
+    // steps to `out`; returns the launch's control block.
+    // every launch is timed with the spare event pair; R->t_peel_kernel_ms
+    // sums them
+    auto run_peel = [&](const uint32_t *alive0, uint32_t na0, uint32_t shift,
+                        uint32_t max_level, unsigned grid, uint64_t want_removed,
+                        std::vector<Level> *out, PeelCtl *hc) -> cds_status {
+      unsigned launched = grid;
+      {
+        CoopGate gate(e, grid);
+        HIPCHK(hipEventRecord(tm.ev[22], st));
+        HIPCHK(peel_persistent(n, k, T, ioff.p, inc.p, L.cliques.p, deg.p, core.p, kill.p,
+                               stamp.p, sbase, s32.p, s64.p, ctl.p, lvl.p, lmin.p, lrem.p,
+                               lkil.p, sdeg.p,
+                               alive0, na0, shift, max_level, &launched, st));
+        HIPCHK(hipEventRecord(tm.ev[23], st));
+      }
+      note_width(R, &R->peel_grid, grid, launched);
+      CHK(readback(e, hc, ctl.p, 1));
+      if (hc->bar.abort) return barrier_failed("persistent peel", launched);
+      HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
+      R->t_peel_kernel_ms += tm.ms(22, 23);
+      R->peel_launches++;
+      if (hc->rounds >= (1ull << 30) || sbase + hc->rounds + 1 >= (1u << 30))
+        return fail(CDS_ERR_INTERNAL, "persistent peel: %llu sub-rounds", hc->rounds);
+      sbase += (uint32_t)hc->rounds + 1;
+      if (hc->removed != want_removed || hc->nlevels == 0 || hc->nlevels > n)
+        return fail(CDS_ERR_INTERNAL, "persistent peel removed %llu of %llu (%u levels)",
+                    hc->removed, (unsigned long long)want_removed, hc->nlevels);
+      std::vector<uint32_t> hl(hc->nlevels), hm(hc->nlevels);
+      std::vector<unsigned long long> hr(hc->nlevels), hk(hc->nlevels);
+      CHK(readback(e, hl.data(), lvl.p, hc->nlevels));
+      CHK(readback(e, hm.data(), lmin.p, hc->nlevels));
+      CHK(readback(e, hr.data(), lrem.p, hc->nlevels));
+      CHK(readback(e, hk.data(), lkil.p, hc->nlevels));
+      for (uint32_t i = 0; i < hc->nlevels; ++i) {
+        Level lv{hl[i], hr[i], hk[i]};
+        if (hm[i] < hl[i]) lv.lo = hm[i];
+        out->push_back(lv);
+      }
+      peel_rounds += (uint32_t)hc->rounds;
+      if (wall_khz > 0) {
+        R->peel_split_ms[0] += (double)hc->t_bar / wall_khz;
+        R->peel_split_ms[1] += (double)hc->t_scan / wall_khz;
+        R->peel_split_ms[2] += (double)hc->t_proc / wall_khz;
+        R->peel_split_ms[3] += (double)hc->t_flush / wall_khz;
+      }
+      return CDS_OK;
+    };
+    if (n < (1u << 24)) {
+      // every level (or coarse step) in one cooperative launch
+      HIPCHK(s32.alloc(6ull * n));
+      HIPCHK(s64.alloc(3ull * n));
+      HIPCHK(lvl.alloc(n));
+      HIPCHK(lmin.alloc(n));
+      HIPCHK(lrem.alloc(n));
+      HIPCHK(lkil.alloc(n));
+      HIPCHK(ctl.alloc(1));
+      HIPCHK(sdeg.alloc(n));
+      HIPCHK(stamp.alloc(n));
+      const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
+      if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
+      PeelCtl hc;
+      CHK(run_peel(nullptr, 0, cshift, kAliveHost, grid, n, &levels, &hc));
+      removed_total = n;
+    }
+    HIPCHK(fa.alloc(n));
+    HIPCHK(fb.alloc(n));
+    HIPCHK(cnt.alloc(4));  // [0] min, [1] count A, [2] count B
+    HIPCHK(killed.alloc(1));
+    while (removed_total < n) {
+      HIPCHK(peel_min_alive(n, deg.p, core.p, cnt.p, st));
+      HIPCHK(hipMemsetAsync(cnt.p + 1, 0, 8, st));
+      HIPCHK(killed.zero());
+      CHK(readback(e, hp, cnt.p, 1));
+      const uint32_t level = hp[0];
+      if (level == kAliveHost)
+        return fail(CDS_ERR_INTERNAL, "peel: no alive vertex but %llu removed",
+                    (unsigned long long)removed_total);
+      HIPCHK(peel_frontier(n, level, deg.p, core.p, fa.p, cnt.p + 1, st));
+      CHK(readback(e, hp + 1, cnt.p + 1, 1));
+      uint32_t f = hp[1];
+      uint64_t removed = 0;
+      uint32_t *cur = fa.p, *nxt = fb.p, *ccur = cnt.p + 1, *cnxt = cnt.p + 2;
+      while (f > 0) {
+        removed += f;
+        HIPCHK(hipMemsetAsync(cnxt, 0, 4, st));
+        HIPCHK(peel_process(k, level, cur, ccur, f, ioff.p, inc.p, L.cliques.p,
+                            deg.p, core.p, kill.p, nxt, cnxt, killed.p, st));
+        CHK(readback(e, hp + 2, cnxt, 1));
+        f = hp[2];
+        std::swap(cur, nxt);
+        std::swap(ccur, cnxt);
+        ++peel_rounds;
+      }
+      unsigned long long kl = 0;
+      CHK(readback(e, &kl, killed.p, 1));
+      levels.push_back({level, removed, kl});
+      removed_total += removed;
+    }
+    fa.reset();
+    fb.reset();
+    R->t_incidence_ms = tm.ms(ev_inc0, ev_inc1);
+
+    // Exact re-peel of coarse step i: its vertices and cliques become alive
+    // again (degrees from sdeg), later steps are parked, and the step's
+    // entry is replaced by its exact levels.
+    auto refine_step = [&](size_t i) -> cds_status {
+      const Level lv = levels[i];
+      if (!ralive.p) {
+        HIPCHK(ralive.alloc(n));
+        HIPCHK(rcount.alloc(1));
+      }
+      HIPCHK(refine_prepare(n, T, lv.level, core.p, deg.p, sdeg.p, kill.p, ralive.p,
+                            rcount.p, st));
+      uint32_t na = 0;
+      CHK(readback(e, &na, rcount.p, 1));
+      if (na != lv.removed)
+        return fail(CDS_ERR_INTERNAL, "coarse step %u holds %u of %llu vertices", lv.level,
+                    na, (unsigned long long)lv.removed);
+      // as wide as the first pass: the re-peel is bound by its cliques'
+      // kills (measured 127 ms against 157 ms with a grid sized by the
+      // step's own clique count on R-MAT scale 20, k = 3)
+      const unsigned grid = coop_width(e, peel_grid_blocks(n, T, k, e->device, e->cur.peel_blocks));
+      if (grid == 0) return fail(CDS_ERR_HIP, "the persistent peel cannot be launched");
+      // Exact levels at once: re-peeling a step in narrower steps first and
+      // refining those re-processes the step's cliques at every depth, which
+      // measured slower on R-MAT scale 20 k=3 and scale 17 k=4.
+      std::vector<Level> exact;
+      PeelCtl hc;
+      CHK(run_peel(ralive.p, na, 0, lv.level, grid, na, &exact, &hc));
+      HIPCHK(refine_restore(n, core.p, deg.p, st));
+      uint64_t kl = 0;
+      for (const Level &x : exact) {
+        kl += x.killed;
+        if (x.level < lv.lo || x.level > lv.level)
+          return fail(CDS_ERR_INTERNAL, "coarse step [%u, %u] re-peeled to level %u", lv.lo,
+                      lv.level, x.level);
+      }
+      if (kl != lv.killed)
+        return fail(CDS_ERR_INTERNAL, "coarse step %u killed %llu of %llu cliques", lv.level,
+                    (unsigned long long)kl, (unsigned long long)lv.killed);
+      levels.erase(levels.begin() + (ptrdiff_t)i);
+      levels.insert(levels.begin() + (ptrdiff_t)i, exact.begin(), exact.end());
+      R->peel_refined_steps++;
+      return CDS_OK;
+    };
+
+    // Coarse steps are refined and exact levels replayed until neither list
+    // has work: each changes the table or rho, so the summary is recomputed.
+    std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
+    const bool all_cores = p->want_vertex_arrays != 0;
+    auto summarize = [&]() -> cds_status {
+      double rho_r = 0;
+      for (const auto &r : replayed)
+        if (r.second.found)
+          rho_r = std::max(rho_r, (double)(int64_t)r.second.best_num /
+                                      static_cast<double>(r.second.best_den));
+      return summarize_levels(levels, n, T, &PS, all_cores, rho_r);
+    };
+    CHK(summarize());
+    {
+      DBuf<uint32_t> pv(&A), pdeg(&A), posmap(&A), rdead(&A), pc(&A);
+      DBuf<uint64_t> pioff(&A);
+      DBuf<uint8_t> palive(&A);
+      DBuf<ReplayOut> rout(&A);
+      bool allocated = false;
+      double refine_ms = 0;
+      const int ev_r0 = tm.mark();
+      for (;;) {
+        if (!PS.refine.empty()) {
+          const auto t0 = std::chrono::steady_clock::now();
+          CHK(refine_step(PS.refine.front()));
+          HIPCHK(hipStreamSynchronize(st));
+          refine_ms += std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t0)
+                           .count();
+          CHK(summarize());
+          continue;
+        }
+        size_t ii = SIZE_MAX;
+        for (size_t cand : PS.replay) {
+          bool done = false;
+          for (const auto &r : replayed) done |= r.first == levels[cand].level;
+          if (!done) {
+            ii = cand;
+            break;
+          }
+        }
+        if (ii == SIZE_MAX) break;
+        const Level &lv = levels[ii];
+        const uint64_t M = lv.removed;
+        if (!allocated) {
+          HIPCHK(pv.alloc(n));
+          HIPCHK(pdeg.alloc(n));
+          HIPCHK(posmap.alloc(n));
+          HIPCHK(palive.alloc(n));
+          HIPCHK(rdead.alloc(T));
+          HIPCHK(rdead.zero());
+          HIPCHK(pc.alloc(1));
+          HIPCHK(pioff.alloc((size_t)n + 1));
+          HIPCHK(rout.alloc(1));
+          allocated = true;
+        }
+        DBuf<uint32_t> pinc(&A), partial(&A);
+        DBuf<GridBarrier> gbar(&A);
+        // Few vertices with many cliques each: replay on the whole GPU.
+        unsigned rgrid = 0;
+        if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
+          rgrid = coop_width(e, replay_grid_blocks(e->device, lv.killed * k / M,
+                                                   e->cur.replay_blocks));
+        if (rgrid) {
+          HIPCHK(partial.alloc((size_t)rgrid * M));
+          HIPCHK(gbar.alloc(1));
+          R->replay_grid_phases++;
+        } else {
+          HIPCHK(pinc.alloc(lv.killed * k * k));
+        }
+        std::optional<CoopGate> gate;
+        if (rgrid) gate.emplace(e, rgrid);
+        HIPCHK(hipEventRecord(tm.ev[22], st));
+        const unsigned rasked = rgrid;
+        HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
+                            T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
+                            pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
+                            pioff.p, pinc.p, rout.p, partial.p, gbar.p, &rgrid,
+                            sdeg.p, A, tm.ev[23], (uint64_t)PS.kp,
+                            replay_nstar(k, (uint64_t)PS.kp), st));
+        gate.reset();
+        if (rasked) note_width(R, &R->replay_grid, rasked, rgrid);
+        ReplayOut ro;
+        CHK(readback(e, &ro, rout.p, 1));
+        if (rgrid > 1) {
+          unsigned ab = 0;
+          CHK(readback(e, &ab, &gbar.p->abort, 1));
+          if (ab) return barrier_failed("whole-GPU replay", rgrid);
+        }
+        {
+          float prep = 0;
+          (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
+          R->t_replay_prep_ms += prep;
+        }
+        R->replay_phases++;
+        if (ro.steps > M || (!ro.stopped && ro.steps != M))
+          return fail(CDS_ERR_INTERNAL, "replay of level %u ran %u of %llu steps",
+                      lv.level, ro.steps, (unsigned long long)M);
+        replayed.push_back({lv.level, ro});
+        // a replay can raise k': the refine list (steps straddling k') and
+        // the replay candidates are recomputed against the new value
+        CHK(summarize());
+      }
+      const int ev_r1 = tm.mark();
+      HIPCHK(hipStreamSynchronize(st));
+      R->t_replay_ms = std::max(0.0, tm.ms(ev_r0, ev_r1) - refine_ms);
+    }
   }
   const std::vector<uint64_t> &rm_before = PS.rm_before, &kl_before = PS.kl_before;
   R->peel_levels = peel_rounds;
@@ -1995,6 +2106,146 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
 // launch, larger components on several workgroups each in cooperative
 // launches).  Results equal cds_solve on each graph.
 // ---------------------------------------------------------------------------
+
+// A device graph built here (an induced subgraph); frees its CSR.
+struct OwnedDgraph {
+  cds_dgraph g;
+  DBuf<uint32_t> off, adj;
+  explicit OwnedDgraph(DeviceArena *a) : off(a), adj(a) {}
+};
+
+// The pruned path: the graph is cut to its tau-clique-core S by clique-count
+// passes over induced subgraphs (prune.hip: every vertex with fewer than
+// tau cliques in the current S goes, until none does; then tau rises to
+// ceil(T(S) / |S|) while that grows), and the whole solve runs on G[S].
+// Every S at a point where nothing drops is a boundary state of the
+// oracle's peel (its tau-clique-core), so tau <= k' throughout, and the
+// peel rows above S are at most max(rho(S), tau - 1): k', kmax, the densest
+// core and everything after it are those of G[S].  The count L0 of the
+// whole graph gives T and the clique degrees.
+cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
+                        cds_result *R, bool core_only, Listing &L0) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  DeviceArena &A = *e->arena;
+  hipStream_t st = e->stream;
+  const uint32_t n = g->n, k = p->k;
+  const uint64_t T = L0.T;
+  DBuf<uint8_t> keep(&A);
+  DBuf<uint32_t> newid(&A), cnt(&A), sel(&A), dropped(&A);
+  HIPCHK(keep.alloc(n));
+  HIPCHK(hipMemsetAsync(keep.p, 1, n, st));
+  HIPCHK(newid.alloc((size_t)n + 1));
+  HIPCHK(cnt.alloc(n));
+  HIPCHK(sel.alloc(n));
+  HIPCHK(dropped.alloc(1));
+  auto gs = std::make_unique<OwnedDgraph>(&A);
+  std::unique_ptr<Listing> Ls;
+  uint64_t tau = (T + n - 1) / n, Ts = T;
+  uint32_t ns = n, passes = 0;
+  const uint64_t *cds = L0.cd.p;
+  bool first = true;
+  Timer tm(st);
+  for (bool rebuild = true;;) {
+    if (rebuild) {
+      // G[S] from keep[]
+      HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, cnt.p, nullptr, nullptr,
+                              nullptr, false, A));
+      CHK(readback(e, &ns, newid.p + n, 1));
+      if (ns == 0) return fail(CDS_ERR_INTERNAL, "core pruning left no vertex (tau %llu)",
+                               (unsigned long long)tau);
+      auto ng = std::make_unique<OwnedDgraph>(&A);
+      HIPCHK(ng->off.alloc((size_t)ns + 1));
+      HIPCHK(scan_u32_32(cnt.p, ng->off.p, ns, A));
+      ng->g.host_off.resize((size_t)ns + 1);
+      CHK(readback(e, ng->g.host_off.data(), ng->off.p, (size_t)ns + 1));
+      const uint64_t arcs = ng->g.host_off[ns];
+      HIPCHK(ng->adj.alloc(arcs));
+      HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, nullptr, ng->off.p,
+                              ng->adj.p, sel.p, true, A));
+      ng->g.n = ns;
+      ng->g.m = arcs / 2;
+      ng->g.off = ng->off.p;
+      ng->g.adj = ng->adj.p;
+      uint32_t md = 0;
+      for (uint32_t v = 0; v < ns; ++v)
+        md = std::max(md, ng->g.host_off[v + 1] - ng->g.host_off[v]);
+      ng->g.max_degree = md;
+      gs = std::move(ng);
+      if (!first) {
+        // clique counts inside S
+        Ls = std::make_unique<Listing>(&A);
+        CHK(run_dag(e, &gs->g, *Ls));
+        CHK(run_listing(e, &gs->g, k, false, *Ls, &tm));
+        Ts = Ls->T;
+        cds = Ls->cd.p;
+        ++passes;
+      }
+      first = false;
+      rebuild = false;
+    }
+    HIPCHK(drop_below(ns, cds, tau, sel.p, keep.p, dropped.p, st));
+    uint32_t nd = 0;
+    CHK(readback(e, &nd, dropped.p, 1));
+    if (nd) {
+      rebuild = true;
+      continue;
+    }
+    // S is the tau-clique-core; its density is a peel row
+    const uint64_t t2 = (Ts + ns - 1) / ns;
+    if (t2 <= tau) break;
+    tau = t2;
+  }
+  Ls.reset();
+  // the solve on G[S]: persistent peel if its cliques fit the incidence,
+  // else the bitset peel of a dense core
+  const char *fd = getenv("CDS_DENSE_PEEL");
+  bool dense = (unsigned __int128)k * Ts >= 0xFFFFFFFFull || needs_prune(k, Ts);
+  if (fd) dense = atoi(fd) != 0;
+  if (dense && ns > kDenseMaxVertices)
+    return fail(CDS_ERR_RANGE,
+                "the %llu-clique-core has %u vertices and %llu cliques: too many to "
+                "materialise, too many vertices for the dense peel (at most %u)",
+                (unsigned long long)tau, ns, (unsigned long long)Ts, kDenseMaxVertices);
+  cds_result RS;
+  CHK(solve_impl(e, &gs->g, p, &RS, core_only, dense ? kSolveDenseCore : kSolveCore));
+  std::unique_ptr<cds_result, void (*)(cds_result *)> hold(&RS, cds_result_free);
+  // the answer in the input's ids
+  std::vector<uint32_t> hsel(ns);
+  CHK(readback(e, hsel.data(), sel.p, ns));
+  *R = RS;
+  memset(&RS, 0, sizeof(RS));  // arrays now owned by R
+  R->n = n;
+  R->m = g->m;
+  R->max_degree = g->max_degree;
+  R->total_cliques = T;
+  R->deleted_cliques = T - R->core_cliques;
+  R->dag_arcs = g->m;
+  R->list_work = L0.work;
+  R->max_out_degree = L0.max_out;
+  R->list_items = L0.items_n;
+  R->t_list_count_ms = L0.t_count;
+  if (T * k >= (1ull << 31)) R->oracle_quirk_flags |= 4;  // the oracle's int total wraps
+  for (uint32_t i = 0; i < R->set_size; ++i) R->vertices[i] = hsel[R->vertices[i]];
+  if (p->want_vertex_arrays) {
+    std::vector<uint32_t> core(n, kAliveHost);
+    if (R->core)
+      for (uint32_t i = 0; i < ns; ++i) core[hsel[i]] = R->core[i];
+    free(R->core);
+    free(R->clique_degree);
+    R->core = host_alloc<uint32_t>(n);
+    memcpy(R->core, core.data(), (size_t)n * 4);
+    R->clique_degree = host_alloc<uint64_t>(n);
+    CHK(readback(e, R->clique_degree, L0.cd.p, n));
+  }
+  R->prune_tau = tau;
+  R->prune_vertices = ns;
+  R->prune_passes = passes;
+  R->dense_peel = dense ? 1 : 0;
+  R->times.total = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - wall0)
+                       .count();
+  return CDS_OK;
+}
 
 cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
                        const std::vector<const cds_params *> &ps,

@@ -78,6 +78,28 @@ hipError_t fill_u64(uint64_t *a, size_t n, uint64_t v, hipStream_t s);
 // *out = sum of in[0..n) (out is cleared by a kernel first).
 hipError_t sum_u32(const uint32_t *in, size_t n, unsigned long long *out, hipStream_t s);
 
+// ---- core pruning and the dense-core peel (prune.hip) ----
+// Vertices of the largest core peeled on adjacency bitsets.
+constexpr uint32_t kDenseMaxVertices = 16384;
+// Induced subgraph on the vertices with keep[v] != 0, in two calls:
+// fill = false computes newid (exclusive scan of keep) and cnt[newid] (the
+// induced degrees); the caller scans cnt into off2; fill = true writes adj2
+// (new ids, ascending) and sel[newid] = old id.
+hipError_t induced_subgraph(uint32_t n, const uint32_t *off, const uint32_t *adj,
+                            const uint8_t *keep, uint32_t *newid, uint32_t *cnt,
+                            uint32_t *off2, uint32_t *adj2, uint32_t *sel, bool fill,
+                            DeviceArena &a);
+// keep[sel[i]] = 0 for i < ns with cd[i] < tau; *dropped = how many.
+hipError_t drop_below(uint32_t ns, const uint64_t *cd, uint64_t tau, const uint32_t *sel,
+                      uint8_t *keep, uint32_t *dropped, hipStream_t s);
+// The sequential peel of a graph of ns <= kDenseMaxVertices vertices with
+// clique degrees cd: rv[i], rval[i] = the vertex removed i-th and its clique
+// count then.  bits: ns * ceil(ns / 64) words; alive: ceil(ns / 64); cur: ns.
+hipError_t dense_peel(uint32_t ns, uint32_t k, const uint32_t *off, const uint32_t *adj,
+                      const uint64_t *cd, unsigned long long *bits, unsigned long long *alive,
+                      unsigned long long *cur, uint32_t *step_v, uint32_t *rv,
+                      unsigned long long *rval, hipStream_t s);
+
 // ---- DAG (dag.hip) ----
 // Degree-ordered orientation: u -> v iff (deg u, u) < (deg v, v).
 hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,

@@ -132,6 +132,11 @@ std::string render(const cds_result *r) {
   j.u64("peel_levels", r->peel_levels);
   j.u64("peel_distinct_levels", r->peel_distinct_levels);
   j.u64("replay_phases", r->replay_phases);
+  // solved on the prune_tau-clique-core (0: on the whole graph)
+  j.u64("prune_tau", r->prune_tau);
+  j.u64("prune_vertices", r->prune_vertices);
+  j.u64("prune_passes", r->prune_passes);
+  j.u64("dense_peel", r->dense_peel);
   j.close('}');
   j.open("search", '{');
   j.u64("components", r->num_components);

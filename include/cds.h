@@ -179,6 +179,18 @@ typedef struct {
   uint32_t fragile_decisions;
   uint8_t *trace_fragile;           /* trace_len entries */
   double *trace_drift;              /* trace_len entries */
+  /* Core pruning, for graphs whose cliques are too many to materialise
+   * (k*T slots beyond 32 bits or beyond the device): the peel and the
+   * search ran on the prune_tau-clique-core (prune_vertices vertices, found
+   * by prune_passes clique-count passes), which holds the densest core
+   * whenever prune_tau <= k_prime; 0 when the whole graph was materialised.
+   * core[] then holds 0xFFFFFFFF for the vertices outside that core (core
+   * number below prune_tau).  dense_peel = 1: the core was peeled on
+   * adjacency bitsets, one removal at a time. */
+  uint64_t prune_tau;
+  uint32_t prune_vertices;
+  uint32_t prune_passes;
+  uint32_t dense_peel;
 } cds_result;
 
 typedef struct {
