@@ -1752,10 +1752,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     return CDS_OK;
   }
 
-  // The flow networks index core clique slots with 32 bits: a core beyond
-  // that is refused here, before pruning and bucketing it.
-  if (core_T * k >= 0xFFFFFFFFull)
-    return fail(CDS_ERR_RANGE, "core clique slots exceed 32 bits");
+  // Flow networks of 2^32 clique-member slots or more run the wide max flow
+  // (clique-id incidence, 32-bit residuals); clique ids stay 32-bit, so a
+  // core beyond that is refused here, before pruning and bucketing it.
+  // CDS_FLOW_WIDE=1 sends every network to the wide kernel (tests).
+  const char *fwide = getenv("CDS_FLOW_WIDE");
+  const bool wide = (fwide && atoi(fwide) != 0) || core_T * k >= 0xFFFFFFFFull;
+  if (core_T >= 0xFFFFFFFFull)
+    return fail(CDS_ERR_RANGE, "%llu core cliques exceed 32-bit ids",
+                (unsigned long long)core_T);
   // ---- prune ----
   DBuf<uint8_t> mark(&A);
   {
@@ -1851,8 +1856,6 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   if (hcoff[ncomp] != core_T)
     return fail(CDS_ERR_INTERNAL, "bucketed %llu of %llu core cliques",
                 (unsigned long long)hcoff[ncomp], (unsigned long long)core_T);
-  if (core_T * k >= 0xFFFFFFFFull)
-    return fail(CDS_ERR_RANGE, "core clique slots exceed 32 bits");
   HIPCHK(coff.alloc(ncomp + 1));
   CHK(upload(e, coff.p, hcoff.data(), ncomp + 1));
   HIPCHK(ccur.alloc(ncomp));
@@ -1875,9 +1878,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
   HIPCHK(finc.alloc(core_T * k));
   HIPCHK(ficur.alloc(core_n));
-  HIPCHK(flow_incidence_fill(core_T, k, fmem.p, fioff.p, core_n, ficur.p, finc.p, st));
+  HIPCHK(flow_incidence_fill(core_T, k, fmem.p, fioff.p, core_n, ficur.p, finc.p, wide, st));
   HIPCHK(hipStreamSynchronize(st));
   ficur.reset();
+  if (wide) {
+    // room for the flow state: the full clique list and kill levels are
+    // not read again (the answer's cliques are counted over fmem)
+    L.cliques.reset();
+    kill.reset();
+  }
   const int ev_cc = tm.mark();
 
   // ---- (c) dynamicExact (helpers.cpp:1120-1254) ----
@@ -1895,6 +1904,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<int> freach(&A);
   bool flow_alloc = false;
   DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
+  DBuf<int> gx32(&A), gexc32(&A);
   DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
   DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
   DBuf<uint64_t> glc0(&A), glc1(&A), garc(&A);
@@ -1912,6 +1922,11 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         (__int128)capT * C.V >= ((__int128)1 << 62) ||
         (__int128)C.T * k * U >= ((__int128)1 << 62))
       return fail(CDS_ERR_RANGE, "component %u exceeds the int64 flow range", ci);
+    // wide kernel: a slot's residual stays in [-U, (k-1)U] and a clique's
+    // excess below k(k-1)U, both kept in 32 bits
+    if (wide && (__int128)k * k * U >= ((__int128)1 << 31))
+      return fail(CDS_ERR_RANGE, "component %u: unit %lld too large for 32-bit residuals",
+                  ci, U);
     FlowProblem P;
     P.k = k;
     P.V = C.V;
@@ -1925,12 +1940,17 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     // push-relabel on as many workgroups as the network needs (one for a
     // small component); state sized for the largest component, allocated
     // at the first flow of the solve
-    if (!gx.p) {
-      HIPCHK(gx.alloc(maxT * k));
+    if (!gx.p && !gx32.p) {
+      if (wide) {
+        HIPCHK(gx32.alloc(maxT * k));
+        HIPCHK(gexc32.alloc(maxT));
+      } else {
+        HIPCHK(gx.alloc(maxT * k));
+        HIPCHK(gexc.alloc(maxT));
+      }
       HIPCHK(grvs.alloc(maxV));
       HIPCHK(gexv.alloc(maxV));
       HIPCHK(grem.alloc(maxV));
-      HIPCHK(gexc.alloc(maxT));
       HIPCHK(ghv.alloc(maxV));
       HIPCHK(ghmin.alloc(maxV));
       HIPCHK(ghc.alloc(maxT));
@@ -1946,6 +1966,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     }
     GridFlowState G;
     G.x = gx.p;
+    G.x32 = gx32.p;
+    G.exc32 = gexc32.p;
     G.rvs = grvs.p;
     G.exv = gexv.p;
     G.rem = grem.p;
@@ -1972,7 +1994,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     }
     unsigned launched = 0;
     {
-      const unsigned fgrid = coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e)));
+      const unsigned fgrid =
+          coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e), wide));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
       launched = fgrid;
@@ -2067,8 +2090,19 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(mask.zero());
     std::vector<uint8_t> hm(n, 0);
     for (uint32_t v : verts) hm[v] = 1;
-    CHK(upload(e, mask.p, hm.data(), n));
-    HIPCHK(count_cliques_in_mask(T, k, L.cliques.p, mask.p, acc.p, st));
+    if (wide) {
+      // the set lies in the densest core, so do its cliques: count them
+      // over the core's cliques, in flow-vertex ids
+      std::vector<uint8_t> hf(core_n, 0);
+      for (uint32_t v : verts)
+        if (hcore[v] >= kpu) hf[fv_of[v]] = 1;
+      HIPCHK(mask.alloc(core_n));
+      CHK(upload(e, mask.p, hf.data(), core_n));
+      HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st));
+    } else {
+      CHK(upload(e, mask.p, hm.data(), n));
+      HIPCHK(count_cliques_in_mask(T, k, L.cliques.p, mask.p, acc.p, st));
+    }
     unsigned long long sc = 0;
     CHK(readback(e, &sc, acc.p, 1));
     R->set_cliques = sc;
@@ -2741,7 +2775,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
     DBuf<uint32_t> ficur(&A);
     HIPCHK(ficur.alloc(core_total));
     HIPCHK(flow_incidence_fill(core_T_total, k, fmem.p, fioff.p, core_total, ficur.p,
-                               finc.p, st));
+                               finc.p, false, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   inc.reset();

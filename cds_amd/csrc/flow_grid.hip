@@ -137,16 +137,32 @@ template <class T> __device__ __forceinline__ T ald(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Adds d to a clique's excess; returns the value before.
+__device__ __forceinline__ long long exc_add(long long *p, long long d) {
+  return (long long)atomicAdd((unsigned long long *)p, (unsigned long long)d);
+}
+__device__ __forceinline__ long long exc_add(int *p, long long d) {
+  return (long long)(int)atomicAdd((unsigned int *)p, (unsigned int)(int)d);
+}
+
+// XT: the per-(clique, member) residual x and the clique excess (int64, or
+// int32 for networks whose capacities fit 31 bits).  kByClique: the
+// vertex incidence holds clique ids (from cbase) instead of clique-member
+// slots (from sbase), for networks with 2^32 slots or more; the member's
+// slot is found among the clique's k members.
+template <class XT, bool kByClique_>
 struct G {
+  static constexpr bool kByClique = kByClique_;
   uint32_t k, V, T;
   int INF;
   uint32_t vbase;
-  uint64_t sbase;
+  uint64_t sbase, cbase;
   long long U;
   const uint32_t *mem;
   const uint64_t *ioff;
   const uint32_t *inc;
-  long long *x, *rvs, *exv, *exc, *rem;
+  XT *x, *exc;
+  long long *rvs, *exv, *rem;
   int *hv, *hc, *hmin;
   // Double buffers are separate members selected with ?: (an array member
   // indexed at run time would live in scratch memory).
@@ -163,25 +179,31 @@ struct G {
 };
 
 // Barrier over the workgroups working on one problem.
-__device__ __forceinline__ void gsync(const G &g) {
+template <class Gx>
+__device__ __forceinline__ void gsync(const Gx &g) {
   group_barrier(&g.ctl->bar, g.nblocks, g.bid);
 }
 
-__device__ __forceinline__ uint32_t *glist(const G &g, int b) {
+template <class Gx>
+__device__ __forceinline__ uint32_t *glist(const Gx &g, int b) {
   return b ? g.list1 : g.list0;
 }
-__device__ __forceinline__ uint64_t *glchunk(const G &g, int b) {
+template <class Gx>
+__device__ __forceinline__ uint64_t *glchunk(const Gx &g, int b) {
   return b ? g.lchunk1 : g.lchunk0;
 }
-__device__ __forceinline__ uint32_t *gcq(const G &g, int b) {
+template <class Gx>
+__device__ __forceinline__ uint32_t *gcq(const Gx &g, int b) {
   return b ? g.cq1 : g.cq0;
 }
 
-__device__ __forceinline__ uint64_t nchunks(const G &g, uint32_t v) {
+template <class Gx>
+__device__ __forceinline__ uint64_t nchunks(const Gx &g, uint32_t v) {
   return (g.ioff[v + 1] - g.ioff[v] + kGridChunk - 1) / kGridChunk;
 }
 
-__device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
+template <class Gx>
+__device__ __forceinline__ void list_push(const Gx &g, int b, uint32_t v) {
   const unsigned long long old =
       atomicAdd(&g.ctl->lpack[b], (1ull << 40) + (unsigned long long)nchunks(g, v));
   glist(g, b)[old >> 40] = v;
@@ -189,7 +211,8 @@ __device__ __forceinline__ void list_push(const G &g, int b, uint32_t v) {
 }
 
 // Appends v with `chunks` work chunks to vertex list b.
-__device__ __forceinline__ void list_push_n(const G &g, int b, uint32_t v,
+template <class Gx>
+__device__ __forceinline__ void list_push_n(const Gx &g, int b, uint32_t v,
                                             uint32_t chunks) {
   const unsigned long long old =
       atomicAdd(&g.ctl->lpack[b], (1ull << 40) + (unsigned long long)chunks);
@@ -200,7 +223,8 @@ __device__ __forceinline__ void list_push_n(const G &g, int b, uint32_t v,
 // First chunk and chunk count a vertex with excess scans this round: its
 // current-arc window, or the whole list once the current arc has passed
 // the end (a full scan, after which it relabels).
-__device__ __forceinline__ void discharge_range(const G &g, uint32_t v,
+template <class Gx>
+__device__ __forceinline__ void discharge_range(const Gx &g, uint32_t v,
                                                 uint32_t *first, uint32_t *count) {
   const uint32_t nch = (uint32_t)nchunks(g, v);
   const uint64_t a = g.arc[v];
@@ -217,7 +241,8 @@ __device__ __forceinline__ void discharge_range(const G &g, uint32_t v,
 // Maps work chunk ci of list b (count entries) to (vertex, incidence range);
 // `windowed`: the vertex's chunks start at its discharge range.  Called by
 // whole waves with a wave-uniform ci.
-__device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
+template <class Gx>
+__device__ __forceinline__ void chunk_of(const Gx &g, int b, uint32_t count,
                                          uint64_t ci, uint32_t *v, uint64_t *lo,
                                          uint64_t *hi, bool windowed = false,
                                          uint32_t *vchunk = nullptr) {
@@ -232,10 +257,26 @@ __device__ __forceinline__ void chunk_of(const G &g, int b, uint32_t count,
   *hi = *lo + kGridChunk < e ? *lo + kGridChunk : e;
 }
 
+// Slot (clique * k + member position, from the component's first clique)
+// of incidence entry p of vertex v.
+template <class Gx>
+__device__ __forceinline__ uint64_t slot_at(const Gx &g, uint64_t p, uint32_t v) {
+  if constexpr (Gx::kByClique) {
+    const uint64_t base = ((uint64_t)g.inc[p] - g.cbase) * g.k;
+    const uint32_t gv = v + g.vbase;
+    uint32_t j = 0;
+    while (j + 1 < g.k && g.mem[base + j] != gv) ++j;  // v is one of the k
+    return base + j;
+  } else {
+    return g.inc[p] - g.sbase;
+  }
+}
+
 // Exact BFS distances to the sink over reversed residual arcs; returns the
 // number of active nodes (excess > 0 and finite height).  Uses list buffer
 // 0/1 for vertex frontiers and cq[0]/cq[1] for clique frontiers.
-__device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
+template <class Gx>
+__device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64_t arc_reset) {
   GridFlowCtl *ctl = g.ctl;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
   for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.hc[c] = g.INF;
@@ -304,7 +345,7 @@ __device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
         f[it] = false;
         cs[it] = 0;
         if (p >= hi) continue;
-        const uint64_t s = g.inc[p] - g.sbase;
+        const uint64_t s = slot_at(g, p, v);
         const uint32_t c = (uint32_t)(s / g.k);
         // a plain load filters: a stale INF only costs a failed CAS; most
         // cliques met after the first level are labelled already
@@ -328,7 +369,7 @@ __device__ unsigned long long global_relabel(const G &g, uint64_t arc_reset) {
           const uint64_t p = p0 + lane();
           bool hit = false;
           if (p < e) {
-            const uint64_t s = g.inc[p] - g.sbase;
+            const uint64_t s = slot_at(g, p, v);
             hit = g.hc[s / g.k] == d + 1 && (long long)(g.k - 1) * g.U - g.x[s] > 0;
           }
           if (__ballot(hit)) {
@@ -398,22 +439,33 @@ constexpr int kGridFlowThreads = 1024;
 constexpr int kFlowBatchThreads = 512;
 
 // Problem fields of G; the caller sets the thread geometry.
-__device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
+template <class Gx>
+__device__ __forceinline__ void flow_setup(Gx &g, const FlowProblem &P,
                                            const GridFlowState &S, long long U) {
   g.k = P.k;
   g.V = P.V;
   g.T = P.T;
-  g.INF = (int)(P.V + P.T) + 1;
+  // "unreachable" height: above every finite distance to the sink, which
+  // alternates vertex and clique nodes (at most 2V + 1); V + T + 1 while
+  // that fits an int comfortably (the labels of a valid labelling never
+  // exceed a node's distance, so either bound keeps the flow exact)
+  g.INF = (uint64_t)P.V + P.T + 1 < (1u << 30) ? (int)(P.V + P.T) + 1 : (int)(2 * P.V + 3);
   g.vbase = P.vbase;
   g.sbase = P.sbase;
+  g.cbase = P.k ? P.sbase / P.k : 0;
   g.U = U;
   g.mem = P.mem;
   g.ioff = P.ioff;
   g.inc = P.inc;
-  g.x = S.x;
+  if constexpr (Gx::kByClique) {
+    g.x = S.x32;
+    g.exc = S.exc32;
+  } else {
+    g.x = S.x;
+    g.exc = S.exc;
+  }
   g.rvs = S.rvs;
   g.exv = S.exv;
-  g.exc = S.exc;
   g.rem = S.rem;
   g.hv = S.hv;
   g.hc = S.hc;
@@ -436,8 +488,8 @@ __device__ __forceinline__ void flow_setup(G &g, const FlowProblem &P,
 // registers for pass 2 (the whole-grid kernel, one workgroup per CU); the
 // batched kernel re-reads them and keeps its registers for two workgroups
 // per CU.
-template <bool kCacheArcs>
-__device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
+template <bool kCacheArcs, class Gx>
+__device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long U, uint32_t max_rounds,
                          long long *lds_ex, bool lds_v) {
   GridFlowCtl *ctl = S.ctl;
@@ -608,7 +660,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
           sv[it] = 0;
           const uint64_t p = lo + (uint64_t)it * kWave + lane();
           if (p >= hi) continue;
-          const uint64_t s = g.inc[p] - g.sbase;
+          const uint64_t s = slot_at(g, p, v);
           const long long res = (long long)(g.k - 1) * U - g.x[s];
           if (res <= 0) continue;
           const int hcc = g.hc[s / g.k];
@@ -622,7 +674,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         }
       } else {
         for (uint64_t p = lo + lane(); p < hi; p += kWave) {
-          const uint64_t s = g.inc[p] - g.sbase;
+          const uint64_t s = slot_at(g, p, v);
           const long long res = (long long)(g.k - 1) * U - g.x[s];
           if (res <= 0) continue;
           const int hcc = g.hc[s / g.k];
@@ -663,7 +715,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         } else {
           const uint64_t p = p0 + lane();
           if (p < hi) {
-            s = g.inc[p] - g.sbase;
+            s = slot_at(g, p, v);
             const long long res = (long long)(g.k - 1) * U - g.x[s];
             if (res > 0 && g.hc[s / g.k] < hvv) ra = res;
           }
@@ -680,8 +732,7 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
         if (take > 0) {
           g.x[s] += take;
           fc[it] = (uint32_t)(s / g.k);
-          fresh[it] = atomicAdd((unsigned long long *)&g.exc[fc[it]],
-                                (unsigned long long)take) == 0;
+          fresh[it] = exc_add(&g.exc[fc[it]], take) == 0;
         }
         before += __shfl(incl, kWave - 1, kWave);
       }
@@ -767,11 +818,15 @@ __device__ void flow_run(G &g, const FlowProblem &P, const GridFlowState &S,
   }
 }
 
-__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
-                                                   GridFlowState S,
-                                                   long long cap_t, long long U,
-                                                   uint32_t max_rounds) {
-  G g;
+// The whole-GPU max flow: Gx = G<long long, false> for networks of fewer
+// than 2^32 clique-member slots, G<int, true> beyond (clique-id incidence,
+// 32-bit residuals; the arcs are re-read in the discharge's second pass to
+// keep registers for the slot lookup).
+template <class Gx, bool kCacheArcs>
+__device__ __forceinline__ void grid_flow_body(FlowProblem P, GridFlowState S,
+                                               long long cap_t, long long U,
+                                               uint32_t max_rounds) {
+  Gx g;
   flow_setup(g, P, S, U);
   g.gtid = blockIdx.x * blockDim.x + threadIdx.x;
   g.gthreads = gridDim.x * blockDim.x;
@@ -783,7 +838,30 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   __shared__ uint32_t app[kMaxFlowWaves + 1];
   g.app = app;
   extern __shared__ long long lds_ex[];
-  flow_run<true>(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
+  if constexpr (Gx::kByClique) {
+    // inlined here: as a call, G would live in scratch memory
+    [[clang::always_inline]] flow_run<kCacheArcs>(g, P, S, cap_t, U, max_rounds, lds_ex,
+                                                  g.V <= kGridLdsVertices);
+  } else {
+    flow_run<kCacheArcs>(g, P, S, cap_t, U, max_rounds, lds_ex, g.V <= kGridLdsVertices);
+  }
+}
+
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
+                                                   GridFlowState S,
+                                                   long long cap_t, long long U,
+                                                   uint32_t max_rounds) {
+  grid_flow_body<G<long long, false>, true>(P, S, cap_t, U, max_rounds);
+}
+
+// 512-thread workgroups: the slot lookups need more registers than the
+// 128 a lane of a 1024-thread workgroup may hold.
+constexpr int kGridFlowWideThreads = 512;
+__global__ __launch_bounds__(kGridFlowWideThreads) void k_grid_flow_wide(FlowProblem P,
+                                                        GridFlowState S,
+                                                        long long cap_t, long long U,
+                                                        uint32_t max_rounds) {
+  grid_flow_body<G<int, true>, false>(P, S, cap_t, U, max_rounds);
 }
 
 // Many independent problems in one launch; item i owns workgroups
@@ -803,7 +881,7 @@ __global__ __launch_bounds__(kFlowBatchThreads) void k_flow_batch(
       hi = mid - 1;
   }
   const FlowBatchItem &it = items[lo];
-  G g;
+  G<long long, false> g;
   flow_setup(g, it.P, it.S, it.U);
   g.nblocks = it.nblocks;
   g.bid = blockIdx.x - it.first_block;
@@ -888,12 +966,13 @@ unsigned flow_batch_capacity(uint32_t max_v, int device) {
   return (unsigned)cus;
 }
 
-unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks) {
+unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks, bool wide) {
   int per_cu = 0, cus = 0;
   const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;
+  const void *fn = wide ? reinterpret_cast<const void *>(k_grid_flow_wide)
+                        : reinterpret_cast<const void *>(k_grid_flow);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(k_grid_flow), kGridFlowThreads,
-          lds) != hipSuccess ||
+          &per_cu, fn, wide ? kGridFlowWideThreads : kGridFlowThreads, lds) != hipSuccess ||
       per_cu < 1)
     return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
@@ -923,8 +1002,10 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
   FlowProblem p = P;
   GridFlowState st = S;
   void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
-  return launch_grid_shrinking(reinterpret_cast<const void *>(k_grid_flow), grid,
-                               kGridFlowThreads, args, lds, s);
+  const void *fn = S.x32 ? reinterpret_cast<const void *>(k_grid_flow_wide)
+                        : reinterpret_cast<const void *>(k_grid_flow);
+  return launch_grid_shrinking(fn, grid, S.x32 ? kGridFlowWideThreads : kGridFlowThreads,
+                               args, lds, s);
 }
 
 }  // namespace cds

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_deg(
 __global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem,
     const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
-    uint32_t *__restrict__ finc) {
+    uint32_t *__restrict__ finc, uint32_t div) {
   __shared__ uint32_t hkey[kFlowHash];
   __shared__ uint32_t hcnt[kFlowHash];
   const uint64_t tiles = (Tc + kFlowTile - 1) / kFlowTile;
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
       const uint32_t p = r * kFlowThreads + threadIdx.x;
       if (p >= pairs) break;
       const uint32_t x = hkey[slot[r]];
-      finc[fioff[x] + hcnt[slot[r]] + rank[r]] = (uint32_t)(c0 * k + p);
+      finc[fioff[x] + hcnt[slot[r]] + rank[r]] = (uint32_t)((c0 * k + p) / div);
     }
     __syncthreads();
   }
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_deg_direct(
 __global__ __launch_bounds__(kFlowThreads) void k_flow_fill_direct(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem, uint32_t nv,
     const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
-    uint32_t *__restrict__ finc) {
+    uint32_t *__restrict__ finc, uint32_t div) {
   __shared__ uint32_t cnt[kFlowDirect];
   for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) cnt[i] = 0;
   __syncthreads();
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_fill_direct(
   __syncthreads();
   for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads) {
     const uint32_t x = fmem[p];
-    finc[fioff[x] + atomicAdd(&cnt[x], 1u)] = (uint32_t)p;
+    finc[fioff[x] + atomicAdd(&cnt[x], 1u)] = (uint32_t)(p / div);
   }
 }
 
@@ -565,16 +565,17 @@ hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
 
 hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
-                               uint32_t *finc, hipStream_t s) {
-  if (k > 4) return hipErrorInvalidValue;
+                               uint32_t *finc, bool by_clique, hipStream_t s) {
+  if (k > 4 || k < 1) return hipErrorInvalidValue;
+  const uint32_t div = by_clique ? k : 1;
   hipError_t e = hipMemsetAsync(fcur, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
   if (nv <= kFlowDirect)
     hipLaunchKernelGGL(k_flow_fill_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
-                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc);
+                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc, div);
   else
     hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
-                       0, s, Tc, k, fmem, fioff, fcur, finc);
+                       0, s, Tc, k, fmem, fioff, fcur, finc, div);
   return hipGetLastError();
 }
 

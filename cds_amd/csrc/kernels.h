@@ -354,13 +354,14 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
 // Flow-network incidence from the core's clique bucket fmem (Tc cliques,
 // k flow-vertex ids each, nv flow vertices): fcd[v] = number of pairs of
 // vertex v (zeroed here); then, with fioff the exclusive prefix sum of fcd,
-// finc[fioff[v] ..) = the pair indices s*k+j whose vertex is v (fcur is
+// finc[fioff[v] ..) = the pair indices s*k+j whose vertex is v, or with
+// by_clique their cliques s (for networks of 2^32 pairs or more; fcur is
 // nv words of scratch, zeroed here).
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                   uint32_t nv, uint32_t *fcd, hipStream_t s);
 hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
-                               uint32_t *finc, hipStream_t s);
+                               uint32_t *finc, bool by_clique, hipStream_t s);
 
 // ---- batched solves: the disjoint union of many graphs (batch.hip) ----
 struct UnionPart {
@@ -437,6 +438,10 @@ struct GridFlowState {
   long long *x;            // T*k
   long long *rvs, *exv, *rem;  // V each
   long long *exc;          // T
+  // networks of 2^32 clique-member slots or more: 32-bit residuals and
+  // clique excess here (x, exc unused), and P.inc holds clique ids from
+  // P.sbase / k (the wide kernel); null otherwise
+  int *x32 = nullptr, *exc32 = nullptr;
   int *hv, *hmin;          // V
   int *hc;                 // T
   uint32_t *list0, *list1; // V each
@@ -477,8 +482,9 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
                          unsigned *grid, hipStream_t s);
 // Workgroups for grid_max_flow on P (0: cannot launch); `blocks` != 0
-// overrides the size-based choice.
-unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks);
+// overrides the size-based choice; `wide`: the kernel for 2^32 slots or more.
+unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks,
+                          bool wide = false);
 
 // Cliques of a component whose members all have reach != 0.
 hipError_t count_cliques_in_reach(uint32_t T, uint32_t k, const uint32_t *mem,

@@ -135,3 +135,33 @@ def test_batched_flows_on_several_workgroups(grid_engine):
                                   vertex_arrays=True)
     for i, (g, k, r) in enumerate(zip(graphs, ks, got)):
         compare(r, oracle_solve(g, k), f"batched multi-workgroup flow item {i}")
+
+
+def _with_wide(fn):
+    """Runs fn() with CDS_FLOW_WIDE=1: every max flow on the wide kernel
+    (clique-id incidence, 32-bit residuals), which otherwise serves only
+    networks of 2^32 clique-member slots or more."""
+    old = os.environ.get("CDS_FLOW_WIDE")
+    os.environ["CDS_FLOW_WIDE"] = "1"
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["CDS_FLOW_WIDE"]
+        else:
+            os.environ["CDS_FLOW_WIDE"] = old
+
+
+@pytest.mark.parametrize("blocks", ["one", "three"])
+@pytest.mark.parametrize("seed,k", [(s, k) for s in range(6) for k in (3, 4)])
+def test_wide_flow_small(engine, grid_engine, seed, k, blocks):
+    eng = engine if blocks == "one" else grid_engine
+    rng = np.random.default_rng(seed + 40)
+    n = int(rng.integers(20, 600))
+    g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                               float(rng.uniform(3, 12)), int(rng.integers(0, 25)),
+                               0.8, seed=seed + 40)
+    label = f"wide flow ({blocks} workgroups) chung_lu seed={seed} n={n} k={k}"
+    _announce(label)
+    got = _with_wide(lambda: eng.solve(eng.upload(g), k, vertex_arrays=True))
+    compare(got, oracle_solve(g, k), label)

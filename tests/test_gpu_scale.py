@@ -45,10 +45,13 @@ def _digest(a, dtype):
     return hashlib.sha256(np.ascontiguousarray(a, dtype=dtype).tobytes()).hexdigest()
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["flow", "wide_flow"])
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[6:-5] for p in FIXTURES])
-def test_full_size_against_oracle_fixture(engine, path):
+def test_full_size_against_oracle_fixture(engine, path, wide):
+    """wide: the max flows on the kernel for networks of 2^32 slots or more
+    (CDS_FLOW_WIDE=1), which R-MAT scale 18 edge factor 24 at k = 4 needs."""
     fx = _load(path)
-    print(f"[case] {fx['name']} k={fx['k']}", flush=True)
+    print(f"[case] {fx['name']} k={fx['k']} {'wide flow' if wide else ''}", flush=True)
     sys.stdout.flush()
     g = _graph(fx)
     off, nb = g.csr()
@@ -58,7 +61,15 @@ def test_full_size_against_oracle_fixture(engine, path):
     assert (g.n, g.m, h.hexdigest()) == (fx["n"], fx["m"], fx["csr_sha256"]), \
         "generator output changed: the fixture no longer describes this graph"
     dg = engine.upload(g)
-    got = engine.solve(dg, fx["k"], vertex_arrays=True)
+    old = os.environ.get("CDS_FLOW_WIDE")
+    os.environ["CDS_FLOW_WIDE"] = "1" if wide else "0"
+    try:
+        got = engine.solve(dg, fx["k"], vertex_arrays=True)
+    finally:
+        if old is None:
+            del os.environ["CDS_FLOW_WIDE"]
+        else:
+            os.environ["CDS_FLOW_WIDE"] = old
     dg.free()
     errs = []
     for key in ("total_cliques", "k_prime", "kmax", "core_size", "core_edges",
