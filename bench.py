@@ -300,6 +300,18 @@ def peel_bytes(n, T, k):
     return 4 * k * T + (4 * k + 8) * T + 8 * (k - 1) * T + 12 * n
 
 
+def flow_bytes_wide(r, k):
+    """flow_bytes for the wide kernel (networks of 2^32 clique-member slots
+    or more): per clique visit its excess and height (8 B) and per arc the
+    member id, the 4-byte residual and the member height (12 B); per vertex
+    arc the clique id (4 B), the clique's k member ids to find the slot
+    (4k B), the residual and the clique height (8 B); per global relabel
+    every arc once (12 B); per round the vertex state (12 B)."""
+    arcs = k * r["core_cliques"]
+    return (r["flow_clique_visits"] * (8 + 12 * k) + r["flow_arc_visits"] * (12 + 4 * k)
+            + r["flow_relabels"] * 12 * arcs + r["flow_sweeps"] * 12 * r["core_size"])
+
+
 def flow_bytes(r, k):
     """Algorithmic HBM bytes of the whole-network max flows of one solve, from
     the work the rounds did: per clique work-list visit its excess and
@@ -569,26 +581,41 @@ def main():
     peel_launches = avg("peel_launches") or 1
     list_ms = sum(x["times"]["dag"] + x["times"]["listing"] for x in results) / len(results)
     write_ms = avg("t_list_write_ms")
-    kernels = {
+    # networks of 2^32 clique-member slots or more run the wide flow kernel
+    wide = r["core_cliques"] * w["k"] >= 2 ** 32
+    kernels = {}
+    if w["k"] == 3:
         # two launches per solve: the count pass, then the write pass that
         # also stores every clique (12 bytes per triangle more)
-        "k_list3": {"launches_per_solve": 2 if write_ms else 1,
-                    "total_ms_per_solve": cnt_ms + write_ms,
-                    "algorithmic_bytes_per_solve":
-                        list3_bytes(g.n, r["dag_arcs"], r["list_work"]) * (2 if write_ms else 1)
-                        + (4 * w["k"] * T if write_ms else 0)},
-        "k_peel_persistent": {"launches_per_solve": peel_launches,
-                              "total_ms_per_solve": avg("t_peel_kernel_ms"),
-                              "algorithmic_bytes_per_solve": peel_bytes(g.n, T, w["k"])},
-        "k_grid_flow": {"launches_per_solve": avg("flow_launches") or 1,
-                        "total_ms_per_solve": avg("t_flow_kernel_ms"),
-                        "algorithmic_bytes_per_solve": flow_bytes(r, w["k"])},
-        "k_build_inc": {"launches_per_solve": 1,
-                        "total_ms_per_solve": avg("t_incidence_ms"),
-                        # read the cliques, write k entries of k words each
-                        "algorithmic_bytes_per_solve":
-                            4 * w["k"] * T + 4 * w["k"] * w["k"] * T},
-    }
+        kernels["k_list3"] = {
+            "launches_per_solve": 2 if write_ms else 1,
+            "total_ms_per_solve": cnt_ms + write_ms,
+            "algorithmic_bytes_per_solve":
+                list3_bytes(g.n, r["dag_arcs"], r["list_work"]) * (2 if write_ms else 1)
+                + (4 * w["k"] * T if write_ms else 0)}
+    else:
+        kernels["k_list4 (count)"] = {"launches_per_solve": 1, "total_ms_per_solve": cnt_ms,
+                                      "algorithmic_bytes_per_solve": None}
+    if r["dense_peel"]:
+        # the bitset peel of the pruned core: two launches per removal
+        kernels["k_dense_argmin + k_dense_dec"] = {
+            "launches_per_solve": 2 * r["prune_vertices"],
+            "total_ms_per_solve": avg("t_peel_kernel_ms"),
+            "algorithmic_bytes_per_solve": None}
+    else:
+        kernels["k_peel_persistent"] = {
+            "launches_per_solve": peel_launches,
+            "total_ms_per_solve": avg("t_peel_kernel_ms"),
+            "algorithmic_bytes_per_solve": peel_bytes(g.n, T, w["k"])}
+        kernels["k_build_inc"] = {"launches_per_solve": 1,
+                                  "total_ms_per_solve": avg("t_incidence_ms"),
+                                  # read the cliques, write k entries of k words each
+                                  "algorithmic_bytes_per_solve":
+                                      4 * w["k"] * T + 4 * w["k"] * w["k"] * T}
+    kernels["k_grid_flow_wide" if wide else "k_grid_flow"] = {
+        "launches_per_solve": avg("flow_launches") or 1,
+        "total_ms_per_solve": avg("t_flow_kernel_ms"),
+        "algorithmic_bytes_per_solve": (flow_bytes_wide if wide else flow_bytes)(r, w["k"])}
     if avg("t_replay_ms"):
         # exact replay of the top level(s); its bytes depend on the phase
         # (one pass over the phase's incidence per removal), not counted here
@@ -653,7 +680,10 @@ def main():
                  "refined_steps": r["peel_refined_steps"],
                  "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms"),
                  # one thread's wall-clock split of the persistent launches
-                 "split_ms": r["peel_split_ms"]},
+                 "split_ms": r["peel_split_ms"],
+                 # graphs too large to materialise: solved on the prune_tau-core
+                 "prune_tau": r["prune_tau"], "prune_vertices": r["prune_vertices"],
+                 "prune_passes": r["prune_passes"], "dense_peel": r["dense_peel"]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
