@@ -327,6 +327,19 @@ def flow_bytes(r, k):
             + r["flow_relabels"] * 16 * arcs + r["flow_sweeps"] * 12 * r["core_size"])
 
 
+def flow_bytes_imp(r, k):
+    """Algorithmic HBM bytes of the implicit-clique max flows of one solve:
+    per vertex incidence arc posted for discharge its clique slot (4 B) and
+    the clique's record (member ids and residuals, stride words of 4 B, one
+    line); per global relabel pass every arc once (the same 4 B + record);
+    per round the vertex state (excess, inflow 8 B each, height 4 B)."""
+    stride = 4 if 2 * k <= 4 else 8 if 2 * k <= 8 else 16
+    arc = 4 + 4 * stride
+    arcs = k * r["core_cliques"]
+    return (r["flow_arc_visits"] * arc + r["flow_relabels"] * arc * arcs
+            + r["flow_sweeps"] * 20 * r["core_size"])
+
+
 def percentile(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -612,10 +625,16 @@ def main():
                                   # read the cliques, write k entries of k words each
                                   "algorithmic_bytes_per_solve":
                                       4 * w["k"] * T + 4 * w["k"] * w["k"] * T}
-    kernels["k_grid_flow_wide" if wide else "k_grid_flow"] = {
-        "launches_per_solve": avg("flow_launches") or 1,
-        "total_ms_per_solve": avg("t_flow_kernel_ms"),
-        "algorithmic_bytes_per_solve": (flow_bytes_wide if wide else flow_bytes)(r, w["k"])}
+    if r["flow_implicit"] and r["flow_implicit"] == r["flow_launches"]:
+        kernels["k_grid_flow_imp_wide" if wide else "k_grid_flow_imp"] = {
+            "launches_per_solve": avg("flow_launches") or 1,
+            "total_ms_per_solve": avg("t_flow_kernel_ms"),
+            "algorithmic_bytes_per_solve": flow_bytes_imp(r, w["k"])}
+    else:
+        kernels["k_grid_flow_wide" if wide else "k_grid_flow"] = {
+            "launches_per_solve": avg("flow_launches") or 1,
+            "total_ms_per_solve": avg("t_flow_kernel_ms"),
+            "algorithmic_bytes_per_solve": (flow_bytes_wide if wide else flow_bytes)(r, w["k"])}
     if avg("t_replay_ms"):
         # exact replay of the top level(s); its bytes depend on the phase
         # (one pass over the phase's incidence per removal), not counted here

@@ -86,6 +86,15 @@ __device__ __forceinline__ uint32_t wave_last_le(const uint64_t *key,
   return lo;
 }
 
+// Member of pair q (clique q / k, position q % k) in a member array that
+// holds ms words per clique (ms = k: plain lists; ms > k: clique records).
+__device__ __forceinline__ uint32_t member_at(const uint32_t *mem, uint64_t q, uint32_t k,
+                                              uint32_t ms) {
+  if (ms == k) return mem[q];
+  const uint64_t c = k == 2 ? q >> 1 : k == 4 ? q >> 2 : k == 3 ? q / 3 : q / k;
+  return mem[c * ms + (q - c * k)];
+}
+
 // Index of `key` in the ascending array a[0..len), or -1.
 __device__ __forceinline__ int find_sorted(const uint32_t *a, int len,
                                            uint32_t key) {

@@ -1860,15 +1860,19 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   CHK(upload(e, coff.p, hcoff.data(), ncomp + 1));
   HIPCHK(ccur.alloc(ncomp));
   HIPCHK(ccur.zero());
-  HIPCHK(fmem.alloc(core_T * k));
+  // The core's cliques as flow records: per clique ms words, its k member
+  // flow-vertex ids then the k residual words of the implicit-clique max
+  // flow (the explicit kernels read the members with the same stride).
+  const uint32_t ms = imp_record_words(k);
+  HIPCHK(fmem.alloc(core_T * ms));
   {
     DBuf<uint32_t> cslot(&A);  // flow index of each core clique
     HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
-                              d_fv_of.p, ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
+                              d_fv_of.p, ncomp, coff.p, ccur.p, fmem.p, cslot.p, st, ms));
   }
   // flow incidence: per core vertex, its (clique slot, position) pairs
-  HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st));
+  HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st, ms));
   CHK(readback(e, hfcd.data(), fcd.p, core_n));
   for (uint32_t f = 0; f < core_n; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
   if (hfioff[core_n] != core_T * k)
@@ -1878,7 +1882,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   CHK(upload(e, fioff.p, hfioff.data(), core_n + 1));
   HIPCHK(finc.alloc(core_T * k));
   HIPCHK(ficur.alloc(core_n));
-  HIPCHK(flow_incidence_fill(core_T, k, fmem.p, fioff.p, core_n, ficur.p, finc.p, wide, st));
+  HIPCHK(flow_incidence_fill(core_T, k, fmem.p, fioff.p, core_n, ficur.p, finc.p, wide, st,
+                             ms));
   HIPCHK(hipStreamSynchronize(st));
   ficur.reset();
   if (wide) {
@@ -1904,8 +1909,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<int> freach(&A);
   bool flow_alloc = false;
   DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
-  DBuf<int> gx32(&A), gexc32(&A);
+  DBuf<int> gx32(&A), gexc32(&A), grec(&A);
   DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
+  // Clique records (residuals, height and excess of a clique in one
+  // 32-byte line) unless the network is wide or CDS_FLOW_REC=0; a flow
+  // whose unit is too large for 32-bit residuals uses the split arrays.
+  const char *frec = getenv("CDS_FLOW_REC");
+  const bool rec_ok = !wide && k <= 6 && maxT * 8 <= 0xFFFFFFFFull &&
+                      (frec && atoi(frec) != 0);
+  // Implicit clique nodes (records of member ids and residuals, no clique
+  // heights, excess or work lists) unless CDS_FLOW_IMPLICIT=0; built once
+  // for every core clique at the first flow that uses them.
+  const char *fimp = getenv("CDS_FLOW_IMPLICIT");
+  const bool imp_ok = !(fimp && atoi(fimp) == 0);
+  DBuf<long long> ginflow(&A);
+  bool state_alloc = false;
   DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
   DBuf<uint64_t> glc0(&A), glc1(&A), garc(&A);
   DBuf<uint32_t> gastop(&A);
@@ -1933,33 +1951,44 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     P.T = (uint32_t)C.T;
     P.vbase = C.vbase;
     P.sbase = C.cbase * k;
-    P.mem = fmem.p + C.cbase * k;
+    P.mem = fmem.p + C.cbase * ms;
+    P.mstride = ms;
     P.ioff = fioff.p + C.vbase;
     P.inc = finc.p;
     P.cd = fcd.p + C.vbase;
     // push-relabel on as many workgroups as the network needs (one for a
     // small component); state sized for the largest component, allocated
-    // at the first flow of the solve
-    if (!gx.p && !gx32.p) {
-      if (wide) {
-        HIPCHK(gx32.alloc(maxT * k));
-        HIPCHK(gexc32.alloc(maxT));
-      } else {
-        HIPCHK(gx.alloc(maxT * k));
-        HIPCHK(gexc.alloc(maxT));
-      }
+    // at the first flow of the solve that needs it
+    const bool use_imp = imp_ok && (__int128)k * k * U < ((__int128)1 << 31);
+    const bool use_rec = !use_imp && rec_ok && (__int128)k * k * U < ((__int128)1 << 31);
+    P.by_clique = wide ? 1u : 0u;
+    if (use_imp && !ginflow.p) HIPCHK(ginflow.alloc(maxV));
+    if (use_rec && !grec.p) HIPCHK(grec.alloc(maxT * 8));
+    if (!use_imp && !use_rec && !wide && !gx.p) {
+      HIPCHK(gx.alloc(maxT * k));
+      HIPCHK(gexc.alloc(maxT));
+      HIPCHK(ghc.alloc(maxT));
+    }
+    if (!use_imp && wide && !gx32.p) {
+      HIPCHK(gx32.alloc(maxT * k));
+      HIPCHK(gexc32.alloc(maxT));
+      HIPCHK(ghc.alloc(maxT));
+    }
+    if (!use_imp && !gq0.p) {
+      HIPCHK(gq0.alloc(maxT));
+      HIPCHK(gq1.alloc(maxT));
+    }
+    if (!state_alloc) {
+      state_alloc = true;
       HIPCHK(grvs.alloc(maxV));
       HIPCHK(gexv.alloc(maxV));
       HIPCHK(grem.alloc(maxV));
       HIPCHK(ghv.alloc(maxV));
       HIPCHK(ghmin.alloc(maxV));
-      HIPCHK(ghc.alloc(maxT));
       HIPCHK(gl0.alloc(maxV));
       HIPCHK(gl1.alloc(maxV));
       HIPCHK(glc0.alloc(maxV));
       HIPCHK(glc1.alloc(maxV));
-      HIPCHK(gq0.alloc(maxT));
-      HIPCHK(gq1.alloc(maxT));
       HIPCHK(garc.alloc(maxV));
       HIPCHK(gastop.alloc(maxV));
       HIPCHK(gctl.alloc(1));
@@ -1968,6 +1997,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     G.x = gx.p;
     G.x32 = gx32.p;
     G.exc32 = gexc32.p;
+    G.rec = use_rec ? grec.p : nullptr;
+    G.rec32 = use_imp ? fmem.p + C.cbase * ms : nullptr;
+    G.inflow = ginflow.p;
     G.rvs = grvs.p;
     G.exv = gexv.p;
     G.rem = grem.p;
@@ -1995,7 +2027,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     unsigned launched = 0;
     {
       const unsigned fgrid =
-          coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e), wide));
+          coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e), wide, use_rec,
+                                         use_imp));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
       launched = fgrid;
@@ -2016,6 +2049,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                   "grid max flow of component %u hit %u rounds", ci, hc.rounds);
     sweeps += hc.rounds;
     R->flow_launches++;
+    R->flow_implicit += use_imp ? 1 : 0;
     *F = (long long)hc.sink;
     if (G.trace) {
       const uint32_t nr = std::min(hc.rounds, kFlowTraceRounds);
@@ -2053,8 +2087,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
           const Component &C = comps[rq.comp];
           reach_h.resize(C.V);
           CHK(readback(e, reach_h.data(), freach.p, C.V));
-          HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * k,
-                                        C.vbase, freach.p, acc.p, st));
+          HIPCHK(count_cliques_in_reach((uint32_t)C.T, k, fmem.p + C.cbase * ms,
+                                        C.vbase, freach.p, acc.p, st, ms));
           unsigned long long cc = 0;
           CHK(readback(e, &cc, acc.p, 1));
           CHK(search.feed_reach(i, reach_h.data(), cc));
@@ -2098,7 +2132,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         if (hcore[v] >= kpu) hf[fv_of[v]] = 1;
       HIPCHK(mask.alloc(core_n));
       CHK(upload(e, mask.p, hf.data(), core_n));
-      HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st));
+      HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st, ms));
     } else {
       CHK(upload(e, mask.p, hm.data(), n));
       HIPCHK(count_cliques_in_mask(T, k, L.cliques.p, mask.p, acc.p, st));

@@ -149,11 +149,19 @@ __device__ __forceinline__ long long exc_add(int *p, long long d) {
 // int32 for networks whose capacities fit 31 bits).  kByClique: the
 // vertex incidence holds clique ids (from cbase) instead of clique-member
 // slots (from sbase), for networks with 2^32 slots or more; the member's
-// slot is found among the clique's k members.
-template <class XT, bool kByClique_>
+// slot is found among the clique's k members.  kRec: one 32-byte record
+// per clique (int32 words: the k residuals, then its height at word 6 and
+// its excess at word 7) instead of three arrays, so the vertex discharge
+// reads an arc's residual and its clique's height from one line.
+template <class XT, bool kByClique_, bool kRec_ = false>
 struct G {
   static constexpr bool kByClique = kByClique_;
+  static constexpr bool kRec = kRec_;
+  using X = XT;
+  static_assert(!(kByClique_ && kRec_), "records use slot incidence");
   uint32_t k, V, T;
+  uint32_t kmag, ksh;  // div_k's magic and shift for k
+  uint32_t ms;         // words per clique in mem
   int INF;
   uint32_t vbase;
   uint64_t sbase, cbase;
@@ -162,6 +170,7 @@ struct G {
   const uint64_t *ioff;
   const uint32_t *inc;
   XT *x, *exc;
+  int *rec;
   long long *rvs, *exv, *rem;
   int *hv, *hc, *hmin;
   // Double buffers are separate members selected with ?: (an array member
@@ -177,6 +186,45 @@ struct G {
   bool leader;
   uint32_t nblocks, bid;  // workgroups on this problem, this one's index
 };
+
+// Words per clique record (kRec) and the positions of height and excess.
+constexpr uint32_t kRecWords = 8, kRecHeight = 6, kRecExcess = 7;
+constexpr uint32_t kRecMaxK = 6;
+
+// s / k for 2 <= k <= 6 as a multiply-high and a shift: magic m and shift
+// sh with floor(s * m / 2^(32 + sh)) = floor(s / k) for every 32-bit s
+// (checked exhaustively for k = 2..6).
+__device__ __forceinline__ void div_magic(uint32_t k, uint32_t *m, uint32_t *sh) {
+  *m = k == 2 ? 0x80000000u : k == 4 ? 0x40000000u : k == 5 ? 0xCCCCCCCDu : 0xAAAAAAABu;
+  *sh = k == 3 ? 1u : (k == 5 || k == 6) ? 2u : 0u;
+}
+template <class Gx>
+__device__ __forceinline__ uint32_t div_k(const Gx &g, uint32_t s) {
+  return __umulhi(s, g.kmag) >> g.ksh;
+}
+
+// Residual word of member j of clique c, the clique's height and excess.
+template <class Gx>
+__device__ __forceinline__ auto &xcj(const Gx &g, uint64_t c, uint32_t j) {
+  if constexpr (Gx::kRec) return g.rec[c * kRecWords + j];
+  else return g.x[c * g.k + j];
+}
+template <class Gx>
+__device__ __forceinline__ int &hcl(const Gx &g, uint64_t c) {
+  if constexpr (Gx::kRec) return g.rec[c * kRecWords + kRecHeight];
+  else return g.hc[c];
+}
+template <class Gx>
+__device__ __forceinline__ auto &excl(const Gx &g, uint64_t c) {
+  if constexpr (Gx::kRec) return g.rec[c * kRecWords + kRecExcess];
+  else return g.exc[c];
+}
+// Residual word by its index a (as arc_at returns it).
+template <class Gx>
+__device__ __forceinline__ auto &xa(const Gx &g, uint64_t a) {
+  if constexpr (Gx::kRec) return g.rec[a];
+  else return g.x[a];
+}
 
 // Barrier over the workgroups working on one problem.
 template <class Gx>
@@ -257,18 +305,31 @@ __device__ __forceinline__ void chunk_of(const Gx &g, int b, uint32_t count,
   *hi = *lo + kGridChunk < e ? *lo + kGridChunk : e;
 }
 
-// Slot (clique * k + member position, from the component's first clique)
-// of incidence entry p of vertex v.
+// Clique of a residual word index below 2^32 (slot incidence).
 template <class Gx>
-__device__ __forceinline__ uint64_t slot_at(const Gx &g, uint64_t p, uint32_t v) {
+__device__ __forceinline__ uint32_t clique_of_arc(const Gx &g, uint32_t a) {
+  static_assert(!Gx::kByClique, "slot incidence only");
+  if constexpr (Gx::kRec) return a / kRecWords;
+  else return div_k(g, a);
+}
+
+// Incidence entry p of vertex v: the index of its residual word (for xa)
+// and its clique (local ids).
+template <class Gx>
+__device__ __forceinline__ uint64_t arc_at(const Gx &g, uint64_t p, uint32_t v,
+                                           uint32_t *c) {
   if constexpr (Gx::kByClique) {
-    const uint64_t base = ((uint64_t)g.inc[p] - g.cbase) * g.k;
+    *c = (uint32_t)((uint64_t)g.inc[p] - g.cbase);
+    const uint64_t mbase = (uint64_t)*c * g.ms;
     const uint32_t gv = v + g.vbase;
     uint32_t j = 0;
-    while (j + 1 < g.k && g.mem[base + j] != gv) ++j;  // v is one of the k
-    return base + j;
+    while (j + 1 < g.k && g.mem[mbase + j] != gv) ++j;  // v is one of the k
+    return (uint64_t)*c * g.k + j;
   } else {
-    return g.inc[p] - g.sbase;
+    const uint32_t s = (uint32_t)((uint64_t)g.inc[p] - g.sbase);
+    *c = div_k(g, s);
+    if constexpr (Gx::kRec) return (uint64_t)*c * kRecWords + (s - *c * g.k);
+    else return s;
   }
 }
 
@@ -279,7 +340,7 @@ template <class Gx>
 __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64_t arc_reset) {
   GridFlowCtl *ctl = g.ctl;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
-  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.hc[c] = g.INF;
+  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) hcl(g, c) = g.INF;
   if (g.leader) {
     ctl->lpack[0] = 0;
     ctl->lpack[1] = 0;
@@ -321,14 +382,14 @@ __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64
           const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
           f[it] = false;
           cs[it] = (uint32_t)c;
-          if (c >= g.T || g.hc[c] != g.INF) continue;
-          const uint64_t base = c * g.k;
+          if (c >= g.T || hcl(g, c) != g.INF) continue;
+          const uint64_t base = c * g.ms;
           for (uint32_t j = 0; j < g.k; ++j)
-            if (g.hv[g.mem[base + j] - g.vbase] == d && g.U + g.x[base + j] > 0) {
+            if (g.hv[g.mem[base + j] - g.vbase] == d && g.U + xcj(g, c, j) > 0) {
               f[it] = true;
               break;
             }
-          if (f[it]) g.hc[c] = d + 1;
+          if (f[it]) hcl(g, c) = d + 1;
         }
         block_append(g.app, &ctl->cqn[cb], gcq(g, cb), f, cs);
       }
@@ -345,13 +406,13 @@ __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64
         f[it] = false;
         cs[it] = 0;
         if (p >= hi) continue;
-        const uint64_t s = slot_at(g, p, v);
-        const uint32_t c = (uint32_t)(s / g.k);
+        uint32_t c;
+        const uint64_t a = arc_at(g, p, v, &c);
         // a plain load filters: a stale INF only costs a failed CAS; most
         // cliques met after the first level are labelled already
-        if (g.hc[c] != g.INF) continue;
-        if (g.U + g.x[s] <= 0) continue;
-        f[it] = atomicCAS(&g.hc[c], g.INF, d + 1) == g.INF;
+        if (hcl(g, c) != g.INF) continue;
+        if (g.U + xa(g, a) <= 0) continue;
+        f[it] = atomicCAS(&hcl(g, c), g.INF, d + 1) == g.INF;
         cs[it] = c;
       }
       wave_append(&ctl->cqn[cb], gcq(g, cb), f, cs);
@@ -369,8 +430,9 @@ __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64
           const uint64_t p = p0 + lane();
           bool hit = false;
           if (p < e) {
-            const uint64_t s = slot_at(g, p, v);
-            hit = g.hc[s / g.k] == d + 1 && (long long)(g.k - 1) * g.U - g.x[s] > 0;
+            uint32_t c;
+            const uint64_t a = arc_at(g, p, v, &c);
+            hit = hcl(g, c) == d + 1 && (long long)(g.k - 1) * g.U - xa(g, a) > 0;
           }
           if (__ballot(hit)) {
             if (lane() == 0) {
@@ -384,9 +446,9 @@ __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64
     } else
     for (uint32_t i = g.gtid; i < nc; i += g.gthreads) {
       const uint32_t c = gcq(g, cb)[i];
-      const uint64_t base = (uint64_t)c * g.k;
+      const uint64_t base = (uint64_t)c * g.ms;
       for (uint32_t j = 0; j < g.k; ++j) {
-        if ((long long)(g.k - 1) * g.U - g.x[base + j] <= 0) continue;
+        if ((long long)(g.k - 1) * g.U - xcj(g, c, j) <= 0) continue;
         const uint32_t v = g.mem[base + j] - g.vbase;
         if (g.hv[v] != g.INF) continue;  // plain filter, as above
         if (atomicCAS(&g.hv[v], g.INF, d + 2) == g.INF) list_push(g, 1 - vb, v);
@@ -417,7 +479,7 @@ __device__ __forceinline__ unsigned long long global_relabel(const Gx &g, uint64
 #pragma unroll
     for (int it = 0; it < kScan; ++it) {
       const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
-      on[it] = c < g.T && g.exc[c] > 0 && g.hc[c] < g.INF;
+      on[it] = c < g.T && excl(g, c) > 0 && hcl(g, c) < g.INF;
       cs[it] = (uint32_t)c;
       act += on[it];
     }
@@ -443,6 +505,8 @@ template <class Gx>
 __device__ __forceinline__ void flow_setup(Gx &g, const FlowProblem &P,
                                            const GridFlowState &S, long long U) {
   g.k = P.k;
+  div_magic(P.k, &g.kmag, &g.ksh);
+  g.ms = P.mstride ? P.mstride : P.k;
   g.V = P.V;
   g.T = P.T;
   // "unreachable" height: above every finite distance to the sink, which
@@ -457,7 +521,9 @@ __device__ __forceinline__ void flow_setup(Gx &g, const FlowProblem &P,
   g.mem = P.mem;
   g.ioff = P.ioff;
   g.inc = P.inc;
-  if constexpr (Gx::kByClique) {
+  if constexpr (Gx::kRec) {
+    g.rec = S.rec;
+  } else if constexpr (Gx::kByClique) {
     g.x = S.x32;
     g.exc = S.exc32;
   } else {
@@ -489,14 +555,19 @@ __device__ __forceinline__ void flow_setup(Gx &g, const FlowProblem &P,
 // batched kernel re-reads them and keeps its registers for two workgroups
 // per CU.
 template <bool kCacheArcs, class Gx>
-__device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
+__device__ __forceinline__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long U, uint32_t max_rounds,
                          long long *lds_ex, bool lds_v) {
   GridFlowCtl *ctl = S.ctl;
   const uint64_t slots = (uint64_t)g.T * g.k;
 
-  for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
-  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
+  if constexpr (Gx::kRec) {
+    // residuals and excess start at zero; heights come from the relabel
+    for (uint64_t w = g.gtid; w < (uint64_t)g.T * kRecWords; w += g.gthreads) g.rec[w] = 0;
+  } else {
+    for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
+    for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
+  }
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
     g.rvs[v] = (long long)P.cd[v] * U;
     g.exv[v] = cap_t;
@@ -549,20 +620,20 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
       const uint64_t i = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
       bool keep = false;
       const uint32_t c = i < nwl ? wl[i] : 0;
-      long long e = i < nwl ? g.exc[c] : 0;
-      const int hcc = i < nwl ? g.hc[c] : g.INF;
+      long long e = i < nwl ? (long long)excl(g, c) : 0;
+      const int hcc = i < nwl ? hcl(g, c) : g.INF;
       if (e > 0 && hcc < g.INF) {
         ++act;
-        const uint64_t base = (uint64_t)c * g.k;
+        const uint64_t base = (uint64_t)c * g.ms;
         int hmin = g.INF;
         for (uint32_t j = 0; j < g.k && e > 0; ++j) {
-          const long long res = U + g.x[base + j];
+          const long long res = U + xcj(g, c, j);
           if (res <= 0) continue;
           const uint32_t v = g.mem[base + j] - g.vbase;
           const int hvv = g.hv[v];  // heights change only in phase 4
           if (hvv < hcc) {
             const long long d = e < res ? e : res;
-            g.x[base + j] -= d;
+            xcj(g, c, j) -= d;
             e -= d;
             if (lds_v)
               atomicAdd((unsigned long long *)&lds_ex[v], (unsigned long long)d);
@@ -573,15 +644,16 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
             hmin = min(hmin, hvv);
           }
         }
-        g.exc[c] = e;
+        excl(g, c) = e;
         if (e > 0) {
           // all lower members are saturated; recompute over residual arcs
           hmin = g.INF;
           for (uint32_t j = 0; j < g.k; ++j)
-            if (U + g.x[base + j] > 0)
+            if (U + xcj(g, c, j) > 0)
               hmin = min(hmin, g.hv[g.mem[base + j] - g.vbase]);
-          g.hc[c] = hmin >= g.INF - 1 ? g.INF : hmin + 1;
-          keep = g.hc[c] < g.INF;
+          const int hn = hmin >= g.INF - 1 ? g.INF : hmin + 1;
+          hcl(g, c) = hn;
+          keep = hn < g.INF;
         }
       }
       keeps[it] = keep;
@@ -651,7 +723,7 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
       const int hvv = g.hv[v];
       int hm = g.INF;
       long long radm = 0;
-      long long rav[kCacheArcs ? kChunkSteps : 1];
+      typename Gx::X rav[kCacheArcs ? kChunkSteps : 1];
       uint32_t sv[kCacheArcs ? kChunkSteps : 1];
       if constexpr (kCacheArcs) {
 #pragma unroll
@@ -660,24 +732,26 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
           sv[it] = 0;
           const uint64_t p = lo + (uint64_t)it * kWave + lane();
           if (p >= hi) continue;
-          const uint64_t s = slot_at(g, p, v);
-          const long long res = (long long)(g.k - 1) * U - g.x[s];
+          uint32_t c;
+          const uint64_t a = arc_at(g, p, v, &c);
+          const long long res = (long long)(g.k - 1) * U - xa(g, a);
           if (res <= 0) continue;
-          const int hcc = g.hc[s / g.k];
+          const int hcc = hcl(g, c);
           if (hcc < hvv) {
             radm += res;
-            rav[it] = res;
-            sv[it] = (uint32_t)s;
+            rav[it] = (typename Gx::X)res;
+            sv[it] = (uint32_t)a;
           } else {
             hm = min(hm, hcc);
           }
         }
       } else {
         for (uint64_t p = lo + lane(); p < hi; p += kWave) {
-          const uint64_t s = slot_at(g, p, v);
-          const long long res = (long long)(g.k - 1) * U - g.x[s];
+          uint32_t c;
+          const uint64_t a = arc_at(g, p, v, &c);
+          const long long res = (long long)(g.k - 1) * U - xa(g, a);
           if (res <= 0) continue;
-          const int hcc = g.hc[s / g.k];
+          const int hcc = hcl(g, c);
           if (hcc < hvv)
             radm += res;
           else
@@ -708,16 +782,18 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
         const uint64_t p0 = lo + (uint64_t)it * kWave;
         if (p0 >= hi || before >= avail) continue;  // wave-uniform
         long long ra = 0;
-        uint64_t s = 0;
+        uint64_t a = 0;
+        uint32_t c = 0;
         if constexpr (kCacheArcs) {
           ra = rav[it];
-          s = sv[it];
+          a = sv[it];
+          c = clique_of_arc(g, (uint32_t)a);
         } else {
           const uint64_t p = p0 + lane();
           if (p < hi) {
-            s = slot_at(g, p, v);
-            const long long res = (long long)(g.k - 1) * U - g.x[s];
-            if (res > 0 && g.hc[s / g.k] < hvv) ra = res;
+            a = arc_at(g, p, v, &c);
+            const long long res = (long long)(g.k - 1) * U - xa(g, a);
+            if (res > 0 && hcl(g, c) < hvv) ra = res;
           }
         }
         long long incl = ra;
@@ -730,9 +806,9 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
         long long take = avail - start;
         take = take < 0 ? 0 : (take < ra ? take : ra);
         if (take > 0) {
-          g.x[s] += take;
-          fc[it] = (uint32_t)(s / g.k);
-          fresh[it] = exc_add(&g.exc[fc[it]], take) == 0;
+          xa(g, a) += take;
+          fc[it] = c;
+          fresh[it] = exc_add(&excl(g, c), take) == 0;
         }
         before += __shfl(incl, kWave - 1, kWave);
       }
@@ -818,10 +894,610 @@ __device__ void flow_run(Gx &g, const FlowProblem &P, const GridFlowState &S,
   }
 }
 
-// The whole-GPU max flow: Gx = G<long long, false> for networks of fewer
-// than 2^32 clique-member slots, G<int, true> beyond (clique-id incidence,
-// 32-bit residuals; the arcs are re-read in the discharge's second pass to
-// keep registers for the slot lookup).
+// ---------------------------------------------------------------------------
+// Implicit clique nodes.  A clique node's excess is always zero and its
+// height is derived, not stored: h(c) = 1 + the lowest height among the
+// members its residual arcs c->w (U + x > 0) reach, INF if none.  A vertex
+// v discharging into c therefore pushes straight through it: flow v->c
+// (x[v] += t) and c->w (x[w] -= t) to the lowest members w in one step,
+// so there is no clique phase, no clique work list and no clique height or
+// excess array.  Each clique is one record of rs words: its k member ids,
+// then its k residual words x (int32; a residual stays in [-U, (k-1)U]).
+//
+// A round: (A) vertices push to the sink and post their discharge windows;
+// (B) chunked discharge: pass 1 sums per arc what can go through the
+// clique (min of the arc's residual and the residual to the clique's
+// lowest members), one atomic reserves that much of the vertex's excess,
+// pass 2 pushes it, taking each c->w residual with a compare-and-swap so no
+// arc is ever over-used (a push that finds less left returns the rest to
+// the vertex); what reaches a member is summed in LDS per workgroup and
+// added to its inflow; (D) vertices keep the remainder plus inflow as
+// excess, move their current arc, and relabel after a whole-list scan.
+//
+// Heights stay a valid labelling.  Pushes go from v to members two levels
+// down (h(w) <= h(v) - 2, h(c) = h(w) + 1).  Flow into v through c makes
+// v->c residual again with h(c) = h(v) + 1, so a vertex that received
+// inflow in a round relabels to at most h(v) + 2.  Vertex heights change
+// only in (D) and clique heights never fall (a clique gains a residual arc
+// c->u only when u pushes into it from above), so an arc skipped as not
+// admissible stays so until the vertex relabels, as in the explicit
+// network.  The answer never rests on the labels: the solve ends when an
+// exact BFS from the sink finds no vertex with excess that reaches it.
+
+template <bool kByClique_>
+struct H {
+  static constexpr bool kByClique = kByClique_;
+  uint32_t k, V, T, rs;
+  uint32_t kmag, ksh;
+  int INF;
+  uint32_t vbase;
+  uint64_t sbase, cbase;
+  long long U;
+  uint32_t *rec;  // records of this problem's cliques (local clique ids)
+  const uint64_t *ioff;
+  const uint32_t *inc;
+  long long *rvs, *exv, *rem, *inflow;
+  int *hv, *hmin;
+  uint32_t *list0, *list1;
+  uint64_t *lchunk0, *lchunk1;
+  uint64_t *arc;
+  uint32_t *astop;
+  uint32_t *app;
+  int *lds_h;           // LDS copy of hv (V entries) or null
+  long long *lds_in;    // LDS inflow accumulator (V entries) or null
+  GridFlowCtl *ctl;
+  uint32_t gtid, gthreads, gwave, gwaves;
+  bool leader;
+  uint32_t nblocks, bid;
+};
+
+constexpr uint32_t kImpMaxK = 6;
+
+// Record stride in words for clique size k: 2k words rounded up to 4, 8
+// or 16, so a record never crosses a 64-byte line.
+__host__ __device__ inline uint32_t imp_stride(uint32_t k) {
+  return 2 * k <= 4 ? 4u : 2 * k <= 8 ? 8u : 16u;
+}
+
+// Clique (local id) and the member position of incidence entry p of v.
+template <class Hx>
+__device__ __forceinline__ uint32_t imp_arc(const Hx &g, uint64_t p, uint32_t v,
+                                            uint32_t *jv) {
+  if constexpr (Hx::kByClique) {
+    const uint32_t c = (uint32_t)((uint64_t)g.inc[p] - g.cbase);
+    const uint32_t *r = g.rec + (uint64_t)c * g.rs;
+    const uint32_t gv = v + g.vbase;
+    uint32_t j = 0;
+    while (j + 1 < g.k && r[j] != gv) ++j;
+    *jv = j;
+    return c;
+  } else {
+    const uint32_t s = (uint32_t)((uint64_t)g.inc[p] - g.sbase);
+    const uint32_t c = __umulhi(s, g.kmag) >> g.ksh;
+    *jv = s - c * g.k;
+    return c;
+  }
+}
+
+// A record's k member ids (local) and residual words.  The arrays are
+// indexed by unrolled constants only (a run-time index would put them in
+// scratch memory); pick() selects by a run-time position.
+template <class Hx>
+__device__ __forceinline__ void imp_load(const Hx &g, uint32_t c, uint32_t (&m)[kImpMaxK],
+                                         int (&x)[kImpMaxK]) {
+  const uint32_t *r = g.rec + (uint64_t)c * g.rs;
+#pragma unroll
+  for (uint32_t j = 0; j < kImpMaxK; ++j) {
+    m[j] = 0;
+    x[j] = 0;
+  }
+  if (g.rs == 8) {  // k = 3 or 4
+    const uint4 a = *reinterpret_cast<const uint4 *>(r);
+    const uint4 b = *reinterpret_cast<const uint4 *>(r + 4);
+    const bool k3 = g.k == 3;
+    m[0] = a.x - g.vbase;
+    m[1] = a.y - g.vbase;
+    m[2] = a.z - g.vbase;
+    m[3] = k3 ? 0u : a.w - g.vbase;
+    x[0] = (int)(k3 ? a.w : b.x);
+    x[1] = (int)(k3 ? b.x : b.y);
+    x[2] = (int)(k3 ? b.y : b.z);
+    x[3] = k3 ? 0 : (int)b.w;
+  } else if (g.rs == 4) {  // k = 2
+    const uint4 a = *reinterpret_cast<const uint4 *>(r);
+    m[0] = a.x - g.vbase;
+    m[1] = a.y - g.vbase;
+    x[0] = (int)a.z;
+    x[1] = (int)a.w;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kImpMaxK; ++j) {
+      if (j < g.k) {
+        m[j] = r[j] - g.vbase;
+        x[j] = (int)r[g.k + j];
+      }
+    }
+  }
+}
+
+// Residual word j of clique c, read from the record (a select over the
+// loaded words would be turned into a scratch-memory lookup).
+template <class Hx>
+__device__ __forceinline__ int imp_x(const Hx &g, uint32_t c, uint32_t j) {
+  return (int)g.rec[(uint64_t)c * g.rs + g.k + j];
+}
+
+template <class Hx>
+__device__ __forceinline__ int imp_h(const Hx &g, uint32_t v) {
+  return g.lds_h ? g.lds_h[v] : g.hv[v];
+}
+
+// Copies hv into the workgroup's LDS copy (if any); every thread calls it.
+template <class Hx>
+__device__ __forceinline__ void imp_stage_h(const Hx &g) {
+  if (!g.lds_h) return;
+  for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) g.lds_h[i] = g.hv[i];
+  __syncthreads();
+}
+
+// Exact BFS distances to the sink over the residual network, vertex levels
+// only (a clique sits one above its lowest residual member); returns the
+// vertices with excess that reach the sink.
+template <class Hx>
+__device__ __forceinline__ unsigned long long imp_relabel(const Hx &g, uint64_t arc_reset) {
+  GridFlowCtl *ctl = g.ctl;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
+  if (g.leader) {
+    ctl->lpack[0] = 0;
+    ctl->lpack[1] = 0;
+    ctl->active = 0;
+  }
+  gsync(g);
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads)
+    if (g.rvs[v] > 0) {
+      g.hv[v] = 1;
+      list_push(g, 0, v);
+    }
+  gsync(g);
+  const long long kU = (long long)(g.k - 1) * g.U;
+  const uint64_t arcs = (uint64_t)g.T * g.k;
+  int vb = 0, d = 1;
+  for (;;) {
+    const unsigned long long pack = ald(&ctl->lpack[vb]);
+    const uint32_t nv = (uint32_t)(pack >> 40);
+    if (nv == 0) break;
+    const uint64_t nch = pack & kMask40;
+    if (g.leader) ctl->lpack[1 - vb] = 0;
+    gsync(g);
+    imp_stage_h(g);
+    if (nch * kGridChunk * 4 >= arcs) {
+      // pull: each unlabelled vertex looks for a residual arc to a clique
+      // with a residual arc to a vertex at d, and stops at the first
+      for (uint32_t v = g.gwave; v < g.V; v += g.gwaves) {
+        if (imp_h(g, v) != g.INF) continue;  // wave-uniform
+        const uint64_t e = g.ioff[v + 1];
+        for (uint64_t p0 = g.ioff[v]; p0 < e; p0 += kWave) {
+          const uint64_t p = p0 + lane();
+          bool hit = false;
+          if (p < e) {
+            uint32_t jv;
+            const uint32_t c = imp_arc(g, p, v, &jv);
+            uint32_t m[kImpMaxK];
+            int x[kImpMaxK];
+            imp_load(g, c, m, x);
+            if (kU - imp_x(g, c, jv) > 0) {
+#pragma unroll
+              for (uint32_t j = 0; j < kImpMaxK; ++j)
+                if (j < g.k && j != jv && g.U + x[j] > 0 && imp_h(g, m[j]) == d) hit = true;
+            }
+          }
+          if (__ballot(hit)) {
+            if (lane() == 0) {
+              g.hv[v] = d + 2;
+              list_push(g, 1 - vb, v);
+            }
+            break;
+          }
+        }
+      }
+    } else {
+      // push: the frontier's arcs label the other members behind them
+      for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
+        uint32_t w;
+        uint64_t lo, hi;
+        chunk_of(g, vb, nv, ci, &w, &lo, &hi);
+        for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+          uint32_t jw;
+          const uint32_t c = imp_arc(g, p, w, &jw);
+          uint32_t m[kImpMaxK];
+          int x[kImpMaxK];
+          imp_load(g, c, m, x);
+          if (g.U + imp_x(g, c, jw) <= 0) continue;
+#pragma unroll
+          for (uint32_t j = 0; j < kImpMaxK; ++j) {
+            if (j >= g.k || j == jw || kU - x[j] <= 0) continue;
+            const uint32_t v = m[j];
+            if (g.hv[v] != g.INF) continue;  // plain filter; the CAS decides
+            if (atomicCAS(&g.hv[v], g.INF, d + 2) == g.INF) list_push(g, 1 - vb, v);
+          }
+        }
+      }
+    }
+    gsync(g);
+    vb = 1 - vb;
+    d += 2;
+  }
+  unsigned long long act = 0;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+    act += g.exv[v] > 0 && g.hv[v] < g.INF;
+    g.arc[v] = arc_reset;
+  }
+  act = wave_sum64(act);
+  if (lane() == 0 && act) atomicAdd(&ctl->active, act);
+  gsync(g);
+  const unsigned long long total = ald(&ctl->active);
+  if (g.leader) {
+    ctl->lpack[0] = 0;
+    ctl->lpack[1] = 0;
+  }
+  gsync(g);
+  return total;
+}
+
+template <class Hx>
+__device__ __forceinline__ void imp_setup(Hx &g, const FlowProblem &P, const GridFlowState &S,
+                                          long long U) {
+  g.k = P.k;
+  div_magic(P.k, &g.kmag, &g.ksh);
+  g.rs = imp_stride(P.k);
+  g.V = P.V;
+  g.T = P.T;
+  // no path to the sink is longer than 2V + 1 vertex levels apart
+  g.INF = (int)(2 * P.V + 3);
+  g.vbase = P.vbase;
+  g.sbase = P.sbase;
+  g.cbase = P.k ? P.sbase / P.k : 0;
+  g.U = U;
+  g.rec = S.rec32;
+  g.ioff = P.ioff;
+  g.inc = P.inc;
+  g.rvs = S.rvs;
+  g.exv = S.exv;
+  g.rem = S.rem;
+  g.inflow = S.inflow;
+  g.hv = S.hv;
+  g.hmin = S.hmin;
+  g.list0 = S.list0;
+  g.list1 = S.list1;
+  g.lchunk0 = S.lchunk0;
+  g.lchunk1 = S.lchunk1;
+  g.arc = S.arc;
+  g.astop = S.astop;
+  g.ctl = S.ctl;
+}
+
+constexpr int kImpSteps = kChunkSteps;  // wave passes per chunk
+
+template <class Hx>
+__device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridFlowState &S,
+                                        long long cap_t, long long U, uint32_t max_rounds) {
+  GridFlowCtl *ctl = S.ctl;
+  const long long kU = (long long)(g.k - 1) * U;
+  // residual words start at zero
+  for (uint64_t c = g.gtid; c < g.T; c += g.gthreads) {
+    uint32_t *r = g.rec + c * g.rs;
+    for (uint32_t j = 0; j < g.k; ++j) r[g.k + j] = 0;
+  }
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+    g.rvs[v] = (long long)P.cd[v] * U;
+    g.exv[v] = cap_t;
+    g.inflow[v] = 0;
+    g.arc[v] = kArcFull;
+  }
+  if (g.leader) {
+    ctl->sink = 0;
+    ctl->rounds = 0;
+    ctl->limit = 0;
+    ctl->relabels = 0;
+  }
+  gsync(g);
+  unsigned long long active = imp_relabel(g, kArcFull);
+  uint32_t rounds = 0, relabels = 1, since = 0;
+  unsigned long long sink = 0, ch_visits = 0;
+  bool limit = false;
+  const uint64_t slots = (uint64_t)g.T * g.k;
+  while (active) {
+    if (rounds >= max_rounds) {
+      limit = true;
+      break;
+    }
+    const unsigned long long tr0 = (S.trace && g.leader) ? wall_clock64() : 0;
+    if (g.leader) {
+      ctl->lpack[0] = 0;
+      ctl->active = 0;
+    }
+    // (A) sink pushes; active vertices post their windows; inflow restarts
+    unsigned long long act = 0;
+    for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+      g.inflow[v] = 0;
+      long long e = g.exv[v];
+      if (e <= 0 || g.hv[v] >= g.INF) {
+        g.rem[v] = e > 0 ? e : 0;  // stranded excess is kept as it is
+        continue;
+      }
+      ++act;
+      const long long r = g.rvs[v];
+      if (r > 0) {
+        const long long d = e < r ? e : r;
+        g.rvs[v] = r - d;
+        e -= d;
+        sink += (unsigned long long)d;
+      }
+      g.exv[v] = e;
+      g.rem[v] = e;
+      g.hmin[v] = g.INF;
+      if (e > 0) {
+        uint32_t first, count;
+        discharge_range(g, v, &first, &count);
+        g.astop[v] = 0xFFFFFFFFu;
+        list_push_n(g, 0, v, count);
+      }
+    }
+    act = wave_sum64(act);
+    if (lane() == 0 && act) atomicAdd(&ctl->active, act);
+    gsync(g);
+    const unsigned long long tr1 = (S.trace && g.leader) ? wall_clock64() : 0;
+    const unsigned long long pack = ald(&ctl->lpack[0]);
+    const uint32_t nv = (uint32_t)(pack >> 40);
+    const uint64_t nch = pack & kMask40;
+    const unsigned long long any = ald(&ctl->active);
+    imp_stage_h(g);
+    if (g.lds_in) {
+      for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) g.lds_in[i] = 0;
+      __syncthreads();
+    }
+    // (B) chunked discharge through the cliques
+    for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
+      uint32_t v;
+      uint64_t lo, hi;
+      uint32_t vchunk;
+      chunk_of(g, 0, nv, ci, &v, &lo, &hi, true, &vchunk);
+      const int hvv = imp_h(g, v);
+      int hm = g.INF;
+      long long radm = 0;
+      int av[kImpSteps];       // what this lane's arc can pass
+      uint32_t cv[kImpSteps];  // its clique
+      uint32_t tv[kImpSteps];  // lowest member height << 3 | v's position
+#pragma unroll
+      for (int it = 0; it < kImpSteps; ++it) {
+        av[it] = 0;
+        cv[it] = 0;
+        tv[it] = 0;
+        const uint64_t p = lo + (uint64_t)it * kWave + lane();
+        if (p >= hi) continue;
+        uint32_t jv;
+        const uint32_t c = imp_arc(g, p, v, &jv);
+        uint32_t m[kImpMaxK];
+        int x[kImpMaxK];
+        imp_load(g, c, m, x);
+        const long long rvc = kU - imp_x(g, c, jv);
+        if (rvc <= 0) continue;  // saturated: not a residual arc
+        int mh = g.INF;
+        long long through = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kImpMaxK; ++j) {
+          if (j >= g.k) break;
+          const long long rc = U + x[j];
+          if (rc <= 0) continue;
+          const int h = imp_h(g, m[j]);
+          if (h < mh) {
+            mh = h;
+            through = rc;
+          } else if (h == mh) {
+            through += rc;
+          }
+        }
+        const int hcc = mh >= g.INF - 1 ? g.INF : mh + 1;
+        if (hcc < hvv) {
+          const long long a = rvc < through ? rvc : through;
+          radm += a;
+          av[it] = (int)a;
+          cv[it] = c;
+          tv[it] = (uint32_t)mh << 3 | jv;
+        } else {
+          hm = min(hm, hcc);
+        }
+      }
+      radm = (long long)wave_sum64((uint64_t)radm);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) hm = min(hm, __shfl_xor(hm, off, kWave));
+      if (lane() == 0 && hm < g.INF) atomicMin(&g.hmin[v], hm);
+      if (radm == 0) continue;
+      long long avail = 0;
+      if (lane() == 0) {
+        const long long old = (long long)atomicAdd(
+            (unsigned long long *)&g.rem[v], (unsigned long long)(-radm));
+        avail = old < 0 ? 0 : (old < radm ? old : radm);
+      }
+      avail = __shfl(avail, 0, kWave);
+      if (lane() == 0 && avail < radm) atomicMin(&g.astop[v], vchunk);
+      long long before = 0, back = 0;
+#pragma unroll
+      for (int it = 0; it < kImpSteps; ++it) {
+        if (before >= avail) break;  // wave-uniform
+        const long long ra = av[it];
+        long long incl = ra;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const long long t = __shfl_up(incl, off, kWave);
+          if ((int)lane() >= off) incl += t;
+        }
+        const long long start = before + incl - ra;
+        long long take = avail - start;
+        take = take < 0 ? 0 : (take < ra ? take : ra);
+        if (take > 0) {
+          const uint32_t c = cv[it], jv = tv[it] & 7u;
+          const int mh = (int)(tv[it] >> 3);
+          uint32_t *r = g.rec + (uint64_t)c * g.rs;
+          long long left = take;
+          for (uint32_t j = 0; j < g.k && left > 0; ++j) {
+            if (j == jv) continue;
+            const uint32_t w = r[j] - g.vbase;
+            if (imp_h(g, w) != mh) continue;
+            int *xp = reinterpret_cast<int *>(r + g.k + j);
+            int old = __hip_atomic_load(xp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+              const long long room = U + old;
+              if (room <= 0) break;
+              const int t = (int)(left < room ? left : room);
+              const int prev = atomicCAS(xp, old, old - t);
+              if (prev == old) {
+                left -= t;
+                if (g.lds_in)
+                  atomicAdd((unsigned long long *)&g.lds_in[w], (unsigned long long)t);
+                else
+                  atomicAdd((unsigned long long *)&g.inflow[w], (unsigned long long)t);
+                break;
+              }
+              old = prev;
+            }
+          }
+          const long long got = take - left;
+          if (got > 0)
+            atomicAdd(reinterpret_cast<int *>(r + g.k + jv), (int)got);
+          back += left;
+        }
+        before += __shfl(incl, kWave - 1, kWave);
+      }
+      // what a push could not pass on stays v's excess; rem may already be
+      // below zero (the reservation can overshoot), so it joins the inflow
+      back = (long long)wave_sum64((uint64_t)back);
+      if (lane() == 0 && back)
+        atomicAdd((unsigned long long *)&g.inflow[v], (unsigned long long)back);
+    }
+    if (g.lds_in) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) {
+        const long long a = g.lds_in[i];
+        if (a) atomicAdd((unsigned long long *)&g.inflow[i], (unsigned long long)a);
+      }
+    }
+    gsync(g);
+    const unsigned long long tr2 = (S.trace && g.leader) ? wall_clock64() : 0;
+    // (D) current arcs and relabels of the listed vertices (their excess
+    // left in rem), then every vertex's excess = remainder + inflow
+    for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
+      const uint32_t v = g.list0[i];
+      const long long r = g.rem[v];
+      uint32_t first, count;
+      discharge_range(g, v, &first, &count);
+      const uint32_t nchv = (uint32_t)nchunks(g, v);
+      const uint32_t win = (uint32_t)(g.arc[v] >> 32);
+      uint32_t cur, nwin;
+      if (r > 0 && first == 0 && count == nchv) {
+        const int hm = g.hmin[v];
+        int hn = hm >= g.INF - 1 ? g.INF : hm + 1;
+        // flow that came in through a clique may have re-opened an arc to
+        // it, and such a clique sits one above v (inflow also holds
+        // excess a push returned; the bound then only slows the rise)
+        if (g.inflow[v] > 0 && hn > g.hv[v] + 2) hn = g.hv[v] + 2;
+        g.hv[v] = hn;
+        cur = 0;
+        nwin = kArcWindowRelabel;
+      } else if (r > 0) {
+        cur = first + count;
+        nwin = min(2 * win, kArcWindowMax);
+      } else {
+        const uint32_t stop = g.astop[v];
+        cur = stop < first + count ? stop : first + count;
+        const uint32_t used = cur - first + 1;
+        nwin = min(max(2 * used, kArcWindow0), kArcWindowMax);
+      }
+      g.arc[v] = (uint64_t)cur | (uint64_t)nwin << 32;
+    }
+    for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+      const long long r = g.rem[v];
+      g.exv[v] = (r > 0 ? r : 0) + g.inflow[v];
+    }
+    gsync(g);
+    const unsigned long long tr3 = (S.trace && g.leader) ? wall_clock64() : 0;
+    const bool after_relabel = since == 0;
+    ++rounds;
+    ++since;
+    if (any == 0 || since >= relabel_rounds(slots)) {
+      active = imp_relabel(g, kArcAfterRelabel);
+      ++relabels;
+      since = 0;
+    }
+    ch_visits += nch;
+    if (S.trace && g.leader && rounds - 1 < S.trace_cap) {
+      unsigned long long *t = S.trace + 8ull * (rounds - 1);
+      t[0] = 0;
+      t[1] = nv;
+      t[2] = nch;
+      t[3] = any | (after_relabel ? 1ull << 63 : 0);
+      t[4] = tr1 - tr0;  // (A)
+      t[5] = tr2 - tr1;  // (B)
+      t[6] = tr3 - tr2;  // (D)
+      t[7] = wall_clock64() - tr3;
+    }
+  }
+  if (!limit) imp_relabel(g, kArcFull);
+  sink = wave_sum64(sink);
+  if (lane() == 0 && sink) atomicAdd(&ctl->sink, sink);
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) S.reach[v] = g.hv[v] < g.INF ? 1 : 0;
+  if (g.leader) {
+    ctl->rounds = rounds;
+    ctl->limit = limit ? 1 : 0;
+    ctl->relabels = relabels;
+    ctl->clique_visits = 0;
+    ctl->chunk_visits = ch_visits;
+  }
+}
+
+template <bool kByClique>
+__device__ __forceinline__ void imp_grid_body(FlowProblem P, GridFlowState S, long long cap_t,
+                                              long long U, uint32_t max_rounds) {
+  H<kByClique> g;
+  imp_setup(g, P, S, U);
+  g.gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  g.gthreads = gridDim.x * blockDim.x;
+  g.gwave = g.gtid / kWave;
+  g.gwaves = g.gthreads / kWave;
+  g.leader = g.gtid == 0;
+  g.nblocks = gridDim.x;
+  g.bid = blockIdx.x;
+  __shared__ uint32_t app[kMaxFlowWaves + 1];
+  g.app = app;
+  extern __shared__ long long lds_imp[];
+  const bool lds = g.V <= kGridLdsVertices;
+  g.lds_in = lds ? lds_imp : nullptr;
+  g.lds_h = lds ? reinterpret_cast<int *>(lds_imp + g.V) : nullptr;
+  imp_run(g, P, S, cap_t, U, max_rounds);
+}
+
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow_imp(FlowProblem P,
+                                                       GridFlowState S,
+                                                       long long cap_t, long long U,
+                                                       uint32_t max_rounds) {
+  imp_grid_body<false>(P, S, cap_t, U, max_rounds);
+}
+
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow_imp_wide(FlowProblem P,
+                                                            GridFlowState S,
+                                                            long long cap_t, long long U,
+                                                            uint32_t max_rounds) {
+  imp_grid_body<true>(P, S, cap_t, U, max_rounds);
+}
+
+
+// The whole-GPU max flow: Gx = G<int, false, true> (clique records, the
+// discharge's admissible arcs kept in registers between its two passes)
+// for networks of fewer than 2^32 clique-member slots whose unit fits
+// 32-bit residuals, G<long long, false> (int64 arrays) when it does not,
+// G<int, true> for 2^32 slots or more (clique-id incidence, 32-bit
+// residuals).  The last two re-read the arcs in the second pass: their
+// lookups leave no registers for the cache.
 template <class Gx, bool kCacheArcs>
 __device__ __forceinline__ void grid_flow_body(FlowProblem P, GridFlowState S,
                                                long long cap_t, long long U,
@@ -851,7 +1527,15 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
                                                    GridFlowState S,
                                                    long long cap_t, long long U,
                                                    uint32_t max_rounds) {
-  grid_flow_body<G<long long, false>, true>(P, S, cap_t, U, max_rounds);
+  grid_flow_body<G<long long, false>, false>(P, S, cap_t, U, max_rounds);
+}
+
+// Clique records (int32 residuals, height and excess in one 32-byte line).
+__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow_rec(FlowProblem P,
+                                                       GridFlowState S,
+                                                       long long cap_t, long long U,
+                                                       uint32_t max_rounds) {
+  grid_flow_body<G<int, false, true>, true>(P, S, cap_t, U, max_rounds);
 }
 
 // 512-thread workgroups: the slot lookups need more registers than the
@@ -904,7 +1588,8 @@ __global__ void k_flow_batch_count(const FlowBatchItem *__restrict__ items,
   for (uint32_t t = threadIdx.x; t < it.P.T; t += blockDim.x) {
     bool all = true;
     for (uint32_t j = 0; j < it.P.k; ++j)
-      all &= it.S.reach[it.P.mem[(uint64_t)t * it.P.k + j] - it.P.vbase] != 0;
+      all &= it.S.reach[it.P.mem[(uint64_t)t * (it.P.mstride ? it.P.mstride : it.P.k) + j] -
+                        it.P.vbase] != 0;
     c += all;
   }
   c = wave_sum64(c);
@@ -952,6 +1637,8 @@ hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
   return hipGetLastError();
 }
 
+uint32_t imp_record_words(uint32_t k) { return imp_stride(k); }
+
 unsigned flow_batch_capacity(uint32_t max_v, int device) {
   const uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
   int per_cu = 0, cus = 0;
@@ -966,11 +1653,17 @@ unsigned flow_batch_capacity(uint32_t max_v, int device) {
   return (unsigned)cus;
 }
 
-unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks, bool wide) {
+unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks, bool wide,
+                          bool rec, bool imp) {
   int per_cu = 0, cus = 0;
-  const size_t lds = P.V <= kGridLdsVertices ? (size_t)P.V * sizeof(long long) : 0;
-  const void *fn = wide ? reinterpret_cast<const void *>(k_grid_flow_wide)
-                        : reinterpret_cast<const void *>(k_grid_flow);
+  const size_t lds = P.V <= kGridLdsVertices
+                         ? (size_t)P.V * (imp ? 12 : sizeof(long long))
+                         : 0;
+  const void *fn = imp   ? (P.by_clique ? reinterpret_cast<const void *>(k_grid_flow_imp_wide)
+                                        : reinterpret_cast<const void *>(k_grid_flow_imp))
+                   : wide ? reinterpret_cast<const void *>(k_grid_flow_wide)
+                   : rec  ? reinterpret_cast<const void *>(k_grid_flow_rec)
+                          : reinterpret_cast<const void *>(k_grid_flow);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, fn, wide ? kGridFlowWideThreads : kGridFlowThreads, lds) != hipSuccess ||
       per_cu < 1)
@@ -1002,8 +1695,18 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
   FlowProblem p = P;
   GridFlowState st = S;
   void *args[] = {&p, &st, &cap_t, &unit, &max_rounds};
-  const void *fn = S.x32 ? reinterpret_cast<const void *>(k_grid_flow_wide)
-                        : reinterpret_cast<const void *>(k_grid_flow);
+  if (S.rec32) {
+    if (P.k < 2 || P.k > kImpMaxK) return hipErrorInvalidValue;
+    const size_t ilds = P.V <= kGridLdsVertices ? (size_t)P.V * 12 : 0;
+    const void *ifn = P.by_clique ? reinterpret_cast<const void *>(k_grid_flow_imp_wide)
+                                  : reinterpret_cast<const void *>(k_grid_flow_imp);
+    return launch_grid_shrinking(ifn, grid, kGridFlowThreads, args, ilds, s);
+  }
+  if (S.rec && (P.k > kRecMaxK || (uint64_t)P.T * kRecWords > 0xFFFFFFFFull))
+    return hipErrorInvalidValue;
+  const void *fn = S.x32   ? reinterpret_cast<const void *>(k_grid_flow_wide)
+                   : S.rec ? reinterpret_cast<const void *>(k_grid_flow_rec)
+                           : reinterpret_cast<const void *>(k_grid_flow);
   return launch_grid_shrinking(fn, grid, S.x32 ? kGridFlowWideThreads : kGridFlowThreads,
                                args, lds, s);
 }

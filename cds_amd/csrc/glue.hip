@@ -240,12 +240,13 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_count2(
 // Places every core clique in its component's range of the flow arrays:
 // fmem (flow-vertex ids of the members) and cslot[c] = its flow index.
 __device__ __forceinline__ void comp_place(uint64_t c, uint64_t f, uint32_t k,
+                                           uint32_t ms,
                                            const uint32_t *__restrict__ cl,
                                            const uint32_t *__restrict__ fv_of,
                                            uint32_t *__restrict__ fmem,
                                            uint32_t *__restrict__ cslot) {
   cslot[c] = (uint32_t)f;
-  for (uint32_t j = 0; j < k; ++j) fmem[f * k + j] = fv_of[cl[c * k + j]];
+  for (uint32_t j = 0; j < k; ++j) fmem[f * ms + j] = fv_of[cl[c * k + j]];
 }
 
 __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
     const uint32_t *__restrict__ comp_of, const uint32_t *__restrict__ fv_of,
     uint32_t ncomp, const uint64_t *__restrict__ coff,
     uint32_t *__restrict__ ccur, uint32_t *__restrict__ fmem,
-    uint32_t *__restrict__ cslot) {
+    uint32_t *__restrict__ cslot, uint32_t ms) {
   __shared__ uint32_t cnt[kCompLds], cbase[kCompLds];
   if (ncomp > kCompLds) {
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
       const bool in = c < T && kill[c] >= kp;
       const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
       const uint32_t slot = aggregated_reserve(ccur, comp, in);
-      if (in) comp_place(c, coff[comp] + slot, k, cl, fv_of, fmem, cslot);
+      if (in) comp_place(c, coff[comp] + slot, k, ms, cl, fv_of, fmem, cslot);
     }
     return;
   }
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
     for (int it = 0; it < kCompTile; ++it) {
       if (!in[it]) continue;
       const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
-      comp_place(c, coff[comp[it]] + cbase[comp[it]] + local[it], k, cl, fv_of,
+      comp_place(c, coff[comp[it]] + cbase[comp[it]] + local[it], k, ms, cl, fv_of,
                  fmem, cslot);
     }
     __syncthreads();  // cbase is rewritten by the next tile
@@ -321,6 +322,7 @@ constexpr uint32_t kFlowPerThread = kFlowPairs / kFlowThreads;
 // Inserts the tile's pairs into the hash; slot[] / rank[] per pair (rank:
 // position among the tile's pairs of the same vertex).
 __device__ __forceinline__ void flow_tile_hash(const uint32_t *__restrict__ fmem,
+                                               uint32_t k, uint32_t ms,
                                                uint64_t p0, uint32_t pairs,
                                                uint32_t *hkey, uint32_t *hcnt,
                                                uint32_t *slot, uint32_t *rank) {
@@ -328,7 +330,7 @@ __device__ __forceinline__ void flow_tile_hash(const uint32_t *__restrict__ fmem
   for (uint32_t r = 0; r < kFlowPerThread; ++r) {
     const uint32_t p = r * kFlowThreads + threadIdx.x;
     if (p >= pairs) break;
-    const uint32_t x = fmem[p0 + p];
+    const uint32_t x = member_at(fmem, p0 + p, k, ms);
     uint32_t h = (x * 2654435761u) & (kFlowHash - 1);
     for (;;) {  // at most pairs < kFlowHash keys: a free slot exists
       const uint32_t prev = atomicCAS(&hkey[h], kAlive, x);
@@ -342,7 +344,7 @@ __device__ __forceinline__ void flow_tile_hash(const uint32_t *__restrict__ fmem
 
 __global__ __launch_bounds__(kFlowThreads) void k_flow_deg(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem,
-    uint32_t *__restrict__ fcd) {
+    uint32_t *__restrict__ fcd, uint32_t ms) {
   __shared__ uint32_t hkey[kFlowHash];
   __shared__ uint32_t hcnt[kFlowHash];
   const uint64_t tiles = (Tc + kFlowTile - 1) / kFlowTile;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_deg(
     const uint64_t c0 = tile * kFlowTile;
     const uint32_t pairs = (uint32_t)min<uint64_t>(kFlowTile, Tc - c0) * k;
     uint32_t slot[kFlowPerThread], rank[kFlowPerThread];
-    flow_tile_hash(fmem, c0 * k, pairs, hkey, hcnt, slot, rank);
+    flow_tile_hash(fmem, k, ms, c0 * k, pairs, hkey, hcnt, slot, rank);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads)
       if (hcnt[i]) atomicAdd(&fcd[hkey[i]], hcnt[i]);
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_deg(
 __global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem,
     const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
-    uint32_t *__restrict__ finc, uint32_t div) {
+    uint32_t *__restrict__ finc, uint32_t div, uint32_t ms) {
   __shared__ uint32_t hkey[kFlowHash];
   __shared__ uint32_t hcnt[kFlowHash];
   const uint64_t tiles = (Tc + kFlowTile - 1) / kFlowTile;
@@ -379,7 +381,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
     const uint64_t c0 = tile * kFlowTile;
     const uint32_t pairs = (uint32_t)min<uint64_t>(kFlowTile, Tc - c0) * k;
     uint32_t slot[kFlowPerThread], rank[kFlowPerThread];
-    flow_tile_hash(fmem, c0 * k, pairs, hkey, hcnt, slot, rank);
+    flow_tile_hash(fmem, k, ms, c0 * k, pairs, hkey, hcnt, slot, rank);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kFlowHash; i += kFlowThreads)
       if (hcnt[i]) hcnt[i] = atomicAdd(&fcur[hkey[i]], hcnt[i]);  // -> run base
@@ -480,13 +482,14 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *comp_of, const uint32_t *fv_of,
                               uint32_t ncomp, const uint64_t *coff,
                               uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
-                              hipStream_t s) {
+                              hipStream_t s, uint32_t ms) {
+  if (ms == 0) ms = k;
   const unsigned grid = ncomp > kCompLds
                             ? grid_for(T, kCompThreads, 65535)
                             : grid_for(T, kCompThreads * kCompTile, 4096);
   hipLaunchKernelGGL(k_comp_scatter2, dim3(grid), dim3(kCompThreads), 0, s, T, k,
                      cliques, kill, kp, comp_of, fv_of, ncomp, coff, ccur, fmem,
-                     cslot);
+                     cslot, ms);
   return hipGetLastError();
 }
 
@@ -515,14 +518,14 @@ __device__ __forceinline__ void flow_range(uint64_t Tc, uint64_t *lo, uint64_t *
 
 __global__ __launch_bounds__(kFlowThreads) void k_flow_deg_direct(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem, uint32_t nv,
-    uint32_t *__restrict__ fcd) {
+    uint32_t *__restrict__ fcd, uint32_t ms) {
   __shared__ uint32_t cnt[kFlowDirect];
   for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) cnt[i] = 0;
   __syncthreads();
   uint64_t lo, hi;
   flow_range(Tc, &lo, &hi);
   for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads)
-    atomicAdd(&cnt[fmem[p]], 1u);
+    atomicAdd(&cnt[member_at(fmem, p, k, ms)], 1u);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
     if (cnt[i]) atomicAdd(&fcd[i], cnt[i]);
@@ -531,51 +534,55 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_deg_direct(
 __global__ __launch_bounds__(kFlowThreads) void k_flow_fill_direct(
     uint64_t Tc, uint32_t k, const uint32_t *__restrict__ fmem, uint32_t nv,
     const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
-    uint32_t *__restrict__ finc, uint32_t div) {
+    uint32_t *__restrict__ finc, uint32_t div, uint32_t ms) {
   __shared__ uint32_t cnt[kFlowDirect];
   for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) cnt[i] = 0;
   __syncthreads();
   uint64_t lo, hi;
   flow_range(Tc, &lo, &hi);
   for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads)
-    atomicAdd(&cnt[fmem[p]], 1u);
+    atomicAdd(&cnt[member_at(fmem, p, k, ms)], 1u);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
     if (cnt[i]) cnt[i] = atomicAdd(&fcur[i], cnt[i]);  // -> this range's run
   __syncthreads();
   for (uint64_t p = lo * k + threadIdx.x; p < hi * k; p += kFlowThreads) {
-    const uint32_t x = fmem[p];
+    const uint32_t x = member_at(fmem, p, k, ms);
     finc[fioff[x] + atomicAdd(&cnt[x], 1u)] = (uint32_t)(p / div);
   }
 }
 
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
-                                  uint32_t nv, uint32_t *fcd, hipStream_t s) {
+                                  uint32_t nv, uint32_t *fcd, hipStream_t s, uint32_t ms) {
   if (k > 4) return hipErrorInvalidValue;
+  if (ms == 0) ms = k;
   hipError_t e = hipMemsetAsync(fcd, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
   if (nv <= kFlowDirect)
     hipLaunchKernelGGL(k_flow_deg_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
-                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fcd);
+                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fcd, ms);
   else
     hipLaunchKernelGGL(k_flow_deg, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
-                       0, s, Tc, k, fmem, fcd);
+                       0, s, Tc, k, fmem, fcd, ms);
   return hipGetLastError();
 }
 
 hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
-                               uint32_t *finc, bool by_clique, hipStream_t s) {
+                               uint32_t *finc, bool by_clique, hipStream_t s,
+                               uint32_t ms) {
   if (k > 4 || k < 1) return hipErrorInvalidValue;
+  if (ms == 0) ms = k;
   const uint32_t div = by_clique ? k : 1;
   hipError_t e = hipMemsetAsync(fcur, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
   if (nv <= kFlowDirect)
     hipLaunchKernelGGL(k_flow_fill_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
-                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc, div);
+                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc, div,
+                       ms);
   else
     hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
-                       0, s, Tc, k, fmem, fioff, fcur, finc, div);
+                       0, s, Tc, k, fmem, fioff, fcur, finc, div, ms);
   return hipGetLastError();
 }
 

@@ -350,18 +350,21 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *comp_of, const uint32_t *fv_of,
                               uint32_t ncomp, const uint64_t *coff,
                               uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
-                              hipStream_t s);
+                              hipStream_t s, uint32_t ms = 0);
 // Flow-network incidence from the core's clique bucket fmem (Tc cliques,
 // k flow-vertex ids each, nv flow vertices): fcd[v] = number of pairs of
 // vertex v (zeroed here); then, with fioff the exclusive prefix sum of fcd,
 // finc[fioff[v] ..) = the pair indices s*k+j whose vertex is v, or with
 // by_clique their cliques s (for networks of 2^32 pairs or more; fcur is
 // nv words of scratch, zeroed here).
+// ms: words per clique in fmem (0: k; clique records hold more).
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
-                                  uint32_t nv, uint32_t *fcd, hipStream_t s);
+                                  uint32_t nv, uint32_t *fcd, hipStream_t s,
+                                  uint32_t ms = 0);
 hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                const uint64_t *fioff, uint32_t nv, uint32_t *fcur,
-                               uint32_t *finc, bool by_clique, hipStream_t s);
+                               uint32_t *finc, bool by_clique, hipStream_t s,
+                               uint32_t ms = 0);
 
 // ---- batched solves: the disjoint union of many graphs (batch.hip) ----
 struct UnionPart {
@@ -413,10 +416,14 @@ struct FlowProblem {
   uint32_t k, V, T;       // vertices, cliques of this component
   uint32_t vbase;         // flow-vertex id of local vertex 0
   uint64_t sbase;         // slot id of local slot 0 (first clique * k)
-  const uint32_t *mem;    // T*k member flow-vertex ids (subtract vbase)
+  const uint32_t *mem;    // member flow-vertex ids (subtract vbase), mstride
+                          // words per clique, the first k of them
   const uint64_t *ioff;   // V+1 offsets into inc, for local vertices
   const uint32_t *inc;    // slot ids (subtract sbase) of each vertex
   const uint32_t *cd;     // V clique degrees
+  // inc holds clique ids (from sbase / k) instead of slot ids
+  uint32_t by_clique = 0;
+  uint32_t mstride = 0;   // words per clique in mem (0: k)
 };
 
 // Exact max flow of one component in one cooperative launch on as many
@@ -442,6 +449,18 @@ struct GridFlowState {
   // clique excess here (x, exc unused), and P.inc holds clique ids from
   // P.sbase / k (the wide kernel); null otherwise
   int *x32 = nullptr, *exc32 = nullptr;
+  // clique records (k <= 6, T * 8 < 2^32): per clique 8 int32 words, the k
+  // residuals, the height at word 6 and the excess at word 7 (x, exc and hc
+  // unused); residuals stay in [-U, (k-1)U] and the excess below k*k*U,
+  // so k*k*U < 2^31 is required; null otherwise
+  int *rec = nullptr;
+  // implicit clique nodes (k_grid_flow_imp): records of imp_record_words(k)
+  // words per clique, the k member ids then the k residual words, from this
+  // problem's first clique (P.mem may point at the same records, with
+  // P.mstride = imp_record_words(k)); inflow: V words; x, exc, hc, cq0, cq1
+  // unused
+  uint32_t *rec32 = nullptr;
+  long long *inflow = nullptr;
   int *hv, *hmin;          // V
   int *hc;                 // T
   uint32_t *list0, *list1; // V each
@@ -484,15 +503,19 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
 // Workgroups for grid_max_flow on P (0: cannot launch); `blocks` != 0
 // overrides the size-based choice; `wide`: the kernel for 2^32 slots or more.
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks,
-                          bool wide = false);
+                          bool wide = false, bool rec = false, bool imp = false);
+// Words per clique record of the implicit-clique max flow: the k member
+// ids, then the k residual words, padded so no record crosses a 64-byte line.
+uint32_t imp_record_words(uint32_t k);
 
 // Cliques of a component whose members all have reach != 0.
+// ms: words per clique in mem / cl (0: k, plain member lists).
 hipError_t count_cliques_in_reach(uint32_t T, uint32_t k, const uint32_t *mem,
                                   uint32_t vbase, const int *reach,
-                                  unsigned long long *out, hipStream_t s);
+                                  unsigned long long *out, hipStream_t s, uint32_t ms = 0);
 // Cliques (AoS, k ids) whose members all have mask[id] != 0.
 hipError_t count_cliques_in_mask(uint64_t T, uint32_t k, const uint32_t *cl,
                                  const uint8_t *mask, unsigned long long *out,
-                                 hipStream_t s);
+                                 hipStream_t s, uint32_t ms = 0);
 
 }  // namespace cds

@@ -191,6 +191,10 @@ typedef struct {
   uint32_t prune_vertices;
   uint32_t prune_passes;
   uint32_t dense_peel;
+  /* Max flows that ran with implicit clique nodes (clique records of member
+   * ids and residuals, no clique heights, excess or work lists; the default
+   * whenever k*k*U fits 31 bits); the others ran the explicit network. */
+  uint32_t flow_implicit;
 } cds_result;
 
 typedef struct {

@@ -137,19 +137,25 @@ def test_batched_flows_on_several_workgroups(grid_engine):
         compare(r, oracle_solve(g, k), f"batched multi-workgroup flow item {i}")
 
 
-def _with_wide(fn):
-    """Runs fn() with CDS_FLOW_WIDE=1: every max flow on the wide kernel
-    (clique-id incidence, 32-bit residuals), which otherwise serves only
-    networks of 2^32 clique-member slots or more."""
-    old = os.environ.get("CDS_FLOW_WIDE")
-    os.environ["CDS_FLOW_WIDE"] = "1"
+def _with_env(env, fn):
+    """Runs fn() with the variables of `env` set (solves read them)."""
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return fn()
     finally:
-        if old is None:
-            del os.environ["CDS_FLOW_WIDE"]
-        else:
-            os.environ["CDS_FLOW_WIDE"] = old
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+def _with_wide(fn):
+    """Runs fn() with CDS_FLOW_WIDE=1: every max flow on clique-id incidence
+    (the member's position found in the clique's record), which otherwise
+    serves only networks of 2^32 clique-member slots or more."""
+    return _with_env({"CDS_FLOW_WIDE": "1"}, fn)
 
 
 @pytest.mark.parametrize("blocks", ["one", "three"])
@@ -165,3 +171,44 @@ def test_wide_flow_small(engine, grid_engine, seed, k, blocks):
     _announce(label)
     got = _with_wide(lambda: eng.solve(eng.upload(g), k, vertex_arrays=True))
     compare(got, oracle_solve(g, k), label)
+
+
+def _small_graph(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 500))
+    return cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)),
+                                  float(rng.uniform(3, 12)), int(rng.integers(0, 25)),
+                                  0.8, seed=seed), n
+
+
+@pytest.mark.parametrize("blocks", ["one", "three"])
+@pytest.mark.parametrize("seed,k", [(s, k) for s in range(6) for k in (2, 3, 4)])
+def test_implicit_flow_small(engine, grid_engine, seed, k, blocks):
+    """The implicit-clique max flow (the default) on one and on three
+    workgroups, k = 2 (16-byte records) to 4, against the oracle; every
+    whole-network flow of the solve ran it."""
+    eng = engine if blocks == "one" else grid_engine
+    g, n = _small_graph(seed + 70)
+    label = f"implicit flow ({blocks} workgroups) chung_lu seed={seed + 70} n={n} k={k}"
+    _announce(label)
+    got = eng.solve(eng.upload(g), k, vertex_arrays=True)
+    compare(got, oracle_solve(g, k), label)
+    assert got["flow_implicit"] == got["flow_launches"], label
+
+
+@pytest.mark.parametrize("blocks,wide", [("one", "0"), ("three", "0"), ("three", "1")])
+@pytest.mark.parametrize("seed,k", [(s, k) for s in range(6) for k in (3, 4)])
+def test_explicit_flow_small(engine, grid_engine, seed, k, blocks, wide):
+    """CDS_FLOW_IMPLICIT=0: the explicit clique network (clique heights,
+    excess and work lists), which serves flows whose unit is too large for
+    32-bit residuals, against the oracle; wide = 1 runs its clique-id
+    incidence variant."""
+    eng = engine if blocks == "one" else grid_engine
+    g, n = _small_graph(seed + 90)
+    label = (f"explicit flow ({blocks} workgroups, wide={wide}) chung_lu "
+             f"seed={seed + 90} n={n} k={k}")
+    _announce(label)
+    got = _with_env({"CDS_FLOW_IMPLICIT": "0", "CDS_FLOW_WIDE": wide},
+                    lambda: eng.solve(eng.upload(g), k, vertex_arrays=True))
+    compare(got, oracle_solve(g, k), label)
+    assert got["flow_implicit"] == 0, label
