@@ -1909,17 +1909,13 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<int> freach(&A);
   bool flow_alloc = false;
   DBuf<long long> gx(&A), grvs(&A), gexv(&A), grem(&A), gexc(&A);
-  DBuf<int> gx32(&A), gexc32(&A), grec(&A);
+  DBuf<int> gx32(&A), gexc32(&A);
   DBuf<int> ghv(&A), ghmin(&A), ghc(&A);
-  // Clique records (residuals, height and excess of a clique in one
-  // 32-byte line) unless the network is wide or CDS_FLOW_REC=0; a flow
-  // whose unit is too large for 32-bit residuals uses the split arrays.
-  const char *frec = getenv("CDS_FLOW_REC");
-  const bool rec_ok = !wide && k <= 6 && maxT * 8 <= 0xFFFFFFFFull &&
-                      (frec && atoi(frec) != 0);
-  // Implicit clique nodes (records of member ids and residuals, no clique
-  // heights, excess or work lists) unless CDS_FLOW_IMPLICIT=0; built once
-  // for every core clique at the first flow that uses them.
+  // Implicit clique nodes (the core's clique records hold member ids and
+  // residuals; no clique heights, excess or work lists) unless
+  // CDS_FLOW_IMPLICIT=0; a flow whose unit is too large for 32-bit
+  // residuals runs the explicit network (int64 residual, height and excess
+  // arrays, clique work lists).
   const char *fimp = getenv("CDS_FLOW_IMPLICIT");
   const bool imp_ok = !(fimp && atoi(fimp) == 0);
   DBuf<long long> ginflow(&A);
@@ -1960,11 +1956,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     // small component); state sized for the largest component, allocated
     // at the first flow of the solve that needs it
     const bool use_imp = imp_ok && (__int128)k * k * U < ((__int128)1 << 31);
-    const bool use_rec = !use_imp && rec_ok && (__int128)k * k * U < ((__int128)1 << 31);
     P.by_clique = wide ? 1u : 0u;
     if (use_imp && !ginflow.p) HIPCHK(ginflow.alloc(maxV));
-    if (use_rec && !grec.p) HIPCHK(grec.alloc(maxT * 8));
-    if (!use_imp && !use_rec && !wide && !gx.p) {
+    if (!use_imp && !wide && !gx.p) {
       HIPCHK(gx.alloc(maxT * k));
       HIPCHK(gexc.alloc(maxT));
       HIPCHK(ghc.alloc(maxT));
@@ -1997,7 +1991,6 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     G.x = gx.p;
     G.x32 = gx32.p;
     G.exc32 = gexc32.p;
-    G.rec = use_rec ? grec.p : nullptr;
     G.rec32 = use_imp ? fmem.p + C.cbase * ms : nullptr;
     G.inflow = ginflow.p;
     G.rvs = grvs.p;
@@ -2027,8 +2020,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     unsigned launched = 0;
     {
       const unsigned fgrid =
-          coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e), wide, use_rec,
-                                         use_imp));
+          coop_width(e, flow_grid_blocks(P, e->device, flow_blocks_of(e), wide, use_imp));
       if (fgrid == 0) return fail(CDS_ERR_HIP, "the max flow cannot be launched");
       CoopGate gate(e, fgrid);
       launched = fgrid;

@@ -149,16 +149,11 @@ __device__ __forceinline__ long long exc_add(int *p, long long d) {
 // int32 for networks whose capacities fit 31 bits).  kByClique: the
 // vertex incidence holds clique ids (from cbase) instead of clique-member
 // slots (from sbase), for networks with 2^32 slots or more; the member's
-// slot is found among the clique's k members.  kRec: one 32-byte record
-// per clique (int32 words: the k residuals, then its height at word 6 and
-// its excess at word 7) instead of three arrays, so the vertex discharge
-// reads an arc's residual and its clique's height from one line.
-template <class XT, bool kByClique_, bool kRec_ = false>
+// slot is found among the clique's k members.
+template <class XT, bool kByClique_>
 struct G {
   static constexpr bool kByClique = kByClique_;
-  static constexpr bool kRec = kRec_;
   using X = XT;
-  static_assert(!(kByClique_ && kRec_), "records use slot incidence");
   uint32_t k, V, T;
   uint32_t kmag, ksh;  // div_k's magic and shift for k
   uint32_t ms;         // words per clique in mem
@@ -170,7 +165,6 @@ struct G {
   const uint64_t *ioff;
   const uint32_t *inc;
   XT *x, *exc;
-  int *rec;
   long long *rvs, *exv, *rem;
   int *hv, *hc, *hmin;
   // Double buffers are separate members selected with ?: (an array member
@@ -187,10 +181,6 @@ struct G {
   uint32_t nblocks, bid;  // workgroups on this problem, this one's index
 };
 
-// Words per clique record (kRec) and the positions of height and excess.
-constexpr uint32_t kRecWords = 8, kRecHeight = 6, kRecExcess = 7;
-constexpr uint32_t kRecMaxK = 6;
-
 // s / k for 2 <= k <= 6 as a multiply-high and a shift: magic m and shift
 // sh with floor(s * m / 2^(32 + sh)) = floor(s / k) for every 32-bit s
 // (checked exhaustively for k = 2..6).
@@ -206,24 +196,20 @@ __device__ __forceinline__ uint32_t div_k(const Gx &g, uint32_t s) {
 // Residual word of member j of clique c, the clique's height and excess.
 template <class Gx>
 __device__ __forceinline__ auto &xcj(const Gx &g, uint64_t c, uint32_t j) {
-  if constexpr (Gx::kRec) return g.rec[c * kRecWords + j];
-  else return g.x[c * g.k + j];
+  return g.x[c * g.k + j];
 }
 template <class Gx>
 __device__ __forceinline__ int &hcl(const Gx &g, uint64_t c) {
-  if constexpr (Gx::kRec) return g.rec[c * kRecWords + kRecHeight];
-  else return g.hc[c];
+  return g.hc[c];
 }
 template <class Gx>
 __device__ __forceinline__ auto &excl(const Gx &g, uint64_t c) {
-  if constexpr (Gx::kRec) return g.rec[c * kRecWords + kRecExcess];
-  else return g.exc[c];
+  return g.exc[c];
 }
 // Residual word by its index a (as arc_at returns it).
 template <class Gx>
 __device__ __forceinline__ auto &xa(const Gx &g, uint64_t a) {
-  if constexpr (Gx::kRec) return g.rec[a];
-  else return g.x[a];
+  return g.x[a];
 }
 
 // Barrier over the workgroups working on one problem.
@@ -309,8 +295,7 @@ __device__ __forceinline__ void chunk_of(const Gx &g, int b, uint32_t count,
 template <class Gx>
 __device__ __forceinline__ uint32_t clique_of_arc(const Gx &g, uint32_t a) {
   static_assert(!Gx::kByClique, "slot incidence only");
-  if constexpr (Gx::kRec) return a / kRecWords;
-  else return div_k(g, a);
+  return div_k(g, a);
 }
 
 // Incidence entry p of vertex v: the index of its residual word (for xa)
@@ -328,8 +313,7 @@ __device__ __forceinline__ uint64_t arc_at(const Gx &g, uint64_t p, uint32_t v,
   } else {
     const uint32_t s = (uint32_t)((uint64_t)g.inc[p] - g.sbase);
     *c = div_k(g, s);
-    if constexpr (Gx::kRec) return (uint64_t)*c * kRecWords + (s - *c * g.k);
-    else return s;
+    return s;
   }
 }
 
@@ -521,9 +505,7 @@ __device__ __forceinline__ void flow_setup(Gx &g, const FlowProblem &P,
   g.mem = P.mem;
   g.ioff = P.ioff;
   g.inc = P.inc;
-  if constexpr (Gx::kRec) {
-    g.rec = S.rec;
-  } else if constexpr (Gx::kByClique) {
+  if constexpr (Gx::kByClique) {
     g.x = S.x32;
     g.exc = S.exc32;
   } else {
@@ -561,13 +543,8 @@ __device__ __forceinline__ void flow_run(Gx &g, const FlowProblem &P, const Grid
   GridFlowCtl *ctl = S.ctl;
   const uint64_t slots = (uint64_t)g.T * g.k;
 
-  if constexpr (Gx::kRec) {
-    // residuals and excess start at zero; heights come from the relabel
-    for (uint64_t w = g.gtid; w < (uint64_t)g.T * kRecWords; w += g.gthreads) g.rec[w] = 0;
-  } else {
-    for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
-    for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
-  }
+  for (uint64_t s = g.gtid; s < slots; s += g.gthreads) g.x[s] = 0;
+  for (uint32_t c = g.gtid; c < g.T; c += g.gthreads) g.exc[c] = 0;
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
     g.rvs[v] = (long long)P.cd[v] * U;
     g.exv[v] = cap_t;
@@ -1530,14 +1507,6 @@ __global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow(FlowProblem P,
   grid_flow_body<G<long long, false>, false>(P, S, cap_t, U, max_rounds);
 }
 
-// Clique records (int32 residuals, height and excess in one 32-byte line).
-__global__ __launch_bounds__(kGridFlowThreads) void k_grid_flow_rec(FlowProblem P,
-                                                       GridFlowState S,
-                                                       long long cap_t, long long U,
-                                                       uint32_t max_rounds) {
-  grid_flow_body<G<int, false, true>, true>(P, S, cap_t, U, max_rounds);
-}
-
 // 512-thread workgroups: the slot lookups need more registers than the
 // 128 a lane of a 1024-thread workgroup may hold.
 constexpr int kGridFlowWideThreads = 512;
@@ -1654,7 +1623,7 @@ unsigned flow_batch_capacity(uint32_t max_v, int device) {
 }
 
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks, bool wide,
-                          bool rec, bool imp) {
+                          bool imp) {
   int per_cu = 0, cus = 0;
   const size_t lds = P.V <= kGridLdsVertices
                          ? (size_t)P.V * (imp ? 12 : sizeof(long long))
@@ -1662,10 +1631,10 @@ unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks, boo
   const void *fn = imp   ? (P.by_clique ? reinterpret_cast<const void *>(k_grid_flow_imp_wide)
                                         : reinterpret_cast<const void *>(k_grid_flow_imp))
                    : wide ? reinterpret_cast<const void *>(k_grid_flow_wide)
-                   : rec  ? reinterpret_cast<const void *>(k_grid_flow_rec)
                           : reinterpret_cast<const void *>(k_grid_flow);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, fn, wide ? kGridFlowWideThreads : kGridFlowThreads, lds) != hipSuccess ||
+          &per_cu, fn, !imp && wide ? kGridFlowWideThreads : kGridFlowThreads, lds) !=
+              hipSuccess ||
       per_cu < 1)
     return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
@@ -1702,10 +1671,7 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                                   : reinterpret_cast<const void *>(k_grid_flow_imp);
     return launch_grid_shrinking(ifn, grid, kGridFlowThreads, args, ilds, s);
   }
-  if (S.rec && (P.k > kRecMaxK || (uint64_t)P.T * kRecWords > 0xFFFFFFFFull))
-    return hipErrorInvalidValue;
   const void *fn = S.x32   ? reinterpret_cast<const void *>(k_grid_flow_wide)
-                   : S.rec ? reinterpret_cast<const void *>(k_grid_flow_rec)
                            : reinterpret_cast<const void *>(k_grid_flow);
   return launch_grid_shrinking(fn, grid, S.x32 ? kGridFlowWideThreads : kGridFlowThreads,
                                args, lds, s);

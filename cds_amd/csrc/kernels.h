@@ -449,11 +449,6 @@ struct GridFlowState {
   // clique excess here (x, exc unused), and P.inc holds clique ids from
   // P.sbase / k (the wide kernel); null otherwise
   int *x32 = nullptr, *exc32 = nullptr;
-  // clique records (k <= 6, T * 8 < 2^32): per clique 8 int32 words, the k
-  // residuals, the height at word 6 and the excess at word 7 (x, exc and hc
-  // unused); residuals stay in [-U, (k-1)U] and the excess below k*k*U,
-  // so k*k*U < 2^31 is required; null otherwise
-  int *rec = nullptr;
   // implicit clique nodes (k_grid_flow_imp): records of imp_record_words(k)
   // words per clique, the k member ids then the k residual words, from this
   // problem's first clique (P.mem may point at the same records, with
@@ -503,7 +498,7 @@ hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
 // Workgroups for grid_max_flow on P (0: cannot launch); `blocks` != 0
 // overrides the size-based choice; `wide`: the kernel for 2^32 slots or more.
 unsigned flow_grid_blocks(const FlowProblem &P, int device, unsigned blocks,
-                          bool wide = false, bool rec = false, bool imp = false);
+                          bool wide = false, bool imp = false);
 // Words per clique record of the implicit-clique max flow: the k member
 // ids, then the k residual words, padded so no record crosses a 64-byte line.
 uint32_t imp_record_words(uint32_t k);
