@@ -9,8 +9,6 @@
 // are the union-specific pieces: building the union, per-graph totals, the
 // per-graph level tables, the per-graph densest-core flags and the
 // per-graph counts of the result.
-#include <hipcub/hipcub.hpp>
-
 #include "dev.h"
 #include "kernels.h"
 
@@ -76,6 +74,134 @@ __global__ void k_level_keys(uint32_t n, const uint32_t *__restrict__ gid,
     key[v] = ((uint64_t)gid[v] << 32) | core[v];
     val[v] = v;
   }
+}
+
+// ---- (graph, core) level runs: a stable LSD radix sort of 64-bit keys
+// with 32-bit values, 8-bit digits over the key bits that vary (the low
+// bits of the core word, then of the graph word), then run-length codes.
+constexpr int kRsThreads = 256, kRsRounds = 8;
+constexpr uint32_t kRsTile = kRsThreads * kRsRounds;  // items per workgroup
+constexpr int kRsWaves = kRsThreads / kWave;
+
+__device__ __forceinline__ uint32_t rs_digit(uint64_t key, uint32_t shift) {
+  return (uint32_t)(key >> shift) & 255u;
+}
+
+// hist[d * nb + b] = items of tile b whose digit is d.
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(uint32_t n,
+                                                        const uint64_t *__restrict__ key,
+                                                        uint32_t shift,
+                                                        uint32_t *__restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t t0 = blockIdx.x * kRsTile;
+  for (uint32_t i = t0 + threadIdx.x; i < n && i < t0 + kRsTile; i += kRsThreads)
+    atomicAdd(&cnt[rs_digit(key[i], shift)], 1u);
+  __syncthreads();
+  hist[threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter of tile b to the positions the scanned histogram gives:
+// items are taken in index order, 256 per round; an item's place among the
+// round's items of its digit is its rank among the equal-digit lanes of its
+// wave (a match over the digit's 8 bits) plus the counts of earlier waves.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
+    uint32_t n, const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t shift,
+    const uint32_t *__restrict__ hoff) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[kRsWaves][256];
+  const uint32_t w = threadIdx.x / kWave;
+  base[threadIdx.x] = hoff[threadIdx.x * gridDim.x + blockIdx.x];
+  for (int x = 0; x < kRsWaves; ++x) wcnt[x][threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t t0 = blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsRounds; ++r) {
+    const uint32_t i = t0 + (uint32_t)r * kRsThreads + threadIdx.x;
+    const bool in = i < n;
+    const uint64_t kv = in ? kin[i] : 0;
+    const uint32_t d = rs_digit(kv, shift);
+    uint64_t peers = __ballot(in);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(peers & lanes_below());
+    if (in && rank == 0) wcnt[w][d] = __popcll(peers);
+    __syncthreads();
+    if (in) {
+      uint32_t pos = base[d] + rank;
+      for (uint32_t x = 0; x < w; ++x) pos += wcnt[x][d];
+      kout[pos] = kv;
+      vout[pos] = vin[i];
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int x = 0; x < kRsWaves; ++x) {
+      add += wcnt[x][threadIdx.x];
+      wcnt[x][threadIdx.x] = 0;
+    }
+    base[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Largest graph id and core: m[0] = max gid, m[1] = max core (m zeroed).
+__global__ void k_key_max(uint32_t n, const uint32_t *__restrict__ gid,
+                          const uint32_t *__restrict__ core, uint32_t *m) {
+  uint32_t a = 0, b = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x) {
+    a = max(a, gid[v]);
+    b = max(b, core[v]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = max(a, (uint32_t)__shfl_xor(a, off, kWave));
+    b = max(b, (uint32_t)__shfl_xor(b, off, kWave));
+  }
+  if (lane() == 0) {
+    if (a) atomicMax(&m[0], a);
+    if (b) atomicMax(&m[1], b);
+  }
+}
+
+// head[i] = 1 where a run of equal sorted keys starts.
+__global__ void k_run_heads(uint32_t n, const uint64_t *__restrict__ skey,
+                            uint32_t *__restrict__ head) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    head[i] = (i == 0 || skey[i] != skey[i - 1]) ? 1u : 0u;
+}
+
+// Run r (idx = exclusive scan of head, idx[n] = runs): its key and its
+// first position; starts[runs] = n.
+__global__ void k_run_write(uint32_t n, const uint64_t *__restrict__ skey,
+                            const uint32_t *__restrict__ head,
+                            const uint32_t *__restrict__ idx, uint64_t *__restrict__ ukey,
+                            uint32_t *__restrict__ starts, uint32_t *__restrict__ nruns) {
+  const uint32_t runs = idx[n];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    if (!head[i]) continue;
+    ukey[idx[i]] = skey[i];
+    starts[idx[i]] = i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *nruns = runs;
+    starts[runs] = n;
+  }
+}
+
+__global__ void k_run_lengths(const uint32_t *__restrict__ starts,
+                              const uint32_t *__restrict__ nruns,
+                              uint32_t *__restrict__ counts) {
+  const uint32_t runs = *nruns;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+       r += gridDim.x * blockDim.x)
+    counts[r] = starts[r + 1] - starts[r];
 }
 
 // Index of `key` in the ascending array a[0..len) (it is present).
@@ -210,26 +336,91 @@ hipError_t level_runs(uint32_t n, const uint32_t *gid, const uint32_t *core,
                       uint64_t *key, uint32_t *val, uint64_t *skey, uint32_t *sval,
                       uint64_t *ukey, uint32_t *counts, uint32_t *nruns,
                       DeviceArena &arena, hipStream_t s) {
+  if (n == 0) return hipMemsetAsync(nruns, 0, sizeof(uint32_t), s);
   hipLaunchKernelGGL(k_level_keys, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
                      n, gid, core, key, val);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t b1 = 0, b2 = 0;
-  if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, b1, key, skey, val, sval,
-                                              (int)n, 0, 64, s)) != hipSuccess)
+  // the key bits that vary: low bits of the core, then of the graph id
+  uint32_t *mx = nullptr;
+  if ((e = arena.alloc((void **)&mx, 2 * sizeof(uint32_t))) != hipSuccess) return e;
+  uint32_t hm[2] = {0, 0};
+  e = hipMemsetAsync(mx, 0, 2 * sizeof(uint32_t), s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_key_max, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, n, gid,
+                       core, mx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  arena.release(mx);
+  if (e != hipSuccess) return e;
+  auto bits = [](uint32_t x) {
+    uint32_t b = 0;
+    while (b < 32 && (x >> b)) ++b;
+    return b;
+  };
+  std::vector<uint32_t> shifts;
+  for (uint32_t sh = 0; sh < bits(hm[1]); sh += 8) shifts.push_back(sh);
+  for (uint32_t sh = 0; sh < bits(hm[0]); sh += 8) shifts.push_back(32 + sh);
+  const uint32_t nb = (n + kRsTile - 1) / kRsTile;
+  uint32_t *hist = nullptr, *hoff = nullptr;
+  if ((e = arena.alloc((void **)&hist, (size_t)256 * nb * sizeof(uint32_t))) != hipSuccess)
     return e;
-  if ((e = hipcub::DeviceRunLengthEncode::Encode(nullptr, b2, skey, ukey, counts,
-                                                 nruns, (int)n, s)) != hipSuccess)
+  if ((e = arena.alloc((void **)&hoff, ((size_t)256 * nb + 1) * sizeof(uint32_t))) !=
+      hipSuccess) {
+    arena.release(hist);
     return e;
-  void *tmp = nullptr;
-  const size_t bytes = b1 > b2 ? b1 : b2;
-  if ((e = arena.alloc(&tmp, bytes ? bytes : 1)) != hipSuccess) return e;
-  e = hipcub::DeviceRadixSort::SortPairs(tmp, b1, key, skey, val, sval, (int)n, 0,
-                                         64, s);
-  if (e == hipSuccess)
-    e = hipcub::DeviceRunLengthEncode::Encode(tmp, b2, skey, ukey, counts, nruns,
-                                              (int)n, s);
-  arena.release(tmp);
+  }
+  // ping-pong between (key, val) and (skey, sval); an even pass count
+  // starts by copying so the last pass lands in (skey, sval)
+  uint64_t *ka = key, *kb = skey;
+  uint32_t *va = val, *vb = sval;
+  if (shifts.size() % 2 == 0) {
+    e = hipMemcpyAsync(skey, key, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sval, val, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  for (size_t pi = 0; pi < shifts.size() && e == hipSuccess; ++pi) {
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsThreads), 0, s, n, ka, shifts[pi], hist);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = scan_u32_32(hist, hoff, (size_t)256 * nb, arena);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsThreads), 0, s, n, ka, va, kb, vb,
+                         shifts[pi], hoff);
+      e = hipGetLastError();
+    }
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  // runs of equal keys
+  uint32_t *head = nullptr, *idx = nullptr, *starts = nullptr;
+  if (e == hipSuccess) e = arena.alloc((void **)&head, (size_t)n * sizeof(uint32_t));
+  if (e == hipSuccess) e = arena.alloc((void **)&idx, ((size_t)n + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = arena.alloc((void **)&starts, ((size_t)n + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, skey,
+                       head);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = scan_u32_32(head, idx, n, arena);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_run_write, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, skey,
+                       head, idx, ukey, starts, nruns);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_run_lengths, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, starts,
+                       nruns, counts);
+    e = hipGetLastError();
+  }
+  arena.release(starts);
+  arena.release(idx);
+  arena.release(head);
+  arena.release(hoff);
+  arena.release(hist);
   return e;
 }
 

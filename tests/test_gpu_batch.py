@@ -87,3 +87,29 @@ def test_batch_error_reports_item(engine):
 
 def test_batch_empty(engine):
     assert engine.solve_batch([], []) == []
+
+
+@pytest.mark.parametrize("count,kk", [(120, 2), (300, 2), (40, 4)])
+def test_batch_level_runs(engine, count, kk):
+    """The per-graph level tables come from a radix sort of (graph, core)
+    keys over the bits that vary, 8 at a time: 120 small k = 2 graphs need
+    one pass over the core bits and one over the graph bits (an even count,
+    which starts from a copy), 300 graphs a second graph pass (odd), and
+    the k = 4 graphs deeper cores.  Every answer equals the plain path's."""
+    rng = np.random.default_rng(count + kk)
+    items = []
+    for i in range(count):
+        n = int(rng.integers(30, 160))
+        items.append(cds_amd.Graph.chung_lu(n, float(rng.uniform(2.2, 3.0)),
+                                            float(rng.uniform(3, 8)),
+                                            int(rng.integers(6, 14)), 0.9,
+                                            seed=77000 + 1000 * kk + i))
+    dgs = [engine.upload(g) for g in items]
+    _announce(f"batch of {count} graphs, k={kk}: level-run sort")
+    batch = engine.solve_batch(dgs, [kk] * count)
+    for i, (g, dg, rb) in enumerate(zip(items, dgs, batch)):
+        rp = engine.solve(dg, kk)
+        diff = same(rb, rp)
+        assert diff is None, f"graph {i} (n={g.n}, k={kk}): batched {diff} differs"
+    for dg in dgs:
+        dg.free()
