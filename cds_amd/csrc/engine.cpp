@@ -310,6 +310,9 @@ constexpr double kFloatDriftScale = 2.0;
 
 // Round bound of the whole-GPU flow kernel (NOT_CONVERGED beyond it).
 constexpr uint32_t kGridFlowRounds = 20000;
+// Round limit of the implicit-clique max flow before the explicit network
+// takes the flow over.
+constexpr uint32_t kImpFlowRounds = 4096;
 // Incidence entries (k*T) from which the plain path peels in coarse steps.
 constexpr uint64_t kCoarseMinSlots = 1ull << 24;
 constexpr uint32_t kFlowTraceRounds = 4096;
@@ -1919,6 +1922,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   const char *fimp = getenv("CDS_FLOW_IMPLICIT");
   const bool imp_ok = !(fimp && atoi(fimp) == 0);
   DBuf<long long> ginflow(&A);
+  DBuf<int> gpend(&A);
   bool state_alloc = false;
   DBuf<uint32_t> gl0(&A), gl1(&A), gq0(&A), gq1(&A);
   DBuf<uint64_t> glc0(&A), glc1(&A), garc(&A);
@@ -1954,10 +1958,17 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     P.cd = fcd.p + C.vbase;
     // push-relabel on as many workgroups as the network needs (one for a
     // small component); state sized for the largest component, allocated
-    // at the first flow of the solve that needs it
-    const bool use_imp = imp_ok && (__int128)k * k * U < ((__int128)1 << 31);
+    // at the first flow of the solve that needs it.  An implicit-clique
+    // flow that reaches its round limit is solved again on the explicit
+    // network (the result counts the flows each one finished).
+    for (int attempt = 0;; ++attempt) {
+    const bool use_imp =
+        imp_ok && attempt == 0 && (__int128)k * k * U < ((__int128)1 << 31);
     P.by_clique = wide ? 1u : 0u;
-    if (use_imp && !ginflow.p) HIPCHK(ginflow.alloc(maxV));
+    if (use_imp && !ginflow.p) {
+      HIPCHK(ginflow.alloc(maxV));
+      HIPCHK(gpend.alloc(maxV));
+    }
     if (!use_imp && !wide && !gx.p) {
       HIPCHK(gx.alloc(maxT * k));
       HIPCHK(gexc.alloc(maxT));
@@ -1993,6 +2004,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     G.exc32 = gexc32.p;
     G.rec32 = use_imp ? fmem.p + C.cbase * ms : nullptr;
     G.inflow = ginflow.p;
+    G.pend = gpend.p;
     G.rvs = grvs.p;
     G.exv = gexv.p;
     G.rem = grem.p;
@@ -2025,7 +2037,10 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       CoopGate gate(e, fgrid);
       launched = fgrid;
       HIPCHK(hipEventRecord(tm.ev[22], st));
-      HIPCHK(grid_max_flow(P, G, capT, U, kGridFlowRounds, &launched, st));
+      uint32_t imp_rounds = kImpFlowRounds;
+      if (const char *ir = getenv("CDS_IMP_FLOW_ROUNDS")) imp_rounds = (uint32_t)atoi(ir);
+      HIPCHK(grid_max_flow(P, G, capT, U, use_imp ? imp_rounds : kGridFlowRounds,
+                           &launched, st));
       HIPCHK(hipEventRecord(tm.ev[23], st));
       note_width(R, &R->flow_grid, fgrid, launched);
     }
@@ -2036,9 +2051,14 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     R->flow_clique_visits += hc.clique_visits;
     R->flow_arc_visits += hc.chunk_visits * kFlowChunkArcs;
     R->flow_relabels += hc.relabels;
-    if (hc.limit)
+    if (hc.limit) {
+      if (flow_trace_on())
+        fprintf(stderr, "[flow] component %u V=%u T=%u: %s flow hit %u rounds\n", ci, P.V,
+                P.T, use_imp ? "implicit" : "explicit", hc.rounds);
+      if (use_imp) continue;
       return fail(CDS_ERR_NOT_CONVERGED,
                   "grid max flow of component %u hit %u rounds", ci, hc.rounds);
+    }
     sweeps += hc.rounds;
     R->flow_launches++;
     R->flow_implicit += use_imp ? 1 : 0;
@@ -2062,6 +2082,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       }
     }
     return CDS_OK;
+    }  // attempts
   };
   {
     std::vector<FlowRequest> round;

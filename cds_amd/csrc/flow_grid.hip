@@ -889,7 +889,10 @@ __device__ __forceinline__ void flow_run(Gx &g, const FlowProblem &P, const Grid
 // arc is ever over-used (a push that finds less left returns the rest to
 // the vertex); what reaches a member is summed in LDS per workgroup and
 // added to its inflow; (D) vertices keep the remainder plus inflow as
-// excess, move their current arc, and relabel after a whole-list scan.
+// excess, move their current arc, and relabel after a whole-list scan
+// that found no admissible arc (a push limited by the clique's lowest
+// members leaves v->c residual, the clique one level up: such a vertex
+// scans its whole list again instead).
 //
 // Heights stay a valid labelling.  Pushes go from v to members two levels
 // down (h(w) <= h(v) - 2, h(c) = h(w) + 1).  Flow into v through c makes
@@ -914,7 +917,7 @@ struct H {
   const uint64_t *ioff;
   const uint32_t *inc;
   long long *rvs, *exv, *rem, *inflow;
-  int *hv, *hmin;
+  int *hv, *hmin, *pend;
   uint32_t *list0, *list1;
   uint64_t *lchunk0, *lchunk1;
   uint64_t *arc;
@@ -929,6 +932,10 @@ struct H {
 };
 
 constexpr uint32_t kImpMaxK = 6;
+// No relabel pending for a vertex (pend[]); below it, pend holds the
+// lowest height of a clique behind a residual arc (the new height is one
+// more), or -1 when the vertex found an admissible arc instead.
+constexpr int kNoPend = 0x7FFFFFFF;
 
 // Record stride in words for clique size k: 2k words rounded up to 4, 8
 // or 16, so a record never crosses a 64-byte line.
@@ -1023,7 +1030,10 @@ __device__ __forceinline__ void imp_stage_h(const Hx &g) {
 template <class Hx>
 __device__ __forceinline__ unsigned long long imp_relabel(const Hx &g, uint64_t arc_reset) {
   GridFlowCtl *ctl = g.ctl;
-  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) g.hv[v] = g.INF;
+  for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+    g.hv[v] = g.INF;
+    g.pend[v] = kNoPend;  // exact heights replace a pending relabel
+  }
   if (g.leader) {
     ctl->lpack[0] = 0;
     ctl->lpack[1] = 0;
@@ -1144,6 +1154,7 @@ __device__ __forceinline__ void imp_setup(Hx &g, const FlowProblem &P, const Gri
   g.inflow = S.inflow;
   g.hv = S.hv;
   g.hmin = S.hmin;
+  g.pend = S.pend;
   g.list0 = S.list0;
   g.list1 = S.list1;
   g.lchunk0 = S.lchunk0;
@@ -1154,6 +1165,39 @@ __device__ __forceinline__ void imp_setup(Hx &g, const FlowProblem &P, const Gri
 }
 
 constexpr int kImpSteps = kChunkSteps;  // wave passes per chunk
+
+// One arc of v as the relabel scan and the discharge see it: the clique's
+// height derived from its residual members (INF if none), whether v->c is
+// residual, and through = what c can pass on to its lowest members.
+template <class Hx>
+__device__ __forceinline__ void imp_view(const Hx &g, uint64_t p, uint32_t v, long long kU,
+                                         long long U, uint32_t *c, uint32_t *jv,
+                                         long long *rvc, int *hcc, int *mh,
+                                         long long *through) {
+  *c = imp_arc(g, p, v, jv);
+  uint32_t m[kImpMaxK];
+  int x[kImpMaxK];
+  imp_load(g, *c, m, x);
+  *rvc = kU - imp_x(g, *c, *jv);
+  int lo = g.INF;
+  long long th = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kImpMaxK; ++j) {
+    if (j >= g.k) break;
+    const long long rc = U + x[j];
+    if (rc <= 0) continue;
+    const int h = imp_h(g, m[j]);
+    if (h < lo) {
+      lo = h;
+      th = rc;
+    } else if (h == lo) {
+      th += rc;
+    }
+  }
+  *mh = lo;
+  *through = th;
+  *hcc = lo >= g.INF - 1 ? g.INF : lo + 1;
+}
 
 template <class Hx>
 __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridFlowState &S,
@@ -1176,6 +1220,7 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
     ctl->rounds = 0;
     ctl->limit = 0;
     ctl->relabels = 0;
+    ctl->cqn[0] = 0;
   }
   gsync(g);
   unsigned long long active = imp_relabel(g, kArcFull);
@@ -1183,19 +1228,26 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
   unsigned long long sink = 0, ch_visits = 0;
   bool limit = false;
   const uint64_t slots = (uint64_t)g.T * g.k;
+  // Counters: list 0 (this round's discharge windows) and the round's
+  // active-vertex count (cqn[0]) are cleared in (C) and filled in (A); list
+  // 1 (relabel candidates) is cleared in (A) and filled in (D); each is
+  // read a phase later.
   while (active) {
     if (rounds >= max_rounds) {
       limit = true;
       break;
     }
     const unsigned long long tr0 = (S.trace && g.leader) ? wall_clock64() : 0;
-    if (g.leader) {
-      ctl->lpack[0] = 0;
-      ctl->active = 0;
-    }
-    // (A) sink pushes; active vertices post their windows; inflow restarts
+    if (g.leader) ctl->lpack[1] = 0;
+    // (A) pending relabels take effect; sink pushes; active vertices post
+    // their windows; inflow restarts
     unsigned long long act = 0;
     for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+      const int pd = g.pend[v];
+      if (pd != kNoPend) {
+        if (pd >= 0) g.hv[v] = pd >= g.INF - 1 ? g.INF : pd + 1;
+        g.pend[v] = kNoPend;
+      }
       g.inflow[v] = 0;
       long long e = g.exv[v];
       if (e <= 0 || g.hv[v] >= g.INF) {
@@ -1212,7 +1264,7 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
       }
       g.exv[v] = e;
       g.rem[v] = e;
-      g.hmin[v] = g.INF;
+      g.hmin[v] = 0;
       if (e > 0) {
         uint32_t first, count;
         discharge_range(g, v, &first, &count);
@@ -1221,26 +1273,26 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
       }
     }
     act = wave_sum64(act);
-    if (lane() == 0 && act) atomicAdd(&ctl->active, act);
+    if (lane() == 0 && act) atomicAdd(&ctl->cqn[0], (unsigned)act);
     gsync(g);
     const unsigned long long tr1 = (S.trace && g.leader) ? wall_clock64() : 0;
     const unsigned long long pack = ald(&ctl->lpack[0]);
     const uint32_t nv = (uint32_t)(pack >> 40);
     const uint64_t nch = pack & kMask40;
-    const unsigned long long any = ald(&ctl->active);
+    const unsigned long long any = ald(&ctl->cqn[0]);
     imp_stage_h(g);
     if (g.lds_in) {
       for (uint32_t i = threadIdx.x; i < g.V; i += blockDim.x) g.lds_in[i] = 0;
       __syncthreads();
     }
-    // (B) chunked discharge through the cliques
+    // (B) chunked discharge through the cliques.  hmin[v] becomes 1 when
+    // v has an admissible arc this round (it then does not relabel).
     for (uint64_t ci = g.gwave; ci < nch; ci += g.gwaves) {
       uint32_t v;
       uint64_t lo, hi;
       uint32_t vchunk;
       chunk_of(g, 0, nv, ci, &v, &lo, &hi, true, &vchunk);
       const int hvv = imp_h(g, v);
-      int hm = g.INF;
       long long radm = 0;
       int av[kImpSteps];       // what this lane's arc can pass
       uint32_t cv[kImpSteps];  // its clique
@@ -1252,46 +1304,22 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
         tv[it] = 0;
         const uint64_t p = lo + (uint64_t)it * kWave + lane();
         if (p >= hi) continue;
-        uint32_t jv;
-        const uint32_t c = imp_arc(g, p, v, &jv);
-        uint32_t m[kImpMaxK];
-        int x[kImpMaxK];
-        imp_load(g, c, m, x);
-        const long long rvc = kU - imp_x(g, c, jv);
-        if (rvc <= 0) continue;  // saturated: not a residual arc
-        int mh = g.INF;
-        long long through = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kImpMaxK; ++j) {
-          if (j >= g.k) break;
-          const long long rc = U + x[j];
-          if (rc <= 0) continue;
-          const int h = imp_h(g, m[j]);
-          if (h < mh) {
-            mh = h;
-            through = rc;
-          } else if (h == mh) {
-            through += rc;
-          }
-        }
-        const int hcc = mh >= g.INF - 1 ? g.INF : mh + 1;
-        if (hcc < hvv) {
-          const long long a = rvc < through ? rvc : through;
-          radm += a;
-          av[it] = (int)a;
-          cv[it] = c;
-          tv[it] = (uint32_t)mh << 3 | jv;
-        } else {
-          hm = min(hm, hcc);
-        }
+        uint32_t c, jv;
+        long long rvc, through;
+        int hcc, mh;
+        imp_view(g, p, v, kU, U, &c, &jv, &rvc, &hcc, &mh, &through);
+        if (rvc <= 0 || hcc >= hvv) continue;  // not a residual / admissible arc
+        const long long a = rvc < through ? rvc : through;
+        radm += a;
+        av[it] = (int)a;
+        cv[it] = c;
+        tv[it] = (uint32_t)mh << 3 | jv;
       }
       radm = (long long)wave_sum64((uint64_t)radm);
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) hm = min(hm, __shfl_xor(hm, off, kWave));
-      if (lane() == 0 && hm < g.INF) atomicMin(&g.hmin[v], hm);
       if (radm == 0) continue;
       long long avail = 0;
       if (lane() == 0) {
+        g.hmin[v] = 1;
         const long long old = (long long)atomicAdd(
             (unsigned long long *)&g.rem[v], (unsigned long long)(-radm));
         avail = old < 0 ? 0 : (old < radm ? old : radm);
@@ -1361,8 +1389,10 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
     }
     gsync(g);
     const unsigned long long tr2 = (S.trace && g.leader) ? wall_clock64() : 0;
-    // (D) current arcs and relabels of the listed vertices (their excess
-    // left in rem), then every vertex's excess = remainder + inflow
+    // (D) current arcs of the listed vertices (their excess left in rem);
+    // a vertex that scanned its whole list, found no admissible arc and
+    // kept excess becomes a relabel candidate (list 1); every vertex's
+    // excess = remainder + inflow
     for (uint32_t i = g.gtid; i < nv; i += g.gthreads) {
       const uint32_t v = g.list0[i];
       const long long r = g.rem[v];
@@ -1371,14 +1401,9 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
       const uint32_t nchv = (uint32_t)nchunks(g, v);
       const uint32_t win = (uint32_t)(g.arc[v] >> 32);
       uint32_t cur, nwin;
-      if (r > 0 && first == 0 && count == nchv) {
-        const int hm = g.hmin[v];
-        int hn = hm >= g.INF - 1 ? g.INF : hm + 1;
-        // flow that came in through a clique may have re-opened an arc to
-        // it, and such a clique sits one above v (inflow also holds
-        // excess a push returned; the bound then only slows the rise)
-        if (g.inflow[v] > 0 && hn > g.hv[v] + 2) hn = g.hv[v] + 2;
-        g.hv[v] = hn;
+      if (r > 0 && first == 0 && count == nchv && g.hmin[v] == 0) {
+        g.pend[v] = g.INF;
+        list_push_n(g, 1, v, nchv);
         cur = 0;
         nwin = kArcWindowRelabel;
       } else if (r > 0) {
@@ -1398,6 +1423,45 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
     }
     gsync(g);
     const unsigned long long tr3 = (S.trace && g.leader) ? wall_clock64() : 0;
+    // (C) relabel scan of the candidates over the settled network: no
+    // residual word changes in this phase and heights only in (A), so the
+    // lowest clique height behind a residual arc is exact; a candidate
+    // that now finds an admissible arc (flow came in through a clique)
+    // does not relabel.  The new heights apply at the next (A).
+    {
+      const unsigned long long pk = ald(&ctl->lpack[1]);
+      const uint32_t nv1 = (uint32_t)(pk >> 40);
+      const uint64_t nch1 = pk & kMask40;
+      if (g.leader) {
+        ctl->lpack[0] = 0;
+        ctl->cqn[0] = 0;
+      }
+      for (uint64_t ci = g.gwave; ci < nch1; ci += g.gwaves) {
+        uint32_t v;
+        uint64_t lo, hi;
+        chunk_of(g, 1, nv1, ci, &v, &lo, &hi);
+        const int hvv = imp_h(g, v);
+        int hm = g.INF;
+        bool adm = false;
+        for (uint64_t p = lo + lane(); p < hi; p += kWave) {
+          uint32_t c, jv;
+          long long rvc, through;
+          int hcc, mh;
+          imp_view(g, p, v, kU, U, &c, &jv, &rvc, &hcc, &mh, &through);
+          if (rvc <= 0) continue;
+          if (hcc < hvv)
+            adm = true;
+          else
+            hm = min(hm, hcc);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) hm = min(hm, __shfl_xor(hm, off, kWave));
+        if (__ballot(adm)) hm = -1;
+        if (lane() == 0) atomicMin(&g.pend[v], hm);
+      }
+      ch_visits += nch1;
+    }
+    gsync(g);
     const bool after_relabel = since == 0;
     ++rounds;
     ++since;
@@ -1416,7 +1480,7 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
       t[4] = tr1 - tr0;  // (A)
       t[5] = tr2 - tr1;  // (B)
       t[6] = tr3 - tr2;  // (D)
-      t[7] = wall_clock64() - tr3;
+      t[7] = wall_clock64() - tr3;  // (C) and the global relabel
     }
   }
   if (!limit) imp_relabel(g, kArcFull);

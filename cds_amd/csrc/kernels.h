@@ -456,6 +456,7 @@ struct GridFlowState {
   // unused
   uint32_t *rec32 = nullptr;
   long long *inflow = nullptr;
+  int *pend = nullptr;  // V words: pending relabels of the implicit flow
   int *hv, *hmin;          // V
   int *hc;                 // T
   uint32_t *list0, *list1; // V each

@@ -7,6 +7,7 @@ checked on every component (including components that need one flow per
 alpha query).  Likewise CDS_REPLAY_GRID_WORK=0 sends every exact phase
 replay with at most 16384 vertices to the whole-GPU replay kernel.
 """
+import math
 import os
 
 import numpy as np
@@ -212,3 +213,42 @@ def test_explicit_flow_small(engine, grid_engine, seed, k, blocks, wide):
                     lambda: eng.solve(eng.upload(g), k, vertex_arrays=True))
     compare(got, oracle_solve(g, k), label)
     assert got["flow_implicit"] == 0, label
+
+
+def _served_mix(count, seed, first):
+    """Graphs of the batched workload's generator (Chung-Lu, n log-uniform
+    in [200, 20000], exponent [2.1, 3.0], mean degree [4, 24], one planted
+    G(s, 0.9), s in [8, 60]) with k drawn from {2, 3, 4} (30/50/20)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(math.exp(rng.uniform(math.log(200), math.log(20000))))
+        g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 3.0)), float(rng.uniform(4, 24)),
+                                   int(rng.integers(8, 61)), 0.9, seed=first + i)
+        k = int(rng.choice([2, 3, 4], p=[0.3, 0.5, 0.2]))
+        out.append((g, k))
+    return out
+
+
+def test_implicit_flow_mixed_stream(engine):
+    """300 graphs of the served stream's small share: every max flow finishes
+    on the implicit-clique network (no fall-back to the explicit one), the
+    answers equal the explicit network's, and those of up to 3000 vertices
+    equal the oracle's.  The sample holds a graph whose flow did not finish
+    in 20000 rounds while an admissible arc limited by its clique's lowest
+    members could still trigger a relabel."""
+    _announce("implicit flow on 300 mixed-stream graphs")
+    for i, (g, k) in enumerate(_served_mix(300, 5, 9000)):
+        dg = engine.upload(g)
+        got = engine.solve(dg, k, vertex_arrays=True)
+        label = f"mixed graph {i} n={g.n} k={k}"
+        assert got["flow_implicit"] == got["flow_launches"], label
+        ref = _with_env({"CDS_FLOW_IMPLICIT": "0"},
+                        lambda: engine.solve(dg, k, vertex_arrays=True))
+        assert ref["flow_implicit"] == 0, label
+        for f in ("total_cliques", "k_prime", "core_size", "size", "set_cliques",
+                  "set_size", "num_components", "trace_alpha", "trace_margin", "vertices"):
+            assert got[f] == ref[f], f"{label}: {f}"
+        if g.n <= 3000:
+            compare(got, oracle_solve(g, k), label)
+        dg.free()
