@@ -167,3 +167,37 @@ def test_server_lanes_keep_cooperative_kernels_within_half_the_cus():
         assert same(again, alone) is None
     finally:
         eng.close()
+
+
+def test_served_answers_match_the_oracle(serve_engine):
+    """A sample of the served stream's mix (batched-workload Chung-Lu graphs
+    and two R-MAT scale-11 graphs for the one-at-a-time lane, k in {2, 3,
+    4}) submitted from two threads while earlier requests are still on the
+    GPU: every served answer equals the oracle's (cpuCode/c++ restated), not
+    only the same engine's plain solve."""
+    from oracle_ref import oracle_solve
+    from test_gpu_parity import compare
+
+    reqs = request_graphs(24, seed=33)
+    reqs.insert(5, (cds_amd.Graph.rmat(11, 16, seed=41), 3))
+    reqs.insert(17, (cds_amd.Graph.rmat(11, 16, seed=42), 4))
+    srv = cds_amd.Server(serve_engine, max_batch=8, large_vertices=2000)
+    tickets = [None] * len(reqs)
+
+    def producer(part):
+        for i in range(part, len(reqs), 2):
+            g, k = reqs[i]
+            tickets[i] = srv.submit(g, k, vertex_arrays=True)
+
+    _announce(f"serve {len(reqs)} requests against the oracle")
+    threads = [threading.Thread(target=producer, args=(p,)) for p in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    served = [srv.wait(t, timeout_s=120) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    assert stats["batches"] < len(reqs), "no request shared a batch"
+    for i, ((g, k), got) in enumerate(zip(reqs, served)):
+        compare(got, oracle_solve(g, k), f"served request {i} n={g.n} k={k}")
