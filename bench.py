@@ -248,11 +248,15 @@ def bench_serving(args):
         def wait_plain(done_at, t0):
             for _ in range(len(stream)):
                 i, g, k = q.get()
+                s0 = time.perf_counter()
                 try:
                     plain[i] = eng.solve(eng.upload(g), k)  # device graph freed here
                 except cds_amd.CdsError as ex:
                     plain[i] = ex.status
                 done_at[i] = time.perf_counter() - t0
+                if done_at[i] - (s0 - t0) > 2.0:  # a slow solve is named as it ends
+                    print(f"[bench] load {rate}/s plain: request {i} (n={g.n}, k={k}) "
+                          f"took {done_at[i] - (s0 - t0):.1f}s", file=sys.stderr, flush=True)
                 progress("plain", n_done[0])
                 n_done[0] += 1
 

@@ -3096,6 +3096,10 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
 
 }  // namespace
 
+namespace cds {
+bool solves_on_core(uint32_t k, uint64_t T) { return needs_prune(k, T); }
+}  // namespace cds
+
 extern "C" cds_status cds_solve(cds_engine *e, const cds_dgraph *g,
                                 const cds_params *p, cds_result *out) {
   if (!e || !g || !p || !out) return fail(CDS_ERR_INVALID, "null argument");

@@ -18,6 +18,9 @@ void copy_tuning(cds_engine *dst, cds_engine *src);
 // Marks an engine as sharing its device with another one, or not
 // (engine.cpp): shared engines keep cooperative kernels within half the CUs.
 void share_device(cds_engine *e, bool shared);
+// Whether a graph with T k-cliques is solved on its clique core instead of
+// being materialised whole (engine.cpp).
+bool solves_on_core(uint32_t k, uint64_t T);
 
 // Allocates a host array with malloc (so the C ABI can free it with free()).
 template <class T> T *host_alloc(size_t count) {

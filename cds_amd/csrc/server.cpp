@@ -189,6 +189,11 @@ void serve(cds_server *s, Lane *L) {
         uint64_t need = 0;
         if (cs == CDS_OK) {
           need = estimate_counted_bytes(r->n, g.m, r->params.k, T);
+          // A graph the engine solves on its clique core (too many cliques
+          // to materialise) needs far less than its whole clique list; it
+          // runs with the whole budget to itself instead of being refused.
+          if (need > s->opts.hbm_budget_bytes && cds::solves_on_core(r->params.k, T))
+            need = s->opts.hbm_budget_bytes;
           if (need > s->opts.hbm_budget_bytes)
             cs = fail(CDS_ERR_OOM, "request needs ~%llu bytes for %llu cliques, budget %llu",
                       (unsigned long long)need, (unsigned long long)T,
