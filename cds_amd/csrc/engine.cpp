@@ -63,6 +63,9 @@ struct cds_engine {
   // keep every cooperative kernel within half the CUs, so neither lane's
   // kernel needs a CU the other lane's kernels may hold.
   bool coop_half = false;
+  // the batched search's single-workgroup flow steps as one HIP graph
+  // (built on first use; CDS_BATCH_GRAPH=0 launches them eagerly)
+  FlowBatchGraph *flow_graph = nullptr;
 };
 
 namespace {
@@ -844,6 +847,7 @@ extern "C" void cds_engine_destroy(cds_engine *e) {
     if (hipDeviceGetDefaultMemPool(&pool, e->device) == hipSuccess && pool)
       (void)hipMemPoolTrimTo(pool, 0);
   }
+  flow_batch_graph_free(e->flow_graph);
   if (e->pinned) (void)hipHostFree(e->pinned);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -2842,6 +2846,9 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   std::vector<char> active(B, 1);
   DBuf<unsigned long long> acc(&A);
   HIPCHK(acc.alloc(2));
+  const char *bg = getenv("CDS_BATCH_GRAPH");
+  const bool use_graph = !(bg && atoi(bg) == 0);
+  uint32_t graph_steps = 0;
   for (;;) {
     struct Pending {
       uint32_t b;
@@ -2945,10 +2952,12 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
       it.S.astop = fastop.p + vo[i];
       it.capT = pend[i].rq.capT;
       it.U = pend[i].rq.U;
-      // one workgroup per 64k arcs, at most a quarter of the CUs
+      // one workgroup per 64k arcs, at most a quarter of the CUs (the
+      // tuned flow width is the whole-network flow's of one graph of the
+      // union's size; only CDS_FLOW_BLOCKS overrides the per-problem size)
       const uint64_t want = (C.T * k + 65535) / 65536;
       it.nblocks = (uint32_t)std::max<uint64_t>(
-          1, std::min<uint64_t>(flow_blocks_of(e) ? flow_blocks_of(e) : want,
+          1, std::min<uint64_t>(e->flow_blocks ? e->flow_blocks : want,
                                 std::max(1u, coop_cap / 2)));
       it.first_block = 0;
     }
@@ -2990,10 +2999,15 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
         li.push_back(it);
       }
       coop = coop && blocks > 1;
-      FlowBatchItem *dli = fitems.p + item_at;
-      HIPCHK(hipMemcpyAsync(dli, li.data(), li.size() * sizeof(FlowBatchItem),
-                            hipMemcpyHostToDevice, st));
-      {
+      if (!coop && use_graph) {
+        // single-workgroup problems: the step is one graph launch
+        HIPCHK(flow_batch_graph(&e->flow_graph, li.data(), (uint32_t)li.size(), blocks,
+                                kGridFlowRounds, mv, fcnt.p + item_at, st));
+        ++graph_steps;
+      } else {
+        FlowBatchItem *dli = fitems.p + item_at;
+        HIPCHK(hipMemcpyAsync(dli, li.data(), li.size() * sizeof(FlowBatchItem),
+                              hipMemcpyHostToDevice, st));
         CoopGate gate(e, coop ? blocks : 1);
         HIPCHK(flow_batch(dli, (uint32_t)li.size(), blocks, coop, kGridFlowRounds, mv,
                           fcnt.p + item_at, st));
@@ -3027,6 +3041,7 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   for (uint32_t b = 0; b < B; ++b) {
     ExactSearch &S = *search[b];
     cds_result *R = rs[b];
+    R->flow_graph_launches = graph_steps;
     R->ub1_val = S.ub1_val;
     R->ub2_val = S.ub2_val;
     R->flow_networks_solved = S.flows_solved;

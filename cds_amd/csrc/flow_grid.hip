@@ -44,6 +44,8 @@
 // alternating vertex and clique levels) runs every relabel_rounds() rounds and
 // whenever no node is active; the solve ends only when that BFS finds no
 // active node, which certifies the preflow maximum.
+#include <cstring>
+
 #include "dev.h"
 #include "kernels.h"
 
@@ -1671,6 +1673,115 @@ hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
 }
 
 uint32_t imp_record_words(uint32_t k) { return imp_stride(k); }
+
+// ---- the batched search's flow step as a HIP graph ----
+// A graph of three nodes -- copy the step's items to the device, the
+// batched max flow (single-workgroup problems, a plain launch), the clique
+// count of each min-cut side -- built on the first step and re-aimed at
+// every later step's items, sizes and counters through the executable
+// graph's node parameters, so a step is one graph launch instead of three
+// stream operations.  The items are staged in pinned host memory the graph
+// copies from; the device copy is the graph's own buffer.
+struct FlowBatchGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t copy = nullptr, flow = nullptr, count = nullptr;
+  FlowBatchItem *host = nullptr, *dev = nullptr;
+  size_t cap = 0;  // items the two buffers hold
+  unsigned long long launches = 0;
+};
+
+namespace {
+hipKernelNodeParams flow_node(void **args, uint32_t blocks, uint32_t lds_cap) {
+  hipKernelNodeParams kp{};
+  kp.func = reinterpret_cast<void *>(k_flow_batch);
+  kp.gridDim = dim3(blocks);
+  kp.blockDim = dim3(kFlowBatchThreads);
+  kp.sharedMemBytes = (unsigned)((size_t)lds_cap * sizeof(long long));
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  return kp;
+}
+hipKernelNodeParams count_node(void **args, uint32_t count) {
+  hipKernelNodeParams kp{};
+  kp.func = reinterpret_cast<void *>(k_flow_batch_count);
+  kp.gridDim = dim3(count);
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  return kp;
+}
+}  // namespace
+
+void flow_batch_graph_free(FlowBatchGraph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->dev) (void)hipFree(g->dev);
+  if (g->host) (void)hipHostFree(g->host);
+  delete g;
+}
+
+unsigned long long flow_batch_graph_launches(const FlowBatchGraph *g) {
+  return g ? g->launches : 0;
+}
+
+hipError_t flow_batch_graph(FlowBatchGraph **pg, const FlowBatchItem *items, uint32_t count,
+                            uint32_t total_blocks, uint32_t max_rounds, uint32_t max_v,
+                            unsigned long long *reach_cliques, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  if (!*pg) *pg = new FlowBatchGraph();
+  FlowBatchGraph *g = *pg;
+  hipError_t e = hipSuccess;
+  if (count > g->cap) {
+    // the previous step's copy has run: every step ends in a readback
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    size_t cap = g->cap ? g->cap : 256;
+    while (cap < count) cap *= 2;
+    if (g->dev) (void)hipFree(g->dev);
+    if (g->host) (void)hipHostFree(g->host);
+    g->dev = nullptr;
+    g->host = nullptr;
+    g->cap = 0;
+    if ((e = hipMalloc(&g->dev, cap * sizeof(FlowBatchItem))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&g->host, cap * sizeof(FlowBatchItem), hipHostMallocDefault)) !=
+        hipSuccess)
+      return e;
+    g->cap = cap;
+  } else if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+    return e;  // the staging buffer may still feed the previous copy
+  }
+  memcpy(g->host, items, count * sizeof(FlowBatchItem));
+  uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;
+  const FlowBatchItem *dev = g->dev;
+  void *fargs[] = {(void *)&dev, &count, &max_rounds, &lds_cap};
+  void *cargs[] = {(void *)&dev, &reach_cliques};
+  const hipKernelNodeParams fp = flow_node(fargs, total_blocks, lds_cap);
+  const hipKernelNodeParams cp = count_node(cargs, count);
+  const size_t bytes = count * sizeof(FlowBatchItem);
+  if (!g->exec) {
+    if ((e = hipGraphCreate(&g->graph, 0)) != hipSuccess) return e;
+    if ((e = hipGraphAddMemcpyNode1D(&g->copy, g->graph, nullptr, 0, g->dev, g->host, bytes,
+                                     hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    if ((e = hipGraphAddKernelNode(&g->flow, g->graph, &g->copy, 1, &fp)) != hipSuccess)
+      return e;
+    if ((e = hipGraphAddKernelNode(&g->count, g->graph, &g->flow, 1, &cp)) != hipSuccess)
+      return e;
+    if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+      return e;
+  } else {
+    if ((e = hipGraphExecMemcpyNodeSetParams1D(g->exec, g->copy, g->dev, g->host, bytes,
+                                               hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    if ((e = hipGraphExecKernelNodeSetParams(g->exec, g->flow, &fp)) != hipSuccess) return e;
+    if ((e = hipGraphExecKernelNodeSetParams(g->exec, g->count, &cp)) != hipSuccess) return e;
+  }
+  if ((e = hipGraphLaunch(g->exec, s)) != hipSuccess) return e;
+  ++g->launches;
+  return hipSuccess;
+}
 
 unsigned flow_batch_capacity(uint32_t max_v, int device) {
   const uint32_t lds_cap = max_v < kGridLdsVertices ? max_v : kGridLdsVertices;

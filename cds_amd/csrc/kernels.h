@@ -493,6 +493,16 @@ hipError_t flow_batch(const FlowBatchItem *items, uint32_t count,
 // Workgroups of one cooperative flow_batch launch that are surely
 // co-resident (one per CU), 0 if the kernel cannot launch.
 unsigned flow_batch_capacity(uint32_t max_v, int device);
+// flow_batch of single-workgroup problems only (a plain launch) as one HIP
+// graph launch: *g (created on the first call, freed by
+// flow_batch_graph_free) keeps the graph, its pinned staging and device
+// copy of the items; `items` are host items (copied before return).
+struct FlowBatchGraph;
+hipError_t flow_batch_graph(FlowBatchGraph **g, const FlowBatchItem *items, uint32_t count,
+                            uint32_t total_blocks, uint32_t max_rounds, uint32_t max_v,
+                            unsigned long long *reach_cliques, hipStream_t s);
+void flow_batch_graph_free(FlowBatchGraph *g);
+unsigned long long flow_batch_graph_launches(const FlowBatchGraph *g);
 hipError_t grid_max_flow(const FlowProblem &P, const GridFlowState &S,
                          long long cap_t, long long unit, uint32_t max_rounds,
                          unsigned *grid, hipStream_t s);

@@ -151,6 +151,7 @@ std::string render(const cds_result *r) {
   j.u64("flow_arc_visits", r->flow_arc_visits);
   j.u64("flow_relabels", r->flow_relabels);
   j.u64("flow_implicit", r->flow_implicit);
+  j.u64("flow_graph_launches", r->flow_graph_launches);
   // steps whose exact decision the reference's float Edmonds-Karp may take
   // the other way (cds.h: trace_fragile)
   j.u64("fragile_decisions", r->fragile_decisions);

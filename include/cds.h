@@ -195,6 +195,10 @@ typedef struct {
    * ids and residuals, no clique heights, excess or work lists; the default
    * whenever k*k*U fits 31 bits); the others ran the explicit network. */
   uint32_t flow_implicit;
+  /* Batched solves (cds_solve_batch): flow steps of the call whose
+   * single-workgroup max flows ran as one HIP graph launch (the same for
+   * every graph of the call); 0 for cds_solve. */
+  uint32_t flow_graph_launches;
 } cds_result;
 
 typedef struct {

@@ -113,3 +113,35 @@ def test_batch_level_runs(engine, count, kk):
         assert diff is None, f"graph {i} (n={g.n}, k={kk}): batched {diff} differs"
     for dg in dgs:
         dg.free()
+
+
+def test_batch_flow_steps_as_graph_launches(engine):
+    """The batched search launches each step's single-workgroup max flows as
+    one HIP graph (built on the first step, re-aimed at every later step's
+    items through the executable graph's node parameters).  Its answers
+    equal the eager launches' (CDS_BATCH_GRAPH=0) and the plain path's over
+    the steps of two calls in a row, with the engine's other work in
+    between."""
+    import os
+
+    items = mixed_graphs(24, seed=17, nmax=2000)
+    dgs = [engine.upload(g) for g, _ in items]
+    ks = [k for _, k in items]
+    _announce(f"batch of {len(items)} graphs: flow steps as graph launches")
+    first = engine.solve_batch(dgs, ks)
+    second = engine.solve_batch(dgs[::-1], ks[::-1])[::-1]
+    os.environ["CDS_BATCH_GRAPH"] = "0"
+    try:
+        eager = engine.solve_batch(dgs, ks)
+    finally:
+        del os.environ["CDS_BATCH_GRAPH"]
+    # (a call's union of one k may need no flow at all: bounds decide it)
+    assert max(r["flow_graph_launches"] for r in first) > 0
+    assert max(r["flow_graph_launches"] for r in eager) == 0
+    for i, ((g, k), dg) in enumerate(zip(items, dgs)):
+        rp = engine.solve(dg, k)
+        for name, rb in (("graph", first[i]), ("graph again", second[i]), ("eager", eager[i])):
+            diff = same(rb, rp)
+            assert diff is None, f"graph {i} (n={g.n}, k={k}): {name} batched {diff} differs"
+    for dg in dgs:
+        dg.free()
