@@ -640,11 +640,16 @@ def main():
             "total_ms_per_solve": avg("t_flow_kernel_ms"),
             "algorithmic_bytes_per_solve": (flow_bytes_wide if wide else flow_bytes)(r, w["k"])}
     if avg("t_replay_ms"):
-        # exact replay of the top level(s); its bytes depend on the phase
-        # (one pass over the phase's incidence per removal), not counted here
-        kernels["k_replay_grid"] = {"launches_per_solve": max(1, avg("replay_grid_phases")),
-                                    "total_ms_per_solve": avg("t_replay_ms"),
-                                    "algorithmic_bytes_per_solve": None}
+        # exact replay of the levels the answer depends on; its bytes depend
+        # on the phase (a pass over the phase's incidence, or over bitset
+        # rows, per removal), not counted here
+        bits, grid = avg("replay_bits_phases"), avg("replay_grid_phases")
+        name = ("k_replay_bits" if bits and not grid else
+                "k_replay_grid" if grid and not bits else
+                "k_replay_bits + k_replay_grid" if bits else "k_replay")
+        kernels[name] = {"launches_per_solve": max(1, avg("replay_phases")),
+                         "total_ms_per_solve": avg("t_replay_ms"),
+                         "algorithmic_bytes_per_solve": None}
     for kv in kernels.values():
         t = kv["total_ms_per_solve"]
         if kv["algorithmic_bytes_per_solve"] is None:
@@ -702,6 +707,7 @@ def main():
                  "coarse_shift": r["peel_coarse_shift"],
                  "refined_steps": r["peel_refined_steps"],
                  "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms"),
+                 "replay_bits_phases": r["replay_bits_phases"],
                  # one thread's wall-clock split of the persistent launches
                  "split_ms": r["peel_split_ms"],
                  # graphs too large to materialise: solved on the prune_tau-core

@@ -331,6 +331,15 @@ static bool union_grid_replay() {
   return on;
 }
 
+// Bitset replay of levels with few vertices alive (k <= 3): -1 where the
+// whole-GPU replay would run (the default), 0 never (CDS_REPLAY_BITS=0),
+// 1 for every level it can take (CDS_REPLAY_BITS=1).
+static int replay_bits_mode() {
+  const char *v = getenv("CDS_REPLAY_BITS");
+  if (!v || !*v) return -1;
+  return atoi(v) != 0 ? 1 : 0;
+}
+
 static bool flow_trace_on() {
   const char *v = getenv("CDS_FLOW_TRACE");
   return v && *v && *v != '0';
@@ -1643,9 +1652,37 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         }
         DBuf<uint32_t> pinc(&A), partial(&A);
         DBuf<GridBarrier> gbar(&A);
+        const uint64_t N_start = n - PS.rm_before[ii];
+        const bool heavy = M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M;
+        // Few vertices alive, k <= 3: one workgroup on adjacency bitsets
+        // (CDS_REPLAY_BITS=0 never, =1 for every level it can take).
+        if ((k == 2 || k == 3) && sdeg.p && N_start <= kReplayBitsMaxVertices &&
+            (replay_bits_mode() == 1 || (replay_bits_mode() < 0 && heavy))) {
+          const uint32_t Nb = (uint32_t)N_start, Wb = (Nb + 63) / 64;
+          DBuf<unsigned long long> bits(&A);
+          HIPCHK(bits.alloc((size_t)Nb * Wb));
+          HIPCHK(hipEventRecord(tm.ev[22], st));
+          HIPCHK(replay_phase_bits(k, lv.level, n, g->off, g->adj, core.p, sdeg.p,
+                                   T - PS.kl_before[ii], Nb, (uint32_t)M, pv.p, posmap.p,
+                                   pc.p, bits.p, rout.p, tm.ev[23], (uint64_t)PS.kp,
+                                   replay_nstar(k, (uint64_t)PS.kp), st));
+          ReplayOut ro;
+          CHK(readback(e, &ro, rout.p, 1));
+          float prep = 0;
+          (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
+          R->t_replay_prep_ms += prep;
+          R->replay_phases++;
+          R->replay_bits_phases++;
+          if (ro.steps > M || (!ro.stopped && ro.steps != M))
+            return fail(CDS_ERR_INTERNAL, "bitset replay of level %u ran %u of %llu steps",
+                        lv.level, ro.steps, (unsigned long long)M);
+          replayed.push_back({lv.level, ro});
+          CHK(summarize());
+          continue;
+        }
         // Few vertices with many cliques each: replay on the whole GPU.
         unsigned rgrid = 0;
-        if (M <= kReplayGridMaxVertices && lv.killed * k >= e->replay_grid_work * M)
+        if (heavy)
           rgrid = coop_width(e, replay_grid_blocks(e->device, lv.killed * k / M,
                                                    e->cur.replay_blocks));
         if (rgrid) {

@@ -991,6 +991,209 @@ __global__ void k_zero_barrier(GridBarrier *b, unsigned long long limit) {
   if (threadIdx.x == 0) b->limit = limit;
 }
 
+// ---- replay on adjacency bitsets (k = 2, 3; few vertices alive) ----
+//
+// Positions 0..N-1 are the N vertices alive at the level's start (core >=
+// level: the phase vertices and those of higher levels); row i of `bits`
+// holds position i's neighbours among them (W = ceil(N/64) words).  When
+// phase vertex v goes, a phase vertex u adjacent to v loses the alive
+// cliques holding both: 1 for k = 2, and for k = 3 the triangles {u, v, w}
+// with w alive, popcount(N(u) & N(v) & alive).  One workgroup does every
+// removal with LDS barriers only: no incidence list is read and no grid
+// barrier is crossed.
+constexpr uint32_t kReplayBitsWords = kReplayBitsMaxVertices / 64;
+
+__global__ void k_collect_alive(uint32_t n, uint32_t level,
+                                const uint32_t *__restrict__ core,
+                                uint32_t *__restrict__ av,
+                                uint32_t *__restrict__ posmap, uint32_t *count) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t v = base + threadIdx.x;
+    const bool take = v < n && core[v] >= level;
+    unsigned cnt;
+    const unsigned slot = wave_slot(take, &cnt);
+    uint32_t start = 0;
+    if (lane() == 0 && cnt) start = atomicAdd(count, cnt);
+    start = __shfl(start, 0, kWave);
+    if (take) {
+      av[start + slot] = v;
+      posmap[v] = start + slot;
+    }
+  }
+}
+
+// Row i: the alive neighbours of av[i], one wave per row.  Positions at or
+// beyond N (an alive count other than the host's) are left out; the replay
+// then refuses to run.
+__global__ __launch_bounds__(256) void k_replay_rows(
+    uint32_t N, uint32_t W, uint32_t level, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ av, const uint32_t *__restrict__ posmap,
+    const uint32_t *__restrict__ core, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ adj, unsigned long long *__restrict__ bits) {
+  const uint32_t rows = min(N, *count);
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < rows;
+       i += waves) {
+    const uint32_t v = av[i];
+    unsigned long long *row = bits + (size_t)i * W;
+    for (uint32_t p = off[v] + lane(); p < off[v + 1]; p += kWave) {
+      const uint32_t u = adj[p];
+      if (u == v || core[u] < level) continue;
+      const uint32_t q = posmap[u];
+      if (q < rows) atomicOr(&row[q >> 6], 1ull << (q & 63));
+    }
+  }
+}
+
+// The replay itself: the removal order, recorded densities and early end of
+// k_replay_grid, from the level-start degrees sdeg.
+template <uint32_t K>
+__global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
+    uint32_t level, uint32_t N, uint32_t M, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ av, const uint32_t *__restrict__ posmap,
+    const uint32_t *__restrict__ core, const uint32_t *__restrict__ sdeg,
+    const unsigned long long *__restrict__ bits, uint64_t T_start, ReplayOut *out,
+    uint64_t kp, uint64_t nstar) {
+  constexpr uint32_t kWaves = kReplayThreads / kWave;
+  constexpr uint32_t kU = 8;  // rows in flight per lane
+  __shared__ uint32_t deg[kReplayBitsMaxVertices];
+  __shared__ uint32_t vid[kReplayBitsMaxVertices];
+  __shared__ uint32_t ul[kReplayBitsMaxVertices];
+  __shared__ unsigned long long alive[kReplayBitsWords], phase[kReplayBitsWords],
+      nbr[kReplayBitsWords];
+  __shared__ uint64_t s_wmin[kWaves];
+  __shared__ uint64_t s_key;
+  __shared__ uint32_t s_nu, s_stop;
+  const uint32_t tid = threadIdx.x, wid = tid / kWave, ln = lane();
+  if (*count != N || N > kReplayBitsMaxVertices) {
+    if (tid == 0) {  // not the alive set the host sized: 0 steps, an error
+      out->steps = 0;
+      out->found = 0;
+      out->stopped = 0;
+    }
+    return;
+  }
+  const uint32_t W = (N + 63) / 64;
+  for (uint32_t i = tid; i < N; i += kReplayThreads) {
+    const uint32_t v = av[i];
+    vid[i] = v;
+    deg[i] = core[v] == level ? sdeg[v] : 0;
+  }
+  for (uint32_t b = wid * kWave; b < W * kWave; b += kReplayThreads) {
+    const uint32_t i = b + ln;
+    const bool in = i < N;
+    const bool ph = in && core[av[i]] == level;
+    const unsigned long long ma = __ballot(in), mp = __ballot(ph);
+    if (ln == 0) {
+      alive[b / kWave] = ma;
+      phase[b / kWave] = mp;
+    }
+  }
+  // lanes per row: the least power of two >= W
+  uint32_t G = 1;
+  while (G < W) G <<= 1;
+  const uint32_t gpw = kWave / G, gl = ln % G;
+  const uint32_t chunk = kU * gpw;  // list entries per wave and pass
+  uint64_t R = 0, best_num = 0, best_den = 1;
+  uint32_t found = 0, steps = 0, stopped = 0;
+  __syncthreads();
+  for (uint32_t j = 0; j < M; ++j) {
+    uint64_t key = ~0ull;
+    for (uint32_t w = wid; w < W; w += kWaves) {
+      const unsigned long long live = alive[w] & phase[w];
+      if ((live >> ln) & 1ull) {
+        const uint32_t i = w * kWave + ln;
+        const uint64_t kk = ((uint64_t)deg[i] << 32) | vid[i];
+        key = kk < key ? kk : key;
+      }
+    }
+    key = wave_min64(key);
+    if (ln == 0) s_wmin[wid] = key;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t m = s_wmin[0];
+      for (uint32_t w = 1; w < kWaves; ++w) m = s_wmin[w] < m ? s_wmin[w] : m;
+      s_key = m;
+      s_stop = 0;
+      if (m != ~0ull) {
+        const uint32_t iv = posmap[(uint32_t)m];
+        alive[iv / kWave] &= ~(1ull << (iv % kWave));
+        R += m >> 32;
+        ++steps;
+        if (j + 1 < M) {
+          const uint64_t num = T_start - R, den = (uint64_t)N - (j + 1);
+          if (!found || frac_gt(num, den, best_num, best_den)) {
+            best_num = num;
+            best_den = den;
+            found = 1;
+          }
+          // no later row can exceed kp: end after this removal
+          if (nstar && (den <= nstar || num <= kp * nstar)) s_stop = stopped = 1;
+        }
+      }
+    }
+    __syncthreads();
+    const uint64_t kk = s_key;
+    if (kk == ~0ull) break;  // inconsistent counts; reported via steps
+    const uint32_t iv = posmap[(uint32_t)kk];
+    // v's alive neighbours; the phase ones among them listed in ul
+    if (wid == 0) {
+      unsigned long long cm = 0;
+      if (ln < W) {
+        const unsigned long long a = bits[(size_t)iv * W + ln] & alive[ln];
+        nbr[ln] = a;
+        cm = a & phase[ln];
+      }
+      const uint32_t c = (uint32_t)__popcll(cm);
+      uint32_t incl = c;
+#pragma unroll
+      for (int o = 1; o < (int)kWave; o <<= 1) {
+        const uint32_t x = __shfl_up(incl, o, kWave);
+        if ((int)ln >= o) incl += x;
+      }
+      uint32_t at = incl - c;
+      while (cm) {
+        ul[at++] = ln * kWave + (uint32_t)(__ffsll((long long)cm) - 1);
+        cm &= cm - 1;
+      }
+      if (ln == kWave - 1) s_nu = incl;
+    }
+    __syncthreads();
+    const uint32_t nu = s_nu;
+    if (K == 2) {
+      for (uint32_t t = tid; t < nu; t += kReplayThreads) deg[ul[t]] -= 1;
+    } else {
+      // one list entry per group of G lanes, kU of them per lane in flight
+      for (uint32_t base = wid * chunk; base < nu; base += kWaves * chunk) {
+        unsigned long long x[kU];
+#pragma unroll
+        for (uint32_t r = 0; r < kU; ++r) {
+          const uint32_t t = base + r * gpw + ln / G;
+          x[r] = 0;
+          if (t < nu && gl < W) x[r] = bits[(size_t)ul[t] * W + gl] & nbr[gl];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kU; ++r) {
+          const uint32_t t = base + r * gpw + ln / G;
+          uint32_t c = (uint32_t)__popcll(x[r]);
+          for (uint32_t o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, (int)o, kWave);
+          if (gl == 0 && t < nu) deg[ul[t]] -= c;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_stop) break;
+  }
+  if (tid == 0) {
+    out->best_num = best_num;
+    out->best_den = best_den;
+    out->steps = steps;
+    out->found = found;
+    out->stopped = stopped;
+  }
+}
+
 }  // namespace
 
 hipError_t build_incidence(uint64_t T, uint32_t k, const uint32_t *cliques,
@@ -1153,6 +1356,34 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
                   &rdead, &partial, &bar, &T_start, &N_start, &out, &kp, &nstar};
   return launch_grid_shrinking(reinterpret_cast<const void *>(k_replay_grid), grid,
                                kReplayThreads, args, 0, s);
+}
+
+hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *off,
+                             const uint32_t *adj, const uint32_t *core, const uint32_t *sdeg,
+                             uint64_t T_start, uint32_t N, uint32_t M, uint32_t *av,
+                             uint32_t *posmap, uint32_t *pcount, unsigned long long *bits,
+                             ReplayOut *out, hipEvent_t prepared, uint64_t kp, uint64_t nstar,
+                             hipStream_t s) {
+  if ((k != 2 && k != 3) || N == 0 || N > kReplayBitsMaxVertices || M == 0 || M > N)
+    return hipErrorInvalidValue;
+  const uint32_t W = (N + 63) / 64;
+  hipError_t e = hipMemsetAsync(pcount, 0, sizeof(uint32_t), s);
+  if (e != hipSuccess) return e;
+  if ((e = hipMemsetAsync(out, 0, sizeof(ReplayOut), s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(bits, 0, (size_t)N * W * 8, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_collect_alive, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
+                     level, core, av, posmap, pcount);
+  hipLaunchKernelGGL(k_replay_rows, dim3(grid_for((size_t)N * kWave, 256, 8192)), dim3(256),
+                     0, s, N, W, level, pcount, av, posmap, core, off, adj, bits);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (prepared) (void)hipEventRecord(prepared, s);
+  if (k == 2)
+    hipLaunchKernelGGL(k_replay_bits<2>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
+                       pcount, av, posmap, core, sdeg, bits, T_start, out, kp, nstar);
+  else
+    hipLaunchKernelGGL(k_replay_bits<3>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
+                       pcount, av, posmap, core, sdeg, bits, T_start, out, kp, nstar);
+  return hipGetLastError();
 }
 
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,

@@ -199,6 +199,9 @@ typedef struct {
    * single-workgroup max flows ran as one HIP graph launch (the same for
    * every graph of the call); 0 for cds_solve. */
   uint32_t flow_graph_launches;
+  /* Replayed levels (of replay_phases) that ran on adjacency bitsets in one
+   * workgroup (k <= 3, at most 4096 vertices alive at the level's start). */
+  uint32_t replay_bits_phases;
 } cds_result;
 
 typedef struct {
