@@ -68,6 +68,90 @@ __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
   return v;
 }
 
+// ---- DPP lane exchanges (a VALU operand modifier: no LDS round trip) ----
+// Inside rows of 16 lanes: xor 1 and xor 2 (quad permutes), the mirror
+// within 8 lanes and within 16 lanes; row shifts right by 1, 2, 4, 8 (lanes
+// shifted in from outside the row read the `old` operand, 0); and the GFX9
+// row broadcasts (lane 15 of each row into the next row; lane 31 into rows
+// 2 and 3).  After xor 1, xor 2, half mirror and mirror every lane of a row
+// holds a commutative reduction over the row.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
+constexpr int kDppMirror = 0x140, kDppHalfMirror = 0x141;
+constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xF, false);
+}
+
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = dpp<kCtrl>((uint32_t)v), hi = dpp<kCtrl>((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum over aligned groups of G lanes (G = 8, 16, 32 or 64), in every lane
+// of the group.  Every lane of the wave must be active.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
+  static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups of 8 to 64 lanes");
+  v += dpp<kDppXor1>(v);
+  v += dpp<kDppXor2>(v);
+  v += dpp<kDppHalfMirror>(v);
+  if (G >= 16) v += dpp<kDppMirror>(v);
+  if (G >= 32) v += __shfl_xor(v, 16, kWave);
+  if (G >= 64) v += __shfl_xor(v, 32, kWave);
+  return v;
+}
+
+// Smallest 64-bit key of the wave with a 32-bit payload from the lane that
+// holds it, in every lane (keys distinct, or equal keys with equal
+// payloads).  Every lane of the wave must be active.
+template <int kCtrl>
+__device__ __forceinline__ void min_step(uint64_t &key, uint32_t &pay) {
+  const uint64_t ok = dpp64<kCtrl>(key);
+  const uint32_t op = dpp<kCtrl>(pay);
+  if (ok < key) {
+    key = ok;
+    pay = op;
+  }
+}
+__device__ __forceinline__ void wave_argmin64(uint64_t &key, uint32_t &pay) {
+  min_step<kDppXor1>(key, pay);
+  min_step<kDppXor2>(key, pay);
+  min_step<kDppHalfMirror>(key, pay);
+  min_step<kDppMirror>(key, pay);
+#pragma unroll
+  for (int off = 16; off <= 32; off <<= 1) {
+    const uint64_t ok = __shfl_xor(key, off, kWave);
+    const uint32_t op = __shfl_xor(pay, off, kWave);
+    if (ok < key) {
+      key = ok;
+      pay = op;
+    }
+  }
+}
+// The same over the first 16 lanes (each row of 16 reduces on its own).
+__device__ __forceinline__ void row_argmin64(uint64_t &key, uint32_t &pay) {
+  min_step<kDppXor1>(key, pay);
+  min_step<kDppXor2>(key, pay);
+  min_step<kDppHalfMirror>(key, pay);
+  min_step<kDppMirror>(key, pay);
+}
+
+// Inclusive prefix sum over the wave's 64 lanes.  Every lane of the wave
+// must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += dpp<kDppShr1>(v);
+  v += dpp<kDppShr2>(v);
+  v += dpp<kDppShr4>(v);
+  v += dpp<kDppShr8>(v);
+  v += dpp<kDppBcast15, 0xA>(v);
+  v += dpp<kDppBcast31, 0xC>(v);
+  return v;
+}
+
 // Last index i < count with key[i] <= x, for non-decreasing key[] with
 // key[0] <= x.  Wave-cooperative 64-ary search: each step every lane loads
 // one sample, so the dependent loads number ceil(log64(count)) instead of

@@ -1662,7 +1662,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
           DBuf<unsigned long long> bits(&A);
           HIPCHK(bits.alloc((size_t)Nb * Wb));
           HIPCHK(hipEventRecord(tm.ev[22], st));
-          HIPCHK(replay_phase_bits(k, lv.level, n, g->off, g->adj, core.p, sdeg.p,
+          HIPCHK(replay_phase_bits(k, lv.level, n, L.doff.p, L.dadj.p, core.p, sdeg.p,
                                    T - PS.kl_before[ii], Nb, (uint32_t)M, pv.p, posmap.p,
                                    pc.p, bits.p, rout.p, tm.ev[23], (uint64_t)PS.kp,
                                    replay_nstar(k, (uint64_t)PS.kp), st));

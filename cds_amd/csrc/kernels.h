@@ -290,12 +290,12 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
 // Replays level `level` (M phase vertices) on adjacency bitsets, for k = 2
 // or 3 when the N vertices alive at the level's start (core >= level) are
 // at most kReplayBitsMaxVertices: one workgroup, no incidence lists, no grid
-// barriers.  off/adj: the graph's CSR; sdeg: the degrees at the level's
-// start; av and posmap hold n entries, bits N * ceil(N/64) words.  The
+// barriers.  doff/dadj: the degree-ordered DAG; sdeg: the degrees at the
+// level's start; av and posmap hold n entries, bits N * ceil(N/64) words.  The
 // kernel checks that N vertices are alive and otherwise reports 0 steps.
 constexpr uint32_t kReplayBitsMaxVertices = 4096;
-hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *off,
-                             const uint32_t *adj, const uint32_t *core, const uint32_t *sdeg,
+hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *doff,
+                             const uint32_t *dadj, const uint32_t *core, const uint32_t *sdeg,
                              uint64_t T_start, uint32_t N, uint32_t M, uint32_t *av,
                              uint32_t *posmap, uint32_t *pcount, unsigned long long *bits,
                              ReplayOut *out, hipEvent_t prepared, uint64_t kp, uint64_t nstar,

@@ -1023,48 +1023,90 @@ __global__ void k_collect_alive(uint32_t n, uint32_t level,
   }
 }
 
-// Row i: the alive neighbours of av[i], one wave per row.  Positions at or
-// beyond N (an alive count other than the host's) are left out; the replay
-// then refuses to run.
+// Rows from the degree-ordered DAG: every arc v -> u between alive
+// vertices sets bit u in v's row and bit v in u's row (hubs have short
+// out-lists, so the dense top of the graph costs little).  One wave per
+// position; positions at or beyond N (an alive count other than the host's)
+// are left out, and the replay then refuses to run.
 __global__ __launch_bounds__(256) void k_replay_rows(
     uint32_t N, uint32_t W, uint32_t level, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ av, const uint32_t *__restrict__ posmap,
-    const uint32_t *__restrict__ core, const uint32_t *__restrict__ off,
-    const uint32_t *__restrict__ adj, unsigned long long *__restrict__ bits) {
+    const uint32_t *__restrict__ core, const uint32_t *__restrict__ doff,
+    const uint32_t *__restrict__ dadj, unsigned long long *__restrict__ bits) {
   const uint32_t rows = min(N, *count);
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < rows;
        i += waves) {
     const uint32_t v = av[i];
-    unsigned long long *row = bits + (size_t)i * W;
-    for (uint32_t p = off[v] + lane(); p < off[v + 1]; p += kWave) {
-      const uint32_t u = adj[p];
-      if (u == v || core[u] < level) continue;
+    for (uint32_t p = doff[v] + lane(); p < doff[v + 1]; p += kWave) {
+      const uint32_t u = dadj[p];
+      if (core[u] < level) continue;
       const uint32_t q = posmap[u];
-      if (q < rows) atomicOr(&row[q >> 6], 1ull << (q & 63));
+      if (q >= rows) continue;
+      atomicOr(&bits[(size_t)i * W + (q >> 6)], 1ull << (q & 63));
+      atomicOr(&bits[(size_t)q * W + (i >> 6)], 1ull << (i & 63));
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Degree updates of one removal: list entry t (a phase position u adjacent
+// to v) loses popcount(row u & nbr), summed over a group of G >= W lanes
+// (one word each); kU entries per lane are loaded before any is summed.
+template <uint32_t K, uint32_t G>
+__device__ __forceinline__ void bits_dec(uint32_t nu, uint32_t W, const uint32_t *ul,
+                                         const unsigned long long *__restrict__ bits,
+                                         const unsigned long long *nbr, uint32_t *deg) {
+  constexpr uint32_t kWaves = kReplayThreads / kWave;
+  constexpr uint32_t kU = 16;
+  constexpr uint32_t gpw = kWave / G, chunk = kU * gpw;
+  const uint32_t wid = threadIdx.x / kWave, ln = lane();
+  const uint32_t gl = ln % G, grp = ln / G;
+  const unsigned long long mine = gl < W ? nbr[gl] : 0ull;
+  for (uint32_t base = wid * chunk; base < nu; base += kWaves * chunk) {
+    unsigned long long x[kU];
+#pragma unroll
+    for (uint32_t r = 0; r < kU; ++r) {
+      const uint32_t t = base + r * gpw + grp;
+      x[r] = 0;
+      if (t < nu && gl < W) x[r] = bits[(size_t)ul[t] * W + gl] & mine;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kU; ++r) {
+      const uint32_t t = base + r * gpw + grp;
+      const uint32_t c = group_sum<G>((uint32_t)__popcll(x[r]));
+      if (gl == 0 && t < nu) deg[ul[t]] -= c;
     }
   }
 }
 
 // The replay itself: the removal order, recorded densities and early end of
-// k_replay_grid, from the level-start degrees sdeg.
+// k_replay_grid, from the level-start degrees sdeg.  Per removal, three
+// workgroup barriers: (1) each wave's smallest (degree, id) key with its
+// position; (2) every wave reduces the keys, loads v's row and lists v's
+// alive phase neighbours for its share of the words; (3) the degree updates.
+// Wave reductions and scans go through DPP lane exchanges.
 template <uint32_t K>
 __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     uint32_t level, uint32_t N, uint32_t M, const uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ av, const uint32_t *__restrict__ posmap,
-    const uint32_t *__restrict__ core, const uint32_t *__restrict__ sdeg,
-    const unsigned long long *__restrict__ bits, uint64_t T_start, ReplayOut *out,
-    uint64_t kp, uint64_t nstar) {
+    const uint32_t *__restrict__ av, const uint32_t *__restrict__ core,
+    const uint32_t *__restrict__ sdeg, const unsigned long long *__restrict__ bits,
+    uint64_t T_start, ReplayOut *out, uint64_t kp, uint64_t nstar) {
   constexpr uint32_t kWaves = kReplayThreads / kWave;
-  constexpr uint32_t kU = 8;  // rows in flight per lane
+  static_assert(kWaves == 16, "the key reduction spans one row of 16 lanes");
   __shared__ uint32_t deg[kReplayBitsMaxVertices];
   __shared__ uint32_t vid[kReplayBitsMaxVertices];
   __shared__ uint32_t ul[kReplayBitsMaxVertices];
   __shared__ unsigned long long alive[kReplayBitsWords], phase[kReplayBitsWords],
       nbr[kReplayBitsWords];
   __shared__ uint64_t s_wmin[kWaves];
-  __shared__ uint64_t s_key;
-  __shared__ uint32_t s_nu, s_stop;
+  __shared__ uint32_t s_wpos[kWaves];
+  __shared__ uint32_t s_stop;
   const uint32_t tid = threadIdx.x, wid = tid / kWave, ln = lane();
   if (*count != N || N > kReplayBitsMaxVertices) {
     if (tid == 0) {  // not the alive set the host sized: 0 steps, an error
@@ -1075,10 +1117,13 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     return;
   }
   const uint32_t W = (N + 63) / 64;
-  for (uint32_t i = tid; i < N; i += kReplayThreads) {
-    const uint32_t v = av[i];
-    vid[i] = v;
-    deg[i] = core[v] == level ? sdeg[v] : 0;
+  for (uint32_t i = tid; i < kReplayBitsMaxVertices; i += kReplayThreads) {
+    ul[i] = 0;  // every entry a valid position, listed or not
+    if (i < N) {
+      const uint32_t v = av[i];
+      vid[i] = v;
+      deg[i] = core[v] == level ? sdeg[v] : 0;
+    }
   }
   for (uint32_t b = wid * kWave; b < W * kWave; b += kReplayThreads) {
     const uint32_t i = b + ln;
@@ -1090,97 +1135,85 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
       phase[b / kWave] = mp;
     }
   }
-  // lanes per row: the least power of two >= W
-  uint32_t G = 1;
-  while (G < W) G <<= 1;
-  const uint32_t gpw = kWave / G, gl = ln % G;
-  const uint32_t chunk = kU * gpw;  // list entries per wave and pass
+  if (tid == 0) s_stop = 0;
+  const uint32_t G = W <= 8 ? 8 : W <= 16 ? 16 : W <= 32 ? 32 : 64;
   uint64_t R = 0, best_num = 0, best_den = 1;
   uint32_t found = 0, steps = 0, stopped = 0;
   __syncthreads();
   for (uint32_t j = 0; j < M; ++j) {
+    // (1) this wave's smallest key among the alive phase positions
     uint64_t key = ~0ull;
+    uint32_t pos = 0;
     for (uint32_t w = wid; w < W; w += kWaves) {
       const unsigned long long live = alive[w] & phase[w];
       if ((live >> ln) & 1ull) {
         const uint32_t i = w * kWave + ln;
         const uint64_t kk = ((uint64_t)deg[i] << 32) | vid[i];
-        key = kk < key ? kk : key;
-      }
-    }
-    key = wave_min64(key);
-    if (ln == 0) s_wmin[wid] = key;
-    __syncthreads();
-    if (tid == 0) {
-      uint64_t m = s_wmin[0];
-      for (uint32_t w = 1; w < kWaves; ++w) m = s_wmin[w] < m ? s_wmin[w] : m;
-      s_key = m;
-      s_stop = 0;
-      if (m != ~0ull) {
-        const uint32_t iv = posmap[(uint32_t)m];
-        alive[iv / kWave] &= ~(1ull << (iv % kWave));
-        R += m >> 32;
-        ++steps;
-        if (j + 1 < M) {
-          const uint64_t num = T_start - R, den = (uint64_t)N - (j + 1);
-          if (!found || frac_gt(num, den, best_num, best_den)) {
-            best_num = num;
-            best_den = den;
-            found = 1;
-          }
-          // no later row can exceed kp: end after this removal
-          if (nstar && (den <= nstar || num <= kp * nstar)) s_stop = stopped = 1;
+        if (kk < key) {
+          key = kk;
+          pos = i;
         }
       }
     }
-    __syncthreads();
-    const uint64_t kk = s_key;
-    if (kk == ~0ull) break;  // inconsistent counts; reported via steps
-    const uint32_t iv = posmap[(uint32_t)kk];
-    // v's alive neighbours; the phase ones among them listed in ul
-    if (wid == 0) {
-      unsigned long long cm = 0;
-      if (ln < W) {
-        const unsigned long long a = bits[(size_t)iv * W + ln] & alive[ln];
-        nbr[ln] = a;
-        cm = a & phase[ln];
-      }
-      const uint32_t c = (uint32_t)__popcll(cm);
-      uint32_t incl = c;
-#pragma unroll
-      for (int o = 1; o < (int)kWave; o <<= 1) {
-        const uint32_t x = __shfl_up(incl, o, kWave);
-        if ((int)ln >= o) incl += x;
-      }
-      uint32_t at = incl - c;
-      while (cm) {
-        ul[at++] = ln * kWave + (uint32_t)(__ffsll((long long)cm) - 1);
-        cm &= cm - 1;
-      }
-      if (ln == kWave - 1) s_nu = incl;
+    wave_argmin64(key, pos);
+    if (ln == 0) {
+      s_wmin[wid] = key;
+      s_wpos[wid] = pos;
     }
     __syncthreads();
-    const uint32_t nu = s_nu;
+    // (2) every wave: the removed position iv and v's alive neighbours
+    uint64_t mk = ln < kWaves ? s_wmin[ln] : ~0ull;
+    uint32_t mp = ln < kWaves ? s_wpos[ln] : 0;
+    row_argmin64(mk, mp);
+    const uint64_t m = readlane64(mk, 0);
+    const uint32_t iv = (uint32_t)__builtin_amdgcn_readlane((int)mp, 0);
+    if (m == ~0ull) break;  // inconsistent counts; reported via steps
+    unsigned long long a = 0, cm = 0;
+    if (ln < W) {
+      a = bits[(size_t)iv * W + ln] & alive[ln];
+      if (ln == iv / kWave) a &= ~(1ull << (iv % kWave));
+      cm = a & phase[ln];
+    }
+    const uint32_t c = (uint32_t)__popcll(cm);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t excl = incl - c;
+    const uint32_t nu = min((uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1), N);
+    for (uint32_t w = wid; w < W; w += kWaves) {
+      const unsigned long long cw = readlane64(cm, w), aw = readlane64(a, w);
+      const uint32_t ew = (uint32_t)__builtin_amdgcn_readlane((int)excl, (int)w);
+      if ((cw >> ln) & 1ull) {
+        const uint32_t at = ew + (uint32_t)__popcll(cw & ((1ull << ln) - 1));
+        if (at < kReplayBitsMaxVertices) ul[at] = w * kWave + ln;
+      }
+      if (ln == 0) nbr[w] = aw;
+    }
+    if (tid == 0) {
+      alive[iv / kWave] &= ~(1ull << (iv % kWave));
+      R += m >> 32;
+      ++steps;
+      if (j + 1 < M) {
+        const uint64_t num = T_start - R, den = (uint64_t)N - (j + 1);
+        if (!found || frac_gt(num, den, best_num, best_den)) {
+          best_num = num;
+          best_den = den;
+          found = 1;
+        }
+        // no later row can exceed kp: end after this removal
+        if (nstar && (den <= nstar || num <= kp * nstar)) s_stop = stopped = 1;
+      }
+    }
+    __syncthreads();
+    // (3) degree updates of the listed neighbours
     if (K == 2) {
       for (uint32_t t = tid; t < nu; t += kReplayThreads) deg[ul[t]] -= 1;
+    } else if (G == 8) {
+      bits_dec<K, 8>(nu, W, ul, bits, nbr, deg);
+    } else if (G == 16) {
+      bits_dec<K, 16>(nu, W, ul, bits, nbr, deg);
+    } else if (G == 32) {
+      bits_dec<K, 32>(nu, W, ul, bits, nbr, deg);
     } else {
-      // one list entry per group of G lanes, kU of them per lane in flight
-      for (uint32_t base = wid * chunk; base < nu; base += kWaves * chunk) {
-        unsigned long long x[kU];
-#pragma unroll
-        for (uint32_t r = 0; r < kU; ++r) {
-          const uint32_t t = base + r * gpw + ln / G;
-          x[r] = 0;
-          if (t < nu && gl < W) x[r] = bits[(size_t)ul[t] * W + gl] & nbr[gl];
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < kU; ++r) {
-          const uint32_t t = base + r * gpw + ln / G;
-          uint32_t c = (uint32_t)__popcll(x[r]);
-          for (uint32_t o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, (int)o, kWave);
-          if (gl == 0 && t < nu) deg[ul[t]] -= c;
-        }
-      }
+      bits_dec<K, 64>(nu, W, ul, bits, nbr, deg);
     }
     __syncthreads();
     if (s_stop) break;
@@ -1358,8 +1391,8 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
                                kReplayThreads, args, 0, s);
 }
 
-hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *off,
-                             const uint32_t *adj, const uint32_t *core, const uint32_t *sdeg,
+hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *doff,
+                             const uint32_t *dadj, const uint32_t *core, const uint32_t *sdeg,
                              uint64_t T_start, uint32_t N, uint32_t M, uint32_t *av,
                              uint32_t *posmap, uint32_t *pcount, unsigned long long *bits,
                              ReplayOut *out, hipEvent_t prepared, uint64_t kp, uint64_t nstar,
@@ -1374,15 +1407,15 @@ hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint3
   hipLaunchKernelGGL(k_collect_alive, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
                      level, core, av, posmap, pcount);
   hipLaunchKernelGGL(k_replay_rows, dim3(grid_for((size_t)N * kWave, 256, 8192)), dim3(256),
-                     0, s, N, W, level, pcount, av, posmap, core, off, adj, bits);
+                     0, s, N, W, level, pcount, av, posmap, core, doff, dadj, bits);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (prepared) (void)hipEventRecord(prepared, s);
   if (k == 2)
     hipLaunchKernelGGL(k_replay_bits<2>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
-                       pcount, av, posmap, core, sdeg, bits, T_start, out, kp, nstar);
+                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar);
   else
     hipLaunchKernelGGL(k_replay_bits<3>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
-                       pcount, av, posmap, core, sdeg, bits, T_start, out, kp, nstar);
+                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar);
   return hipGetLastError();
 }
 
