@@ -227,6 +227,8 @@ __global__ __launch_bounds__(256) void k_peel_process(
 // serialising on one word).  Kill words are derived after the launch
 // (k_kill_from_core).
 constexpr uint32_t kPeelChunk = 64;     // one wave pass per chunk
+constexpr uint32_t kPeelBatch = 4;      // chunks per wave pass with loads in flight
+constexpr uint32_t kPeelMaxK = 4;       // incidence entries hold k <= 4 words
 constexpr int kPeelThreads = 1024;
 constexpr uint64_t kChunkMask = (1ull << 40) - 1;
 // alive vertices aggregated in LDS (128 KB; the persistent peel runs one
@@ -446,35 +448,95 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
       const uint64_t *FCc = FC[fb];
       uint32_t nk = 0;
       unsigned long long tp = leader ? wall_clock64() : 0;
-      for (uint64_t ci = gwave; ci < nchunks; ci += gwaves) {
-        const uint32_t lo = wave_last_le(FCc, fcount, ci);  // ci's vertex slot
-        const uint32_t v = Fc[lo];
-        const uint64_t b = ioff[v] + (ci - FCc[lo]) * kPeelChunk;
-        const uint64_t e = min(ioff[v + 1], b + kPeelChunk);
-        for (uint64_t q = b + lane(); q < e; q += kWave) {
-          const uint32_t *en = inc + q * k;
-          bool claim = true;
-          for (uint32_t t = 1; t < k; ++t) {
-            const uint32_t u = en[t];
-            const uint32_t su = ld(&stamp[u]);
-            claim = claim && su >= sr && (su != sr || u > v);
-          }
-          if (!claim) continue;
-          ++nk;
-          for (uint32_t t = 1; t < k; ++t) {
-            const uint32_t u = en[t];
-            if (ld(&core[u]) != kAlive) continue;
-            if (agg) {
-              atomicAdd(&lds_cnt[pos[u]], 1u);
-              continue;
+      // Each wave takes a contiguous range of the sub-round's chunks: one
+      // search finds the first chunk's vertex, later chunks walk forward
+      // (a vertex change is rare next to the chunks of the large lists), and
+      // kPeelBatch chunks per pass have their loads in flight together.
+      const uint64_t per = (nchunks + gwaves - 1) / gwaves;
+      const uint64_t c0 = min(nchunks, (uint64_t)gwave * per);
+      const uint64_t c1 = min(nchunks, c0 + per);
+      if (c0 < c1) {
+        uint32_t lo = wave_last_le(FCc, fcount, c0);  // c0's vertex slot
+        uint32_t v = Fc[lo];
+        uint64_t vb = ioff[v], ve = ioff[v + 1], vfc = FCc[lo];
+        uint64_t nfc = lo + 1 < fcount ? FCc[lo + 1] : ~0ull;
+        for (uint64_t ci = c0; ci < c1; ci += kPeelBatch) {
+          uint32_t bv[kPeelBatch];
+          uint64_t bq[kPeelBatch];
+          bool bok[kPeelBatch];
+#pragma unroll
+          for (uint32_t s = 0; s < kPeelBatch; ++s) {
+            const uint64_t cs = ci + s;
+            bok[s] = false;
+            bv[s] = 0;
+            bq[s] = 0;
+            if (cs < c1) {
+              while (cs >= nfc) {  // next vertex (empty lists own no chunk)
+                ++lo;
+                v = Fc[lo];
+                vb = ioff[v];
+                ve = ioff[v + 1];
+                vfc = nfc;
+                nfc = lo + 1 < fcount ? FCc[lo + 1] : ~0ull;
+              }
+              const uint64_t q = vb + (cs - vfc) * kPeelChunk + lane();
+              bok[s] = q < ve;
+              bv[s] = v;
+              bq[s] = q;
             }
-            if (hagg && hash_add(lds_cnt, u)) continue;
-            const uint32_t old = atomicSub(&deg[u], 1u);
-            if (old == level + 1) {
-              core[u] = level;
-              stamp[u] = sr + 1;
-              frontier_push(ctl, nb, F[nb], FC[nb], u,
-                            chunks_of(ioff[u + 1] - ioff[u]));
+          }
+          // the entries' other members, then their stamps (and alive-list
+          // positions when aggregating), all loads of the batch issued
+          // before any is used
+          uint32_t mu[kPeelBatch][kPeelMaxK - 1], msu[kPeelBatch][kPeelMaxK - 1],
+              mpos[kPeelBatch][kPeelMaxK - 1];
+#pragma unroll
+          for (uint32_t s = 0; s < kPeelBatch; ++s)
+#pragma unroll
+            for (uint32_t t = 1; t < kPeelMaxK; ++t)
+              mu[s][t - 1] = bok[s] && t < k ? inc[bq[s] * k + t] : 0u;
+#pragma unroll
+          for (uint32_t s = 0; s < kPeelBatch; ++s)
+#pragma unroll
+            for (uint32_t t = 1; t < kPeelMaxK; ++t) {
+              msu[s][t - 1] = kAlive;
+              mpos[s][t - 1] = 0;
+              if (bok[s] && t < k) {
+                msu[s][t - 1] = ld(&stamp[mu[s][t - 1]]);
+                if (agg) mpos[s][t - 1] = pos[mu[s][t - 1]];
+              }
+            }
+#pragma unroll
+          for (uint32_t s = 0; s < kPeelBatch; ++s) {
+            if (!bok[s]) continue;
+            bool claim = true;
+#pragma unroll
+            for (uint32_t t = 1; t < kPeelMaxK; ++t)
+              if (t < k) {
+                const uint32_t su = msu[s][t - 1];
+                claim = claim && su >= sr && (su != sr || mu[s][t - 1] > bv[s]);
+              }
+            if (!claim) continue;
+            ++nk;
+#pragma unroll
+            for (uint32_t t = 1; t < kPeelMaxK; ++t) {
+              if (t >= k) continue;
+              // members of this frontier (stamp sr) are removed and parked
+              // ones keep their core value in deg: neither is decremented
+              const uint32_t su = msu[s][t - 1], u = mu[s][t - 1];
+              if (su == sr || su == kParked) continue;
+              if (agg) {
+                atomicAdd(&lds_cnt[mpos[s][t - 1]], 1u);
+                continue;
+              }
+              if (hagg && hash_add(lds_cnt, u)) continue;
+              const uint32_t old = atomicSub(&deg[u], 1u);
+              if (old == level + 1) {
+                core[u] = level;
+                stamp[u] = sr + 1;
+                frontier_push(ctl, nb, F[nb], FC[nb], u,
+                              chunks_of(ioff[u + 1] - ioff[u]));
+              }
             }
           }
         }
@@ -591,15 +653,17 @@ __global__ void k_refine_restore(uint32_t n, uint32_t *__restrict__ core,
     if (core[v] == kParked) core[v] = deg[v];
 }
 
-// Stamps before a launch: vertices the launch may meet alive (alive, or
-// parked in a later coarse step) compare as never removed; vertices removed
-// by earlier launches keep stamps below the launch's first sub-round.
+// Stamps before a launch: vertices the launch may meet alive compare as
+// never removed: kAlive, or kParked for those parked in a later coarse step
+// (they take part in claims but get no decrements: their deg holds their
+// core value); vertices removed by earlier launches keep stamps below the
+// launch's first sub-round.
 __global__ void k_stamp_reset(uint32_t n, const uint32_t *__restrict__ core,
                               uint32_t *__restrict__ stamp, uint32_t first) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     const uint32_t c = core[v];
-    if (c == kAlive || c == kParked) stamp[v] = kAlive;
+    if (c == kAlive || c == kParked) stamp[v] = c;
     else if (first) stamp[v] = 0;
   }
 }
