@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <optional>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -1766,11 +1767,27 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   const uint32_t kpu = (uint32_t)kp;
   DBuf<unsigned long long> acc(&A);
   HIPCHK(acc.alloc(2));
-  HIPCHK(core_edge_count(n, g->off, g->adj, core.p, kpu, acc.p, st));
+  // the core's vertices (cv, numbered by cidx) and its cliques (cc) as
+  // lists, so the core stages below visit them instead of every vertex
+  // and clique of the graph
+  DBuf<uint32_t> cv(&A), cidx(&A), ccount1(&A), cc(&A);
+  HIPCHK(cv.alloc(core_n));
+  HIPCHK(cidx.alloc(n));
+  HIPCHK(ccount1.alloc(1));
+  HIPCHK(core_list(n, core.p, kpu, cidx.p, cv.p, core_n, ccount1.p, st));
+  {
+    uint32_t listed = 0;
+    CHK(readback(e, &listed, ccount1.p, 1));
+    if (listed != core_n)
+      return fail(CDS_ERR_INTERNAL, "core list holds %u of %u vertices", listed, core_n);
+  }
+  HIPCHK(core_edge_count(core_n, g->off, g->adj, core.p, kpu, acc.p, st, cv.p));
   unsigned long long ce = 0;
   CHK(readback(e, &ce, acc.p, 1));
   R->core_edges = ce / 2;
-  HIPCHK(count_kill_ge(T, kill.p, kpu, acc.p, st));
+  const bool list_cc = !core_only && core_T < 0xFFFFFFFFull;
+  if (list_cc) HIPCHK(cc.alloc(core_T));
+  HIPCHK(core_cliques(T, kill.p, kpu, list_cc ? cc.p : nullptr, acc.p, st));
   unsigned long long ct = 0;
   CHK(readback(e, &ct, acc.p, 1));
   if (ct != core_T)
@@ -1812,19 +1829,13 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(mark.alloc(na));
     HIPCHK(mark.zero());
     if (core_n <= kDenseCoreMax) {
-      DBuf<uint32_t> cidx(&A), ccount(&A), cmat(&A);
-      HIPCHK(cidx.alloc(n));
-      HIPCHK(ccount.alloc(1));
+      DBuf<uint32_t> cmat(&A);
       HIPCHK(cmat.alloc((size_t)core_n * core_n));
-      HIPCHK(prune_mark_dense(T, k, L.cliques.p, kill.p, kpu, n, core.p, g->off, g->adj,
-                              core_n, cidx.p, ccount.p, cmat.p, mark.p, st));
-      uint32_t counted = 0;
-      CHK(readback(e, &counted, ccount.p, 1));
-      if (counted != core_n)
-        return fail(CDS_ERR_INTERNAL, "dense prune indexed %u of %u core vertices", counted,
-                    core_n);
+      HIPCHK(prune_mark_dense(core_T, k, L.cliques.p, cc.p, cv.p, core_n, cidx.p, g->off,
+                              g->adj, cmat.p, mark.p, st));
     } else {
-      HIPCHK(prune_mark(T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st));
+      HIPCHK(prune_mark(core_T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st,
+                        cc.p));
     }
     HIPCHK(count_nonzero_u8(na, mark.p, acc.p, st));
     unsigned long long pe = 0;
@@ -1837,27 +1848,43 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint32_t> parent(&A);
   HIPCHK(parent.alloc(n));
   HIPCHK(uf_init(n, parent.p, st));
-  // components of the pruned graph = classes of "shares a core clique"
-  HIPCHK(uf_union_edges(n, g->off, g->adj, mark.p, core.p, kpu, parent.p, st));
+  // components of the pruned graph = classes of "shares a core clique";
+  // only the core vertices are visited, read back and numbered
+  HIPCHK(uf_union_edges(core_n, g->off, g->adj, mark.p, core.p, kpu, parent.p, st, cv.p));
   HIPCHK(uf_flatten(n, parent.p, st));
-  std::vector<uint32_t> hpar(n), hcore(n);
-  CHK(readback(e, hpar.data(), parent.p, n));
-  CHK(readback(e, hcore.data(), core.p, n));
-  std::vector<uint32_t> comp_of(n, 0), fv_of(n, 0), fv_orig(core_n);
+  std::vector<uint32_t> hcv(core_n), hpar(core_n);
+  {
+    DBuf<uint32_t> cpar(&A);
+    HIPCHK(cpar.alloc(core_n));
+    HIPCHK(gather_u32(core_n, cv.p, parent.p, cpar.p, st));
+    CHK(readback(e, hcv.data(), cv.p, core_n));
+    CHK(readback(e, hpar.data(), cpar.p, core_n));
+  }
+  // core vertices ascending: components in order of their smallest vertex
+  // (every root is its component's smallest vertex), flow ids ascending
+  // within each component
+  std::vector<uint32_t> ord(core_n);
+  for (uint32_t i = 0; i < core_n; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return hcv[a] < hcv[b]; });
+  std::vector<uint32_t> comp_of(core_n, 0), fv_of(core_n, 0), fv_orig(core_n);
   std::vector<Component> comps;
   {
-    std::vector<uint32_t> cidx(n, 0);
-    for (uint32_t v = 0; v < n; ++v) {
-      if (hcore[v] < kpu) continue;
-      if (hpar[v] == v) {
-        cidx[v] = (uint32_t)comps.size();
+    std::unordered_map<uint32_t, uint32_t> root_comp;
+    root_comp.reserve(core_n);
+    for (uint32_t r = 0; r < core_n; ++r) {
+      const uint32_t i = ord[r], v = hcv[i];
+      if (hpar[i] == v) {
+        root_comp[v] = (uint32_t)comps.size();
         Component c;
         c.first = v;
         comps.push_back(c);
       }
-      const uint32_t ci = cidx[hpar[v]];
-      comp_of[v] = ci;
-      comps[ci].V++;
+      const auto it = root_comp.find(hpar[i]);
+      if (it == root_comp.end())
+        return fail(CDS_ERR_INTERNAL, "core vertex %u has root %u outside the core", v,
+                    hpar[i]);
+      comp_of[i] = it->second;
+      comps[it->second].V++;
     }
     uint32_t base = 0;
     for (auto &c : comps) {
@@ -1867,11 +1894,11 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     if (base != core_n)
       return fail(CDS_ERR_INTERNAL, "component sizes %u != core %u", base, core_n);
     std::vector<uint32_t> fill(comps.size(), 0);
-    for (uint32_t v = 0; v < n; ++v) {
-      if (hcore[v] < kpu) continue;
-      const uint32_t ci = comp_of[v];
-      fv_of[v] = comps[ci].vbase + fill[ci]++;
-      fv_orig[fv_of[v]] = v;
+    for (uint32_t r = 0; r < core_n; ++r) {
+      const uint32_t i = ord[r];
+      const uint32_t ci = comp_of[i];
+      fv_of[i] = comps[ci].vbase + fill[ci]++;
+      fv_orig[fv_of[i]] = hcv[i];
     }
   }
   const uint32_t ncomp = (uint32_t)comps.size();
@@ -1881,13 +1908,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint64_t> coff(&A), fioff(&A);
   HIPCHK(d_comp_of.alloc(n));
   HIPCHK(d_fv_of.alloc(n));
-  CHK(upload(e, d_comp_of.p, comp_of.data(), n));
-  CHK(upload(e, d_fv_of.p, fv_of.data(), n));
+  {
+    // by vertex id, set for the core vertices only (the only ones read)
+    DBuf<uint32_t> lcomp(&A), lfv(&A);
+    HIPCHK(lcomp.alloc(core_n));
+    HIPCHK(lfv.alloc(core_n));
+    CHK(upload(e, lcomp.p, comp_of.data(), core_n));
+    CHK(upload(e, lfv.p, fv_of.data(), core_n));
+    HIPCHK(scatter_u32x2(core_n, cv.p, lcomp.p, lfv.p, d_comp_of.p, d_fv_of.p, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   HIPCHK(ccount.alloc(ncomp));
   HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_n));
-  HIPCHK(comp_clique_totals(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, ncomp,
-                            ccount.p, st));
+  HIPCHK(comp_clique_totals(core_T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, ncomp,
+                            ccount.p, st, cc.p));
   std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0);
   std::vector<uint64_t> hfioff(core_n + 1, 0);
   CHK(readback(e, hcc.data(), ccount.p, ncomp));
@@ -1909,12 +1944,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // flow (the explicit kernels read the members with the same stride).
   const uint32_t ms = imp_record_words(k);
   HIPCHK(fmem.alloc(core_T * ms));
-  {
-    DBuf<uint32_t> cslot(&A);  // flow index of each core clique
-    HIPCHK(cslot.alloc(T));
-    HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill.p, kpu, d_comp_of.p,
-                              d_fv_of.p, ncomp, coff.p, ccur.p, fmem.p, cslot.p, st, ms));
-  }
+  HIPCHK(comp_cliques_build(core_T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, d_fv_of.p,
+                            ncomp, coff.p, ccur.p, fmem.p, nullptr, st, ms, cc.p));
+  cc.reset();
   // flow incidence: per core vertex, its (clique slot, position) pairs
   HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st, ms));
   CHK(readback(e, hfcd.data(), fcd.p, core_n));
@@ -2162,35 +2194,26 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   R->density = (float)search.best_density;
   R->size = search.best_size;
   std::vector<uint32_t> verts;
+  // the set lies in one component of the densest core: its flow-vertex ids
+  std::vector<uint8_t> hf(core_n, 0);
   {
     const Component &C = comps[search.best_comp];
     for (uint32_t v = 0; v < C.V; ++v)
-      if (search.best_res.empty() || search.best_res[v] != -1)
+      if (search.best_res.empty() || search.best_res[v] != -1) {
         verts.push_back(fv_orig[C.vbase + v]);
+        hf[C.vbase + v] = 1;
+      }
   }
   R->set_size = (uint32_t)verts.size();
   R->vertices = host_alloc<uint32_t>(verts.size());
   memcpy(R->vertices, verts.data(), verts.size() * 4);
   {
-    // k-cliques inside the reported set (exact numerator)
+    // k-cliques inside the reported set (exact numerator): they are core
+    // cliques, so they are counted over the core's clique records
     DBuf<uint8_t> mask(&A);
-    HIPCHK(mask.alloc(n));
-    HIPCHK(mask.zero());
-    std::vector<uint8_t> hm(n, 0);
-    for (uint32_t v : verts) hm[v] = 1;
-    if (wide) {
-      // the set lies in the densest core, so do its cliques: count them
-      // over the core's cliques, in flow-vertex ids
-      std::vector<uint8_t> hf(core_n, 0);
-      for (uint32_t v : verts)
-        if (hcore[v] >= kpu) hf[fv_of[v]] = 1;
-      HIPCHK(mask.alloc(core_n));
-      CHK(upload(e, mask.p, hf.data(), core_n));
-      HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st, ms));
-    } else {
-      CHK(upload(e, mask.p, hm.data(), n));
-      HIPCHK(count_cliques_in_mask(T, k, L.cliques.p, mask.p, acc.p, st));
-    }
+    HIPCHK(mask.alloc(core_n));
+    CHK(upload(e, mask.p, hf.data(), core_n));
+    HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st, ms));
     unsigned long long sc = 0;
     CHK(readback(e, &sc, acc.p, 1));
     R->set_cliques = sc;
@@ -2200,7 +2223,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     R->clique_degree = host_alloc<uint64_t>(n);
     memcpy(R->clique_degree, cd.data(), n * 8);
     R->core = host_alloc<uint32_t>(n);
-    memcpy(R->core, hcore.data(), n * 4);
+    CHK(readback(e, R->core, core.p, n));
   }
   HIPCHK(hipStreamSynchronize(st));
   R->times.dag = tm.ms(ev_start, ev_dag);
@@ -2844,10 +2867,8 @@ cds_status solve_union(cds_engine *e, const std::vector<const cds_dgraph *> &gs,
   HIPCHK(ccur.zero());
   HIPCHK(fmem.alloc(core_T_total * k));
   {
-    DBuf<uint32_t> cslot(&A);
-    HIPCHK(cslot.alloc(T));
     HIPCHK(comp_cliques_build(T, k, L.cliques.p, kill01.p, 1, d_comp_of.p, d_fv_of.p,
-                              ncomp, coff.p, ccur.p, fmem.p, cslot.p, st));
+                              ncomp, coff.p, ccur.p, fmem.p, nullptr, st));
   }
   HIPCHK(flow_incidence_degrees(core_T_total, k, fmem.p, core_total, fcd.p, st));
   CHK(readback(e, hfcd.data(), fcd.p, core_total));

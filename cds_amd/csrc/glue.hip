@@ -20,15 +20,40 @@
 namespace cds {
 namespace {
 
+// Core vertices: every vertex (n of them) with core >= kp, or the n listed
+// in cv when cv is given (all of them core vertices).
+__device__ __forceinline__ bool core_vertex(const uint32_t *cv, const uint32_t *core,
+                                            uint32_t kp, uint32_t i, uint32_t *v) {
+  if (cv) {
+    *v = cv[i];
+    return true;
+  }
+  *v = i;
+  return core[i] >= kp;
+}
+
+// A core clique by index: the i-th of the list cc when given (T entries),
+// else clique i when its kill level is >= kp.
+__device__ __forceinline__ bool core_clique(const uint32_t *cc, const uint32_t *kill,
+                                            uint32_t kp, uint64_t i, uint64_t *c) {
+  if (cc) {
+    *c = cc[i];
+    return true;
+  }
+  *c = i;
+  return kill[i] >= kp;
+}
+
 __global__ __launch_bounds__(256) void k_core_edges(
     uint32_t n, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ adj, const uint32_t *__restrict__ core,
-    uint32_t kp, unsigned long long *out) {
+    uint32_t kp, unsigned long long *out, const uint32_t *__restrict__ cv) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   unsigned long long c = 0;
-  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       v < n; v += waves) {
-    if (core[v] < kp) continue;
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       i < n; i += waves) {
+    uint32_t v;
+    if (!core_vertex(cv, core, kp, i, &v)) continue;
     for (uint32_t j = off[v] + lane(); j < off[v + 1]; j += kWave)
       c += core[adj[j]] >= kp;
   }
@@ -36,14 +61,38 @@ __global__ __launch_bounds__(256) void k_core_edges(
   if (lane() == 0 && c) atomicAdd(out, c);
 }
 
-__global__ void k_count_kill_ge(uint64_t T, const uint32_t *__restrict__ kill,
-                                uint32_t kp, unsigned long long *out) {
-  unsigned long long c = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    c += kill[i] >= kp;
-  c = wave_sum64(c);
-  if (lane() == 0 && c) atomicAdd(out, c);
+// Counts the cliques with kill level >= kp and, when cc is given, lists
+// their ids there (the order is not fixed).  A wave takes 64 * kCoreSpan
+// consecutive cliques per pass and reserves its slots with one atomic.
+constexpr uint32_t kCoreSpan = 16;
+
+__global__ void k_core_cliques(uint64_t T, const uint32_t *__restrict__ kill,
+                               uint32_t kp, uint32_t *__restrict__ cc,
+                               unsigned long long *out) {
+  const uint64_t span = (uint64_t)kWave * kCoreSpan;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t b = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * span;
+       b < T; b += waves * span) {
+    uint64_t m[kCoreSpan];
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kCoreSpan; ++r) {
+      const uint64_t i = b + (uint64_t)r * kWave + lane();
+      m[r] = __ballot(i < T && kill[i] >= kp);
+      tot += (uint32_t)__popcll(m[r]);
+    }
+    if (tot == 0) continue;  // wave-uniform
+    unsigned long long base = 0;
+    if (lane() == 0) base = atomicAdd(out, (unsigned long long)tot);
+    base = __shfl(base, 0, kWave);
+    if (!cc) continue;
+#pragma unroll
+    for (uint32_t r = 0; r < kCoreSpan; ++r) {
+      if ((m[r] >> lane()) & 1ull)
+        cc[base + __popcll(m[r] & lanes_below())] = (uint32_t)(b + (uint64_t)r * kWave + lane());
+      base += __popcll(m[r]);
+    }
+  }
 }
 
 __global__ void k_uf_init(uint32_t n, uint32_t *__restrict__ p) {
@@ -89,11 +138,12 @@ __device__ void uf_unite(uint32_t *p, uint32_t a, uint32_t b) {
 __global__ __launch_bounds__(256) void k_uf_union_edges(
     uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
     const uint8_t *__restrict__ mark, const uint32_t *__restrict__ core,
-    uint32_t kp, uint32_t *__restrict__ p) {
+    uint32_t kp, uint32_t *__restrict__ p, const uint32_t *__restrict__ cv) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-       a < n; a += waves) {
-    if (core[a] < kp) continue;
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       i < n; i += waves) {
+    uint32_t a;
+    if (!core_vertex(cv, core, kp, i, &a)) continue;
     for (uint32_t j = off[a] + lane(); j < off[a + 1]; j += kWave) {
       const uint32_t b = adj[j];
       if (b > a && mark[j]) uf_unite(p, a, b);
@@ -112,10 +162,12 @@ __global__ void k_prune_mark(uint64_t T, uint32_t k,
                              const uint32_t *__restrict__ kill, uint32_t kp,
                              const uint32_t *__restrict__ off,
                              const uint32_t *__restrict__ adj,
-                             uint8_t *__restrict__ mark) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] < kp) continue;
+                             uint8_t *__restrict__ mark,
+                             const uint32_t *__restrict__ cc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c;
+    if (!core_clique(cc, kill, kp, i, &c)) continue;
     for (uint32_t i = 0; i < k; ++i) {
       const uint32_t a = cl[c * k + i];
       const uint32_t ab = off[a], ad = off[a + 1] - ab;
@@ -135,15 +187,19 @@ __global__ void k_prune_mark(uint64_t T, uint32_t k,
 // search through a hub's adjacency list.
 constexpr uint32_t kNoEdge = 0xFFFFFFFFu;
 
-__global__ void k_core_index(uint32_t n, const uint32_t *__restrict__ core,
-                             uint32_t kp, uint32_t *__restrict__ cidx,
-                             uint32_t *__restrict__ count) {
+// Numbers the core vertices: cidx[v] = slot (kNoEdge off the core) and
+// cv[slot] = v, for slots below cap (the host checks the count).
+__global__ void k_core_list(uint32_t n, const uint32_t *__restrict__ core,
+                            uint32_t kp, uint32_t *__restrict__ cidx,
+                            uint32_t *__restrict__ cv, uint32_t cap,
+                            uint32_t *__restrict__ count) {
   for (uint32_t base = blockIdx.x * blockDim.x; base < n;
        base += gridDim.x * blockDim.x) {
     const uint32_t v = base + threadIdx.x;
     const bool in = v < n && core[v] >= kp;
     const uint32_t slot = aggregated_reserve(count, 0, in);
     if (v < n) cidx[v] = in ? slot : kNoEdge;
+    if (in && slot < cap) cv[slot] = v;
   }
 }
 
@@ -153,15 +209,16 @@ __global__ void k_fill_edges(uint64_t n, uint32_t *__restrict__ a) {
     a[i] = kNoEdge;
 }
 
-__global__ void k_core_edge_matrix(uint32_t n, const uint32_t *__restrict__ off,
+// Row iu of the matrix from the adjacency of core vertex cv[iu].
+__global__ void k_core_edge_matrix(const uint32_t *__restrict__ cv,
+                                   const uint32_t *__restrict__ off,
                                    const uint32_t *__restrict__ adj,
                                    const uint32_t *__restrict__ cidx, uint32_t cn,
                                    uint32_t *__restrict__ cmat) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n;
-       u += waves) {
-    const uint32_t iu = cidx[u];
-    if (iu >= cn) continue;  // not a core vertex (wave-uniform)
+  for (uint32_t iu = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; iu < cn;
+       iu += waves) {
+    const uint32_t u = cv[iu];
     for (uint32_t p = off[u] + lane(); p < off[u + 1]; p += kWave) {
       const uint32_t iv = cidx[adj[p]];
       if (iv < cn) cmat[(uint64_t)iu * cn + iv] = p;
@@ -171,13 +228,13 @@ __global__ void k_core_edge_matrix(uint32_t n, const uint32_t *__restrict__ off,
 
 __global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
                                    const uint32_t *__restrict__ cl,
-                                   const uint32_t *__restrict__ kill, uint32_t kp,
+                                   const uint32_t *__restrict__ cc,
                                    const uint32_t *__restrict__ cidx, uint32_t cn,
                                    const uint32_t *__restrict__ cmat,
                                    uint8_t *__restrict__ mark) {
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] < kp) continue;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = cc[i];
     for (uint32_t i = 0; i < k; ++i) {
       const uint32_t ci = cidx[cl[c * k + i]];
       if (ci >= cn) continue;  // cannot happen for a core clique
@@ -216,7 +273,7 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_count2(
     uint64_t T, const uint32_t *__restrict__ cl, uint32_t k,
     const uint32_t *__restrict__ kill, uint32_t kp,
     const uint32_t *__restrict__ comp_of, uint32_t ncomp,
-    uint32_t *__restrict__ ccount) {
+    uint32_t *__restrict__ ccount, const uint32_t *__restrict__ cc) {
   __shared__ uint32_t cnt[kCompLds];
   const bool lds = ncomp <= kCompLds;
   if (lds) {
@@ -225,8 +282,8 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_count2(
   }
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
        base += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t c = base + threadIdx.x;
-    const bool in = c < T && kill[c] >= kp;
+    uint64_t c = 0;
+    const bool in = base + threadIdx.x < T && core_clique(cc, kill, kp, base + threadIdx.x, &c);
     const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
     (void)aggregated_reserve(lds ? cnt : ccount, comp, in);
   }
@@ -245,7 +302,7 @@ __device__ __forceinline__ void comp_place(uint64_t c, uint64_t f, uint32_t k,
                                            const uint32_t *__restrict__ fv_of,
                                            uint32_t *__restrict__ fmem,
                                            uint32_t *__restrict__ cslot) {
-  cslot[c] = (uint32_t)f;
+  if (cslot) cslot[c] = (uint32_t)f;
   for (uint32_t j = 0; j < k; ++j) fmem[f * ms + j] = fv_of[cl[c * k + j]];
 }
 
@@ -255,13 +312,13 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
     const uint32_t *__restrict__ comp_of, const uint32_t *__restrict__ fv_of,
     uint32_t ncomp, const uint64_t *__restrict__ coff,
     uint32_t *__restrict__ ccur, uint32_t *__restrict__ fmem,
-    uint32_t *__restrict__ cslot, uint32_t ms) {
+    uint32_t *__restrict__ cslot, uint32_t ms, const uint32_t *__restrict__ cc) {
   __shared__ uint32_t cnt[kCompLds], cbase[kCompLds];
   if (ncomp > kCompLds) {
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
          base += (uint64_t)gridDim.x * blockDim.x) {
-      const uint64_t c = base + threadIdx.x;
-      const bool in = c < T && kill[c] >= kp;
+      uint64_t c = 0;
+      const bool in = base + threadIdx.x < T && core_clique(cc, kill, kp, base + threadIdx.x, &c);
       const uint32_t comp = in ? comp_of[cl[c * k]] : 0;
       const uint32_t slot = aggregated_reserve(ccur, comp, in);
       if (in) comp_place(c, coff[comp] + slot, k, ms, cl, fv_of, fmem, cslot);
@@ -276,12 +333,14 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
   for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < T;
        t0 += (uint64_t)gridDim.x * tile) {
     uint32_t comp[kCompTile], local[kCompTile];
+    uint64_t cid[kCompTile];
     bool in[kCompTile];
 #pragma unroll
     for (int it = 0; it < kCompTile; ++it) {
-      const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
-      in[it] = c < T && kill[c] >= kp;
-      comp[it] = in[it] ? comp_of[cl[c * k]] : 0;
+      const uint64_t i = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
+      cid[it] = 0;
+      in[it] = i < T && core_clique(cc, kill, kp, i, &cid[it]);
+      comp[it] = in[it] ? comp_of[cl[cid[it] * k]] : 0;
       local[it] = aggregated_reserve(cnt, comp[it], in[it]);
     }
     __syncthreads();
@@ -296,8 +355,7 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
 #pragma unroll
     for (int it = 0; it < kCompTile; ++it) {
       if (!in[it]) continue;
-      const uint64_t c = t0 + (uint64_t)it * blockDim.x + threadIdx.x;
-      comp_place(c, coff[comp[it]] + cbase[comp[it]] + local[it], k, ms, cl, fv_of,
+      comp_place(cid[it], coff[comp[it]] + cbase[comp[it]] + local[it], k, ms, cl, fv_of,
                  fmem, cslot);
     }
     __syncthreads();  // cbase is rewritten by the next tile
@@ -399,23 +457,32 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_fill(
 
 }  // namespace
 
-hipError_t core_edge_count(uint32_t n, const uint32_t *off,
-                           const uint32_t *adj, const uint32_t *core,
-                           uint32_t kp, unsigned long long *out,
-                           hipStream_t s) {
-  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+hipError_t core_list(uint32_t n, const uint32_t *core, uint32_t kp, uint32_t *cidx,
+                     uint32_t *cv, uint32_t cap, uint32_t *count, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(count, 0, sizeof(uint32_t), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_core_edges, dim3(grid_for((size_t)n * kWave, 256, 16384)),
-                     dim3(256), 0, s, n, off, adj, core, kp, out);
+  hipLaunchKernelGGL(k_core_list, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, core,
+                     kp, cidx, cv, cap, count);
   return hipGetLastError();
 }
 
-hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
-                         unsigned long long *out, hipStream_t s) {
+hipError_t core_edge_count(uint32_t n, const uint32_t *off,
+                           const uint32_t *adj, const uint32_t *core,
+                           uint32_t kp, unsigned long long *out,
+                           hipStream_t s, const uint32_t *cv) {
   hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_count_kill_ge, dim3(grid_for(T, 256, 8192)), dim3(256),
-                     0, s, T, kill, kp, out);
+  hipLaunchKernelGGL(k_core_edges, dim3(grid_for((size_t)n * kWave, 256, 16384)),
+                     dim3(256), 0, s, n, off, adj, core, kp, out, cv);
+  return hipGetLastError();
+}
+
+hipError_t core_cliques(uint64_t T, const uint32_t *kill, uint32_t kp, uint32_t *cc,
+                        unsigned long long *out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_core_cliques, dim3(grid_for((T + kCoreSpan - 1) / kCoreSpan, 256, 8192)),
+                     dim3(256), 0, s, T, kill, kp, cc, out);
   return hipGetLastError();
 }
 
@@ -427,10 +494,11 @@ hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s) {
 
 hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
                           const uint8_t *mark, const uint32_t *core,
-                          uint32_t kp, uint32_t *parent, hipStream_t s) {
+                          uint32_t kp, uint32_t *parent, hipStream_t s,
+                          const uint32_t *cv) {
   hipLaunchKernelGGL(k_uf_union_edges,
                      dim3(grid_for((size_t)n * kWave, 256, 16384)), dim3(256),
-                     0, s, n, off, adj, mark, core, kp, parent);
+                     0, s, n, off, adj, mark, core, kp, parent, cv);
   return hipGetLastError();
 }
 
@@ -440,31 +508,59 @@ hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t prune_mark_dense(uint64_t T, uint32_t k, const uint32_t *cliques,
-                            const uint32_t *kill, uint32_t kp, uint32_t n,
-                            const uint32_t *core, const uint32_t *off,
-                            const uint32_t *adj, uint32_t cn, uint32_t *cidx,
-                            uint32_t *count, uint32_t *cmat, uint8_t *mark,
-                            hipStream_t s) {
-  hipError_t e = hipMemsetAsync(count, 0, 4, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_core_index, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n,
-                     core, kp, cidx, count);
+hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *cc, const uint32_t *cv, uint32_t cn,
+                            const uint32_t *cidx, const uint32_t *off, const uint32_t *adj,
+                            uint32_t *cmat, uint8_t *mark, hipStream_t s) {
   const uint64_t cells = (uint64_t)cn * cn;
   hipLaunchKernelGGL(k_fill_edges, dim3(grid_for(cells, 256, 16384)), dim3(256), 0, s,
                      cells, cmat);
-  hipLaunchKernelGGL(k_core_edge_matrix, dim3(grid_for((size_t)n * kWave, 256, 8192)),
-                     dim3(256), 0, s, n, off, adj, cidx, cn, cmat);
-  hipLaunchKernelGGL(k_prune_mark_dense, dim3(grid_for(T, 256, 65535)), dim3(256), 0, s,
-                     T, k, cliques, kill, kp, cidx, cn, cmat, mark);
+  hipLaunchKernelGGL(k_core_edge_matrix, dim3(grid_for((size_t)cn * kWave, 256, 8192)),
+                     dim3(256), 0, s, cv, off, adj, cidx, cn, cmat);
+  if (Tc)
+    hipLaunchKernelGGL(k_prune_mark_dense, dim3(grid_for(Tc, 256, 65535)), dim3(256), 0, s,
+                       Tc, k, cliques, cc, cidx, cn, cmat, mark);
   return hipGetLastError();
 }
 
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *kill, uint32_t kp, const uint32_t *off,
-                      const uint32_t *adj, uint8_t *mark, hipStream_t s) {
+                      const uint32_t *adj, uint8_t *mark, hipStream_t s,
+                      const uint32_t *cc) {
+  if (T == 0) return hipSuccess;
   hipLaunchKernelGGL(k_prune_mark, dim3(grid_for(T, 256, 65535)), dim3(256), 0,
-                     s, T, k, cliques, kill, kp, off, adj, mark);
+                     s, T, k, cliques, kill, kp, off, adj, mark, cc);
+  return hipGetLastError();
+}
+
+__global__ void k_gather_u32(uint32_t m, const uint32_t *__restrict__ idx,
+                             const uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+__global__ void k_scatter_u32x2(uint32_t m, const uint32_t *__restrict__ idx,
+                                const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                uint32_t *__restrict__ da, uint32_t *__restrict__ db) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    da[idx[i]] = a[i];
+    db[idx[i]] = b[i];
+  }
+}
+
+hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,
+                      hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, m, idx, src,
+                     dst);
+  return hipGetLastError();
+}
+
+hipError_t scatter_u32x2(uint32_t m, const uint32_t *idx, const uint32_t *a, const uint32_t *b,
+                         uint32_t *da, uint32_t *db, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_u32x2, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, m, idx,
+                     a, b, da, db);
   return hipGetLastError();
 }
 
@@ -482,24 +578,26 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *comp_of, const uint32_t *fv_of,
                               uint32_t ncomp, const uint64_t *coff,
                               uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
-                              hipStream_t s, uint32_t ms) {
+                              hipStream_t s, uint32_t ms, const uint32_t *cc) {
   if (ms == 0) ms = k;
+  if (T == 0) return hipSuccess;
   const unsigned grid = ncomp > kCompLds
                             ? grid_for(T, kCompThreads, 65535)
                             : grid_for(T, kCompThreads * kCompTile, 4096);
   hipLaunchKernelGGL(k_comp_scatter2, dim3(grid), dim3(kCompThreads), 0, s, T, k,
                      cliques, kill, kp, comp_of, fv_of, ncomp, coff, ccur, fmem,
-                     cslot, ms);
+                     cslot, ms, cc);
   return hipGetLastError();
 }
 
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, uint32_t ncomp,
-                              uint32_t *ccount, hipStream_t s) {
+                              uint32_t *ccount, hipStream_t s, const uint32_t *cc) {
+  if (T == 0) return hipSuccess;
   const unsigned grid = grid_for(T, kCompThreads, ncomp > kCompLds ? 65535 : 4096);
   hipLaunchKernelGGL(k_comp_count2, dim3(grid), dim3(kCompThreads), 0, s, T,
-                     cliques, k, kill, kp, comp_of, ncomp, ccount);
+                     cliques, k, kill, kp, comp_of, ncomp, ccount, cc);
   return hipGetLastError();
 }
 

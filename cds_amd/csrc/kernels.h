@@ -323,47 +323,69 @@ constexpr uint32_t kReplayGridMaxVertices = 16384;
 // entries_per_step incidence entries each (CDS_REPLAY_BLOCKS overrides).
 unsigned replay_grid_blocks(int device, uint64_t entries_per_step, unsigned force = 0);
 
+// The densest core's vertices (core >= kp): cidx[v] = their numbering
+// (0xFFFFFFFF off the core), cv[slot] = v for the first cap slots, *count =
+// how many (the order of the slots is not fixed).
+hipError_t core_list(uint32_t n, const uint32_t *core, uint32_t kp, uint32_t *cidx,
+                     uint32_t *cv, uint32_t cap, uint32_t *count, hipStream_t s);
+// Directed edges among core vertices: over every vertex (core >= kp), or
+// over the n vertices listed in cv when it is given.
 hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,
                            uint32_t kp, unsigned long long *out,
-                           hipStream_t s);
-hipError_t count_kill_ge(uint64_t T, const uint32_t *kill, uint32_t kp,
-                         unsigned long long *out, hipStream_t s);
+                           hipStream_t s, const uint32_t *cv = nullptr);
+// *out = cliques with kill level >= kp (the core's cliques); with cc their
+// ids are also listed there (order not fixed).
+hipError_t core_cliques(uint64_t T, const uint32_t *kill, uint32_t kp, uint32_t *cc,
+                        unsigned long long *out, hipStream_t s);
+// dst[i] = src[idx[i]] and da[idx[i]] = a[i], db[idx[i]] = b[i], i < m.
+hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,
+                      hipStream_t s);
+hipError_t scatter_u32x2(uint32_t m, const uint32_t *idx, const uint32_t *a, const uint32_t *b,
+                         uint32_t *da, uint32_t *db, hipStream_t s);
 hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s);
-// Unions the endpoints of the marked (clique-covered) edges of core vertices.
+// Unions the endpoints of the marked (clique-covered) edges of core vertices
+// (every vertex with core >= kp, or the n listed in cv).
 hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
                           const uint8_t *mark, const uint32_t *core,
-                          uint32_t kp, uint32_t *parent, hipStream_t s);
+                          uint32_t kp, uint32_t *parent, hipStream_t s,
+                          const uint32_t *cv = nullptr);
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s);
-// Same marks for a densest core of cn <= kDenseCoreMax vertices (those
-// with core >= kp) through a cn x cn matrix of edge positions; cidx (n),
-// count (1) and cmat (cn * cn) are scratch.
+// Same marks for a densest core of cn <= kDenseCoreMax vertices (cv, cidx
+// from core_list) and its Tc cliques listed in cc, through a cn x cn matrix
+// of edge positions (cmat, cn * cn words of scratch).
 constexpr uint32_t kDenseCoreMax = 4096;
-hipError_t prune_mark_dense(uint64_t T, uint32_t k, const uint32_t *cliques,
-                            const uint32_t *kill, uint32_t kp, uint32_t n,
-                            const uint32_t *core, const uint32_t *off,
-                            const uint32_t *adj, uint32_t cn, uint32_t *cidx,
-                            uint32_t *count, uint32_t *cmat, uint8_t *mark,
-                            hipStream_t s);
+hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
+                            const uint32_t *cc, const uint32_t *cv, uint32_t cn,
+                            const uint32_t *cidx, const uint32_t *off, const uint32_t *adj,
+                            uint32_t *cmat, uint8_t *mark, hipStream_t s);
+// Marks every edge of a core clique: the cliques with kill >= kp among T,
+// or the T listed in cc when it is given.
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *kill, uint32_t kp, const uint32_t *off,
-                      const uint32_t *adj, uint8_t *mark, hipStream_t s);
+                      const uint32_t *adj, uint8_t *mark, hipStream_t s,
+                      const uint32_t *cc = nullptr);
 hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
                             unsigned long long *out, hipStream_t s);
 // Per-component clique buckets of ncomp components (reservations summed
 // per workgroup in LDS when ncomp is small, else wave-aggregated):
 //  comp_clique_totals  ccount[comp] += 1 per core clique
-//  comp_cliques_build  fmem / cslot (flow index of each core clique)
+//  comp_cliques_build  fmem, and cslot[c] = the flow index of core clique c
+//                      when cslot is given
+// The core cliques are those with kill >= kp among T, or the T listed in cc
+// when it is given.
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, uint32_t ncomp,
-                              uint32_t *ccount, hipStream_t s);
+                              uint32_t *ccount, hipStream_t s,
+                              const uint32_t *cc = nullptr);
 hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, const uint32_t *fv_of,
                               uint32_t ncomp, const uint64_t *coff,
                               uint32_t *ccur, uint32_t *fmem, uint32_t *cslot,
-                              hipStream_t s, uint32_t ms = 0);
+                              hipStream_t s, uint32_t ms = 0,
+                              const uint32_t *cc = nullptr);
 // Flow-network incidence from the core's clique bucket fmem (Tc cliques,
 // k flow-vertex ids each, nv flow vertices): fcd[v] = number of pairs of
 // vertex v (zeroed here); then, with fioff the exclusive prefix sum of fcd,
