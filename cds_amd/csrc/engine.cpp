@@ -1396,6 +1396,16 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
 
   // ---- (b) peel ----
   DBuf<uint32_t> core(&A), kill(&A);
+  // kill words are derived from the core numbers only when a stage reads
+  // them (the persistent peel claims cliques by stamps, not kill words)
+  bool kill_valid = true;
+  auto ensure_kill = [&]() -> cds_status {
+    if (!kill_valid) {
+      HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
+      kill_valid = true;
+    }
+    return CDS_OK;
+  };
   std::vector<Level> levels;
   PeelSummary PS;
   uint32_t peel_rounds = 0;
@@ -1467,7 +1477,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       note_width(R, &R->peel_grid, grid, launched);
       CHK(readback(e, hc, ctl.p, 1));
       if (hc->bar.abort) return barrier_failed("persistent peel", launched);
-      HIPCHK(peel_derive_kill(T, k, L.cliques.p, core.p, kill.p, st));
+      kill_valid = false;
       R->t_peel_kernel_ms += tm.ms(22, 23);
       R->peel_launches++;
       if (hc->rounds >= (1ull << 30) || sbase + hc->rounds + 1 >= (1u << 30))
@@ -1560,7 +1570,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         HIPCHK(ralive.alloc(n));
         HIPCHK(rcount.alloc(1));
       }
-      HIPCHK(refine_prepare(n, T, lv.level, core.p, deg.p, sdeg.p, kill.p, ralive.p,
+      HIPCHK(refine_prepare(n, T, lv.level, core.p, deg.p, sdeg.p, nullptr, ralive.p,
                             rcount.p, st));
       uint32_t na = 0;
       CHK(readback(e, &na, rcount.p, 1));
@@ -1697,6 +1707,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         if (rgrid) gate.emplace(e, rgrid);
         HIPCHK(hipEventRecord(tm.ev[22], st));
         const unsigned rasked = rgrid;
+        CHK(ensure_kill());
         HIPCHK(replay_phase(k, lv.level, n, core.p, kill.p, ioff.p, inc.p,
                             T - PS.kl_before[ii], n - PS.rm_before[ii], (uint32_t)M,
                             pv.p, pdeg.p, posmap.p, palive.p, rdead.p, pc.p,
@@ -1787,7 +1798,9 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   R->core_edges = ce / 2;
   const bool list_cc = !core_only && core_T < 0xFFFFFFFFull;
   if (list_cc) HIPCHK(cc.alloc(core_T));
-  HIPCHK(core_cliques(T, kill.p, kpu, list_cc ? cc.p : nullptr, acc.p, st));
+  else CHK(ensure_kill());  // the stages below read kill words
+  HIPCHK(core_cliques(T, k, L.cliques.p, core.p, kill_valid ? kill.p : nullptr, kpu,
+                      list_cc ? cc.p : nullptr, acc.p, st));
   unsigned long long ct = 0;
   CHK(readback(e, &ct, acc.p, 1));
   if (ct != core_T)

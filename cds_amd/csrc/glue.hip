@@ -61,12 +61,24 @@ __global__ __launch_bounds__(256) void k_core_edges(
   if (lane() == 0 && c) atomicAdd(out, c);
 }
 
-// Counts the cliques with kill level >= kp and, when cc is given, lists
-// their ids there (the order is not fixed).  A wave takes 64 * kCoreSpan
+// Counts the cliques with kill level >= kp (with kill null: whose members
+// all have core >= kp, the same cliques) and, when cc is given, lists their
+// ids there (the order is not fixed).  A wave takes 64 * kCoreSpan
 // consecutive cliques per pass and reserves its slots with one atomic.
 constexpr uint32_t kCoreSpan = 16;
 
-__global__ void k_core_cliques(uint64_t T, const uint32_t *__restrict__ kill,
+__device__ __forceinline__ bool in_core(uint64_t i, uint32_t k, const uint32_t *cl,
+                                        const uint32_t *core, const uint32_t *kill,
+                                        uint32_t kp) {
+  if (kill) return kill[i] >= kp;
+  bool all = true;
+  for (uint32_t j = 0; j < k; ++j) all &= core[cl[i * k + j]] >= kp;
+  return all;
+}
+
+__global__ void k_core_cliques(uint64_t T, uint32_t k, const uint32_t *__restrict__ cl,
+                               const uint32_t *__restrict__ core,
+                               const uint32_t *__restrict__ kill,
                                uint32_t kp, uint32_t *__restrict__ cc,
                                unsigned long long *out) {
   const uint64_t span = (uint64_t)kWave * kCoreSpan;
@@ -78,7 +90,7 @@ __global__ void k_core_cliques(uint64_t T, const uint32_t *__restrict__ kill,
 #pragma unroll
     for (uint32_t r = 0; r < kCoreSpan; ++r) {
       const uint64_t i = b + (uint64_t)r * kWave + lane();
-      m[r] = __ballot(i < T && kill[i] >= kp);
+      m[r] = __ballot(i < T && in_core(i, k, cl, core, kill, kp));
       tot += (uint32_t)__popcll(m[r]);
     }
     if (tot == 0) continue;  // wave-uniform
@@ -477,12 +489,13 @@ hipError_t core_edge_count(uint32_t n, const uint32_t *off,
   return hipGetLastError();
 }
 
-hipError_t core_cliques(uint64_t T, const uint32_t *kill, uint32_t kp, uint32_t *cc,
+hipError_t core_cliques(uint64_t T, uint32_t k, const uint32_t *cliques, const uint32_t *core,
+                        const uint32_t *kill, uint32_t kp, uint32_t *cc,
                         unsigned long long *out, hipStream_t s) {
   hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_core_cliques, dim3(grid_for((T + kCoreSpan - 1) / kCoreSpan, 256, 8192)),
-                     dim3(256), 0, s, T, kill, kp, cc, out);
+                     dim3(256), 0, s, T, k, cliques, core, kill, kp, cc, out);
   return hipGetLastError();
 }
 

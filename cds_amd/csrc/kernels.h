@@ -205,9 +205,9 @@ struct PeelCtl {
 // alive0 (na0 vertices, or null for all n): the vertices alive at the
 // start; every other vertex must have core != kAlive and every clique
 // containing one must be dead or never reached.
-// kill[] (T words) is not touched: after the launch, peel_derive_kill sets
-// every clique still marked alive (kAlive) to the level that killed it, the
-// smallest core value among its members.  stamp[] (n words) holds each
+// kill[] (T words) is not touched: once no launch is left (and no vertex
+// parked), peel_derive_kill sets every clique's word to the level that
+// killed it, the smallest core value among its members.  stamp[] (n words) holds each
 // vertex's sub-round stamp; sbase is the launch's first sub-round: 1 for the
 // first launch on a graph, above every stamp of earlier launches (their
 // sub-round counts + 1) for a re-peel.
@@ -228,7 +228,8 @@ hipError_t peel_persistent(uint32_t n, uint32_t k, uint64_t T, const uint64_t *i
 // carry the value `level`: those vertices become alive (core = kAlive,
 // degree = sdeg, listed in alive[], count in *count), vertices of later
 // steps (core > level) are parked (core = kParked, their core value kept
-// in deg) and the step's cliques become alive again.
+// in deg) and, when kill is given, the step's cliques become alive again
+// (kill null: the kill words are derived afresh after the last launch).
 constexpr uint32_t kParked = 0xFFFFFFFEu;
 hipError_t refine_prepare(uint32_t n, uint64_t T, uint32_t level,
                           uint32_t *core, uint32_t *deg, const uint32_t *sdeg,
@@ -334,9 +335,11 @@ hipError_t core_edge_count(uint32_t n, const uint32_t *off,
                            const uint32_t *adj, const uint32_t *core,
                            uint32_t kp, unsigned long long *out,
                            hipStream_t s, const uint32_t *cv = nullptr);
-// *out = cliques with kill level >= kp (the core's cliques); with cc their
-// ids are also listed there (order not fixed).
-hipError_t core_cliques(uint64_t T, const uint32_t *kill, uint32_t kp, uint32_t *cc,
+// *out = cliques with kill level >= kp (the core's cliques; with kill
+// null, those whose k members all have core >= kp); with cc their ids are
+// also listed there (order not fixed).
+hipError_t core_cliques(uint64_t T, uint32_t k, const uint32_t *cliques, const uint32_t *core,
+                        const uint32_t *kill, uint32_t kp, uint32_t *cc,
                         unsigned long long *out, hipStream_t s);
 // dst[i] = src[idx[i]] and da[idx[i]] = a[i], db[idx[i]] = b[i], i < m.
 hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,

@@ -668,17 +668,15 @@ __global__ void k_stamp_reset(uint32_t n, const uint32_t *__restrict__ core,
   }
 }
 
-// After a persistent peel: every clique still marked alive in kill[] was
-// claimed during the launch, at the level its first member was removed,
-// which is the smallest core value among its members (parked vertices of
-// later steps carry kParked, above every level).
+// After the persistent peel launches: every clique was claimed at the level
+// its first member was removed, which is the smallest core value among its
+// members.  Every kill word is written (no parked vertex may remain).
 __global__ void k_kill_from_core(uint64_t T, uint32_t k,
                                  const uint32_t *__restrict__ cl,
                                  const uint32_t *__restrict__ core,
                                  uint32_t *__restrict__ kill) {
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
        c += (uint64_t)gridDim.x * blockDim.x) {
-    if (kill[c] != kAlive) continue;
     uint32_t m = kAlive;
     for (uint32_t j = 0; j < k; ++j) m = min(m, core[cl[c * k + j]]);
     kill[c] = m;
@@ -1354,8 +1352,9 @@ hipError_t refine_prepare(uint32_t n, uint64_t T, uint32_t level,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_refine_vertices, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s,
                      n, level, core, deg, sdeg, alive, count);
-  hipLaunchKernelGGL(k_refine_cliques, dim3(grid_for(T, 256, 16384)), dim3(256), 0, s,
-                     T, level, kill);
+  if (kill)
+    hipLaunchKernelGGL(k_refine_cliques, dim3(grid_for(T, 256, 16384)), dim3(256), 0, s,
+                       T, level, kill);
   return hipGetLastError();
 }
 
