@@ -1797,10 +1797,14 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   CHK(readback(e, &ce, acc.p, 1));
   R->core_edges = ce / 2;
   const bool list_cc = !core_only && core_T < 0xFFFFFFFFull;
-  if (list_cc) HIPCHK(cc.alloc(core_T));
-  else CHK(ensure_kill());  // the stages below read kill words
-  HIPCHK(core_cliques(T, k, L.cliques.p, core.p, kill_valid ? kill.p : nullptr, kpu,
-                      list_cc ? cc.p : nullptr, acc.p, st));
+  if (list_cc) {
+    HIPCHK(cc.alloc(core_T));
+    HIPCHK(core_cliques_ranged(core_n, cv.p, k, L.doff.p, L.item_off.p, L.item_base.p,
+                               L.cliques.p, core.p, kpu, cc.p, acc.p, A));
+  } else {
+    CHK(ensure_kill());  // the stages below read kill words
+    HIPCHK(core_cliques(T, k, L.cliques.p, core.p, kill.p, kpu, nullptr, acc.p, st));
+  }
   unsigned long long ct = 0;
   CHK(readback(e, &ct, acc.p, 1));
   if (ct != core_T)
@@ -1841,7 +1845,11 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const size_t na = g->host_off[n];
     HIPCHK(mark.alloc(na));
     HIPCHK(mark.zero());
-    if (core_n <= kDenseCoreMax) {
+    if (core_n <= kDenseCoreMax && k <= 3) {
+      DBuf<unsigned long long> cbits(&A);
+      HIPCHK(cbits.alloc((size_t)core_n * ((core_n + 63) / 64)));
+      HIPCHK(prune_mark_bits(k, cv.p, core_n, cidx.p, g->off, g->adj, cbits.p, mark.p, st));
+    } else if (core_n <= kDenseCoreMax) {
       DBuf<uint32_t> cmat(&A);
       HIPCHK(cmat.alloc((size_t)core_n * core_n));
       HIPCHK(prune_mark_dense(core_T, k, L.cliques.p, cc.p, cv.p, core_n, cidx.p, g->off,

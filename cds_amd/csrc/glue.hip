@@ -107,6 +107,69 @@ __global__ void k_core_cliques(uint64_t T, uint32_t k, const uint32_t *__restric
   }
 }
 
+// The cliques whose first listed member (the lowest in the degree order)
+// is core vertex cv[i]: one contiguous range of the clique array, from the
+// DAG offsets (k = 2) or the scanned listing items (k >= 3).  rchunks[i]:
+// the range in spans of 64 * kCoreSpan cliques.
+__global__ void k_core_ranges(uint32_t cn, const uint32_t *__restrict__ cv, uint32_t k,
+                              const uint32_t *__restrict__ doff,
+                              const uint64_t *__restrict__ item_off,
+                              const uint64_t *__restrict__ item_base,
+                              uint64_t *__restrict__ rstart, uint64_t *__restrict__ rend,
+                              uint32_t *__restrict__ rchunks) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cn;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t u = cv[i];
+    uint64_t b, e;
+    if (k == 2) {
+      b = doff[u];
+      e = doff[u + 1];
+    } else {
+      b = item_base[item_off[u]];
+      e = item_base[item_off[u + 1]];
+    }
+    rstart[i] = b;
+    rend[i] = e;
+    rchunks[i] = (uint32_t)((e - b + kWave * kCoreSpan - 1) / (kWave * kCoreSpan));
+  }
+}
+
+// Lists the core's cliques from the core vertices' ranges only: a core
+// clique's first member is a core vertex.  One span per wave and pass.
+__global__ void k_core_cliques_ranged(uint32_t cn, const uint64_t *__restrict__ choff,
+                                      const uint64_t *__restrict__ rstart,
+                                      const uint64_t *__restrict__ rend, uint32_t k,
+                                      const uint32_t *__restrict__ cl,
+                                      const uint32_t *__restrict__ core, uint32_t kp,
+                                      uint32_t *__restrict__ cc, unsigned long long *out) {
+  const uint64_t total = choff[cn];
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t ch = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       ch < total; ch += waves) {
+    const uint32_t i = wave_last_le(choff, cn, ch);
+    const uint64_t b = rstart[i] + (ch - choff[i]) * (kWave * kCoreSpan);
+    const uint64_t e = min(rend[i], b + (uint64_t)kWave * kCoreSpan);
+    uint64_t m[kCoreSpan];
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kCoreSpan; ++r) {
+      const uint64_t c = b + (uint64_t)r * kWave + lane();
+      m[r] = __ballot(c < e && in_core(c, k, cl, core, nullptr, kp));
+      tot += (uint32_t)__popcll(m[r]);
+    }
+    if (tot == 0) continue;  // wave-uniform
+    unsigned long long base = 0;
+    if (lane() == 0) base = atomicAdd(out, (unsigned long long)tot);
+    base = __shfl(base, 0, kWave);
+#pragma unroll
+    for (uint32_t r = 0; r < kCoreSpan; ++r) {
+      if ((m[r] >> lane()) & 1ull)
+        cc[base + __popcll(m[r] & lanes_below())] = (uint32_t)(b + (uint64_t)r * kWave + lane());
+      base += __popcll(m[r]);
+    }
+  }
+}
+
 __global__ void k_uf_init(uint32_t n, uint32_t *__restrict__ p) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x)
@@ -259,6 +322,47 @@ __global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
         // threads racing on the same mark all store 1)
         if (pos != kNoEdge && !mark[pos]) mark[pos] = 1;
       }
+    }
+  }
+}
+
+// k <= 3 and a small core: the core's adjacency as bitset rows (row i:
+// the core neighbours of cv[i], by their cidx numbers, W words), then a
+// directed core edge (a, b) lies in a core clique iff k = 2, or (k = 3) a
+// and b have a common core neighbour: rows a and b intersect.
+__global__ __launch_bounds__(256) void k_core_rows(const uint32_t *__restrict__ cv, uint32_t cn,
+                                                   uint32_t W, const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ cidx,
+                                                   unsigned long long *__restrict__ bits) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t ia = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; ia < cn;
+       ia += waves) {
+    const uint32_t a = cv[ia];
+    for (uint32_t p = off[a] + lane(); p < off[a + 1]; p += kWave) {
+      const uint32_t ib = cidx[adj[p]];
+      if (ib < cn) atomicOr(&bits[(size_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_prune_mark_bits(
+    uint32_t k, const uint32_t *__restrict__ cv, uint32_t cn, uint32_t W,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+    const uint32_t *__restrict__ cidx, const unsigned long long *__restrict__ bits,
+    uint8_t *__restrict__ mark) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t ia = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; ia < cn;
+       ia += waves) {
+    const uint32_t a = cv[ia];
+    const unsigned long long *ra = bits + (size_t)ia * W;
+    for (uint32_t p = off[a] + lane(); p < off[a + 1]; p += kWave) {
+      const uint32_t ib = cidx[adj[p]];
+      if (ib >= cn) continue;
+      bool in = k == 2;
+      const unsigned long long *rb = bits + (size_t)ib * W;
+      for (uint32_t w = 0; w < W && !in; ++w) in = (ra[w] & rb[w]) != 0;
+      if (in) mark[p] = 1;
     }
   }
 }
@@ -505,6 +609,36 @@ hipError_t uf_init(uint32_t n, uint32_t *parent, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t core_cliques_ranged(uint32_t cn, const uint32_t *cv, uint32_t k,
+                               const uint32_t *doff, const uint64_t *item_off,
+                               const uint64_t *item_base, const uint32_t *cliques,
+                               const uint32_t *core, uint32_t kp, uint32_t *cc,
+                               unsigned long long *out, DeviceArena &a) {
+  hipStream_t s = a.stream();
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(*out), s);
+  if (e != hipSuccess || cn == 0) return e;
+  void *rs = nullptr, *re = nullptr, *rc = nullptr, *co = nullptr;
+  auto done = [&](hipError_t r) {
+    for (void *p : {rs, re, rc, co})
+      if (p) a.release(p);
+    return r;
+  };
+  if ((e = a.alloc(&rs, (size_t)cn * 8)) != hipSuccess) return done(e);
+  if ((e = a.alloc(&re, (size_t)cn * 8)) != hipSuccess) return done(e);
+  if ((e = a.alloc(&rc, (size_t)cn * 4)) != hipSuccess) return done(e);
+  if ((e = a.alloc(&co, ((size_t)cn + 1) * 8)) != hipSuccess) return done(e);
+  auto *rstart = static_cast<uint64_t *>(rs), *rend = static_cast<uint64_t *>(re);
+  auto *rchunks = static_cast<uint32_t *>(rc);
+  auto *choff = static_cast<uint64_t *>(co);
+  hipLaunchKernelGGL(k_core_ranges, dim3(grid_for(cn, 256, 1024)), dim3(256), 0, s, cn, cv, k,
+                     doff, item_off, item_base, rstart, rend, rchunks);
+  if ((e = hipGetLastError()) != hipSuccess) return done(e);
+  if ((e = scan_u32(rchunks, choff, cn, a)) != hipSuccess) return done(e);
+  hipLaunchKernelGGL(k_core_cliques_ranged, dim3(4096), dim3(256), 0, s, cn, choff, rstart,
+                     rend, k, cliques, core, kp, cc, out);
+  return done(hipGetLastError());
+}
+
 hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
                           const uint8_t *mark, const uint32_t *core,
                           uint32_t kp, uint32_t *parent, hipStream_t s,
@@ -533,6 +667,20 @@ hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
   if (Tc)
     hipLaunchKernelGGL(k_prune_mark_dense, dim3(grid_for(Tc, 256, 65535)), dim3(256), 0, s,
                        Tc, k, cliques, cc, cidx, cn, cmat, mark);
+  return hipGetLastError();
+}
+
+hipError_t prune_mark_bits(uint32_t k, const uint32_t *cv, uint32_t cn, const uint32_t *cidx,
+                           const uint32_t *off, const uint32_t *adj,
+                           unsigned long long *bits, uint8_t *mark, hipStream_t s) {
+  if (k < 2 || k > 3) return hipErrorInvalidValue;
+  const uint32_t W = (cn + 63) / 64;
+  hipError_t e = hipMemsetAsync(bits, 0, (size_t)cn * W * 8, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_core_rows, dim3(grid_for((size_t)cn * kWave, 256, 8192)), dim3(256), 0,
+                     s, cv, cn, W, off, adj, cidx, bits);
+  hipLaunchKernelGGL(k_prune_mark_bits, dim3(grid_for((size_t)cn * kWave, 256, 8192)),
+                     dim3(256), 0, s, k, cv, cn, W, off, adj, cidx, bits, mark);
   return hipGetLastError();
 }
 

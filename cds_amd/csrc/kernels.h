@@ -341,6 +341,15 @@ hipError_t core_edge_count(uint32_t n, const uint32_t *off,
 hipError_t core_cliques(uint64_t T, uint32_t k, const uint32_t *cliques, const uint32_t *core,
                         const uint32_t *kill, uint32_t kp, uint32_t *cc,
                         unsigned long long *out, hipStream_t s);
+// The same list (cc, *out) from the cn core vertices cv only: a core
+// clique's first listed member is a core vertex, and each vertex's cliques
+// are one range of the clique array (k = 2: DAG arc order, doff; k >= 3:
+// listing item order, item_off / item_base).
+hipError_t core_cliques_ranged(uint32_t cn, const uint32_t *cv, uint32_t k,
+                               const uint32_t *doff, const uint64_t *item_off,
+                               const uint64_t *item_base, const uint32_t *cliques,
+                               const uint32_t *core, uint32_t kp, uint32_t *cc,
+                               unsigned long long *out, DeviceArena &a);
 // dst[i] = src[idx[i]] and da[idx[i]] = a[i], db[idx[i]] = b[i], i < m.
 hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,
                       hipStream_t s);
@@ -362,6 +371,11 @@ hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
                             const uint32_t *cc, const uint32_t *cv, uint32_t cn,
                             const uint32_t *cidx, const uint32_t *off, const uint32_t *adj,
                             uint32_t *cmat, uint8_t *mark, hipStream_t s);
+// The same marks for k = 2 or 3 from the core's adjacency alone: bits is
+// cn * ceil(cn / 64) words of scratch for the core's bitset rows.
+hipError_t prune_mark_bits(uint32_t k, const uint32_t *cv, uint32_t cn, const uint32_t *cidx,
+                           const uint32_t *off, const uint32_t *adj,
+                           unsigned long long *bits, uint8_t *mark, hipStream_t s);
 // Marks every edge of a core clique: the cliques with kill >= kp among T,
 // or the T listed in cc when it is given.
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
