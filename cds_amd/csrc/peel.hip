@@ -48,10 +48,22 @@ constexpr uint32_t kIncPairs = 8192;                     // pairs per tile: 8192
 constexpr uint32_t kIncHash = 2 * kIncPairs;             // load <= 1/2
 constexpr uint32_t kIncPerThread = kIncPairs / kIncThreads;
 
+// An entry of k words stored with one instruction (k = 3: 12 bytes at
+// 4-byte alignment), so a scattered entry costs one memory request.
+struct __attribute__((packed, aligned(4))) Entry3 {
+  uint32_t w[3];
+};
+struct __attribute__((packed, aligned(4))) Entry4 {
+  uint32_t w[4];
+};
+
+// With small (T * k < 2^32) the reservation pass turns each vertex's slot
+// count into the absolute position of its run, so the write pass reads no
+// ioff word per pair.
 __global__ __launch_bounds__(kIncThreads) void k_build_inc(
     uint64_t T, uint32_t k, const uint32_t *__restrict__ cl,
     const uint64_t *__restrict__ ioff, uint32_t *__restrict__ icur,
-    uint32_t *__restrict__ inc) {
+    uint32_t *__restrict__ inc, uint32_t small) {
   __shared__ uint32_t hkey[kIncHash];
   __shared__ uint32_t hcnt[kIncHash];
   // occupied slots of the tile, so the reservation pass and the reset visit
@@ -92,8 +104,9 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
     __syncthreads();
     const uint32_t nu = nused;
     for (uint32_t i = threadIdx.x; i < nu; i += kIncThreads) {
-      const uint32_t h = used[i];
-      hcnt[h] = atomicAdd(&icur[hkey[h]], hcnt[h]);  // -> base
+      const uint32_t h = used[i], x = hkey[h];
+      const uint32_t r0 = atomicAdd(&icur[x], hcnt[h]);  // -> base
+      hcnt[h] = small ? (uint32_t)(ioff[x] + r0) : r0;
     }
     __syncthreads();
 #pragma unroll
@@ -102,11 +115,23 @@ __global__ __launch_bounds__(kIncThreads) void k_build_inc(
       if (p >= pairs) break;
       const uint32_t q = p / k, j = p - q * k;
       const uint64_t c = c0 + q;
-      const uint32_t x = hkey[slot[r]];
-      uint32_t *e = inc + (ioff[x] + hcnt[slot[r]] + rank[r]) * k;
-      e[0] = (uint32_t)c;
-      for (uint32_t t = 0, o = 1; t < k; ++t)
-        if (t != j) e[o++] = cl[c * k + t];
+      const uint64_t at = small ? (uint64_t)hcnt[slot[r]] + rank[r]
+                                : ioff[hkey[slot[r]]] + hcnt[slot[r]] + rank[r];
+      uint32_t *e = inc + at * k;
+      if (k == 3) {
+        const uint32_t a = cl[c * 3 + (j == 0 ? 1 : 0)], b = cl[c * 3 + (j == 2 ? 1 : 2)];
+        *reinterpret_cast<Entry3 *>(e) = Entry3{{(uint32_t)c, a, b}};
+      } else if (k == 4) {
+        Entry4 v;
+        v.w[0] = (uint32_t)c;
+        for (uint32_t t = 0, o = 1; t < 4; ++t)
+          if (t != j) v.w[o++] = cl[c * 4 + t];
+        *reinterpret_cast<Entry4 *>(e) = v;
+      } else {
+        e[0] = (uint32_t)c;
+        for (uint32_t t = 0, o = 1; t < k; ++t)
+          if (t != j) e[o++] = cl[c * k + t];
+      }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nu; i += kIncThreads) {
@@ -1296,8 +1321,9 @@ hipError_t build_incidence(uint64_t T, uint32_t k, const uint32_t *cliques,
                            uint32_t *inc, hipStream_t s) {
   if (k > 4) return hipErrorInvalidValue;
   if (k < 2) return hipErrorInvalidValue;
+  const uint32_t small = (unsigned __int128)T * k < ((unsigned __int128)1 << 32) ? 1u : 0u;
   hipLaunchKernelGGL(k_build_inc, dim3(grid_for(T, kIncPairs / k, 16384)),
-                     dim3(kIncThreads), 0, s, T, k, cliques, ioff, icur, inc);
+                     dim3(kIncThreads), 0, s, T, k, cliques, ioff, icur, inc, small);
   return hipGetLastError();
 }
 
