@@ -109,6 +109,11 @@ __device__ __forceinline__ void finish_count(
     }
 }
 
+// A triangle's three ids stored with one 12-byte instruction.
+struct __attribute__((packed, aligned(4))) Clique3 {
+  uint32_t w[3];
+};
+
 template <int THREADS, int CHUNK>
 __global__ __launch_bounds__(THREADS) void k_list3(
     uint64_t num_items, const uint2 *__restrict__ items,
@@ -154,10 +159,7 @@ __global__ __launch_bounds__(THREADS) void k_list3(
       ++found;
       if (write) {
         const uint32_t slot = atomicAdd(&t.cursor, 1u);
-        uint32_t *o = out + 3ull * (obase + slot);
-        o[0] = u;
-        o[1] = t.vid[lo];
-        o[2] = w;
+        *reinterpret_cast<Clique3 *>(out + 3ull * (obase + slot)) = Clique3{{u, t.vid[lo], w}};
       } else if (staged) {
         atomicAdd(&scnt[p], 1u);
         atomicAdd(&scnt[j0 + lo], 1u);

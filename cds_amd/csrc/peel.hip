@@ -252,6 +252,11 @@ __global__ __launch_bounds__(256) void k_peel_process(
 // serialising on one word).  Kill words are derived after the launch
 // (k_kill_from_core).
 constexpr uint32_t kPeelChunk = 64;     // one wave pass per chunk
+// Two member words read with one 8-byte load at 4-byte alignment.
+struct __attribute__((packed, aligned(4))) Members2 {
+  uint32_t x, y;
+  __device__ operator uint2() const { return make_uint2(x, y); }
+};
 constexpr uint32_t kPeelBatch = 4;      // chunks per wave pass with loads in flight
 constexpr uint32_t kPeelMaxK = 4;       // incidence entries hold k <= 4 words
 constexpr int kPeelThreads = 1024;
@@ -516,10 +521,19 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
           uint32_t mu[kPeelBatch][kPeelMaxK - 1], msu[kPeelBatch][kPeelMaxK - 1],
               mpos[kPeelBatch][kPeelMaxK - 1];
 #pragma unroll
-          for (uint32_t s = 0; s < kPeelBatch; ++s)
+          for (uint32_t s = 0; s < kPeelBatch; ++s) {
+            if (k == 3) {  // both members in one 8-byte load
+              uint2 m2 = make_uint2(0u, 0u);
+              if (bok[s]) m2 = *reinterpret_cast<const Members2 *>(inc + bq[s] * 3 + 1);
+              mu[s][0] = m2.x;
+              mu[s][1] = m2.y;
+              mu[s][2] = 0u;
+              continue;
+            }
 #pragma unroll
             for (uint32_t t = 1; t < kPeelMaxK; ++t)
               mu[s][t - 1] = bok[s] && t < k ? inc[bq[s] * k + t] : 0u;
+          }
 #pragma unroll
           for (uint32_t s = 0; s < kPeelBatch; ++s)
 #pragma unroll
