@@ -62,7 +62,39 @@ __global__ __launch_bounds__(256) void k_dag_fill(
   }
 }
 
+// out[0] = max d+(v) (as a 64-bit max), out[1] = sum_v (deg(v) - d+(v)) d+(v),
+// the listing work W: every arc (u, v) contributes d+(v) candidates.
+__global__ void k_dag_stats(uint32_t n, const uint32_t *__restrict__ off,
+                            const uint32_t *__restrict__ dcnt, unsigned long long *out) {
+  unsigned long long mx = 0, w = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += gridDim.x * blockDim.x) {
+    const uint64_t d = dcnt[v], deg = off[v + 1] - off[v];
+    mx = d > mx ? d : mx;
+    w += (deg - d) * d;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(mx, o, kWave);
+    mx = x > mx ? x : mx;
+  }
+  w = wave_sum64(w);
+  if (lane() == 0) {
+    if (mx) atomicMax(&out[0], mx);
+    if (w) atomicAdd(&out[1], w);
+  }
+}
+
 }  // namespace
+
+hipError_t dag_stats(uint32_t n, const uint32_t *off, const uint32_t *dcnt,
+                     unsigned long long *out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_dag_stats, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, off, dcnt,
+                     out);
+  return hipGetLastError();
+}
 
 hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,
                      uint32_t *dcnt, hipStream_t s) {

@@ -341,6 +341,12 @@ static int replay_bits_mode() {
   return atoi(v) != 0 ? 1 : 0;
 }
 
+// CDS_PEEL_TRACE=1: one stderr line per replayed level (diagnostics).
+static bool peel_trace_on() {
+  const char *v = getenv("CDS_PEEL_TRACE");
+  return v && *v && *v != '0';
+}
+
 static bool flow_trace_on() {
   const char *v = getenv("CDS_FLOW_TRACE");
   return v && *v && *v != '0';
@@ -923,14 +929,14 @@ namespace {
 // DAG + listing shared by cds_solve and cds_count_cliques.  On return
 // (materialise = true) `cliques` holds T*k ids AoS.
 struct Listing {
-  DBuf<uint32_t> dcnt, doff, dadj, ipv, item_count, cliques, maxd;
+  DBuf<uint32_t> dcnt, doff, dadj, ipv, item_count, cliques;
   DBuf<uint64_t> item_off, item_base, cd;
   DBuf<uint2> items;
   uint64_t T = 0, items_n = 0, work = 0;
   uint32_t max_out = 0;
   double t_count = 0, t_write = 0;
   explicit Listing(DeviceArena *a)
-      : dcnt(a), doff(a), dadj(a), ipv(a), item_count(a), cliques(a), maxd(a),
+      : dcnt(a), doff(a), dadj(a), ipv(a), item_count(a), cliques(a),
         item_off(a), item_base(a), cd(a), items(a) {}
 };
 
@@ -943,18 +949,14 @@ cds_status run_dag(cds_engine *e, const cds_dgraph *g, Listing &L) {
   HIPCHK(scan_u32_32(L.dcnt.p, L.doff.p, n, A));
   HIPCHK(L.dadj.alloc(g->m));
   HIPCHK(dag_fill(n, g->off, g->adj, L.doff.p, L.dadj.p, e->stream));
-  HIPCHK(L.maxd.alloc(1));
-  HIPCHK(reduce_max_u32(L.dcnt.p, n, L.maxd.p, e->stream));
-  CHK(readback(e, &L.max_out, L.maxd.p, 1));
-  // W = sum over DAG arcs (u,v) of d+(v) = sum_v indeg(v) * d+(v).
-  std::vector<uint32_t> dc(n);
-  CHK(readback(e, dc.data(), L.dcnt.p, n));
-  uint64_t w = 0;
-  for (uint32_t v = 0; v < n; ++v) {
-    const uint64_t deg = g->host_off[v + 1] - g->host_off[v];
-    w += (deg - dc[v]) * (uint64_t)dc[v];
-  }
-  L.work = w;
+  // max d+ and W = sum over DAG arcs (u,v) of d+(v) = sum_v indeg(v) * d+(v)
+  DBuf<unsigned long long> st2(&A);
+  HIPCHK(st2.alloc(2));
+  HIPCHK(dag_stats(n, g->off, L.dcnt.p, st2.p, e->stream));
+  unsigned long long hs[2] = {0, 0};
+  CHK(readback(e, hs, st2.p, 2));
+  L.max_out = (uint32_t)hs[0];
+  L.work = hs[1];
   return CDS_OK;
 }
 
@@ -1677,11 +1679,21 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                                    T - PS.kl_before[ii], Nb, (uint32_t)M, pv.p, posmap.p,
                                    pc.p, bits.p, rout.p, tm.ev[23], (uint64_t)PS.kp,
                                    replay_nstar(k, (uint64_t)PS.kp), st));
+          const auto tb = std::chrono::steady_clock::now();
           ReplayOut ro;
           CHK(readback(e, &ro, rout.p, 1));
           float prep = 0;
           (void)hipEventElapsedTime(&prep, tm.ev[22], tm.ev[23]);
           R->t_replay_prep_ms += prep;
+          if (peel_trace_on())
+            fprintf(stderr,
+                    "[peel] bitset replay of level %u: %u of %llu removals (%s), %u alive, "
+                    "prep %.3f ms, replay+wait %.3f ms\n",
+                    lv.level, ro.steps, (unsigned long long)M, ro.stopped ? "stopped" : "all",
+                    Nb, prep,
+                    std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - tb)
+                        .count());
           R->replay_phases++;
           R->replay_bits_phases++;
           if (ro.steps > M || (!ro.stopped && ro.steps != M))

@@ -106,6 +106,10 @@ hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,
                      uint32_t *dcnt, hipStream_t s);
 hipError_t dag_fill(uint32_t n, const uint32_t *off, const uint32_t *adj,
                     const uint32_t *doff, uint32_t *dadj, hipStream_t s);
+// out[0] = max out-degree, out[1] = the listing work sum_v indeg(v) d+(v)
+// (two device words, cleared here).
+hipError_t dag_stats(uint32_t n, const uint32_t *off, const uint32_t *dcnt,
+                     unsigned long long *out, hipStream_t s);
 // max over dcnt (result written to *out, device)
 hipError_t reduce_max_u32(const uint32_t *in, size_t n, uint32_t *out,
                           hipStream_t s);

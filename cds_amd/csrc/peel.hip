@@ -257,8 +257,7 @@ struct __attribute__((packed, aligned(4))) Members2 {
   uint32_t x, y;
   __device__ operator uint2() const { return make_uint2(x, y); }
 };
-constexpr uint32_t kPeelBatch = 4;      // chunks per wave pass with loads in flight
-constexpr uint32_t kPeelMaxK = 4;       // incidence entries hold k <= 4 words
+constexpr uint32_t kPeelBatch = 2;      // chunks per wave pass with loads in flight
 constexpr int kPeelThreads = 1024;
 constexpr uint64_t kChunkMask = (1ull << 40) - 1;
 // alive vertices aggregated in LDS (128 KB; the persistent peel runs one
@@ -318,6 +317,102 @@ __device__ __forceinline__ void frontier_push(PeelCtl *ctl, int b, uint32_t *F,
   const uint32_t slot = (uint32_t)(old >> 40);
   F[slot] = v;
   FC[slot] = old & kChunkMask;
+}
+
+// A wave's walk over its contiguous chunk range: the current vertex slot lo
+// and vertex v, v's incidence range [vb, ve), v's first chunk vfc, the next
+// vertex's first chunk nfc, and q0, an entry of the range.
+struct PeelWalk {
+  uint32_t lo, v;
+  uint64_t vb, ve, vfc, nfc, q0;
+};
+
+// kPeelBatch chunks [ci, ci + kPeelBatch) of the wave's range (below c1):
+// the entries' K-1 other members, their stamps and (AGG) alive-list
+// positions are all loaded before any is used; every load is unconditional
+// (a lane with no entry re-reads entry q0) and branch-free, so they are in
+// flight together.  Then the claims and the degree decrements.
+template <uint32_t K, bool AGG>
+__device__ __forceinline__ void peel_batch(
+    uint64_t ci, uint64_t c1, PeelWalk &wk, uint32_t fcount, const uint32_t *Fc,
+    const uint64_t *FCc, const uint64_t *__restrict__ ioff, const uint32_t *__restrict__ inc,
+    uint32_t *stamp, const uint32_t *pos, uint32_t *lds_cnt, uint32_t hagg, uint32_t *deg,
+    uint32_t *core, uint32_t sr, uint32_t level, PeelCtl *ctl, int nb, uint32_t *Fn,
+    uint64_t *FCn, uint32_t &nk) {
+  constexpr uint32_t KM = K - 1;
+  uint32_t bv[kPeelBatch];
+  uint64_t bq[kPeelBatch];
+  bool bok[kPeelBatch];
+#pragma unroll
+  for (uint32_t s = 0; s < kPeelBatch; ++s) {
+    const uint64_t cs = ci + s;
+    bok[s] = false;
+    bv[s] = 0;
+    bq[s] = wk.q0;
+    if (cs < c1) {
+      while (cs >= wk.nfc) {  // next vertex (empty lists own no chunk)
+        ++wk.lo;
+        wk.v = Fc[wk.lo];
+        wk.vb = ioff[wk.v];
+        wk.ve = ioff[wk.v + 1];
+        wk.vfc = wk.nfc;
+        wk.nfc = wk.lo + 1 < fcount ? FCc[wk.lo + 1] : ~0ull;
+      }
+      const uint64_t q = wk.vb + (cs - wk.vfc) * kPeelChunk + lane();
+      bok[s] = q < wk.ve;
+      bv[s] = wk.v;
+      bq[s] = bok[s] ? q : wk.q0;
+    }
+  }
+  uint32_t mu[kPeelBatch][KM], msu[kPeelBatch][KM], mpos[kPeelBatch][KM];
+#pragma unroll
+  for (uint32_t s = 0; s < kPeelBatch; ++s) {
+    if (K == 3) {  // both members in one 8-byte load
+      const uint2 m2 = *reinterpret_cast<const Members2 *>(inc + bq[s] * 3 + 1);
+      mu[s][0] = m2.x;
+      mu[s][KM - 1] = m2.y;
+    } else {
+#pragma unroll
+      for (uint32_t t = 0; t < KM; ++t) mu[s][t] = inc[bq[s] * K + t + 1];
+    }
+  }
+#pragma unroll
+  for (uint32_t s = 0; s < kPeelBatch; ++s)
+#pragma unroll
+    for (uint32_t t = 0; t < KM; ++t) {
+      msu[s][t] = ld(&stamp[mu[s][t]]);
+      mpos[s][t] = AGG ? pos[mu[s][t]] : 0u;
+    }
+#pragma unroll
+  for (uint32_t s = 0; s < kPeelBatch; ++s) {
+    if (!bok[s]) continue;
+    bool claim = true;
+#pragma unroll
+    for (uint32_t t = 0; t < KM; ++t) {
+      const uint32_t su = msu[s][t];
+      claim = claim && su >= sr && (su != sr || mu[s][t] > bv[s]);
+    }
+    if (!claim) continue;
+    ++nk;
+#pragma unroll
+    for (uint32_t t = 0; t < KM; ++t) {
+      // members of this frontier (stamp sr) are removed and parked ones
+      // keep their core value in deg: neither is decremented
+      const uint32_t su = msu[s][t], u = mu[s][t];
+      if (su == sr || su == kParked) continue;
+      if (AGG) {
+        atomicAdd(&lds_cnt[mpos[s][t]], 1u);
+        continue;
+      }
+      if (hagg && hash_add(lds_cnt, u)) continue;
+      const uint32_t old = atomicSub(&deg[u], 1u);
+      if (old == level + 1) {
+        core[u] = level;
+        stamp[u] = sr + 1;
+        frontier_push(ctl, nb, Fn, FCn, u, chunks_of(ioff[u + 1] - ioff[u]));
+      }
+    }
+  }
 }
 
 __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
@@ -489,95 +584,22 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_persistent(
         uint32_t lo = wave_last_le(FCc, fcount, c0);  // c0's vertex slot
         uint32_t v = Fc[lo];
         uint64_t vb = ioff[v], ve = ioff[v + 1], vfc = FCc[lo];
+        const uint64_t q0 = vb + (c0 - vfc) * kPeelChunk;  // an entry of the range
         uint64_t nfc = lo + 1 < fcount ? FCc[lo + 1] : ~0ull;
+        PeelWalk wk{lo, v, vb, ve, vfc, nfc, q0};
         for (uint64_t ci = c0; ci < c1; ci += kPeelBatch) {
-          uint32_t bv[kPeelBatch];
-          uint64_t bq[kPeelBatch];
-          bool bok[kPeelBatch];
-#pragma unroll
-          for (uint32_t s = 0; s < kPeelBatch; ++s) {
-            const uint64_t cs = ci + s;
-            bok[s] = false;
-            bv[s] = 0;
-            bq[s] = 0;
-            if (cs < c1) {
-              while (cs >= nfc) {  // next vertex (empty lists own no chunk)
-                ++lo;
-                v = Fc[lo];
-                vb = ioff[v];
-                ve = ioff[v + 1];
-                vfc = nfc;
-                nfc = lo + 1 < fcount ? FCc[lo + 1] : ~0ull;
-              }
-              const uint64_t q = vb + (cs - vfc) * kPeelChunk + lane();
-              bok[s] = q < ve;
-              bv[s] = v;
-              bq[s] = q;
-            }
-          }
-          // the entries' other members, then their stamps (and alive-list
-          // positions when aggregating), all loads of the batch issued
-          // before any is used
-          uint32_t mu[kPeelBatch][kPeelMaxK - 1], msu[kPeelBatch][kPeelMaxK - 1],
-              mpos[kPeelBatch][kPeelMaxK - 1];
-#pragma unroll
-          for (uint32_t s = 0; s < kPeelBatch; ++s) {
-            if (k == 3) {  // both members in one 8-byte load
-              uint2 m2 = make_uint2(0u, 0u);
-              if (bok[s]) m2 = *reinterpret_cast<const Members2 *>(inc + bq[s] * 3 + 1);
-              mu[s][0] = m2.x;
-              mu[s][1] = m2.y;
-              mu[s][2] = 0u;
-              continue;
-            }
-#pragma unroll
-            for (uint32_t t = 1; t < kPeelMaxK; ++t)
-              mu[s][t - 1] = bok[s] && t < k ? inc[bq[s] * k + t] : 0u;
-          }
-#pragma unroll
-          for (uint32_t s = 0; s < kPeelBatch; ++s)
-#pragma unroll
-            for (uint32_t t = 1; t < kPeelMaxK; ++t) {
-              msu[s][t - 1] = kAlive;
-              mpos[s][t - 1] = 0;
-              if (bok[s] && t < k) {
-                msu[s][t - 1] = ld(&stamp[mu[s][t - 1]]);
-                if (agg) mpos[s][t - 1] = pos[mu[s][t - 1]];
-              }
-            }
-#pragma unroll
-          for (uint32_t s = 0; s < kPeelBatch; ++s) {
-            if (!bok[s]) continue;
-            bool claim = true;
-#pragma unroll
-            for (uint32_t t = 1; t < kPeelMaxK; ++t)
-              if (t < k) {
-                const uint32_t su = msu[s][t - 1];
-                claim = claim && su >= sr && (su != sr || mu[s][t - 1] > bv[s]);
-              }
-            if (!claim) continue;
-            ++nk;
-#pragma unroll
-            for (uint32_t t = 1; t < kPeelMaxK; ++t) {
-              if (t >= k) continue;
-              // members of this frontier (stamp sr) are removed and parked
-              // ones keep their core value in deg: neither is decremented
-              const uint32_t su = msu[s][t - 1], u = mu[s][t - 1];
-              if (su == sr || su == kParked) continue;
-              if (agg) {
-                atomicAdd(&lds_cnt[mpos[s][t - 1]], 1u);
-                continue;
-              }
-              if (hagg && hash_add(lds_cnt, u)) continue;
-              const uint32_t old = atomicSub(&deg[u], 1u);
-              if (old == level + 1) {
-                core[u] = level;
-                stamp[u] = sr + 1;
-                frontier_push(ctl, nb, F[nb], FC[nb], u,
-                              chunks_of(ioff[u + 1] - ioff[u]));
-              }
-            }
-          }
+#define CDS_PEEL_BATCH(K)                                                                  \
+  (agg ? peel_batch<K, true>(ci, c1, wk, fcount, Fc, FCc, ioff, inc, stamp, pos, lds_cnt, \
+                             hagg, deg, core, sr, level, ctl, nb, F[nb], FC[nb], nk)       \
+       : peel_batch<K, false>(ci, c1, wk, fcount, Fc, FCc, ioff, inc, stamp, pos, lds_cnt,\
+                              hagg, deg, core, sr, level, ctl, nb, F[nb], FC[nb], nk))
+          if (k == 3)
+            CDS_PEEL_BATCH(3);
+          else if (k == 2)
+            CDS_PEEL_BATCH(2);
+          else
+            CDS_PEEL_BATCH(4);
+#undef CDS_PEEL_BATCH
         }
       }
       if (leader) {
@@ -1169,18 +1191,20 @@ __device__ __forceinline__ void bits_dec(uint32_t nu, uint32_t W, const uint32_t
   const uint32_t wid = threadIdx.x / kWave, ln = lane();
   const uint32_t gl = ln % G, grp = ln / G;
   const unsigned long long mine = gl < W ? nbr[gl] : 0ull;
+  const uint32_t gw = min(gl, W - 1);  // lanes past the row re-read its last word
   for (uint32_t base = wid * chunk; base < nu; base += kWaves * chunk) {
+    // every load unconditional (entries past the list re-read the last one),
+    // so all kU are in flight before the first is used
     unsigned long long x[kU];
 #pragma unroll
     for (uint32_t r = 0; r < kU; ++r) {
-      const uint32_t t = base + r * gpw + grp;
-      x[r] = 0;
-      if (t < nu && gl < W) x[r] = bits[(size_t)ul[t] * W + gl] & mine;
+      const uint32_t t = min(base + r * gpw + grp, nu - 1);
+      x[r] = bits[(size_t)ul[t] * W + gw];
     }
 #pragma unroll
     for (uint32_t r = 0; r < kU; ++r) {
       const uint32_t t = base + r * gpw + grp;
-      const uint32_t c = group_sum<G>((uint32_t)__popcll(x[r]));
+      const uint32_t c = group_sum<G>((uint32_t)__popcll(x[r] & mine));
       if (gl == 0 && t < nu) deg[ul[t]] -= c;
     }
   }
