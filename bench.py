@@ -643,7 +643,8 @@ def main():
         # exact replay of the levels the answer depends on; its bytes depend
         # on the phase (a pass over the phase's incidence, or over bitset
         # rows, per removal), not counted here
-        bits, grid = avg("replay_bits_phases"), avg("replay_grid_phases")
+        bits = avg("replay_bits_phases") + avg("refine_bits_steps")
+        grid = avg("replay_grid_phases")
         name = ("k_replay_bits" if bits and not grid else
                 "k_replay_grid" if grid and not bits else
                 "k_replay_bits + k_replay_grid" if bits else "k_replay")
@@ -708,6 +709,7 @@ def main():
                  "refined_steps": r["peel_refined_steps"],
                  "replay_phases": r["replay_phases"], "replay_ms": avg("t_replay_ms"),
                  "replay_bits_phases": r["replay_bits_phases"],
+                 "refine_bits_steps": r["refine_bits_steps"],
                  # one thread's wall-clock split of the persistent launches
                  "split_ms": r["peel_split_ms"],
                  # graphs too large to materialise: solved on the prune_tau-core

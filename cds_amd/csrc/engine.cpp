@@ -1563,11 +1563,108 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     fb.reset();
     R->t_incidence_ms = tm.ms(ev_inc0, ev_inc1);
 
+    std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
+    // Replays of exact levels taken from a bitset re-peel's removal
+    // sequence; each is used when its level becomes a replay candidate.
+    std::vector<std::pair<uint32_t, ReplayOut>> seq_replays;
+    // Exact re-peel of coarse step i on adjacency bitsets (k <= 3, at most
+    // kReplayBitsMaxVertices alive at the step's start): one workgroup
+    // removes the step's vertices one at a time in the oracle's order, so
+    // the sequence gives the core numbers (running maximum of the removal
+    // counts), the exact levels, and every level's replay at once.
+    auto refine_step_bits = [&](size_t i) -> cds_status {
+      const Level lv = levels[i];
+      const uint32_t Nb = (uint32_t)(n - PS.rm_before[i]), M = (uint32_t)lv.removed;
+      const uint64_t T_start = T - PS.kl_before[i];
+      DBuf<uint32_t> av(&A), pmap(&A), cnt(&A), seq(&A), hvd(&A), hcd(&A);
+      DBuf<unsigned long long> bits(&A);
+      DBuf<ReplayOut> rod(&A);
+      HIPCHK(av.alloc(n));
+      HIPCHK(pmap.alloc(n));
+      HIPCHK(cnt.alloc(1));
+      HIPCHK(seq.alloc(2ull * M));
+      HIPCHK(bits.alloc((size_t)Nb * ((Nb + 63) / 64)));
+      HIPCHK(rod.alloc(1));
+      HIPCHK(replay_phase_bits(k, lv.level, n, L.doff.p, L.dadj.p, core.p, sdeg.p, T_start, Nb,
+                               M, av.p, pmap.p, cnt.p, bits.p, rod.p, nullptr, 0, 0, st,
+                               seq.p));
+      ReplayOut ro;
+      CHK(readback(e, &ro, rod.p, 1));
+      if (ro.steps != M)
+        return fail(CDS_ERR_INTERNAL, "bitset re-peel of step %u ran %u of %u removals",
+                    lv.level, ro.steps, M);
+      std::vector<uint32_t> hs(2ull * M), hv(M), hc(M);
+      CHK(readback(e, hs.data(), seq.p, 2ull * M));
+      std::vector<Level> exact;
+      uint32_t run = i > 0 ? levels[i - 1].level : 0;
+      for (uint32_t j = 0; j < M; ++j) {
+        run = std::max(run, hs[2 * j + 1]);
+        hv[j] = hs[2 * j];
+        hc[j] = run;
+        if (hv[j] >= n) return fail(CDS_ERR_INTERNAL, "bitset re-peel removed vertex %u", hv[j]);
+        if (exact.empty() || exact.back().level != run) exact.push_back({run, 0, 0});
+        exact.back().removed++;
+        exact.back().killed += hs[2 * j + 1];
+      }
+      uint64_t kl = 0;
+      for (const Level &x : exact) {
+        kl += x.killed;
+        if (x.level < lv.lo || x.level > lv.level)
+          return fail(CDS_ERR_INTERNAL, "coarse step [%u, %u] re-peeled to level %u", lv.lo,
+                      lv.level, x.level);
+      }
+      if (kl != lv.killed)
+        return fail(CDS_ERR_INTERNAL, "coarse step %u killed %llu of %llu cliques", lv.level,
+                    (unsigned long long)kl, (unsigned long long)lv.killed);
+      // each exact level's replay: its densest row strictly inside it
+      uint64_t Tl = T_start, Nl = Nb;
+      uint32_t j0 = 0;
+      for (const Level &x : exact) {
+        if (x.removed >= 2) {
+          ReplayOut r{};
+          r.best_den = 1;
+          uint64_t Rl = 0;
+          for (uint64_t q = 0; q < x.removed; ++q) {
+            Rl += hs[2 * (j0 + q) + 1];
+            if (q + 1 < x.removed) {
+              const uint64_t num = Tl - Rl, den = Nl - (q + 1);
+              if (!r.found || (unsigned __int128)num * r.best_den >
+                                  (unsigned __int128)r.best_num * den) {
+                r.best_num = num;
+                r.best_den = den;
+                r.found = 1;
+              }
+            }
+          }
+          r.steps = (uint32_t)x.removed;
+          seq_replays.push_back({x.level, r});
+        }
+        Tl -= x.killed;
+        Nl -= x.removed;
+        j0 += (uint32_t)x.removed;
+      }
+      HIPCHK(hvd.alloc(M));
+      HIPCHK(hcd.alloc(M));
+      CHK(upload(e, hvd.p, hv.data(), M));
+      CHK(upload(e, hcd.p, hc.data(), M));
+      HIPCHK(scatter_u32(M, hvd.p, hcd.p, core.p, st));
+      HIPCHK(hipStreamSynchronize(st));
+      kill_valid = false;
+      levels.erase(levels.begin() + (ptrdiff_t)i);
+      levels.insert(levels.begin() + (ptrdiff_t)i, exact.begin(), exact.end());
+      R->peel_refined_steps++;
+      R->refine_bits_steps++;
+      return CDS_OK;
+    };
+
     // Exact re-peel of coarse step i: its vertices and cliques become alive
     // again (degrees from sdeg), later steps are parked, and the step's
     // entry is replaced by its exact levels.
     auto refine_step = [&](size_t i) -> cds_status {
       const Level lv = levels[i];
+      if ((k == 2 || k == 3) && sdeg.p && replay_bits_mode() != 0 &&
+          n - PS.rm_before[i] <= kReplayBitsMaxVertices)
+        return refine_step_bits(i);
       if (!ralive.p) {
         HIPCHK(ralive.alloc(n));
         HIPCHK(rcount.alloc(1));
@@ -1609,7 +1706,6 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
 
     // Coarse steps are refined and exact levels replayed until neither list
     // has work: each changes the table or rho, so the summary is recomputed.
-    std::vector<std::pair<uint32_t, ReplayOut>> replayed;  // by level
     const bool all_cores = p->want_vertex_arrays != 0;
     auto summarize = [&]() -> cds_status {
       double rho_r = 0;
@@ -1631,11 +1727,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       for (;;) {
         if (!PS.refine.empty()) {
           const auto t0 = std::chrono::steady_clock::now();
+          const uint32_t bits_before = R->refine_bits_steps;
           CHK(refine_step(PS.refine.front()));
           HIPCHK(hipStreamSynchronize(st));
-          refine_ms += std::chrono::duration<double, std::milli>(
-                           std::chrono::steady_clock::now() - t0)
-                           .count();
+          // a bitset re-peel is also its levels' replay: its time stays in
+          // the replay time, a grid re-peel's goes to the peel
+          if (R->refine_bits_steps == bits_before)
+            refine_ms += std::chrono::duration<double, std::milli>(
+                             std::chrono::steady_clock::now() - t0)
+                             .count();
           CHK(summarize());
           continue;
         }
@@ -1651,6 +1751,20 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
         if (ii == SIZE_MAX) break;
         const Level &lv = levels[ii];
         const uint64_t M = lv.removed;
+        {
+          // already replayed by a bitset re-peel of its coarse step
+          auto it = std::find_if(seq_replays.begin(), seq_replays.end(),
+                                 [&](const std::pair<uint32_t, ReplayOut> &r) {
+                                   return r.first == lv.level;
+                                 });
+          if (it != seq_replays.end()) {
+            R->replay_phases++;
+            R->replay_bits_phases++;
+            replayed.push_back(*it);
+            CHK(summarize());
+            continue;
+          }
+        }
         if (!allocated) {
           HIPCHK(pv.alloc(n));
           HIPCHK(pdeg.alloc(n));

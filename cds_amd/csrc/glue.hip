@@ -709,6 +709,20 @@ __global__ void k_scatter_u32x2(uint32_t m, const uint32_t *__restrict__ idx,
   }
 }
 
+__global__ void k_scatter_u32(uint32_t m, const uint32_t *__restrict__ idx,
+                              const uint32_t *__restrict__ a, uint32_t *__restrict__ dst) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    dst[idx[i]] = a[i];
+}
+
+hipError_t scatter_u32(uint32_t m, const uint32_t *idx, const uint32_t *a, uint32_t *dst,
+                       hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_u32, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, m, idx, a,
+                     dst);
+  return hipGetLastError();
+}
+
 hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,
                       hipStream_t s) {
   if (m == 0) return hipSuccess;

@@ -298,13 +298,15 @@ hipError_t replay_phase_list(uint32_t k, uint32_t level, const uint32_t *pv, uin
 // barriers.  doff/dadj: the degree-ordered DAG; sdeg: the degrees at the
 // level's start; av and posmap hold n entries, bits N * ceil(N/64) words.  The
 // kernel checks that N vertices are alive and otherwise reports 0 steps.
+// seq (2 M words, or null): the j-th removal's vertex and its clique count
+// then, at seq[2j] and seq[2j+1].
 constexpr uint32_t kReplayBitsMaxVertices = 4096;
 hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint32_t *doff,
                              const uint32_t *dadj, const uint32_t *core, const uint32_t *sdeg,
                              uint64_t T_start, uint32_t N, uint32_t M, uint32_t *av,
                              uint32_t *posmap, uint32_t *pcount, unsigned long long *bits,
                              ReplayOut *out, hipEvent_t prepared, uint64_t kp, uint64_t nstar,
-                             hipStream_t s);
+                             hipStream_t s, uint32_t *seq = nullptr);
 hipError_t replay_phase(uint32_t k, uint32_t level, uint32_t n,
                         const uint32_t *core, const uint32_t *kill,
                         const uint64_t *ioff, const uint32_t *inc,
@@ -354,6 +356,9 @@ hipError_t core_cliques_ranged(uint32_t cn, const uint32_t *cv, uint32_t k,
                                const uint64_t *item_base, const uint32_t *cliques,
                                const uint32_t *core, uint32_t kp, uint32_t *cc,
                                unsigned long long *out, DeviceArena &a);
+// dst[idx[i]] = a[i], i < m.
+hipError_t scatter_u32(uint32_t m, const uint32_t *idx, const uint32_t *a, uint32_t *dst,
+                       hipStream_t s);
 // dst[i] = src[idx[i]] and da[idx[i]] = a[i], db[idx[i]] = b[i], i < m.
 hipError_t gather_u32(uint32_t m, const uint32_t *idx, const uint32_t *src, uint32_t *dst,
                       hipStream_t s);

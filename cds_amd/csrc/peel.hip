@@ -1181,31 +1181,44 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
 // Degree updates of one removal: list entry t (a phase position u adjacent
 // to v) loses popcount(row u & nbr), summed over a group of G >= W lanes
 // (one word each); kU entries per lane are loaded before any is summed.
-template <uint32_t K, uint32_t G>
+template <uint32_t G, uint32_t WPL>
 __device__ __forceinline__ void bits_dec(uint32_t nu, uint32_t W, const uint32_t *ul,
                                          const unsigned long long *__restrict__ bits,
                                          const unsigned long long *nbr, uint32_t *deg) {
   constexpr uint32_t kWaves = kReplayThreads / kWave;
-  constexpr uint32_t kU = 16;
+  constexpr uint32_t kU = 8;
   constexpr uint32_t gpw = kWave / G, chunk = kU * gpw;
   const uint32_t wid = threadIdx.x / kWave, ln = lane();
   const uint32_t gl = ln % G, grp = ln / G;
-  const unsigned long long mine = gl < W ? nbr[gl] : 0ull;
-  const uint32_t gw = min(gl, W - 1);  // lanes past the row re-read its last word
+  // this lane's words of a row: gl, gl + G, ... (WPL of them, those past
+  // the row re-read its last word and are masked off)
+  unsigned long long mine[WPL];
+  uint32_t gw[WPL];
+#pragma unroll
+  for (uint32_t q = 0; q < WPL; ++q) {
+    const uint32_t w = gl + q * G;
+    mine[q] = w < W ? nbr[w] : 0ull;
+    gw[q] = min(w, W - 1);
+  }
   for (uint32_t base = wid * chunk; base < nu; base += kWaves * chunk) {
     // every load unconditional (entries past the list re-read the last one),
-    // so all kU are in flight before the first is used
-    unsigned long long x[kU];
+    // so all kU * WPL are in flight before the first is used
+    unsigned long long x[kU][WPL];
 #pragma unroll
     for (uint32_t r = 0; r < kU; ++r) {
-      const uint32_t t = min(base + r * gpw + grp, nu - 1);
-      x[r] = bits[(size_t)ul[t] * W + gw];
+      const unsigned long long *row =
+          bits + (size_t)ul[min(base + r * gpw + grp, nu - 1)] * W;
+#pragma unroll
+      for (uint32_t q = 0; q < WPL; ++q) x[r][q] = row[gw[q]];
     }
 #pragma unroll
     for (uint32_t r = 0; r < kU; ++r) {
       const uint32_t t = base + r * gpw + grp;
-      const uint32_t c = group_sum<G>((uint32_t)__popcll(x[r] & mine));
-      if (gl == 0 && t < nu) deg[ul[t]] -= c;
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < WPL; ++q) c += (uint32_t)__popcll(x[r][q] & mine[q]);
+      c = group_sum<G>(c);
+      if (gl == 0 && t < nu) atomicSub(&deg[ul[t]], c);  // no return: no wait
     }
   }
 }
@@ -1221,7 +1234,7 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     uint32_t level, uint32_t N, uint32_t M, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ av, const uint32_t *__restrict__ core,
     const uint32_t *__restrict__ sdeg, const unsigned long long *__restrict__ bits,
-    uint64_t T_start, ReplayOut *out, uint64_t kp, uint64_t nstar) {
+    uint64_t T_start, ReplayOut *out, uint64_t kp, uint64_t nstar, uint32_t *__restrict__ seq) {
   constexpr uint32_t kWaves = kReplayThreads / kWave;
   static_assert(kWaves == 16, "the key reduction spans one row of 16 lanes");
   __shared__ uint32_t deg[kReplayBitsMaxVertices];
@@ -1261,7 +1274,9 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     }
   }
   if (tid == 0) s_stop = 0;
-  const uint32_t G = W <= 8 ? 8 : W <= 16 ? 16 : W <= 32 ? 32 : 64;
+  // lanes per row for the degree updates: groups of 8 (DPP-only sums) while
+  // a lane's share of a row stays at most 3 words
+  const uint32_t G = W <= 24 ? 8 : W <= 48 ? 16 : 32;
   uint64_t R = 0, best_num = 0, best_den = 1;
   uint32_t found = 0, steps = 0, stopped = 0;
   __syncthreads();
@@ -1315,6 +1330,10 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     if (tid == 0) {
       alive[iv / kWave] &= ~(1ull << (iv % kWave));
       R += m >> 32;
+      if (seq) {  // the removal sequence: vertex, then its clique count
+        seq[2 * j] = (uint32_t)m;
+        seq[2 * j + 1] = (uint32_t)(m >> 32);
+      }
       ++steps;
       if (j + 1 < M) {
         const uint64_t num = T_start - R, den = (uint64_t)N - (j + 1);
@@ -1332,13 +1351,11 @@ __global__ __launch_bounds__(kReplayThreads) void k_replay_bits(
     if (K == 2) {
       for (uint32_t t = tid; t < nu; t += kReplayThreads) deg[ul[t]] -= 1;
     } else if (G == 8) {
-      bits_dec<K, 8>(nu, W, ul, bits, nbr, deg);
+      bits_dec<8, 3>(nu, W, ul, bits, nbr, deg);
     } else if (G == 16) {
-      bits_dec<K, 16>(nu, W, ul, bits, nbr, deg);
-    } else if (G == 32) {
-      bits_dec<K, 32>(nu, W, ul, bits, nbr, deg);
+      bits_dec<16, 3>(nu, W, ul, bits, nbr, deg);
     } else {
-      bits_dec<K, 64>(nu, W, ul, bits, nbr, deg);
+      bits_dec<32, 2>(nu, W, ul, bits, nbr, deg);
     }
     __syncthreads();
     if (s_stop) break;
@@ -1523,7 +1540,7 @@ hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint3
                              uint64_t T_start, uint32_t N, uint32_t M, uint32_t *av,
                              uint32_t *posmap, uint32_t *pcount, unsigned long long *bits,
                              ReplayOut *out, hipEvent_t prepared, uint64_t kp, uint64_t nstar,
-                             hipStream_t s) {
+                             hipStream_t s, uint32_t *seq) {
   if ((k != 2 && k != 3) || N == 0 || N > kReplayBitsMaxVertices || M == 0 || M > N)
     return hipErrorInvalidValue;
   const uint32_t W = (N + 63) / 64;
@@ -1539,10 +1556,10 @@ hipError_t replay_phase_bits(uint32_t k, uint32_t level, uint32_t n, const uint3
   if (prepared) (void)hipEventRecord(prepared, s);
   if (k == 2)
     hipLaunchKernelGGL(k_replay_bits<2>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
-                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar);
+                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar, seq);
   else
     hipLaunchKernelGGL(k_replay_bits<3>, dim3(1), dim3(kReplayThreads), 0, s, level, N, M,
-                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar);
+                       pcount, av, core, sdeg, bits, T_start, out, kp, nstar, seq);
   return hipGetLastError();
 }
 

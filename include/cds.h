@@ -202,6 +202,9 @@ typedef struct {
   /* Replayed levels (of replay_phases) that ran on adjacency bitsets in one
    * workgroup (k <= 3, at most 4096 vertices alive at the level's start). */
   uint32_t replay_bits_phases;
+  /* Coarse peel steps (of peel_refined_steps) re-peeled exactly on
+   * adjacency bitsets in one workgroup, their levels' replays included. */
+  uint32_t refine_bits_steps;
 } cds_result;
 
 typedef struct {
