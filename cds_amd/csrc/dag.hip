@@ -63,16 +63,29 @@ __global__ __launch_bounds__(256) void k_dag_fill(
 }
 
 // out[0] = max d+(v) (as a 64-bit max), out[1] = sum_v (deg(v) - d+(v)) d+(v),
-// the listing work W: every arc (u, v) contributes d+(v) candidates.
-__global__ void k_dag_stats(uint32_t n, const uint32_t *__restrict__ off,
-                            const uint32_t *__restrict__ dcnt, unsigned long long *out) {
-  unsigned long long mx = 0, w = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += gridDim.x * blockDim.x) {
-    const uint64_t d = dcnt[v], deg = off[v + 1] - off[v];
-    mx = d > mx ? d : mx;
-    w += (deg - d) * d;
+// the listing work W: every arc (u, v) contributes d+(v) candidates;
+// out[2] = sum over arcs (u, v) of min(d+(u), d+(v)), a bound on the
+// triangles (each arc closes at most that many).  One wave per vertex u.
+__global__ __launch_bounds__(256) void k_dag_stats(uint32_t n, const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ dcnt,
+                                                   const uint32_t *__restrict__ doff,
+                                                   const uint32_t *__restrict__ dadj,
+                                                   unsigned long long *out) {
+  unsigned long long mx = 0, w = 0, tb = 0;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    const uint64_t d = dcnt[u], deg = off[u + 1] - off[u];
+    if (lane() == 0) {
+      mx = d > mx ? d : mx;
+      w += (deg - d) * d;
+    }
+    for (uint32_t p = doff[u] + lane(); p < doff[u + 1]; p += kWave) {
+      const uint64_t dv = dcnt[dadj[p]];
+      tb += dv < d ? dv : d;
+    }
   }
+  tb = wave_sum64(tb);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long x = __shfl_xor(mx, o, kWave);
@@ -82,17 +95,19 @@ __global__ void k_dag_stats(uint32_t n, const uint32_t *__restrict__ off,
   if (lane() == 0) {
     if (mx) atomicMax(&out[0], mx);
     if (w) atomicAdd(&out[1], w);
+    if (tb) atomicAdd(&out[2], tb);
   }
 }
 
 }  // namespace
 
 hipError_t dag_stats(uint32_t n, const uint32_t *off, const uint32_t *dcnt,
-                     unsigned long long *out, hipStream_t s) {
-  hipError_t e = hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), s);
+                     const uint32_t *doff, const uint32_t *dadj, unsigned long long *out,
+                     hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, 3 * sizeof(unsigned long long), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_dag_stats, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, off, dcnt,
-                     out);
+  hipLaunchKernelGGL(k_dag_stats, dim3(grid_for((size_t)n * kWave, 256, 8192)), dim3(256), 0, s,
+                     n, off, dcnt, doff, dadj, out);
   return hipGetLastError();
 }
 

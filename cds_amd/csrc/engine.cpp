@@ -932,12 +932,19 @@ struct Listing {
   DBuf<uint32_t> dcnt, doff, dadj, ipv, item_count, cliques;
   DBuf<uint64_t> item_off, item_base, cd;
   DBuf<uint2> items;
-  uint64_t T = 0, items_n = 0, work = 0;
+  // staged triangles of the count pass (k = 3), moved into place by the
+  // write step instead of a second listing pass
+  DBuf<uint32_t> stage, sover;
+  DBuf<uint64_t> item_tmp;
+  DBuf<unsigned long long> scur;
+  bool staged = false;
+  uint64_t T = 0, items_n = 0, work = 0, tri_bound = 0;
   uint32_t max_out = 0;
   double t_count = 0, t_write = 0;
   explicit Listing(DeviceArena *a)
       : dcnt(a), doff(a), dadj(a), ipv(a), item_count(a), cliques(a),
-        item_off(a), item_base(a), cd(a), items(a) {}
+        item_off(a), item_base(a), cd(a), items(a), stage(a), sover(a), item_tmp(a),
+        scur(a) {}
 };
 
 cds_status run_dag(cds_engine *e, const cds_dgraph *g, Listing &L) {
@@ -949,21 +956,31 @@ cds_status run_dag(cds_engine *e, const cds_dgraph *g, Listing &L) {
   HIPCHK(scan_u32_32(L.dcnt.p, L.doff.p, n, A));
   HIPCHK(L.dadj.alloc(g->m));
   HIPCHK(dag_fill(n, g->off, g->adj, L.doff.p, L.dadj.p, e->stream));
-  // max d+ and W = sum over DAG arcs (u,v) of d+(v) = sum_v indeg(v) * d+(v)
-  DBuf<unsigned long long> st2(&A);
-  HIPCHK(st2.alloc(2));
-  HIPCHK(dag_stats(n, g->off, L.dcnt.p, st2.p, e->stream));
-  unsigned long long hs[2] = {0, 0};
-  CHK(readback(e, hs, st2.p, 2));
+  // max d+, W = sum over DAG arcs (u,v) of d+(v) = sum_v indeg(v) * d+(v),
+  // and the triangle bound sum over arcs of min(d+(u), d+(v))
+  DBuf<unsigned long long> st3(&A);
+  HIPCHK(st3.alloc(3));
+  HIPCHK(dag_stats(n, g->off, L.dcnt.p, L.doff.p, L.dadj.p, st3.p, e->stream));
+  unsigned long long hs[3] = {0, 0, 0};
+  CHK(readback(e, hs, st3.p, 3));
   L.max_out = (uint32_t)hs[0];
   L.work = hs[1];
+  L.tri_bound = hs[2];
   return CDS_OK;
 }
 
 cds_status run_listing_write(cds_engine *e, uint32_t k, Listing &L, Timer *tm);
 
+// Staged triangle listing (on unless CDS_LIST_STAGE=0).
+static bool list_stage_on() {
+  const char *v = getenv("CDS_LIST_STAGE");
+  return !v || !*v || *v != '0';
+}
+
+// will_write: the cliques are written later (run_listing_write), so the
+// count pass may stage them.
 cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
-                       bool materialise, Listing &L, Timer *tm) {
+                       bool materialise, Listing &L, Timer *tm, bool will_write = false) {
   DeviceArena &A = *e->arena;
   const uint32_t n = g->n;
   HIPCHK(L.cd.alloc(n));
@@ -997,13 +1014,43 @@ cds_status run_listing(cds_engine *e, const cds_dgraph *g, uint32_t k,
   HIPCHK(list_item_fill(n, shape.chunk, L.doff.p, L.item_off.p, L.items.p, e->stream));
   HIPCHK(L.item_count.alloc(L.items_n));
   HIPCHK(L.item_base.alloc(L.items_n + 1));
+  // Triangles: the count pass also stages every triangle it finds, each
+  // item in a reservation of its worst case, when those fit a quarter of
+  // the free device memory (CDS_LIST_STAGE=0 turns it off)
+  ListStage stg;
+  if (k == 3 && (materialise || will_write) && list_stage_on()) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+        (unsigned __int128)L.tri_bound * 12 <= free_b / 4 && L.tri_bound < (1ull << 40)) {
+      HIPCHK(L.stage.alloc(3 * std::max<uint64_t>(L.tri_bound, 1)));
+      HIPCHK(L.item_tmp.alloc(L.items_n));
+      HIPCHK(L.scur.alloc(1));
+      HIPCHK(L.scur.zero());
+      HIPCHK(L.sover.alloc(1));
+      HIPCHK(L.sover.zero());
+      stg.buf = L.stage.p;
+      stg.cursor = L.scur.p;
+      stg.cap = L.tri_bound;
+      stg.item_tmp = L.item_tmp.p;
+      stg.over = L.sover.p;
+    }
+  }
   int t0 = tm->mark();
   HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
-                      L.item_count.p, L.cd.p, nullptr, nullptr, e->stream));
+                      L.item_count.p, L.cd.p, nullptr, nullptr, e->stream, stg));
   int t1 = tm->mark();
   HIPCHK(scan_u32(L.item_count.p, L.item_base.p, L.items_n, A));
   CHK(readback(e, &L.T, L.item_base.p + L.items_n, 1));
   L.t_count = tm->ms(t0, t1);
+  if (stg.buf) {
+    uint32_t over = 1;
+    CHK(readback(e, &over, L.sover.p, 1));
+    L.staged = over == 0;
+    if (!L.staged) {  // the bound did not hold (it always should): list again
+      L.stage.reset();
+      L.item_tmp.reset();
+    }
+  }
   if (materialise) CHK(run_listing_write(e, k, L, tm));
   return CDS_OK;
 }
@@ -1017,11 +1064,18 @@ cds_status run_listing_write(cds_engine *e, uint32_t k, Listing &L, Timer *tm) {
   const ListShape shape = list_shape(e);
   HIPCHK(L.cliques.alloc(L.T * k));
   int t2 = tm->mark();
-  HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
-                      nullptr, nullptr, L.item_base.p, L.cliques.p, e->stream));
+  if (L.staged) {
+    HIPCHK(list_unstage(L.items_n, L.item_tmp.p, L.item_count.p, L.item_base.p, L.stage.p,
+                        L.cliques.p, e->stream));
+  } else {
+    HIPCHK(list_cliques(k, shape, L.items_n, L.items.p, L.doff.p, L.dadj.p, L.max_out,
+                        nullptr, nullptr, L.item_base.p, L.cliques.p, e->stream));
+  }
   int t3 = tm->mark();
   HIPCHK(hipStreamSynchronize(e->stream));
   L.t_write = tm->ms(t2, t3);
+  L.stage.reset();
+  L.item_tmp.reset();
   return CDS_OK;
 }
 
@@ -1367,10 +1421,13 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   Listing L(&A);
   CHK(run_dag(e, g, L));
   const int ev_dag = tm.mark();
-  CHK(run_listing(e, g, k, k == 2, L, &tm));
+  CHK(run_listing(e, g, k, k == 2, L, &tm, true));
   if (k > 2) {
-    if (mode == kSolveInput && needs_prune(k, L.T))
+    if (mode == kSolveInput && needs_prune(k, L.T)) {
+      L.stage.reset();
+      L.item_tmp.reset();
       return solve_pruned(e, g, p, R, core_only, L);
+    }
     CHK(run_listing_write(e, k, L, &tm));
   }
   const int ev_list = tm.mark();

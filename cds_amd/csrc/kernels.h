@@ -106,10 +106,12 @@ hipError_t dag_count(uint32_t n, const uint32_t *off, const uint32_t *adj,
                      uint32_t *dcnt, hipStream_t s);
 hipError_t dag_fill(uint32_t n, const uint32_t *off, const uint32_t *adj,
                     const uint32_t *doff, uint32_t *dadj, hipStream_t s);
-// out[0] = max out-degree, out[1] = the listing work sum_v indeg(v) d+(v)
-// (two device words, cleared here).
+// out[0] = max out-degree, out[1] = the listing work sum_v indeg(v) d+(v),
+// out[2] = sum over DAG arcs (u, v) of min(d+(u), d+(v)), a bound on the
+// triangle count (three device words, cleared here).
 hipError_t dag_stats(uint32_t n, const uint32_t *off, const uint32_t *dcnt,
-                     unsigned long long *out, hipStream_t s);
+                     const uint32_t *doff, const uint32_t *dadj, unsigned long long *out,
+                     hipStream_t s);
 // max over dcnt (result written to *out, device)
 hipError_t reduce_max_u32(const uint32_t *in, size_t n, uint32_t *out,
                           hipStream_t s);
@@ -136,11 +138,30 @@ hipError_t list_item_fill(uint32_t n, uint32_t chunk, const uint32_t *doff,
 // item i, clique_degree += per-vertex contributions.  Write mode: cliques
 // written AoS (k ids each) at out + k * item_base[i].  hipErrorInvalidValue
 // for a shape not listed above.
+// Staging (k = 3, count mode; st.buf null: none): each item reserves its
+// worst case, sum over its arcs (u, v) of min(d+(u), d+(v)) triangles, at
+// *st.cursor in st.buf (st.cap triangles) and writes the triangles it finds
+// there (3 ids each, item_tmp[i] = the reservation, ~0 and *st.over = 1 if
+// it did not fit), so the cliques can be moved into place by
+// list_unstage instead of being listed again.
+struct ListStage {
+  uint32_t *buf = nullptr;
+  unsigned long long *cursor = nullptr;
+  uint64_t cap = 0;
+  uint64_t *item_tmp = nullptr;
+  uint32_t *over = nullptr;
+};
 hipError_t list_cliques(uint32_t k, ListShape shape, uint64_t num_items,
                         const uint2 *items, const uint32_t *doff,
                         const uint32_t *dadj, uint32_t max_out_degree,
                         uint32_t *item_count, uint64_t *clique_degree,
-                        const uint64_t *item_base, uint32_t *out, hipStream_t s);
+                        const uint64_t *item_base, uint32_t *out, hipStream_t s,
+                        ListStage st = ListStage());
+// The staged triangles of every item, item_count[i] of them at
+// item_tmp[i], to their place item_base[i] in out (3 ids each).
+hipError_t list_unstage(uint64_t num_items, const uint64_t *item_tmp,
+                        const uint32_t *item_count, const uint64_t *item_base,
+                        const uint32_t *stage, uint32_t *out, hipStream_t s);
 // k = 2: every DAG arc is a clique.
 hipError_t list_edges(uint32_t n, const uint32_t *doff, const uint32_t *dadj,
                       uint64_t *clique_degree, uint32_t *out, hipStream_t s);

@@ -57,13 +57,17 @@ struct ChunkTable {
   uint32_t vbase[CHUNK];
   uint32_t wave_found[WAVES];
   uint32_t cursor;
+  unsigned long long bound;  // sum over the chunk of min(d+(v), d+(u))
+  unsigned long long sbase;  // staging: the item's reservation
 };
+
+constexpr unsigned long long kNoStage = ~0ull;
 
 // First wave: load the chunk's v ids and scan their out-degrees.
 template <class Table>
 __device__ __forceinline__ void load_chunk(Table &t, const uint32_t *dadj,
                                            const uint32_t *doff, uint32_t ub,
-                                           uint32_t j0, uint32_t jn) {
+                                           uint32_t j0, uint32_t jn, uint32_t du) {
   if (threadIdx.x < kWave) {
     uint32_t dv = 0;
     if (threadIdx.x < jn) {
@@ -80,9 +84,11 @@ __device__ __forceinline__ void load_chunk(Table &t, const uint32_t *dadj,
       if ((int)lane() >= off) incl += x;
     }
     if (threadIdx.x < jn) t.pre[threadIdx.x + 1] = incl;
+    const uint64_t b = wave_sum64(dv < du ? dv : du);
     if (threadIdx.x == 0) {
       t.pre[0] = 0;
       t.cursor = 0;
+      t.bound = b;
     }
   }
 }
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(THREADS) void k_list3(
     const uint32_t *__restrict__ doff, const uint32_t *__restrict__ dadj,
     uint32_t lds_cap, uint32_t *__restrict__ item_count,
     unsigned long long *__restrict__ cd, const uint64_t *__restrict__ item_base,
-    uint32_t *__restrict__ out) {
+    uint32_t *__restrict__ out, ListStage st) {
   extern __shared__ uint32_t dyn[];
   uint32_t *su = dyn;             // N+(u), sorted
   uint32_t *scnt = dyn + lds_cap; // clique-degree counters by position
@@ -138,8 +144,19 @@ __global__ __launch_bounds__(THREADS) void k_list3(
         su[i] = dadj[ub + i];
         scnt[i] = 0;
       }
-    load_chunk(t, dadj, doff, ub, j0, jn);
+    load_chunk(t, dadj, doff, ub, j0, jn, du);
     __syncthreads();
+    const bool stage = st.buf && !write;
+    if (stage) {
+      if (threadIdx.x == 0) {  // the item's worst case, reserved at once
+        const unsigned long long b = t.bound;
+        const unsigned long long at = b ? atomicAdd(st.cursor, b) : 0ull;
+        t.sbase = at + b <= st.cap ? at : kNoStage;
+        if (t.sbase == kNoStage) *st.over = 1;
+        st.item_tmp[it] = t.sbase;
+      }
+      __syncthreads();
+    }
     const uint32_t total = t.pre[jn];
     const uint32_t *nu = staged ? su : dadj + ub;
     const uint64_t obase = write ? item_base[it] : 0;
@@ -163,9 +180,19 @@ __global__ __launch_bounds__(THREADS) void k_list3(
       } else if (staged) {
         atomicAdd(&scnt[p], 1u);
         atomicAdd(&scnt[j0 + lo], 1u);
+        if (stage && t.sbase != kNoStage) {
+          const uint32_t slot = atomicAdd(&t.cursor, 1u);
+          *reinterpret_cast<Clique3 *>(st.buf + 3ull * (t.sbase + slot)) =
+              Clique3{{u, t.vid[lo], w}};
+        }
       } else {
         atomicAdd(&cd[w], 1ull);
         atomicAdd(&cd[t.vid[lo]], 1ull);
+        if (stage && t.sbase != kNoStage) {
+          const uint32_t slot = atomicAdd(&t.cursor, 1u);
+          *reinterpret_cast<Clique3 *>(st.buf + 3ull * (t.sbase + slot)) =
+              Clique3{{u, t.vid[lo], w}};
+        }
       }
     }
     if (!write) finish_count<WAVES>(t, found, it, u, staged, su, scnt, du, item_count, cd);
@@ -297,16 +324,17 @@ hipError_t launch_list(uint32_t k, uint64_t num_items, const uint2 *items,
                        const uint32_t *doff, const uint32_t *dadj,
                        uint32_t max_out_degree, uint32_t *item_count,
                        unsigned long long *cd, const uint64_t *item_base,
-                       uint32_t *out, hipStream_t s) {
+                       uint32_t *out, hipStream_t s, ListStage st) {
   const unsigned grid = (unsigned)(num_items < 16384 ? num_items : 16384);
   if (k == 3) {
     const uint32_t cap = max_out_degree < kListLdsCap ? max_out_degree : kListLdsCap;
     const size_t lds = 2ull * (cap ? cap : 1) * sizeof(uint32_t);
     hipLaunchKernelGGL((k_list3<THREADS, CHUNK>), dim3(grid), dim3(THREADS), lds, s,
                        num_items, items, doff, dadj, cap ? cap : 1, item_count,
-                       cd, item_base, out);
+                       cd, item_base, out, st);
     return hipGetLastError();
   }
+  if (st.buf) return hipErrorInvalidValue;  // staging is for triangles only
   if (k == 4) {
     const uint32_t cap = max_out_degree ? max_out_degree : 1;
     if (cap > kList4LdsCap) return hipErrorInvalidValue;
@@ -324,13 +352,14 @@ hipError_t list_cliques(uint32_t k, ListShape shape, uint64_t num_items,
                         const uint2 *items, const uint32_t *doff,
                         const uint32_t *dadj, uint32_t max_out_degree,
                         uint32_t *item_count, uint64_t *clique_degree,
-                        const uint64_t *item_base, uint32_t *out, hipStream_t s) {
+                        const uint64_t *item_base, uint32_t *out, hipStream_t s,
+                        ListStage st) {
   if (num_items == 0) return hipSuccess;
   auto *cd = reinterpret_cast<unsigned long long *>(clique_degree);
 #define CDS_LIST_SHAPE(TH, CH)                                                      \
   if (shape.threads == TH && shape.chunk == CH)                                     \
     return launch_list<TH, CH>(k, num_items, items, doff, dadj, max_out_degree,     \
-                               item_count, cd, item_base, out, s);
+                               item_count, cd, item_base, out, s, st);
   CDS_LIST_SHAPE(256, 64)
   CDS_LIST_SHAPE(256, 32)
   CDS_LIST_SHAPE(512, 64)
@@ -338,6 +367,34 @@ hipError_t list_cliques(uint32_t k, ListShape shape, uint64_t num_items,
   CDS_LIST_SHAPE(128, 32)
 #undef CDS_LIST_SHAPE
   return hipErrorInvalidValue;
+}
+
+namespace {
+// One wave per item: its staged triangles, contiguous, to their place.
+__global__ __launch_bounds__(256) void k_list_unstage(uint64_t num_items,
+                                                      const uint64_t *__restrict__ item_tmp,
+                                                      const uint32_t *__restrict__ item_count,
+                                                      const uint64_t *__restrict__ item_base,
+                                                      const uint32_t *__restrict__ stage,
+                                                      uint32_t *__restrict__ out) {
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       it < num_items; it += waves) {
+    const uint64_t words = 3ull * item_count[it];
+    const uint32_t *src = stage + 3ull * item_tmp[it];
+    uint32_t *dst = out + 3ull * item_base[it];
+    for (uint64_t i = lane(); i < words; i += kWave) dst[i] = src[i];
+  }
+}
+}  // namespace
+
+hipError_t list_unstage(uint64_t num_items, const uint64_t *item_tmp,
+                        const uint32_t *item_count, const uint64_t *item_base,
+                        const uint32_t *stage, uint32_t *out, hipStream_t s) {
+  if (num_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_list_unstage, dim3(grid_for(num_items * kWave, 256, 16384)), dim3(256),
+                     0, s, num_items, item_tmp, item_count, item_base, stage, out);
+  return hipGetLastError();
 }
 
 hipError_t list_edges(uint32_t n, const uint32_t *doff, const uint32_t *dadj,
