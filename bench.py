@@ -601,7 +601,17 @@ def main():
     # networks of 2^32 clique-member slots or more run the wide flow kernel
     wide = r["core_cliques"] * w["k"] >= 2 ** 32
     kernels = {}
-    if w["k"] == 3:
+    if w["k"] == 3 and r["list_staged"]:
+        # one listing launch that also stages every triangle (12 bytes
+        # each), then a copy of the staged runs into place (read + write)
+        kernels["k_list3"] = {
+            "launches_per_solve": 1, "total_ms_per_solve": cnt_ms,
+            "algorithmic_bytes_per_solve":
+                list3_bytes(g.n, r["dag_arcs"], r["list_work"]) + 4 * w["k"] * T}
+        kernels["k_list_unstage"] = {
+            "launches_per_solve": 1, "total_ms_per_solve": write_ms,
+            "algorithmic_bytes_per_solve": 2 * 4 * w["k"] * T}
+    elif w["k"] == 3:
         # two launches per solve: the count pass, then the write pass that
         # also stores every clique (12 bytes per triangle more)
         kernels["k_list3"] = {

@@ -111,7 +111,8 @@ class CResult(ctypes.Structure):
         ("prune_tau", ctypes.c_uint64), ("prune_vertices", ctypes.c_uint32),
         ("prune_passes", ctypes.c_uint32), ("dense_peel", ctypes.c_uint32),
         ("flow_implicit", ctypes.c_uint32), ("flow_graph_launches", ctypes.c_uint32),
-        ("replay_bits_phases", ctypes.c_uint32), ("refine_bits_steps", ctypes.c_uint32)]
+        ("replay_bits_phases", ctypes.c_uint32), ("refine_bits_steps", ctypes.c_uint32),
+        ("list_staged", ctypes.c_uint32)]
 
 
 class COpts(ctypes.Structure):

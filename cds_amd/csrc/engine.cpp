@@ -1439,6 +1439,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   R->list_items = L.items_n;
   R->t_list_count_ms = L.t_count;
   R->t_list_write_ms = L.t_write;
+  R->list_staged = L.staged ? 1 : 0;
   std::vector<uint64_t> cd(n);
   CHK(readback(e, cd.data(), L.cd.p, n));
   uint64_t sum_cd = 0, max_cd = 0;

@@ -205,6 +205,9 @@ typedef struct {
   /* Coarse peel steps (of peel_refined_steps) re-peeled exactly on
    * adjacency bitsets in one workgroup, their levels' replays included. */
   uint32_t refine_bits_steps;
+  /* 1: the triangle count pass also staged the triangles (t_list_write_ms
+   * then times moving them into place); 0: they were listed twice. */
+  uint32_t list_staged;
 } cds_result;
 
 typedef struct {
