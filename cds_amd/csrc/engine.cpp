@@ -2029,7 +2029,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     const size_t na = g->host_off[n];
     HIPCHK(mark.alloc(na));
     HIPCHK(mark.zero());
-    if (core_n <= kDenseCoreMax && k <= 3) {
+    if (core_n <= kDenseCoreMax && k <= 4) {
       DBuf<unsigned long long> cbits(&A);
       HIPCHK(cbits.alloc((size_t)core_n * ((core_n + 63) / 64)));
       HIPCHK(prune_mark_bits(k, cv.p, core_n, cidx.p, g->off, g->adj, cbits.p, mark.p, st));

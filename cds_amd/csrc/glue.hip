@@ -326,10 +326,11 @@ __global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
   }
 }
 
-// k <= 3 and a small core: the core's adjacency as bitset rows (row i:
-// the core neighbours of cv[i], by their cidx numbers, W words), then a
-// directed core edge (a, b) lies in a core clique iff k = 2, or (k = 3) a
-// and b have a common core neighbour: rows a and b intersect.
+// A small core: the core's adjacency as bitset rows (row i: the core
+// neighbours of cv[i], by their cidx numbers, W words), then a directed
+// core edge (a, b) lies in a core clique iff k = 2; (k = 3) a and b have a
+// common core neighbour: rows a and b intersect; (k = 4) their common core
+// neighbourhood X holds an edge: some c in X has a neighbour in X.
 __global__ __launch_bounds__(256) void k_core_rows(const uint32_t *__restrict__ cv, uint32_t cn,
                                                    uint32_t W, const uint32_t *__restrict__ off,
                                                    const uint32_t *__restrict__ adj,
@@ -361,7 +362,19 @@ __global__ __launch_bounds__(256) void k_prune_mark_bits(
       if (ib >= cn) continue;
       bool in = k == 2;
       const unsigned long long *rb = bits + (size_t)ib * W;
-      for (uint32_t w = 0; w < W && !in; ++w) in = (ra[w] & rb[w]) != 0;
+      if (k == 3) {
+        for (uint32_t w = 0; w < W && !in; ++w) in = (ra[w] & rb[w]) != 0;
+      } else if (k == 4) {
+        for (uint32_t w = 0; w < W && !in; ++w) {
+          unsigned long long x = ra[w] & rb[w];
+          while (x && !in) {
+            const uint32_t c = w * 64 + (uint32_t)(__ffsll((long long)x) - 1);
+            x &= x - 1;
+            const unsigned long long *rc = bits + (size_t)c * W;
+            for (uint32_t w2 = 0; w2 < W && !in; ++w2) in = (rc[w2] & ra[w2] & rb[w2]) != 0;
+          }
+        }
+      }
       if (in) mark[p] = 1;
     }
   }
@@ -673,7 +686,7 @@ hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
 hipError_t prune_mark_bits(uint32_t k, const uint32_t *cv, uint32_t cn, const uint32_t *cidx,
                            const uint32_t *off, const uint32_t *adj,
                            unsigned long long *bits, uint8_t *mark, hipStream_t s) {
-  if (k < 2 || k > 3) return hipErrorInvalidValue;
+  if (k < 2 || k > 4) return hipErrorInvalidValue;
   const uint32_t W = (cn + 63) / 64;
   hipError_t e = hipMemsetAsync(bits, 0, (size_t)cn * W * 8, s);
   if (e != hipSuccess) return e;

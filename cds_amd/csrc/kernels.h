@@ -401,7 +401,7 @@ hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
                             const uint32_t *cc, const uint32_t *cv, uint32_t cn,
                             const uint32_t *cidx, const uint32_t *off, const uint32_t *adj,
                             uint32_t *cmat, uint8_t *mark, hipStream_t s);
-// The same marks for k = 2 or 3 from the core's adjacency alone: bits is
+// The same marks for k = 2, 3 or 4 from the core's adjacency alone: bits is
 // cn * ceil(cn / 64) words of scratch for the core's bitset rows.
 hipError_t prune_mark_bits(uint32_t k, const uint32_t *cv, uint32_t cn, const uint32_t *cidx,
                            const uint32_t *off, const uint32_t *adj,
