@@ -724,7 +724,8 @@ def main():
                  "split_ms": r["peel_split_ms"],
                  # graphs too large to materialise: solved on the prune_tau-core
                  "prune_tau": r["prune_tau"], "prune_vertices": r["prune_vertices"],
-                 "prune_passes": r["prune_passes"], "dense_peel": r["dense_peel"]},
+                 "prune_passes": r["prune_passes"], "prune_unlinks": r["prune_unlinks"],
+                 "dense_peel": r["dense_peel"]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -112,7 +112,7 @@ class CResult(ctypes.Structure):
         ("prune_passes", ctypes.c_uint32), ("dense_peel", ctypes.c_uint32),
         ("flow_implicit", ctypes.c_uint32), ("flow_graph_launches", ctypes.c_uint32),
         ("replay_bits_phases", ctypes.c_uint32), ("refine_bits_steps", ctypes.c_uint32),
-        ("list_staged", ctypes.c_uint32)]
+        ("list_staged", ctypes.c_uint32), ("prune_unlinks", ctypes.c_uint32)]
 
 
 class COpts(ctypes.Structure):

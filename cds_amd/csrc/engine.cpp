@@ -2480,71 +2480,129 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   hipStream_t st = e->stream;
   const uint32_t n = g->n, k = p->k;
   const uint64_t T = L0.T;
-  DBuf<uint8_t> keep(&A);
-  DBuf<uint32_t> newid(&A), cnt(&A), sel(&A), dropped(&A);
+  DBuf<uint8_t> keep(&A), state(&A);
+  DBuf<uint32_t> newid(&A), cnt(&A), sel(&A), dropped(&A), dlist(&A), ddeg(&A), dpoff(&A);
+  DBuf<unsigned long long> dstats(&A);
   HIPCHK(keep.alloc(n));
   HIPCHK(hipMemsetAsync(keep.p, 1, n, st));
   HIPCHK(newid.alloc((size_t)n + 1));
   HIPCHK(cnt.alloc(n));
   HIPCHK(sel.alloc(n));
   HIPCHK(dropped.alloc(1));
+  HIPCHK(state.alloc(n));
+  HIPCHK(dlist.alloc(n));
+  HIPCHK(ddeg.alloc(n));
+  HIPCHK(dpoff.alloc((size_t)n + 1));
+  HIPCHK(dstats.alloc(3));
   auto gs = std::make_unique<OwnedDgraph>(&A);
   std::unique_ptr<Listing> Ls;
   uint64_t tau = (T + n - 1) / n, Ts = T;
-  uint32_t ns = n, passes = 0;
+  uint32_t ns = n, alive = n, passes = 0, unlinks = 0;
   const uint64_t *cds = L0.cd.p;
+  // Drops in place (CDS_PRUNE_UNLINK=1 whenever possible, 0 never): the
+  // cliques through the dropped vertices are taken from the counts of the
+  // rest instead of counting the rebuilt subgraph again, when at most
+  // 1/16 of the cliques of S pass through them.
+  int unlink_mode = -1;
+  if (const char *f = getenv("CDS_PRUNE_UNLINK")) unlink_mode = atoi(f) != 0 ? 1 : 0;
+  // G[S] from keep[]
+  auto build_subgraph = [&]() -> cds_status {
+    HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, cnt.p, nullptr, nullptr,
+                            nullptr, false, A));
+    CHK(readback(e, &ns, newid.p + n, 1));
+    if (ns == 0) return fail(CDS_ERR_INTERNAL, "core pruning left no vertex (tau %llu)",
+                             (unsigned long long)tau);
+    auto ng = std::make_unique<OwnedDgraph>(&A);
+    HIPCHK(ng->off.alloc((size_t)ns + 1));
+    HIPCHK(scan_u32_32(cnt.p, ng->off.p, ns, A));
+    ng->g.host_off.resize((size_t)ns + 1);
+    CHK(readback(e, ng->g.host_off.data(), ng->off.p, (size_t)ns + 1));
+    const uint64_t arcs = ng->g.host_off[ns];
+    HIPCHK(ng->adj.alloc(arcs));
+    HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, nullptr, ng->off.p,
+                            ng->adj.p, sel.p, true, A));
+    ng->g.n = ns;
+    ng->g.m = arcs / 2;
+    ng->g.off = ng->off.p;
+    ng->g.adj = ng->adj.p;
+    uint32_t md = 0;
+    for (uint32_t v = 0; v < ns; ++v)
+      md = std::max(md, ng->g.host_off[v + 1] - ng->g.host_off[v]);
+    ng->g.max_degree = md;
+    gs = std::move(ng);
+    alive = ns;
+    HIPCHK(hipMemsetAsync(state.p, 1, ns, st));
+    return CDS_OK;
+  };
   bool first = true;
   Timer tm(st);
   for (bool rebuild = true;;) {
     if (rebuild) {
-      // G[S] from keep[]
-      HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, cnt.p, nullptr, nullptr,
-                              nullptr, false, A));
-      CHK(readback(e, &ns, newid.p + n, 1));
-      if (ns == 0) return fail(CDS_ERR_INTERNAL, "core pruning left no vertex (tau %llu)",
-                               (unsigned long long)tau);
-      auto ng = std::make_unique<OwnedDgraph>(&A);
-      HIPCHK(ng->off.alloc((size_t)ns + 1));
-      HIPCHK(scan_u32_32(cnt.p, ng->off.p, ns, A));
-      ng->g.host_off.resize((size_t)ns + 1);
-      CHK(readback(e, ng->g.host_off.data(), ng->off.p, (size_t)ns + 1));
-      const uint64_t arcs = ng->g.host_off[ns];
-      HIPCHK(ng->adj.alloc(arcs));
-      HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, nullptr, ng->off.p,
-                              ng->adj.p, sel.p, true, A));
-      ng->g.n = ns;
-      ng->g.m = arcs / 2;
-      ng->g.off = ng->off.p;
-      ng->g.adj = ng->adj.p;
-      uint32_t md = 0;
-      for (uint32_t v = 0; v < ns; ++v)
-        md = std::max(md, ng->g.host_off[v + 1] - ng->g.host_off[v]);
-      ng->g.max_degree = md;
-      gs = std::move(ng);
+      CHK(build_subgraph());
       if (!first) {
         // clique counts inside S
+        const auto tp0 = std::chrono::steady_clock::now();
         Ls = std::make_unique<Listing>(&A);
         CHK(run_dag(e, &gs->g, *Ls));
         CHK(run_listing(e, &gs->g, k, false, *Ls, &tm));
         Ts = Ls->T;
         cds = Ls->cd.p;
         ++passes;
+        if (peel_trace_on())
+          fprintf(stderr, "[prune] pass %u tau %llu: %u vertices %llu arcs %llu cliques, "
+                  "count %.2f ms\n", passes, (unsigned long long)tau, ns,
+                  (unsigned long long)gs->g.host_off[ns], (unsigned long long)Ts,
+                  std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - tp0).count());
       }
       first = false;
       rebuild = false;
     }
-    HIPCHK(drop_below(ns, cds, tau, sel.p, keep.p, dropped.p, st));
+    HIPCHK(hipMemsetAsync(dropped.p, 0, sizeof(uint32_t), st));
+    HIPCHK(hipMemsetAsync(dstats.p, 0, 3 * sizeof(unsigned long long), st));
+    HIPCHK(prune_select(ns, cds, tau, gs->g.off, state.p, dlist.p, dropped.p, dstats.p, st));
     uint32_t nd = 0;
+    unsigned long long stats[2] = {0, 0};
     CHK(readback(e, &nd, dropped.p, 1));
     if (nd) {
+      CHK(readback(e, stats, dstats.p, 2));
+      const bool in_place = Ls && (k == 3 || k == 4) && unlink_mode != 0 &&
+                            stats[1] <= kUnlinkMaxDegree &&
+                            (unlink_mode == 1 || stats[0] <= Ts / 16);
+      if (in_place) {
+        const auto tp0 = std::chrono::steady_clock::now();
+        HIPCHK(prune_unlink(k, nd, dlist.p, (uint32_t)stats[1], gs->g.off, gs->g.adj, state.p,
+                            reinterpret_cast<unsigned long long *>(Ls->cd.p), dstats.p + 2,
+                            ddeg.p, dpoff.p, A));
+        HIPCHK(prune_finish(nd, dlist.p, sel.p, state.p, keep.p, st));
+        unsigned long long removed = 0;
+        CHK(readback(e, &removed, dstats.p + 2, 1));
+        if (removed > Ts)
+          return fail(CDS_ERR_INTERNAL, "pruning removed %llu of %llu cliques", removed,
+                      (unsigned long long)Ts);
+        Ts -= removed;
+        alive -= nd;
+        ++unlinks;
+        if (peel_trace_on())
+          fprintf(stderr, "[prune] in place tau %llu: %u dropped (max degree %llu), %llu "
+                  "cliques removed, %u vertices %llu cliques left, %.2f ms\n",
+                  (unsigned long long)tau, nd, stats[1], removed, alive,
+                  (unsigned long long)Ts,
+                  std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - tp0).count());
+        continue;
+      }
+      HIPCHK(prune_finish(nd, dlist.p, sel.p, state.p, keep.p, st));
       rebuild = true;
       continue;
     }
     // S is the tau-clique-core; its density is a peel row
-    const uint64_t t2 = (Ts + ns - 1) / ns;
+    const uint64_t t2 = (Ts + alive - 1) / alive;
     if (t2 <= tau) break;
     tau = t2;
   }
+  // S lost vertices in place since G[S] was built: build it again
+  if (alive != ns) CHK(build_subgraph());
   Ls.reset();
   // the solve on G[S]: persistent peel if its cliques fit the incidence,
   // else the bitset peel of a dense core
@@ -2559,6 +2617,9 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   cds_result RS;
   CHK(solve_impl(e, &gs->g, p, &RS, core_only, dense ? kSolveDenseCore : kSolveCore));
   std::unique_ptr<cds_result, void (*)(cds_result *)> hold(&RS, cds_result_free);
+  if (RS.total_cliques != Ts)
+    return fail(CDS_ERR_INTERNAL, "the pruned core holds %llu cliques, %llu were kept",
+                (unsigned long long)RS.total_cliques, (unsigned long long)Ts);
   // the answer in the input's ids
   std::vector<uint32_t> hsel(ns);
   CHK(readback(e, hsel.data(), sel.p, ns));
@@ -2590,6 +2651,7 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   R->prune_tau = tau;
   R->prune_vertices = ns;
   R->prune_passes = passes;
+  R->prune_unlinks = unlinks;
   R->dense_peel = dense ? 1 : 0;
   R->times.total = std::chrono::duration<double, std::milli>(
                        std::chrono::steady_clock::now() - wall0)

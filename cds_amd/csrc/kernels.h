@@ -89,9 +89,28 @@ hipError_t induced_subgraph(uint32_t n, const uint32_t *off, const uint32_t *adj
                             const uint8_t *keep, uint32_t *newid, uint32_t *cnt,
                             uint32_t *off2, uint32_t *adj2, uint32_t *sel, bool fill,
                             DeviceArena &a);
-// keep[sel[i]] = 0 for i < ns with cd[i] < tau; *dropped = how many.
-hipError_t drop_below(uint32_t ns, const uint64_t *cd, uint64_t tau, const uint32_t *sel,
-                      uint8_t *keep, uint32_t *dropped, hipStream_t s);
+// In-place drops of the pruned path, on the current induced subgraph with
+// a state byte per vertex (1 in S, 2 dropped this round, 0 dropped
+// before).  prune_select moves the vertices in S with cd < tau to state 2
+// and lists them in D (*nd, cleared by the caller), stats[0] += their
+// clique degrees, stats[1] = max(stats[1], their degree).
+hipError_t prune_select(uint32_t ns, const uint64_t *cd, uint64_t tau, const uint32_t *off,
+                        uint8_t *state, uint32_t *D, uint32_t *nd,
+                        unsigned long long *stats, hipStream_t s);
+// Largest degree of a dropped vertex prune_unlink takes (its neighbour
+// lists are staged in LDS, 4 per workgroup).
+constexpr uint32_t kUnlinkMaxDegree = 2048;
+// Removes the k-cliques (k = 3, 4) that contain a vertex of D from the
+// clique degrees of their members in S (each clique once, from its
+// smallest dropped member); *removed += how many.  deg (nd words) and
+// poff (nd + 1) are workspace.
+hipError_t prune_unlink(uint32_t k, uint32_t nd, const uint32_t *D, uint32_t max_deg,
+                        const uint32_t *off, const uint32_t *adj, const uint8_t *state,
+                        unsigned long long *cd, unsigned long long *removed, uint32_t *deg,
+                        uint32_t *poff, DeviceArena &a);
+// state[D[j]] = 0 and keep[sel[D[j]]] = 0.
+hipError_t prune_finish(uint32_t nd, const uint32_t *D, const uint32_t *sel, uint8_t *state,
+                        uint8_t *keep, hipStream_t s);
 // The sequential peel of a graph of ns <= kDenseMaxVertices vertices with
 // clique degrees cd: rv[i], rval[i] = the vertex removed i-th and its clique
 // count then.  bits: ns * ceil(ns / 64) words; alive: ceil(ns / 64); cur: ns.

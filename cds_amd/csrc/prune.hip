@@ -2,11 +2,14 @@
 // dense core on adjacency bitsets.
 //
 // A graph with too many k-cliques to materialise (k T slots beyond 32 bits
-// or beyond the device) is first cut to its tau-clique-core S: count
-// passes over induced subgraphs drop every vertex with fewer than tau
-// cliques until none is left to drop, and tau rises to the ceiling of the
-// density of S while that grows (each S is a boundary state of the peel,
-// so its density is a peel row and tau never passes k').  The rows of the
+// or beyond the device) is first cut to its tau-clique-core S: every vertex
+// with fewer than tau cliques in S is dropped until none is left to drop,
+// and tau rises to the ceiling of the density of S while that grows (each
+// S is a boundary state of the peel, so its density is a peel row and tau
+// never passes k').  After a large drop the clique degrees come from a
+// count pass over the rebuilt induced subgraph; after a small one (k = 3,
+// 4) the cliques through the dropped vertices are enumerated from their
+// neighbourhoods and taken from the degrees of the rest, in place.  The rows of the
 // peel above S (vertices with core numbers below tau) have densities at
 // most max(rho(S), tau - 1): each removed one of them with at most tau - 1
 // cliques.  So k' = ceil(max row density) is decided inside S.
@@ -75,19 +78,134 @@ __global__ void k_induced_fill(uint32_t n, const uint32_t *__restrict__ off,
   }
 }
 
-// keep[sel[i]] = 0 for the induced vertices i with fewer than tau cliques.
-__global__ void k_drop_below(uint32_t ns, const uint64_t *__restrict__ cd, uint64_t tau,
-                             const uint32_t *__restrict__ sel, uint8_t *__restrict__ keep,
-                             uint32_t *__restrict__ dropped) {
-  uint32_t d = 0;
+// Drops in place (state per vertex of the current subgraph: 1 in S, 2
+// dropped this round, 0 dropped before): vertices in S with fewer than tau
+// cliques go to state 2 and to the list D, with the sum of their clique
+// degrees (stats[0]) and their largest degree (stats[1]).
+__global__ void k_prune_select(uint32_t ns, const uint64_t *__restrict__ cd, uint64_t tau,
+                               const uint32_t *__restrict__ off, uint8_t *__restrict__ state,
+                               uint32_t *__restrict__ D, uint32_t *__restrict__ nd,
+                               unsigned long long *__restrict__ stats) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns;
-       i += gridDim.x * blockDim.x)
-    if (cd[i] < tau) {
-      keep[sel[i]] = 0;
-      ++d;
+       i += gridDim.x * blockDim.x) {
+    if (state[i] != 1 || cd[i] >= tau) continue;
+    state[i] = 2;
+    D[atomicAdd(nd, 1u)] = i;
+    atomicAdd(&stats[0], (unsigned long long)cd[i]);
+    atomicMax(&stats[1], (unsigned long long)(off[i + 1] - off[i]));
+  }
+}
+
+__global__ void k_prune_drop_degrees(uint32_t nd, const uint32_t *__restrict__ D,
+                                     const uint32_t *__restrict__ off,
+                                     uint32_t *__restrict__ deg) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nd;
+       j += gridDim.x * blockDim.x)
+    deg[j] = off[D[j] + 1] - off[D[j]];
+}
+
+// x may share a clique with the dropped vertex d that is counted from d:
+// x is in S, or dropped this round with a larger id (each clique with
+// dropped members is removed from its smallest one).
+__device__ __forceinline__ bool unlink_eligible(uint8_t sx, uint32_t x, uint32_t d) {
+  return sx == 1 || (sx == 2 && x > d);
+}
+
+// The K-cliques that contain a dropped vertex, removed from the clique
+// degrees of their members in S.  One wave per (dropped vertex d, its
+// neighbour a) pair with a eligible: the eligible neighbours b > a of d
+// adjacent to a go to an LDS list (one region of `cap` words per wave,
+// cap >= every dropped degree); K = 3: each b closes a triangle {d, a, b};
+// K = 4: each pair b < c of the list with c in N(b) closes {d, a, b, c}.
+// removed[0] += the cliques removed.
+template <uint32_t K>
+__global__ __launch_bounds__(256) void k_prune_unlink(
+    uint32_t nd, const uint32_t *__restrict__ D, const uint32_t *__restrict__ poff,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+    const uint8_t *__restrict__ state, unsigned long long *__restrict__ cd,
+    unsigned long long *__restrict__ removed, uint32_t cap) {
+  extern __shared__ uint32_t lds_list[];
+  uint32_t *list = lds_list + (threadIdx.x / kWave) * cap;
+  const uint32_t total = poff[nd];
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  unsigned long long rem = 0;
+  for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < total; q += waves) {
+    // the dropped vertex owning pair q: the last j with poff[j] <= q
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (poff[mid] <= q)
+        lo = mid;
+      else
+        hi = mid;
     }
-  d = wave_sum(d);
-  if (lane() == 0 && d) atomicAdd(dropped, d);
+    const uint32_t d = D[lo];
+    const uint32_t pa = off[d] + (q - poff[lo]), pend = off[d + 1];
+    const uint32_t a = adj[pa];
+    const uint8_t sa = state[a];
+    if (!unlink_eligible(sa, a, d)) continue;
+    const uint32_t a0 = off[a], alen = off[a + 1] - a0;
+    uint32_t nl = 0;
+    for (uint32_t p0 = pa + 1; p0 < pend; p0 += kWave) {
+      const uint32_t p = p0 + lane();
+      bool in = false;
+      uint32_t b = 0;
+      if (p < pend) {
+        b = adj[p];
+        in = unlink_eligible(state[b], b, d) && find_sorted(adj + a0, (int)alen, b) >= 0;
+      }
+      unsigned cnt = 0;
+      const unsigned slot = wave_slot(in, &cnt);
+      if (in) list[nl + slot] = b;
+      nl += cnt;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long ca = 0;
+    if (K == 3) {
+      ca = nl;
+      for (uint32_t i = lane(); i < nl; i += kWave) {
+        const uint32_t b = list[i];
+        if (state[b] == 1) atomicAdd(&cd[b], ~0ull);
+      }
+    } else {
+      for (uint32_t i = 0; i + 1 < nl; ++i) {
+        const uint32_t b = list[i];
+        const uint32_t b0 = off[b], blen = off[b + 1] - b0;
+        uint32_t hits = 0;
+        for (uint32_t j = i + 1 + lane(); j < nl; j += kWave) {
+          const uint32_t c = list[j];
+          if (find_sorted(adj + b0, (int)blen, c) >= 0) {
+            ++hits;
+            if (state[c] == 1) atomicAdd(&cd[c], ~0ull);
+          }
+        }
+        hits = wave_sum(hits);
+        if (lane() == 0 && hits && state[b] == 1) atomicAdd(&cd[b], 0ull - hits);
+        ca += hits;
+      }
+    }
+    if (lane() == 0 && ca) {
+      if (sa == 1) atomicAdd(&cd[a], 0ull - ca);
+      rem += ca;
+    }
+    // the list is rewritten by the next pair of this wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane() == 0 && rem) atomicAdd(removed, rem);
+}
+
+__global__ void k_prune_finish(uint32_t nd, const uint32_t *__restrict__ D,
+                               const uint32_t *__restrict__ sel, uint8_t *__restrict__ state,
+                               uint8_t *__restrict__ keep) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nd;
+       j += gridDim.x * blockDim.x) {
+    state[D[j]] = 0;
+    keep[sel[D[j]]] = 0;
+  }
 }
 
 // Adjacency bitsets: row v holds W 64-bit words.
@@ -233,12 +351,42 @@ hipError_t induced_subgraph(uint32_t n, const uint32_t *off, const uint32_t *adj
   return hipGetLastError();
 }
 
-hipError_t drop_below(uint32_t ns, const uint64_t *cd, uint64_t tau, const uint32_t *sel,
-                      uint8_t *keep, uint32_t *dropped, hipStream_t s) {
-  hipError_t e = hipMemsetAsync(dropped, 0, sizeof(uint32_t), s);
+hipError_t prune_select(uint32_t ns, const uint64_t *cd, uint64_t tau, const uint32_t *off,
+                        uint8_t *state, uint32_t *D, uint32_t *nd,
+                        unsigned long long *stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_prune_select, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, ns, cd,
+                     tau, off, state, D, nd, stats);
+  return hipGetLastError();
+}
+
+hipError_t prune_unlink(uint32_t k, uint32_t nd, const uint32_t *D, uint32_t max_deg,
+                        const uint32_t *off, const uint32_t *adj, const uint8_t *state,
+                        unsigned long long *cd, unsigned long long *removed, uint32_t *deg,
+                        uint32_t *poff, DeviceArena &a) {
+  if ((k != 3 && k != 4) || max_deg > kUnlinkMaxDegree) return hipErrorInvalidValue;
+  if (nd == 0) return hipSuccess;
+  hipStream_t s = a.stream();
+  hipLaunchKernelGGL(k_prune_drop_degrees, dim3(grid_for(nd, 256, 1024)), dim3(256), 0, s, nd,
+                     D, off, deg);
+  hipError_t e = scan_u32_32(deg, poff, nd, a);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_drop_below, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, ns, cd,
-                     tau, sel, keep, dropped);
+  const uint32_t cap = max_deg < 1 ? 1 : max_deg;
+  const size_t lds = (size_t)4 * cap * sizeof(uint32_t);
+  const unsigned grid = grid_for((size_t)nd * cap * kWave, 256, 8192);
+  if (k == 3)
+    hipLaunchKernelGGL(k_prune_unlink<3>, dim3(grid), dim3(256), lds, s, nd, D, poff, off,
+                       adj, state, cd, removed, cap);
+  else
+    hipLaunchKernelGGL(k_prune_unlink<4>, dim3(grid), dim3(256), lds, s, nd, D, poff, off,
+                       adj, state, cd, removed, cap);
+  return hipGetLastError();
+}
+
+hipError_t prune_finish(uint32_t nd, const uint32_t *D, const uint32_t *sel, uint8_t *state,
+                        uint8_t *keep, hipStream_t s) {
+  if (nd == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prune_finish, dim3(grid_for(nd, 256, 1024)), dim3(256), 0, s, nd, D,
+                     sel, state, keep);
   return hipGetLastError();
 }
 

@@ -155,6 +155,7 @@ std::string render(const cds_result *r) {
   j.u64("replay_bits_phases", r->replay_bits_phases);
   j.u64("refine_bits_steps", r->refine_bits_steps);
   j.u64("list_staged", r->list_staged);
+  j.u64("prune_unlinks", r->prune_unlinks);
   // steps whose exact decision the reference's float Edmonds-Karp may take
   // the other way (cds.h: trace_fragile)
   j.u64("fragile_decisions", r->fragile_decisions);

@@ -208,6 +208,10 @@ typedef struct {
   /* 1: the triangle count pass also staged the triangles (t_list_write_ms
    * then times moving them into place); 0: they were listed twice. */
   uint32_t list_staged;
+  /* Rounds of the pruned path that dropped vertices in place, removing the
+   * cliques through them from the counts of the rest, instead of counting
+   * a rebuilt subgraph again (k = 3, 4; few cliques through the drops). */
+  uint32_t prune_unlinks;
 } cds_result;
 
 typedef struct {

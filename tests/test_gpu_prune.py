@@ -27,16 +27,19 @@ SCALE = sorted(glob.glob(os.path.join(HERE, "golden", "scale_*.json")))
 NONE = 0xFFFFFFFF
 
 
-def _solve(engine, dg, k, dense, vertex_arrays=True):
-    saved = {v: os.environ.get(v) for v in ("CDS_CORE_PRUNE", "CDS_DENSE_PEEL")}
+def _solve(engine, dg, k, dense, vertex_arrays=True, unlink=None):
+    saved = {v: os.environ.get(v) for v in ("CDS_CORE_PRUNE", "CDS_DENSE_PEEL",
+                                            "CDS_PRUNE_UNLINK")}
     os.environ["CDS_CORE_PRUNE"] = "1"
     os.environ["CDS_DENSE_PEEL"] = "1" if dense else "0"
+    if unlink is not None:
+        os.environ["CDS_PRUNE_UNLINK"] = unlink
     try:
         return engine.solve(dg, k, vertex_arrays=vertex_arrays)
     finally:
         for v, old in saved.items():
             if old is None:
-                del os.environ[v]
+                os.environ.pop(v, None)
             else:
                 os.environ[v] = old
 
@@ -76,6 +79,52 @@ def test_pruned_path_matches_oracle(engine, kind, seed, k, dense):
     dg.free()
     assert got["dense_peel"] == int(dense)
     _check(got, oracle_solve(g, k), label)
+
+
+@pytest.mark.parametrize("kind,seed,k", GRAPHS)
+def test_pruned_path_in_place_drops(engine, kind, seed, k):
+    """Every drop after the first count pass done in place (the cliques
+    through the dropped vertices enumerated from their neighbourhoods and
+    taken from the counts of the rest): the same answer as the oracle, and
+    the solve's own count of G[S] equals the kept count (checked inside)."""
+    if kind == "chung_lu":
+        rng = np.random.default_rng(seed + 300)
+        n = int(rng.integers(200, 3000))
+        g = cds_amd.Graph.chung_lu(n, float(rng.uniform(2.1, 2.8)), float(rng.uniform(6, 14)),
+                                   int(rng.integers(10, 40)), 0.85, seed=seed + 300)
+    else:
+        g = cds_amd.Graph.rmat(10, 16, seed=seed + 70)
+    label = f"pruned in place {kind} seed={seed} k={k}"
+    _announce(label)
+    dg = engine.upload(g)
+    got = _solve(engine, dg, k, False, unlink="1")
+    dg.free()
+    if got["max_degree"] <= 2048:
+        assert got["prune_passes"] <= 1, label
+    _check(got, oracle_solve(g, k), label)
+
+
+@pytest.mark.parametrize("scale,ef,k,seed", [(14, 16, 3, 81), (14, 16, 4, 82), (15, 12, 4, 83)])
+def test_in_place_drops_match_count_passes(engine, scale, ef, k, seed):
+    """R-MAT graphs beyond the oracle: drops in place (forced, default) and
+    a count pass after every drop give the same pruned core and answer."""
+    g = cds_amd.Graph.rmat(scale, ef, seed=seed)
+    dg = engine.upload(g)
+    _announce(f"in-place drops vs count passes R-MAT scale {scale} ef {ef} k={k}")
+    base = _solve(engine, dg, k, False, vertex_arrays=False, unlink="0")
+    assert base["prune_unlinks"] == 0
+    for mode in ("1", None):
+        got = _solve(engine, dg, k, False, vertex_arrays=False, unlink=mode)
+        for f in ("prune_tau", "prune_vertices", "kmax", "set_size", "total_cliques",
+                  "k_prime", "core_size", "core_edges", "core_cliques", "deleted_cliques",
+                  "pruned_edges", "num_components", "size", "set_cliques"):
+            assert got[f] == base[f], (mode, f, got[f], base[f])
+        assert got["prune_passes"] + got["prune_unlinks"] >= base["prune_passes"]
+        if mode == "1" and base["prune_passes"] > 1 and got["max_degree"] <= 2048:
+            assert got["prune_unlinks"] > 0
+        assert np.float32(got["density"]) == np.float32(base["density"])
+        assert got["vertices"] == base["vertices"]
+    dg.free()
 
 
 @pytest.mark.parametrize("dense", [False, True], ids=["persistent", "bitset"])
