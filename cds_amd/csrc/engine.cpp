@@ -1311,8 +1311,8 @@ namespace {
 // Peel stage of a small dense graph (n <= kDenseMaxVertices): every row of
 // the oracle's sequential peel (cpuCode/c++/src/helpers.cpp:16-150) from
 // the bitset kernels of prune.hip, so k' is exact without replays.  Fills
-// the exact level table, its summary (rho and k' over all rows), core[]
-// and the kill level of every listed clique.
+// the exact level table, its summary (rho and k' over all rows) and core[];
+// kill is allocated for T words, to be derived from core[] when read.
 cds_status dense_peel_stage(cds_engine *e, const cds_dgraph *g, const Listing &L, uint32_t k,
                             cds_result *R, DBuf<uint32_t> *core, DBuf<uint32_t> *kill,
                             std::vector<Level> *levels, PeelSummary *PS) {
@@ -1371,9 +1371,6 @@ cds_status dense_peel_stage(cds_engine *e, const cds_dgraph *g, const Listing &L
   HIPCHK(core->alloc(n));
   CHK(upload(e, core->p, hcore.data(), n));
   HIPCHK(kill->alloc(T));
-  // every clique alive (0xFFFFFFFF), then its level from its members' cores
-  HIPCHK(hipMemsetAsync(kill->p, 0xFF, T * sizeof(uint32_t), st));
-  HIPCHK(peel_derive_kill(T, k, L.cliques.p, core->p, kill->p, st));
   HIPCHK(hipStreamSynchronize(st));  // hcore is read by the copy above
   R->peel_levels = n;
   R->peel_launches = 0;
@@ -1472,6 +1469,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   if (dense) {
     // a dense core: the exact sequential peel on adjacency bitsets
     CHK(dense_peel_stage(e, g, L, k, R, &core, &kill, &levels, &PS));
+    kill_valid = false;
   } else {
     DBuf<uint64_t> ioff(&A);
     DBuf<uint32_t> icur(&A), inc(&A), deg(&A), fa(&A),
@@ -1980,8 +1978,15 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   unsigned long long ce = 0;
   CHK(readback(e, &ce, acc.p, 1));
   R->core_edges = ce / 2;
-  const bool list_cc = !core_only && core_T < 0xFFFFFFFFull;
-  if (list_cc) {
+  // every clique in the densest core (the graph is its own densest core,
+  // as a pruned core often is): the core stages take all cliques, with no
+  // list and no kill words
+  const bool all_core = core_T == T;
+  const bool list_cc = !core_only && core_T < 0xFFFFFFFFull && !all_core;
+  const uint32_t *ckill = kill.p;
+  if (all_core) {
+    ckill = nullptr;
+  } else if (list_cc) {
     HIPCHK(cc.alloc(core_T));
     HIPCHK(core_cliques_ranged(core_n, cv.p, k, L.doff.p, L.item_off.p, L.item_base.p,
                                L.cliques.p, core.p, kpu, cc.p, acc.p, A));
@@ -1989,8 +1994,8 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     CHK(ensure_kill());  // the stages below read kill words
     HIPCHK(core_cliques(T, k, L.cliques.p, core.p, kill.p, kpu, nullptr, acc.p, st));
   }
-  unsigned long long ct = 0;
-  CHK(readback(e, &ct, acc.p, 1));
+  unsigned long long ct = T;
+  if (!all_core) CHK(readback(e, &ct, acc.p, 1));
   if (ct != core_T)
     return fail(CDS_ERR_INTERNAL, "core clique count %llu != %llu", ct,
                 (unsigned long long)core_T);
@@ -2039,7 +2044,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       HIPCHK(prune_mark_dense(core_T, k, L.cliques.p, cc.p, cv.p, core_n, cidx.p, g->off,
                               g->adj, cmat.p, mark.p, st));
     } else {
-      HIPCHK(prune_mark(core_T, k, L.cliques.p, kill.p, kpu, g->off, g->adj, mark.p, st,
+      HIPCHK(prune_mark(core_T, k, L.cliques.p, ckill, kpu, g->off, g->adj, mark.p, st,
                         cc.p));
     }
     HIPCHK(count_nonzero_u8(na, mark.p, acc.p, st));
@@ -2126,7 +2131,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   HIPCHK(ccount.alloc(ncomp));
   HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_n));
-  HIPCHK(comp_clique_totals(core_T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, ncomp,
+  HIPCHK(comp_clique_totals(core_T, k, L.cliques.p, ckill, kpu, d_comp_of.p, ncomp,
                             ccount.p, st, cc.p));
   std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0);
   std::vector<uint64_t> hfioff(core_n + 1, 0);
@@ -2149,7 +2154,7 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // flow (the explicit kernels read the members with the same stride).
   const uint32_t ms = imp_record_words(k);
   HIPCHK(fmem.alloc(core_T * ms));
-  HIPCHK(comp_cliques_build(core_T, k, L.cliques.p, kill.p, kpu, d_comp_of.p, d_fv_of.p,
+  HIPCHK(comp_cliques_build(core_T, k, L.cliques.p, ckill, kpu, d_comp_of.p, d_fv_of.p,
                             ncomp, coff.p, ccur.p, fmem.p, nullptr, st, ms, cc.p));
   cc.reset();
   // flow incidence: per core vertex, its (clique slot, position) pairs

@@ -33,7 +33,8 @@ __device__ __forceinline__ bool core_vertex(const uint32_t *cv, const uint32_t *
 }
 
 // A core clique by index: the i-th of the list cc when given (T entries),
-// else clique i when its kill level is >= kp.
+// else clique i when its kill level is >= kp, or clique i when kill is
+// null too (every clique is in the core).
 __device__ __forceinline__ bool core_clique(const uint32_t *cc, const uint32_t *kill,
                                             uint32_t kp, uint64_t i, uint64_t *c) {
   if (cc) {
@@ -41,7 +42,7 @@ __device__ __forceinline__ bool core_clique(const uint32_t *cc, const uint32_t *
     return true;
   }
   *c = i;
-  return kill[i] >= kp;
+  return !kill || kill[i] >= kp;
 }
 
 __global__ __launch_bounds__(256) void k_core_edges(
@@ -309,7 +310,7 @@ __global__ void k_prune_mark_dense(uint64_t T, uint32_t k,
                                    uint8_t *__restrict__ mark) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t c = cc[i];
+    const uint64_t c = cc ? cc[i] : i;
     for (uint32_t i = 0; i < k; ++i) {
       const uint32_t ci = cidx[cl[c * k + i]];
       if (ci >= cn) continue;  // cannot happen for a core clique

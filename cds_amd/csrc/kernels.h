@@ -414,7 +414,8 @@ hipError_t uf_union_edges(uint32_t n, const uint32_t *off, const uint32_t *adj,
 hipError_t uf_flatten(uint32_t n, uint32_t *parent, hipStream_t s);
 // Same marks for a densest core of cn <= kDenseCoreMax vertices (cv, cidx
 // from core_list) and its Tc cliques listed in cc, through a cn x cn matrix
-// of edge positions (cmat, cn * cn words of scratch).
+// of edge positions (cmat, cn * cn words of scratch); cc null: all T
+// cliques are core cliques.
 constexpr uint32_t kDenseCoreMax = 4096;
 hipError_t prune_mark_dense(uint64_t Tc, uint32_t k, const uint32_t *cliques,
                             const uint32_t *cc, const uint32_t *cv, uint32_t cn,
@@ -426,7 +427,7 @@ hipError_t prune_mark_bits(uint32_t k, const uint32_t *cv, uint32_t cn, const ui
                            const uint32_t *off, const uint32_t *adj,
                            unsigned long long *bits, uint8_t *mark, hipStream_t s);
 // Marks every edge of a core clique: the cliques with kill >= kp among T,
-// or the T listed in cc when it is given.
+// or the T listed in cc when it is given, or all T when both are null.
 hipError_t prune_mark(uint64_t T, uint32_t k, const uint32_t *cliques,
                       const uint32_t *kill, uint32_t kp, const uint32_t *off,
                       const uint32_t *adj, uint8_t *mark, hipStream_t s,
@@ -439,7 +440,7 @@ hipError_t count_nonzero_u8(size_t n, const uint8_t *a,
 //  comp_cliques_build  fmem, and cslot[c] = the flow index of core clique c
 //                      when cslot is given
 // The core cliques are those with kill >= kp among T, or the T listed in cc
-// when it is given.
+// when it is given, or all T when both are null.
 hipError_t comp_clique_totals(uint64_t T, uint32_t k, const uint32_t *cliques,
                               const uint32_t *kill, uint32_t kp,
                               const uint32_t *comp_of, uint32_t ncomp,
