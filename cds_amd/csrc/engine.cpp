@@ -2488,6 +2488,7 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint8_t> keep(&A), state(&A);
   DBuf<uint32_t> newid(&A), cnt(&A), sel(&A), dropped(&A), dlist(&A), ddeg(&A), dpoff(&A);
   DBuf<unsigned long long> dstats(&A);
+  DBuf<uint64_t> cd_first(&A);  // the whole graph's counts, when dropped from in place
   HIPCHK(keep.alloc(n));
   HIPCHK(hipMemsetAsync(keep.p, 1, n, st));
   HIPCHK(newid.alloc((size_t)n + 1));
@@ -2571,13 +2572,21 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     CHK(readback(e, &nd, dropped.p, 1));
     if (nd) {
       CHK(readback(e, stats, dstats.p, 2));
-      const bool in_place = Ls && (k == 3 || k == 4) && unlink_mode != 0 &&
+      const bool in_place = (k == 3 || k == 4) && unlink_mode != 0 &&
                             stats[1] <= kUnlinkMaxDegree &&
                             (unlink_mode == 1 || stats[0] <= Ts / 16);
       if (in_place) {
         const auto tp0 = std::chrono::steady_clock::now();
+        if (!Ls && !cd_first.p) {
+          // the whole graph's counts are reported: drop from a copy
+          HIPCHK(cd_first.alloc(n));
+          HIPCHK(hipMemcpyAsync(cd_first.p, L0.cd.p, (size_t)n * sizeof(uint64_t),
+                                hipMemcpyDeviceToDevice, st));
+          cds = cd_first.p;
+        }
+        uint64_t *cdw = Ls ? Ls->cd.p : cd_first.p;
         HIPCHK(prune_unlink(k, nd, dlist.p, (uint32_t)stats[1], gs->g.off, gs->g.adj, state.p,
-                            reinterpret_cast<unsigned long long *>(Ls->cd.p), dstats.p + 2,
+                            reinterpret_cast<unsigned long long *>(cdw), dstats.p + 2,
                             ddeg.p, dpoff.p, A));
         HIPCHK(prune_finish(nd, dlist.p, sel.p, state.p, keep.p, st));
         unsigned long long removed = 0;
