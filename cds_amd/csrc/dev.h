@@ -46,6 +46,13 @@ __device__ __forceinline__ uint32_t aggregated_reserve(uint32_t *ctr,
   return slot;
 }
 
+// Lane l's 64-bit value, in every lane (l wave-uniform).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Wave-wide sum of a 32-bit value.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll

@@ -2489,6 +2489,8 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   DBuf<uint32_t> newid(&A), cnt(&A), sel(&A), dropped(&A), dlist(&A), ddeg(&A), dpoff(&A);
   DBuf<unsigned long long> dstats(&A);
   DBuf<uint64_t> cd_first(&A);  // the whole graph's counts, when dropped from in place
+  DBuf<uint64_t> cd_bits(&A);   // counts of G[S] from its adjacency bitsets
+  uint64_t *cdw = nullptr;      // the counts of G[S] that drops in place update
   HIPCHK(keep.alloc(n));
   HIPCHK(hipMemsetAsync(keep.p, 1, n, st));
   HIPCHK(newid.alloc((size_t)n + 1));
@@ -2511,6 +2513,10 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   // 1/16 of the cliques of S pass through them.
   int unlink_mode = -1;
   if (const char *f = getenv("CDS_PRUNE_UNLINK")) unlink_mode = atoi(f) != 0 ? 1 : 0;
+  // a G[S] of at most kBitsCountMax vertices (k = 3, 4) is counted on its
+  // adjacency bitsets instead of by listing (CDS_PRUNE_BITS=0: never)
+  bool count_bits = k == 3 || k == 4;
+  if (const char *f = getenv("CDS_PRUNE_BITS")) count_bits = count_bits && atoi(f) != 0;
   // G[S] from keep[]
   auto build_subgraph = [&]() -> cds_status {
     HIPCHK(induced_subgraph(n, g->off, g->adj, keep.p, newid.p, cnt.p, nullptr, nullptr,
@@ -2548,16 +2554,37 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
       if (!first) {
         // clique counts inside S
         const auto tp0 = std::chrono::steady_clock::now();
-        Ls = std::make_unique<Listing>(&A);
-        CHK(run_dag(e, &gs->g, *Ls));
-        CHK(run_listing(e, &gs->g, k, false, *Ls, &tm));
-        Ts = Ls->T;
-        cds = Ls->cd.p;
+        Ls.reset();
+        const uint64_t arcs = gs->g.host_off[ns];
+        const bool bits = count_bits && ns <= kBitsCountMax;
+        if (bits) {
+          DBuf<unsigned long long> rows(&A), tot(&A);
+          DBuf<uint32_t> src(&A);
+          HIPCHK(rows.alloc((size_t)ns * ((ns + 63) / 64)));
+          HIPCHK(tot.alloc(1));
+          HIPCHK(src.alloc(arcs));
+          HIPCHK(cd_bits.alloc(ns));
+          HIPCHK(bits_clique_count(k, ns, gs->g.off, gs->g.adj, (uint32_t)arcs, rows.p, src.p,
+                                   reinterpret_cast<unsigned long long *>(cd_bits.p), tot.p,
+                                   st));
+          unsigned long long t = 0;
+          CHK(readback(e, &t, tot.p, 1));
+          Ts = t;
+          cdw = cd_bits.p;
+        } else {
+          cd_bits.reset();
+          Ls = std::make_unique<Listing>(&A);
+          CHK(run_dag(e, &gs->g, *Ls));
+          CHK(run_listing(e, &gs->g, k, false, *Ls, &tm));
+          Ts = Ls->T;
+          cdw = Ls->cd.p;
+        }
+        cds = cdw;
         ++passes;
         if (peel_trace_on())
           fprintf(stderr, "[prune] pass %u tau %llu: %u vertices %llu arcs %llu cliques, "
-                  "count %.2f ms\n", passes, (unsigned long long)tau, ns,
-                  (unsigned long long)gs->g.host_off[ns], (unsigned long long)Ts,
+                  "count%s %.2f ms\n", passes, (unsigned long long)tau, ns,
+                  (unsigned long long)arcs, (unsigned long long)Ts, bits ? " (bitsets)" : "",
                   std::chrono::duration<double, std::milli>(
                       std::chrono::steady_clock::now() - tp0).count());
       }
@@ -2577,14 +2604,13 @@ cds_status solve_pruned(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                             (unlink_mode == 1 || stats[0] <= Ts / 16);
       if (in_place) {
         const auto tp0 = std::chrono::steady_clock::now();
-        if (!Ls && !cd_first.p) {
+        if (!cdw) {
           // the whole graph's counts are reported: drop from a copy
           HIPCHK(cd_first.alloc(n));
           HIPCHK(hipMemcpyAsync(cd_first.p, L0.cd.p, (size_t)n * sizeof(uint64_t),
                                 hipMemcpyDeviceToDevice, st));
-          cds = cd_first.p;
+          cds = cdw = cd_first.p;
         }
-        uint64_t *cdw = Ls ? Ls->cd.p : cd_first.p;
         HIPCHK(prune_unlink(k, nd, dlist.p, (uint32_t)stats[1], gs->g.off, gs->g.adj, state.p,
                             reinterpret_cast<unsigned long long *>(cdw), dstats.p + 2,
                             ddeg.p, dpoff.p, A));

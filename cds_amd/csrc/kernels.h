@@ -89,6 +89,14 @@ hipError_t induced_subgraph(uint32_t n, const uint32_t *off, const uint32_t *adj
                             const uint8_t *keep, uint32_t *newid, uint32_t *cnt,
                             uint32_t *off2, uint32_t *adj2, uint32_t *sel, bool fill,
                             DeviceArena &a);
+// Clique count (*total) and clique degrees (cd, ns words) of a graph of
+// ns <= kBitsCountMax vertices, k = 3 or 4, from its adjacency bitsets
+// (bits: ns * ceil(ns / 64) words; src: arcs words of workspace).
+constexpr uint32_t kBitsCountMax = 4096;
+hipError_t bits_clique_count(uint32_t k, uint32_t ns, const uint32_t *off,
+                             const uint32_t *adj, uint32_t arcs, unsigned long long *bits,
+                             uint32_t *src, unsigned long long *cd,
+                             unsigned long long *total, hipStream_t s);
 // In-place drops of the pruned path, on the current induced subgraph with
 // a state byte per vertex (1 in S, 2 dropped this round, 0 dropped
 // before).  prune_select moves the vertices in S with cd < tau to state 2

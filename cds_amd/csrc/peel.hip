@@ -1172,12 +1172,6 @@ __global__ __launch_bounds__(256) void k_replay_rows(
   }
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Degree updates of one removal: list entry t (a phase position u adjacent
 // to v) loses popcount(row u & nbr), summed over a group of G >= W lanes
 // (one word each); kU entries per lane are loaded before any is summed.

@@ -223,6 +223,106 @@ __global__ void k_bitset_rows(uint32_t ns, const uint32_t *__restrict__ off,
   }
 }
 
+// src[p] = the vertex whose adjacency holds arc p, one wave per vertex.
+__global__ void k_arc_src(uint32_t ns, const uint32_t *__restrict__ off,
+                          uint32_t *__restrict__ src) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < ns; v += waves)
+    for (uint32_t p = off[v] + lane(); p < off[v + 1]; p += kWave) src[p] = v;
+}
+
+// Clique degrees and clique count of a graph of ns <= kBitsCountMax
+// vertices from its adjacency bitsets (W <= 64 words a row, G >= W lanes a
+// group), every clique once from its two smallest members u < v, one wave
+// per edge: C = N(u) & N(v) & {ids > v}, one word per lane of each group.
+// K = 3: the triangles {u, v, y}, y in C.  K = 4: the 4-cliques {u, v, y,
+// x}, y < x adjacent in C: |N(y) & C| of them through y (one member y per
+// group at a time), e(C) = half their sum through u and through v.
+// Counts gather in LDS (ns words) and are added to cd once per workgroup.
+template <uint32_t K, uint32_t G>
+__global__ __launch_bounds__(1024) void k_bits_count(
+    uint32_t ns, uint32_t W, const uint32_t *__restrict__ src,
+    const uint32_t *__restrict__ adj, uint32_t arcs,
+    const unsigned long long *__restrict__ bits, unsigned long long *__restrict__ cd,
+    unsigned long long *__restrict__ total) {
+  extern __shared__ unsigned long long lcd[];
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) lcd[i] = 0;
+  __syncthreads();
+  constexpr uint32_t kGroups = kWave / G;
+  const uint32_t j = lane() % G, gid = lane() / G;
+  const uint32_t jj = j < W ? j : W - 1;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  unsigned long long tot = 0;
+  for (uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; p < arcs; p += waves) {
+    const uint32_t u = src[p], v = adj[p];
+    if (v <= u) continue;
+    const unsigned long long x = bits[(size_t)u * W + jj] & bits[(size_t)v * W + jj];
+    const uint32_t vw = v >> 6, vb = v & 63;
+    const unsigned long long gt =
+        j < vw ? 0ull : (j > vw ? ~0ull : (vb == 63 ? 0ull : (~0ull << (vb + 1))));
+    const unsigned long long c = j < W ? (x & gt) : 0ull;
+    if (K == 3) {
+      const uint32_t n1 = gid == 0 ? (uint32_t)__popcll(c) : 0u;
+      const uint32_t nc = group_sum<kWave>(n1);
+      if (gid == 0)
+        for (unsigned long long w = c; w; w &= w - 1)
+          atomicAdd(&lcd[j * 64 + __builtin_ctzll(w)], 1ull);
+      if (lane() == 0 && nc) {
+        atomicAdd(&lcd[u], (unsigned long long)nc);
+        atomicAdd(&lcd[v], (unsigned long long)nc);
+        tot += nc;
+      }
+    } else {
+      // members of C in ascending order, kGroups at a time
+      uint64_t bal = __ballot(gid == 0 && c != 0);
+      unsigned long long w = 0;
+      uint32_t wl = 0;
+      uint32_t e2 = 0;  // this group's sum of |N(y) & C| (in its leader)
+      for (;;) {
+        uint32_t ys[kGroups];
+        uint32_t got = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < kGroups; ++g) {
+          while (!w && bal) {
+            wl = (uint32_t)__builtin_ctzll(bal);
+            bal &= bal - 1;
+            w = readlane64(c, wl);
+          }
+          ys[g] = 0;
+          if (w) {
+            ys[g] = wl * 64 + (uint32_t)__builtin_ctzll(w);
+            w &= w - 1;
+            ++got;
+          }
+        }
+        if (!got) break;
+        uint32_t y = ys[0];
+#pragma unroll
+        for (uint32_t g = 1; g < kGroups; ++g)
+          if (gid == g) y = ys[g];
+        const bool valid = gid < got;
+        const unsigned long long r = bits[(size_t)y * W + jj] & c;
+        const uint32_t py = group_sum<G>(valid ? (uint32_t)__popcll(r) : 0u);
+        if (j == 0 && valid && py) {
+          atomicAdd(&lcd[y], (unsigned long long)py);
+          e2 += py;
+        }
+      }
+      const uint32_t es = group_sum<kWave>(e2);
+      if (lane() == 0 && es) {
+        const unsigned long long e = es / 2;
+        atomicAdd(&lcd[u], e);
+        atomicAdd(&lcd[v], e);
+        tot += e;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x)
+    if (lcd[i]) atomicAdd(&cd[i], lcd[i]);
+  if (lane() == 0 && tot) atomicAdd(total, tot);
+}
+
 __global__ void k_alive_init(uint32_t ns, uint32_t W, unsigned long long *__restrict__ alive) {
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W;
        w += gridDim.x * blockDim.x) {
@@ -348,6 +448,40 @@ hipError_t induced_subgraph(uint32_t n, const uint32_t *off, const uint32_t *adj
   }
   hipLaunchKernelGGL(k_induced_fill, dim3(grid_for((size_t)n * kWave, 256, 16384)),
                      dim3(256), 0, s, n, off, adj, keep, newid, off2, adj2, sel);
+  return hipGetLastError();
+}
+
+hipError_t bits_clique_count(uint32_t k, uint32_t ns, const uint32_t *off,
+                             const uint32_t *adj, uint32_t arcs, unsigned long long *bits,
+                             uint32_t *src, unsigned long long *cd,
+                             unsigned long long *total, hipStream_t s) {
+  if ((k != 3 && k != 4) || ns == 0 || ns > kBitsCountMax) return hipErrorInvalidValue;
+  const uint32_t W = (ns + 63) / 64;
+  hipLaunchKernelGGL(k_bitset_rows, dim3(grid_for((size_t)ns * kWave, 256, 8192)), dim3(256),
+                     0, s, ns, off, adj, W, bits);
+  hipLaunchKernelGGL(k_arc_src, dim3(grid_for((size_t)ns * kWave, 256, 8192)), dim3(256), 0, s,
+                     ns, off, src);
+  hipError_t e = hipMemsetAsync(cd, 0, (size_t)ns * sizeof(unsigned long long), s);
+  if (e == hipSuccess) e = hipMemsetAsync(total, 0, sizeof(unsigned long long), s);
+  if (e != hipSuccess) return e;
+  if (arcs == 0) return hipGetLastError();
+  const size_t lds = (size_t)ns * sizeof(unsigned long long);
+  const unsigned grid = grid_for((size_t)arcs * kWave, 1024, 512);
+#define CDS_BITS_COUNT(KK, GG)                                                              \
+  hipLaunchKernelGGL((k_bits_count<KK, GG>), dim3(grid), dim3(1024), lds, s, ns, W, src, adj, \
+                     arcs, bits, cd, total)
+  if (k == 3) {
+    CDS_BITS_COUNT(3, 64);
+  } else if (W <= 8) {
+    CDS_BITS_COUNT(4, 8);
+  } else if (W <= 16) {
+    CDS_BITS_COUNT(4, 16);
+  } else if (W <= 32) {
+    CDS_BITS_COUNT(4, 32);
+  } else {
+    CDS_BITS_COUNT(4, 64);
+  }
+#undef CDS_BITS_COUNT
   return hipGetLastError();
 }
 

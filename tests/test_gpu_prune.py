@@ -27,13 +27,15 @@ SCALE = sorted(glob.glob(os.path.join(HERE, "golden", "scale_*.json")))
 NONE = 0xFFFFFFFF
 
 
-def _solve(engine, dg, k, dense, vertex_arrays=True, unlink=None):
+def _solve(engine, dg, k, dense, vertex_arrays=True, unlink=None, bits=None):
     saved = {v: os.environ.get(v) for v in ("CDS_CORE_PRUNE", "CDS_DENSE_PEEL",
-                                            "CDS_PRUNE_UNLINK")}
+                                            "CDS_PRUNE_UNLINK", "CDS_PRUNE_BITS")}
     os.environ["CDS_CORE_PRUNE"] = "1"
     os.environ["CDS_DENSE_PEEL"] = "1" if dense else "0"
     if unlink is not None:
         os.environ["CDS_PRUNE_UNLINK"] = unlink
+    if bits is not None:
+        os.environ["CDS_PRUNE_BITS"] = bits
     try:
         return engine.solve(dg, k, vertex_arrays=vertex_arrays)
     finally:
@@ -104,21 +106,25 @@ def test_pruned_path_in_place_drops(engine, kind, seed, k):
     _check(got, oracle_solve(g, k), label)
 
 
-@pytest.mark.parametrize("scale,ef,k,seed", [(14, 16, 3, 81), (14, 16, 4, 82), (15, 12, 4, 83)])
+@pytest.mark.parametrize("scale,ef,k,seed", [(14, 16, 3, 81), (14, 16, 4, 82), (15, 12, 4, 83),
+                                              (13, 24, 4, 84)])
 def test_in_place_drops_match_count_passes(engine, scale, ef, k, seed):
-    """R-MAT graphs beyond the oracle: drops in place (forced, default) and
-    a count pass after every drop give the same pruned core and answer."""
+    """R-MAT graphs beyond the oracle: drops in place (forced, default),
+    subgraphs counted on bitsets, and a listing count pass after every drop
+    give the same pruned core and answer."""
     g = cds_amd.Graph.rmat(scale, ef, seed=seed)
     dg = engine.upload(g)
     _announce(f"in-place drops vs count passes R-MAT scale {scale} ef {ef} k={k}")
-    base = _solve(engine, dg, k, False, vertex_arrays=False, unlink="0")
+    base = _solve(engine, dg, k, False, vertex_arrays=False, unlink="0", bits="0")
     assert base["prune_unlinks"] == 0
-    for mode in ("1", None):
-        got = _solve(engine, dg, k, False, vertex_arrays=False, unlink=mode)
+    for mode, bits in (("1", "0"), ("0", None), ("1", None), (None, None)):
+        got = _solve(engine, dg, k, False, vertex_arrays=False, unlink=mode, bits=bits)
         for f in ("prune_tau", "prune_vertices", "kmax", "set_size", "total_cliques",
                   "k_prime", "core_size", "core_edges", "core_cliques", "deleted_cliques",
                   "pruned_edges", "num_components", "size", "set_cliques"):
             assert got[f] == base[f], (mode, f, got[f], base[f])
+        if mode == "0":
+            assert got["prune_passes"] == base["prune_passes"]
         assert got["prune_passes"] + got["prune_unlinks"] >= base["prune_passes"]
         if mode == "1" and base["prune_passes"] > 1 and got["max_degree"] <= 2048:
             assert got["prune_unlinks"] > 0
