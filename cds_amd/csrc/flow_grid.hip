@@ -1051,10 +1051,15 @@ __device__ __forceinline__ unsigned long long imp_relabel(const Hx &g, uint64_t 
   const long long kU = (long long)(g.k - 1) * g.U;
   const uint64_t arcs = (uint64_t)g.T * g.k;
   int vb = 0, d = 1;
+  uint32_t labelled = 0;  // vertices with a height so far (each is listed once)
   for (;;) {
     const unsigned long long pack = ald(&ctl->lpack[vb]);
     const uint32_t nv = (uint32_t)(pack >> 40);
-    if (nv == 0) break;
+    // clique heights are derived, so once every vertex has its height the
+    // next level's scan would label nothing (at the start every vertex
+    // with sink residual, usually all of them, is at height 1)
+    labelled += nv;
+    if (nv == 0 || labelled >= g.V) break;
     const uint64_t nch = pack & kMask40;
     if (g.leader) ctl->lpack[1 - vb] = 0;
     gsync(g);
@@ -1485,7 +1490,8 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
       t[7] = wall_clock64() - tr3;  // (C) and the global relabel
     }
   }
-  if (!limit) imp_relabel(g, kArcFull);
+  // without the round limit the loop ended on a relabel that found no
+  // active vertex, so hv already holds the final exact heights
   sink = wave_sum64(sink);
   if (lane() == 0 && sink) atomicAdd(&ctl->sink, sink);
   for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) S.reach[v] = g.hv[v] < g.INF ? 1 : 0;
