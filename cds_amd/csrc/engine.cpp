@@ -2128,14 +2128,29 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
     HIPCHK(scatter_u32x2(core_n, cv.p, lcomp.p, lfv.p, d_comp_of.p, d_fv_of.p, st));
     HIPCHK(hipStreamSynchronize(st));
   }
-  HIPCHK(ccount.alloc(ncomp));
-  HIPCHK(ccount.zero());
   HIPCHK(fcd.alloc(core_n));
-  HIPCHK(comp_clique_totals(core_T, k, L.cliques.p, ckill, kpu, d_comp_of.p, ncomp,
-                            ccount.p, st, cc.p));
   std::vector<uint32_t> hcc(ncomp), hfcd(core_n, 0);
   std::vector<uint64_t> hfioff(core_n + 1, 0);
-  CHK(readback(e, hcc.data(), ccount.p, ncomp));
+  if (all_core) {
+    // every clique is a core clique: a flow vertex's slots are its clique
+    // degree, and a component's cliques a k-th of their sum
+    HIPCHK(core_flow_degrees(core_n, cv.p, L.cd.p, d_fv_of.p, fcd.p, st));
+    CHK(readback(e, hfcd.data(), fcd.p, core_n));
+    for (uint32_t c = 0; c < ncomp; ++c) {
+      uint64_t s = 0;
+      for (uint32_t f = comps[c].vbase; f < comps[c].vbase + comps[c].V; ++f) s += hfcd[f];
+      if (s % k)
+        return fail(CDS_ERR_INTERNAL, "component %u holds %llu clique slots", c,
+                    (unsigned long long)s);
+      hcc[c] = (uint32_t)(s / k);
+    }
+  } else {
+    HIPCHK(ccount.alloc(ncomp));
+    HIPCHK(ccount.zero());
+    HIPCHK(comp_clique_totals(core_T, k, L.cliques.p, ckill, kpu, d_comp_of.p, ncomp,
+                              ccount.p, st, cc.p));
+    CHK(readback(e, hcc.data(), ccount.p, ncomp));
+  }
   std::vector<uint64_t> hcoff(ncomp + 1, 0);
   for (uint32_t c = 0; c < ncomp; ++c) {
     comps[c].T = hcc[c];
@@ -2158,8 +2173,10 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
                             ncomp, coff.p, ccur.p, fmem.p, nullptr, st, ms, cc.p));
   cc.reset();
   // flow incidence: per core vertex, its (clique slot, position) pairs
-  HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st, ms));
-  CHK(readback(e, hfcd.data(), fcd.p, core_n));
+  if (!all_core) {
+    HIPCHK(flow_incidence_degrees(core_T, k, fmem.p, core_n, fcd.p, st, ms));
+    CHK(readback(e, hfcd.data(), fcd.p, core_n));
+  }
   for (uint32_t f = 0; f < core_n; ++f) hfioff[f + 1] = hfioff[f] + hfcd[f];
   if (hfioff[core_n] != core_T * k)
     return fail(CDS_ERR_INTERNAL, "flow incidence holds %llu of %llu pairs",
@@ -2420,12 +2437,16 @@ cds_status solve_impl(cds_engine *e, const cds_dgraph *g, const cds_params *p,
   {
     // k-cliques inside the reported set (exact numerator): they are core
     // cliques, so they are counted over the core's clique records
-    DBuf<uint8_t> mask(&A);
-    HIPCHK(mask.alloc(core_n));
-    CHK(upload(e, mask.p, hf.data(), core_n));
-    HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st, ms));
-    unsigned long long sc = 0;
-    CHK(readback(e, &sc, acc.p, 1));
+    // (a whole component holds exactly its own cliques)
+    const Component &C = comps[search.best_comp];
+    unsigned long long sc = C.T;
+    if (verts.size() != C.V) {
+      DBuf<uint8_t> mask(&A);
+      HIPCHK(mask.alloc(core_n));
+      CHK(upload(e, mask.p, hf.data(), core_n));
+      HIPCHK(count_cliques_in_mask(core_T, k, fmem.p, mask.p, acc.p, st, ms));
+      CHK(readback(e, &sc, acc.p, 1));
+    }
     R->set_cliques = sc;
   }
   fill_trace(R, search, k, comps);

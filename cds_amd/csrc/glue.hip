@@ -426,6 +426,14 @@ __global__ __launch_bounds__(kCompThreads) void k_comp_count2(
 
 // Places every core clique in its component's range of the flow arrays:
 // fmem (flow-vertex ids of the members) and cslot[c] = its flow index.
+// Three member ids moved with one 12-byte load or store.
+struct __attribute__((packed, aligned(4))) Words3 {
+  uint32_t w[3];
+};
+
+// Clique c's members in flow ids at record f (stride ms words): one vector
+// load and one vector store per clique for k = 3 and 4 (clique lists and
+// records start 256-byte aligned; a k = 4 record stride is 4 or 8 words).
 __device__ __forceinline__ void comp_place(uint64_t c, uint64_t f, uint32_t k,
                                            uint32_t ms,
                                            const uint32_t *__restrict__ cl,
@@ -433,7 +441,17 @@ __device__ __forceinline__ void comp_place(uint64_t c, uint64_t f, uint32_t k,
                                            uint32_t *__restrict__ fmem,
                                            uint32_t *__restrict__ cslot) {
   if (cslot) cslot[c] = (uint32_t)f;
-  for (uint32_t j = 0; j < k; ++j) fmem[f * ms + j] = fv_of[cl[c * k + j]];
+  if (k == 4 && (ms & 3) == 0) {
+    const uint4 m = *reinterpret_cast<const uint4 *>(cl + c * 4);
+    *reinterpret_cast<uint4 *>(fmem + f * ms) =
+        make_uint4(fv_of[m.x], fv_of[m.y], fv_of[m.z], fv_of[m.w]);
+  } else if (k == 3) {
+    const Words3 m = *reinterpret_cast<const Words3 *>(cl + c * 3);
+    *reinterpret_cast<Words3 *>(fmem + f * ms) =
+        Words3{{fv_of[m.w[0]], fv_of[m.w[1]], fv_of[m.w[2]]}};
+  } else {
+    for (uint32_t j = 0; j < k; ++j) fmem[f * ms + j] = fv_of[cl[c * k + j]];
+  }
 }
 
 __global__ __launch_bounds__(kCompThreads) void k_comp_scatter2(
@@ -776,6 +794,25 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
   hipLaunchKernelGGL(k_comp_scatter2, dim3(grid), dim3(kCompThreads), 0, s, T, k,
                      cliques, kill, kp, comp_of, fv_of, ncomp, coff, ccur, fmem,
                      cslot, ms, cc);
+  return hipGetLastError();
+}
+
+__global__ void k_core_flow_degrees(uint32_t cn, const uint32_t *__restrict__ cv,
+                                    const uint64_t *__restrict__ cd,
+                                    const uint32_t *__restrict__ fv_of,
+                                    uint32_t *__restrict__ fcd) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cn;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t v = cv[i];
+    fcd[fv_of[v]] = (uint32_t)cd[v];
+  }
+}
+
+hipError_t core_flow_degrees(uint32_t cn, const uint32_t *cv, const uint64_t *cd,
+                             const uint32_t *fv_of, uint32_t *fcd, hipStream_t s) {
+  if (cn == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_core_flow_degrees, dim3(grid_for(cn, 256, 4096)), dim3(256), 0, s, cn,
+                     cv, cd, fv_of, fcd);
   return hipGetLastError();
 }
 

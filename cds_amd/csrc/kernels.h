@@ -468,6 +468,10 @@ hipError_t comp_cliques_build(uint64_t T, uint32_t k, const uint32_t *cliques,
 // by_clique their cliques s (for networks of 2^32 pairs or more; fcur is
 // nv words of scratch, zeroed here).
 // ms: words per clique in fmem (0: k; clique records hold more).
+// fcd[fv_of[cv[i]]] = cd[cv[i]] for the cn core vertices: the flow
+// incidence degrees when every clique of the graph is a core clique.
+hipError_t core_flow_degrees(uint32_t cn, const uint32_t *cv, const uint64_t *cd,
+                             const uint32_t *fv_of, uint32_t *fcd, hipStream_t s);
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                   uint32_t nv, uint32_t *fcd, hipStream_t s,
                                   uint32_t ms = 0);
