@@ -1228,6 +1228,7 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
     ctl->limit = 0;
     ctl->relabels = 0;
     ctl->cqn[0] = 0;
+    ctl->cqn[1] = 0;
   }
   gsync(g);
   unsigned long long active = imp_relabel(g, kArcFull);
@@ -1239,6 +1240,43 @@ __device__ __forceinline__ void imp_run(Hx &g, const FlowProblem &P, const GridF
   // active-vertex count (cqn[0]) are cleared in (C) and filled in (A); list
   // 1 (relabel candidates) is cleared in (A) and filled in (D); each is
   // read a phase later.
+  // The first round, when no flow has moved yet and every vertex has its
+  // sink arc (height 1, so every clique is at height 2): the sink pushes,
+  // and since no clique arc is admissible, a vertex that keeps excess would
+  // scan its whole list twice to find the relabel to 3 (every arc v->c is
+  // residual) that is set here directly, as those scans would set it.
+  {
+    unsigned off1 = 0;
+    for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) off1 += g.hv[v] != 1 ? 1u : 0u;
+    off1 = wave_sum(off1);
+    if (lane() == 0 && off1) atomicAdd(&ctl->cqn[1], off1);
+    gsync(g);
+  }
+  if (active && ald(&ctl->cqn[1]) == 0u && max_rounds > 0) {
+    for (uint32_t v = g.gtid; v < g.V; v += g.gthreads) {
+      g.inflow[v] = 0;
+      long long e = g.exv[v];
+      const long long r = g.rvs[v];
+      if (e > 0 && r > 0) {
+        const long long d = e < r ? e : r;
+        g.rvs[v] = r - d;
+        e -= d;
+        sink += (unsigned long long)d;
+      }
+      g.exv[v] = e > 0 ? e : 0;
+      g.rem[v] = e > 0 ? e : 0;
+      g.hmin[v] = 0;
+      if (e > 0) {
+        g.pend[v] = 2;
+        g.arc[v] = (uint64_t)kArcWindowRelabel << 32;
+      }
+    }
+    if (S.trace && g.leader && S.trace_cap > 0)
+      for (int j = 0; j < 8; ++j) S.trace[j] = 0;
+    gsync(g);
+    rounds = 1;
+    since = 1;
+  }
   while (active) {
     if (rounds >= max_rounds) {
       limit = true;
