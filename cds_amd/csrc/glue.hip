@@ -876,15 +876,107 @@ __global__ __launch_bounds__(kFlowThreads) void k_flow_fill_direct(
   }
 }
 
+// The same two kernels by clique for k = 2, 3, 4: one vector load of a
+// record's K members (no slot division), counters in dynamic LDS sized to
+// the core, so small cores leave room for several workgroups per CU.
+// Largest core the by-clique kernels take (its counters in 64 KB of LDS).
+constexpr uint32_t kFlowCliqueLds = 16384;
+
+template <uint32_t K>
+__device__ __forceinline__ void record_members(const uint32_t *__restrict__ fmem, uint64_t c,
+                                               uint32_t ms, uint32_t (&m)[K]) {
+  const uint32_t *r = fmem + c * ms;
+  if (K == 4 && (ms & 3) == 0) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(r);
+    m[0] = v.x;
+    m[1] = v.y;
+    m[2] = v.z;
+    m[3] = v.w;
+  } else if (K == 3) {
+    const Words3 v = *reinterpret_cast<const Words3 *>(r);
+    m[0] = v.w[0];
+    m[1] = v.w[1];
+    m[2] = v.w[2];
+  } else if (K == 2 && (ms & 1) == 0) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(r);
+    m[0] = v.x;
+    m[1] = v.y;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) m[j] = r[j];
+  }
+}
+
+template <uint32_t K>
+__global__ __launch_bounds__(kFlowThreads) void k_flow_deg_cliques(
+    uint64_t Tc, const uint32_t *__restrict__ fmem, uint32_t nv, uint32_t *__restrict__ fcd,
+    uint32_t ms) {
+  extern __shared__ uint32_t lcnt[];
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) lcnt[i] = 0;
+  __syncthreads();
+  uint64_t lo, hi;
+  flow_range(Tc, &lo, &hi);
+  for (uint64_t c = lo + threadIdx.x; c < hi; c += kFlowThreads) {
+    uint32_t m[K];
+    record_members<K>(fmem, c, ms, m);
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) atomicAdd(&lcnt[m[j]], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
+    if (lcnt[i]) atomicAdd(&fcd[i], lcnt[i]);
+}
+
+template <uint32_t K>
+__global__ __launch_bounds__(kFlowThreads) void k_flow_fill_cliques(
+    uint64_t Tc, const uint32_t *__restrict__ fmem, uint32_t nv,
+    const uint64_t *__restrict__ fioff, uint32_t *__restrict__ fcur,
+    uint32_t *__restrict__ finc, bool by_clique, uint32_t ms) {
+  extern __shared__ uint32_t lcnt[];
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads) lcnt[i] = 0;
+  __syncthreads();
+  uint64_t lo, hi;
+  flow_range(Tc, &lo, &hi);
+  for (uint64_t c = lo + threadIdx.x; c < hi; c += kFlowThreads) {
+    uint32_t m[K];
+    record_members<K>(fmem, c, ms, m);
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) atomicAdd(&lcnt[m[j]], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nv; i += kFlowThreads)
+    if (lcnt[i]) lcnt[i] = atomicAdd(&fcur[i], lcnt[i]);  // -> this range's run
+  __syncthreads();
+  for (uint64_t c = lo + threadIdx.x; c < hi; c += kFlowThreads) {
+    uint32_t m[K];
+    record_members<K>(fmem, c, ms, m);
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j)
+      finc[fioff[m[j]] + atomicAdd(&lcnt[m[j]], 1u)] =
+          by_clique ? (uint32_t)c : (uint32_t)(c * K + j);
+  }
+}
+
 hipError_t flow_incidence_degrees(uint64_t Tc, uint32_t k, const uint32_t *fmem,
                                   uint32_t nv, uint32_t *fcd, hipStream_t s, uint32_t ms) {
   if (k > 4) return hipErrorInvalidValue;
   if (ms == 0) ms = k;
   hipError_t e = hipMemsetAsync(fcd, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
-  if (nv <= kFlowDirect)
-    hipLaunchKernelGGL(k_flow_deg_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
-                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fcd, ms);
+  const size_t lds = (size_t)nv * sizeof(uint32_t);
+  const unsigned dgrid = grid_for(Tc, 16 * kFlowTile, 1024);
+  if (nv <= kFlowCliqueLds && k == 4)
+    hipLaunchKernelGGL(k_flow_deg_cliques<4>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fcd, ms);
+  else if (nv <= kFlowCliqueLds && k == 3)
+    hipLaunchKernelGGL(k_flow_deg_cliques<3>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fcd, ms);
+  else if (nv <= kFlowCliqueLds && k == 2)
+    hipLaunchKernelGGL(k_flow_deg_cliques<2>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fcd, ms);
+  else if (nv <= kFlowDirect)
+    hipLaunchKernelGGL(k_flow_deg_direct, dim3(dgrid), dim3(kFlowThreads), 0, s, Tc, k, fmem,
+                       nv, fcd, ms);
   else
     hipLaunchKernelGGL(k_flow_deg, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
                        0, s, Tc, k, fmem, fcd, ms);
@@ -900,10 +992,20 @@ hipError_t flow_incidence_fill(uint64_t Tc, uint32_t k, const uint32_t *fmem,
   const uint32_t div = by_clique ? k : 1;
   hipError_t e = hipMemsetAsync(fcur, 0, (size_t)(nv ? nv : 1) * sizeof(uint32_t), s);
   if (e != hipSuccess || Tc == 0) return e;
-  if (nv <= kFlowDirect)
-    hipLaunchKernelGGL(k_flow_fill_direct, dim3(grid_for(Tc, 16 * kFlowTile, 1024)),
-                       dim3(kFlowThreads), 0, s, Tc, k, fmem, nv, fioff, fcur, finc, div,
-                       ms);
+  const size_t lds = (size_t)nv * sizeof(uint32_t);
+  const unsigned dgrid = grid_for(Tc, 16 * kFlowTile, 1024);
+  if (nv <= kFlowCliqueLds && k == 4)
+    hipLaunchKernelGGL(k_flow_fill_cliques<4>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fioff, fcur, finc, by_clique, ms);
+  else if (nv <= kFlowCliqueLds && k == 3)
+    hipLaunchKernelGGL(k_flow_fill_cliques<3>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fioff, fcur, finc, by_clique, ms);
+  else if (nv <= kFlowCliqueLds && k == 2)
+    hipLaunchKernelGGL(k_flow_fill_cliques<2>, dim3(dgrid), dim3(kFlowThreads), lds, s, Tc,
+                       fmem, nv, fioff, fcur, finc, by_clique, ms);
+  else if (nv <= kFlowDirect)
+    hipLaunchKernelGGL(k_flow_fill_direct, dim3(dgrid), dim3(kFlowThreads), 0, s, Tc, k,
+                       fmem, nv, fioff, fcur, finc, div, ms);
   else
     hipLaunchKernelGGL(k_flow_fill, dim3(grid_for(Tc, kFlowTile, 4096)), dim3(kFlowThreads),
                        0, s, Tc, k, fmem, fioff, fcur, finc, div, ms);
