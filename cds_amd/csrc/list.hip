@@ -161,14 +161,21 @@ __global__ __launch_bounds__(THREADS) void k_list3(
     const uint32_t *nu = staged ? su : dadj + ub;
     const uint64_t obase = write ? item_base[it] : 0;
     uint32_t found = 0;
+    // lo = the last vi with pre[vi] <= i; i only grows, so lo moves on from
+    // where it was (pre[jn] = total > i bounds it) and is searched for only
+    // when i has passed its out-list
+    int lo = 0;
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-      int lo = 0, hi = (int)jn - 1;  // last vi with pre[vi] <= i
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (t.pre[mid] <= i)
-          lo = mid;
-        else
-          hi = mid - 1;
+      if (t.pre[lo + 1] <= i) {
+        int l = lo + 1, hi = (int)jn - 1;
+        while (l < hi) {
+          const int mid = (l + hi + 1) >> 1;
+          if (t.pre[mid] <= i)
+            l = mid;
+          else
+            hi = mid - 1;
+        }
+        lo = l;
       }
       const uint32_t w = dadj[t.vbase[lo] + (i - t.pre[lo])];
       const int p = find_sorted(nu, (int)du, w);
