@@ -63,6 +63,14 @@ struct ChunkTable {
 
 constexpr unsigned long long kNoStage = ~0ull;
 
+// Value ranges of a staged N+(u) for the k = 3 probes (k_list3), and when
+// an item uses them: a long list and many probes (building the table costs
+// each lane about one search).
+constexpr uint32_t kListBucketBits = 8;
+constexpr uint32_t kListBuckets = 1u << kListBucketBits;
+constexpr uint32_t kListBucketMinDegree = 64;
+constexpr uint32_t kListBucketMinProbes = 2048;
+
 // First wave: load the chunk's v ids and scan their out-degrees.
 template <class Table>
 __device__ __forceinline__ void load_chunk(Table &t, const uint32_t *dadj,
@@ -132,6 +140,7 @@ __global__ __launch_bounds__(THREADS) void k_list3(
   uint32_t *scnt = dyn + lds_cap; // clique-degree counters by position
   constexpr int WAVES = THREADS / kWave;
   __shared__ ChunkTable<CHUNK, WAVES> t;
+  __shared__ uint32_t bstart[kListBuckets + 1];
   const bool write = out != nullptr;
   for (uint64_t it = blockIdx.x; it < num_items; it += gridDim.x) {
     const uint2 item = items[it];
@@ -160,6 +169,30 @@ __global__ __launch_bounds__(THREADS) void k_list3(
     const uint32_t total = t.pre[jn];
     const uint32_t *nu = staged ? su : dadj + ub;
     const uint64_t obase = write ? item_base[it] : 0;
+    // Items with many probes against a long N+(u) first cut its values into
+    // kListBuckets ranges of 2^bsh ids (bstart[b] = first position in range
+    // b), so a probe searches one range instead of the whole list.
+    const bool bucketed = staged && du >= kListBucketMinDegree && total >= kListBucketMinProbes;
+    uint32_t blo = 0, bspan = 0, bsh = 0;
+    if (bucketed) {
+      blo = su[0];
+      bspan = su[du - 1] - blo;
+      const uint32_t bits = bspan ? 32u - (uint32_t)__clz(bspan) : 0u;
+      bsh = bits > kListBucketBits ? bits - kListBucketBits : 0u;
+      for (uint32_t b = threadIdx.x; b <= kListBuckets; b += blockDim.x) {
+        const uint64_t thr = (uint64_t)b << bsh;
+        int l = 0, h = (int)du;
+        while (l < h) {
+          const int mid = (l + h) >> 1;
+          if ((uint64_t)(su[mid] - blo) < thr)
+            l = mid + 1;
+          else
+            h = mid;
+        }
+        bstart[b] = (uint32_t)l;
+      }
+      __syncthreads();
+    }
     uint32_t found = 0;
     // lo = the last vi with pre[vi] <= i; i only grows, so lo moves on from
     // where it was (pre[jn] = total > i bounds it) and is searched for only
@@ -178,7 +211,18 @@ __global__ __launch_bounds__(THREADS) void k_list3(
         lo = l;
       }
       const uint32_t w = dadj[t.vbase[lo] + (i - t.pre[lo])];
-      const int p = find_sorted(nu, (int)du, w);
+      int p;
+      if (bucketed) {
+        const uint32_t off = w - blo;  // wraps past bspan when w < blo
+        p = -1;
+        if (off <= bspan) {
+          const uint32_t b = off >> bsh, l0 = bstart[b];
+          const int r = find_sorted(su + l0, (int)(bstart[b + 1] - l0), w);
+          if (r >= 0) p = (int)l0 + r;
+        }
+      } else {
+        p = find_sorted(nu, (int)du, w);
+      }
       if (p < 0) continue;
       ++found;
       if (write) {
